@@ -1,0 +1,202 @@
+"""Benchmark: time to relative residual 1e-10 and SpMV HBM bandwidth on one MI355X.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--n 128] [--no-cpu]
+
+Workload: 3-D 7-point variable-coefficient diffusion on 128^3 cells
+(n = 2,097,152, 14,581,760 nonzeros), face coefficients lognormal(0, 1) from
+seed 0, right-hand side uniform[-1, 1] from seed 1, x0 = 0, FP64 values with
+int32 indices.  A step is one complete FP64 solve, CG + Jacobi to rtol 1e-10
+(PETSc's default convergence test, preconditioned norm), with matrix, vectors
+and preconditioner already resident in HBM.  `value` is the mean time to
+solution over the K timed steps (lower is better).
+
+Also reported: the SpMV kernel's average launch time from HIP events on the
+library's stream and its achieved bandwidth from algorithmic bytes
+((8 + 4) nnz + 4 (n + 1) + 2 * 8 n), CG + IC(0) time to solution on the same
+system, and a CPU baseline: the oracle (the C restatement of the reference's
+single-threaded path) timed on a bounded number of iterations on one core and
+extrapolated to the GPU's iteration count.
+
+One GPU only: --gpus above 1 exits with a message.
+"""
+
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import subprocess
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path[:0] = [os.path.join(ROOT, "mojo-petsc4py_amd"), ROOT]
+
+import numpy as np  # noqa: E402
+
+HBM_PEAK_GBS = 8000.0
+
+
+def spmv_bytes(n: int, nnz: int) -> int:
+    return 12 * nnz + 4 * (n + 1) + 16 * n
+
+
+def build_system(N: int):
+    from mojopetsc4py import PETSc
+    from mpk import workloads as W
+    (ip, ix, va), b = W.diffusion_3d(N)
+    n = len(ip) - 1
+    A = PETSc.Mat().createSeqAIJ((n, n), nnz=7)
+    A.setValuesCSR(ip, ix, va)
+    A.assemble()
+    bv = PETSc.Vec().createWithArray(b)
+    return A, bv, (ip, ix, va), b
+
+
+def make_ksp(A, pc_type: str):
+    from mojopetsc4py import PETSc
+    pc = PETSc.PC().create()
+    pc.setType(pc_type)
+    pc.setOperators(A)
+    ksp = PETSc.KSP().create()
+    ksp.setPC(pc)
+    ksp.setType("cg")
+    ksp.setOperators(A)
+    ksp.setTolerances(rtol=1e-10, max_it=20000)
+    ksp.setUp()
+    return ksp
+
+
+def timed_solves(ksp, bv, x, steps: int, warmup: int):
+    from mpk import _native
+    lib = _native.lib()
+    for _ in range(warmup):
+        ksp.solve(bv, x)
+    lib.mpk_synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        ksp.solve(bv, x)
+    lib.mpk_synchronize()
+    return (time.perf_counter() - t0) / max(steps, 1)
+
+
+def true_relres(A, bv, x):
+    r = A.createVecLeft()
+    A.mult(x, r)
+    r.aypx(-1.0, bv)
+    return r.norm() / bv.norm()
+
+
+def cpu_baseline(csr, b, gpu_its: int, sample_its: int):
+    """The oracle's CG + Jacobi on one core for `sample_its` iterations,
+    extrapolated to `gpu_its` iterations."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as O
+    O.set_mode("ref")
+    A = O.CSR(*csr)
+    pc = O.PCSetup(A, "jacobi")
+    t0 = time.perf_counter()
+    r = O.ksp_solve(A, b, "cg", pc, rtol=1e-10, max_it=sample_its)
+    dt = time.perf_counter() - t0
+    per_it = dt / max(r.its, 1)
+    return per_it * gpu_its, per_it, r.its
+
+
+def cpu_model() -> str:
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown CPU"
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--n", type=int, default=128, help="cells per dimension")
+    ap.add_argument("--spmv-reps", type=int, default=50)
+    ap.add_argument("--cpu-its", type=int, default=60, help="iterations in the CPU sample")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-icc", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py runs on one GPU: this build has no multi-GPU path (see DESIGN.md for how it"
+              " would shard); run with --gpus 1", file=sys.stderr)
+        return 2
+
+    from mpk import _native
+    lib = _native.lib()
+    dev_name = _native.device_name()
+    N = args.n
+    t_build = time.perf_counter()
+    A, bv, csr, b = build_system(N)
+    n = A.getSize()[0]
+    nnz = int(csr[0][-1])
+    x = bv.duplicate()
+
+    ksp = make_ksp(A, "jacobi")
+    t_solve = timed_solves(ksp, bv, x, args.steps, args.warmup)
+    its, reason = ksp.getIterationNumber(), ksp.getConvergedReason()
+    relres = true_relres(A, bv, x)
+
+    # SpMV: average launch time, HIP events on the library stream
+    d = A._dev()
+    xs = bv.duplicate()
+    xs.set(1.0)
+    ys = bv.duplicate()
+    xa, ya = xs._d(), ys._dw()
+    lib.mpk_time_spmv(xa, ya, d.rp.ptr, d.ci.ptr, d.v.ptr, n, 5)
+    spmv_ms = lib.mpk_time_spmv(xa, ya, d.rp.ptr, d.ci.ptr, d.v.ptr, n, args.spmv_reps)
+    _native.check()
+    sbytes = spmv_bytes(n, nnz)
+    spmv_gbs = sbytes / (spmv_ms * 1e-3) / 1e9
+
+    extra = {}
+    if not args.no_icc:
+        ksp2 = make_ksp(A, "icc")
+        x2 = bv.duplicate()
+        t_icc = timed_solves(ksp2, bv, x2, 1, 1)
+        extra["cg_icc0"] = {"seconds": t_icc, "its": ksp2.getIterationNumber(),
+                            "reason": ksp2.getConvergedReason(), "true_relres": true_relres(A, bv, x2)}
+
+    cpu = None
+    if not args.no_cpu:
+        est, per_it, sample = cpu_baseline(csr, b, its, args.cpu_its)
+        cpu = {"value": est, "unit": "s", "cores": 1, "kind": "port",
+               "sample": f"oracle CG+Jacobi, {sample} iterations at {N}^3 on one core of {cpu_model()}"
+                         f" ({per_it * 1e3:.1f} ms/iteration), extrapolated to the GPU's {its} iterations"}
+
+    out = {
+        "metric": "time to rel. residual 1e-10 + SpMV HBM GB/s; mixed/batched vs plain FP64",
+        "value": t_solve,
+        "unit": "s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": t_solve * 1e3,
+        "higher_is_better": False,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": f"diffusion_3d {N}^3 7-point lognormal(0,1), CG+Jacobi rtol 1e-10, FP64 int32 CSR",
+                   "n": n, "nnz": nnz, "its": its, "reason": reason, "true_relres": relres,
+                   "device": dev_name},
+        "roofline": {"bound": "hbm", "kernel": "k_spmv_row", "achieved": spmv_gbs, "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": spmv_gbs / HBM_PEAK_GBS, "traffic": None,
+                     "bytes_per_launch": sbytes, "avg_launch_ms": spmv_ms},
+        "cpu_baseline": cpu,
+        "solves": extra,
+        "setup_s": time.perf_counter() - t_build,
+    }
+    print(json.dumps(out))
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

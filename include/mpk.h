@@ -1,0 +1,165 @@
+/*
+ * mpk.h -- C ABI of libmpk.so, the MI355X (gfx950) Krylov core.
+ *
+ * The 59 mp_* entry points carry the names, arity and argument kinds of the
+ * reference's C ABI (src/capi.mojo, declared for ctypes in
+ * python/mojopetsc4py/_lib.py:23-87): every argument is an int64 (an address
+ * or an integer) or a double, and nothing is thrown across the boundary.
+ * What changes is where the buffers live:
+ *
+ *   [D]  device address (hipMalloc'd, e.g. by mpk_malloc); index arrays on
+ *        the device are int32 (row pointers, columns, permutations, factor
+ *        index arrays), values float64;
+ *   [H]  host address; host index arrays are int64 as in the reference.
+ *
+ * Scalar results (dot, norms, min/max) are written to [H] slots and the call
+ * returns after they are ready.  Vector work is queued on the library's
+ * stream; a call that returns data to the host synchronises it.  Callers own
+ * all memory, as in the reference; the library keeps only its own scratch
+ * (reduction partials, sweep buffers, solver state).
+ *
+ * Errors: the first failure is recorded; mpk_status() returns its code
+ * (0 = none) and mpk_last_error() its message.
+ */
+#ifndef MPK_H
+#define MPK_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ------------------------------------------------------------------ runtime
+ * No counterpart in the reference (it has no device): context, memory and
+ * error plumbing for the device buffers the mp_* calls take. */
+int64_t mpk_init(int64_t device);                        /* -1: current device */
+int64_t mpk_status(void);
+int64_t mpk_last_error(int64_t buf, int64_t len);       /* [H] char buffer */
+void mpk_clear_error(void);
+int64_t mpk_device_name(int64_t buf, int64_t len);      /* [H] char buffer */
+int64_t mpk_malloc(int64_t bytes);                       /* returns [D] or 0 */
+void mpk_free(int64_t ptr);
+void mpk_memcpy_htod(int64_t dst, int64_t src, int64_t bytes);  /* synchronous */
+void mpk_memcpy_dtoh(int64_t dst, int64_t src, int64_t bytes);  /* synchronous */
+void mpk_memcpy_dtod(int64_t dst, int64_t src, int64_t bytes);  /* queued */
+void mpk_memset(int64_t dst, int64_t value, int64_t bytes);     /* queued */
+void mpk_synchronize(void);
+void mpk_forget_matrix(int64_t irow);  /* drop cached transpose maps of a matrix */
+/* average ms of `reps` SpMV launches, HIP events on the library stream */
+double mpk_time_spmv(int64_t x, int64_t y, int64_t irow, int64_t icol, int64_t data, int64_t nrow,
+                     int64_t reps);
+
+/* ------------------------------------------------------------------ Mat
+ * capi.mojo:32-65 -> ported.mojo:380-461.  irow/icol [D] int32, data [D]. */
+void mp_mat_mult(int64_t x, int64_t y, int64_t irow, int64_t icol, int64_t data, int64_t nrow);
+void mp_mat_mult_add(int64_t x, int64_t y, int64_t irow, int64_t icol, int64_t data, int64_t nrow);
+void mp_mat_mult_transpose(int64_t x, int64_t y, int64_t irow, int64_t icol, int64_t data, int64_t nrow,
+                           int64_t ncols);
+void mp_mat_get_diagonal(int64_t diag, int64_t irow, int64_t icol, int64_t data, int64_t nrow);
+void mp_mat_scale(int64_t data, int64_t nnz, double alpha);
+void mp_mat_zero_entries(int64_t data, int64_t nnz);
+
+/* ------------------------------------------------------------------ Vec
+ * capi.mojo:69-238 -> ported.mojo:75-335.  Vectors [D]; val, z, idx, sval,
+ * rval, v [H]; mp_vec_maxpy's alpha [H] doubles and x [H] array of [D]. */
+void mp_vec_pointwise_mult(int64_t w, int64_t x, int64_t y, int64_t n);
+void mp_vec_pointwise_divide(int64_t w, int64_t x, int64_t y, int64_t n);
+void mp_vec_pointwise_min(int64_t w, int64_t x, int64_t y, int64_t n);
+void mp_vec_pointwise_max(int64_t w, int64_t x, int64_t y, int64_t n);
+void mp_vec_pointwise_max_abs(int64_t w, int64_t x, int64_t y, int64_t n);
+void mp_vec_norm(int64_t x, int64_t ntype, int64_t z, int64_t n);
+void mp_vec_dot(int64_t x, int64_t y, int64_t val, int64_t n);
+void mp_vec_tdot(int64_t x, int64_t y, int64_t val, int64_t n);
+void mp_vec_sum(int64_t x, int64_t val, int64_t n);
+void mp_vec_mean(int64_t x, int64_t val, int64_t n);
+void mp_vec_dot_norm2(int64_t x, int64_t y, int64_t sval, int64_t rval, int64_t n);
+void mp_vec_max(int64_t x, int64_t idx, int64_t z, int64_t n);
+void mp_vec_min(int64_t x, int64_t idx, int64_t z, int64_t n);
+void mp_vec_normalize(int64_t x, int64_t v, int64_t n);
+void mp_vec_reciprocal(int64_t x, int64_t n);
+void mp_vec_exp(int64_t x, int64_t n);
+void mp_vec_log(int64_t x, int64_t n);
+void mp_vec_sqrtabs(int64_t x, int64_t n);
+void mp_vec_chop(int64_t x, int64_t n, double tol);
+void mp_vec_abs(int64_t x, int64_t n);
+void mp_vec_conjugate(int64_t x, int64_t n);
+void mp_vec_copy(int64_t x, int64_t y, int64_t n);
+void mp_vec_swap(int64_t x, int64_t y, int64_t n);
+void mp_vec_set(int64_t x, double alpha, int64_t n);
+void mp_vec_zero_entries(int64_t x, int64_t n);
+void mp_vec_scale(int64_t x, double alpha, int64_t n);
+void mp_vec_shift(int64_t x, double alpha, int64_t n);
+void mp_vec_axpy(int64_t y, double alpha, int64_t x, int64_t n);
+void mp_vec_aypx(int64_t y, double alpha, int64_t x, int64_t n);
+void mp_vec_axpby(int64_t y, double alpha, double beta, int64_t x, int64_t n);
+void mp_vec_axpbypcz(int64_t z, double alpha, double beta, double gamma, int64_t x, int64_t y, int64_t n);
+void mp_vec_waxpy(int64_t w, double alpha, int64_t x, int64_t y, int64_t n);
+void mp_vec_maxpy(int64_t y, int64_t nv, int64_t alpha, int64_t x, int64_t n);
+void mp_vec_max_pointwise_divide(int64_t x, int64_t y, int64_t rval, int64_t n);
+
+/* ------------------------------------------------------------------ PC
+ * capi.mojo:242-379. */
+/* ported.mojo:354-377; irow/icol/diag [D] int32; returns the diagDense flag */
+int64_t mp_mat_get_diagonal_markers(int64_t irow, int64_t icol, int64_t diag, int64_t nrow);
+/* ported.mojo:502-567; all arrays [D] */
+void mp_pc_jacobi_setup(int64_t diag, int64_t jtype, int64_t useabs, int64_t fixdiag, double scale, int64_t irow,
+                        int64_t icol, int64_t data, int64_t nrow, int64_t didx);
+void mp_pc_jacobi_setup_symmetric(int64_t diagsqrt, int64_t jtype, int64_t irow, int64_t icol, int64_t data,
+                                  int64_t nrow);
+void mp_pc_jacobi_apply(int64_t y, int64_t x, int64_t diag, int64_t n);
+/* ported.mojo:633-748 with its = lits = 1 (capi.mojo:272-280); all arrays [D] */
+void mp_pc_sor_apply(int64_t x, int64_t y, int64_t irow, int64_t icol, int64_t data, int64_t dmark, int64_t idiag,
+                     int64_t mdiag, int64_t t, double omega, double fshift, int64_t flag, int64_t nrow);
+/* ported.mojo:864-899; factor arrays [D] int32 (layout of factor.mojo:31-67) */
+void mp_pc_ilu_apply(int64_t b, int64_t x, int64_t tmp, int64_t bi, int64_t bj, int64_t ba, int64_t bdiag,
+                     int64_t r, int64_t c, int64_t nrow);
+/* ported.mojo:902-952; factor arrays [D] int32; returns 0 */
+int64_t mp_pc_icc_apply(int64_t b, int64_t x, int64_t ui, int64_t uj, int64_t ua, int64_t udiag, int64_t c,
+                        int64_t scratch, int64_t nrow);
+/* ported.mojo:957-1124, factor.mojo:31-412: set up on the host, all [H] int64 */
+int64_t mp_pc_ilu_numeric(int64_t ba, int64_t rtmp, int64_t irow, int64_t icol, int64_t data, int64_t bi,
+                          int64_t bj, int64_t bdiag, int64_t r, int64_t ic, int64_t nrow);
+void mp_pc_icc_numeric(int64_t ua, int64_t rtmp, int64_t il, int64_t c2r, int64_t irow, int64_t icol, int64_t data,
+                       int64_t ui, int64_t uj, int64_t udiag, int64_t r, int64_t riip, int64_t nrow);
+int64_t mp_ilu_symbolic_ilu0(int64_t bi, int64_t bj, int64_t bdiag, int64_t adiag, int64_t irow, int64_t icol,
+                             int64_t nrow);
+int64_t mp_ilu_symbolic(int64_t bi, int64_t bj, int64_t bjlvl, int64_t bdiag, int64_t irow, int64_t icol,
+                        int64_t r, int64_t ic, int64_t levels, int64_t cap, int64_t scratch, int64_t nrow);
+void mp_ilu_contiguous_lu(int64_t bjsrc, int64_t bi, int64_t bdiag, int64_t bj, int64_t n);
+int64_t mp_icc_symbolic(int64_t ui, int64_t uj, int64_t ujlvl, int64_t udiag, int64_t irow, int64_t icol,
+                        int64_t r, int64_t riip, int64_t levels, int64_t cap, int64_t scratch, int64_t nrow,
+                        int64_t scratchlen);
+void mp_icc_contiguous_cholesky(int64_t ujsrc, int64_t ui, int64_t udiag, int64_t bi, int64_t bdiag, int64_t bj,
+                                int64_t n);
+void mp_ordering_rcm(int64_t perm, int64_t level, int64_t visited, int64_t queue, int64_t irow, int64_t icol,
+                     int64_t nrow);
+/* No counterpart in the reference: the transpose of an IC factor's strict
+ * part, which the device forward sweep gathers over.  [D] int32 arrays:
+ * trp[nrow + 1], trow[m], tpos[m], m = mpk_icc_transpose_size(). */
+int64_t mpk_icc_transpose_size(int64_t ui, int64_t udiag, int64_t nrow);
+int64_t mpk_icc_transpose(int64_t ui, int64_t uj, int64_t udiag, int64_t nrow, int64_t trp, int64_t trow,
+                          int64_t tpos);
+
+/* ------------------------------------------------------------------ KSP
+ * capi.mojo:383-424 -> ksp.mojo.  irow/icol/data, b, x, work, vv, prevec [D];
+ * cfg, cfgi, pci, pcf, hs [H].  pci holds [D] addresses: Jacobi {2, diag,
+ * diagsqrt}; SOR {4, irow, icol, data, dmark, idiag, mdiag, work, ..., flag at
+ * 10}; ILU {5, bi, bj, ba, bdiag, r, c}; IC {6, bi, bj, ba, bdiag, r, trp,
+ * trow, tpos} (the last three may be 0: the library then builds them). */
+void mp_ksp_solve_cg(int64_t n, int64_t irow, int64_t icol, int64_t data, int64_t pci, int64_t pcf, int64_t pstmp,
+                     int64_t work, int64_t cfg, int64_t cfgi, int64_t b, int64_t x);
+void mp_ksp_solve_bcgs(int64_t n, int64_t irow, int64_t icol, int64_t data, int64_t pci, int64_t pcf,
+                       int64_t pstmp, int64_t work, int64_t cfg, int64_t cfgi, int64_t b, int64_t x);
+void mp_ksp_solve_gmres(int64_t n, int64_t irow, int64_t icol, int64_t data, int64_t pci, int64_t pcf,
+                        int64_t pstmp, int64_t work, int64_t vv, int64_t hs, int64_t cfg, int64_t cfgi, int64_t b,
+                        int64_t x);
+void mp_ksp_solve_fgmres(int64_t n, int64_t irow, int64_t icol, int64_t data, int64_t pci, int64_t pcf,
+                         int64_t pstmp, int64_t work, int64_t vv, int64_t prevec, int64_t hs, int64_t cfg,
+                         int64_t cfgi, int64_t b, int64_t x);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* MPK_H */

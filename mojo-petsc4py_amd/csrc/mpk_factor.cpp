@@ -1,0 +1,406 @@
+// mpk_factor.cpp -- host-side set up of the incomplete factorisations.
+//
+// The symbolic phases are integer work on the host.  They must produce the
+// reference's fill pattern entry for entry, so they follow its steps
+// (src/factor.mojo, with the sorted fill list of src/ported.mojo:759-861):
+// the same list walk, the same level rule, the same drain order.  The numeric
+// factorisations run here too and give the same bits as
+// MatILUFactorNumeric_SeqAIJ / MatICCFactorNumeric_SeqAIJ (ported.mojo:957-1124);
+// the factor is then uploaded for the device sweeps.
+//
+// All arrays in this file are host arrays; index arrays are int64, as the
+// reference's Python layer allocates them (PC.py:376-492).
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+
+#include "../../include/mpk.h"
+
+namespace {
+
+typedef int64_t I;
+
+// The sorted fill list: a singly linked list threaded through `next`, headed
+// at slot `head` (= n), with a fill level per entry and a membership bit.
+struct FillList {
+  I *next, *lvl, *mark;
+  I head;
+
+  // PetscLLInsertLocation: walk from `from` to the insertion point of `entry`
+  // and splice it in; returns the new resume point (the entry itself).
+  I splice(I from, I entry) {
+    I loc = from, succ;
+    for (;;) {
+      succ = next[loc];
+      if (entry > succ)
+        loc = succ;
+      else
+        break;
+    }
+    next[loc] = entry;
+    next[entry] = succ;
+    return entry;
+  }
+  // mark `entry`, reporting whether it was already present
+  bool test_and_mark(I entry) {
+    bool was = mark[entry] != 0;
+    mark[entry] = 1;
+    return was;
+  }
+  // IncompleteLLInit: the columns of a row of A, permuted, at level 0.
+  I seed(I count, const I *cols, const I *perm) {
+    I added = 0, resume = head;
+    for (I k = 0; k < count; k++) {
+      I e = perm[cols[k]];
+      if (!test_and_mark(e)) {
+        I from = (k != 0 && e < resume) ? head : resume;
+        resume = splice(from, e);
+        added++;
+        lvl[e] = 0;
+      }
+    }
+    return added;
+  }
+  // IncompleteLLAdd with sorted input: merge a pivot row's entries whose
+  // level lvl_k + offset + 1 stays within `levels`; keep the smaller level of
+  // an entry already present.
+  I merge(I count, const I *cols, const I *clvl, I levels, I offset) {
+    I added = 0, resume = head;
+    for (I k = 0; k < count; k++) {
+      I inc = clvl[k] + offset + 1;
+      if ((double)inc <= (double)levels) {
+        I e = cols[k];
+        if (!test_and_mark(e)) {
+          resume = splice(resume, e);
+          added++;
+          lvl[e] = inc;
+        } else if (lvl[e] > inc) {
+          lvl[e] = inc;
+        }
+      }
+    }
+    return added;
+  }
+  // IncompleteLLClean: write out `count` entries in list order, reset.
+  void drain(I count, I *out, I *outlvl) {
+    I at = head;
+    for (I j = 0; j < count; j++) {
+      at = next[at];
+      out[j] = at;
+      outlvl[j] = lvl[at];
+      lvl[at] = -1;
+      mark[at] = 0;
+    }
+    next[head] = head;
+  }
+};
+
+}  // namespace
+
+extern "C" {
+
+// factor.mojo:31-67 -- ILU(0), natural ordering: L = strict lower part of A,
+// U rows (strict upper then the diagonal) stored from the last row down.
+int64_t mp_ilu_symbolic_ilu0(int64_t bi_, int64_t bj_, int64_t bdiag_, int64_t adiag_, int64_t irow_,
+                             int64_t icol_, int64_t nrow) {
+  I *bi = (I *)bi_, *bj = (I *)bj_, *bdiag = (I *)bdiag_;
+  const I *adiag = (const I *)adiag_, *rp = (const I *)irow_, *ci = (const I *)icol_;
+  I w = 0;
+  bi[0] = 0;
+  for (I i = 0; i < nrow; i++) {
+    I nl = adiag[i] - rp[i];
+    bi[i + 1] = bi[i] + nl;
+    memcpy(bj + w, ci + rp[i], sizeof(I) * (size_t)nl);
+    w += nl;
+  }
+  bdiag[nrow] = bi[nrow] - 1;
+  for (I i = nrow - 1; i >= 0; i--) {
+    I nu = rp[i + 1] - adiag[i] - 1;
+    memcpy(bj + w, ci + adiag[i] + 1, sizeof(I) * (size_t)nu);
+    w += nu;
+    bj[w++] = i;
+    bdiag[i] = bdiag[i + 1] + nu + 1;
+  }
+  return w;
+}
+
+// factor.mojo:70-122 -- ILU(k): rows staged contiguously (L, diagonal, U),
+// bdiag[i] = number of L entries; negative return = capacity needed.
+int64_t mp_ilu_symbolic(int64_t bi_, int64_t bj_, int64_t bjlvl_, int64_t bdiag_, int64_t irow_, int64_t icol_,
+                        int64_t r_, int64_t ic_, int64_t levels, int64_t cap, int64_t scratch_, int64_t nrow) {
+  I *bi = (I *)bi_, *bj = (I *)bj_, *bjlvl = (I *)bjlvl_, *bdiag = (I *)bdiag_;
+  const I *rp = (const I *)irow_, *ci = (const I *)icol_, *r = (const I *)r_, *ic = (const I *)ic_;
+  I *scr = (I *)scratch_;
+  const I n1 = nrow + 1;
+  FillList L{scr, scr + n1, scr + 2 * n1, nrow};
+  L.next[nrow] = nrow;
+  bi[0] = 0;
+  bdiag[0] = 0;
+  for (I i = 0; i < nrow; i++) {
+    const I row = r[i];
+    L.next[i] = -1;  // marker: the diagonal has not been seen yet
+    I nz = L.seed(rp[row + 1] - rp[row], ci + rp[row], ic);
+    I npiv = 0;
+    for (I p = L.next[nrow]; p < i; p = L.next[p]) {
+      const I nl = bdiag[p];
+      const I start = bi[p] + nl + 1;
+      const I cnt = bi[p + 1] - bi[p] - nl - 1;
+      nz += L.merge(cnt, bj + start, bjlvl + start, levels, L.lvl[p]);
+      npiv++;
+    }
+    bdiag[i] = npiv;
+    if (bi[i] + nz > cap) return -(bi[i] + nz);
+    bi[i + 1] = bi[i] + nz;
+    L.drain(nz, bj + bi[i], bjlvl + bi[i]);
+  }
+  return bi[nrow];
+}
+
+// factor.mojo:125-273 -- IC(k).  Each row is drained into the staging block
+// at its start and then written out with the strict columns ascending and
+// the diagonal last.  Pivot rows are read back from the staging block at the
+// offset the row's position implies (factor.mojo:204-206), which holds the
+// most recently drained row: the reference's pattern is defined by that read.
+int64_t mp_icc_symbolic(int64_t ui_, int64_t uj_, int64_t ujlvl_, int64_t udiag_, int64_t irow_, int64_t icol_,
+                        int64_t rip_, int64_t riip_, int64_t levels, int64_t cap, int64_t scratch_, int64_t nrow,
+                        int64_t scratchlen) {
+  I *ui = (I *)ui_, *uj = (I *)uj_, *ujlvl = (I *)ujlvl_, *udiag = (I *)udiag_;
+  const I *rp = (const I *)irow_, *ci = (const I *)icol_, *rip = (const I *)rip_, *riip = (const I *)riip_;
+  const I n1 = nrow + 1;
+  if (6 * n1 + 2 * cap > scratchlen) return -(6 * n1 + 2 * cap);
+  I *scr = (I *)scratch_;
+  FillList L{scr, scr + n1, scr + 2 * n1, nrow};
+  I *chain = scr + 3 * n1;   // jl: rows waiting on a column, linked
+  I *first = scr + 4 * n1;   // il: next unused entry of a pivot row
+  I *upper = scr + 5 * n1;   // the strictly upper columns of A's row
+  I *stage = scr + 6 * n1, *stagelvl = stage + cap;
+  for (I i = 0; i < nrow; i++) {
+    chain[i] = nrow;
+    first[i] = 0;
+  }
+  L.next[nrow] = nrow;
+  ui[0] = 0;
+  for (I k = 0; k < nrow; k++) {
+    const I rk = rip[k];
+    I nu = 0;
+    for (I j = rp[rk]; j < rp[rk + 1]; j++)
+      if (riip[ci[j]] > k) upper[nu++] = ci[j];
+    I nz = L.seed(nu, upper, riip);
+    if (L.mark[rk] == 0) {
+      L.splice(nrow, rk);  // an unsorted insert walks from the head
+      nz++;
+      L.lvl[rk] = 0;
+      L.mark[rk] = 1;
+    }
+    const I uk = ui[k];
+    for (I p = chain[k]; p < k;) {
+      const I pnext = chain[p];
+      const I jmin = first[p] + 1, jmax = ui[p + 1];
+      const I off = jmin - ui[p];
+      const I *cols = stage + off;
+      const I *clvl = stagelvl + off;
+      if (jmax - jmin > 0) nz += L.merge(jmax - jmin, cols, clvl, levels, clvl[-1]);
+      if (jmin < jmax) {
+        first[p] = jmin;
+        const I cj = cols[0];
+        chain[p] = chain[cj];
+        chain[cj] = p;
+      }
+      p = pnext;
+    }
+    if (uk + nz > cap) return -(uk + nz);
+    L.drain(nz, stage, stagelvl);
+    if (nz > 1) {
+      const I cj = stage[1];
+      chain[k] = chain[cj];
+      chain[cj] = k;
+      first[k] = uk + 1;
+    }
+    I at = -1;
+    for (I c = 0; c < nz; c++)
+      if (stage[c] == rk) {
+        at = c;
+        break;
+      }
+    I w = 0;
+    for (I c = 0; c < nz; c++)
+      if (c != at) {
+        uj[uk + w] = stage[c];
+        ujlvl[uk + w] = stagelvl[c];
+        w++;
+      }
+    // ascending strict columns (insertion sort: the drained order is only
+    // sorted when every merge source was)
+    for (I c = 1; c < w; c++) {
+      const I key = uj[uk + c];
+      I q = c - 1;
+      while (q >= 0 && uj[uk + q] > key) {
+        uj[uk + q + 1] = uj[uk + q];
+        q--;
+      }
+      uj[uk + q + 1] = key;
+    }
+    uj[uk + nz - 1] = rk;
+    ujlvl[uk + nz - 1] = stagelvl[at];
+    udiag[k] = uk + nz - 1;
+    ui[k + 1] = uk + nz;
+  }
+  return ui[nrow];
+}
+
+// factor.mojo:279-332 -- reverse Cuthill-McKee by breadth-first sweeps that
+// visit neighbours in index order, one reversed component at a time.
+void mp_ordering_rcm(int64_t perm_, int64_t level_, int64_t visited_, int64_t queue_, int64_t irow_,
+                     int64_t icol_, int64_t nrow) {
+  I *perm = (I *)perm_, *level = (I *)level_, *seen = (I *)visited_, *q = (I *)queue_;
+  const I *rp = (const I *)irow_, *ci = (const I *)icol_;
+  for (I i = 0; i < nrow; i++) {
+    level[i] = rp[i + 1] - rp[i];
+    seen[i] = 0;
+  }
+  I out = 0;
+  for (I s = 0; s < nrow; s++) {
+    if (seen[s]) continue;
+    seen[s] = 1;
+    q[0] = s;
+    I head = 0, tail = 1;
+    while (head < tail) {
+      const I v = q[head++];
+      for (I j = rp[v]; j < rp[v + 1]; j++)
+        if (!seen[ci[j]]) {
+          seen[ci[j]] = 1;
+          q[tail++] = ci[j];
+        }
+    }
+    for (I a = 0, b = tail - 1; a < b; a++, b--) {
+      I t = q[a];
+      q[a] = q[b];
+      q[b] = t;
+    }
+    memcpy(perm + out, q, sizeof(I) * (size_t)tail);
+    out += tail;
+  }
+}
+
+// factor.mojo:335-381 -- from staged rows to the factor layout: L rows
+// ascending, U rows (strict part, then the diagonal) from the end backwards.
+void mp_ilu_contiguous_lu(int64_t bjsrc_, int64_t bi_, int64_t bdiag_, int64_t bj_, int64_t n) {
+  const I *src = (const I *)bjsrc_;
+  I *bi = (I *)bi_, *bdiag = (I *)bdiag_, *bj = (I *)bj_;
+  I utop = bi[n];
+  I at = 0;
+  I prev_l = n ? bdiag[0] : 0;
+  for (I row = 0; row < n; row++) {
+    const I len = bi[row + 1] - bi[row];
+    bi[row] = row == 0 ? 0 : bi[row - 1] + prev_l;
+    const I nl = bdiag[row];
+    prev_l = nl;
+    memcpy(bj + bi[row], src + at, sizeof(I) * (size_t)nl);
+    bdiag[row] = utop - 1;
+    bj[utop - 1] = row;
+    const I nu = len - nl;
+    utop -= nu;
+    memcpy(bj + utop, src + at + nl + 1, sizeof(I) * (size_t)(nu - 1));
+    at += len;
+  }
+  if (n) {
+    bi[n] = bi[n - 1] + prev_l;
+    bdiag[n] = bdiag[n - 1] - 1;
+  }
+}
+
+// factor.mojo:384-412 -- copy the staged IC rows out and set the row starts;
+// bdiag[n] carries the spare entry the numeric pass reads for the last row.
+void mp_icc_contiguous_cholesky(int64_t ujsrc_, int64_t ui_, int64_t udiag_, int64_t bi_, int64_t bdiag_,
+                                int64_t bj_, int64_t n) {
+  const I *src = (const I *)ujsrc_, *ui = (const I *)ui_, *udiag = (const I *)udiag_;
+  I *bi = (I *)bi_, *bdiag = (I *)bdiag_, *bj = (I *)bj_;
+  for (I row = 0; row < n; row++)
+    for (I j = ui[row]; j <= udiag[row]; j++) bj[j] = src[j];
+  if (n) {
+    bi[0] = 0;
+    for (I row = 0; row < n; row++) bi[row + 1] = udiag[row] + 1;
+    bdiag[n] = bi[n];
+  }
+}
+
+// ported.mojo:957-1037 -- row-wise IKJ elimination with a dense work row.
+int64_t mp_pc_ilu_numeric(int64_t ba_, int64_t rtmp_, int64_t irow_, int64_t icol_, int64_t data_, int64_t bi_,
+                          int64_t bj_, int64_t bdiag_, int64_t r_, int64_t ic_, int64_t nrow) {
+  double *ba = (double *)ba_, *w = (double *)rtmp_;
+  const I *rp = (const I *)irow_, *ci = (const I *)icol_, *bi = (const I *)bi_, *bj = (const I *)bj_,
+          *bd = (const I *)bdiag_, *r = (const I *)r_, *ic = (const I *)ic_;
+  const double *a = (const double *)data_;
+  for (I i = 0; i < nrow; i++) {
+    const I l0 = bi[i], l1 = bi[i + 1];     // L part of row i
+    const I u0 = bd[i + 1] + 1, u1 = bd[i];  // U part (strict), diagonal at bd[i]
+    for (I j = l0; j < l1; j++) w[bj[j]] = 0.0;
+    for (I j = u0; j <= u1; j++) w[bj[j]] = 0.0;  // strict U and the diagonal slot
+    for (I j = rp[r[i]]; j < rp[r[i] + 1]; j++) w[ic[ci[j]]] = a[j];
+    for (I j = l0; j < l1; j++) {
+      const I p = bj[j];
+      const double pc = w[p];
+      if (pc != 0.0) {
+        const double mult = pc * ba[bd[p]];
+        w[p] = mult;
+        for (I q = bd[p + 1] + 1; q < bd[p]; q++) w[bj[q]] = fma(-mult, ba[q], w[bj[q]]);
+      }
+    }
+    for (I j = l0; j < l1; j++) ba[j] = w[bj[j]];
+    for (I j = u0; j < u1; j++) ba[j] = w[bj[j]];
+    ba[bd[i]] = 1.0 / w[i];
+  }
+  return bd[0] + 1;
+}
+
+// ported.mojo:1040-1124 -- up-looking IC with the row chains of each column.
+void mp_pc_icc_numeric(int64_t ua_, int64_t rtmp_, int64_t il_, int64_t c2r_, int64_t irow_, int64_t icol_,
+                       int64_t data_, int64_t ui_, int64_t uj_, int64_t udiag_, int64_t r_, int64_t riip_,
+                       int64_t nrow) {
+  double *ua = (double *)ua_, *w = (double *)rtmp_;
+  I *first = (I *)il_, *chain = (I *)c2r_;
+  const I *rp = (const I *)irow_, *ci = (const I *)icol_, *bi = (const I *)ui_, *bj = (const I *)uj_,
+          *bd = (const I *)udiag_, *r = (const I *)r_, *riip = (const I *)riip_;
+  const double *a = (const double *)data_;
+  for (I i = 0; i < nrow; i++) chain[i] = nrow;
+  if (nrow) first[0] = 0;
+  for (I k = 0; k < nrow; k++) {
+    for (I j = bi[k]; j < bi[k + 1]; j++) w[bj[j]] = 0.0;
+    I slot = bi[k];
+    for (I j = rp[r[k]]; j < rp[r[k] + 1]; j++) {
+      const I col = riip[ci[j]];
+      if (col >= k) {
+        w[col] = a[j];
+        ua[slot++] = 0.0;
+      }
+    }
+    double dk = w[k];
+    for (I i = chain[k]; i < k;) {
+      const I inext = chain[i];
+      const I f = first[i];
+      const double uikdi = -ua[f] * ua[bd[i]];
+      dk = fma(uikdi, ua[f], dk);
+      ua[f] = uikdi;
+      const I jmin = f + 1, jmax = bi[i + 1];
+      if (jmin < jmax) {
+        for (I j = jmin; j < jmax; j++) w[bj[j]] = fma(uikdi, ua[j], w[bj[j]]);
+        first[i] = jmin;
+        const I cj = bj[jmin];
+        chain[i] = chain[cj];
+        chain[cj] = i;
+      }
+      i = inext;
+    }
+    const I jmin = bi[k], jmax = bi[k + 1] - 1;
+    if (jmin < jmax) {
+      for (I j = jmin; j < jmax; j++) ua[j] = w[bj[j]];
+      first[k] = jmin;
+      const I cj = bj[jmin];
+      chain[k] = chain[cj];
+      chain[cj] = k;
+    }
+    ua[bd[k]] = 1.0 / dk;
+  }
+}
+
+}  // extern "C"

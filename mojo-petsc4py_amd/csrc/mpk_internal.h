@@ -1,0 +1,142 @@
+// mpk_internal.h -- shared pieces of the MI355X Krylov library (libmpk.so).
+//
+// Conventions used by every source file:
+//  * one HIP context per process: the current device and one non-blocking
+//    stream at default priority; every kernel goes on that stream, so kernels
+//    never overlap and each one sees everything the previous one wrote;
+//  * vectors, matrices and factors live in device memory; scalar results and
+//    control blocks cross the C ABI through host memory;
+//  * index arrays on the device are int32 (row pointers and column indices);
+//  * all arithmetic is IEEE binary64 with explicit fma() where the reference
+//    expression contracts (the library is built with -ffp-contract=off), so an
+//    elementwise kernel or a serial row sum gives the same bits as the CPU;
+//  * reductions use the fixed tile-tree order below, which depends only on n.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+#include <cstdint>
+
+typedef int64_t i64;
+
+#define MPK_TILE 2048  // elements per reduction tile
+#define MPK_NT 256     // threads per block in tile kernels
+
+namespace mpk {
+
+struct Ctx {
+  int device = -1;
+  hipStream_t stream = nullptr;
+  double *partials = nullptr;  // reduction partials, grown on demand
+  size_t partials_cap = 0;     // in doubles
+  double *dres = nullptr;      // device scalar results (1024 doubles)
+  double *hres = nullptr;      // pinned host mirror of dres
+  int *flags = nullptr;        // per-row ready flags for the sync-free sweeps
+  size_t flags_cap = 0;        // in ints
+  unsigned *tickets = nullptr; // block tickets for the sync-free sweeps (64)
+  double *dtmp = nullptr;      // device scratch vector, grown on demand
+  size_t dtmp_cap = 0;
+};
+
+// The process context, initialised on first use on the current device.
+// Returns nullptr (and records the error) when no HIP device is usable.
+Ctx *ctx();
+hipStream_t stream();
+
+// Error state: the C ABI cannot throw, so failures are recorded here and read
+// back through mpk_status()/mpk_last_error().
+void set_error(int code, const char *fmt, ...);
+bool failed();
+
+double *partials(size_t ndoubles);
+int *flag_buffer(size_t nints);
+double *scratch_vector(size_t ndoubles);
+
+inline unsigned nblocks(i64 n, i64 per) { return (unsigned)((n + per - 1) / per); }
+inline unsigned ntiles(i64 n) { return nblocks(n, MPK_TILE); }
+
+// Synchronise the stream and record any asynchronous error.
+void sync();
+
+}  // namespace mpk
+
+#define HIPCHK(x)                                                                   \
+  do {                                                                              \
+    hipError_t e_ = (x);                                                            \
+    if (e_ != hipSuccess)                                                           \
+      ::mpk::set_error(1, "%s: %s (%s:%d)", #x, hipGetErrorString(e_), __FILE__, __LINE__); \
+  } while (0)
+
+#define MPK_LAUNCH(kernel, grid, block, ...)                                        \
+  do {                                                                              \
+    hipLaunchKernelGGL(kernel, dim3(grid), dim3(block), 0, ::mpk::stream(), __VA_ARGS__); \
+    HIPCHK(hipGetLastError());                                                      \
+  } while (0)
+
+// ------------------------------------------------------------------ device side
+
+namespace mpk {
+
+// Two consecutive doubles; loaded as one 16-byte access when aligned.
+struct D2 {
+  double a, b;
+};
+
+__device__ __forceinline__ D2 ld2(const double *p, i64 e, i64 n, bool al) {
+  D2 r;
+  if (e + 1 < n) {
+    if (al) {
+      double2 v = *reinterpret_cast<const double2 *>(p + e);
+      r.a = v.x;
+      r.b = v.y;
+    } else {
+      r.a = p[e];
+      r.b = p[e + 1];
+    }
+  } else {
+    r.a = e < n ? p[e] : 0.0;
+    r.b = 0.0;
+  }
+  return r;
+}
+
+__device__ __forceinline__ void st2(double *p, i64 e, i64 n, bool al, D2 v) {
+  if (e + 1 < n) {
+    if (al) {
+      *reinterpret_cast<double2 *>(p + e) = make_double2(v.a, v.b);
+    } else {
+      p[e] = v.a;
+      p[e + 1] = v.b;
+    }
+  } else if (e < n) {
+    p[e] = v.a;
+  }
+}
+
+__host__ __device__ __forceinline__ bool aligned16(const void *p) {
+  return (reinterpret_cast<uintptr_t>(p) & 15) == 0;
+}
+
+// Halving tree over the 64 lanes of a wave: u[i] += u[i + w], w = 32..1.
+// Lane 0 ends with the wave's sum; the other lanes hold partial values.
+__device__ __forceinline__ double wave_tree(double u) {
+#pragma unroll
+  for (int w = 32; w >= 1; w >>= 1) u = u + __shfl_down(u, w, 64);
+  return u;
+}
+
+// Block tree over 256 threads: wave trees, then (W0 + W2) + (W1 + W3).
+// Returns the sum in thread 0; `sh` is 4 doubles of shared memory.
+__device__ __forceinline__ double block_tree(double u, double *sh) {
+  u = wave_tree(u);
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  if (lane == 0) sh[wave] = u;
+  __syncthreads();
+  double r = 0.0;
+  if (threadIdx.x == 0) r = (sh[0] + sh[2]) + (sh[1] + sh[3]);
+  __syncthreads();
+  return r;
+}
+
+}  // namespace mpk

@@ -1,0 +1,1088 @@
+// mpk_ksp.hip -- device-resident Krylov drivers: CG, BiCGStab, GMRES, FGMRES.
+//
+// Each driver runs the reference's iteration (src/ksp.mojo:271-903) as a
+// stream of kernels.  The solver's scalars live in a device-side KspState; the
+// scalar steps between vector operations (alpha and beta, the convergence
+// test, breakdown checks, the Givens updates and the back substitution) run in
+// the one-block finalize of the reduction that feeds them.  When a step
+// decides the solve is over it sets `stop`, and every kernel already queued
+// behind it returns at once.  The host queues iterations in chunks and reads
+// one flag per chunk (CG, BiCGStab) or per restart cycle (GMRES, FGMRES).
+//
+// Scalar arithmetic follows the reference expression by expression, with
+// fma() where it contracts, so given the same reductions the iterates are
+// bitwise those of the CPU path.
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+#include "mpk_internal.h"
+#include "mpk_ops.h"
+#include "mpk_pc.h"
+#include "mpk_reduce.h"
+#include "../../include/mpk.h"
+
+namespace mpk {
+
+enum {
+  KSP_CONVERGED_RTOL = 2,
+  KSP_CONVERGED_ATOL = 3,
+  KSP_CONVERGED_ITS = 4,
+  KSP_CONVERGED_HAPPY_BREAKDOWN = 7,
+  KSP_DIVERGED_NULL = -2,
+  KSP_DIVERGED_ITS = -3,
+  KSP_DIVERGED_DTOL = -4,
+  KSP_DIVERGED_BREAKDOWN = -5,
+  KSP_DIVERGED_INDEFINITE_PC = -8,
+  KSP_DIVERGED_NANORINF = -9,
+  KSP_DIVERGED_INDEFINITE_MAT = -10,
+};
+enum { NORM_NONE = 1, NORM_PRECONDITIONED = 2, NORM_UNPRECONDITIONED = 3, NORM_NATURAL = 4 };
+enum { PC_LEFT = 0, PC_RIGHT = 1, PC_SYMMETRIC = 2 };
+
+struct KspState {
+  // configuration (cfg / cfgi, ksp.mojo:9-42)
+  double rtol, atol, divtol, haptol, breakdowntol, rhs_norm;
+  i64 normtype, side, guess_zero, min_it, chknorm, convmaxits, max_it, m, orthog, cgstype, seed_rhs;
+  // KSPConvergedDefault bookkeeping
+  double rnorm0, ttol;
+  i64 seeded;
+  // results
+  i64 its, reason;
+  double rnorm;
+  i64 stop;        // the solve is over
+  i64 inner_stop;  // GMRES: the Arnoldi loop is over (or the solve is)
+  i64 skip_ref;    // GMRES CGS: no refinement pass this step
+  i64 skip_build;  // GMRES: BuildSoln has nothing (more) to do
+  i64 special;     // BiCGStab: the d2 == 0 exit needs x += alpha p
+  i64 stopped_at_init;
+  // CG
+  i64 i;
+  double beta, betaold, dpi, dpiold, a, bcoef, dp;
+  // BiCGStab
+  double rho, rhoold, alpha, omega, omegaold, d1, d2, cz;
+  // GMRES
+  i64 it, itcount, hapend, breason, nbuild;
+  double res, prev_rnorm, rnorm_cycle, tt, hnrm;
+  // device arrays
+  double *hist;  // max_it + 2
+  double *HH, *HES, *CC, *SS, *RS, *LHH;
+};
+
+// ------------------------------------------------------------ scalar steps
+
+// KSPConvergedDefault (ksp.mojo:96-152)
+__device__ i64 conv_default(KspState *s, i64 n, double rnorm) {
+  if (s->convmaxits && n >= s->max_it) return KSP_CONVERGED_ITS;
+  if (s->normtype == NORM_NONE) return 0;
+  if (!n && !s->seeded) {
+    double rnorm0 = rnorm;
+    if (s->seed_rhs) {
+      double snorm = s->rhs_norm;
+      if (!snorm) snorm = rnorm;
+      rnorm0 = snorm;
+    }
+    s->rnorm0 = rnorm0;
+    double ttol = s->rtol * rnorm0;
+    s->ttol = ttol > s->atol ? ttol : s->atol;
+    s->seeded = 1;
+  }
+  if (n <= s->chknorm) return 0;
+  // MAX_REAL, not infinity, as the reference compares (ksp.mojo:85,138)
+  if (rnorm != rnorm || fabs(rnorm) == 1.7976931348623157e308) return KSP_DIVERGED_NANORINF;
+  if (n < s->min_it) return 0;
+  if (rnorm <= s->ttol) {
+    if (rnorm < s->atol) return KSP_CONVERGED_ATOL;
+    return KSP_CONVERGED_RTOL;
+  }
+  if (rnorm >= s->divtol * s->rnorm0) return KSP_DIVERGED_DTOL;
+  return 0;
+}
+
+__device__ void finish(KspState *s, double rnorm, i64 its, i64 reason) {
+  s->rnorm = rnorm;
+  s->its = its;
+  s->reason = reason;
+  s->stop = 1;
+  s->inner_stop = 1;
+  s->skip_ref = 1;
+  s->skip_build = 1;
+}
+
+__device__ i64 sign_real(double x) { return x < 0.0 ? -1 : (x == 0.0 ? 0 : 1); }
+
+// CG: the head of iteration i (ksp.mojo:337-350)
+__device__ void cg_prep(KspState *s) {
+  i64 i = s->i;
+  i64 its = i + 1;
+  if (s->beta == 0.0) {
+    finish(s, s->dp, its, KSP_CONVERGED_ATOL);
+    return;
+  } else if (i > 0 && s->beta * s->betaold < 0.0) {
+    finish(s, s->dp, its, KSP_DIVERGED_INDEFINITE_PC);
+    return;
+  }
+  s->bcoef = i == 0 ? 0.0 : s->beta / s->betaold;
+  s->dpiold = s->dpi;
+}
+
+enum {
+  EPI_CG_INIT,
+  EPI_CG_DPI,
+  EPI_CG_NORM,
+  EPI_BCGS_INIT,
+  EPI_BCGS_RHO,
+  EPI_BCGS_D1,
+  EPI_BCGS_D2,
+  EPI_BCGS_DP,
+  EPI_GM_START,
+  EPI_GM_CGS_A,
+  EPI_GM_CGS_NORM,
+  EPI_GM_CGS_B,
+  EPI_GM_MGS,
+  EPI_GM_STEP,
+  EPI_FGM_START,
+  EPI_FGM_STEP,
+};
+
+// Givens update of the new Hessenberg column (ksp.mojo:589-629); returns
+// the reason and writes the residual estimate.
+__device__ i64 update_hessenberg(KspState *s, i64 it, i64 hapend, double *res) {
+  const i64 m = s->m;
+  double *hh = s->HH, *cc = s->CC, *ss = s->SS, *rs = s->RS;
+  for (i64 j = 1; j <= it; j++) {
+    i64 idx = it * (m + 2) + (j - 1);
+    double tt = hh[idx];
+    hh[idx] = fma(cc[j - 1], tt, ss[j - 1] * hh[idx + 1]);
+    hh[idx + 1] = fma(cc[j - 1], hh[idx + 1], -(ss[j - 1] * tt));
+  }
+  if (!hapend) {
+    i64 base = it * (m + 2) + it;
+    double a = hh[base], b = hh[base + 1];
+    double tt = sqrt(fma(a, a, b * b));
+    if (tt == 0.0) {
+      *res = 0.0;
+      return KSP_DIVERGED_NULL;
+    }
+    cc[it] = a / tt;
+    ss[it] = b / tt;
+    rs[it + 1] = -ss[it] * rs[it];
+    rs[it] = cc[it] * rs[it];
+    hh[base] = fma(cc[it], a, ss[it] * b);
+    *res = fabs(rs[it + 1]);
+    return 0;
+  }
+  *res = 0.0;
+  return 0;
+}
+
+__device__ void ksp_epilogue(KspState *s, int which, int arg, const double *r) {
+  const i64 m = s->m;
+  switch (which) {
+    // ---------------------------------------------------------------- CG
+    case EPI_CG_INIT: {  // r = {z.z, r.r, z.r} (ksp.mojo:313-335)
+      double dp;
+      if (s->normtype == NORM_PRECONDITIONED)
+        dp = sqrt(r[0]);
+      else if (s->normtype == NORM_UNPRECONDITIONED)
+        dp = sqrt(r[1]);
+      else if (s->normtype == NORM_NATURAL)
+        dp = sqrt(fabs(r[2]));
+      else
+        dp = 0.0;
+      s->dp = dp;
+      s->hist[0] = dp;
+      i64 reason = conv_default(s, 0, dp);
+      if (reason) {
+        finish(s, dp, 0, reason);
+        return;
+      }
+      s->beta = r[2];
+      s->i = 0;
+      s->betaold = 1.0;
+      s->dpi = 0.0;
+      cg_prep(s);
+      return;
+    }
+    case EPI_CG_DPI: {  // r = {p.w} (ksp.mojo:350-362)
+      s->dpi = r[0];
+      s->betaold = s->beta;
+      if (s->dpi == 0.0 || (s->i > 0 && sign_real(s->dpi) * sign_real(s->dpiold) < 0)) {
+        finish(s, s->dp, s->i + 1, KSP_DIVERGED_INDEFINITE_MAT);
+        return;
+      }
+      s->a = s->beta / s->dpi;
+      return;
+    }
+    case EPI_CG_NORM: {  // r = {z.z, r.r, z.r} (ksp.mojo:364-386)
+      i64 i = s->i, its = i + 1;
+      double dp;
+      if (s->normtype == NORM_PRECONDITIONED && s->chknorm < i + 2)
+        dp = sqrt(r[0]);
+      else if (s->normtype == NORM_UNPRECONDITIONED && s->chknorm < i + 2)
+        dp = sqrt(r[1]);
+      else if (s->normtype == NORM_NATURAL)
+        dp = sqrt(fabs(r[2]));
+      else
+        dp = 0.0;
+      s->dp = dp;
+      s->hist[its] = dp;
+      i64 reason = conv_default(s, its, dp);
+      if (reason) {
+        finish(s, dp, its, reason);
+        return;
+      }
+      s->beta = r[2];
+      s->i = i + 1;
+      if (s->i >= s->max_it) {
+        finish(s, dp, its, KSP_DIVERGED_ITS);
+        return;
+      }
+      cg_prep(s);
+      return;
+    }
+    // ---------------------------------------------------------------- BiCGStab
+    case EPI_BCGS_INIT: {  // r = {r.r} (ksp.mojo:426-436)
+      double dp = s->normtype != NORM_NONE ? sqrt(r[0]) : 0.0;
+      s->dp = dp;
+      s->hist[0] = dp;
+      i64 reason = conv_default(s, 0, dp);
+      if (reason) {
+        finish(s, dp, 0, reason);
+        s->stopped_at_init = 1;
+        return;
+      }
+      s->rhoold = 1.0;
+      s->alpha = 1.0;
+      s->omegaold = 1.0;
+      s->omega = 1.0;
+      s->rho = 1.0;
+      s->i = 0;
+      s->its = 0;
+      return;
+    }
+    case EPI_BCGS_RHO: {  // r = {r.rp} (ksp.mojo:448-451)
+      s->rho = r[0];
+      s->beta = (s->rho / s->rhoold) * (s->alpha / s->omegaold);
+      s->cz = -s->omegaold * s->beta;
+      return;
+    }
+    case EPI_BCGS_D1: {  // r = {v.rp}
+      s->d1 = r[0];
+      if (s->d1 == 0.0) {
+        finish(s, s->dp, s->its, KSP_DIVERGED_BREAKDOWN);
+        return;
+      }
+      s->alpha = s->rho / s->d1;
+      return;
+    }
+    case EPI_BCGS_D2: {  // r = {s.t, t.t, s.s} (ksp.mojo:466-483)
+      s->d1 = r[0];
+      s->d2 = r[1];
+      if (s->d2 == 0.0) {
+        s->d1 = r[2];
+        if (s->d1 != 0.0) {
+          finish(s, s->dp, s->its, KSP_DIVERGED_BREAKDOWN);
+          return;
+        }
+        s->special = 1;  // x += alpha p, then converged
+        finish(s, 0.0, s->its + 1, KSP_CONVERGED_RTOL);
+        return;
+      }
+      s->omega = s->d1 / s->d2;
+      return;
+    }
+    case EPI_BCGS_DP: {  // r = {r.r} (ksp.mojo:486-504)
+      i64 i = s->i;
+      if (s->normtype != NORM_NONE && s->chknorm < i + 2) s->dp = sqrt(r[0]);
+      s->rhoold = s->rho;
+      s->omegaold = s->omega;
+      s->its += 1;
+      s->hist[s->its] = s->dp;
+      i64 reason = conv_default(s, s->its, s->dp);
+      if (reason) {
+        finish(s, s->dp, s->its, reason);
+        return;
+      }
+      if (s->rho == 0.0) {
+        finish(s, s->dp, s->its, KSP_DIVERGED_BREAKDOWN);
+        return;
+      }
+      s->i = i + 1;
+      if (s->i >= s->max_it) finish(s, s->dp, s->its, KSP_DIVERGED_ITS);
+      return;
+    }
+    // ---------------------------------------------------------------- GMRES
+    case EPI_GM_START: {  // r = {v0.v0}: the cycle head (ksp.mojo:700-720)
+      double res = sqrt(r[0]);
+      s->res = res;
+      if (s->prev_rnorm > 0.0 && fabs(res - s->prev_rnorm) > s->breakdowntol * s->rnorm_cycle) {
+        finish(s, res, s->its, KSP_DIVERGED_BREAKDOWN);
+        return;
+      }
+      s->RS[0] = res;
+      s->rnorm_cycle = res;
+      s->hist[s->its] = res;
+      s->reason = !res ? KSP_CONVERGED_ATOL : conv_default(s, s->its, res);
+      s->it = 0;
+      s->hapend = 0;
+      s->inner_stop = s->reason != 0 || 0 >= m || s->its >= s->max_it;
+      return;
+    }
+    case EPI_FGM_START: {  // r = {v0.v0} (ksp.mojo:809-817)
+      double res = sqrt(r[0]);
+      s->res = res;
+      s->RS[0] = res;
+      s->hist[s->its] = res;
+      s->reason = conv_default(s, s->its, res);
+      s->it = 0;
+      s->hapend = 0;
+      s->inner_stop = s->reason != 0 || 0 >= m || s->its >= s->max_it;
+      return;
+    }
+    case EPI_GM_CGS_A: {  // r[j] = v_new . v_j, j <= it (ksp.mojo:533-566)
+      i64 it = s->it;
+      for (i64 j = 0; j <= it; j++) {
+        s->HH[it * (m + 2) + j] = 0.0;
+        s->HES[it * (m + 1) + j] = 0.0;
+      }
+      for (i64 j = 0; j <= it; j++) s->LHH[j] = -r[j];
+      for (i64 j = 0; j <= it; j++) {
+        s->HH[it * (m + 2) + j] = s->HH[it * (m + 2) + j] - s->LHH[j];
+        s->HES[it * (m + 1) + j] = s->HES[it * (m + 1) + j] - s->LHH[j];
+      }
+      if (s->cgstype == 1) {
+        double hnrm = 0.0;
+        for (i64 j = 0; j <= it; j++) hnrm = fma(s->LHH[j], s->LHH[j], hnrm);
+        s->hnrm = sqrt(hnrm);
+        s->skip_ref = 1;  // decided after the norm of the projected vector
+      } else {
+        s->skip_ref = s->cgstype == 2 ? 0 : 1;
+      }
+      return;
+    }
+    case EPI_GM_CGS_NORM: {  // r = {v_new . v_new}: refine if ||v_new|| < hnrm
+      s->skip_ref = (sqrt(r[0]) < s->hnrm) ? 0 : 1;
+      return;
+    }
+    case EPI_GM_CGS_B: {  // refinement accumulates (ksp.mojo:575-586)
+      i64 it = s->it;
+      for (i64 j = 0; j <= it; j++) s->LHH[j] = -r[j];
+      for (i64 j = 0; j <= it; j++) {
+        s->HH[it * (m + 2) + j] = s->HH[it * (m + 2) + j] - s->LHH[j];
+        s->HES[it * (m + 1) + j] = s->HES[it * (m + 1) + j] - s->LHH[j];
+      }
+      return;
+    }
+    case EPI_GM_MGS: {  // r = {v_new . v_k}, k = arg (ksp.mojo:511-530)
+      i64 it = s->it, k = arg;
+      double d = r[0];
+      s->HH[it * (m + 2) + k] = d;
+      s->HES[it * (m + 1) + k] = d;
+      s->LHH[k] = -d;
+      return;
+    }
+    case EPI_GM_STEP:
+    case EPI_FGM_STEP: {  // r = {v_new . v_new} (ksp.mojo:728-755, 835-860)
+      i64 it = s->it;
+      double tt = sqrt(r[0]);
+      s->tt = tt;
+      s->HH[it * (m + 2) + it + 1] = tt;
+      s->HES[it * (m + 1) + it + 1] = tt;
+      double hapbnd = fabs(tt / s->RS[it]);
+      hapbnd = hapbnd > s->haptol ? s->haptol : hapbnd;
+      i64 hapend;
+      if (which == EPI_GM_STEP)
+        hapend = tt < hapbnd ? 1 : 0;
+      else
+        hapend = tt > hapbnd ? 0 : 1;
+      s->hapend = hapend;
+      double res;
+      i64 reason = update_hessenberg(s, it, hapend, &res);
+      if (which == EPI_GM_STEP) {
+        s->res = res;
+        s->it = it + 1;
+        s->its += 1;
+        s->hist[s->its] = res;
+        if (reason) {
+          s->reason = reason;
+        } else {
+          reason = conv_default(s, s->its, res);
+          s->reason = reason;
+          if (hapend) {
+            if (s->normtype == NORM_NONE)
+              s->reason = KSP_CONVERGED_HAPPY_BREAKDOWN;
+            else if (reason == 0)
+              s->reason = KSP_DIVERGED_BREAKDOWN;
+          }
+        }
+      } else {
+        s->res = res;
+        if (reason) {
+          s->reason = reason;  // FGMRES breaks before counting the step
+        } else {
+          s->it = it + 1;
+          s->its += 1;
+          s->hist[s->its] = res;
+          reason = conv_default(s, s->its, res);
+          s->reason = reason;
+          if (hapend && reason == 0) s->reason = KSP_DIVERGED_BREAKDOWN;
+        }
+      }
+      s->inner_stop = s->reason != 0 || s->it >= m || s->its >= s->max_it;
+      return;
+    }
+  }
+}
+
+struct KspEpi {
+  KspState *s;
+  int which, arg;
+  __device__ void operator()(const double *r) const { ksp_epilogue(s, which, arg, r); }
+};
+
+// ------------------------------------------------------------ scalar kernels
+
+// p = z + bcoef p with VecAYPX's special cases (ported.mojo:285-294); p = z at i = 0
+struct OpCgP {
+  double *p;
+  const double *z;
+  const KspState *s;
+  bool al;
+  __device__ void operator()(i64 e, i64 n) const {
+    if (e >= n) return;
+    D2 zv = ld2(z, e, n, al);
+    if (s->i == 0) {
+      st2(p, e, n, al, zv);
+      return;
+    }
+    double b = s->bcoef;
+    if (b == 0.0) {
+      st2(p, e, n, al, zv);
+      return;
+    }
+    D2 pv = ld2(p, e, n, al);
+    if (b == 1.0)
+      st2(p, e, n, al, D2{fma(b, zv.a, pv.a), fma(b, zv.b, pv.b)});
+    else
+      st2(p, e, n, al, D2{fma(b, pv.a, zv.a), fma(b, pv.b, zv.b)});
+  }
+};
+// x += a p; r += (-a) w  (two VecAXPY sweeps fused, ksp.mojo:363-364)
+struct OpCgXR {
+  double *x, *r;
+  const double *p, *w;
+  const KspState *s;
+  bool al;
+  __device__ void operator()(i64 e, i64 n) const {
+    if (e >= n) return;
+    double a = s->a;
+    D2 xv = ld2(x, e, n, al), pv = ld2(p, e, n, al), rv = ld2(r, e, n, al), wv = ld2(w, e, n, al);
+    st2(x, e, n, al, D2{fma(a, pv.a, xv.a), fma(a, pv.b, xv.b)});
+    st2(r, e, n, al, D2{fma(-a, wv.a, rv.a), fma(-a, wv.b, rv.b)});
+  }
+};
+// p = (1 r + cz v) + beta p  (VecAXPBYPCZ(P, 1, -omegaold beta, beta, R, V))
+struct OpBcgsP {
+  double *p;
+  const double *r, *v;
+  const KspState *s;
+  bool al;
+  __device__ void operator()(i64 e, i64 n) const {
+    if (e >= n) return;
+    double cz = s->cz, bt = s->beta;
+    D2 rv = ld2(r, e, n, al), vv = ld2(v, e, n, al), pv = ld2(p, e, n, al);
+    st2(p, e, n, al, D2{fma(bt, pv.a, fma(1.0, rv.a, cz * vv.a)), fma(bt, pv.b, fma(1.0, rv.b, cz * vv.b))});
+  }
+};
+// w = (-coef) x + y with coef a device scalar (VecWAXPY)
+struct OpWaxpyDev {
+  double *w;
+  const double *x, *y;
+  const double *coef;
+  bool al;
+  __device__ void operator()(i64 e, i64 n) const {
+    if (e >= n) return;
+    double c = -*coef;
+    D2 xv = ld2(x, e, n, al), yv = ld2(y, e, n, al);
+    st2(w, e, n, al, D2{fma(c, xv.a, yv.a), fma(c, xv.b, yv.b)});
+  }
+};
+// x = (alpha p + omega s) + 1 x  (VecAXPBYPCZ(X, alpha, omega, 1, P, S))
+struct OpBcgsX {
+  double *x;
+  const double *p, *sv;
+  const KspState *s;
+  bool al;
+  __device__ void operator()(i64 e, i64 n) const {
+    if (e >= n) return;
+    double al_ = s->alpha, om = s->omega;
+    D2 pv = ld2(p, e, n, al), sv2 = ld2(sv, e, n, al), xv = ld2(x, e, n, al);
+    st2(x, e, n, al,
+        D2{fma(1.0, xv.a, fma(al_, pv.a, om * sv2.a)), fma(1.0, xv.b, fma(al_, pv.b, om * sv2.b))});
+  }
+};
+// x += alpha p, once, for BiCGStab's d2 == 0 exit
+struct OpBcgsSpecial {
+  double *x;
+  const double *p;
+  const KspState *s;
+  bool al;
+  __device__ void operator()(i64 e, i64 n) const {
+    if (e >= n || !s->special) return;
+    double a = s->alpha;
+    D2 xv = ld2(x, e, n, al), pv = ld2(p, e, n, al);
+    st2(x, e, n, al, D2{fma(a, pv.a, xv.a), fma(a, pv.b, xv.b)});
+  }
+};
+// x = x / d with d a device scalar, when d > 0 (VecNormalize's divide)
+struct OpDivDev {
+  double *x;
+  const double *d;
+  bool al;
+  __device__ void operator()(i64 e, i64 n) const {
+    if (e >= n) return;
+    double z = *d;
+    if (!(z > 0.0)) return;
+    D2 xv = ld2(x, e, n, al);
+    st2(x, e, n, al, D2{xv.a / z, xv.b / z});
+  }
+};
+// x = x * (1 / d): FGMRES's VecScale by the reciprocal (ksp.mojo:820,844)
+struct OpScaleRecipDev {
+  double *x;
+  const double *d;
+  const i64 *skip_if_hapend;  // null: always
+  bool al;
+  __device__ void operator()(i64 e, i64 n) const {
+    if (e >= n) return;
+    if (skip_if_hapend && *skip_if_hapend) return;
+    double sc = 1.0 / *d;
+    D2 xv = ld2(x, e, n, al);
+    st2(x, e, n, al, D2{xv.a * sc, xv.b * sc});
+  }
+};
+// y = y + c x with c = sign * coef[k] a device scalar (MGS axpy)
+struct OpAxpyDev {
+  double *y;
+  const double *x;
+  const double *coef;
+  bool al;
+  __device__ void operator()(i64 e, i64 n) const {
+    if (e >= n) return;
+    double c = *coef;
+    D2 xv = ld2(x, e, n, al), yv = ld2(y, e, n, al);
+    st2(y, e, n, al, D2{fma(c, xv.a, yv.a), fma(c, xv.b, yv.b)});
+  }
+};
+
+struct OpCopy {
+  double *y;
+  const double *x;
+  bool al;
+  __device__ void operator()(i64 e, i64 n) const {
+    if (e >= n) return;
+    st2(y, e, n, al, ld2(x, e, n, al));
+  }
+};
+
+template <class Op>
+__global__ void __launch_bounds__(MPK_NT) k_ew2(i64 n, Op op, const int64_t *stop) {
+  if (stop && *stop) return;
+  const i64 base = (i64)blockIdx.x * MPK_TILE + 2 * (i64)threadIdx.x;
+#pragma unroll
+  for (int j = 0; j < 4; j++) op(base + 512 * j, n);
+}
+template <class Op>
+static void ew2(i64 n, const Op &op, const int64_t *stop) {
+  if (n <= 0) return;
+  MPK_LAUNCH(k_ew2<Op>, ntiles(n), MPK_NT, n, op, stop);
+}
+
+// Back substitution of KSPGMRESBuildSoln / _fgmres_buildsoln (ksp.mojo:632-658,
+// 886-900) in one thread.  Sets nbuild = it + 1 vectors to combine, or
+// skip_build when there is nothing to combine.
+__global__ void k_gmres_backsub(KspState *s, int flexible, const int64_t *stop) {
+  if (stop && *stop) return;
+  const i64 m = s->m;
+  i64 it = s->it - 1;
+  double *hh = s->HH, *rs = s->RS;
+  s->breason = 0;
+  s->skip_build = 1;
+  s->nbuild = 0;
+  if (it < 0) return;
+  if (hh[it * (m + 2) + it] != 0.0) {
+    rs[it] = rs[it] / hh[it * (m + 2) + it];
+  } else {
+    if (!flexible) {
+      s->breason = KSP_DIVERGED_BREAKDOWN;
+      return;
+    }
+    rs[it] = 0.0;
+  }
+  for (i64 ii = 1; ii <= it; ii++) {
+    i64 k = it - ii;
+    double tt = rs[k];
+    for (i64 j = k + 1; j <= it; j++) tt = fma(-hh[j * (m + 2) + k], rs[j], tt);
+    if (!flexible && hh[k * (m + 2) + k] == 0.0) {
+      s->breason = KSP_DIVERGED_BREAKDOWN;
+      return;
+    }
+    rs[k] = tt / hh[k * (m + 2) + k];
+  }
+  s->nbuild = it + 1;
+  s->skip_build = 0;
+}
+
+// The tail of a GMRES cycle (ksp.mojo:756-768) / FGMRES cycle (861-867).
+__global__ void k_gmres_cycle_end(KspState *s, int flexible) {
+  if (s->stop) return;
+  if (!flexible) {
+    if (s->reason == 0 && s->breason != 0) s->reason = s->breason;
+    s->itcount += s->it;
+    if (s->itcount >= s->max_it) {
+      if (s->reason == 0) s->reason = KSP_DIVERGED_ITS;
+      finish(s, s->res, s->its, s->reason);
+      return;
+    }
+    s->prev_rnorm = s->res;
+    if (s->reason != 0) finish(s, s->res, s->its, s->reason);
+  } else {
+    if (s->its >= s->max_it) {
+      if (s->reason == 0) s->reason = KSP_DIVERGED_ITS;
+      finish(s, s->res, s->its, s->reason);
+      return;
+    }
+    if (s->reason != 0) finish(s, s->res, s->its, s->reason);
+  }
+}
+
+// Combination for BuildSoln: y = sum_k rs[k] v_k, k < nbuild (device count).
+struct OpBuild {
+  double *y;
+  const double *base;
+  i64 ld;
+  const KspState *s;
+  bool al;
+  __device__ void operator()(i64 e, i64 n) const {
+    if (e >= n) return;
+    i64 nb = s->nbuild;
+    D2 acc{0.0, 0.0};  // VecMAXPBY with beta = 0: the zeroed destination
+    for (i64 k = 0; k < nb; k++) {
+      double c = s->RS[k];
+      D2 v = ld2(base + k * ld, e, n, al);
+      acc.a = fma(c, v.a, acc.a);
+      acc.b = fma(c, v.b, acc.b);
+    }
+    st2(y, e, n, al, acc);
+  }
+};
+
+// ------------------------------------------------------------ host driver
+
+// Device state, scalar block and flag read-back of the solver, reused across
+// solves (one solve runs at a time on the library's stream).
+struct SolveBuffers {
+  KspState *d = nullptr;
+  double *buf = nullptr;
+  size_t cap = 0;
+  int64_t *hflag = nullptr;
+  hipEvent_t ev[2];
+};
+static SolveBuffers &solve_buffers() {
+  static SolveBuffers b;
+  return b;
+}
+
+struct Solve {
+  i64 n;
+  DevCSR A;
+  PCView pc;
+  KspState h;        // host copy of the initial state
+  KspState *d = nullptr;  // device state
+  double *dbuf = nullptr; // history + GMRES scalars
+  int64_t *stop = nullptr, *inner = nullptr, *skipref = nullptr, *skipbuild = nullptr;
+  int64_t *hflag = nullptr;  // pinned host copy of `stop`
+  hipEvent_t ev[2];
+
+  bool start(i64 n_, const int64_t *irow, const int64_t *icol, const int64_t *data, const int64_t *pci,
+             const double *pcf, const double *cfg, const int64_t *cfgi) {
+    n = n_;
+    A = DevCSR{n, n, (const int *)irow, (const int *)icol, (const double *)data};
+    if (!pc.parse(pci, pcf, n)) return false;
+    memset(&h, 0, sizeof h);
+    h.rtol = cfg[0];
+    h.atol = cfg[1];
+    h.divtol = cfg[2];
+    h.haptol = cfg[3];
+    h.breakdowntol = cfg[4];
+    h.rhs_norm = cfg[12];
+    h.normtype = cfgi[0];
+    h.side = cfgi[1];
+    h.guess_zero = cfgi[2];
+    h.min_it = cfgi[3];
+    h.chknorm = cfgi[4];
+    h.convmaxits = cfgi[5];
+    h.max_it = cfgi[6];
+    h.m = cfgi[7];
+    h.orthog = cfgi[8];
+    h.cgstype = cfgi[9];
+    h.seed_rhs = cfgi[12];
+    h.prev_rnorm = -1.0;
+    i64 m = h.m > 0 ? h.m : 1;
+    size_t nh = (size_t)h.max_it + 2;
+    size_t ns = (size_t)((m + 2) * (m + 1) + (m + 1) * (m + 1) + 2 * (m + 1) + 2 * (m + 2));
+    SolveBuffers &B = solve_buffers();
+    if (!B.d) {
+      HIPCHK(hipMalloc(&B.d, sizeof(KspState)));
+      HIPCHK(hipHostMalloc(&B.hflag, 2 * sizeof(int64_t), hipHostMallocDefault));
+      HIPCHK(hipEventCreateWithFlags(&B.ev[0], hipEventDisableTiming));
+      HIPCHK(hipEventCreateWithFlags(&B.ev[1], hipEventDisableTiming));
+    }
+    if (nh + ns > B.cap) {
+      if (B.buf) {
+        HIPCHK(hipStreamSynchronize(stream()));
+        HIPCHK(hipFree(B.buf));
+      }
+      B.cap = nh + ns;
+      HIPCHK(hipMalloc(&B.buf, B.cap * sizeof(double)));
+    }
+    d = B.d;
+    dbuf = B.buf;
+    hflag = B.hflag;
+    ev[0] = B.ev[0];
+    ev[1] = B.ev[1];
+    HIPCHK(hipMemsetAsync(dbuf, 0, (nh + ns) * sizeof(double), stream()));
+    if (failed()) return false;
+    h.hist = dbuf;
+    h.HH = dbuf + nh;
+    h.HES = h.HH + (m + 2) * (m + 1);
+    h.CC = h.HES + (m + 1) * (m + 1);
+    h.SS = h.CC + (m + 1);
+    h.RS = h.SS + (m + 1);
+    h.LHH = h.RS + (m + 2);
+    HIPCHK(hipMemcpyAsync(d, &h, sizeof h, hipMemcpyHostToDevice, stream()));
+    stop = &d->stop;
+    inner = &d->inner_stop;
+    skipref = &d->skip_ref;
+    skipbuild = &d->skip_build;
+    return !failed();
+  }
+
+  // Queue a flag read-back behind the work queued so far.
+  void mark(int slot) {
+    HIPCHK(hipMemcpyAsync(hflag + slot, stop, sizeof(int64_t), hipMemcpyDeviceToHost, stream()));
+    HIPCHK(hipEventRecord(ev[slot], stream()));
+  }
+  bool done(int slot) {
+    HIPCHK(hipEventSynchronize(ev[slot]));
+    return failed() || hflag[slot] != 0;
+  }
+
+  void finish_out(double *cfg, int64_t *cfgi) {
+    sync();
+    KspState out;
+    HIPCHK(hipMemcpy(&out, d, sizeof out, hipMemcpyDeviceToHost));
+    i64 its = out.its;
+    std::vector<double> hist((size_t)h.max_it + 2, 0.0);
+    HIPCHK(hipMemcpy(hist.data(), h.hist, hist.size() * sizeof(double), hipMemcpyDeviceToHost));
+    cfg[7] = out.rnorm;
+    cfgi[10] = its;
+    cfgi[11] = out.reason;
+    // the history is written into cfg[9 + k] (ksp.mojo:325,375,...), which also
+    // holds cfg[12] and cfg[14]; entries the solve wrote replace them
+    i64 lim = its < h.max_it + 1 ? its : h.max_it + 1;
+    for (i64 k = 0; k <= lim; k++) cfg[9 + k] = hist[(size_t)k];
+    cfg[5] = out.rnorm0;
+    cfg[6] = out.ttol;
+    if (lim < 5) cfg[14] = out.seeded ? 1.0 : cfg[14];
+  }
+
+  template <int K>
+  void reduce(const RedSet<K> &rs, int which, int arg, const int64_t *stp) {
+    reduce_launch<K>(rs, n, nullptr, KspEpi{d, which, arg}, stp);
+  }
+
+  // KSP_PCApplyBAorAB (ksp.mojo:246-265)
+  void ba_or_ab(const double *x, double *y, double *w, const int64_t *stp) {
+    if (h.side == PC_RIGHT) {
+      pc.apply(x, w, stp);
+      spmv(A, w, y, 0, stp);
+    } else if (h.side == PC_LEFT) {
+      spmv(A, x, w, 0, stp);
+      pc.apply(w, y, stp);
+    } else {
+      pc.apply_symmetric(x, w, stp);
+      spmv(A, w, y, 0, stp);
+      copy_if(y, w, stp);
+      pc.apply_symmetric(w, y, stp);
+    }
+  }
+  // KSPInitialResidual (ksp.mojo:221-243)
+  void initial_residual(i64 guess_zero, double *vsoln, double *vt1, double *vt2, double *vres, const double *vb,
+                        const int64_t *stp) {
+    if (!guess_zero) {
+      spmv(A, vsoln, vt1, 0, stp);
+      copy_if(vb, vt2, stp);
+      vaxpy(vt2, -1.0, vt1, n, stp);
+    } else {
+      copy_if(vb, vt2, stp);
+    }
+    if (h.side == PC_SYMMETRIC)
+      pc.apply_symmetric(vt2, vres, stp);
+    else if (h.side == PC_LEFT)
+      pc.apply(vt2, vres, stp);
+    else
+      copy_if(vt2, vres, stp);
+  }
+  // KSPUnwindPreconditioner (ksp.mojo:211-218)
+  void unwind(double *vsoln, double *vt1, const int64_t *stp) {
+    copy_if(vsoln, vt1, stp);
+    if (h.side == PC_RIGHT) {
+      pc.apply(vsoln, vt1, stp);
+      copy_if(vt1, vsoln, stp);
+    }
+  }
+  // a copy that honours the stop flag
+  void copy_if(const double *x, double *y, const int64_t *stp) {
+    if (x == y) return;
+    if (!stp) {
+      vcopy(x, y, n);
+      return;
+    }
+    ew2(n, OpCopy{y, x, aligned16(y) && aligned16(x)}, stp);
+  }
+};
+
+// Iterations queued between two reads of the stop flag.
+static int chunk_for(i64 n) { return n < 65536 ? 64 : (n < 1048576 ? 16 : 8); }
+
+}  // namespace mpk
+
+using namespace mpk;
+
+#define P64(a) reinterpret_cast<const int64_t *>(a)
+
+extern "C" {
+
+// ksp.mojo:271-389 KSPSolve_CG
+void mp_ksp_solve_cg(int64_t n, int64_t irow, int64_t icol, int64_t data, int64_t pci, int64_t pcf,
+                     int64_t pstmp, int64_t work, int64_t cfg, int64_t cfgi, int64_t b, int64_t x) {
+  (void)pstmp;
+  Solve S;
+  if (!ctx() || !S.start(n, P64(irow), P64(icol), P64(data), P64(pci), (const double *)pcf,
+                         (const double *)cfg, P64(cfgi)))
+    return;
+  double *X = (double *)x;
+  const double *B = (const double *)b;
+  double *R = (double *)work, *Z = R + n, *P = R + 2 * n, *W = Z;
+  const int64_t *st = S.stop;
+  if (!S.h.guess_zero) {
+    spmv(S.A, X, R, 0, nullptr);
+    vaypx(R, -1.0, B, n, nullptr);
+  } else {
+    vcopy(B, R, n);
+  }
+  S.pc.apply(R, Z, nullptr);
+  S.reduce<3>(RedSet<3>{{{Z, Z, RK_SUMSQ}, {R, R, RK_SUMSQ}, {Z, R, RK_DOT}}}, EPI_CG_INIT, 0, nullptr);
+  const int chunk = chunk_for(n);
+  int slot = 0;
+  bool pending = false;
+  for (i64 queued = 0;; queued += chunk) {
+    for (int c = 0; c < chunk; c++) {
+      ew2(n, OpCgP{P, Z, S.d, aligned16(P) && aligned16(Z)}, st);
+      spmv(S.A, P, W, 0, st);
+      S.reduce<1>(RedSet<1>{{{P, W, RK_DOT}}}, EPI_CG_DPI, 0, st);
+      ew2(n, OpCgXR{X, R, P, W, S.d, aligned16(X) && aligned16(R) && aligned16(P) && aligned16(W)}, st);
+      S.pc.apply(R, Z, st);
+      S.reduce<3>(RedSet<3>{{{Z, Z, RK_SUMSQ}, {R, R, RK_SUMSQ}, {Z, R, RK_DOT}}}, EPI_CG_NORM, 0, st);
+    }
+    S.mark(slot);
+    if (pending && S.done(slot ^ 1)) break;
+    pending = true;
+    slot ^= 1;
+    if (queued > S.h.max_it + 2 * chunk) break;  // the device has stopped by now
+  }
+  S.finish_out((double *)cfg, (int64_t *)cfgi);
+}
+
+// ksp.mojo:395-505 KSPSolve_BCGS
+void mp_ksp_solve_bcgs(int64_t n, int64_t irow, int64_t icol, int64_t data, int64_t pci, int64_t pcf,
+                       int64_t pstmp, int64_t work, int64_t cfg, int64_t cfgi, int64_t b, int64_t x) {
+  (void)pstmp;
+  Solve S;
+  if (!ctx() || !S.start(n, P64(irow), P64(icol), P64(data), P64(pci), (const double *)pcf,
+                         (const double *)cfg, P64(cfgi)))
+    return;
+  double *X = (double *)x, *WK = (double *)work;
+  const double *B = (const double *)b;
+  double *R = WK, *RP = WK + n, *V = WK + 2 * n, *T = WK + 3 * n, *Sv = WK + 4 * n, *P = WK + 5 * n,
+         *GUESS = WK + 6 * n;
+  const int64_t *st = S.stop;
+  const bool has_guess = S.h.side == PC_RIGHT && !S.h.guess_zero;
+  S.initial_residual(S.h.guess_zero, X, V, T, R, B, nullptr);
+  if (has_guess) {
+    vcopy(X, GUESS, n);
+    vset(X, 0.0, n, nullptr);
+  }
+  S.reduce<1>(RedSet<1>{{{R, R, RK_SUMSQ}}}, EPI_BCGS_INIT, 0, nullptr);
+  // the set up of the loop runs unless the initial test stopped the solve
+  S.copy_if(R, RP, st);
+  vset(P, 0.0, n, st);
+  vset(V, 0.0, n, st);
+  const int chunk = chunk_for(n);
+  int slot = 0;
+  bool pending = false;
+  for (i64 queued = 0;; queued += chunk) {
+    for (int c = 0; c < chunk; c++) {
+      S.reduce<1>(RedSet<1>{{{R, RP, RK_DOT}}}, EPI_BCGS_RHO, 0, st);
+      ew2(n, OpBcgsP{P, R, V, S.d, aligned16(P) && aligned16(R) && aligned16(V)}, st);
+      S.ba_or_ab(P, V, T, st);
+      S.reduce<1>(RedSet<1>{{{V, RP, RK_DOT}}}, EPI_BCGS_D1, 0, st);
+      ew2(n, OpWaxpyDev{Sv, V, R, &S.d->alpha, aligned16(Sv) && aligned16(V) && aligned16(R)}, st);
+      S.ba_or_ab(Sv, T, R, st);
+      S.reduce<3>(RedSet<3>{{{Sv, T, RK_DOT}, {T, T, RK_SUMSQ}, {Sv, Sv, RK_SUMSQ}}}, EPI_BCGS_D2, 0, st);
+      ew2(n, OpBcgsX{X, P, Sv, S.d, aligned16(X) && aligned16(P) && aligned16(Sv)}, st);
+      ew2(n, OpWaxpyDev{R, T, Sv, &S.d->omega, aligned16(R) && aligned16(T) && aligned16(Sv)}, st);
+      S.reduce<1>(RedSet<1>{{{R, R, RK_SUMSQ}}}, EPI_BCGS_DP, 0, st);
+    }
+    S.mark(slot);
+    if (pending && S.done(slot ^ 1)) break;
+    pending = true;
+    slot ^= 1;
+    if (queued > S.h.max_it + 2 * chunk) break;
+  }
+  sync();
+  ew2(n, OpBcgsSpecial{X, P, S.d, aligned16(X) && aligned16(P)}, nullptr);
+  KspState hs;
+  HIPCHK(hipMemcpy(&hs, S.d, sizeof hs, hipMemcpyDeviceToHost));
+  if (!hs.stopped_at_init) S.unwind(X, T, nullptr);
+  if (has_guess) vaxpy(X, 1.0, GUESS, n, nullptr);
+  S.finish_out((double *)cfg, (int64_t *)cfgi);
+}
+
+}  // extern "C"
+
+namespace mpk {
+
+// One GMRES / FGMRES Arnoldi orthogonalisation of v_new against v_0..v_it.
+static void orthogonalize(Solve &S, double *vv, i64 it, double *vnew) {
+  const i64 n = S.n;
+  const int64_t *inner = S.inner;
+  if (S.h.orthog == 1) {  // MGS
+    for (i64 k = 0; k <= it; k++) {
+      S.reduce<1>(RedSet<1>{{{vnew, vv + k * n, RK_DOT}}}, EPI_GM_MGS, (int)k, inner);
+      ew2(n, OpAxpyDev{vnew, vv + k * n, S.h.LHH + k, aligned16(vnew) && aligned16(vv + k * n)}, inner);
+    }
+    return;
+  }
+  // CGS: MDot, negate, MAXPY, subtract; refine when asked
+  const int nv = (int)(it + 1);
+  unsigned T = ntiles(n);
+  for (int pass = 0; pass < 2; pass++) {
+    const int64_t *stp = pass == 0 ? inner : S.skipref;
+    double *Pp = partials((size_t)nv * T);
+    for (int g0 = 0; g0 < nv; g0 += 8) {
+      int g = nv - g0 < 8 ? nv - g0 : 8;
+      MPK_LAUNCH(k_partials_mdot<8>, T, MPK_NT, (const double *)vnew, (const double *)(vv + (i64)g0 * n), (i64)n,
+                 g, (i64)n, Pp + (size_t)g0 * T, stp);
+    }
+    MPK_LAUNCH(k_finalize<KspEpi>, 1, MPK_NT, (const double *)Pp, (int)T, nv, (double *)nullptr,
+               KspEpi{S.d, pass == 0 ? EPI_GM_CGS_A : EPI_GM_CGS_B, 0}, stp);
+    vmaxpy_dev(vnew, vv, n, S.h.LHH, nv, 1.0, n, false, stp);
+    if (pass == 0 && S.h.cgstype == 1)
+      S.reduce<1>(RedSet<1>{{{vnew, vnew, RK_SUMSQ}}}, EPI_GM_CGS_NORM, 0, inner);
+    if (pass == 0 && S.h.cgstype == 0) break;
+  }
+}
+
+}  // namespace mpk
+
+extern "C" {
+
+// ksp.mojo:667-769 KSPSolve_GMRES
+void mp_ksp_solve_gmres(int64_t n, int64_t irow, int64_t icol, int64_t data, int64_t pci, int64_t pcf,
+                        int64_t pstmp, int64_t work, int64_t vv_, int64_t hs, int64_t cfg, int64_t cfgi, int64_t b,
+                        int64_t x) {
+  (void)pstmp, (void)hs;
+  Solve S;
+  if (!ctx() || !S.start(n, P64(irow), P64(icol), P64(data), P64(pci), (const double *)pcf,
+                         (const double *)cfg, P64(cfgi)))
+    return;
+  double *X = (double *)x, *vv = (double *)vv_;
+  const double *B = (const double *)b;
+  double *TEMP = (double *)work, *TEMP_MATOP = TEMP + n;
+  const i64 m = S.h.m;
+  const int64_t *st = S.stop, *inner = S.inner;
+  i64 guess_zero = S.h.guess_zero;
+  for (i64 cycle = 0;; cycle++) {
+    S.initial_residual(guess_zero, X, TEMP, TEMP_MATOP, vv, B, st);
+    S.reduce<1>(RedSet<1>{{{vv, vv, RK_SUMSQ}}}, EPI_GM_START, 0, st);
+    ew2(n, OpDivDev{vv, &S.d->res, aligned16(vv)}, st);
+    for (i64 s = 0; s < m; s++) {
+      double *vs = vv + s * n, *vnew = vv + (s + 1) * n;
+      S.ba_or_ab(vs, vnew, TEMP_MATOP, inner);
+      orthogonalize(S, vv, s, vnew);
+      S.reduce<1>(RedSet<1>{{{vnew, vnew, RK_SUMSQ}}}, EPI_GM_STEP, 0, inner);
+      ew2(n, OpDivDev{vnew, &S.d->tt, aligned16(vnew)}, inner);
+    }
+    // KSPGMRESBuildSoln(RS, HH, vv, TEMP, TEMP_MATOP, x, x, m, it - 1)
+    MPK_LAUNCH(k_gmres_backsub, 1, 1, S.d, 0, st);
+    ew2(n, OpBuild{TEMP, vv, n, S.d, aligned16(TEMP) && aligned16(vv) && n % 2 == 0}, S.skipbuild);
+    S.unwind(TEMP, TEMP_MATOP, S.skipbuild);
+    vaxpy(X, 1.0, TEMP, n, S.skipbuild);
+    MPK_LAUNCH(k_gmres_cycle_end, 1, 1, S.d, 0);
+    S.mark(0);
+    if (S.done(0)) break;
+    guess_zero = 0;
+    if (cycle > S.h.max_it + 2) break;
+  }
+  S.finish_out((double *)cfg, (int64_t *)cfgi);
+}
+
+// ksp.mojo:775-868 KSPSolve_FGMRES
+void mp_ksp_solve_fgmres(int64_t n, int64_t irow, int64_t icol, int64_t data, int64_t pci, int64_t pcf,
+                         int64_t pstmp, int64_t work, int64_t vv_, int64_t prevec_, int64_t hs, int64_t cfg,
+                         int64_t cfgi, int64_t b, int64_t x) {
+  (void)pstmp, (void)hs;
+  Solve S;
+  if (!ctx() || !S.start(n, P64(irow), P64(icol), P64(data), P64(pci), (const double *)pcf,
+                         (const double *)cfg, P64(cfgi)))
+    return;
+  double *X = (double *)x, *vv = (double *)vv_, *prevec = (double *)prevec_;
+  const double *B = (const double *)b;
+  double *TEMP = (double *)work;
+  const i64 m = S.h.m;
+  const int64_t *st = S.stop, *inner = S.inner;
+  // _fgmres_residual: vv0 = -1 (A x) + b
+  if (!S.h.guess_zero) {
+    spmv(S.A, X, TEMP, 0, nullptr);
+    vwaxpy(vv, -1.0, TEMP, B, n, nullptr);
+  } else {
+    vcopy(B, vv, n);
+  }
+  for (i64 cycle = 0;; cycle++) {
+    S.reduce<1>(RedSet<1>{{{vv, vv, RK_SUMSQ}}}, EPI_FGM_START, 0, st);
+    ew2(n, OpScaleRecipDev{vv, &S.d->res, nullptr, aligned16(vv)}, inner);
+    for (i64 s = 0; s < m; s++) {
+      double *vs = vv + s * n, *vnew = vv + (s + 1) * n, *ps = prevec + s * n;
+      S.pc.apply(vs, ps, inner);
+      spmv(S.A, ps, vnew, 0, inner);
+      orthogonalize(S, vv, s, vnew);
+      S.reduce<1>(RedSet<1>{{{vnew, vnew, RK_SUMSQ}}}, EPI_FGM_STEP, 0, inner);
+      ew2(n, OpScaleRecipDev{vnew, &S.d->tt, &S.d->hapend, aligned16(vnew)}, inner);
+    }
+    MPK_LAUNCH(k_gmres_backsub, 1, 1, S.d, 1, st);
+    ew2(n, OpBuild{TEMP, prevec, n, S.d, aligned16(TEMP) && aligned16(prevec) && n % 2 == 0}, S.skipbuild);
+    vaxpy(X, 1.0, TEMP, n, S.skipbuild);
+    MPK_LAUNCH(k_gmres_cycle_end, 1, 1, S.d, 1);
+    S.mark(0);
+    if (S.done(0)) break;
+    spmv(S.A, X, TEMP, 0, nullptr);
+    vwaxpy(vv, -1.0, TEMP, B, n, nullptr);
+    if (cycle > S.h.max_it + 2) break;
+  }
+  S.finish_out((double *)cfg, (int64_t *)cfgi);
+}
+
+}  // extern "C"

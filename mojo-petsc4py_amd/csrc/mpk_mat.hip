@@ -1,0 +1,227 @@
+// mpk_mat.hip -- SEQAIJ kernels and the mp_mat_* exports.
+//
+// The device matrix is CSR with int32 row pointers and column indices and
+// float64 values.  Every row sum runs serially in CSR order from 0.0 (or from
+// y_i for MultAdd), one lane per row, with fma(a_ij, x_j, sum) -- the
+// arithmetic of MatMult_SeqAIJ (src/ported.mojo:380-405) -- so a row gives
+// the same bits as the CPU path.
+#include <algorithm>
+#include <map>
+#include <tuple>
+#include <vector>
+
+#include "mpk_internal.h"
+#include "mpk_ops.h"
+#include "../../include/mpk.h"
+
+namespace mpk {
+
+// One lane per row.  Row i's values and indices are contiguous, so a wave
+// reads a contiguous span of the value and index streams spread over its
+// rows; the lines are reused across the row loop from L1/L2.
+__global__ void __launch_bounds__(256) k_spmv_row(i64 nrow, const int *__restrict__ rp,
+                                                  const int *__restrict__ ci,
+                                                  const double *__restrict__ v,
+                                                  const double *__restrict__ x, double *__restrict__ y,
+                                                  int add, const int64_t *stop) {
+  if (stop && *stop) return;
+  i64 i = (i64)blockIdx.x * 256 + threadIdx.x;
+  if (i >= nrow) return;
+  int j = rp[i], je = rp[i + 1];
+  double s = add ? y[i] : 0.0;
+  for (; j < je; j++) s = fma(v[j], x[ci[j]], s);
+  y[i] = s;
+}
+
+void spmv(const DevCSR &A, const double *x, double *y, int add, const int64_t *stop) {
+  if (A.nrow <= 0 || !ctx()) return;
+  MPK_LAUNCH(k_spmv_row, nblocks(A.nrow, 256), 256, A.nrow, A.rp, A.ci, A.v, x, y, add, stop);
+}
+
+// ------------------------------------------------------------ transpose
+
+// y = A^T x as a gather over the transpose: column c's entries are visited in
+// increasing row, which is the order the reference's scatter adds them in
+// (ported.mojo:408-431).  tp[k] is the position of the entry in A's arrays.
+__global__ void __launch_bounds__(256) k_spmv_t(i64 ncol, const int *trp, const int *trow,
+                                                const int *tp, const double *v, const double *x,
+                                                double *y) {
+  i64 c = (i64)blockIdx.x * 256 + threadIdx.x;
+  if (c >= ncol) return;
+  double s = 0.0;
+  for (int k = trp[c]; k < trp[c + 1]; k++) s = fma(v[tp[k]], x[trow[k]], s);
+  y[c] = s;
+}
+
+struct TransposeMap {
+  int *trp = nullptr, *trow = nullptr, *tp = nullptr;
+};
+static std::map<std::tuple<const void *, const void *, i64, i64, i64>, TransposeMap> g_tmaps;
+
+static TransposeMap *transpose_map(const int *rp, const int *ci, i64 nrow, i64 ncol) {
+  Ctx *c = ctx();
+  if (!c) return nullptr;
+  std::vector<int> hrp(nrow + 1);
+  HIPCHK(hipMemcpyAsync(hrp.data(), rp, (nrow + 1) * sizeof(int), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  i64 nnz = hrp[nrow];
+  auto key = std::make_tuple((const void *)rp, (const void *)ci, nrow, ncol, nnz);
+  auto it = g_tmaps.find(key);
+  if (it != g_tmaps.end()) return &it->second;
+  std::vector<int> hci(nnz > 0 ? nnz : 1);
+  if (nnz > 0)
+    HIPCHK(hipMemcpyAsync(hci.data(), ci, nnz * sizeof(int), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  std::vector<int> trp(ncol + 1, 0), trow(nnz > 0 ? nnz : 1), tp(nnz > 0 ? nnz : 1);
+  for (i64 k = 0; k < nnz; k++) trp[hci[k] + 1]++;
+  for (i64 col = 0; col < ncol; col++) trp[col + 1] += trp[col];
+  std::vector<int> fill(trp.begin(), trp.end() - 1);
+  for (i64 i = 0; i < nrow; i++)
+    for (int k = hrp[i]; k < hrp[i + 1]; k++) {
+      int pos = fill[hci[k]]++;
+      trow[pos] = (int)i;
+      tp[pos] = k;
+    }
+  TransposeMap m;
+  HIPCHK(hipMalloc(&m.trp, (ncol + 1) * sizeof(int)));
+  HIPCHK(hipMalloc(&m.trow, trow.size() * sizeof(int)));
+  HIPCHK(hipMalloc(&m.tp, tp.size() * sizeof(int)));
+  HIPCHK(hipMemcpy(m.trp, trp.data(), (ncol + 1) * sizeof(int), hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(m.trow, trow.data(), trow.size() * sizeof(int), hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(m.tp, tp.data(), tp.size() * sizeof(int), hipMemcpyHostToDevice));
+  return &(g_tmaps[key] = m);
+}
+
+// ------------------------------------------------------------ row scans
+
+__global__ void k_get_diagonal(i64 nrow, const int *rp, const int *ci, const double *v, double *d) {
+  i64 i = (i64)blockIdx.x * 256 + threadIdx.x;
+  if (i >= nrow) return;
+  double r = 0.0;
+  for (int j = rp[i]; j < rp[i + 1]; j++)
+    if (ci[j] == i) {
+      r = v[j];
+      break;
+    }
+  d[i] = r;
+}
+
+// diag[i] = position of a_ii in row i, or the row end; *dense cleared to 0
+// by any row without a diagonal (ported.mojo:354-377)
+__global__ void k_diag_markers(i64 nrow, const int *rp, const int *ci, int *diag, int *dense) {
+  i64 i = (i64)blockIdx.x * 256 + threadIdx.x;
+  if (i >= nrow) return;
+  int be = rp[i + 1], m = be;
+  for (int j = rp[i]; j < be; j++)
+    if (ci[j] == i) {
+      m = j;
+      break;
+    }
+  diag[i] = m;
+  if (m == be) atomicAnd(dense, 0);
+}
+
+__global__ void k_scale_vals(i64 nnz, double *v, double alpha) {
+  i64 j = (i64)blockIdx.x * 256 + threadIdx.x;
+  if (j < nnz) v[j] = v[j] * alpha;
+}
+
+}  // namespace mpk
+
+using namespace mpk;
+
+extern "C" {
+
+void mp_mat_mult(int64_t x, int64_t y, int64_t irow, int64_t icol, int64_t data, int64_t nrow) {
+  DevCSR A{nrow, 0, (const int *)irow, (const int *)icol, (const double *)data};
+  spmv(A, (const double *)x, (double *)y, 0, nullptr);
+}
+
+void mp_mat_mult_add(int64_t x, int64_t y, int64_t irow, int64_t icol, int64_t data, int64_t nrow) {
+  DevCSR A{nrow, 0, (const int *)irow, (const int *)icol, (const double *)data};
+  spmv(A, (const double *)x, (double *)y, 1, nullptr);
+}
+
+void mp_mat_mult_transpose(int64_t x, int64_t y, int64_t irow, int64_t icol, int64_t data, int64_t nrow,
+                           int64_t ncols) {
+  if (ncols <= 0 || !ctx()) return;
+  if (nrow <= 0) {
+    vset((double *)y, 0.0, ncols, nullptr);
+    return;
+  }
+  TransposeMap *m = transpose_map((const int *)irow, (const int *)icol, nrow, ncols);
+  if (!m) return;
+  MPK_LAUNCH(k_spmv_t, nblocks(ncols, 256), 256, (i64)ncols, (const int *)m->trp, (const int *)m->trow,
+             (const int *)m->tp, (const double *)data, (const double *)x, (double *)y);
+}
+
+void mp_mat_get_diagonal(int64_t diag, int64_t irow, int64_t icol, int64_t data, int64_t nrow) {
+  if (nrow <= 0 || !ctx()) return;
+  MPK_LAUNCH(k_get_diagonal, nblocks(nrow, 256), 256, (i64)nrow, (const int *)irow, (const int *)icol,
+             (const double *)data, (double *)diag);
+}
+
+void mp_mat_scale(int64_t data, int64_t nnz, double alpha) {
+  if (nnz <= 0 || !ctx()) return;
+  MPK_LAUNCH(k_scale_vals, nblocks(nnz, 256), 256, (i64)nnz, (double *)data, alpha);
+}
+
+void mp_mat_zero_entries(int64_t data, int64_t nnz) {
+  if (nnz <= 0 || !ctx()) return;
+  vset((double *)data, 0.0, nnz, nullptr);
+}
+
+int64_t mp_mat_get_diagonal_markers(int64_t irow, int64_t icol, int64_t diag, int64_t nrow) {
+  Ctx *c = ctx();
+  if (!c) return 0;
+  if (nrow <= 0) return 1;
+  int *dense = reinterpret_cast<int *>(c->dres);
+  int one = 1;
+  HIPCHK(hipMemcpyAsync(dense, &one, sizeof(int), hipMemcpyHostToDevice, c->stream));
+  MPK_LAUNCH(k_diag_markers, nblocks(nrow, 256), 256, (i64)nrow, (const int *)irow, (const int *)icol,
+             (int *)diag, dense);
+  int h = 0;
+  HIPCHK(hipMemcpyAsync(&h, dense, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  return h;
+}
+
+// Drop cached transpose maps (call after changing a matrix's pattern in place).
+void mpk_forget_matrix(int64_t irow) {
+  for (auto it = g_tmaps.begin(); it != g_tmaps.end();) {
+    if (std::get<0>(it->first) == (const void *)irow) {
+      hipFree(it->second.trp);
+      hipFree(it->second.trow);
+      hipFree(it->second.tp);
+      it = g_tmaps.erase(it);
+    } else {
+      ++it;
+    }
+  }
+}
+
+}  // extern "C"
+
+extern "C" {
+
+// Average duration in ms of `reps` back-to-back SpMV launches, timed with HIP
+// events on the library's stream (the stream the kernel runs on).
+double mpk_time_spmv(int64_t x, int64_t y, int64_t irow, int64_t icol, int64_t data, int64_t nrow, int64_t reps) {
+  Ctx *c = ctx();
+  if (!c || reps <= 0) return -1.0;
+  DevCSR A{nrow, 0, (const int *)irow, (const int *)icol, (const double *)data};
+  hipEvent_t e0, e1;
+  HIPCHK(hipEventCreate(&e0));
+  HIPCHK(hipEventCreate(&e1));
+  HIPCHK(hipEventRecord(e0, c->stream));
+  for (int64_t r = 0; r < reps; r++) spmv(A, (const double *)x, (double *)y, 0, nullptr);
+  HIPCHK(hipEventRecord(e1, c->stream));
+  HIPCHK(hipEventSynchronize(e1));
+  float ms = 0.0f;
+  HIPCHK(hipEventElapsedTime(&ms, e0, e1));
+  HIPCHK(hipEventDestroy(e0));
+  HIPCHK(hipEventDestroy(e1));
+  return (double)ms / (double)reps;
+}
+
+}  // extern "C"

@@ -1,0 +1,60 @@
+// mpk_pc.h -- device preconditioner descriptors shared by the PC exports and
+// the Krylov drivers.
+#pragma once
+
+#include "mpk_internal.h"
+
+namespace mpk {
+
+// PC type ordinals of the descriptor block (petscksp.h PCType, PC.py:23-27)
+enum { PCT_NONE = 0, PCT_JACOBI = 2, PCT_SOR = 4, PCT_ILU = 5, PCT_ICC = 6 };
+
+// ILU factor in the reference layout (src/factor.mojo:31-67, 335-381): L rows
+// ascending, U rows in reverse row order each ending on its inverted diagonal.
+struct IluFactor {
+  i64 n;
+  const int *bi, *bj;
+  const double *ba;
+  const int *bdiag, *r, *c;
+};
+
+// IC factor in the reference layout (factor.mojo:125-273): row i holds its
+// strict upper columns then the diagonal slot (1/d_i); plus the transpose of
+// the strict part for the forward gather.
+struct IccFactor {
+  i64 n;
+  const int *ui, *uj;
+  const double *ua;
+  const int *udiag, *c;
+  const int *trp, *trow, *tpos;
+};
+
+struct SorData {
+  i64 n;
+  const int *rp, *ci;
+  const double *a;
+  const int *dmark;
+  double *idiag, *mdiag, *t;
+  double omega, fshift;
+  int flag;
+};
+
+void ilu_apply(const IluFactor &f, const double *b, double *x, const int64_t *stop);
+void icc_apply(const IccFactor &f, const double *b, double *x, const int64_t *stop);
+void sor_apply(const SorData &s, const double *x, double *y, const int64_t *stop);
+
+// A preconditioner as the Krylov drivers see it: parsed from the descriptor
+// block `pci` (device addresses, int32 index arrays) and `pcf` (omega, fshift).
+struct PCView {
+  int type = PCT_NONE;
+  i64 n = 0;
+  const double *jac_diag = nullptr, *jac_diagsqrt = nullptr;
+  SorData sor{};
+  IluFactor ilu{};
+  IccFactor icc{};
+  bool parse(const int64_t *pci, const double *pcf, i64 n);
+  void apply(const double *x, double *y, const int64_t *stop) const;            // y = M^-1 x
+  void apply_symmetric(const double *x, double *y, const int64_t *stop) const;  // half of M^-1
+};
+
+}  // namespace mpk

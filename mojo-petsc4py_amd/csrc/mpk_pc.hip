@@ -1,0 +1,440 @@
+// mpk_pc.hip -- preconditioner kernels: Jacobi set up and apply, SOR, and the
+// ILU / IC triangular solves, plus the mp_pc_* device exports.
+//
+// Triangular solves and SOR sweeps are sync-free (mpk_sweep.h): each row is a
+// serial sum in the order the reference's loop visits it, so the factor apply
+// gives the same bits as ILUApplyFactor_SeqAIJ / ICCApplyFactor_SeqAIJ /
+// PCApply_SOR (src/ported.mojo:633-952) on the same factor.
+#include <map>
+#include <tuple>
+#include <vector>
+
+#include "mpk_internal.h"
+#include "mpk_ops.h"
+#include "mpk_pc.h"
+#include "mpk_sweep.h"
+#include "../../include/mpk.h"
+
+namespace mpk {
+
+// ------------------------------------------------------------ sweep plumbing
+
+// Fills the published-value buffer with the sentinel and zeroes the ticket
+// counter; the sweep that follows on the stream starts from both.
+__global__ void k_sweep_prepare(i64 n, double *p, unsigned *ticket, const int64_t *stop) {
+  if (stop && *stop) return;
+  i64 i = (i64)blockIdx.x * 256 + threadIdx.x;
+  if (i == 0) *ticket = 0u;
+  if (i < n) p[i] = __longlong_as_double((long long)MPK_SENTINEL);
+}
+
+template <class Prog>
+static void run_sweep(const Prog &prog, i64 n, bool forward, double *pub, const int64_t *stop) {
+  Ctx *c = ctx();
+  if (!c || n <= 0) return;
+  MPK_LAUNCH(k_sweep_prepare, nblocks(n, 256), 256, n, pub, c->tickets, stop);
+  MPK_LAUNCH(k_sweep<Prog>, nblocks(n, 64), 64, prog, n, forward ? 1 : 0, c->tickets, 0u, pub, stop);
+}
+
+// ------------------------------------------------------------ row programs
+
+struct IluFwd {  // tmp_i = b[r_i] - sum_{L row i} ba_j tmp[bj_j]
+  const int *bi, *bj, *r;
+  const double *ba, *b;
+  __device__ double init(i64 i) const { return b[r[i]]; }
+  __device__ int begin(i64 i) const { return bi[i]; }
+  __device__ int end(i64 i) const { return bi[i + 1]; }
+  __device__ i64 col(int j) const { return bj[j]; }
+  __device__ double coef(int j) const { return -ba[j]; }
+  __device__ double finish(i64, double acc) const { return acc; }
+  __device__ double stale(i64) const { return 0.0; }
+};
+struct IluBwd {  // sum = tmp_i - sum_{U row i} ba tmp; x[c_i] = sum * ba[bdiag_i]
+  const int *bj, *bdiag, *c;
+  const double *ba, *F;
+  double *x;
+  __device__ double init(i64 i) const { return F[i]; }
+  __device__ int begin(i64 i) const { return bdiag[i + 1] + 1; }
+  __device__ int end(i64 i) const { return bdiag[i]; }
+  __device__ i64 col(int j) const { return bj[j]; }
+  __device__ double coef(int j) const { return -ba[j]; }
+  __device__ double finish(i64 i, double acc) const {
+    double r = acc * ba[bdiag[i]];
+    x[c[i]] = r;
+    return r;
+  }
+  __device__ double stale(i64) const { return 0.0; }
+};
+// IC forward: the reference scatters s_k += ua_ik y_i for i ascending; here
+// row k gathers over the transpose in the same ascending i.
+struct IccFwd {
+  const int *trp, *trow, *tpos, *udiag, *c;
+  const double *ua, *b;
+  double *F;  // scaled values for the backward sweep
+  __device__ double init(i64 k) const { return b[c[k]]; }
+  __device__ int begin(i64 k) const { return trp[k]; }
+  __device__ int end(i64 k) const { return trp[k + 1]; }
+  __device__ i64 col(int j) const { return trow[j]; }
+  __device__ double coef(int j) const { return ua[tpos[j]]; }
+  __device__ double finish(i64 k, double acc) const {
+    F[k] = acc * ua[udiag[k]];
+    return acc;
+  }
+  __device__ double stale(i64) const { return 0.0; }
+};
+struct IccBwd {
+  const int *ui, *uj, *udiag, *c;
+  const double *ua, *F;
+  double *x;
+  __device__ double init(i64 i) const { return F[i]; }
+  __device__ int begin(i64 i) const { return ui[i]; }
+  __device__ int end(i64 i) const { return udiag[i]; }
+  __device__ i64 col(int j) const { return uj[j]; }
+  __device__ double coef(int j) const { return ua[j]; }
+  __device__ double finish(i64 i, double acc) const {
+    x[c[i]] = acc;
+    return acc;
+  }
+  __device__ double stale(i64) const { return 0.0; }
+};
+// SOR, zero initial guess (ported.mojo:655-683)
+struct SorFwd {
+  const int *rp, *ci, *dmark;
+  const double *a, *x, *idiag;
+  double *t, *yout;  // yout: final y when no backward sweep follows (else null)
+  const double *yin;
+  __device__ double init(i64 i) const { return x[i]; }
+  __device__ int begin(i64 i) const { return rp[i]; }
+  __device__ int end(i64 i) const { return dmark[i]; }
+  __device__ i64 col(int j) const { return ci[j]; }
+  __device__ double coef(int j) const { return -a[j]; }
+  __device__ double finish(i64 i, double acc) const {
+    t[i] = acc;
+    double r = acc * idiag[i];
+    if (yout) yout[i] = r;
+    return r;
+  }
+  __device__ double stale(i64 k) const { return yin[k]; }
+};
+struct SorBwd {
+  const int *rp, *ci, *dmark;
+  const double *a, *src, *idiag, *yf;  // src: t after a forward sweep, else x
+  double *y;
+  double omega;
+  int use_t;
+  __device__ double init(i64 i) const { return src[i]; }
+  __device__ int begin(i64 i) const { return dmark[i] + 1; }
+  __device__ int end(i64 i) const { return rp[i + 1]; }
+  __device__ i64 col(int j) const { return ci[j]; }
+  __device__ double coef(int j) const { return -a[j]; }
+  __device__ double finish(i64 i, double acc) const {
+    double r = use_t ? fma(1.0 - omega, yf[i], acc * idiag[i]) : acc * idiag[i];
+    y[i] = r;
+    return r;
+  }
+  __device__ double stale(i64 k) const { return y[k]; }
+};
+
+// MatInvertDiagonalForSOR_SeqAIJ (ported.mojo:610-630)
+__global__ void k_sor_idiag(i64 n, const double *a, const int *dmark, double *idiag, double *mdiag,
+                            double omega, double fshift, const int64_t *stop) {
+  if (stop && *stop) return;
+  i64 i = (i64)blockIdx.x * 256 + threadIdx.x;
+  if (i >= n) return;
+  double d = a[dmark[i]];
+  mdiag[i] = d;
+  idiag[i] = (omega == 1.0 && fshift <= 0.0) ? 1.0 / d : omega / (fshift + d);
+}
+
+// ------------------------------------------------------------ applies
+
+static double *sweep_scratch(i64 n) { return scratch_vector((size_t)3 * (size_t)n); }
+
+void ilu_apply(const IluFactor &f, const double *b, double *x, const int64_t *stop) {
+  i64 n = f.n;
+  double *S = sweep_scratch(n);
+  if (!S || n <= 0) return;
+  double *F = S, *B = S + n;
+  run_sweep(IluFwd{f.bi, f.bj, f.r, f.ba, b}, n, true, F, stop);
+  run_sweep(IluBwd{f.bj, f.bdiag, f.c, f.ba, F, x}, n, false, B, stop);
+}
+
+void icc_apply(const IccFactor &f, const double *b, double *x, const int64_t *stop) {
+  i64 n = f.n;
+  double *S = sweep_scratch(n);
+  if (!S || n <= 0) return;
+  double *Y = S, *F = S + n, *B = S + 2 * n;
+  run_sweep(IccFwd{f.trp, f.trow, f.tpos, f.udiag, f.c, f.ua, b, F}, n, true, Y, stop);
+  run_sweep(IccBwd{f.ui, f.uj, f.udiag, f.c, f.ua, F, x}, n, false, B, stop);
+}
+
+void sor_apply(const SorData &s, const double *x, double *y, const int64_t *stop) {
+  i64 n = s.n;
+  double *S = sweep_scratch(n);
+  if (!S || n <= 0) return;
+  MPK_LAUNCH(k_sor_idiag, nblocks(n, 256), 256, n, s.a, s.dmark, s.idiag, s.mdiag, s.omega, s.fshift, stop);
+  const bool fwd = (s.flag & (1 | 4)) != 0, bwd = (s.flag & (2 | 8)) != 0;
+  double *YF = S, *YB = S + n;
+  if (fwd) run_sweep(SorFwd{s.rp, s.ci, s.dmark, s.a, x, s.idiag, s.t, bwd ? nullptr : y, y}, n, true, YF, stop);
+  if (bwd)
+    run_sweep(SorBwd{s.rp, s.ci, s.dmark, s.a, fwd ? s.t : x, s.idiag, YF, y, s.omega, fwd ? 1 : 0}, n,
+              false, YB, stop);
+  // its * lits > 1 never reaches here: the apply is its = lits = 1 (capi.mojo:272-280)
+}
+
+void PCView::apply(const double *x, double *y, const int64_t *stop) const {
+  switch (type) {
+    case PCT_JACOBI:
+      vpointwise_mult(y, x, jac_diag, n, stop);
+      break;
+    case PCT_ILU:
+      ilu_apply(ilu, x, y, stop);
+      break;
+    case PCT_ICC:
+      icc_apply(icc, x, y, stop);
+      break;
+    case PCT_SOR:
+      sor_apply(sor, x, y, stop);
+      break;
+    default:
+      vcopy(x, y, n);
+  }
+}
+
+// PCApplySymmetric (ksp.mojo:201-208).  The Jacobi square root is read from
+// the descriptor's aux1 slot, where the set up stores it.
+void PCView::apply_symmetric(const double *x, double *y, const int64_t *stop) const {
+  if (type == PCT_JACOBI)
+    vpointwise_mult(y, x, jac_diagsqrt, n, stop);
+  else
+    apply(x, y, stop);
+}
+
+// ------------------------------------------------------------ IC transpose map
+
+struct IccMap {
+  int *trp = nullptr, *trow = nullptr, *tpos = nullptr;
+};
+static std::map<std::tuple<const void *, const void *, const void *, i64>, IccMap> g_iccmaps;
+
+// Transpose of the strict upper part of an IC factor: for column k, the rows
+// i < k holding it, ascending, and the positions of those entries in ua.
+static bool build_icc_map(const int *ui, const int *uj, const int *udiag, i64 n, int *trp_d, int *trow_d,
+                          int *tpos_d) {
+  Ctx *c = ctx();
+  if (!c) return false;
+  std::vector<int> hui(n + 1), hud(n);
+  HIPCHK(hipMemcpy(hui.data(), ui, (n + 1) * sizeof(int), hipMemcpyDeviceToHost));
+  HIPCHK(hipMemcpy(hud.data(), udiag, n * sizeof(int), hipMemcpyDeviceToHost));
+  i64 nnz = n > 0 ? hui[n] : 0;
+  std::vector<int> huj(nnz > 0 ? nnz : 1);
+  if (nnz > 0) HIPCHK(hipMemcpy(huj.data(), uj, nnz * sizeof(int), hipMemcpyDeviceToHost));
+  std::vector<int> trp(n + 1, 0);
+  for (i64 i = 0; i < n; i++)
+    for (int j = hui[i]; j < hud[i]; j++) trp[huj[j] + 1]++;
+  for (i64 k = 0; k < n; k++) trp[k + 1] += trp[k];
+  i64 m = trp[n];
+  std::vector<int> trow(m > 0 ? m : 1), tpos(m > 0 ? m : 1), fill(trp.begin(), trp.end() - 1);
+  for (i64 i = 0; i < n; i++)
+    for (int j = hui[i]; j < hud[i]; j++) {
+      int p = fill[huj[j]]++;
+      trow[p] = (int)i;
+      tpos[p] = j;
+    }
+  HIPCHK(hipMemcpy(trp_d, trp.data(), (n + 1) * sizeof(int), hipMemcpyHostToDevice));
+  if (m > 0) {
+    HIPCHK(hipMemcpy(trow_d, trow.data(), m * sizeof(int), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(tpos_d, tpos.data(), m * sizeof(int), hipMemcpyHostToDevice));
+  }
+  return !failed();
+}
+
+i64 icc_map_count(const int *ui, const int *udiag, i64 n) {
+  std::vector<int> hui(n + 1), hud(n);
+  HIPCHK(hipMemcpy(hui.data(), ui, (n + 1) * sizeof(int), hipMemcpyDeviceToHost));
+  HIPCHK(hipMemcpy(hud.data(), udiag, n * sizeof(int), hipMemcpyDeviceToHost));
+  i64 m = 0;
+  for (i64 i = 0; i < n; i++) m += hud[i] - hui[i];
+  return m;
+}
+
+static IccMap *icc_map(const int *ui, const int *uj, const int *udiag, i64 n) {
+  auto key = std::make_tuple((const void *)ui, (const void *)uj, (const void *)udiag, n);
+  auto it = g_iccmaps.find(key);
+  if (it != g_iccmaps.end()) return &it->second;
+  if (!ctx()) return nullptr;
+  i64 m = icc_map_count(ui, udiag, n);
+  IccMap mp;
+  HIPCHK(hipMalloc(&mp.trp, (n + 1) * sizeof(int)));
+  HIPCHK(hipMalloc(&mp.trow, (m > 0 ? m : 1) * sizeof(int)));
+  HIPCHK(hipMalloc(&mp.tpos, (m > 0 ? m : 1) * sizeof(int)));
+  if (!build_icc_map(ui, uj, udiag, n, mp.trp, mp.trow, mp.tpos)) return nullptr;
+  return &(g_iccmaps[key] = mp);
+}
+
+bool PCView::parse(const int64_t *pci, const double *pcf, i64 n_) {
+  n = n_;
+  type = (int)pci[0];
+  switch (type) {
+    case PCT_JACOBI:
+      jac_diag = (const double *)pci[1];
+      jac_diagsqrt = (const double *)pci[2];
+      return true;
+    case PCT_SOR:
+      sor = SorData{n, (const int *)pci[1], (const int *)pci[2], (const double *)pci[3], (const int *)pci[4],
+                    (double *)pci[5], (double *)pci[6], (double *)pci[7], pcf[0], pcf[1], (int)pci[10]};
+      return true;
+    case PCT_ILU:
+      ilu = IluFactor{n, (const int *)pci[1], (const int *)pci[2], (const double *)pci[3], (const int *)pci[4],
+                      (const int *)pci[5], (const int *)pci[6]};
+      return true;
+    case PCT_ICC: {
+      icc.n = n;
+      icc.ui = (const int *)pci[1];
+      icc.uj = (const int *)pci[2];
+      icc.ua = (const double *)pci[3];
+      icc.udiag = (const int *)pci[4];
+      icc.c = (const int *)pci[5];
+      if (pci[6] && pci[7] && pci[8]) {
+        icc.trp = (const int *)pci[6];
+        icc.trow = (const int *)pci[7];
+        icc.tpos = (const int *)pci[8];
+      } else {
+        IccMap *m = icc_map(icc.ui, icc.uj, icc.udiag, n);
+        if (!m) return false;
+        icc.trp = m->trp;
+        icc.trow = m->trow;
+        icc.tpos = m->tpos;
+      }
+      return true;
+    }
+    default:
+      type = PCT_NONE;
+      return true;
+  }
+}
+
+// ------------------------------------------------------------ Jacobi set up
+
+__device__ __forceinline__ double jacobi_fill(int jtype, i64 i, const int *rp, const int *ci,
+                                              const double *v) {
+  double s = 0.0;
+  if (jtype == 0) {
+    for (int j = rp[i]; j < rp[i + 1]; j++)
+      if (ci[j] == i) {
+        s = v[j];
+        break;
+      }
+  } else if (jtype == 1) {
+    for (int j = rp[i]; j < rp[i + 1]; j++) {
+      double a = fabs(v[j]);
+      s = a > s ? a : s;
+    }
+  } else if (jtype == 2) {
+    for (int j = rp[i]; j < rp[i + 1]; j++) s = s + fabs(v[j]);
+  } else {
+    for (int j = rp[i]; j < rp[i + 1]; j++) s = s + v[j];
+  }
+  return s;
+}
+
+// PCSetUp_Jacobi (ported.mojo:502-541), one row per lane
+__global__ void k_jacobi_setup(i64 n, int jtype, int useabs, int fixdiag, double scale, const int *rp,
+                               const int *ci, const double *v, double *diag, double *didx) {
+  i64 i = (i64)blockIdx.x * 256 + threadIdx.x;
+  if (i >= n) return;
+  double d = jacobi_fill(jtype, i, rp, ci, v);
+  if (jtype == 2) {
+    double dx = jacobi_fill(0, i, rp, ci, v);
+    didx[i] = dx;
+    if (dx < 0.0) d = -d;
+    if (scale != 1.0) d = fma(scale, d - dx, dx);
+  }
+  d = d == 0.0 ? 0.0 : 1.0 / d;
+  if (useabs) d = fabs(d);
+  if (fixdiag && d == 0.0) d = 1.0;
+  diag[i] = d;
+}
+
+// PCSetUp_Jacobi_Symmetric (ported.mojo:544-562)
+__global__ void k_jacobi_setup_sym(i64 n, int jtype, const int *rp, const int *ci, const double *v,
+                                   double *dsq) {
+  i64 i = (i64)blockIdx.x * 256 + threadIdx.x;
+  if (i >= n) return;
+  double d = jacobi_fill(jtype, i, rp, ci, v);
+  if (d < 0.0)
+    d = 1.0 / sqrt(fabs(-d));
+  else if (d > 0.0)
+    d = 1.0 / sqrt(fabs(d));
+  else
+    d = 1.0;
+  dsq[i] = d;
+}
+
+}  // namespace mpk
+
+using namespace mpk;
+
+extern "C" {
+
+void mp_pc_jacobi_setup(int64_t diag, int64_t jtype, int64_t useabs, int64_t fixdiag, double scale, int64_t irow,
+                        int64_t icol, int64_t data, int64_t nrow, int64_t didx) {
+  if (nrow <= 0 || !ctx()) return;
+  MPK_LAUNCH(k_jacobi_setup, nblocks(nrow, 256), 256, (i64)nrow, (int)jtype, (int)useabs, (int)fixdiag, scale,
+             (const int *)irow, (const int *)icol, (const double *)data, (double *)diag, (double *)didx);
+}
+
+void mp_pc_jacobi_setup_symmetric(int64_t diagsqrt, int64_t jtype, int64_t irow, int64_t icol, int64_t data,
+                                  int64_t nrow) {
+  if (nrow <= 0 || !ctx()) return;
+  MPK_LAUNCH(k_jacobi_setup_sym, nblocks(nrow, 256), 256, (i64)nrow, (int)jtype, (const int *)irow,
+             (const int *)icol, (const double *)data, (double *)diagsqrt);
+}
+
+void mp_pc_jacobi_apply(int64_t y, int64_t x, int64_t diag, int64_t n) {
+  vpointwise_mult((double *)y, (const double *)x, (const double *)diag, n, nullptr);
+}
+
+void mp_pc_sor_apply(int64_t x, int64_t y, int64_t irow, int64_t icol, int64_t data, int64_t dmark, int64_t idiag,
+                     int64_t mdiag, int64_t t, double omega, double fshift, int64_t flag, int64_t nrow) {
+  SorData s{nrow, (const int *)irow, (const int *)icol, (const double *)data, (const int *)dmark,
+            (double *)idiag, (double *)mdiag, (double *)t, omega, fshift, (int)flag};
+  sor_apply(s, (const double *)x, (double *)y, nullptr);
+}
+
+void mp_pc_ilu_apply(int64_t b, int64_t x, int64_t tmp, int64_t bi, int64_t bj, int64_t ba, int64_t bdiag,
+                     int64_t r, int64_t c, int64_t nrow) {
+  (void)tmp;  // the sweeps keep their published values in the library's scratch
+  IluFactor f{nrow, (const int *)bi, (const int *)bj, (const double *)ba, (const int *)bdiag, (const int *)r,
+              (const int *)c};
+  ilu_apply(f, (const double *)b, (double *)x, nullptr);
+}
+
+int64_t mp_pc_icc_apply(int64_t b, int64_t x, int64_t ui, int64_t uj, int64_t ua, int64_t udiag, int64_t c,
+                        int64_t scratch, int64_t nrow) {
+  (void)scratch;
+  if (nrow <= 0) return 0;
+  int64_t pci[12] = {PCT_ICC, ui, uj, ua, udiag, c, 0, 0, 0, 0, 0, 0};
+  PCView v;
+  if (!v.parse(pci, nullptr, nrow)) return 0;
+  icc_apply(v.icc, (const double *)b, (double *)x, nullptr);
+  return 0;
+}
+
+// The transpose of an IC factor's strict upper part, for the forward gather:
+// trp[n + 1], trow[m], tpos[m] (device, int32) with m = mpk_icc_transpose_size.
+int64_t mpk_icc_transpose_size(int64_t ui, int64_t udiag, int64_t nrow) {
+  if (nrow <= 0 || !ctx()) return 0;
+  return icc_map_count((const int *)ui, (const int *)udiag, nrow);
+}
+
+int64_t mpk_icc_transpose(int64_t ui, int64_t uj, int64_t udiag, int64_t nrow, int64_t trp, int64_t trow,
+                          int64_t tpos) {
+  if (nrow <= 0) return 0;
+  return build_icc_map((const int *)ui, (const int *)uj, (const int *)udiag, nrow, (int *)trp, (int *)trow,
+                       (int *)tpos)
+             ? 0
+             : -1;
+}
+
+}  // extern "C"

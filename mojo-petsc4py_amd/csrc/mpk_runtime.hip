@@ -1,0 +1,196 @@
+// mpk_runtime.hip -- context, memory and error plumbing behind the C ABI.
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <mutex>
+
+#include "mpk_internal.h"
+#include "../../include/mpk.h"
+
+namespace mpk {
+
+static Ctx g_ctx;
+static bool g_init_done = false;
+static std::mutex g_mu;
+static int g_status = 0;
+static char g_msg[1024] = "";
+
+void set_error(int code, const char *fmt, ...) {
+  if (g_status != 0) return;  // keep the first error
+  g_status = code;
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(g_msg, sizeof g_msg, fmt, ap);
+  va_end(ap);
+}
+
+bool failed() { return g_status != 0; }
+
+static bool init_locked(int device) {
+  if (g_init_done) return g_ctx.device >= 0;
+  g_init_done = true;
+  int count = 0;
+  hipError_t e = hipGetDeviceCount(&count);
+  if (e != hipSuccess || count == 0) {
+    set_error(2, "no HIP device is available (%s)", e != hipSuccess ? hipGetErrorString(e) : "count 0");
+    return false;
+  }
+  if (device < 0) HIPCHK(hipGetDevice(&device));
+  HIPCHK(hipSetDevice(device));
+  // default priority: the stream is created without a priority argument
+  HIPCHK(hipStreamCreateWithFlags(&g_ctx.stream, hipStreamNonBlocking));
+  HIPCHK(hipMalloc(&g_ctx.dres, 1024 * sizeof(double)));
+  HIPCHK(hipHostMalloc(&g_ctx.hres, 1024 * sizeof(double), hipHostMallocDefault));
+  HIPCHK(hipMalloc(&g_ctx.tickets, 64 * sizeof(unsigned)));
+  HIPCHK(hipMemset(g_ctx.tickets, 0, 64 * sizeof(unsigned)));
+  HIPCHK(hipDeviceSynchronize());
+  if (failed()) return false;
+  g_ctx.device = device;
+  return true;
+}
+
+Ctx *ctx() {
+  if (g_init_done) return g_ctx.device >= 0 ? &g_ctx : nullptr;
+  std::lock_guard<std::mutex> lk(g_mu);
+  return init_locked(-1) ? &g_ctx : nullptr;
+}
+
+hipStream_t stream() {
+  Ctx *c = ctx();
+  return c ? c->stream : nullptr;
+}
+
+double *partials(size_t need) {
+  Ctx *c = ctx();
+  if (!c) return nullptr;
+  if (need > c->partials_cap) {
+    if (c->partials) {
+      HIPCHK(hipStreamSynchronize(c->stream));
+      HIPCHK(hipFree(c->partials));
+    }
+    size_t cap = need < 65536 ? 65536 : need;
+    HIPCHK(hipMalloc(&c->partials, cap * sizeof(double)));
+    c->partials_cap = cap;
+  }
+  return c->partials;
+}
+
+int *flag_buffer(size_t need) {
+  Ctx *c = ctx();
+  if (!c) return nullptr;
+  if (need > c->flags_cap) {
+    if (c->flags) {
+      HIPCHK(hipStreamSynchronize(c->stream));
+      HIPCHK(hipFree(c->flags));
+    }
+    size_t cap = need < 65536 ? 65536 : need;
+    HIPCHK(hipMalloc(&c->flags, cap * sizeof(int)));
+    HIPCHK(hipMemset(c->flags, 0, cap * sizeof(int)));
+    c->flags_cap = cap;
+  }
+  return c->flags;
+}
+
+double *scratch_vector(size_t need) {
+  Ctx *c = ctx();
+  if (!c) return nullptr;
+  if (need > c->dtmp_cap) {
+    if (c->dtmp) {
+      HIPCHK(hipStreamSynchronize(c->stream));
+      HIPCHK(hipFree(c->dtmp));
+    }
+    size_t cap = need < 4096 ? 4096 : need;
+    HIPCHK(hipMalloc(&c->dtmp, cap * sizeof(double)));
+    c->dtmp_cap = cap;
+  }
+  return c->dtmp;
+}
+
+void sync() {
+  Ctx *c = ctx();
+  if (!c) return;
+  HIPCHK(hipStreamSynchronize(c->stream));
+}
+
+}  // namespace mpk
+
+using namespace mpk;
+
+extern "C" {
+
+int64_t mpk_init(int64_t device) {
+  std::lock_guard<std::mutex> lk(g_mu);
+  return init_locked((int)device) ? 0 : (g_status ? g_status : 2);
+}
+
+int64_t mpk_status(void) { return g_status; }
+
+int64_t mpk_last_error(int64_t buf, int64_t len) {
+  if (buf && len > 0) {
+    strncpy(reinterpret_cast<char *>(buf), g_msg, (size_t)len - 1);
+    reinterpret_cast<char *>(buf)[len - 1] = 0;
+  }
+  return g_status;
+}
+
+void mpk_clear_error(void) {
+  g_status = 0;
+  g_msg[0] = 0;
+}
+
+int64_t mpk_device_name(int64_t buf, int64_t len) {
+  Ctx *c = ctx();
+  if (!c || !buf || len <= 0) return -1;
+  hipDeviceProp_t p;
+  HIPCHK(hipGetDeviceProperties(&p, c->device));
+  snprintf(reinterpret_cast<char *>(buf), (size_t)len, "%s (%s, %d CUs)", p.name, p.gcnArchName,
+           p.multiProcessorCount);
+  return 0;
+}
+
+int64_t mpk_malloc(int64_t bytes) {
+  if (!ctx()) return 0;
+  void *p = nullptr;
+  size_t b = bytes > 0 ? (size_t)bytes : 16;
+  hipError_t e = hipMalloc(&p, b);
+  if (e != hipSuccess) {
+    set_error(3, "hipMalloc(%lld) failed: %s", (long long)bytes, hipGetErrorString(e));
+    return 0;
+  }
+  return reinterpret_cast<int64_t>(p);
+}
+
+void mpk_free(int64_t ptr) {
+  if (!ptr || !ctx()) return;
+  HIPCHK(hipStreamSynchronize(stream()));
+  HIPCHK(hipFree(reinterpret_cast<void *>(ptr)));
+}
+
+void mpk_memcpy_htod(int64_t dst, int64_t src, int64_t bytes) {
+  if (bytes <= 0 || !ctx()) return;
+  HIPCHK(hipMemcpyAsync(reinterpret_cast<void *>(dst), reinterpret_cast<const void *>(src),
+                        (size_t)bytes, hipMemcpyHostToDevice, stream()));
+  HIPCHK(hipStreamSynchronize(stream()));
+}
+
+void mpk_memcpy_dtoh(int64_t dst, int64_t src, int64_t bytes) {
+  if (bytes <= 0 || !ctx()) return;
+  HIPCHK(hipMemcpyAsync(reinterpret_cast<void *>(dst), reinterpret_cast<const void *>(src),
+                        (size_t)bytes, hipMemcpyDeviceToHost, stream()));
+  HIPCHK(hipStreamSynchronize(stream()));
+}
+
+void mpk_memcpy_dtod(int64_t dst, int64_t src, int64_t bytes) {
+  if (bytes <= 0 || !ctx()) return;
+  HIPCHK(hipMemcpyAsync(reinterpret_cast<void *>(dst), reinterpret_cast<const void *>(src),
+                        (size_t)bytes, hipMemcpyDeviceToDevice, stream()));
+}
+
+void mpk_memset(int64_t dst, int64_t value, int64_t bytes) {
+  if (bytes <= 0 || !ctx()) return;
+  HIPCHK(hipMemsetAsync(reinterpret_cast<void *>(dst), (int)value, (size_t)bytes, stream()));
+}
+
+void mpk_synchronize(void) { sync(); }
+
+}  // extern "C"

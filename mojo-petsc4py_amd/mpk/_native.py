@@ -1,0 +1,159 @@
+"""Loader for libmpk.so, the HIP library behind the PETSc surface.
+
+The library is built in-tree by mojo-petsc4py_amd/csrc/build.py.  There is no
+CPU fallback: if the library is missing, or no HIP device is usable, `lib()`
+raises and nothing runs.
+"""
+
+from __future__ import annotations
+
+import ctypes
+import os
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB = os.environ.get("MPK_LIB") or os.path.join(HERE, "libmpk.so")
+
+I = ctypes.c_int64
+F = ctypes.c_double
+
+# name -> (argtypes, restype); the mp_* rows mirror the reference's table
+# (python/mojopetsc4py/_lib.py:23-87) entry for entry.
+SIGNATURES = {
+    # Mat
+    "mp_mat_mult": ([I, I, I, I, I, I], None),
+    "mp_mat_mult_add": ([I, I, I, I, I, I], None),
+    "mp_mat_mult_transpose": ([I, I, I, I, I, I, I], None),
+    "mp_mat_get_diagonal": ([I, I, I, I, I], None),
+    "mp_mat_scale": ([I, I, F], None),
+    "mp_mat_zero_entries": ([I, I], None),
+    # Vec
+    "mp_vec_pointwise_mult": ([I, I, I, I], None),
+    "mp_vec_pointwise_divide": ([I, I, I, I], None),
+    "mp_vec_pointwise_min": ([I, I, I, I], None),
+    "mp_vec_pointwise_max": ([I, I, I, I], None),
+    "mp_vec_pointwise_max_abs": ([I, I, I, I], None),
+    "mp_vec_norm": ([I, I, I, I], None),
+    "mp_vec_dot": ([I, I, I, I], None),
+    "mp_vec_tdot": ([I, I, I, I], None),
+    "mp_vec_sum": ([I, I, I], None),
+    "mp_vec_mean": ([I, I, I], None),
+    "mp_vec_dot_norm2": ([I, I, I, I, I], None),
+    "mp_vec_max": ([I, I, I, I], None),
+    "mp_vec_min": ([I, I, I, I], None),
+    "mp_vec_normalize": ([I, I, I], None),
+    "mp_vec_reciprocal": ([I, I], None),
+    "mp_vec_exp": ([I, I], None),
+    "mp_vec_log": ([I, I], None),
+    "mp_vec_sqrtabs": ([I, I], None),
+    "mp_vec_chop": ([I, I, F], None),
+    "mp_vec_abs": ([I, I], None),
+    "mp_vec_conjugate": ([I, I], None),
+    "mp_vec_copy": ([I, I, I], None),
+    "mp_vec_swap": ([I, I, I], None),
+    "mp_vec_set": ([I, F, I], None),
+    "mp_vec_zero_entries": ([I, I], None),
+    "mp_vec_scale": ([I, F, I], None),
+    "mp_vec_shift": ([I, F, I], None),
+    "mp_vec_axpy": ([I, F, I, I], None),
+    "mp_vec_aypx": ([I, F, I, I], None),
+    "mp_vec_axpby": ([I, F, F, I, I], None),
+    "mp_vec_axpbypcz": ([I, F, F, F, I, I, I], None),
+    "mp_vec_waxpy": ([I, F, I, I, I], None),
+    "mp_vec_maxpy": ([I, I, I, I, I], None),
+    "mp_vec_max_pointwise_divide": ([I, I, I, I], None),
+    # PC
+    "mp_mat_get_diagonal_markers": ([I, I, I, I], I),
+    "mp_pc_jacobi_setup": ([I, I, I, I, F, I, I, I, I, I], None),
+    "mp_pc_jacobi_setup_symmetric": ([I, I, I, I, I, I], None),
+    "mp_pc_jacobi_apply": ([I, I, I, I], None),
+    "mp_pc_sor_apply": ([I, I, I, I, I, I, I, I, I, F, F, I, I], None),
+    "mp_pc_ilu_apply": ([I] * 10, None),
+    "mp_pc_icc_apply": ([I] * 9, I),
+    "mp_pc_ilu_numeric": ([I] * 11, I),
+    "mp_pc_icc_numeric": ([I] * 13, None),
+    "mp_ilu_symbolic_ilu0": ([I] * 7, I),
+    "mp_ilu_symbolic": ([I] * 12, I),
+    "mp_icc_symbolic": ([I] * 13, I),
+    "mp_ilu_contiguous_lu": ([I] * 5, None),
+    "mp_ordering_rcm": ([I] * 7, None),
+    "mp_icc_contiguous_cholesky": ([I] * 7, None),
+    # KSP
+    "mp_ksp_solve_cg": ([I] * 12, None),
+    "mp_ksp_solve_bcgs": ([I] * 12, None),
+    "mp_ksp_solve_gmres": ([I] * 14, None),
+    "mp_ksp_solve_fgmres": ([I] * 15, None),
+    # runtime and additions without a reference counterpart
+    "mpk_init": ([I], I),
+    "mpk_status": ([], I),
+    "mpk_last_error": ([I, I], I),
+    "mpk_clear_error": ([], None),
+    "mpk_device_name": ([I, I], I),
+    "mpk_malloc": ([I], I),
+    "mpk_free": ([I], None),
+    "mpk_memcpy_htod": ([I, I, I], None),
+    "mpk_memcpy_dtoh": ([I, I, I], None),
+    "mpk_memcpy_dtod": ([I, I, I], None),
+    "mpk_memset": ([I, I, I], None),
+    "mpk_synchronize": ([], None),
+    "mpk_forget_matrix": ([I], None),
+    "mpk_time_spmv": ([I] * 7, F),
+    "mpk_icc_transpose_size": ([I, I, I], I),
+    "mpk_icc_transpose": ([I] * 7, I),
+}
+
+# exported names that only a device call can exercise
+MP_NAMES = [k for k in SIGNATURES if k.startswith("mp_")]
+
+
+class NativeError(RuntimeError):
+    pass
+
+
+_lib = None
+
+
+def load(init: bool = True) -> ctypes.CDLL:
+    """Load libmpk.so and set the signatures; with `init`, also open the HIP
+    device and raise if there is none."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB):
+            raise NativeError(
+                f"{LIB} is not built; run python mojo-petsc4py_amd/csrc/build.py")
+        lib_ = ctypes.CDLL(LIB)
+        for name, (argt, rest) in SIGNATURES.items():
+            fn = getattr(lib_, name)
+            fn.argtypes = argt
+            fn.restype = rest
+        _lib = lib_
+    if init:
+        rc = _lib.mpk_init(-1)
+        if rc != 0:
+            raise NativeError(f"libmpk could not open a HIP device: {last_error()}")
+    return _lib
+
+
+def lib() -> ctypes.CDLL:
+    return load(True)
+
+
+def last_error() -> str:
+    if _lib is None:
+        return ""
+    buf = ctypes.create_string_buffer(1024)
+    _lib.mpk_last_error(ctypes.addressof(buf), 1024)
+    return buf.value.decode(errors="replace")
+
+
+def check() -> None:
+    """Raise if the library has recorded a failure."""
+    if _lib is not None and _lib.mpk_status() != 0:
+        msg = last_error()
+        _lib.mpk_clear_error()
+        raise NativeError(msg)
+
+
+def device_name() -> str:
+    buf = ctypes.create_string_buffer(256)
+    lib().mpk_device_name(ctypes.addressof(buf), 256)
+    return buf.value.decode(errors="replace")

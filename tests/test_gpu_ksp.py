@@ -1,0 +1,181 @@
+"""Whole solves on the GPU against the oracle in the device reduction order:
+the same iteration count, converged reason, residual history and iterate,
+bit for bit.  Driven through the PETSc surface, which calls mp_ksp_solve_*."""
+
+import numpy as np
+import pytest
+
+from _helpers import same_bits
+from mpk import workloads as W
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def O(oracle):
+    oracle.set_mode("dev")
+    yield oracle
+    oracle.set_mode("ref")
+
+
+def _petsc_solve(csr, b, ksp_type, pc_type, levels=0, x0=None, **opt):
+    from mojopetsc4py import PETSc
+    ip, ix, va = csr
+    n = len(ip) - 1
+    A = PETSc.Mat().createSeqAIJ((n, n), nnz=7)
+    A.setValuesCSR(ip, ix, va)
+    A.assemble()
+    pc = PETSc.PC().create()
+    pc.setType(pc_type)
+    if levels:
+        pc.setFactorLevels(levels)
+    if "omega" in opt:
+        pc.setSOROmega(opt.pop("omega"))
+    if "ordering" in opt:
+        pc.setFactorOrdering(opt.pop("ordering"))
+    pc.setOperators(A)
+    ksp = PETSc.KSP().create()
+    ksp.setPC(pc)
+    ksp.setType(ksp_type)
+    ksp.setOperators(A)
+    tol = {k: opt.pop(k) for k in ("rtol", "atol", "divtol", "max_it") if k in opt}
+    if tol:
+        ksp.setTolerances(**tol)
+    if "restart" in opt:
+        ksp.setGMRESRestart(opt.pop("restart"))
+    if "orthog" in opt:
+        ksp.setGMRESOrthogonalization(opt.pop("orthog"))
+    if "cgstype" in opt:
+        ksp.setGMRESCGSRefinementType(opt.pop("cgstype"))
+    if "side" in opt:
+        ksp.setPCSide(opt.pop("side"))
+    if "normtype" in opt:
+        ksp.setNormType(opt.pop("normtype"))
+    assert not opt, opt
+    ksp.setConvergenceHistory()
+    bv = PETSc.Vec().createWithArray(b)
+    xv = PETSc.Vec().createSeq(n)
+    if x0 is not None:
+        xv.setArray(x0)
+        ksp.setInitialGuessNonzero(True)
+    ksp.solve(bv, xv)
+    its = ksp.getIterationNumber()
+    return (xv.getArray().copy(), its, ksp.getConvergedReason(), ksp.getResidualNorm(),
+            ksp.getConvergenceHistory()[: its + 1].copy())
+
+
+def _oracle_solve(O, csr, b, ksp_type, pc_type, levels=0, x0=None, **opt):
+    A = O.CSR(*csr)
+    pcopt = {}
+    if "omega" in opt:
+        pcopt["omega"] = opt.pop("omega")
+    if "ordering" in opt:
+        pcopt["ordering"] = opt.pop("ordering")
+    pc = O.PCSetup(A, pc_type, levels, **pcopt)
+    kw = dict(opt)
+    if x0 is not None:
+        kw["guess_zero"] = 0
+    r = O.ksp_solve(A, b, ksp_type, pc, x0=x0, **kw)
+    return r.x, r.its, r.reason, r.rnorm, r.history
+
+
+def _compare(O, csr, b, ksp_type, pc_type, levels=0, x0=None, **opt):
+    g = _petsc_solve(csr, b, ksp_type, pc_type, levels, x0, **dict(opt))
+    o = _oracle_solve(O, csr, b, ksp_type, pc_type, levels, x0, **dict(opt))
+    assert g[1] == o[1], ("its", g[1], o[1])
+    assert g[2] == o[2], ("reason", g[2], o[2])
+    assert same_bits(np.array([g[3]]), np.array([o[3]])), ("rnorm", g[3], o[3])
+    assert same_bits(g[4], o[4][: len(g[4])]), "history"
+    assert same_bits(g[0], o[0]), ("x", np.max(np.abs(g[0] - o[0])))
+    return g
+
+
+LAP64 = W.laplacian_2d(64)
+ONES = np.ones(64 * 64)
+BENCH_B = np.random.default_rng(5).normal(size=64 * 64)
+
+
+def test_config1_cg_ilu0_ones(native, O):
+    """64x64 Poisson, b = ones, cg + ilu(0), rtol 1e-10, max_it 1000."""
+    g = _compare(O, LAP64, ONES, "cg", "ilu", rtol=1e-10, max_it=1000)
+    assert g[2] == 2
+
+
+def test_config1_published_count(native, O):
+    """The reference's bench RHS gives 77 iterations (README.md:315)."""
+    g = _compare(O, LAP64, BENCH_B, "cg", "ilu", rtol=1e-10, atol=1e-50, divtol=1e30, max_it=2000)
+    assert g[1] == 77
+
+
+@pytest.mark.parametrize("ksp_type,pc_type,its", [("cg", "none", 225), ("cg", "icc", 77), ("gmres", "none", 626),
+                                                  ("fgmres", "none", 626), ("gmres", "jacobi", 626)])
+def test_published_counts(native, O, ksp_type, pc_type, its):
+    g = _compare(O, LAP64, BENCH_B, ksp_type, pc_type, rtol=1e-10, atol=1e-50, divtol=1e30, max_it=2000)
+    assert g[1] == its
+
+
+def test_bcgs_band(native, O):
+    g = _compare(O, LAP64, BENCH_B, "bcgs", "none", rtol=1e-10, atol=1e-50, divtol=1e30, max_it=2000)
+    assert abs(g[1] - 162) <= 20
+
+
+DIFF16, DIFF16B = W.diffusion_3d(16)
+CD12, CD12B = W.convection_diffusion_3d(12)
+
+
+@pytest.mark.parametrize("ksp_type,pc_type,levels", [
+    ("cg", "jacobi", 0), ("cg", "icc", 0), ("cg", "icc", 1), ("cg", "ilu", 0), ("cg", "sor", 0),
+    ("bcgs", "jacobi", 0), ("bcgs", "sor", 0), ("gmres", "ilu", 0), ("fgmres", "ilu", 0), ("gmres", "jacobi", 0),
+])
+def test_diffusion_16(native, O, ksp_type, pc_type, levels):
+    _compare(O, DIFF16, DIFF16B, ksp_type, pc_type, levels, rtol=1e-10, max_it=3000)
+
+
+@pytest.mark.parametrize("opts", [
+    dict(), dict(cgstype=1), dict(cgstype=2), dict(orthog=1), dict(restart=10), dict(side=1),
+])
+def test_gmres_variants_convdiff(native, O, opts):
+    _compare(O, CD12, CD12B, "gmres", "ilu", rtol=1e-10, max_it=2000, **opts)
+
+
+@pytest.mark.parametrize("ksp_type", ["bcgs", "fgmres"])
+def test_convdiff_other(native, O, ksp_type):
+    _compare(O, CD12, CD12B, ksp_type, "ilu", rtol=1e-10, max_it=2000)
+
+
+def test_sor_omega(native, O):
+    _compare(O, DIFF16, DIFF16B, "bcgs", "sor", omega=1.2, rtol=1e-10, max_it=2000)
+
+
+def test_nonzero_guess(native, O):
+    x0 = np.random.default_rng(11).normal(size=len(DIFF16B))
+    for kt, pt in (("cg", "jacobi"), ("gmres", "ilu"), ("bcgs", "jacobi"), ("fgmres", "jacobi")):
+        _compare(O, DIFF16, DIFF16B, kt, pt, x0=x0, rtol=1e-8, max_it=2000)
+
+
+@pytest.mark.parametrize("normtype", [1, 3, 4])
+def test_cg_norm_types(native, O, normtype):
+    _compare(O, DIFF16, DIFF16B, "cg", "jacobi", normtype=normtype, rtol=1e-10, max_it=300)
+
+
+def test_max_it_reports_converged_its(native, O):
+    g = _compare(O, DIFF16, DIFF16B, "cg", "jacobi", rtol=1e-14, max_it=7)
+    assert g[1] == 7 and g[2] == 4
+
+
+def test_nan_rhs(native, O):
+    b = DIFF16B.copy()
+    b[3] = np.nan
+    g = _compare(O, DIFF16, b, "cg", "jacobi", rtol=1e-10, max_it=50)
+    assert g[2] == -9 and g[1] == 0
+
+
+def test_indefinite_cg(native, O):
+    csr = (np.array([0, 1, 2], np.int32), np.array([0, 1], np.int32), np.array([1.0, -1.0]))
+    g = _compare(O, csr, np.array([1.0, 1.0]), "cg", "none", rtol=1e-10, max_it=50)
+    assert g[2] in (-8, -10)
+
+
+def test_tiny_divtol(native, O):
+    g = _compare(O, DIFF16, DIFF16B, "gmres", "none", rtol=1e-10, divtol=1e-30, max_it=50)
+    assert g[2] == -4 and g[1] == 0

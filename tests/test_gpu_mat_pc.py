@@ -1,0 +1,148 @@
+"""SpMV, the Jacobi / SOR set up and apply, and the ILU / IC sweeps on the
+GPU against the oracle, bit for bit, through the C ABI."""
+
+import numpy as np
+import pytest
+
+from _helpers import DevMat, back, dev, rng_vec, same_bits
+from mpk import workloads as W
+
+pytestmark = pytest.mark.gpu
+
+
+def _random_sparse(n, per_row, seed, diag=True):
+    rng = np.random.default_rng(seed)
+    rows, cols = [], []
+    for i in range(n):
+        cs = set(rng.integers(0, n, per_row).tolist())
+        if diag:
+            cs.add(i)
+        for c in sorted(cs):
+            rows.append(i)
+            cols.append(c)
+    vals = rng.normal(size=len(rows))
+    vals[np.array(rows) == np.array(cols)] += 8.0
+    return W._csr_from_coo(n, np.array(rows), np.array(cols), vals)
+
+
+MATS = {
+    "lap64": lambda: W.laplacian_2d(64),
+    "diff12": lambda: W.diffusion_3d(12)[0],
+    "cd10": lambda: W.convection_diffusion_3d(10)[0],
+    "rand500": lambda: _random_sparse(500, 9, 3),
+    "tri33": lambda: W.tridiagonal(33),
+}
+
+
+@pytest.fixture(scope="module")
+def O(oracle):
+    oracle.set_mode("dev")
+    yield oracle
+    oracle.set_mode("ref")
+
+
+@pytest.mark.parametrize("name", sorted(MATS))
+def test_spmv_family_bitwise(native, O, name):
+    ip, ix, va = MATS[name]()
+    A = O.CSR(ip, ix, va)
+    D = DevMat(ip, ix, va)
+    n = A.n
+    x = rng_vec(n, 1)
+    y0 = rng_vec(n, 2)
+    dx, dy = dev(x), dev(y0)
+    native.mp_mat_mult(dx.ptr, dy.ptr, D.rp.ptr, D.ci.ptr, D.v.ptr, n)
+    assert same_bits(back(dy, np.float64, n), A.mult(x))
+    dy = dev(y0)
+    native.mp_mat_mult_add(dx.ptr, dy.ptr, D.rp.ptr, D.ci.ptr, D.v.ptr, n)
+    h = y0.copy()
+    O.lib().or_mat_mult_add(x.ctypes.data, h.ctypes.data, A.irow.ctypes.data, A.icol.ctypes.data,
+                            A.data.ctypes.data, n)
+    assert same_bits(back(dy, np.float64, n), h)
+    native.mp_mat_mult_transpose(dx.ptr, dy.ptr, D.rp.ptr, D.ci.ptr, D.v.ptr, n, n)
+    O.lib().or_mat_mult_transpose(x.ctypes.data, h.ctypes.data, A.irow.ctypes.data, A.icol.ctypes.data,
+                                  A.data.ctypes.data, n, n)
+    assert same_bits(back(dy, np.float64, n), h)
+    native.mp_mat_get_diagonal(dy.ptr, D.rp.ptr, D.ci.ptr, D.v.ptr, n)
+    O.lib().or_mat_get_diagonal(h.ctypes.data, A.irow.ctypes.data, A.icol.ctypes.data, A.data.ctypes.data, n)
+    assert same_bits(back(dy, np.float64, n), h)
+    dm = dev(np.zeros(n, np.int32))
+    dense = native.mp_mat_get_diagonal_markers(D.rp.ptr, D.ci.ptr, dm.ptr, n)
+    hm = np.zeros(n, np.int64)
+    odense = O.lib().or_mat_get_diagonal_markers(A.irow.ctypes.data, A.icol.ctypes.data, hm.ctypes.data, n)
+    assert dense == odense and np.array_equal(back(dm, np.int32, n), hm)
+
+
+@pytest.mark.parametrize("name", ["lap64", "rand500", "diff12"])
+@pytest.mark.parametrize("jtype", [0, 1, 2, 3])
+def test_jacobi_setup_bitwise(native, O, name, jtype):
+    ip, ix, va = MATS[name]()
+    A = O.CSR(ip, ix, va)
+    D = DevMat(ip, ix, va)
+    n = A.n
+    for useabs, fixdiag, scale in ((0, 1, 1.0), (1, 1, 0.5), (0, 0, 2.0)):
+        dd, dx = dev(np.zeros(n)), dev(np.zeros(n))
+        native.mp_pc_jacobi_setup(dd.ptr, jtype, useabs, fixdiag, scale, D.rp.ptr, D.ci.ptr, D.v.ptr, n, dx.ptr)
+        hd, hx = np.zeros(n), np.zeros(n)
+        O.lib().or_pc_jacobi_setup(hd.ctypes.data, jtype, useabs, fixdiag, scale, A.irow.ctypes.data,
+                                   A.icol.ctypes.data, A.data.ctypes.data, n, hx.ctypes.data)
+        assert same_bits(back(dd, np.float64, n), hd)
+    native.mp_pc_jacobi_setup_symmetric(dd.ptr, jtype, D.rp.ptr, D.ci.ptr, D.v.ptr, n)
+    O.lib().or_pc_jacobi_setup_symmetric(hd.ctypes.data, jtype, A.irow.ctypes.data, A.icol.ctypes.data,
+                                         A.data.ctypes.data, n)
+    assert same_bits(back(dd, np.float64, n), hd)
+
+
+@pytest.mark.parametrize("name", ["lap64", "diff12", "cd10", "rand500"])
+@pytest.mark.parametrize("omega,fshift,flag", [(1.0, 0.0, 12 | 16), (1.2, 0.0, 12 | 16), (0.8, 0.1, 1 | 16),
+                                              (1.0, 0.0, 2 | 16)])
+def test_sor_apply_bitwise(native, O, name, omega, fshift, flag):
+    ip, ix, va = MATS[name]()
+    A = O.CSR(ip, ix, va)
+    D = DevMat(ip, ix, va)
+    n = A.n
+    b = rng_vec(n, 4)
+    hm = np.zeros(n, np.int64)
+    O.lib().or_mat_get_diagonal_markers(A.irow.ctypes.data, A.icol.ctypes.data, hm.ctypes.data, n)
+    dm = dev(hm.astype(np.int32))
+    dy, di, dd, dt = dev(np.zeros(n)), dev(np.zeros(n)), dev(np.zeros(n)), dev(np.zeros(3 * n))
+    native.mp_pc_sor_apply(dev(b).ptr, dy.ptr, D.rp.ptr, D.ci.ptr, D.v.ptr, dm.ptr, di.ptr, dd.ptr, dt.ptr, omega,
+                           fshift, flag, n)
+    hy, hi, hd, ht = np.zeros(n), np.zeros(n), np.zeros(n), np.zeros(3 * n)
+    O.lib().or_pc_sor_apply(b.ctypes.data, hy.ctypes.data, A.irow.ctypes.data, A.icol.ctypes.data,
+                            A.data.ctypes.data, hm.ctypes.data, hi.ctypes.data, hd.ctypes.data, ht.ctypes.data,
+                            omega, fshift, flag, n)
+    assert same_bits(back(dy, np.float64, n), hy)
+    assert same_bits(back(di, np.float64, n), hi)
+
+
+def _upload_factor(f, kind, n):
+    nnz = int(f["bi"][n]) if kind == "icc" else int(f["bdiag"][0]) + 1
+    out = {k: dev(f[k].astype(np.int32)) for k in ("bi", "bdiag", "r")}
+    out["bj"] = dev(f["bj"][:max(nnz, 1)].astype(np.int32))
+    out["ba"] = dev(f["ba"])
+    return out
+
+
+@pytest.mark.parametrize("name", sorted(MATS))
+@pytest.mark.parametrize("kind,levels,ordering", [("ilu", 0, "natural"), ("ilu", 1, "natural"), ("ilu", 2, "rcm"),
+                                                  ("icc", 0, "natural"), ("icc", 1, "natural"),
+                                                  ("icc", 0, "rcm")])
+def test_factor_apply_bitwise(native, O, name, kind, levels, ordering):
+    ip, ix, va = MATS[name]()
+    if kind == "icc" and name in ("cd10", "rand500"):
+        pytest.skip("IC needs a symmetric matrix")
+    A = O.CSR(ip, ix, va)
+    n = A.n
+    pc = O.PCSetup(A, kind, levels, ordering)
+    f = pc.factor
+    b = rng_vec(n, 9)
+    want = pc.apply(b)
+    d = _upload_factor(f, kind, n)
+    dx = dev(np.zeros(n))
+    if kind == "ilu":
+        native.mp_pc_ilu_apply(dev(b).ptr, dx.ptr, 0, d["bi"].ptr, d["bj"].ptr, d["ba"].ptr, d["bdiag"].ptr,
+                               d["r"].ptr, d["r"].ptr, n)
+    else:
+        native.mp_pc_icc_apply(dev(b).ptr, dx.ptr, d["bi"].ptr, d["bj"].ptr, d["ba"].ptr, d["bdiag"].ptr,
+                               d["r"].ptr, 0, n)
+    assert same_bits(back(dx, np.float64, n), want)
