@@ -45,7 +45,6 @@ void mpk_memcpy_dtoh(int64_t dst, int64_t src, int64_t bytes);  /* synchronous *
 void mpk_memcpy_dtod(int64_t dst, int64_t src, int64_t bytes);  /* queued */
 void mpk_memset(int64_t dst, int64_t value, int64_t bytes);     /* queued */
 void mpk_synchronize(void);
-void mpk_forget_matrix(int64_t irow);  /* drop cached transpose maps of a matrix */
 /* average ms of `reps` SpMV launches, HIP events on the library stream */
 double mpk_time_spmv(int64_t x, int64_t y, int64_t irow, int64_t icol, int64_t data, int64_t nrow,
                      int64_t reps);
@@ -59,6 +58,13 @@ void mp_mat_mult_transpose(int64_t x, int64_t y, int64_t irow, int64_t icol, int
 void mp_mat_get_diagonal(int64_t diag, int64_t irow, int64_t icol, int64_t data, int64_t nrow);
 void mp_mat_scale(int64_t data, int64_t nnz, double alpha);
 void mp_mat_zero_entries(int64_t data, int64_t nnz);
+/* No counterpart in the reference: the transpose map of a CSR pattern, kept
+ * by the caller so repeated transposed products skip rebuilding it.  [D]
+ * int32 trp[ncols + 1], trow[nnz], tpos[nnz]; returns 0 on success. */
+int64_t mpk_csr_transpose(int64_t irow, int64_t icol, int64_t nrow, int64_t ncols, int64_t trp, int64_t trow,
+                          int64_t tpos);
+void mpk_mat_mult_transpose_t(int64_t x, int64_t y, int64_t data, int64_t ncols, int64_t trp, int64_t trow,
+                              int64_t tpos);
 
 /* ------------------------------------------------------------------ Vec
  * capi.mojo:69-238 -> ported.mojo:75-335.  Vectors [D]; val, z, idx, sval,
@@ -140,13 +146,16 @@ void mp_ordering_rcm(int64_t perm, int64_t level, int64_t visited, int64_t queue
 int64_t mpk_icc_transpose_size(int64_t ui, int64_t udiag, int64_t nrow);
 int64_t mpk_icc_transpose(int64_t ui, int64_t uj, int64_t udiag, int64_t nrow, int64_t trp, int64_t trow,
                           int64_t tpos);
+/* mp_pc_icc_apply with that transpose supplied (no per-call rebuild) */
+int64_t mpk_pc_icc_apply_t(int64_t b, int64_t x, int64_t ui, int64_t uj, int64_t ua, int64_t udiag, int64_t c,
+                           int64_t trp, int64_t trow, int64_t tpos, int64_t nrow);
 
 /* ------------------------------------------------------------------ KSP
  * capi.mojo:383-424 -> ksp.mojo.  irow/icol/data, b, x, work, vv, prevec [D];
  * cfg, cfgi, pci, pcf, hs [H].  pci holds [D] addresses: Jacobi {2, diag,
  * diagsqrt}; SOR {4, irow, icol, data, dmark, idiag, mdiag, work, ..., flag at
  * 10}; ILU {5, bi, bj, ba, bdiag, r, c}; IC {6, bi, bj, ba, bdiag, r, trp,
- * trow, tpos} (the last three may be 0: the library then builds them). */
+ * trow, tpos} with the transpose made by mpk_icc_transpose. */
 void mp_ksp_solve_cg(int64_t n, int64_t irow, int64_t icol, int64_t data, int64_t pci, int64_t pcf, int64_t pstmp,
                      int64_t work, int64_t cfg, int64_t cfgi, int64_t b, int64_t x);
 void mp_ksp_solve_bcgs(int64_t n, int64_t irow, int64_t icol, int64_t data, int64_t pci, int64_t pcf,

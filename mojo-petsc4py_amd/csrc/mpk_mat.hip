@@ -53,25 +53,21 @@ __global__ void __launch_bounds__(256) k_spmv_t(i64 ncol, const int *trp, const 
   y[c] = s;
 }
 
-struct TransposeMap {
-  int *trp = nullptr, *trow = nullptr, *tp = nullptr;
-};
-static std::map<std::tuple<const void *, const void *, i64, i64, i64>, TransposeMap> g_tmaps;
-
-static TransposeMap *transpose_map(const int *rp, const int *ci, i64 nrow, i64 ncol) {
+// Builds the transpose map of a CSR pattern into caller-owned device arrays:
+// trp[ncol + 1], trow[nnz] (source row), tpos[nnz] (position in A's arrays),
+// each column's entries in increasing row.
+bool csr_transpose_map(const int *rp, const int *ci, i64 nrow, i64 ncol, int *trp_d, int *trow_d, int *tpos_d) {
   Ctx *c = ctx();
-  if (!c) return nullptr;
+  if (!c) return false;
   std::vector<int> hrp(nrow + 1);
   HIPCHK(hipMemcpyAsync(hrp.data(), rp, (nrow + 1) * sizeof(int), hipMemcpyDeviceToHost, c->stream));
   HIPCHK(hipStreamSynchronize(c->stream));
   i64 nnz = hrp[nrow];
-  auto key = std::make_tuple((const void *)rp, (const void *)ci, nrow, ncol, nnz);
-  auto it = g_tmaps.find(key);
-  if (it != g_tmaps.end()) return &it->second;
   std::vector<int> hci(nnz > 0 ? nnz : 1);
-  if (nnz > 0)
+  if (nnz > 0) {
     HIPCHK(hipMemcpyAsync(hci.data(), ci, nnz * sizeof(int), hipMemcpyDeviceToHost, c->stream));
-  HIPCHK(hipStreamSynchronize(c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+  }
   std::vector<int> trp(ncol + 1, 0), trow(nnz > 0 ? nnz : 1), tp(nnz > 0 ? nnz : 1);
   for (i64 k = 0; k < nnz; k++) trp[hci[k] + 1]++;
   for (i64 col = 0; col < ncol; col++) trp[col + 1] += trp[col];
@@ -82,14 +78,19 @@ static TransposeMap *transpose_map(const int *rp, const int *ci, i64 nrow, i64 n
       trow[pos] = (int)i;
       tp[pos] = k;
     }
-  TransposeMap m;
-  HIPCHK(hipMalloc(&m.trp, (ncol + 1) * sizeof(int)));
-  HIPCHK(hipMalloc(&m.trow, trow.size() * sizeof(int)));
-  HIPCHK(hipMalloc(&m.tp, tp.size() * sizeof(int)));
-  HIPCHK(hipMemcpy(m.trp, trp.data(), (ncol + 1) * sizeof(int), hipMemcpyHostToDevice));
-  HIPCHK(hipMemcpy(m.trow, trow.data(), trow.size() * sizeof(int), hipMemcpyHostToDevice));
-  HIPCHK(hipMemcpy(m.tp, tp.data(), tp.size() * sizeof(int), hipMemcpyHostToDevice));
-  return &(g_tmaps[key] = m);
+  HIPCHK(hipMemcpyAsync(trp_d, trp.data(), (ncol + 1) * sizeof(int), hipMemcpyHostToDevice, c->stream));
+  if (nnz > 0) {
+    HIPCHK(hipMemcpyAsync(trow_d, trow.data(), nnz * sizeof(int), hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipMemcpyAsync(tpos_d, tp.data(), nnz * sizeof(int), hipMemcpyHostToDevice, c->stream));
+  }
+  HIPCHK(hipStreamSynchronize(c->stream));
+  return !failed();
+}
+
+static i64 csr_nnz(const int *rp, i64 nrow) {
+  int h = 0;
+  HIPCHK(hipMemcpy(&h, rp + nrow, sizeof(int), hipMemcpyDeviceToHost));
+  return h;
 }
 
 // ------------------------------------------------------------ row scans
@@ -144,15 +145,44 @@ void mp_mat_mult_add(int64_t x, int64_t y, int64_t irow, int64_t icol, int64_t d
 
 void mp_mat_mult_transpose(int64_t x, int64_t y, int64_t irow, int64_t icol, int64_t data, int64_t nrow,
                            int64_t ncols) {
+  // the reference signature carries no transpose, so one is built for the
+  // call; mpk_csr_transpose + mpk_mat_mult_transpose_t keep it instead
   if (ncols <= 0 || !ctx()) return;
   if (nrow <= 0) {
     vset((double *)y, 0.0, ncols, nullptr);
     return;
   }
-  TransposeMap *m = transpose_map((const int *)irow, (const int *)icol, nrow, ncols);
-  if (!m) return;
-  MPK_LAUNCH(k_spmv_t, nblocks(ncols, 256), 256, (i64)ncols, (const int *)m->trp, (const int *)m->trow,
-             (const int *)m->tp, (const double *)data, (const double *)x, (double *)y);
+  i64 nnz = csr_nnz((const int *)irow, nrow);
+  int *trp = nullptr, *trow = nullptr, *tpos = nullptr;
+  HIPCHK(hipMalloc(&trp, (ncols + 1) * sizeof(int)));
+  HIPCHK(hipMalloc(&trow, (nnz > 0 ? nnz : 1) * sizeof(int)));
+  HIPCHK(hipMalloc(&tpos, (nnz > 0 ? nnz : 1) * sizeof(int)));
+  if (csr_transpose_map((const int *)irow, (const int *)icol, nrow, ncols, trp, trow, tpos))
+    MPK_LAUNCH(k_spmv_t, nblocks(ncols, 256), 256, (i64)ncols, (const int *)trp, (const int *)trow,
+               (const int *)tpos, (const double *)data, (const double *)x, (double *)y);
+  HIPCHK(hipStreamSynchronize(stream()));
+  HIPCHK(hipFree(trp));
+  HIPCHK(hipFree(trow));
+  HIPCHK(hipFree(tpos));
+}
+
+// The transpose map of a CSR pattern, into caller-owned [D] int32 arrays
+// trp[ncols + 1], trow[nnz], tpos[nnz].  Returns 0 on success.
+int64_t mpk_csr_transpose(int64_t irow, int64_t icol, int64_t nrow, int64_t ncols, int64_t trp, int64_t trow,
+                          int64_t tpos) {
+  if (nrow < 0 || ncols <= 0) return 0;
+  return csr_transpose_map((const int *)irow, (const int *)icol, nrow, ncols, (int *)trp, (int *)trow,
+                           (int *)tpos)
+             ? 0
+             : -1;
+}
+
+// y = A^T x over a map made by mpk_csr_transpose.
+void mpk_mat_mult_transpose_t(int64_t x, int64_t y, int64_t data, int64_t ncols, int64_t trp, int64_t trow,
+                              int64_t tpos) {
+  if (ncols <= 0 || !ctx()) return;
+  MPK_LAUNCH(k_spmv_t, nblocks(ncols, 256), 256, (i64)ncols, (const int *)trp, (const int *)trow,
+             (const int *)tpos, (const double *)data, (const double *)x, (double *)y);
 }
 
 void mp_mat_get_diagonal(int64_t diag, int64_t irow, int64_t icol, int64_t data, int64_t nrow) {
@@ -184,20 +214,6 @@ int64_t mp_mat_get_diagonal_markers(int64_t irow, int64_t icol, int64_t diag, in
   HIPCHK(hipMemcpyAsync(&h, dense, sizeof(int), hipMemcpyDeviceToHost, c->stream));
   HIPCHK(hipStreamSynchronize(c->stream));
   return h;
-}
-
-// Drop cached transpose maps (call after changing a matrix's pattern in place).
-void mpk_forget_matrix(int64_t irow) {
-  for (auto it = g_tmaps.begin(); it != g_tmaps.end();) {
-    if (std::get<0>(it->first) == (const void *)irow) {
-      hipFree(it->second.trp);
-      hipFree(it->second.trow);
-      hipFree(it->second.tp);
-      it = g_tmaps.erase(it);
-    } else {
-      ++it;
-    }
-  }
 }
 
 }  // extern "C"

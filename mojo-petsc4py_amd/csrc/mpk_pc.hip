@@ -212,11 +212,6 @@ void PCView::apply_symmetric(const double *x, double *y, const int64_t *stop) co
 
 // ------------------------------------------------------------ IC transpose map
 
-struct IccMap {
-  int *trp = nullptr, *trow = nullptr, *tpos = nullptr;
-};
-static std::map<std::tuple<const void *, const void *, const void *, i64>, IccMap> g_iccmaps;
-
 // Transpose of the strict upper part of an IC factor: for column k, the rows
 // i < k holding it, ascending, and the positions of those entries in ua.
 static bool build_icc_map(const int *ui, const int *uj, const int *udiag, i64 n, int *trp_d, int *trow_d,
@@ -258,20 +253,6 @@ i64 icc_map_count(const int *ui, const int *udiag, i64 n) {
   return m;
 }
 
-static IccMap *icc_map(const int *ui, const int *uj, const int *udiag, i64 n) {
-  auto key = std::make_tuple((const void *)ui, (const void *)uj, (const void *)udiag, n);
-  auto it = g_iccmaps.find(key);
-  if (it != g_iccmaps.end()) return &it->second;
-  if (!ctx()) return nullptr;
-  i64 m = icc_map_count(ui, udiag, n);
-  IccMap mp;
-  HIPCHK(hipMalloc(&mp.trp, (n + 1) * sizeof(int)));
-  HIPCHK(hipMalloc(&mp.trow, (m > 0 ? m : 1) * sizeof(int)));
-  HIPCHK(hipMalloc(&mp.tpos, (m > 0 ? m : 1) * sizeof(int)));
-  if (!build_icc_map(ui, uj, udiag, n, mp.trp, mp.trow, mp.tpos)) return nullptr;
-  return &(g_iccmaps[key] = mp);
-}
-
 bool PCView::parse(const int64_t *pci, const double *pcf, i64 n_) {
   n = n_;
   type = (int)pci[0];
@@ -295,16 +276,13 @@ bool PCView::parse(const int64_t *pci, const double *pcf, i64 n_) {
       icc.ua = (const double *)pci[3];
       icc.udiag = (const int *)pci[4];
       icc.c = (const int *)pci[5];
-      if (pci[6] && pci[7] && pci[8]) {
-        icc.trp = (const int *)pci[6];
-        icc.trow = (const int *)pci[7];
-        icc.tpos = (const int *)pci[8];
-      } else {
-        IccMap *m = icc_map(icc.ui, icc.uj, icc.udiag, n);
-        if (!m) return false;
-        icc.trp = m->trp;
-        icc.trow = m->trow;
-        icc.tpos = m->tpos;
+      // the transpose of the strict part travels in aux5..aux7 (mpk_icc_transpose)
+      icc.trp = (const int *)pci[6];
+      icc.trow = (const int *)pci[7];
+      icc.tpos = (const int *)pci[8];
+      if (!icc.trp || !icc.trow || !icc.tpos) {
+        set_error(4, "IC descriptor without the transpose of the factor (aux5..aux7)");
+        return false;
       }
       return true;
     }
@@ -412,12 +390,34 @@ void mp_pc_ilu_apply(int64_t b, int64_t x, int64_t tmp, int64_t bi, int64_t bj, 
 
 int64_t mp_pc_icc_apply(int64_t b, int64_t x, int64_t ui, int64_t uj, int64_t ua, int64_t udiag, int64_t c,
                         int64_t scratch, int64_t nrow) {
+  // the reference signature carries no transpose, so one is built for the
+  // call; mpk_icc_transpose + mpk_pc_icc_apply_t keep it instead
   (void)scratch;
-  if (nrow <= 0) return 0;
-  int64_t pci[12] = {PCT_ICC, ui, uj, ua, udiag, c, 0, 0, 0, 0, 0, 0};
-  PCView v;
-  if (!v.parse(pci, nullptr, nrow)) return 0;
-  icc_apply(v.icc, (const double *)b, (double *)x, nullptr);
+  if (nrow <= 0 || !ctx()) return 0;
+  i64 m = icc_map_count((const int *)ui, (const int *)udiag, nrow);
+  int *trp = nullptr, *trow = nullptr, *tpos = nullptr;
+  HIPCHK(hipMalloc(&trp, (nrow + 1) * sizeof(int)));
+  HIPCHK(hipMalloc(&trow, (m > 0 ? m : 1) * sizeof(int)));
+  HIPCHK(hipMalloc(&tpos, (m > 0 ? m : 1) * sizeof(int)));
+  if (build_icc_map((const int *)ui, (const int *)uj, (const int *)udiag, nrow, trp, trow, tpos)) {
+    IccFactor f{nrow, (const int *)ui, (const int *)uj, (const double *)ua, (const int *)udiag, (const int *)c,
+                trp, trow, tpos};
+    icc_apply(f, (const double *)b, (double *)x, nullptr);
+  }
+  HIPCHK(hipStreamSynchronize(stream()));
+  HIPCHK(hipFree(trp));
+  HIPCHK(hipFree(trow));
+  HIPCHK(hipFree(tpos));
+  return 0;
+}
+
+// mp_pc_icc_apply with the transpose of the strict part supplied.
+int64_t mpk_pc_icc_apply_t(int64_t b, int64_t x, int64_t ui, int64_t uj, int64_t ua, int64_t udiag, int64_t c,
+                           int64_t trp, int64_t trow, int64_t tpos, int64_t nrow) {
+  if (nrow <= 0 || !ctx()) return 0;
+  IccFactor f{nrow, (const int *)ui, (const int *)uj, (const double *)ua, (const int *)udiag, (const int *)c,
+              (const int *)trp, (const int *)trow, (const int *)tpos};
+  icc_apply(f, (const double *)b, (double *)x, nullptr);
   return 0;
 }
 

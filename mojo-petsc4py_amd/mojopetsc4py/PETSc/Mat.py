@@ -80,18 +80,33 @@ class MatSORType:
 
 
 class _DevCSR:
-    """The device copy: int32 row pointers and columns, float64 values."""
+    """The device copy: int32 row pointers and columns, float64 values, and
+    (built on first use) the transpose map for A^T products."""
 
-    __slots__ = ("rp", "ci", "v", "nnz")
+    __slots__ = ("rp", "ci", "v", "nnz", "nrows", "ncols", "tmap")
 
-    def __init__(self, irow, icol, data, nrows):
+    def __init__(self, irow, icol, data, nrows, ncols):
         nnz = int(irow[nrows]) if nrows else 0
         if nnz >= 2 ** 31:
             raise ValueError("the device matrix holds int32 indices: nnz must stay below 2**31")
         self.nnz = nnz
+        self.nrows = nrows
+        self.ncols = ncols
         self.rp = DeviceBuffer.from_array(np.asarray(irow[: nrows + 1], dtype=np.int32))
         self.ci = DeviceBuffer.from_array(np.asarray(icol[: max(nnz, 1)], dtype=np.int32))
         self.v = DeviceBuffer.from_array(np.asarray(data[: max(nnz, 1)], dtype=np.float64))
+        self.tmap = None
+
+    def transpose_map(self):
+        if self.tmap is None:
+            trp = DeviceBuffer(4 * (self.ncols + 1))
+            trow = DeviceBuffer(4 * max(self.nnz, 1))
+            tpos = DeviceBuffer(4 * max(self.nnz, 1))
+            lib().mpk_csr_transpose(self.rp.ptr, self.ci.ptr, self.nrows, self.ncols, trp.ptr, trow.ptr,
+                                    tpos.ptr)
+            check()
+            self.tmap = (trp, trow, tpos)
+        return self.tmap
 
 
 class Mat:
@@ -119,8 +134,6 @@ class Mat:
     def _touch(self):
         """The host arrays changed: the device copy and any set up built on
         it are stale."""
-        if self._devc is not None:
-            lib().mpk_forget_matrix(self._devc.rp.ptr)
         self._devc = None
         self._version += 1
 
@@ -131,7 +144,7 @@ class Mat:
                 cols = self._icol[: int(self._irow[self._nrows])]
                 if len(cols) and (cols.min() < 0 or cols.max() >= self._ncols):
                     raise ValueError("a column index lies outside the matrix")
-            self._devc = _DevCSR(self._irow, self._icol, self._data, self._nrows)
+            self._devc = _DevCSR(self._irow, self._icol, self._data, self._nrows, self._ncols)
         return self._devc
 
     # ---------------------------------------------------------- creation
@@ -493,7 +506,12 @@ class Mat:
             raise ValueError(f"incompatible sizes: matrix {self._nrows} rows, vector {x.getSize()}")
         d = self._dev()
         xa = x._d()
-        lib().mp_mat_mult_transpose(xa, out._dw(), d.rp.ptr, d.ci.ptr, d.v.ptr, self._nrows, self._ncols)
+        if self._ncols:
+            if self._nrows:
+                trp, trow, tpos = d.transpose_map()
+                lib().mpk_mat_mult_transpose_t(xa, out._dw(), d.v.ptr, self._ncols, trp.ptr, trow.ptr, tpos.ptr)
+            else:
+                out.zeroEntries()
         check()
         return out
 

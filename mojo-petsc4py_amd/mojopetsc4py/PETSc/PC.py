@@ -469,8 +469,8 @@ class PC:
             L.mp_pc_ilu_apply(xa, y._dw(), 0, b["bi"].ptr, b["bj"].ptr, b["ba"].ptr, b["bdiag"].ptr,
                               b["r"].ptr, b["c"].ptr, n)
         else:
-            L.mp_pc_icc_apply(xa, y._dw(), b["bi"].ptr, b["bj"].ptr, b["ba"].ptr, b["bdiag"].ptr,
-                              b["r"].ptr, 0, n)
+            L.mpk_pc_icc_apply_t(xa, y._dw(), b["bi"].ptr, b["bj"].ptr, b["ba"].ptr, b["bdiag"].ptr,
+                                 b["r"].ptr, b["trp"].ptr, b["trow"].ptr, b["tpos"].ptr, n)
         check()
 
     def apply(self, x: Vec, y: Vec) -> None:

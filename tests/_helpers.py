@@ -7,6 +7,8 @@ import numpy as np
 
 
 def dev(arr):
+    """Upload to a new device buffer.  Keep the returned object alive for as
+    long as its address is in use: the buffer is freed with it."""
     from mpk.device import DeviceBuffer
     return DeviceBuffer.from_array(np.ascontiguousarray(arr))
 

@@ -105,7 +105,8 @@ def test_sor_apply_bitwise(native, O, name, omega, fshift, flag):
     O.lib().or_mat_get_diagonal_markers(A.irow.ctypes.data, A.icol.ctypes.data, hm.ctypes.data, n)
     dm = dev(hm.astype(np.int32))
     dy, di, dd, dt = dev(np.zeros(n)), dev(np.zeros(n)), dev(np.zeros(n)), dev(np.zeros(3 * n))
-    native.mp_pc_sor_apply(dev(b).ptr, dy.ptr, D.rp.ptr, D.ci.ptr, D.v.ptr, dm.ptr, di.ptr, dd.ptr, dt.ptr, omega,
+    db = dev(b)  # keep the buffer alive across the call
+    native.mp_pc_sor_apply(db.ptr, dy.ptr, D.rp.ptr, D.ci.ptr, D.v.ptr, dm.ptr, di.ptr, dd.ptr, dt.ptr, omega,
                            fshift, flag, n)
     hy, hi, hd, ht = np.zeros(n), np.zeros(n), np.zeros(n), np.zeros(3 * n)
     O.lib().or_pc_sor_apply(b.ctypes.data, hy.ctypes.data, A.irow.ctypes.data, A.icol.ctypes.data,
@@ -139,10 +140,11 @@ def test_factor_apply_bitwise(native, O, name, kind, levels, ordering):
     want = pc.apply(b)
     d = _upload_factor(f, kind, n)
     dx = dev(np.zeros(n))
+    db = dev(b)  # keep the buffer alive across the call
     if kind == "ilu":
-        native.mp_pc_ilu_apply(dev(b).ptr, dx.ptr, 0, d["bi"].ptr, d["bj"].ptr, d["ba"].ptr, d["bdiag"].ptr,
+        native.mp_pc_ilu_apply(db.ptr, dx.ptr, 0, d["bi"].ptr, d["bj"].ptr, d["ba"].ptr, d["bdiag"].ptr,
                                d["r"].ptr, d["r"].ptr, n)
     else:
-        native.mp_pc_icc_apply(dev(b).ptr, dx.ptr, d["bi"].ptr, d["bj"].ptr, d["ba"].ptr, d["bdiag"].ptr,
+        native.mp_pc_icc_apply(db.ptr, dx.ptr, d["bi"].ptr, d["bj"].ptr, d["ba"].ptr, d["bdiag"].ptr,
                                d["r"].ptr, 0, n)
     assert same_bits(back(dx, np.float64, n), want)

@@ -64,15 +64,18 @@ def test_minmax_ties_and_nan(native, O):
     for name in ("mp_vec_max", "mp_vec_min"):
         gi, gv = np.zeros(1, np.int64), np.zeros(1)
         oi, ov = np.zeros(1, np.int64), np.zeros(1)
-        getattr(native, name)(dev(x).ptr, gi.ctypes.data, gv.ctypes.data, len(x))
+        dx = dev(x)
+        getattr(native, name)(dx.ptr, gi.ctypes.data, gv.ctypes.data, len(x))
         _oracle(O, name.replace("mp_", "or_"), x.ctypes.data, oi.ctypes.data, ov.ctypes.data, len(x))
         assert gi[0] == oi[0] and same_bits(gv, ov)
     x = np.array([np.nan, 5.0, -1.0])
     gi, gv = np.zeros(1, np.int64), np.zeros(1)
-    native.mp_vec_max(dev(x).ptr, gi.ctypes.data, gv.ctypes.data, 3)
+    dx = dev(x)
+    native.mp_vec_max(dx.ptr, gi.ctypes.data, gv.ctypes.data, 3)
     assert gi[0] == 0 and np.isnan(gv[0])
     z = np.zeros(2)
-    native.mp_vec_norm(dev(np.array([1.0, np.nan, 4.0])).ptr, 3, z.ctypes.data, 3)
+    dn = dev(np.array([1.0, np.nan, 4.0]))
+    native.mp_vec_norm(dn.ptr, 3, z.ctypes.data, 3)
     assert np.isnan(z[0])
 
 
