@@ -142,20 +142,22 @@ void mp_ordering_rcm(int64_t perm, int64_t level, int64_t visited, int64_t queue
                      int64_t nrow);
 /* No counterpart in the reference: the transpose of an IC factor's strict
  * part, which the device forward sweep gathers over.  [D] int32 arrays:
- * trp[nrow + 1], trow[m], tpos[m], m = mpk_icc_transpose_size(). */
+ * trp[nrow + 1], trow[m], tpos[m], tmid[nrow], m = mpk_icc_transpose_size().
+ * Returns the number of "late" entries (strict columns below the diagonal,
+ * which the reference's level-of-fill pattern can hold), or -1. */
 int64_t mpk_icc_transpose_size(int64_t ui, int64_t udiag, int64_t nrow);
 int64_t mpk_icc_transpose(int64_t ui, int64_t uj, int64_t udiag, int64_t nrow, int64_t trp, int64_t trow,
-                          int64_t tpos);
+                          int64_t tpos, int64_t tmid);
 /* mp_pc_icc_apply with that transpose supplied (no per-call rebuild) */
 int64_t mpk_pc_icc_apply_t(int64_t b, int64_t x, int64_t ui, int64_t uj, int64_t ua, int64_t udiag, int64_t c,
-                           int64_t trp, int64_t trow, int64_t tpos, int64_t nrow);
+                           int64_t trp, int64_t trow, int64_t tpos, int64_t tmid, int64_t nlate, int64_t nrow);
 
 /* ------------------------------------------------------------------ KSP
  * capi.mojo:383-424 -> ksp.mojo.  irow/icol/data, b, x, work, vv, prevec [D];
  * cfg, cfgi, pci, pcf, hs [H].  pci holds [D] addresses: Jacobi {2, diag,
  * diagsqrt}; SOR {4, irow, icol, data, dmark, idiag, mdiag, work, ..., flag at
  * 10}; ILU {5, bi, bj, ba, bdiag, r, c}; IC {6, bi, bj, ba, bdiag, r, trp,
- * trow, tpos} with the transpose made by mpk_icc_transpose. */
+ * trow, tpos, tmid, nlate} with the transpose made by mpk_icc_transpose. */
 void mp_ksp_solve_cg(int64_t n, int64_t irow, int64_t icol, int64_t data, int64_t pci, int64_t pcf, int64_t pstmp,
                      int64_t work, int64_t cfg, int64_t cfgi, int64_t b, int64_t x);
 void mp_ksp_solve_bcgs(int64_t n, int64_t irow, int64_t icol, int64_t data, int64_t pci, int64_t pcf,

@@ -19,14 +19,19 @@ struct IluFactor {
 };
 
 // IC factor in the reference layout (factor.mojo:125-273): row i holds its
-// strict upper columns then the diagonal slot (1/d_i); plus the transpose of
-// the strict part for the forward gather.
+// strict columns then the diagonal slot (1/d_i); plus the transpose of the
+// strict part for the forward gather.  With level of fill >= 1 the
+// reference's pattern can hold columns below the diagonal in a row; their
+// forward contributions land after the target row is final ("late" entries,
+// trow > k, the tail of each transposed column from tmid[k]); nlate counts
+// them.
 struct IccFactor {
   i64 n;
   const int *ui, *uj;
   const double *ua;
   const int *udiag, *c;
-  const int *trp, *trow, *tpos;
+  const int *trp, *trow, *tpos, *tmid;
+  i64 nlate;
 };
 
 struct SorData {

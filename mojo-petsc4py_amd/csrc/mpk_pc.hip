@@ -66,14 +66,15 @@ struct IluBwd {  // sum = tmp_i - sum_{U row i} ba tmp; x[c_i] = sum * ba[bdiag_
   __device__ double stale(i64) const { return 0.0; }
 };
 // IC forward: the reference scatters s_k += ua_ik y_i for i ascending; here
-// row k gathers over the transpose in the same ascending i.
+// row k gathers over the transpose in the same ascending i, up to the first
+// row after k (the rest arrive after row k is final: k_icc_late).
 struct IccFwd {
-  const int *trp, *trow, *tpos, *udiag, *c;
+  const int *trp, *trow, *tpos, *tmid, *udiag, *c;
   const double *ua, *b;
   double *F;  // scaled values for the backward sweep
   __device__ double init(i64 k) const { return b[c[k]]; }
   __device__ int begin(i64 k) const { return trp[k]; }
-  __device__ int end(i64 k) const { return trp[k + 1]; }
+  __device__ int end(i64 k) const { return tmid[k]; }
   __device__ i64 col(int j) const { return trow[j]; }
   __device__ double coef(int j) const { return ua[tpos[j]]; }
   __device__ double finish(i64 k, double acc) const {
@@ -95,8 +96,24 @@ struct IccBwd {
     x[c[i]] = acc;
     return acc;
   }
-  __device__ double stale(i64) const { return 0.0; }
+  // a column below the diagonal: the backward loop has not reached it yet,
+  // so the reference reads its forward value
+  __device__ double stale(i64 k) const { return F[k]; }
 };
+
+// The late forward contributions: F[k] += ua_ik y_i for the rows i > k whose
+// strict part holds k, in ascending i, after row k was scaled.
+__global__ void k_icc_late(i64 n, const int *trp, const int *trow, const int *tpos, const int *tmid,
+                           const double *ua, const double *Y, double *F, const int64_t *stop) {
+  if (stop && *stop) return;
+  i64 k = (i64)blockIdx.x * 256 + threadIdx.x;
+  if (k >= n) return;
+  int j0 = tmid[k], j1 = trp[k + 1];
+  if (j0 == j1) return;
+  double f = F[k];
+  for (int j = j0; j < j1; j++) f = fma(ua[tpos[j]], Y[trow[j]], f);
+  F[k] = f;
+}
 // SOR, zero initial guess (ported.mojo:655-683)
 struct SorFwd {
   const int *rp, *ci, *dmark;
@@ -164,7 +181,8 @@ void icc_apply(const IccFactor &f, const double *b, double *x, const int64_t *st
   double *S = sweep_scratch(n);
   if (!S || n <= 0) return;
   double *Y = S, *F = S + n, *B = S + 2 * n;
-  run_sweep(IccFwd{f.trp, f.trow, f.tpos, f.udiag, f.c, f.ua, b, F}, n, true, Y, stop);
+  run_sweep(IccFwd{f.trp, f.trow, f.tpos, f.tmid, f.udiag, f.c, f.ua, b, F}, n, true, Y, stop);
+  if (f.nlate) MPK_LAUNCH(k_icc_late, nblocks(n, 256), 256, n, f.trp, f.trow, f.tpos, f.tmid, f.ua, Y, F, stop);
   run_sweep(IccBwd{f.ui, f.uj, f.udiag, f.c, f.ua, F, x}, n, false, B, stop);
 }
 
@@ -215,7 +233,7 @@ void PCView::apply_symmetric(const double *x, double *y, const int64_t *stop) co
 // Transpose of the strict upper part of an IC factor: for column k, the rows
 // i < k holding it, ascending, and the positions of those entries in ua.
 static bool build_icc_map(const int *ui, const int *uj, const int *udiag, i64 n, int *trp_d, int *trow_d,
-                          int *tpos_d) {
+                          int *tpos_d, int *tmid_d, i64 *nlate) {
   Ctx *c = ctx();
   if (!c) return false;
   std::vector<int> hui(n + 1), hud(n);
@@ -236,11 +254,21 @@ static bool build_icc_map(const int *ui, const int *uj, const int *udiag, i64 n,
       trow[p] = (int)i;
       tpos[p] = j;
     }
+  std::vector<int> tmid(n > 0 ? n : 1);
+  i64 late = 0;
+  for (i64 k = 0; k < n; k++) {
+    int j = trp[k];
+    while (j < trp[k + 1] && trow[j] < k) j++;
+    tmid[k] = j;
+    late += trp[k + 1] - j;
+  }
   HIPCHK(hipMemcpy(trp_d, trp.data(), (n + 1) * sizeof(int), hipMemcpyHostToDevice));
   if (m > 0) {
     HIPCHK(hipMemcpy(trow_d, trow.data(), m * sizeof(int), hipMemcpyHostToDevice));
     HIPCHK(hipMemcpy(tpos_d, tpos.data(), m * sizeof(int), hipMemcpyHostToDevice));
   }
+  if (n > 0) HIPCHK(hipMemcpy(tmid_d, tmid.data(), n * sizeof(int), hipMemcpyHostToDevice));
+  if (nlate) *nlate = late;
   return !failed();
 }
 
@@ -276,12 +304,15 @@ bool PCView::parse(const int64_t *pci, const double *pcf, i64 n_) {
       icc.ua = (const double *)pci[3];
       icc.udiag = (const int *)pci[4];
       icc.c = (const int *)pci[5];
-      // the transpose of the strict part travels in aux5..aux7 (mpk_icc_transpose)
+      // the transpose of the strict part travels in aux5..aux8 and the count
+      // of late entries in slot 10 (mpk_icc_transpose)
       icc.trp = (const int *)pci[6];
       icc.trow = (const int *)pci[7];
       icc.tpos = (const int *)pci[8];
-      if (!icc.trp || !icc.trow || !icc.tpos) {
-        set_error(4, "IC descriptor without the transpose of the factor (aux5..aux7)");
+      icc.tmid = (const int *)pci[9];
+      icc.nlate = pci[10];
+      if (!icc.trp || !icc.trow || !icc.tpos || !icc.tmid) {
+        set_error(4, "IC descriptor without the transpose of the factor (aux5..aux8)");
         return false;
       }
       return true;
@@ -395,45 +426,51 @@ int64_t mp_pc_icc_apply(int64_t b, int64_t x, int64_t ui, int64_t uj, int64_t ua
   (void)scratch;
   if (nrow <= 0 || !ctx()) return 0;
   i64 m = icc_map_count((const int *)ui, (const int *)udiag, nrow);
-  int *trp = nullptr, *trow = nullptr, *tpos = nullptr;
+  int *trp = nullptr, *trow = nullptr, *tpos = nullptr, *tmid = nullptr;
+  i64 nlate = 0;
   HIPCHK(hipMalloc(&trp, (nrow + 1) * sizeof(int)));
   HIPCHK(hipMalloc(&trow, (m > 0 ? m : 1) * sizeof(int)));
   HIPCHK(hipMalloc(&tpos, (m > 0 ? m : 1) * sizeof(int)));
-  if (build_icc_map((const int *)ui, (const int *)uj, (const int *)udiag, nrow, trp, trow, tpos)) {
+  HIPCHK(hipMalloc(&tmid, nrow * sizeof(int)));
+  if (build_icc_map((const int *)ui, (const int *)uj, (const int *)udiag, nrow, trp, trow, tpos, tmid, &nlate)) {
     IccFactor f{nrow, (const int *)ui, (const int *)uj, (const double *)ua, (const int *)udiag, (const int *)c,
-                trp, trow, tpos};
+                trp, trow, tpos, tmid, nlate};
     icc_apply(f, (const double *)b, (double *)x, nullptr);
   }
   HIPCHK(hipStreamSynchronize(stream()));
   HIPCHK(hipFree(trp));
   HIPCHK(hipFree(trow));
   HIPCHK(hipFree(tpos));
+  HIPCHK(hipFree(tmid));
   return 0;
 }
 
 // mp_pc_icc_apply with the transpose of the strict part supplied.
 int64_t mpk_pc_icc_apply_t(int64_t b, int64_t x, int64_t ui, int64_t uj, int64_t ua, int64_t udiag, int64_t c,
-                           int64_t trp, int64_t trow, int64_t tpos, int64_t nrow) {
+                           int64_t trp, int64_t trow, int64_t tpos, int64_t tmid, int64_t nlate, int64_t nrow) {
   if (nrow <= 0 || !ctx()) return 0;
   IccFactor f{nrow, (const int *)ui, (const int *)uj, (const double *)ua, (const int *)udiag, (const int *)c,
-              (const int *)trp, (const int *)trow, (const int *)tpos};
+              (const int *)trp, (const int *)trow, (const int *)tpos, (const int *)tmid, nlate};
   icc_apply(f, (const double *)b, (double *)x, nullptr);
   return 0;
 }
 
-// The transpose of an IC factor's strict upper part, for the forward gather:
-// trp[n + 1], trow[m], tpos[m] (device, int32) with m = mpk_icc_transpose_size.
+// The transpose of an IC factor's strict part, for the forward gather:
+// trp[n + 1], trow[m], tpos[m], tmid[n] (device, int32) with
+// m = mpk_icc_transpose_size.  Returns the number of late entries (>= 0) or
+// -1 on failure.
 int64_t mpk_icc_transpose_size(int64_t ui, int64_t udiag, int64_t nrow) {
   if (nrow <= 0 || !ctx()) return 0;
   return icc_map_count((const int *)ui, (const int *)udiag, nrow);
 }
 
 int64_t mpk_icc_transpose(int64_t ui, int64_t uj, int64_t udiag, int64_t nrow, int64_t trp, int64_t trow,
-                          int64_t tpos) {
+                          int64_t tpos, int64_t tmid) {
   if (nrow <= 0) return 0;
+  i64 nlate = 0;
   return build_icc_map((const int *)ui, (const int *)uj, (const int *)udiag, nrow, (int *)trp, (int *)trow,
-                       (int *)tpos)
-             ? 0
+                       (int *)tpos, (int *)tmid, &nlate)
+             ? nlate
              : -1;
 }
 

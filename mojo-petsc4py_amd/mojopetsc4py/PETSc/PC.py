@@ -126,6 +126,7 @@ class PC:
         self._jacobi_type = PC_JACOBI_DIAGONAL
         self._rowl1_scale = 1.0
         self._factor = None      # host factor arrays (int64 / float64)
+        self._nlate = 0          # IC entries below the diagonal (see mpk_icc_transpose)
         self._dbufs = {}         # device buffers of the set up
         self._pci_cache = None
 
@@ -411,7 +412,8 @@ class PC:
             trp = self._buf("trp", 4 * (n + 1))
             trow = self._buf("trow", 4 * max(mt, 1))
             tpos = self._buf("tpos", 4 * max(mt, 1))
-            L.mpk_icc_transpose(bi, bj, bd, n, trp.ptr, trow.ptr, tpos.ptr)
+            tmid = self._buf("tmid", 4 * max(n, 1))
+            self._nlate = L.mpk_icc_transpose(bi, bj, bd, n, trp.ptr, trow.ptr, tpos.ptr, tmid.ptr)
             check()
 
     # ---------------------------------------------------------- descriptor
@@ -442,8 +444,8 @@ class PC:
             pci[1:7] = [b["bi"].ptr, b["bj"].ptr, b["ba"].ptr, b["bdiag"].ptr, b["r"].ptr, b["c"].ptr]
         elif self._type == PCType.ICC and self._factor is not None:
             pci[0] = _ICC
-            pci[1:9] = [b["bi"].ptr, b["bj"].ptr, b["ba"].ptr, b["bdiag"].ptr, b["r"].ptr, b["trp"].ptr,
-                        b["trow"].ptr, b["tpos"].ptr]
+            pci[1:11] = [b["bi"].ptr, b["bj"].ptr, b["ba"].ptr, b["bdiag"].ptr, b["r"].ptr, b["trp"].ptr,
+                         b["trow"].ptr, b["tpos"].ptr, b["tmid"].ptr, self._nlate]
         self._pci_cache = np.array(pci, dtype=np.int64)
         return self._pci_cache
 
@@ -470,7 +472,8 @@ class PC:
                               b["r"].ptr, b["c"].ptr, n)
         else:
             L.mpk_pc_icc_apply_t(xa, y._dw(), b["bi"].ptr, b["bj"].ptr, b["ba"].ptr, b["bdiag"].ptr,
-                                 b["r"].ptr, b["trp"].ptr, b["trow"].ptr, b["tpos"].ptr, n)
+                                 b["r"].ptr, b["trp"].ptr, b["trow"].ptr, b["tpos"].ptr, b["tmid"].ptr,
+                                 self._nlate, n)
         check()
 
     def apply(self, x: Vec, y: Vec) -> None:

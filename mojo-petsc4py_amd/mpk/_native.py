@@ -97,10 +97,10 @@ SIGNATURES = {
     "mpk_synchronize": ([], None),
     "mpk_csr_transpose": ([I] * 7, I),
     "mpk_mat_mult_transpose_t": ([I] * 7, None),
-    "mpk_pc_icc_apply_t": ([I] * 11, I),
+    "mpk_pc_icc_apply_t": ([I] * 13, I),
     "mpk_time_spmv": ([I] * 7, F),
     "mpk_icc_transpose_size": ([I, I, I], I),
-    "mpk_icc_transpose": ([I] * 7, I),
+    "mpk_icc_transpose": ([I] * 8, I),
 }
 
 # exported names that only a device call can exercise

@@ -124,7 +124,8 @@ CD12, CD12B = W.convection_diffusion_3d(12)
 
 
 @pytest.mark.parametrize("ksp_type,pc_type,levels", [
-    ("cg", "jacobi", 0), ("cg", "icc", 0), ("cg", "icc", 1), ("cg", "ilu", 0), ("cg", "sor", 0),
+    ("cg", "jacobi", 0), ("cg", "icc", 0), ("cg", "icc", 1), ("cg", "icc", 2), ("cg", "ilu", 0),
+    ("cg", "ilu", 2), ("cg", "sor", 0),
     ("bcgs", "jacobi", 0), ("bcgs", "sor", 0), ("gmres", "ilu", 0), ("fgmres", "ilu", 0), ("gmres", "jacobi", 0),
 ])
 def test_diffusion_16(native, O, ksp_type, pc_type, levels):
