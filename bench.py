@@ -150,8 +150,10 @@ def main() -> int:
     xs.set(1.0)
     ys = bv.duplicate()
     xa, ya = xs._d(), ys._dw()
-    lib.mpk_time_spmv(xa, ya, d.rp.ptr, d.ci.ptr, d.v.ptr, n, 5)
-    spmv_ms = lib.mpk_time_spmv(xa, ya, d.rp.ptr, d.ci.ptr, d.v.ptr, n, args.spmv_reps)
+    so, sv, sc = d.sell()
+    lib.mpk_time_spmv(xa, ya, d.rp.ptr, d.ci.ptr, d.v.ptr, n, so.ptr, sv.ptr, sc.ptr, 5)
+    spmv_ms = lib.mpk_time_spmv(xa, ya, d.rp.ptr, d.ci.ptr, d.v.ptr, n, so.ptr, sv.ptr, sc.ptr, args.spmv_reps)
+    row_ms = lib.mpk_time_spmv(xa, ya, d.rp.ptr, d.ci.ptr, d.v.ptr, n, 0, 0, 0, args.spmv_reps)
     _native.check()
     sbytes = spmv_bytes(n, nnz)
     spmv_gbs = sbytes / (spmv_ms * 1e-3) / 1e9
@@ -187,9 +189,10 @@ def main() -> int:
         "config": {"workload": f"diffusion_3d {N}^3 7-point lognormal(0,1), CG+Jacobi rtol 1e-10, FP64 int32 CSR",
                    "n": n, "nnz": nnz, "its": its, "reason": reason, "true_relres": relres,
                    "device": dev_name},
-        "roofline": {"bound": "hbm", "kernel": "k_spmv_row", "achieved": spmv_gbs, "peak": HBM_PEAK_GBS,
+        "roofline": {"bound": "hbm", "kernel": "k_spmv_sell", "achieved": spmv_gbs, "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": spmv_gbs / HBM_PEAK_GBS, "traffic": None,
-                     "bytes_per_launch": sbytes, "avg_launch_ms": spmv_ms},
+                     "bytes_per_launch": sbytes, "avg_launch_ms": spmv_ms,
+                     "row_csr_kernel_ms": row_ms},
         "cpu_baseline": cpu,
         "solves": extra,
         "setup_s": time.perf_counter() - t_build,

@@ -45,9 +45,10 @@ void mpk_memcpy_dtoh(int64_t dst, int64_t src, int64_t bytes);  /* synchronous *
 void mpk_memcpy_dtod(int64_t dst, int64_t src, int64_t bytes);  /* queued */
 void mpk_memset(int64_t dst, int64_t value, int64_t bytes);     /* queued */
 void mpk_synchronize(void);
-/* average ms of `reps` SpMV launches, HIP events on the library stream */
-double mpk_time_spmv(int64_t x, int64_t y, int64_t irow, int64_t icol, int64_t data, int64_t nrow,
-                     int64_t reps);
+/* average ms of `reps` SpMV launches (over the sliced copy when soff != 0),
+ * HIP events on the library stream */
+double mpk_time_spmv(int64_t x, int64_t y, int64_t irow, int64_t icol, int64_t data, int64_t nrow, int64_t soff,
+                     int64_t sv, int64_t sc, int64_t reps);
 
 /* ------------------------------------------------------------------ Mat
  * capi.mojo:32-65 -> ported.mojo:380-461.  irow/icol [D] int32, data [D]. */
@@ -65,6 +66,15 @@ int64_t mpk_csr_transpose(int64_t irow, int64_t icol, int64_t nrow, int64_t ncol
                           int64_t tpos);
 void mpk_mat_mult_transpose_t(int64_t x, int64_t y, int64_t data, int64_t ncols, int64_t trp, int64_t trow,
                               int64_t tpos);
+/* No counterpart in the reference: the sliced (64-row, column-major) copy of a
+ * CSR matrix that the products read with coalesced loads; same row sums bit
+ * for bit.  [D] soff[ceil(nrow/64) + 1] int32, sv[padded] float64,
+ * sc[padded] int32, padded = mpk_sell_size(). */
+int64_t mpk_sell_size(int64_t irow, int64_t nrow);
+int64_t mpk_sell_build(int64_t irow, int64_t icol, int64_t data, int64_t nrow, int64_t soff, int64_t sv,
+                       int64_t sc);
+void mpk_sell_mult(int64_t x, int64_t y, int64_t irow, int64_t soff, int64_t sv, int64_t sc, int64_t nrow,
+                   int64_t add);
 
 /* ------------------------------------------------------------------ Vec
  * capi.mojo:69-238 -> ported.mojo:75-335.  Vectors [D]; val, z, idx, sval,
@@ -154,7 +164,9 @@ int64_t mpk_pc_icc_apply_t(int64_t b, int64_t x, int64_t ui, int64_t uj, int64_t
 
 /* ------------------------------------------------------------------ KSP
  * capi.mojo:383-424 -> ksp.mojo.  irow/icol/data, b, x, work, vv, prevec [D];
- * cfg, cfgi, pci, pcf, hs [H].  pci holds [D] addresses: Jacobi {2, diag,
+ * cfg, cfgi, pci, pcf, hs [H].  cfgi[13..15] (unused by the reference) may
+ * hold a sliced copy of the matrix (soff, sv, sc, see mpk_sell_build); when
+ * they are 0 the solve builds one for its duration.  pci holds [D] addresses: Jacobi {2, diag,
  * diagsqrt}; SOR {4, irow, icol, data, dmark, idiag, mdiag, work, ..., flag at
  * 10}; ILU {5, bi, bj, ba, bdiag, r, c}; IC {6, bi, bj, ba, bdiag, r, trp,
  * trow, tpos, tmid, nlate} with the transpose made by mpk_icc_transpose. */

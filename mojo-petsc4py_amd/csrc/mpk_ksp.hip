@@ -14,6 +14,7 @@
 // bitwise those of the CPU path.
 #include <cmath>
 #include <cstring>
+#include <memory>
 #include <vector>
 
 #include "mpk_internal.h"
@@ -697,6 +698,7 @@ static SolveBuffers &solve_buffers() {
 struct Solve {
   i64 n;
   DevCSR A;
+  std::unique_ptr<SellTemp> sell;
   PCView pc;
   KspState h;        // host copy of the initial state
   KspState *d = nullptr;  // device state
@@ -709,6 +711,15 @@ struct Solve {
              const double *pcf, const double *cfg, const int64_t *cfgi) {
     n = n_;
     A = DevCSR{n, n, (const int *)irow, (const int *)icol, (const double *)data};
+    // the sliced copy of A for the products: the caller's (cfgi[13..15]) or
+    // one built for this solve
+    if (cfgi[13] && cfgi[14] && cfgi[15]) {
+      A.sell_off = (const int *)cfgi[13];
+      A.sell_v = (const double *)cfgi[14];
+      A.sell_c = (const int *)cfgi[15];
+    } else {
+      sell.reset(new SellTemp(A));
+    }
     if (!pc.parse(pci, pcf, n)) return false;
     memset(&h, 0, sizeof h);
     h.rtol = cfg[0];

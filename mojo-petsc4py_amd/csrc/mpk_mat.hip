@@ -33,9 +33,104 @@ __global__ void __launch_bounds__(256) k_spmv_row(i64 nrow, const int *__restric
   y[i] = s;
 }
 
+// ------------------------------------------------------------ sliced ELL
+//
+// The same matrix in slices of 64 rows (one wave): slice s stores entry k of
+// its row l at soff[s] + 64 k + l, so each load instruction of a wave reads
+// 512 contiguous bytes of values (256 of indices).  Row i still runs its
+// entries in CSR order, serially, from 0.0 -- the row sum is the CSR one bit
+// for bit; a lane stops at its own row length, and the padding slots of
+// shorter rows are never read.  Blocks are remapped so that each of the 8
+// XCDs sweeps one contiguous range of rows, which keeps the x entries that
+// neighbouring rows share inside that XCD's L2.
+#define MPK_XCDS 8
+
+__device__ __forceinline__ unsigned xcd_block(unsigned b, unsigned nb) {
+  // hardware dispatch hands block b to XCD b % 8; give XCD x a contiguous run
+  unsigned per = nb / MPK_XCDS, rem = nb % MPK_XCDS, x = b % MPK_XCDS, k = b / MPK_XCDS;
+  return x < rem ? x * (per + 1) + k : rem * (per + 1) + (x - rem) * per + k;
+}
+
+__global__ void __launch_bounds__(256) k_spmv_sell(i64 nrow, const int *__restrict__ rp,
+                                                   const int *__restrict__ soff, const double *__restrict__ sv,
+                                                   const int *__restrict__ sc, const double *__restrict__ x,
+                                                   double *__restrict__ y, int add, const int64_t *stop) {
+  if (stop && *stop) return;
+  const unsigned b = xcd_block(blockIdx.x, gridDim.x);
+  const i64 i = (i64)b * 256 + threadIdx.x;
+  if (i >= nrow) return;
+  const int len = rp[i + 1] - rp[i];
+  const int base = soff[i >> 6] + (int)(i & 63);
+  double s = add ? y[i] : 0.0;
+  for (int k = 0; k < len; k++) {
+    const int p = base + 64 * k;
+    s = fma(sv[p], x[sc[p]], s);
+  }
+  y[i] = s;
+}
+
+// scatter CSR into the slice layout (one lane per row)
+__global__ void k_sell_fill(i64 nrow, const int *rp, const int *ci, const double *v, const int *soff, double *sv,
+                            int *sc) {
+  i64 i = (i64)blockIdx.x * 256 + threadIdx.x;
+  if (i >= nrow) return;
+  const int j0 = rp[i], len = rp[i + 1] - j0;
+  const int base = soff[i >> 6] + (int)(i & 63);
+  for (int k = 0; k < len; k++) {
+    sv[base + 64 * k] = v[j0 + k];
+    sc[base + 64 * k] = ci[j0 + k];
+  }
+}
+
+// slice offsets from the row pointers; returns the padded entry count
+i64 sell_offsets(const int *rp_d, i64 nrow, std::vector<int> &soff) {
+  std::vector<int> rp(nrow + 1);
+  HIPCHK(hipMemcpy(rp.data(), rp_d, (nrow + 1) * sizeof(int), hipMemcpyDeviceToHost));
+  i64 ns = (nrow + 63) / 64;
+  soff.assign(ns + 1, 0);
+  i64 total = 0;
+  for (i64 s = 0; s < ns; s++) {
+    int w = 0;
+    for (i64 i = s * 64; i < nrow && i < s * 64 + 64; i++) w = rp[i + 1] - rp[i] > w ? rp[i + 1] - rp[i] : w;
+    soff[s] = (int)total;
+    total += 64 * (i64)w;
+  }
+  soff[ns] = (int)total;
+  return total;
+}
+
 void spmv(const DevCSR &A, const double *x, double *y, int add, const int64_t *stop) {
   if (A.nrow <= 0 || !ctx()) return;
+  if (A.sell_off) {
+    MPK_LAUNCH(k_spmv_sell, nblocks(A.nrow, 256), 256, A.nrow, A.rp, A.sell_off, A.sell_v, A.sell_c, x, y, add,
+               stop);
+    return;
+  }
   MPK_LAUNCH(k_spmv_row, nblocks(A.nrow, 256), 256, A.nrow, A.rp, A.ci, A.v, x, y, add, stop);
+}
+
+// A temporary slice copy of a CSR matrix, owned by the object.
+SellTemp::SellTemp(DevCSR &A) {
+  if (A.nrow <= 0 || A.sell_off || !ctx()) return;
+  std::vector<int> soff;
+  i64 total = sell_offsets(A.rp, A.nrow, soff);
+  if (total >= (i64)1 << 31) return;  // int32 offsets: keep the row kernel
+  HIPCHK(hipMalloc(&off, soff.size() * sizeof(int)));
+  HIPCHK(hipMalloc(&v, (total > 0 ? total : 1) * sizeof(double)));
+  HIPCHK(hipMalloc(&c, (total > 0 ? total : 1) * sizeof(int)));
+  HIPCHK(hipMemcpy(off, soff.data(), soff.size() * sizeof(int), hipMemcpyHostToDevice));
+  MPK_LAUNCH(k_sell_fill, nblocks(A.nrow, 256), 256, A.nrow, A.rp, A.ci, A.v, (const int *)off, v, c);
+  if (failed()) return;
+  A.sell_off = off;
+  A.sell_v = v;
+  A.sell_c = c;
+}
+SellTemp::~SellTemp() {
+  if (!off) return;
+  HIPCHK(hipStreamSynchronize(stream()));
+  HIPCHK(hipFree(off));
+  HIPCHK(hipFree(v));
+  HIPCHK(hipFree(c));
 }
 
 // ------------------------------------------------------------ transpose
@@ -220,12 +315,43 @@ int64_t mp_mat_get_diagonal_markers(int64_t irow, int64_t icol, int64_t diag, in
 
 extern "C" {
 
+// The sliced copy of a CSR matrix, in caller-owned [D] arrays: soff[nslices + 1]
+// (nslices = ceil(nrow / 64)), sv[padded], sc[padded]; mpk_sell_size returns
+// the padded entry count.
+int64_t mpk_sell_size(int64_t irow, int64_t nrow) {
+  if (nrow <= 0 || !ctx()) return 0;
+  std::vector<int> soff;
+  return sell_offsets((const int *)irow, nrow, soff);
+}
+
+int64_t mpk_sell_build(int64_t irow, int64_t icol, int64_t data, int64_t nrow, int64_t soff, int64_t sv,
+                       int64_t sc) {
+  if (nrow <= 0 || !ctx()) return 0;
+  std::vector<int> off;
+  sell_offsets((const int *)irow, nrow, off);
+  HIPCHK(hipMemcpy((void *)soff, off.data(), off.size() * sizeof(int), hipMemcpyHostToDevice));
+  MPK_LAUNCH(k_sell_fill, nblocks(nrow, 256), 256, (i64)nrow, (const int *)irow, (const int *)icol,
+             (const double *)data, (const int *)soff, (double *)sv, (int *)sc);
+  return failed() ? -1 : 0;
+}
+
+// y = A x (add = 0) or y = y + A x (add = 1) over the sliced copy.
+void mpk_sell_mult(int64_t x, int64_t y, int64_t irow, int64_t soff, int64_t sv, int64_t sc, int64_t nrow,
+                   int64_t add) {
+  if (nrow <= 0 || !ctx()) return;
+  DevCSR A{nrow, 0, (const int *)irow, nullptr, nullptr, (const int *)soff, (const double *)sv,
+           (const int *)sc};
+  spmv(A, (const double *)x, (double *)y, (int)add, nullptr);
+}
+
 // Average duration in ms of `reps` back-to-back SpMV launches, timed with HIP
 // events on the library's stream (the stream the kernel runs on).
-double mpk_time_spmv(int64_t x, int64_t y, int64_t irow, int64_t icol, int64_t data, int64_t nrow, int64_t reps) {
+double mpk_time_spmv(int64_t x, int64_t y, int64_t irow, int64_t icol, int64_t data, int64_t nrow, int64_t soff,
+                     int64_t sv, int64_t sc, int64_t reps) {
   Ctx *c = ctx();
   if (!c || reps <= 0) return -1.0;
-  DevCSR A{nrow, 0, (const int *)irow, (const int *)icol, (const double *)data};
+  DevCSR A{nrow, 0, (const int *)irow, (const int *)icol, (const double *)data, (const int *)soff,
+           (const double *)sv, (const int *)sc};
   hipEvent_t e0, e1;
   HIPCHK(hipEventCreate(&e0));
   HIPCHK(hipEventCreate(&e1));

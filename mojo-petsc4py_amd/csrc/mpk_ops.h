@@ -36,8 +36,24 @@ struct DevCSR {
   const int *rp;    // nrow + 1
   const int *ci;    // nnz
   const double *v;  // nnz
+  // optional sliced copy (64-row slices, column-major within a slice)
+  const int *sell_off = nullptr;
+  const double *sell_v = nullptr;
+  const int *sell_c = nullptr;
 };
-// y = A x (add = 0) or y = y + A x (add = 1); serial row sums in CSR order.
+// y = A x (add = 0) or y = y + A x (add = 1); serial row sums in CSR order,
+// over the sliced copy when the matrix has one.
 void spmv(const DevCSR &A, const double *x, double *y, int add, const int64_t *stop);
+
+// Builds a sliced copy of A for the lifetime of the object and attaches it
+// to A (no-op when A already has one).
+struct SellTemp {
+  int *off = nullptr, *c = nullptr;
+  double *v = nullptr;
+  explicit SellTemp(DevCSR &A);
+  ~SellTemp();
+  SellTemp(const SellTemp &) = delete;
+  SellTemp &operator=(const SellTemp &) = delete;
+};
 
 }  // namespace mpk

@@ -298,6 +298,8 @@ class KSP:
         self._resolve_normtype()
         cfg, cfgi = self._state(n)
         d = self._amat._dev()
+        so, sv, sc = d.sell()
+        cfgi[13], cfgi[14], cfgi[15] = so.ptr, sv.ptr, sc.ptr
         pci = self._pc._pci_block()
         pcf = self._pc._pcf_block()
         L = lib()

@@ -81,9 +81,10 @@ class MatSORType:
 
 class _DevCSR:
     """The device copy: int32 row pointers and columns, float64 values, and
-    (built on first use) the transpose map for A^T products."""
+    (built on first use) the sliced copy the products read and the transpose
+    map for A^T products."""
 
-    __slots__ = ("rp", "ci", "v", "nnz", "nrows", "ncols", "tmap")
+    __slots__ = ("rp", "ci", "v", "nnz", "nrows", "ncols", "tmap", "sellc")
 
     def __init__(self, irow, icol, data, nrows, ncols):
         nnz = int(irow[nrows]) if nrows else 0
@@ -96,6 +97,22 @@ class _DevCSR:
         self.ci = DeviceBuffer.from_array(np.asarray(icol[: max(nnz, 1)], dtype=np.int32))
         self.v = DeviceBuffer.from_array(np.asarray(data[: max(nnz, 1)], dtype=np.float64))
         self.tmap = None
+        self.sellc = None
+
+    def sell(self):
+        """(soff, sv, sc): rows in 64-row slices, column-major inside a slice,
+        so a wave's loads are contiguous; row sums stay in CSR order."""
+        if self.sellc is None:
+            L = lib()
+            padded = L.mpk_sell_size(self.rp.ptr, self.nrows)
+            check()
+            soff = DeviceBuffer(4 * ((self.nrows + 63) // 64 + 1))
+            sv = DeviceBuffer(8 * max(padded, 1))
+            sc = DeviceBuffer(4 * max(padded, 1))
+            if L.mpk_sell_build(self.rp.ptr, self.ci.ptr, self.v.ptr, self.nrows, soff.ptr, sv.ptr, sc.ptr) != 0:
+                check()
+            self.sellc = (soff, sv, sc)
+        return self.sellc
 
     def transpose_map(self):
         if self.tmap is None:
@@ -479,7 +496,8 @@ class Mat:
             raise ValueError(f"incompatible sizes: matrix {self._ncols} columns, vector {x.getSize()}")
         d = self._dev()
         xa = x._d()
-        lib().mp_mat_mult(xa, out._dw(), d.rp.ptr, d.ci.ptr, d.v.ptr, self._nrows)
+        so, sv, sc = d.sell()
+        lib().mpk_sell_mult(xa, out._dw(), d.rp.ptr, so.ptr, sv.ptr, sc.ptr, self._nrows, 0)
         check()
         return out
 
@@ -494,7 +512,8 @@ class Mat:
             v.copy(out)
         d = self._dev()
         xa = x._d()
-        lib().mp_mat_mult_add(xa, out._dw(), d.rp.ptr, d.ci.ptr, d.v.ptr, self._nrows)
+        so, sv, sc = d.sell()
+        lib().mpk_sell_mult(xa, out._dw(), d.rp.ptr, so.ptr, sv.ptr, sc.ptr, self._nrows, 1)
         check()
         return out
 

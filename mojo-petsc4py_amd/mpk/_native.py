@@ -98,7 +98,10 @@ SIGNATURES = {
     "mpk_csr_transpose": ([I] * 7, I),
     "mpk_mat_mult_transpose_t": ([I] * 7, None),
     "mpk_pc_icc_apply_t": ([I] * 13, I),
-    "mpk_time_spmv": ([I] * 7, F),
+    "mpk_time_spmv": ([I] * 10, F),
+    "mpk_sell_size": ([I, I], I),
+    "mpk_sell_build": ([I] * 7, I),
+    "mpk_sell_mult": ([I] * 8, None),
     "mpk_icc_transpose_size": ([I, I, I], I),
     "mpk_icc_transpose": ([I] * 8, I),
 }

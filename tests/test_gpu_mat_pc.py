@@ -41,6 +41,51 @@ def O(oracle):
     oracle.set_mode("ref")
 
 
+def _ragged(n, seed):
+    """Rows of very uneven length: empty rows, a dense row, short rows."""
+    rng = np.random.default_rng(seed)
+    rows, cols = [], []
+    for i in range(n):
+        k = 0 if i % 7 == 3 else (n if i == n // 2 else int(rng.integers(1, 12)))
+        for c in sorted(set(rng.integers(0, n, k).tolist())):
+            rows.append(i)
+            cols.append(c)
+    return W._csr_from_coo(n, np.array(rows, dtype=np.int64), np.array(cols, dtype=np.int64),
+                           rng.normal(size=len(rows)))
+
+
+SELL_MATS = dict(MATS)
+SELL_MATS.update({f"ragged{n}": (lambda n=n: _ragged(n, n)) for n in (1, 63, 64, 65, 130, 1000)})
+
+
+@pytest.mark.parametrize("name", sorted(SELL_MATS))
+def test_sliced_spmv_bitwise(native, O, name):
+    """The sliced (64-row, column-major) copy gives the CSR row sums bit for
+    bit, for y = A x and y = y + A x, including empty rows and partial slices."""
+    ip, ix, va = SELL_MATS[name]()
+    A = O.CSR(ip, ix, va)
+    D = DevMat(ip, ix, va)
+    n = A.n
+    padded = native.mpk_sell_size(D.rp.ptr, n)
+    widths = [max((ip[i + 1] - ip[i]) for i in range(s, min(s + 64, n))) for s in range(0, n, 64)]
+    assert padded == 64 * sum(widths)
+    so = dev(np.zeros((n + 63) // 64 + 1, np.int32))
+    sv = dev(np.zeros(max(padded, 1)))
+    sc = dev(np.zeros(max(padded, 1), np.int32))
+    assert native.mpk_sell_build(D.rp.ptr, D.ci.ptr, D.v.ptr, n, so.ptr, sv.ptr, sc.ptr) == 0
+    x = rng_vec(n, 1)
+    y0 = rng_vec(n, 2)
+    dx, dy = dev(x), dev(y0)
+    native.mpk_sell_mult(dx.ptr, dy.ptr, D.rp.ptr, so.ptr, sv.ptr, sc.ptr, n, 0)
+    assert same_bits(back(dy, np.float64, n), A.mult(x))
+    dy = dev(y0)
+    native.mpk_sell_mult(dx.ptr, dy.ptr, D.rp.ptr, so.ptr, sv.ptr, sc.ptr, n, 1)
+    h = y0.copy()
+    O.lib().or_mat_mult_add(x.ctypes.data, h.ctypes.data, A.irow.ctypes.data, A.icol.ctypes.data,
+                            A.data.ctypes.data, n)
+    assert same_bits(back(dy, np.float64, n), h)
+
+
 @pytest.mark.parametrize("name", sorted(MATS))
 def test_spmv_family_bitwise(native, O, name):
     ip, ix, va = MATS[name]()
