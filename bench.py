@@ -36,6 +36,11 @@ import numpy as np  # noqa: E402
 
 HBM_PEAK_GBS = 8000.0
 
+# HBM bytes per launch of the roofline kernel, from rocprofv3 FETCH_SIZE and
+# WRITE_SIZE runs scaled by a copy kernel of known size
+# (profiles/spmv_sell_diffusion128_traffic.md), keyed by (kernel, n, nnz).
+MEASURED_TRAFFIC = {("k_spmv_sell", 2097152, 14581760): 219285052}
+
 
 def spmv_bytes(n: int, nnz: int) -> int:
     return 12 * nnz + 4 * (n + 1) + 16 * n
@@ -102,6 +107,30 @@ def cpu_baseline(csr, b, gpu_its: int, sample_its: int):
     return per_it * gpu_its, per_it, r.its
 
 
+def spmv_only(lib, A, bv, n, nnz, reps):
+    """The SpMV timing loop alone, then 10 copies of 2^24 doubles (a kernel
+    whose HBM traffic is known: 128 MiB read, 128 MiB written per launch), so a
+    counter run can scale its readings."""
+    from mpk import _native
+    from mpk.device import DeviceBuffer
+    d = A._dev()
+    xs = bv.duplicate()
+    xs.set(1.0)
+    ys = bv.duplicate()
+    so, sv, sc = d.sell()
+    ms = lib.mpk_time_spmv(xs._d(), ys._dw(), d.rp.ptr, d.ci.ptr, d.v.ptr, n, so.ptr, sv.ptr, sc.ptr, reps)
+    m = 1 << 24
+    src, dst = DeviceBuffer(8 * m), DeviceBuffer(8 * m)
+    lib.mp_vec_set(src.ptr, 1.0, m)
+    for _ in range(10):
+        lib.mp_vec_copy(src.ptr, dst.ptr, m)
+    lib.mpk_synchronize()
+    _native.check()
+    print(json.dumps({"spmv_ms": ms, "spmv_bytes": spmv_bytes(n, nnz), "launches": reps,
+                      "copy_bytes_each_way": 8 * m, "copy_launches": 10}))
+    return 0
+
+
 def cpu_model() -> str:
     try:
         with open("/proc/cpuinfo") as f:
@@ -123,6 +152,8 @@ def main() -> int:
     ap.add_argument("--cpu-its", type=int, default=60, help="iterations in the CPU sample")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-icc", action="store_true")
+    ap.add_argument("--spmv-only", action="store_true",
+                    help="only the SpMV timing loop and a copy of known size (for hardware counter runs)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py runs on one GPU: this build has no multi-GPU path (see DESIGN.md for how it"
@@ -138,6 +169,8 @@ def main() -> int:
     n = A.getSize()[0]
     nnz = int(csr[0][-1])
     x = bv.duplicate()
+    if args.spmv_only:
+        return spmv_only(lib, A, bv, n, nnz, args.spmv_reps)
 
     ksp = make_ksp(A, "jacobi")
     t_solve = timed_solves(ksp, bv, x, args.steps, args.warmup)
@@ -190,7 +223,8 @@ def main() -> int:
                    "n": n, "nnz": nnz, "its": its, "reason": reason, "true_relres": relres,
                    "device": dev_name},
         "roofline": {"bound": "hbm", "kernel": "k_spmv_sell", "achieved": spmv_gbs, "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": spmv_gbs / HBM_PEAK_GBS, "traffic": None,
+                     "unit": "GB/s", "frac": spmv_gbs / HBM_PEAK_GBS,
+                     "traffic": MEASURED_TRAFFIC.get(("k_spmv_sell", n, nnz)),
                      "bytes_per_launch": sbytes, "avg_launch_ms": spmv_ms,
                      "row_csr_kernel_ms": row_ms},
         "cpu_baseline": cpu,

@@ -158,6 +158,30 @@ void mp_ordering_rcm(int64_t perm, int64_t level, int64_t visited, int64_t queue
 int64_t mpk_icc_transpose_size(int64_t ui, int64_t udiag, int64_t nrow);
 int64_t mpk_icc_transpose(int64_t ui, int64_t uj, int64_t udiag, int64_t nrow, int64_t trp, int64_t trow,
                           int64_t tpos, int64_t tmid);
+/* No counterpart in the reference: block plans for the two sweeps of the PC a
+ * descriptor describes (SOR, ILU, IC).  A sweep's rows are cut into blocks of
+ * consecutive rows; a workgroup runs a block level by level (levels of the
+ * dependencies inside the block) with the block's values in LDS.  The plan
+ * copies the sweep's coefficients, so it is rebuilt with every new factor.
+ * mpk_pc_schedule_size returns the bytes to allocate; mpk_pc_schedule writes
+ * the plans into plan [D] (cap bytes) and returns the deepest block's level
+ * count (0: the PC has no sweeps; -1: error).  With pci[11] = plan (unused by
+ * the reference, 0 = plain row order) every apply runs its sweeps over the
+ * plans; the arithmetic of each row is unchanged. */
+int64_t mpk_pc_schedule_size(int64_t n, int64_t pci, int64_t pcf);
+int64_t mpk_pc_schedule(int64_t n, int64_t pci, int64_t pcf, int64_t plan, int64_t cap);
+/* No counterpart in the reference: the host part of mpk_pc_schedule for one
+ * sweep.  off [H] int64[n+1], col [H] int32, coef [H] float64: every entry of
+ * every row in the row's own order; forward 1 visits rows 0..n-1, 0 visits
+ * n-1..0; bmax: most rows per block.  out [H] (cap bytes) receives the plan
+ * (layout in mpk_sweep.h).  Returns its size, or minus the size needed.
+ * mpk_sweep_plan_bound: an upper bound of that size.  Needs no device. */
+int64_t mpk_sweep_plan_bound(int64_t n, int64_t m, int64_t bmax);
+int64_t mpk_sweep_plan(int64_t n, int64_t forward, int64_t off, int64_t col, int64_t coef, int64_t bmax, int64_t out,
+                       int64_t cap);
+/* No counterpart in the reference: y = M^-1 x through a descriptor, as the
+ * solvers apply it (KSP_PCApply, ksp.mojo:155-198, left side). x, y [D]. */
+void mpk_pc_apply(int64_t n, int64_t pci, int64_t pcf, int64_t x, int64_t y);
 /* mp_pc_icc_apply with that transpose supplied (no per-call rebuild) */
 int64_t mpk_pc_icc_apply_t(int64_t b, int64_t x, int64_t ui, int64_t uj, int64_t ua, int64_t udiag, int64_t c,
                            int64_t trp, int64_t trow, int64_t tpos, int64_t tmid, int64_t nlate, int64_t nrow);
@@ -169,7 +193,8 @@ int64_t mpk_pc_icc_apply_t(int64_t b, int64_t x, int64_t ui, int64_t uj, int64_t
  * they are 0 the solve builds one for its duration.  pci holds [D] addresses: Jacobi {2, diag,
  * diagsqrt}; SOR {4, irow, icol, data, dmark, idiag, mdiag, work, ..., flag at
  * 10}; ILU {5, bi, bj, ba, bdiag, r, c}; IC {6, bi, bj, ba, bdiag, r, trp,
- * trow, tpos, tmid, nlate} with the transpose made by mpk_icc_transpose. */
+ * trow, tpos, tmid, nlate} with the transpose made by mpk_icc_transpose;
+ * pci[11] may hold block plans from mpk_pc_schedule (SOR, ILU, IC). */
 void mp_ksp_solve_cg(int64_t n, int64_t irow, int64_t icol, int64_t data, int64_t pci, int64_t pcf, int64_t pstmp,
                      int64_t work, int64_t cfg, int64_t cfgi, int64_t b, int64_t x);
 void mp_ksp_solve_bcgs(int64_t n, int64_t irow, int64_t icol, int64_t data, int64_t pci, int64_t pcf,

@@ -13,6 +13,8 @@
 #include <cmath>
 #include <cstdint>
 #include <cstring>
+#include <algorithm>
+#include <vector>
 
 #include "../../include/mpk.h"
 
@@ -401,6 +403,128 @@ void mp_pc_icc_numeric(int64_t ua_, int64_t rtmp_, int64_t il_, int64_t c2r_, in
     }
     ua[bd[k]] = 1.0 / dk;
   }
+}
+
+// Block plan of a triangular sweep (host part; the device part is
+// k_sweep_blk in mpk_sweep.h, which documents the layout and encoding).  off [H]
+// int64[n+1], col [H] int32 and coef [H] float64 hold every entry of every
+// row in the row's own order (the order its serial sum takes).  Forward
+// sweeps visit rows 0..n-1, backward sweeps n-1..0; an entry is a dependency
+// when its row comes earlier in the sweep.  Blocks are bmax consecutive
+// positions when n > bmax, sized so that there are about 256 of them (but not
+// under 1024 rows); a smaller sweep is one block.
+static int64_t plan_rows(int64_t n, int64_t bmax) {
+  if (n <= bmax) return n > 0 ? n : 1;
+  int64_t b = (n + 255) / 256;
+  b = b < 1024 ? 1024 : b;
+  return b > bmax ? bmax : b;
+}
+// Rows with at most this many entries are stored in fixed slots
+// (MPK_PLAN_W in mpk_sweep.h).
+static const int64_t kPlanW = 4;
+static int64_t plan_bytes(int64_t n, int64_t nblk, int64_t nlvl, int64_t slots) {
+  const int64_t ints = 2 * (nblk + 1) + nlvl + 3 * n + 1;
+  return 64 + ((4 * ints + 7) & ~(int64_t)7) + ((12 * slots + 7) & ~(int64_t)7);
+}
+
+// Upper bound of the plan size in bytes for n rows and m entries.
+int64_t mpk_sweep_plan_bound(int64_t n, int64_t m, int64_t bmax) {
+  const int64_t B = plan_rows(n, bmax), nblk = n > 0 ? (n + B - 1) / B : 0;
+  return plan_bytes(n, nblk, n + nblk, std::max(m, kPlanW * n));
+}
+
+// Writes the plan into out [H] (cap bytes).  Returns its size in bytes, or
+// minus the size needed when cap is too small.
+int64_t mpk_sweep_plan(int64_t n, int64_t forward, int64_t off_, int64_t col_, int64_t coef_, int64_t bmax,
+                       int64_t out_, int64_t cap) {
+  const int64_t *off = (const int64_t *)off_;
+  const int *col = (const int *)col_;
+  const double *coef = (const double *)coef_;
+  char *out = (char *)out_;
+  const int64_t m = n > 0 ? off[n] : 0;
+  const int64_t B = plan_rows(n, bmax), nblk = n > 0 ? (n + B - 1) / B : 0;
+  auto spos = [&](int64_t i) { return forward ? i : n - 1 - i; };  // sweep position of row i
+  auto earlier = [&](int64_t k, int64_t i) { return forward ? k < i : k > i; };
+  // local levels
+  std::vector<int> lev(n > 0 ? n : 1, 0);
+  std::vector<int> nlev(nblk > 0 ? nblk : 1, 0);
+  for (int64_t s = 0; s < n; s++) {
+    const int64_t i = forward ? s : n - 1 - s, b = s / B;
+    int l = 0;
+    for (int64_t e = off[i]; e < off[i + 1]; e++) {
+      const int64_t k = col[e];
+      if (earlier(k, i) && spos(k) / B == b) l = std::max(l, lev[k] + 1);
+    }
+    lev[i] = l;
+    nlev[b] = std::max(nlev[b], l + 1);
+  }
+  int64_t nlvl = 0, depth = 0, widest = 0;
+  for (int64_t b = 0; b < nblk; b++) {
+    nlvl += nlev[b] + 1;
+    depth = std::max<int64_t>(depth, nlev[b]);
+  }
+  for (int64_t i = 0; i < n; i++) widest = std::max(widest, off[i + 1] - off[i]);
+  const int64_t w = widest <= kPlanW ? kPlanW : 0, slots = w ? w * n : m;
+  const int64_t bytes = plan_bytes(n, nblk, nlvl, slots);
+  if (bytes > cap) return -bytes;
+  std::memset(out, 0, (size_t)bytes);
+  int64_t *hdr = (int64_t *)out;
+  hdr[0] = nblk;
+  hdr[1] = m;
+  hdr[2] = nlvl;
+  hdr[3] = depth;
+  hdr[4] = bytes;
+  hdr[5] = w;
+  int *blk_pos = (int *)(out + 64), *blk_lev = blk_pos + nblk + 1, *lvl = blk_lev + nblk + 1;
+  int *prow = lvl + nlvl, *pcnt = prow + n, *pent = pcnt + n;
+  const int64_t ints = 2 * (nblk + 1) + nlvl + 3 * n + 1;
+  double *ecoef = (double *)(out + 64 + ((4 * ints + 7) & ~(int64_t)7));
+  int *edep = (int *)(ecoef + slots);
+  // positions: blocks in sweep order, rows by level inside a block (sweep
+  // order within a level)
+  std::vector<int> pos(n > 0 ? n : 1);
+  int64_t li = 0;
+  for (int64_t b = 0; b < nblk; b++) {
+    const int64_t s0 = b * B, s1 = std::min(n, s0 + B);
+    std::vector<int64_t> start((size_t)nlev[b] + 1, 0);
+    for (int64_t s = s0; s < s1; s++) start[(size_t)lev[forward ? s : n - 1 - s] + 1]++;
+    for (size_t l = 0; l + 1 < start.size(); l++) start[l + 1] += start[l];
+    blk_pos[b] = (int)s0;
+    blk_lev[b] = (int)li;
+    for (int64_t l = 0; l < nlev[b]; l++) lvl[li + l] = (int)(s0 + start[(size_t)l]);
+    lvl[li + nlev[b]] = (int)s1;
+    li += nlev[b] + 1;
+    for (int64_t s = s0; s < s1; s++) {
+      const int64_t i = forward ? s : n - 1 - s;
+      const int64_t p = s0 + start[(size_t)lev[i]]++;
+      prow[p] = (int)i;
+      pos[i] = (int)p;
+    }
+  }
+  blk_pos[nblk] = (int)n;
+  blk_lev[nblk] = (int)li;
+  // entries in position order (fixed slots: entry e of position p at p w + e)
+  int64_t e_out = 0;
+  for (int64_t p = 0; p < n; p++) {
+    const int64_t i = prow[p], b = spos(i) / B;
+    pent[p] = (int)e_out;
+    pcnt[p] = (int)(off[i + 1] - off[i]);
+    if (w) e_out = p * w;
+    for (int64_t e = off[i]; e < off[i + 1]; e++, e_out++) {
+      const int64_t k = col[e];
+      ecoef[e_out] = coef[e];
+      if (!earlier(k, i))
+        edep[e_out] = (int)(-k - n - 1);
+      else if (spos(k) / B == b)
+        edep[e_out] = pos[k] - blk_pos[b];
+      else
+        edep[e_out] = (int)(-k - 1);
+    }
+  }
+  pent[n] = (int)(w ? 0 : e_out);
+  if (w)
+    for (int64_t p = 0; p <= n; p++) pent[p] = 0;  // unused with fixed slots
+  return bytes;
 }
 
 }  // extern "C"

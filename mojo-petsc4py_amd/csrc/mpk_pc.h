@@ -16,6 +16,7 @@ struct IluFactor {
   const int *bi, *bj;
   const double *ba;
   const int *bdiag, *r, *c;
+  const char *plan = nullptr;  // block plans of the two sweeps (mpk_pc_schedule), or null
 };
 
 // IC factor in the reference layout (factor.mojo:125-273): row i holds its
@@ -32,6 +33,7 @@ struct IccFactor {
   const int *udiag, *c;
   const int *trp, *trow, *tpos, *tmid;
   i64 nlate;
+  const char *plan = nullptr;  // block plans of the two sweeps (mpk_pc_schedule), or null
 };
 
 struct SorData {
@@ -42,6 +44,7 @@ struct SorData {
   double *idiag, *mdiag, *t;
   double omega, fshift;
   int flag;
+  const char *plan = nullptr;  // block plans of the two sweeps (mpk_pc_schedule), or null
 };
 
 void ilu_apply(const IluFactor &f, const double *b, double *x, const int64_t *stop);
@@ -60,6 +63,11 @@ struct PCView {
   bool parse(const int64_t *pci, const double *pcf, i64 n);
   void apply(const double *x, double *y, const int64_t *stop) const;            // y = M^-1 x
   void apply_symmetric(const double *x, double *y, const int64_t *stop) const;  // half of M^-1
+  // block plans of the PC's two sweeps: bytes needed, then the plans into
+  // plan (cap bytes; returns the deepest block's level count, 0 for a PC
+  // without sweeps, -1 on error)
+  i64 schedule_size() const;
+  i64 schedule(char *plan, i64 cap) const;
 };
 
 }  // namespace mpk

@@ -28,12 +28,92 @@ __global__ void k_sweep_prepare(i64 n, double *p, unsigned *ticket, const int64_
   if (i < n) p[i] = __longlong_as_double((long long)MPK_SENTINEL);
 }
 
+// One sweep: over its block plan when the PC has one (`plan`, and `second`
+// for the PC's second sweep), else in row order.
 template <class Prog>
-static void run_sweep(const Prog &prog, i64 n, bool forward, double *pub, const int64_t *stop) {
+static void run_sweep(const Prog &prog, i64 n, bool forward, double *pub, const int64_t *stop,
+                      const char *plan = nullptr, int second = 0, StagedRows st = StagedRows{}) {
   Ctx *c = ctx();
   if (!c || n <= 0) return;
   MPK_LAUNCH(k_sweep_prepare, nblocks(n, 256), 256, n, pub, c->tickets, stop);
-  MPK_LAUNCH(k_sweep<Prog>, nblocks(n, 64), 64, prog, n, forward ? 1 : 0, c->tickets, 0u, pub, stop);
+  if (plan) {
+    MPK_LAUNCH(k_sweep_stage<Prog>, nblocks(n, 256), 256, prog, n, plan, second, st, stop);
+    // blocks hold at least min(n, 1024) rows (mpk_sweep_plan), so this grid
+    // covers every block; the spare workgroups find no block and return
+    MPK_LAUNCH(k_sweep_blk<Prog>, nblocks(n, 1024), 256, prog, n, plan, second, st, c->tickets, pub, stop);
+  } else {
+    MPK_LAUNCH(k_sweep<Prog>, nblocks(n, 64), 64, prog, n, forward ? 1 : 0, c->tickets, 0u, pub, stop);
+  }
+}
+
+// The entries of one sweep on the host: off[n+1], col, coef (k_sweep_describe).
+template <class Prog>
+static bool describe_sweep(const Prog &prog, i64 n, std::vector<i64> &off, std::vector<int> &col,
+                           std::vector<double> &coef) {
+  int *cnt_d = nullptr, *col_d = nullptr;
+  i64 *off_d = nullptr;
+  double *coef_d = nullptr;
+  std::vector<int> cnt(n);
+  HIPCHK(hipMalloc(&cnt_d, n * sizeof(int)));
+  if (failed()) return false;
+  MPK_LAUNCH(k_sweep_describe<Prog>, nblocks(n, 256), 256, prog, n, 0, cnt_d, (const i64 *)nullptr,
+             (int *)nullptr, (double *)nullptr);
+  HIPCHK(hipMemcpyAsync(cnt.data(), cnt_d, n * sizeof(int), hipMemcpyDeviceToHost, stream()));
+  HIPCHK(hipStreamSynchronize(stream()));
+  off.assign(n + 1, 0);
+  for (i64 i = 0; i < n; i++) off[i + 1] = off[i] + cnt[i];
+  const i64 m = off[n], mm = m > 0 ? m : 1;
+  col.assign(mm, 0);
+  coef.assign(mm, 0.0);
+  HIPCHK(hipMalloc(&off_d, (n + 1) * sizeof(i64)));
+  HIPCHK(hipMalloc(&col_d, mm * sizeof(int)));
+  HIPCHK(hipMalloc(&coef_d, mm * sizeof(double)));
+  if (!failed()) {
+    HIPCHK(hipMemcpy(off_d, off.data(), (n + 1) * sizeof(i64), hipMemcpyHostToDevice));
+    MPK_LAUNCH(k_sweep_describe<Prog>, nblocks(n, 256), 256, prog, n, 1, cnt_d, (const i64 *)off_d, col_d, coef_d);
+    HIPCHK(hipMemcpyAsync(col.data(), col_d, mm * sizeof(int), hipMemcpyDeviceToHost, stream()));
+    HIPCHK(hipMemcpyAsync(coef.data(), coef_d, mm * sizeof(double), hipMemcpyDeviceToHost, stream()));
+    HIPCHK(hipStreamSynchronize(stream()));
+  }
+  HIPCHK(hipFree(cnt_d));
+  HIPCHK(hipFree(off_d));
+  HIPCHK(hipFree(col_d));
+  HIPCHK(hipFree(coef_d));
+  return !failed();
+}
+
+// The block plan of one sweep into dst [D] (cap bytes, at least the bound);
+// returns its size in bytes and the deepest block's level count, or -1.
+template <class Prog>
+static i64 plan_sweep(const Prog &prog, i64 n, bool forward, char *dst, i64 cap, i64 *depth) {
+  std::vector<i64> off;
+  std::vector<int> col;
+  std::vector<double> coef;
+  if (!describe_sweep(prog, n, off, col, coef)) return -1;
+  const i64 bound = mpk_sweep_plan_bound(n, off[n], MPK_BLK_ROWS);
+  if (bound > cap) {
+    set_error(5, "sweep plan buffer too small");
+    return -1;
+  }
+  std::vector<int64_t> host((size_t)(bound + 7) / 8);
+  const i64 bytes = mpk_sweep_plan(n, forward ? 1 : 0, (int64_t)off.data(), (int64_t)col.data(),
+                                   (int64_t)coef.data(), MPK_BLK_ROWS, (int64_t)host.data(), bound);
+  if (bytes < 0) {
+    set_error(5, "sweep plan larger than its bound");
+    return -1;
+  }
+  HIPCHK(hipMemcpy(dst, host.data(), bytes, hipMemcpyHostToDevice));
+  *depth = host[3];
+  return failed() ? -1 : bytes;
+}
+
+// Entry count of one sweep (for the plan bound).
+template <class Prog>
+static i64 sweep_entries(const Prog &prog, i64 n) {
+  std::vector<i64> off;
+  std::vector<int> col;
+  std::vector<double> coef;
+  return describe_sweep(prog, n, off, col, coef) ? off[n] : -1;
 }
 
 // ------------------------------------------------------------ row programs
@@ -41,26 +121,26 @@ static void run_sweep(const Prog &prog, i64 n, bool forward, double *pub, const 
 struct IluFwd {  // tmp_i = b[r_i] - sum_{L row i} ba_j tmp[bj_j]
   const int *bi, *bj, *r;
   const double *ba, *b;
-  __device__ double init(i64 i) const { return b[r[i]]; }
+  __device__ Staged stage(i64 i) const { return Staged{b[r[i]], 0.0, 0.0, 0}; }
   __device__ int begin(i64 i) const { return bi[i]; }
   __device__ int end(i64 i) const { return bi[i + 1]; }
   __device__ i64 col(int j) const { return bj[j]; }
   __device__ double coef(int j) const { return -ba[j]; }
-  __device__ double finish(i64, double acc) const { return acc; }
+  __device__ double fin(i64, double acc, const Staged &) const { return acc; }
   __device__ double stale(i64) const { return 0.0; }
 };
 struct IluBwd {  // sum = tmp_i - sum_{U row i} ba tmp; x[c_i] = sum * ba[bdiag_i]
   const int *bj, *bdiag, *c;
   const double *ba, *F;
   double *x;
-  __device__ double init(i64 i) const { return F[i]; }
+  __device__ Staged stage(i64 i) const { return Staged{F[i], ba[bdiag[i]], 0.0, c[i]}; }
   __device__ int begin(i64 i) const { return bdiag[i + 1] + 1; }
   __device__ int end(i64 i) const { return bdiag[i]; }
   __device__ i64 col(int j) const { return bj[j]; }
   __device__ double coef(int j) const { return -ba[j]; }
-  __device__ double finish(i64 i, double acc) const {
-    double r = acc * ba[bdiag[i]];
-    x[c[i]] = r;
+  __device__ double fin(i64, double acc, const Staged &st) const {
+    double r = acc * st.s1;
+    x[st.o] = r;
     return r;
   }
   __device__ double stale(i64) const { return 0.0; }
@@ -72,28 +152,32 @@ struct IccFwd {
   const int *trp, *trow, *tpos, *tmid, *udiag, *c;
   const double *ua, *b;
   double *F;  // scaled values for the backward sweep
-  __device__ double init(i64 k) const { return b[c[k]]; }
+  __device__ Staged stage(i64 k) const { return Staged{b[c[k]], ua[udiag[k]], 0.0, 0}; }
   __device__ int begin(i64 k) const { return trp[k]; }
   __device__ int end(i64 k) const { return tmid[k]; }
   __device__ i64 col(int j) const { return trow[j]; }
   __device__ double coef(int j) const { return ua[tpos[j]]; }
-  __device__ double finish(i64 k, double acc) const {
-    F[k] = acc * ua[udiag[k]];
+  __device__ double fin(i64 k, double acc, const Staged &st) const {
+    F[k] = acc * st.s1;
     return acc;
   }
   __device__ double stale(i64) const { return 0.0; }
 };
+// IC backward (ported.mojo:902-952): rows n-2 down to 0 add their strict
+// part; the last row is copied as it is, so a strict part it holds (late
+// entries, columns below the diagonal) is not applied.
 struct IccBwd {
   const int *ui, *uj, *udiag, *c;
   const double *ua, *F;
   double *x;
-  __device__ double init(i64 i) const { return F[i]; }
+  i64 last;  // n - 1
+  __device__ Staged stage(i64 i) const { return Staged{F[i], 0.0, 0.0, c[i]}; }
   __device__ int begin(i64 i) const { return ui[i]; }
-  __device__ int end(i64 i) const { return udiag[i]; }
+  __device__ int end(i64 i) const { return i == last ? ui[i] : udiag[i]; }
   __device__ i64 col(int j) const { return uj[j]; }
   __device__ double coef(int j) const { return ua[j]; }
-  __device__ double finish(i64 i, double acc) const {
-    x[c[i]] = acc;
+  __device__ double fin(i64, double acc, const Staged &st) const {
+    x[st.o] = acc;
     return acc;
   }
   // a column below the diagonal: the backward loop has not reached it yet,
@@ -102,7 +186,9 @@ struct IccBwd {
 };
 
 // The late forward contributions: F[k] += ua_ik y_i for the rows i > k whose
-// strict part holds k, in ascending i, after row k was scaled.
+// strict part holds k, in ascending i, after row k was scaled.  A row whose
+// strict part holds its own column (trow == k, which the reference's staging
+// can produce) adds to itself before the scaling overwrites it: no effect.
 __global__ void k_icc_late(i64 n, const int *trp, const int *trow, const int *tpos, const int *tmid,
                            const double *ua, const double *Y, double *F, const int64_t *stop) {
   if (stop && *stop) return;
@@ -111,7 +197,8 @@ __global__ void k_icc_late(i64 n, const int *trp, const int *trow, const int *tp
   int j0 = tmid[k], j1 = trp[k + 1];
   if (j0 == j1) return;
   double f = F[k];
-  for (int j = j0; j < j1; j++) f = fma(ua[tpos[j]], Y[trow[j]], f);
+  for (int j = j0; j < j1; j++)
+    if (trow[j] != k) f = fma(ua[tpos[j]], Y[trow[j]], f);
   F[k] = f;
 }
 // SOR, zero initial guess (ported.mojo:655-683)
@@ -120,14 +207,14 @@ struct SorFwd {
   const double *a, *x, *idiag;
   double *t, *yout;  // yout: final y when no backward sweep follows (else null)
   const double *yin;
-  __device__ double init(i64 i) const { return x[i]; }
+  __device__ Staged stage(i64 i) const { return Staged{x[i], idiag[i], 0.0, 0}; }
   __device__ int begin(i64 i) const { return rp[i]; }
   __device__ int end(i64 i) const { return dmark[i]; }
   __device__ i64 col(int j) const { return ci[j]; }
   __device__ double coef(int j) const { return -a[j]; }
-  __device__ double finish(i64 i, double acc) const {
+  __device__ double fin(i64 i, double acc, const Staged &st) const {
     t[i] = acc;
-    double r = acc * idiag[i];
+    double r = acc * st.s1;
     if (yout) yout[i] = r;
     return r;
   }
@@ -139,13 +226,13 @@ struct SorBwd {
   double *y;
   double omega;
   int use_t;
-  __device__ double init(i64 i) const { return src[i]; }
+  __device__ Staged stage(i64 i) const { return Staged{src[i], idiag[i], use_t ? yf[i] : 0.0, 0}; }
   __device__ int begin(i64 i) const { return dmark[i] + 1; }
   __device__ int end(i64 i) const { return rp[i + 1]; }
   __device__ i64 col(int j) const { return ci[j]; }
   __device__ double coef(int j) const { return -a[j]; }
-  __device__ double finish(i64 i, double acc) const {
-    double r = use_t ? fma(1.0 - omega, yf[i], acc * idiag[i]) : acc * idiag[i];
+  __device__ double fin(i64 i, double acc, const Staged &st) const {
+    double r = use_t ? fma(1.0 - omega, st.s2, acc * st.s1) : acc * st.s1;
     y[i] = r;
     return r;
   }
@@ -165,38 +252,49 @@ __global__ void k_sor_idiag(i64 n, const double *a, const int *dmark, double *id
 
 // ------------------------------------------------------------ applies
 
-static double *sweep_scratch(i64 n) { return scratch_vector((size_t)3 * (size_t)n); }
+// Three published-value buffers of n, then (planned sweeps) the staged rows.
+static double *sweep_scratch(i64 n, bool planned) {
+  return scratch_vector((size_t)(planned ? 7 : 3) * (size_t)n);
+}
+static StagedRows staged_rows(double *S, i64 n) {
+  return StagedRows{S + 3 * n, S + 4 * n, S + 5 * n, reinterpret_cast<i64 *>(S + 6 * n)};
+}
 
 void ilu_apply(const IluFactor &f, const double *b, double *x, const int64_t *stop) {
   i64 n = f.n;
-  double *S = sweep_scratch(n);
+  double *S = sweep_scratch(n, f.plan != nullptr);
   if (!S || n <= 0) return;
   double *F = S, *B = S + n;
-  run_sweep(IluFwd{f.bi, f.bj, f.r, f.ba, b}, n, true, F, stop);
-  run_sweep(IluBwd{f.bj, f.bdiag, f.c, f.ba, F, x}, n, false, B, stop);
+  const StagedRows st = staged_rows(S, n);
+  run_sweep(IluFwd{f.bi, f.bj, f.r, f.ba, b}, n, true, F, stop, f.plan, 0, st);
+  run_sweep(IluBwd{f.bj, f.bdiag, f.c, f.ba, F, x}, n, false, B, stop, f.plan, 1, st);
 }
 
 void icc_apply(const IccFactor &f, const double *b, double *x, const int64_t *stop) {
   i64 n = f.n;
-  double *S = sweep_scratch(n);
+  double *S = sweep_scratch(n, f.plan != nullptr);
   if (!S || n <= 0) return;
   double *Y = S, *F = S + n, *B = S + 2 * n;
-  run_sweep(IccFwd{f.trp, f.trow, f.tpos, f.tmid, f.udiag, f.c, f.ua, b, F}, n, true, Y, stop);
+  const StagedRows st = staged_rows(S, n);
+  run_sweep(IccFwd{f.trp, f.trow, f.tpos, f.tmid, f.udiag, f.c, f.ua, b, F}, n, true, Y, stop, f.plan, 0, st);
   if (f.nlate) MPK_LAUNCH(k_icc_late, nblocks(n, 256), 256, n, f.trp, f.trow, f.tpos, f.tmid, f.ua, Y, F, stop);
-  run_sweep(IccBwd{f.ui, f.uj, f.udiag, f.c, f.ua, F, x}, n, false, B, stop);
+  run_sweep(IccBwd{f.ui, f.uj, f.udiag, f.c, f.ua, F, x, n - 1}, n, false, B, stop, f.plan, 1, st);
 }
 
 void sor_apply(const SorData &s, const double *x, double *y, const int64_t *stop) {
   i64 n = s.n;
-  double *S = sweep_scratch(n);
+  double *S = sweep_scratch(n, s.plan != nullptr);
   if (!S || n <= 0) return;
+  const StagedRows st = staged_rows(S, n);
   MPK_LAUNCH(k_sor_idiag, nblocks(n, 256), 256, n, s.a, s.dmark, s.idiag, s.mdiag, s.omega, s.fshift, stop);
   const bool fwd = (s.flag & (1 | 4)) != 0, bwd = (s.flag & (2 | 8)) != 0;
   double *YF = S, *YB = S + n;
-  if (fwd) run_sweep(SorFwd{s.rp, s.ci, s.dmark, s.a, x, s.idiag, s.t, bwd ? nullptr : y, y}, n, true, YF, stop);
+  if (fwd)
+    run_sweep(SorFwd{s.rp, s.ci, s.dmark, s.a, x, s.idiag, s.t, bwd ? nullptr : y, y}, n, true, YF, stop,
+              s.plan, 0, st);
   if (bwd)
     run_sweep(SorBwd{s.rp, s.ci, s.dmark, s.a, fwd ? s.t : x, s.idiag, YF, y, s.omega, fwd ? 1 : 0}, n,
-              false, YB, stop);
+              false, YB, stop, s.plan, 1, st);
   // its * lits > 1 never reaches here: the apply is its = lits = 1 (capi.mojo:272-280)
 }
 
@@ -217,6 +315,59 @@ void PCView::apply(const double *x, double *y, const int64_t *stop) const {
     default:
       vcopy(x, y, n);
   }
+}
+
+// The two sweeps of a PC as row programs (vectors unset: the plan only needs
+// their entries).
+template <class F>
+static bool with_sweeps(const PCView &pc, F &&f) {
+  switch (pc.type) {
+    case PCT_ILU:
+      f(IluFwd{pc.ilu.bi, pc.ilu.bj, pc.ilu.r, pc.ilu.ba, nullptr}, true);
+      f(IluBwd{pc.ilu.bj, pc.ilu.bdiag, pc.ilu.c, pc.ilu.ba, nullptr, nullptr}, false);
+      return true;
+    case PCT_ICC:
+      f(IccFwd{pc.icc.trp, pc.icc.trow, pc.icc.tpos, pc.icc.tmid, pc.icc.udiag, pc.icc.c, pc.icc.ua, nullptr,
+               nullptr},
+        true);
+      f(IccBwd{pc.icc.ui, pc.icc.uj, pc.icc.udiag, pc.icc.c, pc.icc.ua, nullptr, nullptr, pc.n - 1}, false);
+      return true;
+    case PCT_SOR:
+      f(SorFwd{pc.sor.rp, pc.sor.ci, pc.sor.dmark, pc.sor.a, nullptr, nullptr, nullptr, nullptr, nullptr}, true);
+      f(SorBwd{pc.sor.rp, pc.sor.ci, pc.sor.dmark, pc.sor.a, nullptr, nullptr, nullptr, nullptr, 1.0, 0}, false);
+      return true;
+    default:
+      return false;
+  }
+}
+
+i64 PCView::schedule_size() const {
+  i64 total = 0;
+  bool bad = false;
+  with_sweeps(*this, [&](auto prog, bool) {
+    const i64 m = sweep_entries(prog, n);
+    if (m < 0) bad = true;
+    total += mpk_sweep_plan_bound(n, m, MPK_BLK_ROWS);
+  });
+  return bad ? -1 : total;
+}
+
+i64 PCView::schedule(char *plan, i64 cap) const {
+  i64 used = 0, depth = 0;
+  bool bad = false;
+  const bool any = with_sweeps(*this, [&](auto prog, bool forward) {
+    if (bad) return;
+    i64 d = 0;
+    const i64 bytes = plan_sweep(prog, n, forward, plan + used, cap - used, &d);
+    if (bytes < 0) {
+      bad = true;
+      return;
+    }
+    used += bytes;
+    depth = d > depth ? d : depth;
+  });
+  if (!any) return 0;
+  return bad ? -1 : depth;
 }
 
 // PCApplySymmetric (ksp.mojo:201-208).  The Jacobi square root is read from
@@ -291,11 +442,12 @@ bool PCView::parse(const int64_t *pci, const double *pcf, i64 n_) {
       return true;
     case PCT_SOR:
       sor = SorData{n, (const int *)pci[1], (const int *)pci[2], (const double *)pci[3], (const int *)pci[4],
-                    (double *)pci[5], (double *)pci[6], (double *)pci[7], pcf[0], pcf[1], (int)pci[10]};
+                    (double *)pci[5], (double *)pci[6], (double *)pci[7], pcf[0], pcf[1], (int)pci[10],
+                    (const char *)pci[11]};
       return true;
     case PCT_ILU:
       ilu = IluFactor{n, (const int *)pci[1], (const int *)pci[2], (const double *)pci[3], (const int *)pci[4],
-                      (const int *)pci[5], (const int *)pci[6]};
+                      (const int *)pci[5], (const int *)pci[6], (const char *)pci[11]};
       return true;
     case PCT_ICC: {
       icc.n = n;
@@ -311,6 +463,7 @@ bool PCView::parse(const int64_t *pci, const double *pcf, i64 n_) {
       icc.tpos = (const int *)pci[8];
       icc.tmid = (const int *)pci[9];
       icc.nlate = pci[10];
+      icc.plan = (const char *)pci[11];
       if (!icc.trp || !icc.trow || !icc.tpos || !icc.tmid) {
         set_error(4, "IC descriptor without the transpose of the factor (aux5..aux8)");
         return false;
@@ -472,6 +625,30 @@ int64_t mpk_icc_transpose(int64_t ui, int64_t uj, int64_t udiag, int64_t nrow, i
                        (int *)tpos, (int *)tmid, &nlate)
              ? nlate
              : -1;
+}
+
+// Block plans of the descriptor's two sweeps (see include/mpk.h).
+int64_t mpk_pc_schedule_size(int64_t n, int64_t pci, int64_t pcf) {
+  if (n <= 0 || !ctx()) return 0;
+  PCView pc;
+  if (!pc.parse((const int64_t *)pci, (const double *)pcf, n)) return -1;
+  return pc.schedule_size();
+}
+
+int64_t mpk_pc_schedule(int64_t n, int64_t pci, int64_t pcf, int64_t plan, int64_t cap) {
+  if (n <= 0 || !ctx()) return 0;
+  PCView pc;
+  if (!pc.parse((const int64_t *)pci, (const double *)pcf, n)) return -1;
+  return pc.schedule((char *)plan, cap);
+}
+
+// y = M^-1 x for the preconditioner a descriptor describes (the apply the
+// solvers use, including its level orders).
+void mpk_pc_apply(int64_t n, int64_t pci, int64_t pcf, int64_t x, int64_t y) {
+  if (n <= 0 || !ctx()) return;
+  PCView pc;
+  if (!pc.parse((const int64_t *)pci, (const double *)pcf, n)) return;
+  pc.apply((const double *)x, (double *)y, nullptr);
 }
 
 }  // extern "C"

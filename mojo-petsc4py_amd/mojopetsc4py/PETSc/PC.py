@@ -129,6 +129,7 @@ class PC:
         self._nlate = 0          # IC entries below the diagonal (see mpk_icc_transpose)
         self._dbufs = {}         # device buffers of the set up
         self._pci_cache = None
+        self._sweep_depth = 0    # dependency levels of the deeper sweep
 
     # ---------------------------------------------------------- lifecycle
     def create(self, comm=None) -> "PC":
@@ -282,7 +283,30 @@ class PC:
         check()
         self._setupcalled = True
         self._setup_version = pmat._version
+        self._schedule(n)
         return self
+
+    def _schedule(self, n: int):
+        """Block plans for the two sweeps of SOR / ILU / IC (see
+        mpk_pc_schedule): workgroups run blocks of rows level by level in LDS.
+        The descriptor carries them in pci[11]."""
+        if self._type not in (PCType.SOR, PCType.ILU, PCType.ICC) or n == 0:
+            return
+        if self._type != PCType.SOR and self._factor is None:
+            return
+        L = lib()
+        pci = self._pci_block()
+        pcf = self._pcf_block()
+        size = L.mpk_pc_schedule_size(n, pci.ctypes.data, pcf.ctypes.data)
+        check()
+        plan = DeviceBuffer(max(int(size), 8))
+        depth = L.mpk_pc_schedule(n, pci.ctypes.data, pcf.ctypes.data, plan.ptr, int(size))
+        check()
+        if depth < 0:
+            raise RuntimeError("planning the preconditioner sweeps failed")
+        self._dbufs["sched"] = plan
+        self._sweep_depth = int(depth)
+        self._pci_cache = None
 
     def _buf(self, name, nbytes):
         b = DeviceBuffer(nbytes)
@@ -446,6 +470,8 @@ class PC:
             pci[0] = _ICC
             pci[1:11] = [b["bi"].ptr, b["bj"].ptr, b["ba"].ptr, b["bdiag"].ptr, b["r"].ptr, b["trp"].ptr,
                          b["trow"].ptr, b["tpos"].ptr, b["tmid"].ptr, self._nlate]
+        if "sched" in b:
+            pci[11] = b["sched"].ptr
         self._pci_cache = np.array(pci, dtype=np.int64)
         return self._pci_cache
 
@@ -462,18 +488,9 @@ class PC:
             L.mp_vec_copy(xa, y._dw(), n)
         elif self._type == PCType.JACOBI:
             L.mp_pc_jacobi_apply(y._dw(), xa, b["diag"].ptr, n)
-        elif self._type == PCType.SOR:
-            d = self._pmat._dev()
-            L.mp_pc_sor_apply(xa, y._dw(), d.rp.ptr, d.ci.ptr, d.v.ptr, b["dmark"].ptr, b["idiag"].ptr,
-                              b["mdiag"].ptr, b["sorwork"].ptr, self._sor_omega, self._sor_fshift,
-                              self._sor_flag(), n)
-        elif self._type == PCType.ILU:
-            L.mp_pc_ilu_apply(xa, y._dw(), 0, b["bi"].ptr, b["bj"].ptr, b["ba"].ptr, b["bdiag"].ptr,
-                              b["r"].ptr, b["c"].ptr, n)
         else:
-            L.mpk_pc_icc_apply_t(xa, y._dw(), b["bi"].ptr, b["bj"].ptr, b["ba"].ptr, b["bdiag"].ptr,
-                                 b["r"].ptr, b["trp"].ptr, b["trow"].ptr, b["tpos"].ptr, b["tmid"].ptr,
-                                 self._nlate, n)
+            # SOR / ILU / IC through the descriptor, with its level orders
+            L.mpk_pc_apply(n, self._pci_block().ctypes.data, self._pcf_block().ctypes.data, xa, y._dw())
         check()
 
     def apply(self, x: Vec, y: Vec) -> None:

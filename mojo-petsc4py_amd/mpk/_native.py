@@ -102,6 +102,11 @@ SIGNATURES = {
     "mpk_sell_size": ([I, I], I),
     "mpk_sell_build": ([I] * 7, I),
     "mpk_sell_mult": ([I] * 8, None),
+    "mpk_pc_schedule_size": ([I] * 3, I),
+    "mpk_pc_schedule": ([I] * 5, I),
+    "mpk_sweep_plan_bound": ([I] * 3, I),
+    "mpk_sweep_plan": ([I] * 8, I),
+    "mpk_pc_apply": ([I] * 5, None),
     "mpk_icc_transpose_size": ([I, I, I], I),
     "mpk_icc_transpose": ([I] * 8, I),
 }

@@ -173,7 +173,7 @@ def _upload_factor(f, kind, n):
 @pytest.mark.parametrize("kind,levels,ordering", [("ilu", 0, "natural"), ("ilu", 1, "natural"), ("ilu", 2, "rcm"),
                                                   ("icc", 0, "natural"), ("icc", 1, "natural"),
                                                   ("icc", 2, "natural"), ("icc", 3, "natural"),
-                                                  ("icc", 0, "rcm")])
+                                                  ("icc", 0, "rcm"), ("icc", 1, "rcm"), ("icc", 2, "rcm")])
 def test_factor_apply_bitwise(native, O, name, kind, levels, ordering):
     ip, ix, va = MATS[name]()
     if kind == "icc" and name in ("cd10", "rand500"):

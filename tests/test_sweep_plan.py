@@ -1,0 +1,219 @@
+"""The host half of the block-planned sweeps (mpk_sweep_plan; no GPU needed):
+the plan matches a restatement computed here, keeps every row's entries in
+the row's own order, and every value a row reads is either computed earlier
+in its own block (a lower local level) or belongs to an earlier block -- the
+property the device sweep's progress rests on.  A planted mistake in a
+restated plan (a row reading a later level of its block) is caught by the
+same check."""
+
+import numpy as np
+import pytest
+
+from mpk import workloads as W
+
+
+@pytest.fixture(scope="module")
+def L():
+    from mpk import _native
+    return _native.load(init=False)
+
+
+def _entries(ip, ix, va):
+    return np.asarray(ip, np.int64), np.asarray(ix, np.int32), np.asarray(va, np.float64)
+
+
+def _plan(L, n, forward, off, col, coef, bmax):
+    m = int(off[n])
+    cap = L.mpk_sweep_plan_bound(n, m, bmax)
+    buf = np.zeros((cap + 7) // 8, np.int64)
+    nb = L.mpk_sweep_plan(n, 1 if forward else 0, off.ctypes.data, col.ctypes.data, coef.ctypes.data, bmax,
+                          buf.ctypes.data, cap)
+    assert 0 < nb <= cap
+    return _unpack(buf.view(np.uint8)[:nb], n)
+
+
+def _unpack(raw, n):
+    """The plan's arrays, with the entries in CSR form whichever way they are
+    stored (fixed slots or CSR)."""
+    h = raw[:64].view(np.int64)
+    nblk, nent, nlvl, depth, nbytes, w = (int(x) for x in h[:6])
+    ints = raw[64:].view(np.int32)
+    o = 0
+
+    def take(k):
+        nonlocal o
+        a = ints[o:o + k].astype(np.int64)
+        o += k
+        return a
+    blk_pos, blk_lev, lvl = take(nblk + 1), take(nblk + 1), take(nlvl)
+    prow, pcnt, pent = take(n), take(n), take(n + 1)
+    slots = n * w if w else nent
+    base = 64 + ((4 * (2 * (nblk + 1) + nlvl + 3 * n + 1) + 7) // 8) * 8
+    ecoef = raw[base:base + 8 * slots].view(np.float64)
+    edep = raw[base + 8 * slots:base + 12 * slots].view(np.int32).astype(np.int64)
+    assert nbytes == len(raw)
+    if w:
+        assert pcnt.max(initial=0) <= w
+        take_idx = np.concatenate([p * w + np.arange(c) for p, c in enumerate(pcnt)] or [np.zeros(0, np.int64)])
+        ecoef, edep = ecoef[take_idx.astype(np.int64)], edep[take_idx.astype(np.int64)]
+        pent = np.concatenate([[0], np.cumsum(pcnt)])
+    else:
+        assert np.array_equal(np.diff(pent), pcnt)
+    return dict(nblk=nblk, depth=depth, w=w, blk_pos=blk_pos, blk_lev=blk_lev, lvl=lvl, prow=prow, pent=pent,
+                ecoef=ecoef, edep=edep)
+
+
+def _restated(n, forward, off, col, coef, bmax):
+    B = n if n <= bmax else min(bmax, max(1024, -(-n // 256)))
+    nblk = -(-n // B)
+    spos = (lambda i: i) if forward else (lambda i: n - 1 - i)
+    earlier = (lambda k, i: k < i) if forward else (lambda k, i: k > i)
+    lev = np.zeros(n, np.int64)
+    for s in range(n):
+        i = spos(s)
+        for e in range(off[i], off[i + 1]):
+            k = col[e]
+            if earlier(k, i) and spos(k) // B == s // B:
+                lev[i] = max(lev[i], lev[k] + 1)
+    prow, pos, blk_pos, blk_lev, lvl = [], np.zeros(n, np.int64), [], [], []
+    for b in range(nblk):
+        rows = [spos(s) for s in range(b * B, min(n, (b + 1) * B))]
+        rows.sort(key=lambda i: lev[i])  # stable: sweep order inside a level
+        blk_pos.append(b * B)
+        blk_lev.append(len(lvl))
+        nl = int(max(lev[r] for r in rows)) + 1
+        starts = [b * B + sum(1 for r in rows if lev[r] < lv) for lv in range(nl)]
+        lvl += starts + [b * B + len(rows)]
+        for r in rows:
+            pos[r] = len(prow)
+            prow.append(r)
+    blk_pos.append(n)
+    blk_lev.append(len(lvl))
+    pent, ecoef, edep = [0], [], []
+    for p, i in enumerate(prow):
+        b = spos(i) // B
+        for e in range(off[i], off[i + 1]):
+            k = col[e]
+            ecoef.append(coef[e])
+            if not earlier(k, i):
+                edep.append(-k - n - 1)
+            elif spos(k) // B == b:
+                edep.append(pos[k] - blk_pos[b])
+            else:
+                edep.append(-k - 1)
+        pent.append(len(edep))
+    w = 4 if max((off[i + 1] - off[i] for i in range(n)), default=0) <= 4 else 0
+    return dict(nblk=nblk, w=w, blk_pos=np.array(blk_pos), blk_lev=np.array(blk_lev), lvl=np.array(lvl),
+                prow=np.array(prow), pent=np.array(pent), ecoef=np.array(ecoef), edep=np.array(edep))
+
+
+def _check(plan, n, forward, off, col, coef):
+    """Every row's entries in its own order, and every read resolvable when
+    the row's level runs: local reads from lower levels of the same block,
+    external reads from earlier blocks."""
+    spos = (lambda i: i) if forward else (lambda i: n - 1 - i)
+    prow, pent, edep, ecoef = plan["prow"], plan["pent"], plan["edep"], plan["ecoef"]
+    assert sorted(prow.tolist()) == list(range(n))
+    pos = np.empty(n, np.int64)
+    pos[prow] = np.arange(n)
+    for b in range(plan["nblk"]):
+        p0, p1 = plan["blk_pos"][b], plan["blk_pos"][b + 1]
+        assert sorted(spos(i) for i in prow[p0:p1]) == list(range(p0, p1))
+        starts = plan["lvl"][plan["blk_lev"][b]:plan["blk_lev"][b + 1]]
+        assert starts[0] == p0 and starts[-1] == p1 and np.all(np.diff(starts) > 0)
+        for li in range(len(starts) - 1):
+            for p in range(starts[li], starts[li + 1]):
+                i = prow[p]
+                assert pent[p + 1] - pent[p] == off[i + 1] - off[i]
+                for e, src in zip(range(pent[p], pent[p + 1]), range(off[i], off[i + 1])):
+                    k, d = col[src], edep[e]
+                    assert ecoef[e] == coef[src]
+                    if d >= 0:
+                        if not (p0 + d < starts[li] and prow[p0 + d] == k):
+                            return False
+                    elif d >= -n:
+                        if not (-d - 1 == k and spos(k) < p0):
+                            return False
+                    else:
+                        assert -d - n - 1 == k
+    return True
+
+
+def _random(n, per_row, seed):
+    rng = np.random.default_rng(seed)
+    rows, cols = [], []
+    for i in range(n):
+        for c in set(rng.integers(0, n, per_row).tolist()) | {i}:
+            rows.append(i)
+            cols.append(c)
+    return W._csr_from_coo(n, np.array(rows), np.array(cols), rng.normal(size=len(rows)))
+
+
+def _lower(ip, ix, va, forward):
+    """The part of each row a sweep reads: strictly earlier columns, plus a
+    stray later column in a few rows to exercise the stale path."""
+    n = len(ip) - 1
+    rows, cols, vals = [], [], []
+    for i in range(n):
+        for j in range(ip[i], ip[i + 1]):
+            k = int(ix[j])
+            if (k < i) if forward else (k > i):
+                rows.append(i)
+                cols.append(k)
+                vals.append(va[j])
+        if i % 97 == 5 and 0 < i < n - 1:
+            rows.append(i)
+            cols.append(i + 1 if forward else i - 1)
+            vals.append(0.5)
+    return _entries(*W._csr_from_coo(n, np.array(rows, np.int64), np.array(cols, np.int64), np.array(vals)))
+
+
+SMALL = {
+    "lap20": lambda: W.laplacian_2d(20),
+    "diff7": lambda: W.diffusion_3d(7)[0],
+    "cd6": lambda: W.convection_diffusion_3d(6)[0],
+    "tri40": lambda: W.tridiagonal(40),
+    "rand300": lambda: _random(300, 5, 7),
+    "rand200w": lambda: _random(200, 14, 9),
+}
+
+
+@pytest.mark.parametrize("name", sorted(SMALL))
+@pytest.mark.parametrize("forward", [True, False])
+@pytest.mark.parametrize("bmax", [8192, 64])
+def test_matches_restatement(L, name, forward, bmax):
+    off, col, coef = _lower(*SMALL[name](), forward)
+    n = len(off) - 1
+    got = _plan(L, n, forward, off, col, coef, bmax)
+    want = _restated(n, forward, off, col, coef, bmax)
+    for k in ("blk_pos", "blk_lev", "lvl", "prow", "pent", "edep"):
+        assert np.array_equal(got[k], want[k]), k
+    assert np.array_equal(got["ecoef"].view(np.int64), np.asarray(want["ecoef"]).view(np.int64))
+    assert got["nblk"] == want["nblk"] and got["w"] == want["w"]
+    assert _check(got, n, forward, off, col, coef)
+
+
+@pytest.mark.parametrize("forward", [True, False])
+def test_grid_operator_blocks(L, forward):
+    """A 3-D operator in several blocks: 20^3 rows in 8 blocks of 1024 (the
+    last shorter), every read resolvable."""
+    off, col, coef = _lower(*W.diffusion_3d(20)[0], forward)
+    n = len(off) - 1
+    plan = _plan(L, n, forward, off, col, coef, 1024)
+    assert plan["nblk"] == 8
+    assert _check(plan, n, forward, off, col, coef)
+
+
+def test_check_catches_a_bad_plan(L):
+    """Plant the mistake in the plan on the CPU: swap two levels of a block so
+    rows read values of a level that has not run yet."""
+    off, col, coef = _lower(*W.laplacian_2d(12), True)
+    n = len(off) - 1
+    plan = _plan(L, n, True, off, col, coef, 8192)
+    assert _check(plan, n, True, off, col, coef)
+    bad = dict(plan)
+    lvl = plan["lvl"].copy()
+    # move the boundary between levels 1 and 2 so a level-2 row runs with level 1
+    lvl[2] += 1
+    bad["lvl"] = lvl
+    assert not _check(bad, n, True, off, col, coef)
