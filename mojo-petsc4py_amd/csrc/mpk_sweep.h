@@ -200,6 +200,56 @@ __device__ __forceinline__ double plan_value(const Prog &prog, i64 n, const doub
   return prog.stale(-(i64)d - n - 1);
 }
 
+// A position's inputs, loaded one level ahead of their use.
+struct PlanRow {
+  int p, cnt, base;
+  i64 i;
+  Staged s;
+  double c[MPK_PLAN_W];
+  int d[MPK_PLAN_W];
+};
+
+__device__ __forceinline__ PlanRow plan_row(const PlanView &P, const StagedRows &st, int p, bool valid) {
+  PlanRow r;
+  r.p = valid ? p : -1;
+  if (!valid) return r;
+  r.cnt = P.pcnt[p];
+  r.i = P.prow[p];
+  r.s = Staged{st.init[p], st.s1[p], st.s2[p], st.o[p]};
+  if (P.w == MPK_PLAN_W) {
+    r.base = p * MPK_PLAN_W;
+#pragma unroll
+    for (int e = 0; e < MPK_PLAN_W; e++) {
+      r.c[e] = P.ecoef[(i64)r.base + e];
+      r.d[e] = P.edep[(i64)r.base + e];
+    }
+  } else {
+    r.base = P.pent[p];
+  }
+  return r;
+}
+
+template <class Prog>
+__device__ __forceinline__ void plan_compute(const Prog &prog, i64 n, const PlanView &P, const PlanRow &r,
+                                             double *val, int p0, double *pub) {
+  double acc = r.s.init;
+  if (P.w == MPK_PLAN_W) {
+#pragma unroll
+    for (int e = 0; e < MPK_PLAN_W; e++)
+      if (e < r.cnt) acc = fma(r.c[e], plan_value(prog, n, val, pub, r.d[e]), acc);
+  } else {
+    for (int e = r.base, ee = r.base + r.cnt; e < ee; e++)
+      acc = fma(P.ecoef[e], plan_value(prog, n, val, pub, P.edep[e]), acc);
+  }
+  const double v = prog.fin(r.i, acc, r.s);
+  val[r.p - p0] = v;
+  store_pub(pub, r.i, v);
+}
+
+// One workgroup per block.  Each thread's first row of the next level is
+// loaded while the current level computes (the barrier waits on LDS only, so
+// those loads stay in flight across it); further rows of a wide level are
+// loaded when they run.
 template <class Prog>
 __global__ void __launch_bounds__(256) k_sweep_blk(Prog prog, i64 n, const char *plan, int second, StagedRows st,
                                                    unsigned *ticket, double *pub, const int64_t *stop) {
@@ -212,35 +262,21 @@ __global__ void __launch_bounds__(256) k_sweep_blk(Prog prog, i64 n, const char 
   const PlanView P = plan_view(plan, n);
   const i64 b = tk_sh;
   if (b >= P.nblk) return;
+  const int t = (int)threadIdx.x;
   const int p0 = P.blk_pos[b];
-  const int l0 = P.blk_lev[b], l1 = P.blk_lev[b + 1] - 1;
+  const int l0 = P.blk_lev[b], l1 = P.blk_lev[b + 1] - 1;  // levels l0..l1-1; lvl[l1] ends the block
+  int pa = P.lvl[l0], pb = P.lvl[l0 + 1];
+  PlanRow cur = plan_row(P, st, pa + t, pa + t < pb);
   for (int l = l0; l < l1; l++) {
-    const int pa = P.lvl[l], pb = P.lvl[l + 1];
-    for (int p = pa + (int)threadIdx.x; p < pb; p += 256) {
-      const int cnt = P.pcnt[p];
-      const Staged s{st.init[p], st.s1[p], st.s2[p], st.o[p]};
-      double acc = s.init;
-      if (P.w == MPK_PLAN_W) {
-        double c[MPK_PLAN_W];
-        int d[MPK_PLAN_W];
-#pragma unroll
-        for (int e = 0; e < MPK_PLAN_W; e++) {
-          c[e] = P.ecoef[(i64)p * MPK_PLAN_W + e];
-          d[e] = P.edep[(i64)p * MPK_PLAN_W + e];
-        }
-#pragma unroll
-        for (int e = 0; e < MPK_PLAN_W; e++)
-          if (e < cnt) acc = fma(c[e], plan_value(prog, n, val, pub, d[e]), acc);
-      } else {
-        for (int e = P.pent[p], ee = e + cnt; e < ee; e++)
-          acc = fma(P.ecoef[e], plan_value(prog, n, val, pub, P.edep[e]), acc);
-      }
-      const i64 i = P.prow[p];
-      const double r = prog.fin(i, acc, s);
-      val[p - p0] = r;
-      store_pub(pub, i, r);
-    }
+    const bool more = l + 1 < l1;
+    const int pc = more ? P.lvl[l + 2] : pb;
+    const PlanRow nxt = plan_row(P, st, pb + t, more && pb + t < pc);
+    if (cur.p >= 0) plan_compute(prog, n, P, cur, val, p0, pub);
+    for (int p = pa + t + 256; p < pb; p += 256) plan_compute(prog, n, P, plan_row(P, st, p, true), val, p0, pub);
     __syncthreads();
+    cur = nxt;
+    pa = pb;
+    pb = pc;
   }
 }
 
