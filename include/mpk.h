@@ -206,6 +206,55 @@ void mp_ksp_solve_fgmres(int64_t n, int64_t irow, int64_t icol, int64_t data, in
                          int64_t pstmp, int64_t work, int64_t vv, int64_t prevec, int64_t hs, int64_t cfg,
                          int64_t cfgi, int64_t b, int64_t x);
 
+/* ------------------------------------------------------------------ mixed precision
+ * No counterpart in the reference: solves whose inner Krylov iteration, SpMV
+ * and preconditioner run on FP32 vectors -- matrix and preconditioner values
+ * in FP32 (prec 32) or in FP16 with one FP32 scale per row (prec 16), FP32
+ * accumulation -- inside FP64 iterative refinement; dot products and norms
+ * sum in FP64 and the GMRES least squares is FP64.
+ * mpk_mixed_create: the FP64 matrix [D] (irow, icol, data) with its sliced
+ * copy (soff, sv, sc from mpk_sell_build) and a PC descriptor (pci [H], pcf
+ * [H]; ilu or sor with plans from mpk_pc_schedule, or none).  inner: 0 gmres
+ * (left preconditioned, classical Gram-Schmidt, restart <= 64), 1 bcgs
+ * (right preconditioned).  tri_iters 0: the preconditioner's sweeps run in
+ * dependency order over its block plans, exactly; k > 0: each sweep is
+ * approximated by k Jacobi passes over its entries (a truncated Neumann
+ * series; bandwidth-bound, no ordering).  Returns a handle, 0 on error.  The caller keeps
+ * the matrix, its sliced copy and the PC alive while the handle lives.
+ * mpk_mixed_solve: x [D] from b [D]; cfg [H] float64 in {rtol, inner_rtol,
+ * max_outer, max_inner, gmres refinement (0 never, 1 if needed, 2 always),
+ * x_is_zero}, out cfg[8] final ||b - A x|| / ||b|| (FP64), cfg[9] outer
+ * steps, cfg[10] inner steps, cfg[11] ||b||, cfg[12] 1 if converged.
+ * Returns 0, or -1 on error. */
+int64_t mpk_mixed_create(int64_t n, int64_t irow, int64_t icol, int64_t data, int64_t soff, int64_t sv, int64_t sc,
+                         int64_t pci, int64_t pcf, int64_t prec, int64_t inner, int64_t restart, int64_t tri_iters);
+int64_t mpk_mixed_solve(int64_t handle, int64_t b, int64_t x, int64_t cfg);
+void mpk_mixed_free(int64_t handle);
+
+/* ------------------------------------------------------------------ batched solves
+ * No counterpart in the reference: many independent small systems solved by
+ * one persistent launch, a workgroup per system at a time (largest first):
+ * FP64 iterative refinement around an FP32 inner CG + Jacobi (kind 0) or
+ * GMRES(20) + ILU(0) (kind 1, modified Gram-Schmidt), FP64 dots, scalars and
+ * least squares.  All arrays [D], concatenated over the systems (N rows in
+ * all); sys int64[nsys][8] = {row0, n, kind, flv offset, forward levels, blv
+ * offset, backward levels, 0}; order int32[nsys]; rp int32[N+1] and ci int32
+ * (global columns), v64 float64 / v32 float32 values; jd32 float32[N] Jacobi
+ * inverse diagonal; the ILU(0) sweeps as single-block plans (see
+ * mpk_sweep_plan): prow, cnt int32[N], dep int32[4N] (position slots), coef
+ * float32[4N], level starts flv / blv int32, dinv32 float32[N] (inverted
+ * diagonal).  nmax: the largest n.  The caller keeps the arrays alive.
+ * mpk_batch_solve: x [D] from b [D], stats [D] float64[nsys][4] = {relres,
+ * outer, inner, converged}; cfg [H] float64 {rtol, inner_rtol, max_outer,
+ * max_inner, workgroup (128/256/512), grid, use_graph}: with use_graph the
+ * launch is captured once into a HIP graph and replayed.  0 or -1. */
+int64_t mpk_batch_create(int64_t nsys, int64_t sys, int64_t order, int64_t rp, int64_t ci, int64_t v64, int64_t v32,
+                         int64_t jd32, int64_t fprow, int64_t fcnt, int64_t fdep, int64_t fcoef, int64_t flv,
+                         int64_t bprow, int64_t bcnt, int64_t bdep, int64_t bcoef, int64_t blv, int64_t dinv32,
+                         int64_t nmax);
+int64_t mpk_batch_solve(int64_t handle, int64_t b, int64_t x, int64_t stats, int64_t cfg);
+void mpk_batch_free(int64_t handle);
+
 #ifdef __cplusplus
 }
 #endif

@@ -28,6 +28,8 @@ SOURCES = [
     "mpk_mat.hip",
     "mpk_pc.hip",
     "mpk_ksp.hip",
+    "mpk_mixed.hip",
+    "mpk_batch.hip",
     "mpk_factor.cpp",
 ]
 

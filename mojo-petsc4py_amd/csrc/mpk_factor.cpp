@@ -12,6 +12,7 @@
 // reference's Python layer allocates them (PC.py:376-492).
 #include <cmath>
 #include <cstdint>
+#include <cstdlib>
 #include <cstring>
 #include <algorithm>
 #include <vector>
@@ -414,14 +415,25 @@ void mp_pc_icc_numeric(int64_t ua_, int64_t rtmp_, int64_t il_, int64_t c2r_, in
 // positions when n > bmax, sized so that there are about 256 of them (but not
 // under 1024 rows); a smaller sweep is one block.
 static int64_t plan_rows(int64_t n, int64_t bmax) {
+  // tuning override: MPK_SWEEP_BLOCK_ROWS=k fixes the block size (capped by bmax)
+  if (const char *e = std::getenv("MPK_SWEEP_BLOCK_ROWS")) {
+    const int64_t k = std::atoll(e);
+    if (k > 0) return std::min(k, std::max<int64_t>(n, 1) < bmax ? std::max<int64_t>(n, 1) : bmax);
+  }
   if (n <= bmax) return n > 0 ? n : 1;
   int64_t b = (n + 255) / 256;
   b = b < 1024 ? 1024 : b;
   return b > bmax ? bmax : b;
 }
-// Rows with at most this many entries are stored in fixed slots
-// (MPK_PLAN_W in mpk_sweep.h).
-static const int64_t kPlanW = 4;
+// Entries sit in w fixed slots per position, w the power of two from 4 to
+// this (MPK_PLAN_WMAX in mpk_sweep.h) that holds the longest row; sweeps with
+// longer rows keep CSR form (w = 0).
+static const int64_t kPlanWMax = 32;
+static int64_t plan_w(int64_t widest) {
+  int64_t w = 4;
+  while (w < widest) w *= 2;
+  return w <= kPlanWMax ? w : 0;
+}
 static int64_t plan_bytes(int64_t n, int64_t nblk, int64_t nlvl, int64_t slots) {
   const int64_t ints = 2 * (nblk + 1) + nlvl + 3 * n + 1;
   return 64 + ((4 * ints + 7) & ~(int64_t)7) + ((12 * slots + 7) & ~(int64_t)7);
@@ -430,7 +442,7 @@ static int64_t plan_bytes(int64_t n, int64_t nblk, int64_t nlvl, int64_t slots) 
 // Upper bound of the plan size in bytes for n rows and m entries.
 int64_t mpk_sweep_plan_bound(int64_t n, int64_t m, int64_t bmax) {
   const int64_t B = plan_rows(n, bmax), nblk = n > 0 ? (n + B - 1) / B : 0;
-  return plan_bytes(n, nblk, n + nblk, std::max(m, kPlanW * n));
+  return plan_bytes(n, nblk, n + nblk, std::max(m, kPlanWMax * n));
 }
 
 // Writes the plan into out [H] (cap bytes).  Returns its size in bytes, or
@@ -464,7 +476,7 @@ int64_t mpk_sweep_plan(int64_t n, int64_t forward, int64_t off_, int64_t col_, i
     depth = std::max<int64_t>(depth, nlev[b]);
   }
   for (int64_t i = 0; i < n; i++) widest = std::max(widest, off[i + 1] - off[i]);
-  const int64_t w = widest <= kPlanW ? kPlanW : 0, slots = w ? w * n : m;
+  const int64_t w = plan_w(widest), slots = w ? w * n : m;
   const int64_t bytes = plan_bytes(n, nblk, nlvl, slots);
   if (bytes > cap) return -bytes;
   std::memset(out, 0, (size_t)bytes);

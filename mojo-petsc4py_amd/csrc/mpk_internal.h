@@ -118,6 +118,15 @@ __host__ __device__ __forceinline__ bool aligned16(const void *p) {
   return (reinterpret_cast<uintptr_t>(p) & 15) == 0;
 }
 
+// The hardware hands block b of a launch to XCD b % 8; this maps block b to a
+// tile index so that each XCD sweeps one contiguous run of tiles (neighbouring
+// tiles share data through that XCD's L2).
+#define MPK_XCDS 8
+__device__ __forceinline__ unsigned xcd_block(unsigned b, unsigned nb) {
+  unsigned per = nb / MPK_XCDS, rem = nb % MPK_XCDS, x = b % MPK_XCDS, k = b / MPK_XCDS;
+  return x < rem ? x * (per + 1) + k : rem * (per + 1) + (x - rem) * per + k;
+}
+
 // Halving tree over the 64 lanes of a wave: u[i] += u[i + w], w = 32..1.
 // Lane 0 ends with the wave's sum; the other lanes hold partial values.
 __device__ __forceinline__ double wave_tree(double u) {

@@ -43,13 +43,6 @@ __global__ void __launch_bounds__(256) k_spmv_row(i64 nrow, const int *__restric
 // shorter rows are never read.  Blocks are remapped so that each of the 8
 // XCDs sweeps one contiguous range of rows, which keeps the x entries that
 // neighbouring rows share inside that XCD's L2.
-#define MPK_XCDS 8
-
-__device__ __forceinline__ unsigned xcd_block(unsigned b, unsigned nb) {
-  // hardware dispatch hands block b to XCD b % 8; give XCD x a contiguous run
-  unsigned per = nb / MPK_XCDS, rem = nb % MPK_XCDS, x = b % MPK_XCDS, k = b / MPK_XCDS;
-  return x < rem ? x * (per + 1) + k : rem * (per + 1) + (x - rem) * per + k;
-}
 
 __global__ void __launch_bounds__(256) k_spmv_sell(i64 nrow, const int *__restrict__ rp,
                                                    const int *__restrict__ soff, const double *__restrict__ sv,

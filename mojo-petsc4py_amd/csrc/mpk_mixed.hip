@@ -1,0 +1,943 @@
+// mpk_mixed.hip -- mixed-precision solves: the inner Krylov iteration, its
+// SpMV and its preconditioner run on FP32 vectors (matrix values in FP32, or in
+// FP16 with one FP32 scale per row, FP32 accumulation), inside FP64 iterative
+// refinement.  FP64 stays where the accuracy is decided: the true residual
+// b - A x, the solution update, every dot product and norm (FP32 data summed
+// in FP64), the scalars they feed and the GMRES Hessenberg / Givens least
+// squares.  This is a form beside the drop-in path; it does not reproduce the
+// reference's arithmetic (the FP64 solvers do), only its answer to the
+// refinement tolerance.
+//
+// Data: the matrix's sliced copy (mpk_sell_build) gives the layout; the low
+// precision values sit in the same slots.  The preconditioner's FP64 block
+// plans (mpk_pc_schedule) give the sweeps' structure; a low-precision copy of
+// their coefficients sits in the same slots.
+#include <hip/hip_fp16.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+#include "mpk_internal.h"
+#include "mpk_ops.h"
+#include "mpk_pc.h"
+#include "mpk_sweep.h"
+#include "../../include/mpk.h"
+
+namespace mpk {
+
+__device__ __forceinline__ float lp_f(float v) { return v; }
+__device__ __forceinline__ float lp_f(__half v) { return __half2float(v); }
+template <class CT>
+__device__ __forceinline__ CT lp_from(float v);
+template <>
+__device__ __forceinline__ float lp_from<float>(float v) {
+  return v;
+}
+template <>
+__device__ __forceinline__ __half lp_from<__half>(float v) {
+  return __float2half(v);
+}
+
+// ------------------------------------------------------------ matrix
+
+// Low-precision values in the sliced layout; with rs, value = rs[i] * h and
+// rs[i] = max_j |a_ij| (1 for an all-zero row), so |h| <= 1.
+template <class CT>
+__global__ void k_lp_sell(i64 n, const int *rp, const int *soff, const double *sv, CT *lv, float *rs) {
+  const i64 i = (i64)blockIdx.x * 256 + threadIdx.x;
+  if (i >= n) return;
+  const int len = rp[i + 1] - rp[i], base = soff[i >> 6] + (int)(i & 63);
+  double s = 1.0;
+  if (rs) {
+    double m = 0.0;
+    for (int k = 0; k < len; k++) m = fmax(m, fabs(sv[base + 64 * k]));
+    s = m > 0.0 ? m : 1.0;
+    rs[i] = (float)s;
+  }
+  for (int k = 0; k < len; k++) lv[base + 64 * k] = lp_from<CT>((float)(sv[base + 64 * k] / s));
+}
+
+template <class CT>
+__global__ void __launch_bounds__(256) k_spmv_lp(i64 n, const int *__restrict__ rp, const int *__restrict__ soff,
+                                                 const int *__restrict__ sc, const CT *__restrict__ sv,
+                                                 const float *__restrict__ rs, const float *__restrict__ x,
+                                                 float *__restrict__ y) {
+  const unsigned b = xcd_block(blockIdx.x, gridDim.x);
+  const i64 i = (i64)b * 256 + threadIdx.x;
+  if (i >= n) return;
+  const int len = rp[i + 1] - rp[i], base = soff[i >> 6] + (int)(i & 63);
+  float s = 0.0f;
+  for (int k = 0; k < len; k++) {
+    const int p = base + 64 * k;
+    s = fmaf(lp_f(sv[p]), x[sc[p]], s);
+  }
+  y[i] = rs ? s * rs[i] : s;
+}
+
+// ------------------------------------------------------------ FP32 vectors
+
+static unsigned lp_grid(i64 n) {
+  const i64 g = (n + 255) / 256;
+  return (unsigned)(g < 1 ? 1 : g);
+}
+
+// out = (float)(r * scale[0])
+__global__ void k_lp_from_d(i64 n, const double *r, const double *scale, float *out) {
+  const i64 i = (i64)blockIdx.x * 256 + threadIdx.x;
+  if (i < n) out[i] = (float)(r[i] * scale[0]);
+}
+// x += scale[0] * d (FP64)
+__global__ void k_lp_update(i64 n, double *x, const double *scale, const float *d) {
+  const i64 i = (i64)blockIdx.x * 256 + threadIdx.x;
+  if (i < n) x[i] = fma(scale[0], (double)d[i], x[i]);
+}
+// y = x * s[0]
+__global__ void k_lp_scale(i64 n, const float *x, const double *s, float *y, const int *skip) {
+  if (skip && *skip) return;
+  const i64 i = (i64)blockIdx.x * 256 + threadIdx.x;
+  if (i < n) y[i] = x[i] * (float)s[0];
+}
+// y += sum_j coef[j] V_j, j < nv (V with leading dimension ld)
+__global__ void k_lp_maxpy(i64 n, float *y, const float *V, i64 ld, const double *coef, int nv, const int *skip) {
+  if (skip && *skip) return;
+  const i64 i = (i64)blockIdx.x * 256 + threadIdx.x;
+  if (i >= n) return;
+  float a = y[i];
+  for (int j = 0; j < nv; j++) a = fmaf((float)coef[j], V[(i64)j * ld + i], a);
+  y[i] = a;
+}
+__global__ void k_lp_zero(i64 n, float *y) {
+  const i64 i = (i64)blockIdx.x * 256 + threadIdx.x;
+  if (i < n) y[i] = 0.0f;
+}
+// z = x - z
+__global__ void k_lp_rsub(i64 n, const float *x, float *z) {
+  const i64 i = (i64)blockIdx.x * 256 + threadIdx.x;
+  if (i < n) z[i] = x[i] - z[i];
+}
+__global__ void k_lp_copy(i64 n, const float *x, float *y) {
+  const i64 i = (i64)blockIdx.x * 256 + threadIdx.x;
+  if (i < n) y[i] = x[i];
+}
+
+// ------------------------------------------------------------ reductions
+// FP32 data summed in FP64: per-block partials, then one block per result.
+#define LP_RBLK 512
+
+// P[k * LP_RBLK + b] = sum over block b's share of x_k . y_k (y_k null: x_k . x_k)
+template <int K>
+struct LpDots {
+  const float *x[K];
+  const float *y[K];
+};
+template <int K>
+__global__ void __launch_bounds__(256) k_lp_dots(LpDots<K> d, i64 n, double *P) {
+  __shared__ double sh[4];
+  double a[K];
+#pragma unroll
+  for (int k = 0; k < K; k++) a[k] = 0.0;
+  for (i64 i = (i64)blockIdx.x * 256 + threadIdx.x; i < n; i += (i64)gridDim.x * 256) {
+#pragma unroll
+    for (int k = 0; k < K; k++) {
+      const double xv = d.x[k][i];
+      a[k] = fma(xv, d.y[k] ? (double)d.y[k][i] : xv, a[k]);
+    }
+  }
+#pragma unroll
+  for (int k = 0; k < K; k++) {
+    const double r = block_tree(a[k], sh);
+    if (threadIdx.x == 0) P[(i64)k * gridDim.x + blockIdx.x] = r;
+  }
+}
+// P[j * LP_RBLK + b] = block b's share of V_j . w, j < nv
+__global__ void __launch_bounds__(256) k_lp_mdot(const float *V, i64 ld, int nv, const float *w, i64 n, double *P,
+                                                 const int *skip) {
+  if (skip && *skip) return;
+  __shared__ double sh[4];
+  for (int j0 = 0; j0 < nv; j0 += 8) {
+    double a[8];
+#pragma unroll
+    for (int k = 0; k < 8; k++) a[k] = 0.0;
+    for (i64 i = (i64)blockIdx.x * 256 + threadIdx.x; i < n; i += (i64)gridDim.x * 256) {
+      const double wv = w[i];
+#pragma unroll
+      for (int k = 0; k < 8; k++)
+        if (j0 + k < nv) a[k] = fma((double)V[(i64)(j0 + k) * ld + i], wv, a[k]);
+    }
+#pragma unroll
+    for (int k = 0; k < 8; k++) {
+      const double r = block_tree(a[k], sh);
+      if (threadIdx.x == 0 && j0 + k < nv) P[(i64)(j0 + k) * gridDim.x + blockIdx.x] = r;
+    }
+  }
+}
+// out[v] = sum_b P[v * nb + b], one block per v
+__global__ void __launch_bounds__(256) k_lp_fold(const double *P, int nb, double *out, const int *skip) {
+  if (skip && *skip) return;
+  __shared__ double sh[4];
+  double a = 0.0;
+  for (int b = threadIdx.x; b < nb; b += 256) a += P[(i64)blockIdx.x * nb + b];
+  const double r = block_tree(a, sh);
+  if (threadIdx.x == 0) out[blockIdx.x] = r;
+}
+
+// ------------------------------------------------------------ sweeps
+
+#define MPK_SENTINEL_F 0x7FA5D5A1u  // signalling NaN; arithmetic never yields it
+#define MPK_QNAN_F 0x7FC00000u
+
+__device__ __forceinline__ unsigned load_pub_f(const float *pub, i64 k) {
+  return __hip_atomic_load(reinterpret_cast<const unsigned *>(pub + k), __ATOMIC_RELAXED,
+                           __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void store_pub_f(float *pub, i64 k, float v) {
+  unsigned b = __float_as_uint(v);
+  if (b == MPK_SENTINEL_F) b = MPK_QNAN_F;
+  __hip_atomic_store(reinterpret_cast<unsigned *>(pub + k), b, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__global__ void k_lp_prepare(i64 n, float *pub, unsigned *ticket) {
+  const i64 i = (i64)blockIdx.x * 256 + threadIdx.x;
+  if (i == 0) *ticket = 0u;
+  if (i < n) pub[i] = __uint_as_float(MPK_SENTINEL_F);
+}
+
+struct LpStaged {
+  float init, s, d1, d2;
+  i64 o;
+};
+
+// The row programs of the low-precision sweeps: acc = sum of coef * value in
+// FP32 (coefficients divided by the row scale where there is one), then fin.
+struct LpIluFwd {
+  const int *r;
+  const float *b;
+  __device__ LpStaged stage(i64 i) const { return LpStaged{b[r[i]], 1.0f, 0.0f, 0.0f, 0}; }
+  __device__ float fin(i64, float acc, const LpStaged &s) const { return s.init + acc; }
+  __device__ float stale(i64) const { return 0.0f; }
+};
+struct LpIluBwd {
+  const int *c;
+  const float *F, *dinv;
+  float *x;
+  __device__ LpStaged stage(i64 i) const { return LpStaged{F[i], 1.0f, dinv[i], 0.0f, c[i]}; }
+  __device__ float fin(i64, float acc, const LpStaged &s) const {
+    const float v = (s.init + acc) * s.d1;
+    x[s.o] = v;
+    return v;
+  }
+  __device__ float stale(i64) const { return 0.0f; }
+};
+struct LpSorFwd {
+  const float *x, *rs, *idiag;
+  float *t, *yout;
+  const float *yin;
+  __device__ LpStaged stage(i64 i) const { return LpStaged{x[i], rs ? rs[i] : 1.0f, idiag[i], 0.0f, 0}; }
+  __device__ float fin(i64 i, float acc, const LpStaged &s) const {
+    const float tv = fmaf(s.s, acc, s.init);
+    t[i] = tv;
+    const float v = tv * s.d1;
+    if (yout) yout[i] = v;
+    return v;
+  }
+  __device__ float stale(i64 k) const { return yin[k]; }
+};
+struct LpSorBwd {
+  const float *src, *rs, *idiag, *yf;
+  float *y;
+  float omega;
+  int use_t;
+  __device__ LpStaged stage(i64 i) const {
+    return LpStaged{src[i], rs ? rs[i] : 1.0f, idiag[i], use_t ? yf[i] : 0.0f, 0};
+  }
+  __device__ float fin(i64 i, float acc, const LpStaged &s) const {
+    const float tv = fmaf(s.s, acc, s.init);
+    const float v = use_t ? fmaf(1.0f - omega, s.d2, tv * s.d1) : tv * s.d1;
+    y[i] = v;
+    return v;
+  }
+  __device__ float stale(i64 k) const { return y[k]; }
+};
+
+// The block-plan sweep (k_sweep_blk) in FP32 with low-precision coefficients.
+template <class Prog, class CT>
+__global__ void __launch_bounds__(256) k_sweep_lp(Prog prog, i64 n, const char *plan, int second,
+                                                  const CT *__restrict__ lc, unsigned *ticket, float *pub) {
+  __shared__ float val[MPK_BLK_ROWS];
+  __shared__ unsigned tk_sh;
+  if (threadIdx.x == 0) tk_sh = atomicAdd(ticket, 1u);
+  __syncthreads();
+  if (second) plan += reinterpret_cast<const i64 *>(plan)[4];
+  const PlanView P = plan_view(plan, n);
+  const i64 b = tk_sh;
+  if (b >= P.nblk) return;
+  const int p0 = P.blk_pos[b];
+  const int l0 = P.blk_lev[b], l1 = P.blk_lev[b + 1] - 1;
+  for (int l = l0; l < l1; l++) {
+    const int pa = P.lvl[l], pb = P.lvl[l + 1];
+    for (int p = pa + (int)threadIdx.x; p < pb; p += 256) {
+      const i64 i = P.prow[p];
+      const LpStaged st = prog.stage(i);
+      const int cnt = P.pcnt[p];
+      const int base = P.w ? p * (int)P.w : P.pent[p];
+      float acc = 0.0f;
+      for (int e0 = base, ee = base + cnt; e0 < ee; e0 += 8) {
+        // issue every load of the chunk, then poll what is not published yet
+        // issue every load of the chunk (LDS and global into separate
+        // registers), then resolve, polling what is not published yet
+        const int k = ee - e0 < 8 ? ee - e0 : 8;
+        int d[8];
+        float lv[8];
+        unsigned gb[8];
+#pragma unroll
+        for (int e = 0; e < 8; e++) {
+          d[e] = e < k ? P.edep[e0 + e] : 0;
+          lv[e] = val[d[e] >= 0 ? d[e] : 0];
+          gb[e] = 0u;
+          if (d[e] < 0 && d[e] >= -n) gb[e] = load_pub_f(pub, -(i64)d[e] - 1);
+        }
+#pragma unroll
+        for (int e = 0; e < 8; e++) {
+          if (e < k) {
+            float v;
+            if (d[e] >= 0) {
+              v = lv[e];
+            } else if (d[e] >= -n) {
+              unsigned b = gb[e];
+              while (b == MPK_SENTINEL_F) {
+                __builtin_amdgcn_s_sleep(1);
+                b = load_pub_f(pub, -(i64)d[e] - 1);
+              }
+              v = __uint_as_float(b);
+            } else {
+              v = prog.stale(-(i64)d[e] - n - 1);
+            }
+            acc = fmaf(lp_f(lc[e0 + e]), v, acc);
+          }
+        }
+      }
+      const float r = prog.fin(i, acc, st);
+      val[p - p0] = r;
+      store_pub_f(pub, i, r);
+    }
+    __syncthreads();
+  }
+}
+
+// Approximate sweeps: k Jacobi passes over a sweep's entries (a truncated
+// Neumann series of the triangular solve) instead of the ordered sweep.  Pass
+// 1 (u_in null) keeps only the row's own term; pass j adds the entries' values
+// from pass j - 1.  Every pass is one bandwidth-bound sweep over the sliced
+// entries, with no ordering between rows.
+template <class Prog, class CT>
+__global__ void __launch_bounds__(256) k_tri_jacobi(Prog prog, i64 n, const int *__restrict__ rp,
+                                                    const int *__restrict__ soff, const int *__restrict__ sc,
+                                                    const CT *__restrict__ sv, const float *__restrict__ u_in,
+                                                    float *__restrict__ u_out) {
+  const unsigned b = xcd_block(blockIdx.x, gridDim.x);
+  const i64 i = (i64)b * 256 + threadIdx.x;
+  if (i >= n) return;
+  const LpStaged st = prog.stage(i);
+  float acc = 0.0f;
+  if (u_in) {
+    const int len = rp[i + 1] - rp[i], base = soff[i >> 6] + (int)(i & 63);
+    for (int k = 0; k < len; k++) {
+      const int p = base + 64 * k;
+      acc = fmaf(lp_f(sv[p]), u_in[sc[p]], acc);
+    }
+  }
+  u_out[i] = prog.fin(i, acc, st);
+}
+__global__ void k_f2h(i64 m, const float *x, __half *y) {
+  const i64 i = (i64)blockIdx.x * 256 + threadIdx.x;
+  if (i < m) y[i] = __float2half(x[i]);
+}
+
+// A sweep's entries in the sliced layout (for k_tri_jacobi).
+struct TriSell {
+  int *rp = nullptr, *soff = nullptr, *sc = nullptr;
+  void *sv = nullptr;
+};
+
+// Low-precision coefficients of one sweep's plan: lc[slot] = coef / row scale.
+template <class CT>
+__global__ void k_lp_plan_coef(i64 n, const char *plan, int second, const float *rs, CT *lc) {
+  if (second) plan += reinterpret_cast<const i64 *>(plan)[4];
+  const PlanView P = plan_view(plan, n);
+  const i64 p = (i64)blockIdx.x * 256 + threadIdx.x;
+  if (p >= n) return;
+  const double s = rs ? (double)rs[P.prow[p]] : 1.0;
+  const int base = P.w ? (int)(p * P.w) : P.pent[p];
+  for (int e = base; e < base + P.pcnt[p]; e++) lc[e] = lp_from<CT>((float)(P.ecoef[e] / s));
+}
+
+// FP32 inverted diagonals: ILU's stored 1/u_ii; SOR's as MatInvertDiagonalForSOR.
+__global__ void k_lp_ilu_dinv(i64 n, const double *ba, const int *bdiag, float *dinv) {
+  const i64 i = (i64)blockIdx.x * 256 + threadIdx.x;
+  if (i < n) dinv[i] = (float)ba[bdiag[i]];
+}
+__global__ void k_lp_sor_idiag(i64 n, const double *a, const int *dmark, double omega, double fshift,
+                               float *idiag) {
+  const i64 i = (i64)blockIdx.x * 256 + threadIdx.x;
+  if (i >= n) return;
+  const double d = a[dmark[i]];
+  idiag[i] = (float)((omega == 1.0 && fshift <= 0.0) ? 1.0 / d : omega / (fshift + d));
+}
+
+static i64 plan_slots(const char *plan_d, int second, i64 n) {
+  i64 h[8];
+  HIPCHK(hipMemcpy(h, plan_d, sizeof h, hipMemcpyDeviceToHost));
+  if (second) HIPCHK(hipMemcpy(h, plan_d + h[4], sizeof h, hipMemcpyDeviceToHost));
+  return h[5] ? n * h[5] : h[1];
+}
+
+// ------------------------------------------------------------ the solver
+
+enum { LP_GMRES = 0, LP_BCGS = 1 };
+#define LP_MMAX 64
+
+// Device state of the inner iterations (FP64 scalars).
+struct LpState {
+  double H[(LP_MMAX + 1) * LP_MMAX];  // H(i, j) at j (LP_MMAX + 1) + i
+  double cs[LP_MMAX], sn[LP_MMAX], g[LP_MMAX + 1], y[LP_MMAX];
+  double h[LP_MMAX + 1], h2[LP_MMAX + 1], coef[LP_MMAX + 1];
+  double red[4];
+  double beta, inv, res, wnorm;
+  double rho, rho_old, alpha, omega, bcoef;
+  int norefine, breakdown;
+};
+
+// GMRES: start of a cycle, red[0] = w . w
+__global__ void k_gm_begin(LpState *s) {
+  s->beta = sqrt(s->red[0]);
+  s->g[0] = s->beta;
+  s->res = s->beta;
+  s->inv = s->beta > 0.0 ? 1.0 / s->beta : 0.0;
+  s->breakdown = s->beta > 0.0 ? 0 : 1;
+}
+// coef = -src (the Gram-Schmidt subtraction), unless *skip
+__global__ void k_gm_neg(LpState *s, int nv, const double *src, const int *skip) {
+  if (skip && *skip) return;
+  for (int j = 0; j < nv; j++) s->coef[j] = -src[j];
+}
+// after the first Gram-Schmidt pass, red[0] = w . w: the second pass runs
+// always (refine 2), never (0), or when ||w|| < ||h|| (1, "if needed")
+__global__ void k_gm_decide(LpState *s, int nv, int refine) {
+  double hn = 0.0;
+  for (int j = 0; j < nv; j++) hn = fma(s->h[j], s->h[j], hn);
+  s->wnorm = sqrt(s->red[0]);
+  s->norefine = !(refine == 2 || (refine == 1 && s->wnorm < sqrt(hn)));
+}
+// after the (optional) second pass: h += h2, new norm in red[0]
+__global__ void k_gm_column(LpState *s, int k) {
+  const int nv = k + 1;
+  if (!s->norefine) {
+    for (int j = 0; j < nv; j++) s->h[j] += s->h2[j];
+    s->wnorm = sqrt(s->red[0]);
+  }
+  double *col = s->H + (i64)k * (LP_MMAX + 1);
+  for (int j = 0; j < nv; j++) col[j] = s->h[j];
+  col[nv] = s->wnorm;
+  for (int j = 0; j < k; j++) {  // previous rotations
+    const double a = col[j], b = col[j + 1];
+    col[j] = s->cs[j] * a + s->sn[j] * b;
+    col[j + 1] = -s->sn[j] * a + s->cs[j] * b;
+  }
+  const double a = col[k], b = col[k + 1], r = sqrt(a * a + b * b);
+  if (r == 0.0) {
+    s->breakdown = 1;
+    s->res = 0.0;
+    s->inv = 0.0;
+    return;
+  }
+  s->cs[k] = a / r;
+  s->sn[k] = b / r;
+  col[k] = r;
+  col[k + 1] = 0.0;
+  s->g[k + 1] = -s->sn[k] * s->g[k];
+  s->g[k] = s->cs[k] * s->g[k];
+  s->res = fabs(s->g[k + 1]);
+  s->inv = s->wnorm > 0.0 ? 1.0 / s->wnorm : 0.0;
+  if (s->wnorm == 0.0) s->breakdown = 1;
+}
+// least squares: back substitution on the rotated H, y = R^-1 g (kk columns)
+__global__ void k_gm_solve(LpState *s, int kk) {
+  for (int i = kk - 1; i >= 0; i--) {
+    double t = s->g[i];
+    for (int j = i + 1; j < kk; j++) t -= s->H[(i64)j * (LP_MMAX + 1) + i] * s->y[j];
+    const double d = s->H[(i64)i * (LP_MMAX + 1) + i];
+    s->y[i] = d != 0.0 ? t / d : 0.0;
+  }
+}
+
+// BiCGStab (right preconditioned) scalar steps
+__global__ void k_bc_init(LpState *s) {
+  s->rho_old = s->alpha = s->omega = 1.0;
+  s->res = sqrt(s->red[0]);
+  s->beta = s->res;
+  s->breakdown = s->res > 0.0 ? 0 : 1;
+}
+__global__ void k_bc_rho(LpState *s) {  // red[0] = rhat . r
+  const double rho = s->red[0];
+  if (rho == 0.0 || s->omega == 0.0) s->breakdown = 1;
+  s->bcoef = (s->rho_old != 0.0 && s->omega != 0.0) ? (rho / s->rho_old) * (s->alpha / s->omega) : 0.0;
+  s->rho_old = rho;
+  s->coef[0] = s->bcoef;                // p = r + bcoef (p - omega v)
+  s->coef[1] = -s->bcoef * s->omega;
+}
+__global__ void k_bc_alpha(LpState *s) {  // red[0] = rhat . v
+  s->alpha = s->red[0] != 0.0 ? s->rho_old / s->red[0] : 0.0;
+  if (s->red[0] == 0.0) s->breakdown = 1;
+  s->coef[2] = -s->alpha;
+}
+__global__ void k_bc_omega(LpState *s) {  // red[0] = t . s, red[1] = t . t
+  s->omega = s->red[1] != 0.0 ? s->red[0] / s->red[1] : 0.0;
+  s->coef[3] = s->alpha;
+  s->coef[4] = s->omega;
+  s->coef[5] = -s->omega;
+}
+__global__ void k_bc_res(LpState *s) { s->res = sqrt(s->red[0]); }
+
+// p = r + c0 p + c1 v
+__global__ void k_bc_p(i64 n, float *p, const float *r, const float *v, const double *c) {
+  const i64 i = (i64)blockIdx.x * 256 + threadIdx.x;
+  if (i < n) p[i] = fmaf((float)c[1], v[i], fmaf((float)c[0], p[i], r[i]));
+}
+// y = x + c v
+__global__ void k_lp_waxpy(i64 n, float *y, const float *x, const float *v, const double *c) {
+  const i64 i = (i64)blockIdx.x * 256 + threadIdx.x;
+  if (i < n) y[i] = fmaf((float)c[0], v[i], x[i]);
+}
+// d += c0 a + c1 b
+__global__ void k_lp_axpby_add(i64 n, float *d, const float *a, const float *b, const double *c) {
+  const i64 i = (i64)blockIdx.x * 256 + threadIdx.x;
+  if (i < n) d[i] = fmaf((float)c[1], b[i], fmaf((float)c[0], a[i], d[i]));
+}
+
+struct MixedSolver {
+  i64 n = 0;
+  int prec = 32, inner = LP_GMRES, m = 30, refine = 1;
+  DevCSR A64{};
+  const int *soff = nullptr, *sc = nullptr;
+  void *lv = nullptr;  // low-precision matrix values (float or __half)
+  float *rs = nullptr;
+  PCView pc;
+  void *lc0 = nullptr, *lc1 = nullptr;  // low-precision sweep coefficients
+  float *dinv = nullptr, *idiag = nullptr;
+  int tri_iters = 0;  // > 0: approximate sweeps by this many Jacobi passes
+  TriSell tri[2];
+  float *work = nullptr;  // FP32 vectors
+  double *r64 = nullptr;  // FP64 residual
+  double *P = nullptr;    // reduction partials
+  LpState *st = nullptr;
+  double *hst = nullptr;  // pinned: res, breakdown
+  unsigned *ticket = nullptr;
+
+  ~MixedSolver() {
+    void *bufs[] = {lv,          rs,          lc0,         lc1,       dinv,      idiag,    work,
+                    r64,         P,           st,          ticket,    tri[0].rp, tri[0].soff, tri[0].sc,
+                    tri[0].sv,   tri[1].rp,   tri[1].soff, tri[1].sc, tri[1].sv};
+    hipStreamSynchronize(stream());
+    for (void *b : bufs)
+      if (b) hipFree(b);
+    if (hst) hipHostFree(hst);
+  }
+
+  template <class CT>
+  void spmv_t(const float *x, float *y) {
+    MPK_LAUNCH(k_spmv_lp<CT>, lp_grid(n), 256, n, A64.rp, soff, sc, (const CT *)lv, (const float *)rs, x, y);
+  }
+  void spmv(const float *x, float *y) {
+    if (prec == 16)
+      spmv_t<__half>(x, y);
+    else
+      spmv_t<float>(x, y);
+  }
+
+  template <class Prog>
+  void sweep(const Prog &prog, int second, float *pub) {
+    if (tri_iters > 0) {
+      // Jacobi passes, ping-ponging through the 4th PC vector so that the last
+      // pass lands in pub
+      float *tmp = pc_scratch() + 3 * n;
+      const TriSell &T = tri[second];
+      for (int j = 0; j < tri_iters; j++) {
+        float *out = ((tri_iters - 1 - j) % 2 == 0) ? pub : tmp;
+        const float *in = j == 0 ? nullptr : (((tri_iters - j) % 2 == 0) ? pub : tmp);
+        if (prec == 16)
+          MPK_LAUNCH((k_tri_jacobi<Prog, __half>), lp_grid(n), 256, prog, n, (const int *)T.rp,
+                     (const int *)T.soff, (const int *)T.sc, (const __half *)T.sv, in, out);
+        else
+          MPK_LAUNCH((k_tri_jacobi<Prog, float>), lp_grid(n), 256, prog, n, (const int *)T.rp,
+                     (const int *)T.soff, (const int *)T.sc, (const float *)T.sv, in, out);
+      }
+      return;
+    }
+    MPK_LAUNCH(k_lp_prepare, lp_grid(n), 256, n, pub, ticket);
+    const void *lc = second ? lc1 : lc0;
+    if (prec == 16)
+      MPK_LAUNCH((k_sweep_lp<Prog, __half>), nblocks(n, 1024), 256, prog, n, pc_plan(), second,
+                 (const __half *)lc, ticket, pub);
+    else
+      MPK_LAUNCH((k_sweep_lp<Prog, float>), nblocks(n, 1024), 256, prog, n, pc_plan(), second,
+                 (const float *)lc, ticket, pub);
+  }
+  const char *pc_plan() const {
+    return pc.type == PCT_ILU ? pc.ilu.plan : pc.type == PCT_SOR ? pc.sor.plan : nullptr;
+  }
+  // FP32 vectors: the inner solver's (GMRES: V[m+1], w, z, d; BiCGStab: r,
+  // rhat, p, v, s, t, phat, shat, d -- d last), then 4 for the
+  // preconditioner's sweeps, then the right-hand side
+  i64 nsolver() const { return inner == LP_GMRES ? m + 4 : 9; }
+  float *d_vec() { return work + (nsolver() - 1) * n; }
+  float *pc_scratch() { return work + nsolver() * n; }
+  float *rhs_vec() { return work + (nsolver() + 4) * n; }
+
+  // The sweeps' entries as sliced matrices in low precision (coefficients
+  // divided by the row scale where the matrix has one: SOR at FP16).
+  bool build_tri() {
+    std::vector<float> hrs;
+    if (pc.type == PCT_SOR && rs) {
+      hrs.resize(n);
+      HIPCHK(hipMemcpy(hrs.data(), rs, n * sizeof(float), hipMemcpyDeviceToHost));
+    }
+    for (int w = 0; w < 2; w++) {
+      std::vector<i64> off;
+      std::vector<int> col;
+      std::vector<double> coef;
+      if (!pc_sweep_entries(pc, w, off, col, coef)) return false;
+      const i64 ns = (n + 63) / 64;
+      std::vector<int> rp(n + 1), so(ns + 1, 0);
+      for (i64 i = 0; i <= n; i++) rp[i] = (int)off[i];
+      i64 tot = 0;
+      for (i64 sl = 0; sl < ns; sl++) {
+        i64 wmax = 0;
+        for (i64 i = sl * 64; i < n && i < sl * 64 + 64; i++) wmax = std::max<i64>(wmax, off[i + 1] - off[i]);
+        so[sl] = (int)tot;
+        tot += 64 * wmax;
+      }
+      so[ns] = (int)tot;
+      const i64 slots = tot > 0 ? tot : 1;
+      std::vector<int> sc(slots, 0);
+      std::vector<float> sv(slots, 0.0f);
+      for (i64 i = 0; i < n; i++) {
+        const double sdiv = hrs.empty() ? 1.0 : (double)hrs[i];
+        const i64 base = so[i >> 6] + (i & 63);
+        for (i64 e = off[i]; e < off[i + 1]; e++) {
+          sc[base + 64 * (e - off[i])] = col[e];
+          sv[base + 64 * (e - off[i])] = (float)(coef[e] / sdiv);
+        }
+      }
+      TriSell &T = tri[w];
+      HIPCHK(hipMalloc(&T.rp, (n + 1) * sizeof(int)));
+      HIPCHK(hipMalloc(&T.soff, (ns + 1) * sizeof(int)));
+      HIPCHK(hipMalloc(&T.sc, slots * sizeof(int)));
+      HIPCHK(hipMalloc(&T.sv, slots * (prec == 16 ? 2 : 4)));
+      if (failed()) return false;
+      HIPCHK(hipMemcpy(T.rp, rp.data(), (n + 1) * sizeof(int), hipMemcpyHostToDevice));
+      HIPCHK(hipMemcpy(T.soff, so.data(), (ns + 1) * sizeof(int), hipMemcpyHostToDevice));
+      HIPCHK(hipMemcpy(T.sc, sc.data(), slots * sizeof(int), hipMemcpyHostToDevice));
+      if (prec == 16) {
+        float *tmp = nullptr;
+        HIPCHK(hipMalloc(&tmp, slots * sizeof(float)));
+        HIPCHK(hipMemcpy(tmp, sv.data(), slots * sizeof(float), hipMemcpyHostToDevice));
+        MPK_LAUNCH(k_f2h, lp_grid(slots), 256, slots, (const float *)tmp, (__half *)T.sv);
+        HIPCHK(hipStreamSynchronize(stream()));
+        HIPCHK(hipFree(tmp));
+      } else {
+        HIPCHK(hipMemcpy(T.sv, sv.data(), slots * sizeof(float), hipMemcpyHostToDevice));
+      }
+    }
+    return !failed();
+  }
+
+  // y = M^-1 x in FP32
+  void apply_pc(const float *x, float *y) {
+    float *S = pc_scratch();
+    if (pc.type == PCT_ILU) {
+      float *F = S, *B = S + n;
+      sweep(LpIluFwd{pc.ilu.r, x}, 0, F);
+      sweep(LpIluBwd{pc.ilu.c, F, dinv, y}, 1, B);
+    } else if (pc.type == PCT_SOR) {
+      float *YF = S, *YB = S + n, *T = S + 2 * n;
+      const int flag = pc.sor.flag;
+      const bool fwd = (flag & (1 | 4)) != 0, bwd = (flag & (2 | 8)) != 0;
+      if (fwd) sweep(LpSorFwd{x, rs, idiag, T, bwd ? nullptr : y, y}, 0, YF);
+      if (bwd) sweep(LpSorBwd{fwd ? T : x, rs, idiag, YF, y, (float)pc.sor.omega, fwd ? 1 : 0}, 1, YB);
+    } else {
+      MPK_LAUNCH(k_lp_copy, lp_grid(n), 256, n, x, y);
+    }
+  }
+
+  // FP64 results of K dot products into st->red
+  template <int K>
+  void dots(LpDots<K> d) {
+    MPK_LAUNCH(k_lp_dots<K>, LP_RBLK, 256, d, n, P);
+    MPK_LAUNCH(k_lp_fold, K, 256, (const double *)P, LP_RBLK, st->red, (const int *)nullptr);
+  }
+  LpDots<1> dot1(const float *x, const float *y) { return LpDots<1>{{x}, {y}}; }
+
+  double read_res(int *breakdown) {
+    HIPCHK(hipMemcpyAsync(hst, &st->res, sizeof(double), hipMemcpyDeviceToHost, stream()));
+    HIPCHK(hipMemcpyAsync(hst + 1, &st->breakdown, sizeof(int), hipMemcpyDeviceToHost, stream()));
+    HIPCHK(hipStreamSynchronize(stream()));
+    *breakdown = *reinterpret_cast<int *>(hst + 1);
+    return hst[0];
+  }
+
+  // Inner GMRES(m), left preconditioned, zero guess: d ~ A^-1 rhs to a
+  // relative preconditioned residual rtol (or max_it steps).  Returns steps.
+  i64 gmres(const float *rhs, float *d, double rtol, i64 max_it) {
+    float *V = work, *w = work + (i64)(m + 1) * n, *z = w + n;
+    const i64 ld = n;
+    MPK_LAUNCH(k_lp_zero, lp_grid(n), 256, n, d);
+    MPK_LAUNCH(k_lp_copy, lp_grid(n), 256, n, rhs, z);
+    i64 its = 0;
+    double target = -1.0;
+    while (its < max_it) {
+      // z holds the current unpreconditioned residual
+      apply_pc(z, w);
+      dots(dot1(w, nullptr));
+      MPK_LAUNCH(k_gm_begin, 1, 1, st);
+      MPK_LAUNCH(k_lp_scale, lp_grid(n), 256, n, (const float *)w, (const double *)&st->inv, V,
+                 (const int *)nullptr);
+      int brk = 0;
+      const double beta = read_res(&brk);
+      if (target < 0.0) target = rtol * beta;
+      if (brk || beta <= target) break;
+      int kk = 0;
+      for (int k = 0; k < m && its < max_it; k++) {
+        float *vk = V + (i64)k * ld, *vn = V + (i64)(k + 1) * ld;
+        spmv(vk, z);
+        apply_pc(z, w);
+        const int nv = k + 1;
+        // classical Gram-Schmidt: h = V^T w, w -= V h, then the second pass
+        // (h2 = V^T w, w -= V h2, h += h2) when k_gm_decide asks for it; the
+        // device skips the second pass's kernels otherwise
+        const int *nore = &st->norefine;
+        MPK_LAUNCH(k_lp_mdot, LP_RBLK, 256, (const float *)V, ld, nv, (const float *)w, n, P,
+                   (const int *)nullptr);
+        MPK_LAUNCH(k_lp_fold, nv, 256, (const double *)P, LP_RBLK, st->h, (const int *)nullptr);
+        MPK_LAUNCH(k_gm_neg, 1, 1, st, nv, (const double *)st->h, (const int *)nullptr);
+        MPK_LAUNCH(k_lp_maxpy, lp_grid(n), 256, n, w, (const float *)V, ld, (const double *)st->coef, nv,
+                   (const int *)nullptr);
+        dots(dot1(w, nullptr));
+        MPK_LAUNCH(k_gm_decide, 1, 1, st, nv, refine);
+        MPK_LAUNCH(k_lp_mdot, LP_RBLK, 256, (const float *)V, ld, nv, (const float *)w, n, P, nore);
+        MPK_LAUNCH(k_lp_fold, nv, 256, (const double *)P, LP_RBLK, st->h2, nore);
+        MPK_LAUNCH(k_gm_neg, 1, 1, st, nv, (const double *)st->h2, nore);
+        MPK_LAUNCH(k_lp_maxpy, lp_grid(n), 256, n, w, (const float *)V, ld, (const double *)st->coef, nv, nore);
+        MPK_LAUNCH(k_lp_dots<1>, LP_RBLK, 256, dot1(w, nullptr), n, P);
+        MPK_LAUNCH(k_lp_fold, 1, 256, (const double *)P, LP_RBLK, st->red, nore);
+        MPK_LAUNCH(k_gm_column, 1, 1, st, k);
+        MPK_LAUNCH(k_lp_scale, lp_grid(n), 256, n, (const float *)w, (const double *)&st->inv, vn,
+                   (const int *)nullptr);
+        its++;
+        kk = k + 1;
+        const double res = read_res(&brk);
+        if (brk || res <= target) break;
+      }
+      MPK_LAUNCH(k_gm_solve, 1, 1, st, kk);
+      MPK_LAUNCH(k_lp_maxpy, lp_grid(n), 256, n, d, (const float *)V, ld, (const double *)st->y, kk,
+                 (const int *)nullptr);
+      const double res = read_res(&brk);
+      if (brk || res <= target || its >= max_it) break;
+      // restart: z = rhs - A d
+      spmv(d, z);
+      MPK_LAUNCH(k_lp_rsub, lp_grid(n), 256, n, rhs, z);
+    }
+    return its;
+  }
+
+  // Inner BiCGStab, right preconditioned, zero guess, to a relative residual
+  // rtol (or max_it steps).  Returns steps.
+  i64 bcgs(const float *rhs, float *d, double rtol, i64 max_it) {
+    float *r = work, *rh = r + n, *p = rh + n, *v = p + n, *sv = v + n, *t = sv + n, *ph = t + n, *sh = ph + n;
+    MPK_LAUNCH(k_lp_zero, lp_grid(n), 256, n, d);
+    MPK_LAUNCH(k_lp_copy, lp_grid(n), 256, n, rhs, r);
+    MPK_LAUNCH(k_lp_copy, lp_grid(n), 256, n, rhs, rh);
+    MPK_LAUNCH(k_lp_zero, lp_grid(n), 256, n, p);
+    MPK_LAUNCH(k_lp_zero, lp_grid(n), 256, n, v);
+    dots(dot1(r, nullptr));
+    MPK_LAUNCH(k_bc_init, 1, 1, st);
+    int brk = 0;
+    const double beta = read_res(&brk);
+    const double target = rtol * beta;
+    i64 its = 0;
+    while (!brk && its < max_it) {
+      dots(dot1(rh, r));
+      MPK_LAUNCH(k_bc_rho, 1, 1, st);
+      MPK_LAUNCH(k_bc_p, lp_grid(n), 256, n, p, (const float *)r, (const float *)v, (const double *)st->coef);
+      apply_pc(p, ph);
+      spmv(ph, v);
+      dots(dot1(rh, v));
+      MPK_LAUNCH(k_bc_alpha, 1, 1, st);
+      MPK_LAUNCH(k_lp_waxpy, lp_grid(n), 256, n, sv, (const float *)r, (const float *)v,
+                 (const double *)(st->coef + 2));
+      apply_pc(sv, sh);
+      spmv(sh, t);
+      dots(LpDots<2>{{t, t}, {sv, nullptr}});
+      MPK_LAUNCH(k_bc_omega, 1, 1, st);
+      MPK_LAUNCH(k_lp_axpby_add, lp_grid(n), 256, n, d, (const float *)ph, (const float *)sh,
+                 (const double *)(st->coef + 3));
+      MPK_LAUNCH(k_lp_waxpy, lp_grid(n), 256, n, r, (const float *)sv, (const float *)t,
+                 (const double *)(st->coef + 5));
+      dots(dot1(r, nullptr));
+      MPK_LAUNCH(k_bc_res, 1, 1, st);
+      its++;
+      const double res = read_res(&brk);
+      if (res <= target) break;
+    }
+    return its;
+  }
+};
+
+}  // namespace mpk
+
+using namespace mpk;
+
+extern "C" {
+
+// See include/mpk.h.
+int64_t mpk_mixed_create(int64_t n, int64_t irow, int64_t icol, int64_t data, int64_t soff, int64_t sv,
+                         int64_t sc, int64_t pci, int64_t pcf, int64_t prec, int64_t inner, int64_t restart,
+                         int64_t tri_iters) {
+  if (n <= 0 || !ctx()) return 0;
+  if (prec != 32 && prec != 16) {
+    set_error(6, "mixed solve: precision must be 32 or 16, got %lld", (long long)prec);
+    return 0;
+  }
+  if (inner != LP_GMRES && inner != LP_BCGS) {
+    set_error(6, "mixed solve: inner solver must be 0 (gmres) or 1 (bcgs)");
+    return 0;
+  }
+  if (inner == LP_GMRES && (restart < 1 || restart > LP_MMAX)) {
+    set_error(6, "mixed solve: restart must be in 1..%d", LP_MMAX);
+    return 0;
+  }
+  MixedSolver *S = new MixedSolver();
+  S->n = n;
+  S->prec = (int)prec;
+  S->inner = (int)inner;
+  S->m = (int)restart;
+  S->tri_iters = tri_iters > 0 ? (int)tri_iters : 0;
+  S->A64 = DevCSR{n, n, (const int *)irow, (const int *)icol, (const double *)data, (const int *)soff,
+                  (const double *)sv, (const int *)sc};
+  S->soff = (const int *)soff;
+  S->sc = (const int *)sc;
+  if (!S->pc.parse((const int64_t *)pci, (const double *)pcf, n)) {
+    delete S;
+    return 0;
+  }
+  if (S->pc.type != PCT_ILU && S->pc.type != PCT_SOR && S->pc.type != PCT_NONE) {
+    set_error(6, "mixed solve: the preconditioner must be ilu, sor or none");
+    delete S;
+    return 0;
+  }
+  if ((S->pc.type == PCT_ILU || S->pc.type == PCT_SOR) && !S->pc_plan()) {
+    set_error(6, "mixed solve: the preconditioner has no sweep plans (mpk_pc_schedule)");
+    delete S;
+    return 0;
+  }
+  // low-precision matrix values in the sliced slots
+  std::vector<int> so(((size_t)n + 63) / 64 + 1);
+  HIPCHK(hipMemcpy(so.data(), (const void *)soff, so.size() * sizeof(int), hipMemcpyDeviceToHost));
+  const i64 slots = so.back() > 0 ? so.back() : 1, esz = prec == 16 ? 2 : 4;
+  HIPCHK(hipMalloc(&S->lv, slots * esz));
+  if (prec == 16) {
+    HIPCHK(hipMalloc(&S->rs, n * sizeof(float)));
+    MPK_LAUNCH(k_lp_sell<__half>, lp_grid(n), 256, n, (const int *)irow, (const int *)soff, (const double *)sv,
+               (__half *)S->lv, S->rs);
+  } else {
+    MPK_LAUNCH(k_lp_sell<float>, lp_grid(n), 256, n, (const int *)irow, (const int *)soff, (const double *)sv,
+               (float *)S->lv, (float *)nullptr);
+  }
+  // low-precision preconditioner
+  if (const char *plan = S->pc_plan()) {
+    void **dst[2] = {&S->lc0, &S->lc1};
+    for (int sw = 0; sw < 2; sw++) {
+      const i64 ns = plan_slots(plan, sw, n);
+      HIPCHK(hipMalloc(dst[sw], (ns > 0 ? ns : 1) * esz));
+      const float *rsw = S->pc.type == PCT_SOR ? S->rs : nullptr;
+      if (prec == 16)
+        MPK_LAUNCH(k_lp_plan_coef<__half>, lp_grid(n), 256, n, plan, sw, rsw, (__half *)*dst[sw]);
+      else
+        MPK_LAUNCH(k_lp_plan_coef<float>, lp_grid(n), 256, n, plan, sw, rsw, (float *)*dst[sw]);
+    }
+    if (S->tri_iters > 0 && !S->build_tri()) {
+      delete S;
+      return 0;
+    }
+    if (S->pc.type == PCT_ILU) {
+      HIPCHK(hipMalloc(&S->dinv, n * sizeof(float)));
+      MPK_LAUNCH(k_lp_ilu_dinv, lp_grid(n), 256, n, S->pc.ilu.ba, S->pc.ilu.bdiag, S->dinv);
+    } else {
+      HIPCHK(hipMalloc(&S->idiag, n * sizeof(float)));
+      MPK_LAUNCH(k_lp_sor_idiag, lp_grid(n), 256, n, S->pc.sor.a, S->pc.sor.dmark, S->pc.sor.omega,
+                 S->pc.sor.fshift, S->idiag);
+    }
+  }
+  const i64 nvec = S->nsolver() + 5;
+  HIPCHK(hipMalloc(&S->work, nvec * n * sizeof(float)));
+  HIPCHK(hipMalloc(&S->r64, n * sizeof(double)));
+  HIPCHK(hipMalloc(&S->P, (LP_MMAX + 1) * LP_RBLK * sizeof(double)));
+  HIPCHK(hipMalloc(&S->st, sizeof(LpState)));
+  HIPCHK(hipHostMalloc(&S->hst, 4 * sizeof(double), hipHostMallocDefault));
+  HIPCHK(hipMalloc(&S->ticket, 64 * sizeof(unsigned)));
+  if (!failed()) HIPCHK(hipMemset(S->st, 0, sizeof(LpState)));
+  sync();
+  if (failed()) {
+    delete S;
+    return 0;
+  }
+  return (int64_t)S;
+}
+
+// Iterative refinement: x (FP64, [D]) from b (FP64, [D]).  cfg [H] float64:
+// in  {rtol, inner_rtol, max_outer, max_inner, refine (gmres: 0 never,
+//      1 if needed, 2 always), x_is_zero}
+// out {cfg[8] final relative residual, cfg[9] outer steps, cfg[10] inner
+//      steps, cfg[11] ||b||, cfg[12] converged (1) or not (0)}.
+int64_t mpk_mixed_solve(int64_t h, int64_t b, int64_t x, int64_t cfg_) {
+  MixedSolver *S = (MixedSolver *)h;
+  double *cfg = (double *)cfg_;
+  if (!S || !ctx()) return -1;
+  const i64 n = S->n;
+  const double rtol = cfg[0], irtol = cfg[1];
+  const i64 max_outer = (i64)cfg[2], max_inner = (i64)cfg[3];
+  S->refine = (int)cfg[4];
+  const bool zero = cfg[5] != 0.0;
+  double *X = (double *)x, *R = S->r64;
+  const double *B = (const double *)b;
+  const double bn = norm2_host(B, n);
+  cfg[11] = bn;
+  float *rhs = S->rhs_vec(), *d = S->d_vec();
+  if (zero) vset(X, 0.0, n, nullptr);
+  i64 outer = 0, inner = 0;
+  double rel = 0.0;
+  for (;;) {
+    spmv(S->A64, X, R, 0, nullptr);  // R = b - A x in FP64
+    vaypx(R, -1.0, B, n, nullptr);
+    const double rn = norm2_host(R, n);
+    rel = bn > 0.0 ? rn / bn : rn;
+    if (failed() || rel <= rtol || outer >= max_outer || !(rn > 0.0)) break;
+    // FP32 right-hand side scaled to unit norm; x += ||r|| d
+    double sc[2] = {1.0 / rn, rn};
+    HIPCHK(hipMemcpy(S->st->red + 2, sc, sizeof sc, hipMemcpyHostToDevice));
+    MPK_LAUNCH(k_lp_from_d, lp_grid(n), 256, n, (const double *)R, (const double *)(S->st->red + 2), rhs);
+    inner += S->inner == LP_GMRES ? S->gmres(rhs, d, irtol, max_inner) : S->bcgs(rhs, d, irtol, max_inner);
+    MPK_LAUNCH(k_lp_update, lp_grid(n), 256, n, X, (const double *)(S->st->red + 3), (const float *)d);
+    outer++;
+  }
+  sync();
+  cfg[8] = rel;
+  cfg[9] = (double)outer;
+  cfg[10] = (double)inner;
+  cfg[12] = rel <= rtol ? 1.0 : 0.0;
+  return failed() ? -1 : 0;
+}
+
+void mpk_mixed_free(int64_t h) { delete (MixedSolver *)h; }
+
+}  // extern "C"

@@ -2,6 +2,8 @@
 // the Krylov drivers.
 #pragma once
 
+#include <vector>
+
 #include "mpk_internal.h"
 
 namespace mpk {
@@ -69,5 +71,10 @@ struct PCView {
   i64 schedule_size() const;
   i64 schedule(char *plan, i64 cap) const;
 };
+
+// Every entry of sweep `which` (0: forward / first, 1: second) of a PC with
+// sweeps, in each row's own order, on the host: off[n+1], col, coef.
+bool pc_sweep_entries(const PCView &pc, int which, std::vector<i64> &off, std::vector<int> &col,
+                      std::vector<double> &coef);
 
 }  // namespace mpk

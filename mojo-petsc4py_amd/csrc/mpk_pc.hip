@@ -341,6 +341,16 @@ static bool with_sweeps(const PCView &pc, F &&f) {
   }
 }
 
+bool pc_sweep_entries(const PCView &pc, int which, std::vector<i64> &off, std::vector<int> &col,
+                      std::vector<double> &coef) {
+  int k = 0;
+  bool ok = false;
+  with_sweeps(pc, [&](auto prog, bool) {
+    if (k++ == which) ok = describe_sweep(prog, pc.n, off, col, coef);
+  });
+  return ok;
+}
+
 i64 PCView::schedule_size() const {
   i64 total = 0;
   bool bad = false;

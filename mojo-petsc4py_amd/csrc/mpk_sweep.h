@@ -134,13 +134,15 @@ __global__ void __launch_bounds__(64) k_sweep(Prog prog, i64 n, int forward, uns
 // its entry count, and its entries in the row's order: the coefficient and
 // where the value comes from -- d >= 0: LDS slot d; -n <= d < 0: row -d-1 of
 // an earlier block; d < -n: row -d-n-1, not earlier in the sweep (stale,
-// malformed input only).  When no row has more than MPK_PLAN_W entries they
-// sit in MPK_PLAN_W fixed slots per position (one round of loads per row),
-// else in CSR form over pent.  Before the block kernel, k_sweep_stage gathers
-// each position's Staged values into position order, so a level's loads are
-// all contiguous and none depends on another.
+// malformed input only).  When no row has more than MPK_PLAN_WMAX entries
+// they sit in w fixed slots per position (w = 4, 8, 16 or 32, the smallest
+// that holds the longest row), so one round of independent loads fetches a
+// row; else in CSR form over pent.  Before the block kernel, k_sweep_stage
+// gathers each position's Staged values into position order, so a level's
+// loads are all contiguous and none depends on another.
 #define MPK_BLK_ROWS 8192
-#define MPK_PLAN_W 4
+#define MPK_PLAN_WMAX 32
+#define MPK_PLAN_PF 16  // slots of a row held in registers (loaded a level ahead)
 
 struct PlanView {
   i64 nblk, w;
@@ -200,56 +202,177 @@ __device__ __forceinline__ double plan_value(const Prog &prog, i64 n, const doub
   return prog.stale(-(i64)d - n - 1);
 }
 
-// A position's inputs, loaded one level ahead of their use.
+// A position's inputs, loaded one level ahead of their use: with fixed slots
+// every load is independent of the others (one round); in CSR form only the
+// entry offset is fetched ahead.
+template <int PF>
 struct PlanRow {
   int p, cnt, base;
   i64 i;
   Staged s;
-  double c[MPK_PLAN_W];
-  int d[MPK_PLAN_W];
+  double c[PF];
+  int d[PF];
 };
 
-__device__ __forceinline__ PlanRow plan_row(const PlanView &P, const StagedRows &st, int p, bool valid) {
-  PlanRow r;
+template <int PF>
+__device__ __forceinline__ PlanRow<PF> plan_row(const PlanView &P, const StagedRows &st, int p, bool valid) {
+  PlanRow<PF> r;
   r.p = valid ? p : -1;
   if (!valid) return r;
   r.cnt = P.pcnt[p];
   r.i = P.prow[p];
   r.s = Staged{st.init[p], st.s1[p], st.s2[p], st.o[p]};
-  if (P.w == MPK_PLAN_W) {
-    r.base = p * MPK_PLAN_W;
+  if (P.w) {
+    const int w = (int)P.w;
+    r.base = p * w;
 #pragma unroll
-    for (int e = 0; e < MPK_PLAN_W; e++) {
-      r.c[e] = P.ecoef[(i64)r.base + e];
-      r.d[e] = P.edep[(i64)r.base + e];
-    }
+    for (int e = 0; e < PF; e++)
+      if (e < w) {
+        r.c[e] = P.ecoef[(i64)r.base + e];
+        r.d[e] = P.edep[(i64)r.base + e];
+      }
   } else {
     r.base = P.pent[p];
   }
   return r;
 }
 
-template <class Prog>
-__device__ __forceinline__ void plan_compute(const Prog &prog, i64 n, const PlanView &P, const PlanRow &r,
-                                             double *val, int p0, double *pub) {
-  double acc = r.s.init;
-  if (P.w == MPK_PLAN_W) {
+// The values of up to G entries: every load is issued first (LDS slots and
+// rows of earlier blocks alike), then the rows still holding the sentinel are
+// polled one by one -- so a row waits about one memory round trip, not one
+// per entry.
+// The values of up to G entries, in two phases so that every load is in
+// flight before any is waited on.  Issue: the LDS slot is read for every
+// entry (index clamped: a different register from the global loads, so the
+// compiler need not drain the global loads first) and the rows of earlier
+// blocks are loaded; resolve: pick each entry's value, polling rows still
+// holding the sentinel.
+template <int G>
+struct Gather {
+  double lv[G];
+  unsigned long long gb[G];
+};
+
+template <int G>
+__device__ __forceinline__ void gather_issue(i64 n, const double *val, const double *pub, const int *d, int cnt,
+                                             Gather<G> &g) {
 #pragma unroll
-    for (int e = 0; e < MPK_PLAN_W; e++)
-      if (e < r.cnt) acc = fma(r.c[e], plan_value(prog, n, val, pub, r.d[e]), acc);
-  } else {
-    for (int e = r.base, ee = r.base + r.cnt; e < ee; e++)
-      acc = fma(P.ecoef[e], plan_value(prog, n, val, pub, P.edep[e]), acc);
+  for (int e = 0; e < G; e++) {
+    const int de = e < cnt ? d[e] : 0;
+    g.lv[e] = val[de >= 0 ? de : 0];
+    g.gb[e] = 0;
+    if (de < 0 && de >= -n) g.gb[e] = load_pub(pub, -(i64)de - 1);
+  }
+}
+
+template <int G, class Prog>
+__device__ __forceinline__ void gather_resolve(const Prog &prog, i64 n, const double *pub, const int *d, int cnt,
+                                               const Gather<G> &g, double *v) {
+#pragma unroll
+  for (int e = 0; e < G; e++) {
+    if (e < cnt) {
+      const int de = d[e];
+      if (de >= 0) {
+        v[e] = g.lv[e];
+      } else if (de >= -n) {
+        unsigned long long b = g.gb[e];
+        while (b == MPK_SENTINEL) {
+          __builtin_amdgcn_s_sleep(1);
+          b = load_pub(pub, -(i64)de - 1);
+        }
+        v[e] = __longlong_as_double((long long)b);
+      } else {
+        v[e] = prog.stale(-(i64)de - n - 1);
+      }
+    }
+  }
+}
+
+template <int G, class Prog>
+__device__ __forceinline__ void plan_gather(const Prog &prog, i64 n, const double *val, const double *pub,
+                                            const int *d, int cnt, double *v) {
+  Gather<G> g;
+  gather_issue<G>(n, val, pub, d, cnt, g);
+  gather_resolve<G>(prog, n, pub, d, cnt, g, v);
+}
+
+// A row's sum and finish once its slots' values are gathered (register part
+// in g); entries beyond the registers (or CSR form) are gathered here.
+template <int PF, class Prog>
+__device__ __forceinline__ void plan_finish(const Prog &prog, i64 n, const PlanView &P, const PlanRow<PF> &r,
+                                            const Gather<PF> &g, double *val, int p0, double *pub) {
+  double acc = r.s.init;
+  int e_rest = r.base;  // entries still to do (beyond the registers, or CSR form)
+  if (P.w) {
+    double v[PF];
+    gather_resolve<PF>(prog, n, pub, r.d, r.cnt, g, v);
+#pragma unroll
+    for (int e = 0; e < PF; e++)
+      if (e < r.cnt) acc = fma(r.c[e], v[e], acc);
+    e_rest = r.base + PF;
+  }
+  for (int e0 = e_rest, ee = r.base + r.cnt; e0 < ee; e0 += 8) {
+    const int k = ee - e0 < 8 ? ee - e0 : 8;
+    int d[8];
+    double c[8], v[8];
+#pragma unroll
+    for (int e = 0; e < 8; e++) {
+      d[e] = e < k ? P.edep[e0 + e] : 0;
+      c[e] = e < k ? P.ecoef[e0 + e] : 0.0;
+    }
+    plan_gather<8>(prog, n, val, pub, d, k, v);
+#pragma unroll
+    for (int e = 0; e < 8; e++)
+      if (e < k) acc = fma(c[e], v[e], acc);
   }
   const double v = prog.fin(r.i, acc, r.s);
   val[r.p - p0] = v;
   store_pub(pub, r.i, v);
 }
 
+template <int PF, class Prog>
+__device__ __forceinline__ void plan_compute(const Prog &prog, i64 n, const PlanView &P, const PlanRow<PF> &r,
+                                             double *val, int p0, double *pub) {
+  Gather<PF> g;
+  if (P.w) gather_issue<PF>(n, val, pub, r.d, r.cnt, g);
+  plan_finish<PF>(prog, n, P, r, g, val, p0, pub);
+}
+
+// The levels of block b, holding PF slots of each row in registers.
+template <int PF, class Prog>
+__device__ __forceinline__ void plan_levels(const Prog &prog, i64 n, const PlanView &P, const StagedRows &st,
+                                            i64 b, double *val, double *pub) {
+  const int t = (int)threadIdx.x;
+  const int p0 = P.blk_pos[b];
+  const int l0 = P.blk_lev[b], l1 = P.blk_lev[b + 1] - 1;  // levels l0..l1-1; lvl[l1] ends the block
+  // level bounds run one level ahead of the rows' prefetch: at level l the
+  // thread holds lvl[l], lvl[l+1], lvl[l+2] and fetches lvl[l+3]
+  int pa = P.lvl[l0], pb = P.lvl[l0 + 1];
+  int pc = l0 + 1 < l1 ? P.lvl[l0 + 2] : pb;
+  PlanRow<PF> cur = plan_row<PF>(P, st, pa + t, pa + t < pb);
+  for (int l = l0; l < l1; l++) {
+    const bool more = l + 1 < l1;
+    const int pd = l + 2 < l1 ? P.lvl[l + 3] : pc;
+    // this level's value loads, then the next level's prefetch, then the sums
+    Gather<PF> g;
+    if (cur.p >= 0 && P.w) gather_issue<PF>(n, val, pub, cur.d, cur.cnt, g);
+    const PlanRow<PF> nxt = plan_row<PF>(P, st, pb + t, more && pb + t < pc);
+    if (cur.p >= 0) plan_finish<PF>(prog, n, P, cur, g, val, p0, pub);
+    for (int p = pa + t + 256; p < pb; p += 256)
+      plan_compute<PF>(prog, n, P, plan_row<PF>(P, st, p, true), val, p0, pub);
+    __syncthreads();
+    cur = nxt;
+    pa = pb;
+    pb = pc;
+    pc = pd;
+  }
+}
+
 // One workgroup per block.  Each thread's first row of the next level is
 // loaded while the current level computes (the barrier waits on LDS only, so
 // those loads stay in flight across it); further rows of a wide level are
-// loaded when they run.
+// loaded when they run.  Plans whose rows fit 4 slots keep 4 in registers,
+// others 16 (the rest of a longer row is loaded when it runs).
 template <class Prog>
 __global__ void __launch_bounds__(256) k_sweep_blk(Prog prog, i64 n, const char *plan, int second, StagedRows st,
                                                    unsigned *ticket, double *pub, const int64_t *stop) {
@@ -262,22 +385,10 @@ __global__ void __launch_bounds__(256) k_sweep_blk(Prog prog, i64 n, const char 
   const PlanView P = plan_view(plan, n);
   const i64 b = tk_sh;
   if (b >= P.nblk) return;
-  const int t = (int)threadIdx.x;
-  const int p0 = P.blk_pos[b];
-  const int l0 = P.blk_lev[b], l1 = P.blk_lev[b + 1] - 1;  // levels l0..l1-1; lvl[l1] ends the block
-  int pa = P.lvl[l0], pb = P.lvl[l0 + 1];
-  PlanRow cur = plan_row(P, st, pa + t, pa + t < pb);
-  for (int l = l0; l < l1; l++) {
-    const bool more = l + 1 < l1;
-    const int pc = more ? P.lvl[l + 2] : pb;
-    const PlanRow nxt = plan_row(P, st, pb + t, more && pb + t < pc);
-    if (cur.p >= 0) plan_compute(prog, n, P, cur, val, p0, pub);
-    for (int p = pa + t + 256; p < pb; p += 256) plan_compute(prog, n, P, plan_row(P, st, p, true), val, p0, pub);
-    __syncthreads();
-    cur = nxt;
-    pa = pb;
-    pb = pc;
-  }
+  if (P.w == 4)
+    plan_levels<4>(prog, n, P, st, b, val, pub);
+  else
+    plan_levels<MPK_PLAN_PF>(prog, n, P, st, b, val, pub);
 }
 
 // Entries of a sweep in the rows' own order, for the host-side planner:

@@ -102,6 +102,12 @@ SIGNATURES = {
     "mpk_sell_size": ([I, I], I),
     "mpk_sell_build": ([I] * 7, I),
     "mpk_sell_mult": ([I] * 8, None),
+    "mpk_batch_create": ([I] * 20, I),
+    "mpk_batch_solve": ([I] * 5, I),
+    "mpk_batch_free": ([I], None),
+    "mpk_mixed_create": ([I] * 13, I),
+    "mpk_mixed_solve": ([I] * 4, I),
+    "mpk_mixed_free": ([I], None),
     "mpk_pc_schedule_size": ([I] * 3, I),
     "mpk_pc_schedule": ([I] * 5, I),
     "mpk_sweep_plan_bound": ([I] * 3, I),

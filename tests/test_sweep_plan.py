@@ -102,7 +102,11 @@ def _restated(n, forward, off, col, coef, bmax):
             else:
                 edep.append(-k - 1)
         pent.append(len(edep))
-    w = 4 if max((off[i + 1] - off[i] for i in range(n)), default=0) <= 4 else 0
+    widest = max((off[i + 1] - off[i] for i in range(n)), default=0)
+    w = 4
+    while w < widest:
+        w *= 2
+    w = w if w <= 32 else 0
     return dict(nblk=nblk, w=w, blk_pos=np.array(blk_pos), blk_lev=np.array(blk_lev), lvl=np.array(lvl),
                 prow=np.array(prow), pent=np.array(pent), ecoef=np.array(ecoef), edep=np.array(edep))
 
@@ -175,6 +179,7 @@ SMALL = {
     "tri40": lambda: W.tridiagonal(40),
     "rand300": lambda: _random(300, 5, 7),
     "rand200w": lambda: _random(200, 14, 9),
+    "rand200x": lambda: _random(200, 70, 10),
 }
 
 
