@@ -483,6 +483,47 @@ struct OpCgXR {
     st2(r, e, n, al, D2{fma(-a, wv.a, rv.a), fma(-a, wv.b, rv.b)});
   }
 };
+// CG with Jacobi, one pass for the second half of an iteration:
+// x = x + a p, r = r - a w (OpCgXR), z = r .* diag (PCApply_Jacobi), and the
+// tile partials of z.z, r.r, z.r (k_finalize folds them).  Each thread holds
+// the elements the tile reduction gives it, so every value and every partial
+// is the one the separate kernels produce.  w may alias z (the reference's CG
+// keeps W = Z).
+__global__ void __launch_bounds__(MPK_NT) k_cg_xrz(i64 n, double *x, double *r, double *z, const double *p,
+                                                   const double *w, const double *dg, const KspState *s, double *P,
+                                                   bool al, const int64_t *stop) {
+  if (stop && *stop) return;
+  __shared__ double sh[4];
+  const double a = s->a;
+  const i64 base = (i64)blockIdx.x * MPK_TILE + 2 * (i64)threadIdx.x;
+  double lz[4], lr[4], lzr[4];
+#pragma unroll
+  for (int j = 0; j < 4; j++) {
+    const i64 e = base + 512 * j;
+    D2 rn{0.0, 0.0}, zn{0.0, 0.0};
+    if (e < n) {
+      const D2 xv = ld2(x, e, n, al), pv = ld2(p, e, n, al), rv = ld2(r, e, n, al), wv = ld2(w, e, n, al),
+               dv = ld2(dg, e, n, al);
+      st2(x, e, n, al, D2{fma(a, pv.a, xv.a), fma(a, pv.b, xv.b)});
+      rn = D2{fma(-a, wv.a, rv.a), fma(-a, wv.b, rv.b)};
+      if (e + 1 >= n) rn.b = 0.0;
+      st2(r, e, n, al, rn);
+      zn = D2{rn.a * dv.a, rn.b * dv.b};
+      st2(z, e, n, al, zn);
+    }
+    lz[j] = fma(zn.a, zn.a, zn.b * zn.b);
+    lr[j] = fma(rn.a, rn.a, rn.b * rn.b);
+    lzr[j] = fma(zn.a, rn.a, zn.b * rn.b);
+  }
+  const double u[3] = {(lz[0] + lz[1]) + (lz[2] + lz[3]), (lr[0] + lr[1]) + (lr[2] + lr[3]),
+                       (lzr[0] + lzr[1]) + (lzr[2] + lzr[3])};
+#pragma unroll
+  for (int k = 0; k < 3; k++) {
+    const double t = block_tree(u[k], sh);
+    if (threadIdx.x == 0) P[(i64)k * gridDim.x + blockIdx.x] = t;
+  }
+}
+
 // p = (1 r + cz v) + beta p  (VecAXPBYPCZ(P, 1, -omegaold beta, beta, R, V))
 struct OpBcgsP {
   double *p;
@@ -904,9 +945,20 @@ void mp_ksp_solve_cg(int64_t n, int64_t irow, int64_t icol, int64_t data, int64_
       ew2(n, OpCgP{P, Z, S.d, aligned16(P) && aligned16(Z)}, st);
       spmv(S.A, P, W, 0, st);
       S.reduce<1>(RedSet<1>{{{P, W, RK_DOT}}}, EPI_CG_DPI, 0, st);
-      ew2(n, OpCgXR{X, R, P, W, S.d, aligned16(X) && aligned16(R) && aligned16(P) && aligned16(W)}, st);
-      S.pc.apply(R, Z, st);
-      S.reduce<3>(RedSet<3>{{{Z, Z, RK_SUMSQ}, {R, R, RK_SUMSQ}, {Z, R, RK_DOT}}}, EPI_CG_NORM, 0, st);
+      const bool al = aligned16(X) && aligned16(R) && aligned16(P) && aligned16(W);
+      if (S.pc.type == PCT_JACOBI && n > 0) {
+        // x, r, the Jacobi apply and the three reductions in one pass
+        const unsigned T = ntiles(n);
+        double *Pp = partials((size_t)3 * T);
+        MPK_LAUNCH(k_cg_xrz, T, MPK_NT, n, X, R, Z, (const double *)P, (const double *)W, S.pc.jac_diag,
+                   (const KspState *)S.d, Pp, al && aligned16(Z) && aligned16(S.pc.jac_diag), st);
+        MPK_LAUNCH(k_finalize<KspEpi>, 1, MPK_NT, (const double *)Pp, (int)T, 3, (double *)nullptr,
+                   KspEpi{S.d, EPI_CG_NORM, 0}, st);
+      } else {
+        ew2(n, OpCgXR{X, R, P, W, S.d, al}, st);
+        S.pc.apply(R, Z, st);
+        S.reduce<3>(RedSet<3>{{{Z, Z, RK_SUMSQ}, {R, R, RK_SUMSQ}, {Z, R, RK_DOT}}}, EPI_CG_NORM, 0, st);
+      }
     }
     S.mark(slot);
     if (pending && S.done(slot ^ 1)) break;

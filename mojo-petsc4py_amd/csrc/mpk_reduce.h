@@ -142,6 +142,8 @@ __global__ void __launch_bounds__(MPK_NT) k_finalize(const double *P, int T, int
 }
 
 // Host side: launch the partials for the K specs and the finalize with `epi`.
+// (A last-block finalize inside the partials launch needs a device-scope fence
+// per block, an L2 writeback on this GPU, which costs more than the launch.)
 template <int K, class Epi>
 void reduce_launch(const RedSet<K> &rs, i64 n, double *out, Epi epi, const int64_t *stop) {
   unsigned T = ntiles(n);
