@@ -425,14 +425,16 @@ static int64_t plan_rows(int64_t n, int64_t bmax) {
   b = b < 1024 ? 1024 : b;
   return b > bmax ? bmax : b;
 }
-// Entries sit in w fixed slots per position, w the power of two from 4 to
-// this (MPK_PLAN_WMAX in mpk_sweep.h) that holds the longest row; sweeps with
-// longer rows keep CSR form (w = 0).
-static const int64_t kPlanWMax = 32;
-static int64_t plan_w(int64_t widest) {
+// Entries sit in w fixed slots per position: w the power of two from 4 to 32
+// that holds the longest row, else the multiple of 8 up to 64 that holds it
+// when n w stays within 4 times the entry count; other sweeps keep CSR form
+// (w = 0).
+static int64_t plan_w(int64_t widest, int64_t n, int64_t m) {
   int64_t w = 4;
-  while (w < widest) w *= 2;
-  return w <= kPlanWMax ? w : 0;
+  while (w < widest && w < 32) w *= 2;
+  if (w >= widest) return w;
+  w = (widest + 7) / 8 * 8;
+  return w <= 64 && n * w <= 4 * m ? w : 0;
 }
 static int64_t plan_bytes(int64_t n, int64_t nblk, int64_t nlvl, int64_t slots) {
   const int64_t ints = 2 * (nblk + 1) + nlvl + 3 * n + 1;
@@ -442,7 +444,28 @@ static int64_t plan_bytes(int64_t n, int64_t nblk, int64_t nlvl, int64_t slots) 
 // Upper bound of the plan size in bytes for n rows and m entries.
 int64_t mpk_sweep_plan_bound(int64_t n, int64_t m, int64_t bmax) {
   const int64_t B = plan_rows(n, bmax), nblk = n > 0 ? (n + B - 1) / B : 0;
-  return plan_bytes(n, nblk, n + nblk, std::max(m, kPlanWMax * n));
+  return plan_bytes(n, nblk, n + nblk, std::max(4 * m, 32 * n));
+}
+
+// Ring plans.  A sweep too long for one block whose DAG is deep and narrow
+// (a few hundred rows per level) spends its time waiting on other blocks'
+// rows; such a sweep runs instead as one block over global levels, its LDS
+// holding the values of the last W positions (slot = position mod W, W the
+// largest power of two <= bmax).  Rows sit at their latest possible level
+// (the number of levels is the DAG's depth), which keeps each value close to
+// its first reader.  A value less than W positions behind the end of the
+// reading row's level is read from its slot; an older one (its slot may
+// have been rewritten) from the published values, like a row of an earlier
+// block.  Chosen when the average level holds at most kRingWidth rows and
+// at most one entry in 64 reads an old value; MPK_SWEEP_PLAN=blocks or ring
+// forces the choice (a ring only when n > bmax).
+static const int64_t kRingWidth = 1024;
+static int plan_mode() {
+  const char *e = std::getenv("MPK_SWEEP_PLAN");
+  if (!e) return 0;
+  if (!std::strcmp(e, "blocks")) return 1;
+  if (!std::strcmp(e, "ring")) return 2;
+  return 0;
 }
 
 // Writes the plan into out [H] (cap bytes).  Returns its size in bytes, or
@@ -454,9 +477,94 @@ int64_t mpk_sweep_plan(int64_t n, int64_t forward, int64_t off_, int64_t col_, i
   const double *coef = (const double *)coef_;
   char *out = (char *)out_;
   const int64_t m = n > 0 ? off[n] : 0;
-  const int64_t B = plan_rows(n, bmax), nblk = n > 0 ? (n + B - 1) / B : 0;
   auto spos = [&](int64_t i) { return forward ? i : n - 1 - i; };  // sweep position of row i
   auto earlier = [&](int64_t k, int64_t i) { return forward ? k < i : k > i; };
+  int64_t widest = 0;
+  for (int64_t i = 0; i < n; i++) widest = std::max(widest, off[i + 1] - off[i]);
+  const int64_t w = plan_w(widest, n, m), slots = w ? w * n : m;
+  const int mode = plan_mode();
+  if (n > bmax && mode != 1) {
+    int64_t W = 1;
+    while (2 * W <= bmax) W *= 2;
+    // earliest levels (their count is the depth), then latest levels
+    std::vector<int> lev((size_t)n, 0);
+    int64_t nl = 0;
+    for (int64_t s = 0; s < n; s++) {
+      const int64_t i = forward ? s : n - 1 - s;
+      int l = 0;
+      for (int64_t e = off[i]; e < off[i + 1]; e++)
+        if (earlier(col[e], i)) l = std::max(l, lev[col[e]] + 1);
+      lev[i] = l;
+      nl = std::max<int64_t>(nl, l + 1);
+    }
+    std::fill(lev.begin(), lev.end(), (int)(nl - 1));
+    for (int64_t s = n - 1; s >= 0; s--) {
+      const int64_t i = forward ? s : n - 1 - s;
+      for (int64_t e = off[i]; e < off[i + 1]; e++)
+        if (earlier(col[e], i)) lev[col[e]] = std::min(lev[col[e]], lev[i] - 1);
+    }
+    std::vector<int64_t> start((size_t)nl + 1, 0);
+    for (int64_t i = 0; i < n; i++) start[(size_t)lev[i] + 1]++;
+    for (int64_t l = 0; l < nl; l++) start[(size_t)l + 1] += start[(size_t)l];
+    std::vector<int> pos((size_t)n), prow_h((size_t)n);
+    {
+      std::vector<int64_t> next(start.begin(), start.end() - 1);
+      for (int64_t s = 0; s < n; s++) {
+        const int64_t i = forward ? s : n - 1 - s;
+        const int64_t p = next[(size_t)lev[i]]++;
+        pos[(size_t)i] = (int)p;
+        prow_h[(size_t)p] = (int)i;
+      }
+    }
+    auto in_window = [&](int64_t i, int64_t k) { return start[(size_t)lev[i] + 1] - pos[(size_t)k] <= W; };
+    int64_t old = 0;
+    for (int64_t i = 0; i < n; i++)
+      for (int64_t e = off[i]; e < off[i + 1]; e++)
+        if (earlier(col[e], i) && !in_window(i, col[e])) old++;
+    if (mode == 2 || (n <= kRingWidth * nl && 64 * old <= m)) {
+      const int64_t nlvl = nl + 1, bytes = plan_bytes(n, 1, nlvl, slots);
+      if (bytes > cap) return -bytes;
+      std::memset(out, 0, (size_t)bytes);
+      int64_t *hdr = (int64_t *)out;
+      hdr[0] = 1;
+      hdr[1] = m;
+      hdr[2] = nlvl;
+      hdr[3] = nl;
+      hdr[4] = bytes;
+      hdr[5] = w;
+      hdr[6] = W;
+      int *blk_pos = (int *)(out + 64), *blk_lev = blk_pos + 2, *lvl = blk_lev + 2;
+      int *prow = lvl + nlvl, *pcnt = prow + n, *pent = pcnt + n;
+      const int64_t ints = 2 * 2 + nlvl + 3 * n + 1;
+      double *ecoef = (double *)(out + 64 + ((4 * ints + 7) & ~(int64_t)7));
+      int *edep = (int *)(ecoef + slots);
+      blk_pos[0] = 0;
+      blk_pos[1] = (int)n;
+      blk_lev[0] = 0;
+      blk_lev[1] = (int)nlvl;
+      for (int64_t l = 0; l <= nl; l++) lvl[l] = (int)start[(size_t)l];
+      int64_t e_out = 0;
+      for (int64_t p = 0; p < n; p++) {
+        const int64_t i = prow_h[(size_t)p];
+        prow[p] = (int)i;
+        pent[p] = w ? 0 : (int)e_out;
+        pcnt[p] = (int)(off[i + 1] - off[i]);
+        for (int64_t e = off[i]; e < off[i + 1]; e++, e_out++) {
+          const int64_t k = col[e], q = w ? (e - off[i]) * n + p : e_out;  // slot-major with fixed slots
+          ecoef[q] = coef[e];
+          if (!earlier(k, i))
+            edep[q] = (int)(-k - n - 1);
+          else if (in_window(i, k))
+            edep[q] = (int)(pos[(size_t)k] & (W - 1));
+          else
+            edep[q] = (int)(-k - 1);
+        }
+      }
+      pent[n] = w ? 0 : (int)e_out;
+      return bytes;
+    }
+  }
+  const int64_t B = plan_rows(n, bmax), nblk = n > 0 ? (n + B - 1) / B : 0;
   // local levels
   std::vector<int> lev(n > 0 ? n : 1, 0);
   std::vector<int> nlev(nblk > 0 ? nblk : 1, 0);
@@ -470,13 +578,11 @@ int64_t mpk_sweep_plan(int64_t n, int64_t forward, int64_t off_, int64_t col_, i
     lev[i] = l;
     nlev[b] = std::max(nlev[b], l + 1);
   }
-  int64_t nlvl = 0, depth = 0, widest = 0;
+  int64_t nlvl = 0, depth = 0;
   for (int64_t b = 0; b < nblk; b++) {
     nlvl += nlev[b] + 1;
     depth = std::max<int64_t>(depth, nlev[b]);
   }
-  for (int64_t i = 0; i < n; i++) widest = std::max(widest, off[i + 1] - off[i]);
-  const int64_t w = plan_w(widest), slots = w ? w * n : m;
   const int64_t bytes = plan_bytes(n, nblk, nlvl, slots);
   if (bytes > cap) return -bytes;
   std::memset(out, 0, (size_t)bytes);
@@ -515,22 +621,22 @@ int64_t mpk_sweep_plan(int64_t n, int64_t forward, int64_t off_, int64_t col_, i
   }
   blk_pos[nblk] = (int)n;
   blk_lev[nblk] = (int)li;
-  // entries in position order (fixed slots: entry e of position p at p w + e)
+  // entries in position order (fixed slots: entry e of position p at e n + p,
+  // so that the rows of a level read each slot from consecutive addresses)
   int64_t e_out = 0;
   for (int64_t p = 0; p < n; p++) {
     const int64_t i = prow[p], b = spos(i) / B;
     pent[p] = (int)e_out;
     pcnt[p] = (int)(off[i + 1] - off[i]);
-    if (w) e_out = p * w;
     for (int64_t e = off[i]; e < off[i + 1]; e++, e_out++) {
-      const int64_t k = col[e];
-      ecoef[e_out] = coef[e];
+      const int64_t k = col[e], q = w ? (e - off[i]) * n + p : e_out;
+      ecoef[q] = coef[e];
       if (!earlier(k, i))
-        edep[e_out] = (int)(-k - n - 1);
+        edep[q] = (int)(-k - n - 1);
       else if (spos(k) / B == b)
-        edep[e_out] = pos[k] - blk_pos[b];
+        edep[q] = pos[k] - blk_pos[b];
       else
-        edep[e_out] = (int)(-k - 1);
+        edep[q] = (int)(-k - 1);
     }
   }
   pent[n] = (int)(w ? 0 : e_out);

@@ -280,19 +280,21 @@ __global__ void __launch_bounds__(256) k_sweep_lp(Prog prog, i64 n, const char *
       const i64 i = P.prow[p];
       const LpStaged st = prog.stage(i);
       const int cnt = P.pcnt[p];
-      const int base = P.w ? p * (int)P.w : P.pent[p];
+      const i64 base = P.w ? p : P.pent[p], stride = P.w ? P.n : 1;  // entry j at base + j stride
       float acc = 0.0f;
-      for (int e0 = base, ee = base + cnt; e0 < ee; e0 += 8) {
+      for (int j0 = 0; j0 < cnt; j0 += 8) {
         // issue every load of the chunk, then poll what is not published yet
         // issue every load of the chunk (LDS and global into separate
         // registers), then resolve, polling what is not published yet
-        const int k = ee - e0 < 8 ? ee - e0 : 8;
+        const int k = cnt - j0 < 8 ? cnt - j0 : 8;
         int d[8];
         float lv[8];
         unsigned gb[8];
+        i64 q[8];
 #pragma unroll
         for (int e = 0; e < 8; e++) {
-          d[e] = e < k ? P.edep[e0 + e] : 0;
+          q[e] = base + (i64)(j0 + e) * stride;
+          d[e] = e < k ? P.edep[q[e]] : 0;
           lv[e] = val[d[e] >= 0 ? d[e] : 0];
           gb[e] = 0u;
           if (d[e] < 0 && d[e] >= -n) gb[e] = load_pub_f(pub, -(i64)d[e] - 1);
@@ -313,12 +315,12 @@ __global__ void __launch_bounds__(256) k_sweep_lp(Prog prog, i64 n, const char *
             } else {
               v = prog.stale(-(i64)d[e] - n - 1);
             }
-            acc = fmaf(lp_f(lc[e0 + e]), v, acc);
+            acc = fmaf(lp_f(lc[q[e]]), v, acc);
           }
         }
       }
       const float r = prog.fin(i, acc, st);
-      val[p - p0] = r;
+      val[(p - p0) & P.mask] = r;
       store_pub_f(pub, i, r);
     }
     __syncthreads();
@@ -368,8 +370,11 @@ __global__ void k_lp_plan_coef(i64 n, const char *plan, int second, const float 
   const i64 p = (i64)blockIdx.x * 256 + threadIdx.x;
   if (p >= n) return;
   const double s = rs ? (double)rs[P.prow[p]] : 1.0;
-  const int base = P.w ? (int)(p * P.w) : P.pent[p];
-  for (int e = base; e < base + P.pcnt[p]; e++) lc[e] = lp_from<CT>((float)(P.ecoef[e] / s));
+  const i64 base = P.w ? p : P.pent[p], stride = P.w ? n : 1;
+  for (int j = 0; j < P.pcnt[p]; j++) {
+    const i64 q = base + (i64)j * stride;
+    lc[q] = lp_from<CT>((float)(P.ecoef[q] / s));
+  }
 }
 
 // FP32 inverted diagonals: ILU's stored 1/u_ii; SOR's as MatInvertDiagonalForSOR.

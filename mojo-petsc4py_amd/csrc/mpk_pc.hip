@@ -24,7 +24,10 @@ namespace mpk {
 __global__ void k_sweep_prepare(i64 n, double *p, unsigned *ticket, const int64_t *stop) {
   if (stop && *stop) return;
   i64 i = (i64)blockIdx.x * 256 + threadIdx.x;
-  if (i == 0) *ticket = 0u;
+  if (i == 0) {
+    ticket[0] = 0u;  // block order of a block plan
+    ticket[1] = 0u;  // progress of a ring plan
+  }
   if (i < n) p[i] = __longlong_as_double((long long)MPK_SENTINEL);
 }
 

@@ -134,24 +134,33 @@ __global__ void __launch_bounds__(64) k_sweep(Prog prog, i64 n, int forward, uns
 // its entry count, and its entries in the row's order: the coefficient and
 // where the value comes from -- d >= 0: LDS slot d; -n <= d < 0: row -d-1 of
 // an earlier block; d < -n: row -d-n-1, not earlier in the sweep (stale,
-// malformed input only).  When no row has more than MPK_PLAN_WMAX entries
-// they sit in w fixed slots per position (w = 4, 8, 16 or 32, the smallest
-// that holds the longest row), so one round of independent loads fetches a
-// row; else in CSR form over pent.  Before the block kernel, k_sweep_stage
+// malformed input only).  Entries sit in w fixed slots per position (w = 4,
+// 8, 16 or 32, the smallest that holds the longest row; for longer rows a
+// multiple of 8 up to 64 when the padding stays within 4 times the entries),
+// slot e of position p at e n + p: the loads of a row are independent of each
+// other and the rows of a level (consecutive positions, one per lane) read
+// each slot from consecutive addresses.  Else in CSR form over pent.  Before the block kernel, k_sweep_stage
 // gathers each position's Staged values into position order, so a level's
 // loads are all contiguous and none depends on another.
 #define MPK_BLK_ROWS 8192
-#define MPK_PLAN_WMAX 32
 #define MPK_PLAN_PF 16  // slots of a row held in registers (loaded a level ahead)
 
 struct PlanView {
-  i64 nblk, w;
+  i64 n, nblk, w;
+  int mask;  // LDS slot of position p in block b: (p - blk_pos[b]) & mask
   const int *blk_pos, *blk_lev, *lvl, *prow, *pcnt, *pent, *edep;
   const double *ecoef;
 };
 
+// A ring plan (hdr[6] = W > 0) is one block over the sweep's global levels
+// whose LDS keeps the values of the last W positions: position p lives in
+// slot p mod W.  A row reads a value from its slot when it is less than W
+// positions behind the end of the row's level (so no row of that level can
+// have rewritten the slot), else from `pub` as if from an earlier block.
+// Deep, narrow sweeps run that way, with no waits on other blocks.
+//
 // Layout of one sweep's plan: i64 hdr[8] = {nblk, nent, nlvl, depth, bytes,
-// w, 0, 0}, then int32 blk_pos[nblk+1], blk_lev[nblk+1], lvl[nlvl], prow[n],
+// w, W (0: blocks), 0}, then int32 blk_pos[nblk+1], blk_lev[nblk+1], lvl[nlvl], prow[n],
 // pcnt[n], pent[n+1], padding to 8, double ecoef[slots], int32 edep[slots]
 // with slots = w ? n w : nent.  The plans of a PC's two sweeps sit back to
 // back.
@@ -159,8 +168,10 @@ __host__ __device__ inline i64 plan_align8(i64 x) { return (x + 7) & ~(i64)7; }
 __host__ __device__ inline PlanView plan_view(const char *base, i64 n) {
   const i64 *h = reinterpret_cast<const i64 *>(base);
   PlanView v;
+  v.n = n;
   v.nblk = h[0];
   v.w = h[5];
+  v.mask = h[6] ? (int)(h[6] - 1) : -1;
   const i64 nent = h[1], nlvl = h[2], slots = v.w ? n * v.w : nent;
   v.blk_pos = reinterpret_cast<const int *>(base + 64);
   v.blk_lev = v.blk_pos + v.nblk + 1;
@@ -223,13 +234,12 @@ __device__ __forceinline__ PlanRow<PF> plan_row(const PlanView &P, const StagedR
   r.i = P.prow[p];
   r.s = Staged{st.init[p], st.s1[p], st.s2[p], st.o[p]};
   if (P.w) {
-    const int w = (int)P.w;
-    r.base = p * w;
+    r.base = p;
 #pragma unroll
     for (int e = 0; e < PF; e++)
-      if (e < w) {
-        r.c[e] = P.ecoef[(i64)r.base + e];
-        r.d[e] = P.edep[(i64)r.base + e];
+      if (e < P.w) {
+        r.c[e] = P.ecoef[(i64)e * P.n + p];
+        r.d[e] = P.edep[(i64)e * P.n + p];
       }
   } else {
     r.base = P.pent[p];
@@ -302,23 +312,25 @@ template <int PF, class Prog>
 __device__ __forceinline__ void plan_finish(const Prog &prog, i64 n, const PlanView &P, const PlanRow<PF> &r,
                                             const Gather<PF> &g, double *val, int p0, double *pub) {
   double acc = r.s.init;
-  int e_rest = r.base;  // entries still to do (beyond the registers, or CSR form)
+  int j_rest = 0;  // entries still to do (beyond the registers, or CSR form)
   if (P.w) {
     double v[PF];
     gather_resolve<PF>(prog, n, pub, r.d, r.cnt, g, v);
 #pragma unroll
     for (int e = 0; e < PF; e++)
       if (e < r.cnt) acc = fma(r.c[e], v[e], acc);
-    e_rest = r.base + PF;
+    j_rest = PF;
   }
-  for (int e0 = e_rest, ee = r.base + r.cnt; e0 < ee; e0 += 8) {
-    const int k = ee - e0 < 8 ? ee - e0 : 8;
+  const i64 stride = P.w ? P.n : 1;  // entry j of the row at base + j stride
+  for (int j0 = j_rest; j0 < r.cnt; j0 += 8) {
+    const int k = r.cnt - j0 < 8 ? r.cnt - j0 : 8;
     int d[8];
     double c[8], v[8];
 #pragma unroll
     for (int e = 0; e < 8; e++) {
-      d[e] = e < k ? P.edep[e0 + e] : 0;
-      c[e] = e < k ? P.ecoef[e0 + e] : 0.0;
+      const i64 q = r.base + (i64)(j0 + e) * stride;
+      d[e] = e < k ? P.edep[q] : 0;
+      c[e] = e < k ? P.ecoef[q] : 0.0;
     }
     plan_gather<8>(prog, n, val, pub, d, k, v);
 #pragma unroll
@@ -326,7 +338,7 @@ __device__ __forceinline__ void plan_finish(const Prog &prog, i64 n, const PlanV
       if (e < k) acc = fma(c[e], v[e], acc);
   }
   const double v = prog.fin(r.i, acc, r.s);
-  val[r.p - p0] = v;
+  val[(r.p - p0) & P.mask] = v;
   store_pub(pub, r.i, v);
 }
 
@@ -338,10 +350,91 @@ __device__ __forceinline__ void plan_compute(const Prog &prog, i64 n, const Plan
   plan_finish<PF>(prog, n, P, r, g, val, p0, pub);
 }
 
-// The levels of block b, holding PF slots of each row in registers.
+// Ring plans run in workgroup 0 while workgroups 8, 16, .., 8 MPK_RING_PF
+// (workgroup 0's XCD under round-robin dispatch) load the data of the levels
+// just ahead of it -- each level's rows, staged values and entry slots -- so
+// that the sweep's own loads find it in the XCD's L2 rather than in HBM.  The
+// sweep publishes its level every MPK_RING_STEP levels (MPK_RING_DONE when it
+// ends) in progress; a prefetcher stays at most MPK_RING_AHEAD levels ahead
+// and returns once the sweep has ended or the solve is stopped.
+#define MPK_RING_PF 8
+#define MPK_RING_AHEAD 16
+#define MPK_RING_STEP 4
+#define MPK_RING_DONE 0x7fffffffu
+
+__device__ __forceinline__ unsigned ring_progress(const unsigned *pr) {
+  return __hip_atomic_load(pr, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void ring_publish(unsigned *pr, unsigned v) {
+  __hip_atomic_store(pr, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Prefetcher k (1..MPK_RING_PF) of a ring plan: levels k-1, k-1+MPK_RING_PF, ..
+__device__ __forceinline__ void ring_prefetch(const PlanView &P, const StagedRows &st, int k, unsigned *progress,
+                                              unsigned *sink, const int64_t *stop) {
+  __shared__ int done_sh;
+  const int t = (int)threadIdx.x;
+  const int nl = P.blk_lev[1] - 1;
+  unsigned acc = 0;
+  for (int l = k - 1; l < nl; l += MPK_RING_PF) {
+    if (t == 0) {
+      unsigned pr = ring_progress(progress);
+      while (pr != MPK_RING_DONE && l > (int)pr + MPK_RING_AHEAD) {
+        if (stop && *stop) {
+          pr = MPK_RING_DONE;
+          break;
+        }
+        __builtin_amdgcn_s_sleep(2);
+        pr = ring_progress(progress);
+      }
+      done_sh = pr == MPK_RING_DONE;
+    }
+    __syncthreads();
+    if (done_sh) break;
+    const int s = P.lvl[l], e = P.lvl[l + 1];
+    for (int p = s + t; p < e; p += 256) {
+      acc ^= (unsigned)P.prow[p] ^ (unsigned)P.pcnt[p] ^ (unsigned)st.o[p];
+      acc ^= (unsigned)__double_as_longlong(st.init[p]) ^ (unsigned)__double_as_longlong(st.s1[p]) ^
+             (unsigned)__double_as_longlong(st.s2[p]);
+    }
+    if (P.w) {
+      // each slot's run of the level's positions: whole lines, one per thread
+      const int per = (e - s + 15) / 16 + 1;  // lines per slot run (coefficients; the indices need half)
+      for (int k2 = t; k2 < (int)P.w * per; k2 += 256) {
+        const i64 j = k2 / per, li = k2 % per;
+        const char *c0 = reinterpret_cast<const char *>(P.ecoef + j * P.n + s);
+        const char *c1 = reinterpret_cast<const char *>(P.ecoef + j * P.n + e);
+        const char *q = reinterpret_cast<const char *>(reinterpret_cast<uintptr_t>(c0) & ~(uintptr_t)127) + 128 * li;
+        if (q < c1) acc ^= *reinterpret_cast<const unsigned *>(q);
+        const char *d0 = reinterpret_cast<const char *>(P.edep + j * P.n + s);
+        const char *d1 = reinterpret_cast<const char *>(P.edep + j * P.n + e);
+        const char *qd = reinterpret_cast<const char *>(reinterpret_cast<uintptr_t>(d0) & ~(uintptr_t)127) + 128 * li;
+        if (qd < d1) acc ^= *reinterpret_cast<const unsigned *>(qd);
+      }
+    } else {
+      // the level's entries are contiguous: whole lines, one per thread
+      const char *c0 = reinterpret_cast<const char *>(P.ecoef + P.pent[s]);
+      const char *c1 = reinterpret_cast<const char *>(P.ecoef + P.pent[e]);
+      const char *d0 = reinterpret_cast<const char *>(P.edep + P.pent[s]);
+      const char *d1 = reinterpret_cast<const char *>(P.edep + P.pent[e]);
+      const char *ca = reinterpret_cast<const char *>(reinterpret_cast<uintptr_t>(c0) & ~(uintptr_t)127);
+      const char *da = reinterpret_cast<const char *>(reinterpret_cast<uintptr_t>(d0) & ~(uintptr_t)127);
+      for (i64 li = t; ca + 128 * li < c1; li += 256) {
+        acc ^= *reinterpret_cast<const unsigned *>(ca + 128 * li);
+        if (da + 128 * li < d1) acc ^= *reinterpret_cast<const unsigned *>(da + 128 * li);
+      }
+    }
+    __syncthreads();
+  }
+  if (acc == 0x9E3779B9u) *sink = acc;  // keeps the loads; the value is never used
+}
+
+// The levels of block b, holding PF slots of each row in registers.  With a
+// progress word (ring plans), thread 0 publishes the level every
+// MPK_RING_STEP levels.
 template <int PF, class Prog>
 __device__ __forceinline__ void plan_levels(const Prog &prog, i64 n, const PlanView &P, const StagedRows &st,
-                                            i64 b, double *val, double *pub) {
+                                            i64 b, double *val, double *pub, unsigned *progress) {
   const int t = (int)threadIdx.x;
   const int p0 = P.blk_pos[b];
   const int l0 = P.blk_lev[b], l1 = P.blk_lev[b + 1] - 1;  // levels l0..l1-1; lvl[l1] ends the block
@@ -351,6 +444,7 @@ __device__ __forceinline__ void plan_levels(const Prog &prog, i64 n, const PlanV
   int pc = l0 + 1 < l1 ? P.lvl[l0 + 2] : pb;
   PlanRow<PF> cur = plan_row<PF>(P, st, pa + t, pa + t < pb);
   for (int l = l0; l < l1; l++) {
+    if (progress && t == 0 && ((l - l0) & (MPK_RING_STEP - 1)) == 0) ring_publish(progress, (unsigned)(l - l0));
     const bool more = l + 1 < l1;
     const int pd = l + 2 < l1 ? P.lvl[l + 3] : pc;
     // this level's value loads, then the next level's prefetch, then the sums
@@ -372,23 +466,40 @@ __device__ __forceinline__ void plan_levels(const Prog &prog, i64 n, const PlanV
 // loaded while the current level computes (the barrier waits on LDS only, so
 // those loads stay in flight across it); further rows of a wide level are
 // loaded when they run.  Plans whose rows fit 4 slots keep 4 in registers,
-// others 16 (the rest of a longer row is loaded when it runs).
+// others 16 (the rest of a longer row is loaded when it runs).  ticket[0]
+// orders the blocks of a block plan; ticket[1] is a ring plan's progress
+// word (both zeroed by k_sweep_prepare).
 template <class Prog>
 __global__ void __launch_bounds__(256) k_sweep_blk(Prog prog, i64 n, const char *plan, int second, StagedRows st,
                                                    unsigned *ticket, double *pub, const int64_t *stop) {
   __shared__ double val[MPK_BLK_ROWS];
   __shared__ unsigned tk_sh;
+  if (second) plan += reinterpret_cast<const i64 *>(plan)[4];
+  const PlanView P = plan_view(plan, n);
+  if (P.mask >= 0) {  // ring plan
+    const unsigned g = blockIdx.x;
+    if (g == 0) {
+      if (!(stop && *stop)) {
+        if (P.w == 4)
+          plan_levels<4>(prog, n, P, st, 0, val, pub, ticket + 1);
+        else
+          plan_levels<MPK_PLAN_PF>(prog, n, P, st, 0, val, pub, ticket + 1);
+      }
+      if (threadIdx.x == 0) ring_publish(ticket + 1, MPK_RING_DONE);
+    } else if (g % 8 == 0 && g / 8 <= MPK_RING_PF) {
+      ring_prefetch(P, st, (int)(g / 8), ticket + 1, ticket + 2, stop);
+    }
+    return;
+  }
   if (threadIdx.x == 0) tk_sh = atomicAdd(ticket, 1u);
   __syncthreads();
   if (stop && *stop) return;
-  if (second) plan += reinterpret_cast<const i64 *>(plan)[4];
-  const PlanView P = plan_view(plan, n);
   const i64 b = tk_sh;
   if (b >= P.nblk) return;
   if (P.w == 4)
-    plan_levels<4>(prog, n, P, st, b, val, pub);
+    plan_levels<4>(prog, n, P, st, b, val, pub, nullptr);
   else
-    plan_levels<MPK_PLAN_PF>(prog, n, P, st, b, val, pub);
+    plan_levels<MPK_PLAN_PF>(prog, n, P, st, b, val, pub, nullptr);
 }
 
 // Entries of a sweep in the rows' own order, for the host-side planner:

@@ -88,8 +88,9 @@ def _plan(L, n, forward, off, col, coef):
     pcnt = ints[o + n:o + 2 * n].copy()
     base = 64 + ((4 * (2 * (nblk + 1) + nlvl + 3 * n + 1) + 7) // 8) * 8
     slots = n * w
-    ecoef = raw[base:base + 8 * slots].view(np.float64).astype(np.float32)
-    edep = raw[base + 8 * slots:base + 12 * slots].view(np.int32).copy()
+    # the plan holds slot e of position p at e n + p; the batched kernel reads p w + e
+    ecoef = raw[base:base + 8 * slots].view(np.float64).reshape(w, n).T.astype(np.float32).ravel()
+    edep = raw[base + 8 * slots:base + 12 * slots].view(np.int32).reshape(w, n).T.copy().ravel()
     return lvl, prow, pcnt, edep, ecoef
 
 

@@ -60,6 +60,9 @@ CASES = [
     ("cd12-ilu-gmres-fp32-jac3", lambda: W.convection_diffusion_3d(12), "ilu", 1.0, "gmres", "fp32", 3),
     ("diff24-ilu-gmres-fp16-jac2", lambda: W.diffusion_3d(24), "ilu", 1.0, "gmres", "fp16", 2),
     ("spd3000-sor-bcgs-fp16-jac3", lambda: W.random_spd(3000, band=400), "sor", 1.2, "bcgs", "fp16", 3),
+    # over 8192 rows with a deep, narrow DAG: ring-planned sweeps
+    ("spd20k-sor-bcgs-fp32", lambda: W.random_spd(20000, band=3000), "sor", 1.2, "bcgs", "fp32", "exact"),
+    ("lap120-ilu-gmres-fp16", lambda: (W.laplacian_2d(120), None), "ilu", 1.0, "gmres", "fp16", "exact"),
 ]
 
 

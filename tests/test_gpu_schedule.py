@@ -1,7 +1,8 @@
-"""Block-planned sweeps: the plans mpk_pc_schedule builds for SOR / ILU / IC
-equal the host planner's output on entries restated from the factor, every
-read in them resolves when its level runs (so a sweep cannot stall), and
-applies through them give the oracle's result bit for bit."""
+"""Planned sweeps (block plans and ring plans): the plans mpk_pc_schedule
+builds for SOR / ILU / IC equal the host planner's output on entries restated
+from the factor, every read in them resolves when its level runs (so a sweep
+cannot stall), and applies through them give the oracle's result bit for
+bit."""
 
 import numpy as np
 import pytest
@@ -57,6 +58,8 @@ MATS = {
     "cd8": lambda: W.convection_diffusion_3d(8)[0],
     "tri50": lambda: W.tridiagonal(50),
     "diff24": lambda: W.diffusion_3d(24)[0],
+    "lap100": lambda: W.laplacian_2d(100),
+    "spd20k": lambda: W.random_spd(20000, band=3000)[0],
 }
 
 
@@ -83,15 +86,31 @@ CASES = [("lap32", "ilu", 0, "natural"), ("diff8", "ilu", 1, "natural"), ("cd8",
          ("lap32", "icc", 0, "natural"), ("diff8", "icc", 2, "natural"), ("diff8", "icc", 0, "rcm"),
          ("tri50", "icc", 1, "natural"), ("lap32", "sor", 0, ""), ("cd8", "sor", 0, ""),
          ("tri50", "ilu", 0, "natural"), ("diff24", "ilu", 0, "natural"), ("diff24", "icc", 0, "natural"),
-         ("diff24", "icc", 1, "rcm"), ("diff24", "sor", 0, "")]
+         ("diff24", "icc", 1, "rcm"), ("diff24", "sor", 0, ""), ("lap100", "ilu", 0, "natural"),
+         ("lap100", "icc", 1, "natural"), ("spd20k", "sor", 0, ""), ("spd20k", "ilu", 0, "natural")]
+# the sweeps of over 8192 rows again, forced into the other kind of plan
+MODES = [("auto", c) for c in CASES] + [
+    (m, c) for m in ("blocks", "ring") for c in CASES if c[0] in ("diff24", "lap100", "spd20k")]
+IDS = [f"{m}-{c[0]}-{c[1]}{c[2]}-{c[3]}" for m, c in MODES]
 
 
-@pytest.mark.parametrize("name,kind,levels,ordering", CASES)
-def test_plan_matches_host_planner(native, name, kind, levels, ordering):
+@pytest.fixture
+def plan_mode(monkeypatch, request):
+    mode = request.param
+    if mode != "auto":
+        monkeypatch.setenv("MPK_SWEEP_PLAN", mode)
+    else:
+        monkeypatch.delenv("MPK_SWEEP_PLAN", raising=False)
+    return mode
+
+
+@pytest.mark.parametrize("plan_mode,case", MODES, ids=IDS, indirect=["plan_mode"])
+def test_plan_matches_host_planner(native, plan_mode, case):
     """The plans the library builds from the device factor equal the host
     planner's output on entries restated here from the host factor, byte for
     byte, and every read in them is resolvable when its level runs."""
     from test_sweep_plan import _check, _unpack
+    name, kind, levels, ordering = case
     _, A, pc = _setup(name, kind, levels, ordering)
     n = A.getSize()[0]
     raw = back(pc._dbufs["sched"], np.uint8, pc._dbufs["sched"].nbytes)
@@ -106,14 +125,17 @@ def test_plan_matches_host_planner(native, name, kind, levels, ordering):
         assert np.array_equal(raw[at:at + nb], buf.view(np.uint8)[:nb])
         plan = _unpack(raw[at:at + nb], n)
         assert _check(plan, n, forward, off, col, coef)
+        if plan_mode != "auto":
+            assert (plan["ring"] > 0) == (plan_mode == "ring")
         depth = max(depth, plan["depth"])
         at += nb
     assert pc._sweep_depth == depth
 
 
-@pytest.mark.parametrize("name,kind,levels,ordering", CASES)
-def test_scheduled_apply_bitwise(native, oracle, name, kind, levels, ordering):
+@pytest.mark.parametrize("plan_mode,case", MODES, ids=IDS, indirect=["plan_mode"])
+def test_scheduled_apply_bitwise(native, oracle, plan_mode, case):
     from mojopetsc4py import PETSc
+    name, kind, levels, ordering = case
     oracle.set_mode("dev")
     try:
         csr, A, pc = _setup(name, kind, levels, ordering)

@@ -1,10 +1,11 @@
-"""The host half of the block-planned sweeps (mpk_sweep_plan; no GPU needed):
-the plan matches a restatement computed here, keeps every row's entries in
-the row's own order, and every value a row reads is either computed earlier
-in its own block (a lower local level) or belongs to an earlier block -- the
-property the device sweep's progress rests on.  A planted mistake in a
-restated plan (a row reading a later level of its block) is caught by the
-same check."""
+"""The host half of the planned sweeps (mpk_sweep_plan; no GPU needed): the
+plan matches a restatement computed here (block plans and ring plans, and the
+choice between them), keeps every row's entries in the row's own order, and
+every value a row reads is either computed earlier in its own block (a lower
+level; in a ring plan, one still held in the LDS window) or belongs to an
+earlier block -- the property the device sweep's progress rests on.  Planted
+mistakes in a plan (a row reading a later level of its block; a ring plan
+reading a value the window no longer holds) are caught by the same check."""
 
 import numpy as np
 import pytest
@@ -36,7 +37,7 @@ def _unpack(raw, n):
     """The plan's arrays, with the entries in CSR form whichever way they are
     stored (fixed slots or CSR)."""
     h = raw[:64].view(np.int64)
-    nblk, nent, nlvl, depth, nbytes, w = (int(x) for x in h[:6])
+    nblk, nent, nlvl, depth, nbytes, w, ring = (int(x) for x in h[:7])
     ints = raw[64:].view(np.int32)
     o = 0
 
@@ -54,16 +55,85 @@ def _unpack(raw, n):
     assert nbytes == len(raw)
     if w:
         assert pcnt.max(initial=0) <= w
-        take_idx = np.concatenate([p * w + np.arange(c) for p, c in enumerate(pcnt)] or [np.zeros(0, np.int64)])
+        # slot e of position p at e n + p
+        take_idx = np.concatenate([np.arange(c) * n + p for p, c in enumerate(pcnt)] or [np.zeros(0, np.int64)])
         ecoef, edep = ecoef[take_idx.astype(np.int64)], edep[take_idx.astype(np.int64)]
         pent = np.concatenate([[0], np.cumsum(pcnt)])
     else:
         assert np.array_equal(np.diff(pent), pcnt)
-    return dict(nblk=nblk, depth=depth, w=w, blk_pos=blk_pos, blk_lev=blk_lev, lvl=lvl, prow=prow, pent=pent,
-                ecoef=ecoef, edep=edep)
+    return dict(nblk=nblk, depth=depth, w=w, ring=ring, blk_pos=blk_pos, blk_lev=blk_lev, lvl=lvl, prow=prow,
+                pent=pent, ecoef=ecoef, edep=edep)
 
 
-def _restated(n, forward, off, col, coef, bmax):
+def _slots(off, n):
+    """Fixed slots per position: a power of two up to 32, else a multiple of
+    8 up to 64 padding the entries at most 4 times; 0 (CSR form) beyond."""
+    widest = max((off[i + 1] - off[i] for i in range(n)), default=0)
+    w = 4
+    while w < widest and w < 32:
+        w *= 2
+    if w >= widest:
+        return w
+    w = -(-widest // 8) * 8
+    return w if w <= 64 and n * w <= 4 * off[n] else 0
+
+
+def _restated_ring(n, forward, off, col, coef, bmax, mode):
+    """The ring plan when the planner takes it (None otherwise): rows at
+    their latest level, values W or more positions behind the end of the
+    reading row's level read as published rows, others from slot position
+    mod W."""
+    if n <= bmax or mode == "blocks":
+        return None
+    W = 1
+    while 2 * W <= bmax:
+        W *= 2
+    spos = (lambda i: i) if forward else (lambda i: n - 1 - i)
+    earlier = (lambda k, i: k < i) if forward else (lambda k, i: k > i)
+    deps = [[col[e] for e in range(off[i], off[i + 1]) if earlier(col[e], i)] for i in range(n)]
+    asap = np.zeros(n, np.int64)
+    for s in range(n):
+        i = spos(s)
+        asap[i] = max((asap[k] + 1 for k in deps[i]), default=0)
+    nl = int(asap.max()) + 1
+    # latest level: one before the earliest reader, the last level for none
+    readers = [[] for _ in range(n)]
+    for i in range(n):
+        for k in deps[i]:
+            readers[k].append(i)
+    lev = np.full(n, nl - 1, np.int64)
+    for s in reversed(range(n)):
+        i = spos(s)
+        lev[i] = min((lev[j] - 1 for j in readers[i]), default=nl - 1)
+    assert np.all(lev >= asap)
+    prow = sorted((spos(s) for s in range(n)), key=lambda i: lev[i])  # stable: sweep order in a level
+    pos = np.empty(n, np.int64)
+    pos[prow] = np.arange(n)
+    starts = [int(np.sum(lev < l)) for l in range(nl + 1)]
+    old = sum(1 for i in range(n) for k in deps[i] if starts[lev[i] + 1] - pos[k] > W)
+    if mode != "ring" and (n > 1024 * nl or 64 * old > off[n]):
+        return None
+    pent, ecoef, edep = [0], [], []
+    for i in prow:
+        for e in range(off[i], off[i + 1]):
+            k = col[e]
+            ecoef.append(coef[e])
+            if not earlier(k, i):
+                edep.append(-k - n - 1)
+            elif starts[lev[i] + 1] - pos[k] <= W:
+                edep.append(pos[k] % W)
+            else:
+                edep.append(-k - 1)
+        pent.append(len(edep))
+    return dict(nblk=1, w=_slots(off, n), ring=W, blk_pos=np.array([0, n]), blk_lev=np.array([0, nl + 1]),
+                lvl=np.array(starts), prow=np.array(prow), pent=np.array(pent), ecoef=np.array(ecoef),
+                edep=np.array(edep))
+
+
+def _restated(n, forward, off, col, coef, bmax, mode="auto"):
+    ring = _restated_ring(n, forward, off, col, coef, bmax, mode)
+    if ring is not None:
+        return ring
     B = n if n <= bmax else min(bmax, max(1024, -(-n // 256)))
     nblk = -(-n // B)
     spos = (lambda i: i) if forward else (lambda i: n - 1 - i)
@@ -102,27 +172,30 @@ def _restated(n, forward, off, col, coef, bmax):
             else:
                 edep.append(-k - 1)
         pent.append(len(edep))
-    widest = max((off[i + 1] - off[i] for i in range(n)), default=0)
-    w = 4
-    while w < widest:
-        w *= 2
-    w = w if w <= 32 else 0
-    return dict(nblk=nblk, w=w, blk_pos=np.array(blk_pos), blk_lev=np.array(blk_lev), lvl=np.array(lvl),
+    return dict(nblk=nblk, w=_slots(off, n), ring=0, blk_pos=np.array(blk_pos), blk_lev=np.array(blk_lev), lvl=np.array(lvl),
                 prow=np.array(prow), pent=np.array(pent), ecoef=np.array(ecoef), edep=np.array(edep))
 
 
 def _check(plan, n, forward, off, col, coef):
     """Every row's entries in its own order, and every read resolvable when
-    the row's level runs: local reads from lower levels of the same block,
-    external reads from earlier blocks."""
+    the row's level runs: local reads from lower levels of the same block
+    (in a ring plan, from the slot of a position still inside the window of
+    W positions that ends with the row's level), external reads from earlier
+    blocks."""
     spos = (lambda i: i) if forward else (lambda i: n - 1 - i)
     prow, pent, edep, ecoef = plan["prow"], plan["pent"], plan["edep"], plan["ecoef"]
+    W = plan["ring"]
     assert sorted(prow.tolist()) == list(range(n))
     pos = np.empty(n, np.int64)
     pos[prow] = np.arange(n)
+    if W:
+        assert plan["nblk"] == 1 and W & (W - 1) == 0
     for b in range(plan["nblk"]):
         p0, p1 = plan["blk_pos"][b], plan["blk_pos"][b + 1]
-        assert sorted(spos(i) for i in prow[p0:p1]) == list(range(p0, p1))
+        if W:
+            assert sorted(prow[p0:p1].tolist()) == list(range(n))
+        else:
+            assert sorted(spos(i) for i in prow[p0:p1]) == list(range(p0, p1))
         starts = plan["lvl"][plan["blk_lev"][b]:plan["blk_lev"][b + 1]]
         assert starts[0] == p0 and starts[-1] == p1 and np.all(np.diff(starts) > 0)
         for li in range(len(starts) - 1):
@@ -132,8 +205,18 @@ def _check(plan, n, forward, off, col, coef):
                 for e, src in zip(range(pent[p], pent[p + 1]), range(off[i], off[i + 1])):
                     k, d = col[src], edep[e]
                     assert ecoef[e] == coef[src]
-                    if d >= 0:
+                    if d >= 0 and W:
+                        # the one position of slot d inside [end - W, end)
+                        end = starts[li + 1]
+                        q = end - W + (d - (end - W)) % W
+                        if not (d < W and q < starts[li] and q >= 0 and prow[q] == k):
+                            return False
+                    elif d >= 0:
                         if not (p0 + d < starts[li] and prow[p0 + d] == k):
+                            return False
+                    elif d >= -n and W:
+                        # a value older than the window: published at an earlier level
+                        if not (-d - 1 == k and pos[k] < starts[li]):
                             return False
                     elif d >= -n:
                         if not (-d - 1 == k and spos(k) < p0):
@@ -180,6 +263,7 @@ SMALL = {
     "rand300": lambda: _random(300, 5, 7),
     "rand200w": lambda: _random(200, 14, 9),
     "rand200x": lambda: _random(200, 70, 10),
+    "rand300z": lambda: _random(300, 160, 11),
 }
 
 
@@ -194,14 +278,46 @@ def test_matches_restatement(L, name, forward, bmax):
     for k in ("blk_pos", "blk_lev", "lvl", "prow", "pent", "edep"):
         assert np.array_equal(got[k], want[k]), k
     assert np.array_equal(got["ecoef"].view(np.int64), np.asarray(want["ecoef"]).view(np.int64))
-    assert got["nblk"] == want["nblk"] and got["w"] == want["w"]
+    assert got["nblk"] == want["nblk"] and got["w"] == want["w"] and got["ring"] == want["ring"]
+    assert _check(got, n, forward, off, col, coef)
+
+
+@pytest.mark.parametrize("name", ["lap20", "diff7", "tri40", "rand300"])
+@pytest.mark.parametrize("forward", [True, False])
+@pytest.mark.parametrize("mode", ["blocks", "ring"])
+def test_forced_mode_matches_restatement(L, monkeypatch, name, forward, mode):
+    """MPK_SWEEP_PLAN forces block or ring plans (a sweep that fits one
+    block keeps it)."""
+    monkeypatch.setenv("MPK_SWEEP_PLAN", mode)
+    off, col, coef = _lower(*SMALL[name](), forward)
+    n = len(off) - 1
+    got = _plan(L, n, forward, off, col, coef, 64)
+    want = _restated(n, forward, off, col, coef, 64, mode)
+    for k in ("blk_pos", "blk_lev", "lvl", "prow", "pent", "edep"):
+        assert np.array_equal(got[k], want[k]), k
+    assert got["ring"] == want["ring"]
+    # tri40 fits one block (n <= 64): no ring either way
+    assert got["ring"] == (64 if mode == "ring" and name != "tri40" else 0)
     assert _check(got, n, forward, off, col, coef)
 
 
 @pytest.mark.parametrize("forward", [True, False])
-def test_grid_operator_blocks(L, forward):
+def test_narrow_grid_takes_a_ring(L, forward):
+    """A 2-D operator of 10,000 rows (199 levels of at most 100 rows) over
+    the 8192-row LDS: a ring plan with W = 8192 and one level per
+    anti-diagonal."""
+    off, col, coef = _lower(*W.laplacian_2d(100), forward)
+    n = len(off) - 1
+    plan = _plan(L, n, forward, off, col, coef, 8192)
+    assert plan["ring"] == 8192 and plan["nblk"] == 1 and len(plan["lvl"]) == 200
+    assert _check(plan, n, forward, off, col, coef)
+
+
+@pytest.mark.parametrize("forward", [True, False])
+def test_grid_operator_blocks(L, monkeypatch, forward):
     """A 3-D operator in several blocks: 20^3 rows in 8 blocks of 1024 (the
     last shorter), every read resolvable."""
+    monkeypatch.setenv("MPK_SWEEP_PLAN", "blocks")
     off, col, coef = _lower(*W.diffusion_3d(20)[0], forward)
     n = len(off) - 1
     plan = _plan(L, n, forward, off, col, coef, 1024)
@@ -221,4 +337,31 @@ def test_check_catches_a_bad_plan(L):
     # move the boundary between levels 1 and 2 so a level-2 row runs with level 1
     lvl[2] += 1
     bad["lvl"] = lvl
+    assert not _check(bad, n, True, off, col, coef)
+
+
+def test_ring_reads_old_values_as_published_rows(L, monkeypatch):
+    """A random pattern reads far behind a 64-position window: those entries
+    read published rows, the rest slots, and the plan still checks."""
+    monkeypatch.setenv("MPK_SWEEP_PLAN", "ring")
+    off, col, coef = _lower(*_random(300, 5, 7), True)
+    n = len(off) - 1
+    plan = _plan(L, n, True, off, col, coef, 64)
+    assert plan["ring"] == 64
+    d = plan["edep"]
+    assert np.any((d < 0) & (d >= -n)) and np.any(d >= 0)
+    assert _check(plan, n, True, off, col, coef)
+
+
+def test_check_catches_a_ring_read_outside_the_window(L, monkeypatch):
+    """Plant the mistake on the CPU: shrink the window a ring plan claims
+    until some row reads a slot that has been rewritten."""
+    monkeypatch.setenv("MPK_SWEEP_PLAN", "ring")
+    off, col, coef = _lower(*W.laplacian_2d(20), True)
+    n = len(off) - 1
+    plan = _plan(L, n, True, off, col, coef, 64)
+    assert plan["ring"] == 64 and _check(plan, n, True, off, col, coef)
+    bad = dict(plan)
+    bad["ring"] = 16
+    bad["edep"] = np.where(plan["edep"] >= 0, plan["edep"] % 16, plan["edep"])
     assert not _check(bad, n, True, off, col, coef)
