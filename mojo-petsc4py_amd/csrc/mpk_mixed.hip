@@ -185,27 +185,32 @@ __global__ void __launch_bounds__(256) k_lp_fold(const double *P, int nb, double
 
 // ------------------------------------------------------------ sweeps
 
-#define MPK_SENTINEL_F 0x7FA5D5A1u  // signalling NaN; arithmetic never yields it
-#define MPK_QNAN_F 0x7FC00000u
-
-__device__ __forceinline__ unsigned load_pub_f(const float *pub, i64 k) {
-  return __hip_atomic_load(reinterpret_cast<const unsigned *>(pub + k), __ATOMIC_RELAXED,
-                           __HIP_MEMORY_SCOPE_AGENT);
-}
-__device__ __forceinline__ void store_pub_f(float *pub, i64 k, float v) {
-  unsigned b = __float_as_uint(v);
-  if (b == MPK_SENTINEL_F) b = MPK_QNAN_F;
-  __hip_atomic_store(reinterpret_cast<unsigned *>(pub + k), b, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
+// Sentinel-fills a sweep's published values and zeroes its block ticket and
+// ring progress word.
 __global__ void k_lp_prepare(i64 n, float *pub, unsigned *ticket) {
   const i64 i = (i64)blockIdx.x * 256 + threadIdx.x;
-  if (i == 0) *ticket = 0u;
+  if (i == 0) {
+    ticket[0] = 0u;
+    ticket[1] = 0u;
+  }
   if (i < n) pub[i] = __uint_as_float(MPK_SENTINEL_F);
 }
 
-struct LpStaged {
+struct alignas(8) LpStaged {
   float init, s, d1, d2;
   i64 o;
+};
+// The low-precision row programs add their sum to the staged start in fin.
+__device__ __forceinline__ float row_start(const LpStaged &) { return 0.0f; }
+
+// Plan coefficients from the solver's low-precision copy (slot-major like the
+// plan's own, divided by the row scale where there is one).
+template <class CT>
+struct LpCoef {
+  const CT *c;
+  static constexpr int kBytes = sizeof(CT);
+  __device__ static LpCoef make(const PlanView &, const void *lc) { return LpCoef{(const CT *)lc}; }
+  __device__ float operator()(i64 q) const { return lp_f(c[q]); }
 };
 
 // The row programs of the low-precision sweeps: acc = sum of coef * value in
@@ -259,73 +264,6 @@ struct LpSorBwd {
   }
   __device__ float stale(i64 k) const { return y[k]; }
 };
-
-// The block-plan sweep (k_sweep_blk) in FP32 with low-precision coefficients.
-template <class Prog, class CT>
-__global__ void __launch_bounds__(256) k_sweep_lp(Prog prog, i64 n, const char *plan, int second,
-                                                  const CT *__restrict__ lc, unsigned *ticket, float *pub) {
-  __shared__ float val[MPK_BLK_ROWS];
-  __shared__ unsigned tk_sh;
-  if (threadIdx.x == 0) tk_sh = atomicAdd(ticket, 1u);
-  __syncthreads();
-  if (second) plan += reinterpret_cast<const i64 *>(plan)[4];
-  const PlanView P = plan_view(plan, n);
-  const i64 b = tk_sh;
-  if (b >= P.nblk) return;
-  const int p0 = P.blk_pos[b];
-  const int l0 = P.blk_lev[b], l1 = P.blk_lev[b + 1] - 1;
-  for (int l = l0; l < l1; l++) {
-    const int pa = P.lvl[l], pb = P.lvl[l + 1];
-    for (int p = pa + (int)threadIdx.x; p < pb; p += 256) {
-      const i64 i = P.prow[p];
-      const LpStaged st = prog.stage(i);
-      const int cnt = P.pcnt[p];
-      const i64 base = P.w ? p : P.pent[p], stride = P.w ? P.n : 1;  // entry j at base + j stride
-      float acc = 0.0f;
-      for (int j0 = 0; j0 < cnt; j0 += 8) {
-        // issue every load of the chunk, then poll what is not published yet
-        // issue every load of the chunk (LDS and global into separate
-        // registers), then resolve, polling what is not published yet
-        const int k = cnt - j0 < 8 ? cnt - j0 : 8;
-        int d[8];
-        float lv[8];
-        unsigned gb[8];
-        i64 q[8];
-#pragma unroll
-        for (int e = 0; e < 8; e++) {
-          q[e] = base + (i64)(j0 + e) * stride;
-          d[e] = e < k ? P.edep[q[e]] : 0;
-          lv[e] = val[d[e] >= 0 ? d[e] : 0];
-          gb[e] = 0u;
-          if (d[e] < 0 && d[e] >= -n) gb[e] = load_pub_f(pub, -(i64)d[e] - 1);
-        }
-#pragma unroll
-        for (int e = 0; e < 8; e++) {
-          if (e < k) {
-            float v;
-            if (d[e] >= 0) {
-              v = lv[e];
-            } else if (d[e] >= -n) {
-              unsigned b = gb[e];
-              while (b == MPK_SENTINEL_F) {
-                __builtin_amdgcn_s_sleep(1);
-                b = load_pub_f(pub, -(i64)d[e] - 1);
-              }
-              v = __uint_as_float(b);
-            } else {
-              v = prog.stale(-(i64)d[e] - n - 1);
-            }
-            acc = fmaf(lp_f(lc[q[e]]), v, acc);
-          }
-        }
-      }
-      const float r = prog.fin(i, acc, st);
-      val[(p - p0) & P.mask] = r;
-      store_pub_f(pub, i, r);
-    }
-    __syncthreads();
-  }
-}
 
 // Approximate sweeps: k Jacobi passes over a sweep's entries (a truncated
 // Neumann series of the triangular solve) instead of the ordered sweep.  Pass
@@ -538,10 +476,11 @@ struct MixedSolver {
   LpState *st = nullptr;
   double *hst = nullptr;  // pinned: res, breakdown
   unsigned *ticket = nullptr;
+  LpStaged *lst = nullptr;  // staged rows of the planned sweeps
 
   ~MixedSolver() {
     void *bufs[] = {lv,          rs,          lc0,         lc1,       dinv,      idiag,    work,
-                    r64,         P,           st,          ticket,    tri[0].rp, tri[0].soff, tri[0].sc,
+                    r64,         P,           st,          ticket,    lst,       tri[0].rp, tri[0].soff, tri[0].sc,
                     tri[0].sv,   tri[1].rp,   tri[1].soff, tri[1].sc, tri[1].sv};
     hipStreamSynchronize(stream());
     for (void *b : bufs)
@@ -579,14 +518,17 @@ struct MixedSolver {
       }
       return;
     }
+    // the planned sweep on the FP32 engine: stage, then the block or ring plan
     MPK_LAUNCH(k_lp_prepare, lp_grid(n), 256, n, pub, ticket);
     const void *lc = second ? lc1 : lc0;
+    MPK_LAUNCH((k_sweep_stage<Prog, LpStaged>), lp_grid(n), 256, prog, n, pc_plan(), second, lst,
+               (const int64_t *)nullptr);
     if (prec == 16)
-      MPK_LAUNCH((k_sweep_lp<Prog, __half>), nblocks(n, 1024), 256, prog, n, pc_plan(), second,
-                 (const __half *)lc, ticket, pub);
+      MPK_LAUNCH((k_sweep_blk<Prog, SweepF32, LpCoef<__half>, LpStaged>), nblocks(n, 1024), 256, prog, n, pc_plan(),
+                 second, (const LpStaged *)lst, lc, ticket, pub, (const int64_t *)nullptr);
     else
-      MPK_LAUNCH((k_sweep_lp<Prog, float>), nblocks(n, 1024), 256, prog, n, pc_plan(), second,
-                 (const float *)lc, ticket, pub);
+      MPK_LAUNCH((k_sweep_blk<Prog, SweepF32, LpCoef<float>, LpStaged>), nblocks(n, 1024), 256, prog, n, pc_plan(),
+                 second, (const LpStaged *)lst, lc, ticket, pub, (const int64_t *)nullptr);
   }
   const char *pc_plan() const {
     return pc.type == PCT_ILU ? pc.ilu.plan : pc.type == PCT_SOR ? pc.sor.plan : nullptr;
@@ -890,6 +832,7 @@ int64_t mpk_mixed_create(int64_t n, int64_t irow, int64_t icol, int64_t data, in
   HIPCHK(hipMalloc(&S->st, sizeof(LpState)));
   HIPCHK(hipHostMalloc(&S->hst, 4 * sizeof(double), hipHostMallocDefault));
   HIPCHK(hipMalloc(&S->ticket, 64 * sizeof(unsigned)));
+  HIPCHK(hipMalloc(&S->lst, (n > 0 ? n : 1) * sizeof(LpStaged)));
   if (!failed()) HIPCHK(hipMemset(S->st, 0, sizeof(LpState)));
   sync();
   if (failed()) {

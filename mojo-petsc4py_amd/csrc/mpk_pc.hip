@@ -35,15 +35,16 @@ __global__ void k_sweep_prepare(i64 n, double *p, unsigned *ticket, const int64_
 // for the PC's second sweep), else in row order.
 template <class Prog>
 static void run_sweep(const Prog &prog, i64 n, bool forward, double *pub, const int64_t *stop,
-                      const char *plan = nullptr, int second = 0, StagedRows st = StagedRows{}) {
+                      const char *plan = nullptr, int second = 0, Staged *st = nullptr) {
   Ctx *c = ctx();
   if (!c || n <= 0) return;
   MPK_LAUNCH(k_sweep_prepare, nblocks(n, 256), 256, n, pub, c->tickets, stop);
   if (plan) {
-    MPK_LAUNCH(k_sweep_stage<Prog>, nblocks(n, 256), 256, prog, n, plan, second, st, stop);
+    MPK_LAUNCH((k_sweep_stage<Prog, Staged>), nblocks(n, 256), 256, prog, n, plan, second, st, stop);
     // blocks hold at least min(n, 1024) rows (mpk_sweep_plan), so this grid
     // covers every block; the spare workgroups find no block and return
-    MPK_LAUNCH(k_sweep_blk<Prog>, nblocks(n, 1024), 256, prog, n, plan, second, st, c->tickets, pub, stop);
+    MPK_LAUNCH((k_sweep_blk<Prog, SweepF64, PlanCoef, Staged>), nblocks(n, 1024), 256, prog, n, plan, second,
+               (const Staged *)st, (const void *)nullptr, c->tickets, pub, stop);
   } else {
     MPK_LAUNCH(k_sweep<Prog>, nblocks(n, 64), 64, prog, n, forward ? 1 : 0, c->tickets, 0u, pub, stop);
   }
@@ -255,20 +256,19 @@ __global__ void k_sor_idiag(i64 n, const double *a, const int *dmark, double *id
 
 // ------------------------------------------------------------ applies
 
-// Three published-value buffers of n, then (planned sweeps) the staged rows.
+// Three published-value buffers of n, then (planned sweeps) the staged rows,
+// 32 bytes each from a 16-byte boundary.
 static double *sweep_scratch(i64 n, bool planned) {
-  return scratch_vector((size_t)(planned ? 7 : 3) * (size_t)n);
+  return scratch_vector((size_t)(planned ? 7 : 3) * (size_t)n + 1);
 }
-static StagedRows staged_rows(double *S, i64 n) {
-  return StagedRows{S + 3 * n, S + 4 * n, S + 5 * n, reinterpret_cast<i64 *>(S + 6 * n)};
-}
+static Staged *staged_rows(double *S, i64 n) { return reinterpret_cast<Staged *>(S + 3 * n + (n & 1)); }
 
 void ilu_apply(const IluFactor &f, const double *b, double *x, const int64_t *stop) {
   i64 n = f.n;
   double *S = sweep_scratch(n, f.plan != nullptr);
   if (!S || n <= 0) return;
   double *F = S, *B = S + n;
-  const StagedRows st = staged_rows(S, n);
+  Staged *st = staged_rows(S, n);
   run_sweep(IluFwd{f.bi, f.bj, f.r, f.ba, b}, n, true, F, stop, f.plan, 0, st);
   run_sweep(IluBwd{f.bj, f.bdiag, f.c, f.ba, F, x}, n, false, B, stop, f.plan, 1, st);
 }
@@ -278,7 +278,7 @@ void icc_apply(const IccFactor &f, const double *b, double *x, const int64_t *st
   double *S = sweep_scratch(n, f.plan != nullptr);
   if (!S || n <= 0) return;
   double *Y = S, *F = S + n, *B = S + 2 * n;
-  const StagedRows st = staged_rows(S, n);
+  Staged *st = staged_rows(S, n);
   run_sweep(IccFwd{f.trp, f.trow, f.tpos, f.tmid, f.udiag, f.c, f.ua, b, F}, n, true, Y, stop, f.plan, 0, st);
   if (f.nlate) MPK_LAUNCH(k_icc_late, nblocks(n, 256), 256, n, f.trp, f.trow, f.tpos, f.tmid, f.ua, Y, F, stop);
   run_sweep(IccBwd{f.ui, f.uj, f.udiag, f.c, f.ua, F, x, n - 1}, n, false, B, stop, f.plan, 1, st);
@@ -288,7 +288,7 @@ void sor_apply(const SorData &s, const double *x, double *y, const int64_t *stop
   i64 n = s.n;
   double *S = sweep_scratch(n, s.plan != nullptr);
   if (!S || n <= 0) return;
-  const StagedRows st = staged_rows(S, n);
+  Staged *st = staged_rows(S, n);
   MPK_LAUNCH(k_sor_idiag, nblocks(n, 256), 256, n, s.a, s.dmark, s.idiag, s.mdiag, s.omega, s.fshift, stop);
   const bool fwd = (s.flag & (1 | 4)) != 0, bwd = (s.flag & (2 | 8)) != 0;
   double *YF = S, *YB = S + n;

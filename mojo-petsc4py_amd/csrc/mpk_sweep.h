@@ -1,15 +1,14 @@
 // mpk_sweep.h -- sync-free triangular sweeps.
 //
 // A sweep computes rows in dependency order (forward: row i needs rows < i;
-// backward: rows > i).  One wave owns 64 consecutive rows of the sweep order;
-// each lane computes one row as a serial sum over the row's entries in stored
-// order, so the arithmetic of every row is the CPU's.  Rows publish their
-// value in `pub`, which is filled with a signalling-NaN sentinel before the
-// sweep: one 64-bit relaxed atomic load both tests readiness and fetches the
-// value, so no fence orders a flag against the data.  A dependency inside the
-// lane's own wave is read from LDS instead.  Blocks take their position in the
-// sweep from a ticket counter, so every block a block waits on has already
-// started -- there is no deadlock however the hardware orders dispatch.
+// backward: rows > i).  Each row is one thread's serial sum over the row's
+// entries in stored order, so the arithmetic of every row is the CPU's.  Rows
+// publish their value in `pub`, which is filled with a signalling-NaN sentinel
+// before the sweep: one relaxed atomic load both tests readiness and fetches
+// the value, so no fence orders a flag against the data.  Two drivers:
+// k_sweep (no schedule: one wave per 64 consecutive rows, tickets in sweep
+// order) and k_sweep_blk (a schedule built at set-up: block plans or ring
+// plans, below).  Both drain whatever order the hardware dispatches in.
 #pragma once
 
 #include "mpk_internal.h"
@@ -18,6 +17,8 @@ namespace mpk {
 
 #define MPK_SENTINEL 0x7FF4D5A17B0E0000ull  // signalling NaN; arithmetic never yields it
 #define MPK_QNAN 0x7FF8000000000000ull
+#define MPK_SENTINEL_F 0x7FA5D5A1u  // the same for FP32 values
+#define MPK_QNAN_F 0x7FC00000u
 
 __device__ __forceinline__ unsigned long long load_pub(const double *pub, i64 k) {
   return __hip_atomic_load(reinterpret_cast<const unsigned long long *>(pub + k), __ATOMIC_RELAXED,
@@ -29,6 +30,36 @@ __device__ __forceinline__ void store_pub(double *pub, i64 k, double v) {
   __hip_atomic_store(reinterpret_cast<unsigned long long *>(pub + k), b, __ATOMIC_RELAXED,
                      __HIP_MEMORY_SCOPE_AGENT);
 }
+__device__ __forceinline__ unsigned load_pub_f(const float *pub, i64 k) {
+  return __hip_atomic_load(reinterpret_cast<const unsigned *>(pub + k), __ATOMIC_RELAXED,
+                           __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void store_pub_f(float *pub, i64 k, float v) {
+  unsigned b = __float_as_uint(v);
+  if (b == MPK_SENTINEL_F) b = MPK_QNAN_F;
+  __hip_atomic_store(reinterpret_cast<unsigned *>(pub + k), b, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Value types of a planned sweep: FP64 (the drop-in preconditioners) or FP32
+// (the inner solves of the mixed-precision solver).
+struct SweepF64 {
+  using V = double;
+  using Bits = unsigned long long;
+  static constexpr Bits kSentinel = MPK_SENTINEL;
+  __device__ static Bits load(const V *pub, i64 k) { return load_pub(pub, k); }
+  __device__ static void store(V *pub, i64 k, V v) { store_pub(pub, k, v); }
+  __device__ static V value(Bits b) { return __longlong_as_double((long long)b); }
+  __device__ static V madd(V a, V b, V c) { return fma(a, b, c); }
+};
+struct SweepF32 {
+  using V = float;
+  using Bits = unsigned;
+  static constexpr Bits kSentinel = MPK_SENTINEL_F;
+  __device__ static Bits load(const V *pub, i64 k) { return load_pub_f(pub, k); }
+  __device__ static void store(V *pub, i64 k, V v) { store_pub_f(pub, k, v); }
+  __device__ static V value(Bits b) { return __uint_as_float(b); }
+  __device__ static V madd(V a, V b, V c) { return fmaf(a, b, c); }
+};
 
 // Row programs.  Each provides, for sweep row i:
 //   stage(i) -> Staged: the accumulator start and the per-row values the
@@ -38,15 +69,12 @@ __device__ __forceinline__ void store_pub(double *pub, i64 k, double v) {
 //   col(j), coef(j) -> dependency row and coefficient; acc = fma(coef, v, acc)
 //   fin(i, acc, staged) -> published value (and any side outputs; no loads)
 //   stale(k) -> value of a row that is not earlier in the sweep (malformed input)
-struct Staged {
+struct alignas(16) Staged {
   double init, s1, s2;
   i64 o;
 };
-// Staged values of every position of a planned sweep (see k_sweep_stage).
-struct StagedRows {
-  double *init, *s1, *s2;
-  i64 *o;
-};
+// The accumulator a row's sum starts from.
+__device__ __forceinline__ double row_start(const Staged &s) { return s.init; }
 
 // One step of a row's sum: acc + coef * v, fused as the CPU loop's multiply-add.
 template <class Prog>
@@ -185,60 +213,54 @@ __host__ __device__ inline PlanView plan_view(const char *base, i64 n) {
   return v;
 }
 
-template <class Prog>
-__global__ void k_sweep_stage(Prog prog, i64 n, const char *plan, int second, StagedRows st,
-                              const int64_t *stop) {
+// Coefficient of plan entry q: the plan's own FP64 values (other sources, such
+// as the mixed solver's low-precision copies, provide the same two members).
+struct PlanCoef {
+  const double *c;
+  static constexpr int kBytes = 8;
+  __device__ static PlanCoef make(const PlanView &P, const void *) { return PlanCoef{P.ecoef}; }
+  __device__ double operator()(i64 q) const { return c[q]; }
+};
+
+// Writes each position's staged values (Prog::stage of its row) into
+// position order, so that a level's loads of them are contiguous.
+template <class Prog, class S>
+__global__ void k_sweep_stage(Prog prog, i64 n, const char *plan, int second, S *st, const int64_t *stop) {
   if (stop && *stop) return;
   if (second) plan += reinterpret_cast<const i64 *>(plan)[4];
   const PlanView P = plan_view(plan, n);
   const i64 p = (i64)blockIdx.x * 256 + threadIdx.x;
   if (p >= n) return;
-  const Staged s = prog.stage(P.prow[p]);
-  st.init[p] = s.init;
-  st.s1[p] = s.s1;
-  st.s2[p] = s.s2;
-  st.o[p] = s.o;
-}
-
-// The value of entry d (see the plan encoding above).
-template <class Prog>
-__device__ __forceinline__ double plan_value(const Prog &prog, i64 n, const double *val, const double *pub,
-                                             int d) {
-  if (d >= 0) return val[d];
-  if (d >= -n) {
-    unsigned long long bits;
-    while ((bits = load_pub(pub, -(i64)d - 1)) == MPK_SENTINEL) __builtin_amdgcn_s_sleep(1);
-    return __longlong_as_double((long long)bits);
-  }
-  return prog.stale(-(i64)d - n - 1);
+  st[p] = prog.stage(P.prow[p]);
 }
 
 // A position's inputs, loaded one level ahead of their use: with fixed slots
 // every load is independent of the others (one round); in CSR form only the
 // entry offset is fetched ahead.
-template <int PF>
+template <int PF, class Tr, class S>
 struct PlanRow {
   int p, cnt, base;
   i64 i;
-  Staged s;
-  double c[PF];
+  S s;
+  typename Tr::V c[PF];
   int d[PF];
 };
 
-template <int PF>
-__device__ __forceinline__ PlanRow<PF> plan_row(const PlanView &P, const StagedRows &st, int p, bool valid) {
-  PlanRow<PF> r;
+template <int PF, class Tr, class Coef, class S>
+__device__ __forceinline__ PlanRow<PF, Tr, S> plan_row(const PlanView &P, const Coef &coef, const S *st, int p,
+                                                       bool valid) {
+  PlanRow<PF, Tr, S> r;
   r.p = valid ? p : -1;
   if (!valid) return r;
   r.cnt = P.pcnt[p];
   r.i = P.prow[p];
-  r.s = Staged{st.init[p], st.s1[p], st.s2[p], st.o[p]};
+  r.s = st[p];
   if (P.w) {
     r.base = p;
 #pragma unroll
     for (int e = 0; e < PF; e++)
       if (e < P.w) {
-        r.c[e] = P.ecoef[(i64)e * P.n + p];
+        r.c[e] = coef((i64)e * P.n + p);
         r.d[e] = P.edep[(i64)e * P.n + p];
       }
   } else {
@@ -247,37 +269,33 @@ __device__ __forceinline__ PlanRow<PF> plan_row(const PlanView &P, const StagedR
   return r;
 }
 
-// The values of up to G entries: every load is issued first (LDS slots and
-// rows of earlier blocks alike), then the rows still holding the sentinel are
-// polled one by one -- so a row waits about one memory round trip, not one
-// per entry.
 // The values of up to G entries, in two phases so that every load is in
 // flight before any is waited on.  Issue: the LDS slot is read for every
 // entry (index clamped: a different register from the global loads, so the
 // compiler need not drain the global loads first) and the rows of earlier
 // blocks are loaded; resolve: pick each entry's value, polling rows still
 // holding the sentinel.
-template <int G>
+template <int G, class Tr>
 struct Gather {
-  double lv[G];
-  unsigned long long gb[G];
+  typename Tr::V lv[G];
+  typename Tr::Bits gb[G];
 };
 
-template <int G>
-__device__ __forceinline__ void gather_issue(i64 n, const double *val, const double *pub, const int *d, int cnt,
-                                             Gather<G> &g) {
+template <int G, class Tr>
+__device__ __forceinline__ void gather_issue(i64 n, const typename Tr::V *val, const typename Tr::V *pub,
+                                             const int *d, int cnt, Gather<G, Tr> &g) {
 #pragma unroll
   for (int e = 0; e < G; e++) {
     const int de = e < cnt ? d[e] : 0;
     g.lv[e] = val[de >= 0 ? de : 0];
     g.gb[e] = 0;
-    if (de < 0 && de >= -n) g.gb[e] = load_pub(pub, -(i64)de - 1);
+    if (de < 0 && de >= -n) g.gb[e] = Tr::load(pub, -(i64)de - 1);
   }
 }
 
-template <int G, class Prog>
-__device__ __forceinline__ void gather_resolve(const Prog &prog, i64 n, const double *pub, const int *d, int cnt,
-                                               const Gather<G> &g, double *v) {
+template <int G, class Tr, class Prog>
+__device__ __forceinline__ void gather_resolve(const Prog &prog, i64 n, const typename Tr::V *pub, const int *d,
+                                               int cnt, const Gather<G, Tr> &g, typename Tr::V *v) {
 #pragma unroll
   for (int e = 0; e < G; e++) {
     if (e < cnt) {
@@ -285,12 +303,12 @@ __device__ __forceinline__ void gather_resolve(const Prog &prog, i64 n, const do
       if (de >= 0) {
         v[e] = g.lv[e];
       } else if (de >= -n) {
-        unsigned long long b = g.gb[e];
-        while (b == MPK_SENTINEL) {
+        typename Tr::Bits b = g.gb[e];
+        while (b == Tr::kSentinel) {
           __builtin_amdgcn_s_sleep(1);
-          b = load_pub(pub, -(i64)de - 1);
+          b = Tr::load(pub, -(i64)de - 1);
         }
-        v[e] = __longlong_as_double((long long)b);
+        v[e] = Tr::value(b);
       } else {
         v[e] = prog.stale(-(i64)de - n - 1);
       }
@@ -298,56 +316,53 @@ __device__ __forceinline__ void gather_resolve(const Prog &prog, i64 n, const do
   }
 }
 
-template <int G, class Prog>
-__device__ __forceinline__ void plan_gather(const Prog &prog, i64 n, const double *val, const double *pub,
-                                            const int *d, int cnt, double *v) {
-  Gather<G> g;
-  gather_issue<G>(n, val, pub, d, cnt, g);
-  gather_resolve<G>(prog, n, pub, d, cnt, g, v);
-}
-
 // A row's sum and finish once its slots' values are gathered (register part
 // in g); entries beyond the registers (or CSR form) are gathered here.
-template <int PF, class Prog>
-__device__ __forceinline__ void plan_finish(const Prog &prog, i64 n, const PlanView &P, const PlanRow<PF> &r,
-                                            const Gather<PF> &g, double *val, int p0, double *pub) {
-  double acc = r.s.init;
+template <int PF, class Tr, class Prog, class Coef, class S>
+__device__ __forceinline__ void plan_finish(const Prog &prog, const Coef &coef, i64 n, const PlanView &P,
+                                            const PlanRow<PF, Tr, S> &r, const Gather<PF, Tr> &g,
+                                            typename Tr::V *val, int p0, typename Tr::V *pub) {
+  using V = typename Tr::V;
+  V acc = row_start(r.s);
   int j_rest = 0;  // entries still to do (beyond the registers, or CSR form)
   if (P.w) {
-    double v[PF];
-    gather_resolve<PF>(prog, n, pub, r.d, r.cnt, g, v);
+    V v[PF];
+    gather_resolve<PF, Tr>(prog, n, pub, r.d, r.cnt, g, v);
 #pragma unroll
     for (int e = 0; e < PF; e++)
-      if (e < r.cnt) acc = fma(r.c[e], v[e], acc);
+      if (e < r.cnt) acc = Tr::madd(r.c[e], v[e], acc);
     j_rest = PF;
   }
   const i64 stride = P.w ? P.n : 1;  // entry j of the row at base + j stride
   for (int j0 = j_rest; j0 < r.cnt; j0 += 8) {
     const int k = r.cnt - j0 < 8 ? r.cnt - j0 : 8;
     int d[8];
-    double c[8], v[8];
+    V c[8], v[8];
 #pragma unroll
     for (int e = 0; e < 8; e++) {
       const i64 q = r.base + (i64)(j0 + e) * stride;
       d[e] = e < k ? P.edep[q] : 0;
-      c[e] = e < k ? P.ecoef[q] : 0.0;
+      c[e] = e < k ? coef(q) : V(0);
     }
-    plan_gather<8>(prog, n, val, pub, d, k, v);
+    Gather<8, Tr> g8;
+    gather_issue<8, Tr>(n, val, pub, d, k, g8);
+    gather_resolve<8, Tr>(prog, n, pub, d, k, g8, v);
 #pragma unroll
     for (int e = 0; e < 8; e++)
-      if (e < k) acc = fma(c[e], v[e], acc);
+      if (e < k) acc = Tr::madd(c[e], v[e], acc);
   }
-  const double v = prog.fin(r.i, acc, r.s);
+  const V v = prog.fin(r.i, acc, r.s);
   val[(r.p - p0) & P.mask] = v;
-  store_pub(pub, r.i, v);
+  Tr::store(pub, r.i, v);
 }
 
-template <int PF, class Prog>
-__device__ __forceinline__ void plan_compute(const Prog &prog, i64 n, const PlanView &P, const PlanRow<PF> &r,
-                                             double *val, int p0, double *pub) {
-  Gather<PF> g;
-  if (P.w) gather_issue<PF>(n, val, pub, r.d, r.cnt, g);
-  plan_finish<PF>(prog, n, P, r, g, val, p0, pub);
+template <int PF, class Tr, class Prog, class Coef, class S>
+__device__ __forceinline__ void plan_compute(const Prog &prog, const Coef &coef, i64 n, const PlanView &P,
+                                             const PlanRow<PF, Tr, S> &r, typename Tr::V *val, int p0,
+                                             typename Tr::V *pub) {
+  Gather<PF, Tr> g;
+  if (P.w) gather_issue<PF, Tr>(n, val, pub, r.d, r.cnt, g);
+  plan_finish<PF, Tr>(prog, coef, n, P, r, g, val, p0, pub);
 }
 
 // Ring plans run in workgroup 0 while workgroups 8, 16, .., 8 MPK_RING_PF
@@ -369,9 +384,21 @@ __device__ __forceinline__ void ring_publish(unsigned *pr, unsigned v) {
   __hip_atomic_store(pr, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
+// One word of each 128-byte line of [a, b), the lines spread over the
+// workgroup's threads.
+__device__ __forceinline__ unsigned touch_span(const void *a, const void *b, int t) {
+  unsigned x = 0;
+  const char *q0 = reinterpret_cast<const char *>(reinterpret_cast<uintptr_t>(a) & ~(uintptr_t)127);
+  for (const char *q = q0 + 128 * (i64)t; q < reinterpret_cast<const char *>(b); q += 128 * 256)
+    x ^= *reinterpret_cast<const unsigned *>(q);
+  return x;
+}
+
 // Prefetcher k (1..MPK_RING_PF) of a ring plan: levels k-1, k-1+MPK_RING_PF, ..
-__device__ __forceinline__ void ring_prefetch(const PlanView &P, const StagedRows &st, int k, unsigned *progress,
-                                              unsigned *sink, const int64_t *stop) {
+// (coefficients of cbytes bytes at coef, slot-major like the plan's).
+template <class S>
+__device__ __forceinline__ void ring_prefetch(const PlanView &P, const S *st, const char *coef, int cbytes, int k,
+                                              unsigned *progress, unsigned *sink, const int64_t *stop) {
   __shared__ int done_sh;
   const int t = (int)threadIdx.x;
   const int nl = P.blk_lev[1] - 1;
@@ -392,37 +419,29 @@ __device__ __forceinline__ void ring_prefetch(const PlanView &P, const StagedRow
     __syncthreads();
     if (done_sh) break;
     const int s = P.lvl[l], e = P.lvl[l + 1];
-    for (int p = s + t; p < e; p += 256) {
-      acc ^= (unsigned)P.prow[p] ^ (unsigned)P.pcnt[p] ^ (unsigned)st.o[p];
-      acc ^= (unsigned)__double_as_longlong(st.init[p]) ^ (unsigned)__double_as_longlong(st.s1[p]) ^
-             (unsigned)__double_as_longlong(st.s2[p]);
-    }
+    acc ^= touch_span(P.prow + s, P.prow + e, t) ^ touch_span(P.pcnt + s, P.pcnt + e, t);
+    acc ^= touch_span(st + s, st + e, t);
     if (P.w) {
-      // each slot's run of the level's positions: whole lines, one per thread
-      const int per = (e - s + 15) / 16 + 1;  // lines per slot run (coefficients; the indices need half)
-      for (int k2 = t; k2 < (int)P.w * per; k2 += 256) {
+      // each slot's run of the level's positions: (slot, line) pairs spread
+      // over the threads, four loads in flight per thread
+      const int wb = cbytes > 4 ? cbytes : 4;  // the wider of a coefficient and an index
+      const int per = (int)(((i64)(e - s) * wb + 127) / 128) + 1;  // lines per run, either array
+      const int tot = (int)P.w * per;
+#pragma unroll 4
+      for (int k2 = t; k2 < tot; k2 += 256) {
         const i64 j = k2 / per, li = k2 % per;
-        const char *c0 = reinterpret_cast<const char *>(P.ecoef + j * P.n + s);
-        const char *c1 = reinterpret_cast<const char *>(P.ecoef + j * P.n + e);
+        const char *c0 = coef + ((i64)j * P.n + s) * cbytes, *c1 = coef + ((i64)j * P.n + e) * cbytes;
         const char *q = reinterpret_cast<const char *>(reinterpret_cast<uintptr_t>(c0) & ~(uintptr_t)127) + 128 * li;
-        if (q < c1) acc ^= *reinterpret_cast<const unsigned *>(q);
-        const char *d0 = reinterpret_cast<const char *>(P.edep + j * P.n + s);
-        const char *d1 = reinterpret_cast<const char *>(P.edep + j * P.n + e);
+        const char *d0 = reinterpret_cast<const char *>(P.edep + (i64)j * P.n + s);
+        const char *d1 = reinterpret_cast<const char *>(P.edep + (i64)j * P.n + e);
         const char *qd = reinterpret_cast<const char *>(reinterpret_cast<uintptr_t>(d0) & ~(uintptr_t)127) + 128 * li;
-        if (qd < d1) acc ^= *reinterpret_cast<const unsigned *>(qd);
+        const unsigned a = q < c1 ? *reinterpret_cast<const unsigned *>(q) : 0u;
+        const unsigned b = qd < d1 ? *reinterpret_cast<const unsigned *>(qd) : 0u;
+        acc ^= a ^ b;
       }
     } else {
-      // the level's entries are contiguous: whole lines, one per thread
-      const char *c0 = reinterpret_cast<const char *>(P.ecoef + P.pent[s]);
-      const char *c1 = reinterpret_cast<const char *>(P.ecoef + P.pent[e]);
-      const char *d0 = reinterpret_cast<const char *>(P.edep + P.pent[s]);
-      const char *d1 = reinterpret_cast<const char *>(P.edep + P.pent[e]);
-      const char *ca = reinterpret_cast<const char *>(reinterpret_cast<uintptr_t>(c0) & ~(uintptr_t)127);
-      const char *da = reinterpret_cast<const char *>(reinterpret_cast<uintptr_t>(d0) & ~(uintptr_t)127);
-      for (i64 li = t; ca + 128 * li < c1; li += 256) {
-        acc ^= *reinterpret_cast<const unsigned *>(ca + 128 * li);
-        if (da + 128 * li < d1) acc ^= *reinterpret_cast<const unsigned *>(da + 128 * li);
-      }
+      acc ^= touch_span(coef + (i64)P.pent[s] * cbytes, coef + (i64)P.pent[e] * cbytes, t);
+      acc ^= touch_span(P.edep + P.pent[s], P.edep + P.pent[e], t);
     }
     __syncthreads();
   }
@@ -432,9 +451,10 @@ __device__ __forceinline__ void ring_prefetch(const PlanView &P, const StagedRow
 // The levels of block b, holding PF slots of each row in registers.  With a
 // progress word (ring plans), thread 0 publishes the level every
 // MPK_RING_STEP levels.
-template <int PF, class Prog>
-__device__ __forceinline__ void plan_levels(const Prog &prog, i64 n, const PlanView &P, const StagedRows &st,
-                                            i64 b, double *val, double *pub, unsigned *progress) {
+template <int PF, class Tr, class Prog, class Coef, class S>
+__device__ __forceinline__ void plan_levels(const Prog &prog, const Coef &coef, i64 n, const PlanView &P,
+                                            const S *st, i64 b, typename Tr::V *val, typename Tr::V *pub,
+                                            unsigned *progress) {
   const int t = (int)threadIdx.x;
   const int p0 = P.blk_pos[b];
   const int l0 = P.blk_lev[b], l1 = P.blk_lev[b + 1] - 1;  // levels l0..l1-1; lvl[l1] ends the block
@@ -442,18 +462,18 @@ __device__ __forceinline__ void plan_levels(const Prog &prog, i64 n, const PlanV
   // thread holds lvl[l], lvl[l+1], lvl[l+2] and fetches lvl[l+3]
   int pa = P.lvl[l0], pb = P.lvl[l0 + 1];
   int pc = l0 + 1 < l1 ? P.lvl[l0 + 2] : pb;
-  PlanRow<PF> cur = plan_row<PF>(P, st, pa + t, pa + t < pb);
+  PlanRow<PF, Tr, S> cur = plan_row<PF, Tr>(P, coef, st, pa + t, pa + t < pb);
   for (int l = l0; l < l1; l++) {
     if (progress && t == 0 && ((l - l0) & (MPK_RING_STEP - 1)) == 0) ring_publish(progress, (unsigned)(l - l0));
     const bool more = l + 1 < l1;
     const int pd = l + 2 < l1 ? P.lvl[l + 3] : pc;
     // this level's value loads, then the next level's prefetch, then the sums
-    Gather<PF> g;
-    if (cur.p >= 0 && P.w) gather_issue<PF>(n, val, pub, cur.d, cur.cnt, g);
-    const PlanRow<PF> nxt = plan_row<PF>(P, st, pb + t, more && pb + t < pc);
-    if (cur.p >= 0) plan_finish<PF>(prog, n, P, cur, g, val, p0, pub);
+    Gather<PF, Tr> g;
+    if (cur.p >= 0 && P.w) gather_issue<PF, Tr>(n, val, pub, cur.d, cur.cnt, g);
+    const PlanRow<PF, Tr, S> nxt = plan_row<PF, Tr>(P, coef, st, pb + t, more && pb + t < pc);
+    if (cur.p >= 0) plan_finish<PF, Tr>(prog, coef, n, P, cur, g, val, p0, pub);
     for (int p = pa + t + 256; p < pb; p += 256)
-      plan_compute<PF>(prog, n, P, plan_row<PF>(P, st, p, true), val, p0, pub);
+      plan_compute<PF, Tr>(prog, coef, n, P, plan_row<PF, Tr>(P, coef, st, p, true), val, p0, pub);
     __syncthreads();
     cur = nxt;
     pa = pb;
@@ -468,26 +488,30 @@ __device__ __forceinline__ void plan_levels(const Prog &prog, i64 n, const PlanV
 // loaded when they run.  Plans whose rows fit 4 slots keep 4 in registers,
 // others 16 (the rest of a longer row is loaded when it runs).  ticket[0]
 // orders the blocks of a block plan; ticket[1] is a ring plan's progress
-// word (both zeroed by k_sweep_prepare).
-template <class Prog>
-__global__ void __launch_bounds__(256) k_sweep_blk(Prog prog, i64 n, const char *plan, int second, StagedRows st,
-                                                   unsigned *ticket, double *pub, const int64_t *stop) {
-  __shared__ double val[MPK_BLK_ROWS];
+// word (both zeroed before the sweep).  Tr is the value type, Coef the
+// coefficient source (lc: its array, when it is not the plan's own).
+template <class Prog, class Tr, class Coef, class S>
+__global__ void __launch_bounds__(256) k_sweep_blk(Prog prog, i64 n, const char *plan, int second, const S *st,
+                                                   const void *lc, unsigned *ticket, typename Tr::V *pub,
+                                                   const int64_t *stop) {
+  __shared__ typename Tr::V val[MPK_BLK_ROWS];
   __shared__ unsigned tk_sh;
   if (second) plan += reinterpret_cast<const i64 *>(plan)[4];
   const PlanView P = plan_view(plan, n);
+  const Coef coef = Coef::make(P, lc);
   if (P.mask >= 0) {  // ring plan
     const unsigned g = blockIdx.x;
     if (g == 0) {
       if (!(stop && *stop)) {
         if (P.w == 4)
-          plan_levels<4>(prog, n, P, st, 0, val, pub, ticket + 1);
+          plan_levels<4, Tr>(prog, coef, n, P, st, 0, val, pub, ticket + 1);
         else
-          plan_levels<MPK_PLAN_PF>(prog, n, P, st, 0, val, pub, ticket + 1);
+          plan_levels<MPK_PLAN_PF, Tr>(prog, coef, n, P, st, 0, val, pub, ticket + 1);
       }
       if (threadIdx.x == 0) ring_publish(ticket + 1, MPK_RING_DONE);
     } else if (g % 8 == 0 && g / 8 <= MPK_RING_PF) {
-      ring_prefetch(P, st, (int)(g / 8), ticket + 1, ticket + 2, stop);
+      ring_prefetch(P, st, reinterpret_cast<const char *>(coef.c), Coef::kBytes, (int)(g / 8), ticket + 1,
+                    ticket + 2, stop);
     }
     return;
   }
@@ -497,9 +521,9 @@ __global__ void __launch_bounds__(256) k_sweep_blk(Prog prog, i64 n, const char 
   const i64 b = tk_sh;
   if (b >= P.nblk) return;
   if (P.w == 4)
-    plan_levels<4>(prog, n, P, st, b, val, pub, nullptr);
+    plan_levels<4, Tr>(prog, coef, n, P, st, b, val, pub, nullptr);
   else
-    plan_levels<MPK_PLAN_PF>(prog, n, P, st, b, val, pub, nullptr);
+    plan_levels<MPK_PLAN_PF, Tr>(prog, coef, n, P, st, b, val, pub, nullptr);
 }
 
 // Entries of a sweep in the rows' own order, for the host-side planner:
