@@ -469,6 +469,58 @@ struct OpCgP {
       st2(p, e, n, al, D2{fma(b, pv.a, zv.a), fma(b, pv.b, zv.b)});
   }
 };
+// CG, w = A p (MatMult_SeqAIJ, ported.mojo:380-391, over the sliced copy) and
+// the tile partial of p.w (ksp.mojo:338) in one pass.  Block b owns tile b
+// (2048 rows), 4 * MPK_NT threads: thread T computes rows T and T + 1024 one
+// lane per row, as k_spmv_sell does (each w row is its serial sum).  The even
+// lane of each row pair takes its neighbour's p and w by a shuffle and forms
+// the pair's leaf, which goes to LDS where the tile reduction keeps it (pair
+// 512 j + 2t is thread t's j-th); the leaves are then folded in that
+// reduction's order, so the partial is k_partials'.
+__global__ void __launch_bounds__(4 * MPK_NT) k_cg_wdot(i64 n, const int *__restrict__ rp,
+                                                        const int *__restrict__ soff, const double *__restrict__ sv,
+                                                        const int *__restrict__ sc, const double *__restrict__ p,
+                                                        double *__restrict__ w, double *P, const int64_t *stop) {
+  if (stop && *stop) return;
+  __shared__ double lf[4 * MPK_NT];
+  __shared__ double sh[4];
+  const unsigned tile = xcd_block(blockIdx.x, gridDim.x);
+  const int T = (int)threadIdx.x;
+  const i64 e0 = (i64)tile * MPK_TILE + T, e1 = e0 + MPK_TILE / 2;
+  const int len0 = e0 < n ? rp[e0 + 1] - rp[e0] : 0, len1 = e1 < n ? rp[e1 + 1] - rp[e1] : 0;
+  const int q0 = e0 < n ? soff[e0 >> 6] + (int)(e0 & 63) : 0, q1 = e1 < n ? soff[e1 >> 6] + (int)(e1 & 63) : 0;
+  const int len = len0 > len1 ? len0 : len1;
+  double w0 = 0.0, w1 = 0.0;
+  for (int k = 0; k < len; k++) {
+    if (k < len0) w0 = fma(sv[q0 + 64 * k], p[sc[q0 + 64 * k]], w0);
+    if (k < len1) w1 = fma(sv[q1 + 64 * k], p[sc[q1 + 64 * k]], w1);
+  }
+  double p0 = 0.0, p1 = 0.0;
+  if (e0 < n) {
+    w[e0] = w0;
+    p0 = p[e0];
+  }
+  if (e1 < n) {
+    w[e1] = w1;
+    p1 = p[e1];
+  }
+  // pair (r, r + 1), r = T even, in each half: r = 512 j + 2t with j = 2 h + (T >= 512)
+  const double pa = __shfl_down(p0, 1, 64), wa = __shfl_down(w0, 1, 64);
+  const double pb = __shfl_down(p1, 1, 64), wb = __shfl_down(w1, 1, 64);
+  if ((T & 1) == 0) {
+    const int t = (T & 511) >> 1, j = T >> 9;
+    lf[j * MPK_NT + t] = fma(p0, w0, pa * wa);
+    lf[(j + 2) * MPK_NT + t] = fma(p1, w1, pb * wb);
+  }
+  __syncthreads();
+  // thread t's sum over its four pairs, then the tile tree (block_tree's)
+  double u = 0.0;
+  if (T < MPK_NT) u = wave_tree((lf[T] + lf[T + MPK_NT]) + (lf[T + 2 * MPK_NT] + lf[T + 3 * MPK_NT]));
+  if (T < MPK_NT && (T & 63) == 0) sh[T >> 6] = u;
+  __syncthreads();
+  if (T == 0) P[tile] = (sh[0] + sh[2]) + (sh[1] + sh[3]);
+}
+
 // x += a p; r += (-a) w  (two VecAXPY sweeps fused, ksp.mojo:363-364)
 struct OpCgXR {
   double *x, *r;
@@ -943,8 +995,18 @@ void mp_ksp_solve_cg(int64_t n, int64_t irow, int64_t icol, int64_t data, int64_
   for (i64 queued = 0;; queued += chunk) {
     for (int c = 0; c < chunk; c++) {
       ew2(n, OpCgP{P, Z, S.d, aligned16(P) && aligned16(Z)}, st);
-      spmv(S.A, P, W, 0, st);
-      S.reduce<1>(RedSet<1>{{{P, W, RK_DOT}}}, EPI_CG_DPI, 0, st);
+      if (S.A.sell_off && n > 0) {
+        // the product and its reduction in one pass (k_cg_wdot)
+        const unsigned T = ntiles(n);
+        double *Pp = partials(T);
+        MPK_LAUNCH(k_cg_wdot, T, 4 * MPK_NT, n, S.A.rp, S.A.sell_off, S.A.sell_v, S.A.sell_c, (const double *)P, W,
+                   Pp, st);
+        MPK_LAUNCH(k_finalize<KspEpi>, 1, MPK_NT, (const double *)Pp, (int)T, 1, (double *)nullptr,
+                   KspEpi{S.d, EPI_CG_DPI, 0}, st);
+      } else {
+        spmv(S.A, P, W, 0, st);
+        S.reduce<1>(RedSet<1>{{{P, W, RK_DOT}}}, EPI_CG_DPI, 0, st);
+      }
       const bool al = aligned16(X) && aligned16(R) && aligned16(P) && aligned16(W);
       if (S.pc.type == PCT_JACOBI && n > 0) {
         // x, r, the Jacobi apply and the three reductions in one pass
