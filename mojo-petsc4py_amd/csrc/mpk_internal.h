@@ -50,6 +50,11 @@ void set_error(int code, const char *fmt, ...);
 bool failed();
 
 double *partials(size_t ndoubles);
+// Host <-> device copies ordered after every kernel queued on the library's
+// stream (a non-blocking stream, which plain hipMemcpy does not wait for);
+// they return when the copy is done.
+void copy_dtoh(void *dst, const void *src, size_t bytes);
+void copy_htod(void *dst, const void *src, size_t bytes);
 int *flag_buffer(size_t nints);
 double *scratch_vector(size_t ndoubles);
 

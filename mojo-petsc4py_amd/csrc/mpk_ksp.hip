@@ -886,10 +886,10 @@ struct Solve {
   void finish_out(double *cfg, int64_t *cfgi) {
     sync();
     KspState out;
-    HIPCHK(hipMemcpy(&out, d, sizeof out, hipMemcpyDeviceToHost));
+    copy_dtoh(&out, d, sizeof out);
     i64 its = out.its;
     std::vector<double> hist((size_t)h.max_it + 2, 0.0);
-    HIPCHK(hipMemcpy(hist.data(), h.hist, hist.size() * sizeof(double), hipMemcpyDeviceToHost));
+    copy_dtoh(hist.data(), h.hist, hist.size() * sizeof(double));
     cfg[7] = out.rnorm;
     cfgi[10] = its;
     cfgi[11] = out.reason;
@@ -1080,7 +1080,7 @@ void mp_ksp_solve_bcgs(int64_t n, int64_t irow, int64_t icol, int64_t data, int6
   sync();
   ew2(n, OpBcgsSpecial{X, P, S.d, aligned16(X) && aligned16(P)}, nullptr);
   KspState hs;
-  HIPCHK(hipMemcpy(&hs, S.d, sizeof hs, hipMemcpyDeviceToHost));
+  copy_dtoh(&hs, S.d, sizeof hs);
   if (!hs.stopped_at_init) S.unwind(X, T, nullptr);
   if (has_guess) vaxpy(X, 1.0, GUESS, n, nullptr);
   S.finish_out((double *)cfg, (int64_t *)cfgi);

@@ -78,7 +78,7 @@ __global__ void k_sell_fill(i64 nrow, const int *rp, const int *ci, const double
 // slice offsets from the row pointers; returns the padded entry count
 i64 sell_offsets(const int *rp_d, i64 nrow, std::vector<int> &soff) {
   std::vector<int> rp(nrow + 1);
-  HIPCHK(hipMemcpy(rp.data(), rp_d, (nrow + 1) * sizeof(int), hipMemcpyDeviceToHost));
+  copy_dtoh(rp.data(), rp_d, (nrow + 1) * sizeof(int));
   i64 ns = (nrow + 63) / 64;
   soff.assign(ns + 1, 0);
   i64 total = 0;
@@ -111,7 +111,7 @@ SellTemp::SellTemp(DevCSR &A) {
   HIPCHK(hipMalloc(&off, soff.size() * sizeof(int)));
   HIPCHK(hipMalloc(&v, (total > 0 ? total : 1) * sizeof(double)));
   HIPCHK(hipMalloc(&c, (total > 0 ? total : 1) * sizeof(int)));
-  HIPCHK(hipMemcpy(off, soff.data(), soff.size() * sizeof(int), hipMemcpyHostToDevice));
+  copy_htod(off, soff.data(), soff.size() * sizeof(int));
   MPK_LAUNCH(k_sell_fill, nblocks(A.nrow, 256), 256, A.nrow, A.rp, A.ci, A.v, (const int *)off, v, c);
   if (failed()) return;
   A.sell_off = off;
@@ -177,7 +177,7 @@ bool csr_transpose_map(const int *rp, const int *ci, i64 nrow, i64 ncol, int *tr
 
 static i64 csr_nnz(const int *rp, i64 nrow) {
   int h = 0;
-  HIPCHK(hipMemcpy(&h, rp + nrow, sizeof(int), hipMemcpyDeviceToHost));
+  copy_dtoh(&h, rp + nrow, sizeof(int));
   return h;
 }
 
@@ -322,7 +322,7 @@ int64_t mpk_sell_build(int64_t irow, int64_t icol, int64_t data, int64_t nrow, i
   if (nrow <= 0 || !ctx()) return 0;
   std::vector<int> off;
   sell_offsets((const int *)irow, nrow, off);
-  HIPCHK(hipMemcpy((void *)soff, off.data(), off.size() * sizeof(int), hipMemcpyHostToDevice));
+  copy_htod((void *)soff, off.data(), off.size() * sizeof(int));
   MPK_LAUNCH(k_sell_fill, nblocks(nrow, 256), 256, (i64)nrow, (const int *)irow, (const int *)icol,
              (const double *)data, (const int *)soff, (double *)sv, (int *)sc);
   return failed() ? -1 : 0;

@@ -59,20 +59,95 @@ __global__ void k_lp_sell(i64 n, const int *rp, const int *soff, const double *s
   for (int k = 0; k < len; k++) lv[base + 64 * k] = lp_from<CT>((float)(sv[base + 64 * k] / s));
 }
 
-template <class CT>
-__global__ void __launch_bounds__(256) k_spmv_lp(i64 n, const int *__restrict__ rp, const int *__restrict__ soff,
-                                                 const int *__restrict__ sc, const CT *__restrict__ sv,
-                                                 const float *__restrict__ rs, const float *__restrict__ x,
-                                                 float *__restrict__ y) {
-  const unsigned b = xcd_block(blockIdx.x, gridDim.x);
-  const i64 i = (i64)b * 256 + threadIdx.x;
-  if (i >= n) return;
-  const int len = rp[i + 1] - rp[i], base = soff[i >> 6] + (int)(i & 63);
-  float s = 0.0f;
-  for (int k = 0; k < len; k++) {
-    const int p = base + 64 * k;
-    s = fmaf(lp_f(sv[p]), x[sc[p]], s);
+// Windowed products.  The sliced matrices of the inner solve are read in
+// blocks of LP_WROWS rows.  When the columns one block reads span fewer than
+// LP_XWIN entries of x (matrices whose entries stay near the diagonal), the
+// block first copies that window of x into LDS with coalesced loads and
+// gathers from there; otherwise it gathers from x directly.  The values
+// gathered, and so the results, are the same either way.
+#define LP_WROWS 512
+#define LP_XWIN 12288
+
+// Column window of each block: win[2b], win[2b+1] = the smallest and largest
+// column read by block b's rows (0, -1 when they read none).
+__global__ void __launch_bounds__(LP_WROWS) k_col_window(i64 n, const int *rp, const int *soff, const int *sc,
+                                                         int *win) {
+  __shared__ int lo_sh, hi_sh;
+  if (threadIdx.x == 0) {
+    lo_sh = 0x7fffffff;
+    hi_sh = -1;
   }
+  __syncthreads();
+  const i64 i = (i64)blockIdx.x * LP_WROWS + threadIdx.x;
+  int lo = 0x7fffffff, hi = -1;
+  if (i < n) {
+    const int len = rp[i + 1] - rp[i], base = soff[i >> 6] + (int)(i & 63);
+    for (int k = 0; k < len; k++) {
+      const int c = sc[base + 64 * k];
+      lo = c < lo ? c : lo;
+      hi = c > hi ? c : hi;
+    }
+  }
+  if (hi >= 0) {
+    atomicMin(&lo_sh, lo);
+    atomicMax(&hi_sh, hi);
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    win[2 * blockIdx.x] = hi_sh >= 0 ? lo_sh : 0;
+    win[2 * blockIdx.x + 1] = hi_sh;
+  }
+}
+
+// Copies block b's window of x into xs when it fits; returns whether it did
+// (every thread of the block calls this).
+__device__ __forceinline__ bool stage_window(const float *__restrict__ x, const int *__restrict__ win, unsigned b,
+                                             float *xs, int &lo) {
+  lo = win[2 * b];
+  const int hi = win[2 * b + 1];
+  const bool fit = hi - lo < LP_XWIN;
+  if (fit)
+    for (int c = lo + (int)threadIdx.x; c <= hi; c += LP_WROWS) xs[c - lo] = x[c];
+  __syncthreads();
+  return fit;
+}
+
+// Row i's serial sum over its sliced entries, four entries' loads issued
+// before their values are gathered (from xs when the window is staged).
+template <class CT>
+__device__ __forceinline__ float sliced_row(const int *__restrict__ sc, const CT *__restrict__ sv, int base, int len,
+                                            const float *__restrict__ x, const float *xs, bool fit, int lo) {
+  float s = 0.0f;
+  for (int k0 = 0; k0 < len; k0 += 4) {
+    int c[4];
+    float v[4];
+#pragma unroll
+    for (int e = 0; e < 4; e++) {
+      const int p = base + 64 * (k0 + e);
+      c[e] = k0 + e < len ? sc[p] : 0;
+      v[e] = k0 + e < len ? lp_f(sv[p]) : 0.0f;
+    }
+#pragma unroll
+    for (int e = 0; e < 4; e++)
+      if (k0 + e < len) s = fmaf(v[e], fit ? xs[c[e] - lo] : x[c[e]], s);
+  }
+  return s;
+}
+
+// WIN: the kernel stages windows (matrices whose blocks' windows fit LDS);
+// else it gathers from x with no LDS at all.
+template <class CT, bool WIN>
+__global__ void __launch_bounds__(LP_WROWS) k_spmv_lp(i64 n, const int *__restrict__ rp, const int *__restrict__ soff,
+                                                      const int *__restrict__ sc, const CT *__restrict__ sv,
+                                                      const float *__restrict__ rs, const int *__restrict__ win,
+                                                      const float *__restrict__ x, float *__restrict__ y) {
+  __shared__ float xs[WIN ? LP_XWIN : 1];
+  const unsigned b = xcd_block(blockIdx.x, gridDim.x);
+  int lo = 0;
+  const bool fit = WIN ? stage_window(x, win, b, xs, lo) : false;
+  const i64 i = (i64)b * LP_WROWS + threadIdx.x;
+  if (i >= n) return;
+  const float s = sliced_row(sc, sv, soff[i >> 6] + (int)(i & 63), rp[i + 1] - rp[i], x, xs, fit, lo);
   y[i] = rs ? s * rs[i] : s;
 }
 
@@ -270,23 +345,20 @@ struct LpSorBwd {
 // 1 (u_in null) keeps only the row's own term; pass j adds the entries' values
 // from pass j - 1.  Every pass is one bandwidth-bound sweep over the sliced
 // entries, with no ordering between rows.
-template <class Prog, class CT>
-__global__ void __launch_bounds__(256) k_tri_jacobi(Prog prog, i64 n, const int *__restrict__ rp,
-                                                    const int *__restrict__ soff, const int *__restrict__ sc,
-                                                    const CT *__restrict__ sv, const float *__restrict__ u_in,
-                                                    float *__restrict__ u_out) {
+template <class Prog, class CT, bool WIN>
+__global__ void __launch_bounds__(LP_WROWS) k_tri_jacobi(Prog prog, i64 n, const int *__restrict__ rp,
+                                                         const int *__restrict__ soff, const int *__restrict__ sc,
+                                                         const CT *__restrict__ sv, const int *__restrict__ win,
+                                                         const float *__restrict__ u_in, float *__restrict__ u_out) {
+  __shared__ float xs[WIN ? LP_XWIN : 1];
   const unsigned b = xcd_block(blockIdx.x, gridDim.x);
-  const i64 i = (i64)b * 256 + threadIdx.x;
+  int lo = 0;
+  const bool fit = WIN && u_in ? stage_window(u_in, win, b, xs, lo) : false;  // uniform over the block
+  const i64 i = (i64)b * LP_WROWS + threadIdx.x;
   if (i >= n) return;
   const LpStaged st = prog.stage(i);
-  float acc = 0.0f;
-  if (u_in) {
-    const int len = rp[i + 1] - rp[i], base = soff[i >> 6] + (int)(i & 63);
-    for (int k = 0; k < len; k++) {
-      const int p = base + 64 * k;
-      acc = fmaf(lp_f(sv[p]), u_in[sc[p]], acc);
-    }
-  }
+  const float acc =
+      u_in ? sliced_row(sc, sv, soff[i >> 6] + (int)(i & 63), rp[i + 1] - rp[i], u_in, xs, fit, lo) : 0.0f;
   u_out[i] = prog.fin(i, acc, st);
 }
 __global__ void k_f2h(i64 m, const float *x, __half *y) {
@@ -296,9 +368,23 @@ __global__ void k_f2h(i64 m, const float *x, __half *y) {
 
 // A sweep's entries in the sliced layout (for k_tri_jacobi).
 struct TriSell {
-  int *rp = nullptr, *soff = nullptr, *sc = nullptr;
+  int *rp = nullptr, *soff = nullptr, *sc = nullptr, *win = nullptr;
   void *sv = nullptr;
+  bool windowed = false;
 };
+
+// Column windows of a sliced matrix into win [D]; returns whether the
+// windowed kernels pay: at least 7 in 8 blocks' windows fit LDS.
+static bool col_windows(i64 n, const int *rp, const int *soff, const int *sc, int *win) {
+  const unsigned nb = nblocks(n, LP_WROWS);
+  MPK_LAUNCH(k_col_window, nb, LP_WROWS, n, rp, soff, sc, win);
+  std::vector<int> h(2 * (size_t)nb);
+  copy_dtoh(h.data(), win, h.size() * sizeof(int));
+  if (failed()) return false;
+  i64 fit = 0;
+  for (unsigned b = 0; b < nb; b++) fit += h[2 * b + 1] - h[2 * b] < LP_XWIN;
+  return 8 * fit >= 7 * (i64)nb;
+}
 
 // Low-precision coefficients of one sweep's plan: lc[slot] = coef / row scale.
 template <class CT>
@@ -330,8 +416,8 @@ __global__ void k_lp_sor_idiag(i64 n, const double *a, const int *dmark, double 
 
 static i64 plan_slots(const char *plan_d, int second, i64 n) {
   i64 h[8];
-  HIPCHK(hipMemcpy(h, plan_d, sizeof h, hipMemcpyDeviceToHost));
-  if (second) HIPCHK(hipMemcpy(h, plan_d + h[4], sizeof h, hipMemcpyDeviceToHost));
+  copy_dtoh(h, plan_d, sizeof h);
+  if (second) copy_dtoh(h, plan_d + h[4], sizeof h);
   return h[5] ? n * h[5] : h[1];
 }
 
@@ -477,11 +563,14 @@ struct MixedSolver {
   double *hst = nullptr;  // pinned: res, breakdown
   unsigned *ticket = nullptr;
   LpStaged *lst = nullptr;  // staged rows of the planned sweeps
+  int *winA = nullptr;      // column windows of the low-precision matrix
+  bool winA_on = false;     // products stage them (col_windows)
 
   ~MixedSolver() {
     void *bufs[] = {lv,          rs,          lc0,         lc1,       dinv,      idiag,    work,
-                    r64,         P,           st,          ticket,    lst,       tri[0].rp, tri[0].soff, tri[0].sc,
-                    tri[0].sv,   tri[1].rp,   tri[1].soff, tri[1].sc, tri[1].sv};
+                    r64,         P,           st,          ticket,    lst,       winA,     tri[0].rp,
+                    tri[0].soff, tri[0].sc,   tri[0].sv,   tri[0].win, tri[1].rp, tri[1].soff, tri[1].sc,
+                    tri[1].sv,   tri[1].win};
     hipStreamSynchronize(stream());
     for (void *b : bufs)
       if (b) hipFree(b);
@@ -490,13 +579,28 @@ struct MixedSolver {
 
   template <class CT>
   void spmv_t(const float *x, float *y) {
-    MPK_LAUNCH(k_spmv_lp<CT>, lp_grid(n), 256, n, A64.rp, soff, sc, (const CT *)lv, (const float *)rs, x, y);
+    if (winA_on)
+      MPK_LAUNCH((k_spmv_lp<CT, true>), nblocks(n, LP_WROWS), LP_WROWS, n, A64.rp, soff, sc, (const CT *)lv,
+                 (const float *)rs, (const int *)winA, x, y);
+    else
+      MPK_LAUNCH((k_spmv_lp<CT, false>), nblocks(n, LP_WROWS), LP_WROWS, n, A64.rp, soff, sc, (const CT *)lv,
+                 (const float *)rs, (const int *)winA, x, y);
   }
   void spmv(const float *x, float *y) {
     if (prec == 16)
       spmv_t<__half>(x, y);
     else
       spmv_t<float>(x, y);
+  }
+
+  template <class Prog, class CT>
+  void tri_pass(const Prog &prog, const TriSell &T, const float *in, float *out) {
+    if (T.windowed)
+      MPK_LAUNCH((k_tri_jacobi<Prog, CT, true>), nblocks(n, LP_WROWS), LP_WROWS, prog, n, (const int *)T.rp,
+                 (const int *)T.soff, (const int *)T.sc, (const CT *)T.sv, (const int *)T.win, in, out);
+    else
+      MPK_LAUNCH((k_tri_jacobi<Prog, CT, false>), nblocks(n, LP_WROWS), LP_WROWS, prog, n, (const int *)T.rp,
+                 (const int *)T.soff, (const int *)T.sc, (const CT *)T.sv, (const int *)T.win, in, out);
   }
 
   template <class Prog>
@@ -510,11 +614,9 @@ struct MixedSolver {
         float *out = ((tri_iters - 1 - j) % 2 == 0) ? pub : tmp;
         const float *in = j == 0 ? nullptr : (((tri_iters - j) % 2 == 0) ? pub : tmp);
         if (prec == 16)
-          MPK_LAUNCH((k_tri_jacobi<Prog, __half>), lp_grid(n), 256, prog, n, (const int *)T.rp,
-                     (const int *)T.soff, (const int *)T.sc, (const __half *)T.sv, in, out);
+          tri_pass<Prog, __half>(prog, T, in, out);
         else
-          MPK_LAUNCH((k_tri_jacobi<Prog, float>), lp_grid(n), 256, prog, n, (const int *)T.rp,
-                     (const int *)T.soff, (const int *)T.sc, (const float *)T.sv, in, out);
+          tri_pass<Prog, float>(prog, T, in, out);
       }
       return;
     }
@@ -547,7 +649,7 @@ struct MixedSolver {
     std::vector<float> hrs;
     if (pc.type == PCT_SOR && rs) {
       hrs.resize(n);
-      HIPCHK(hipMemcpy(hrs.data(), rs, n * sizeof(float), hipMemcpyDeviceToHost));
+      copy_dtoh(hrs.data(), rs, n * sizeof(float));
     }
     for (int w = 0; w < 2; w++) {
       std::vector<i64> off;
@@ -582,18 +684,20 @@ struct MixedSolver {
       HIPCHK(hipMalloc(&T.sc, slots * sizeof(int)));
       HIPCHK(hipMalloc(&T.sv, slots * (prec == 16 ? 2 : 4)));
       if (failed()) return false;
-      HIPCHK(hipMemcpy(T.rp, rp.data(), (n + 1) * sizeof(int), hipMemcpyHostToDevice));
-      HIPCHK(hipMemcpy(T.soff, so.data(), (ns + 1) * sizeof(int), hipMemcpyHostToDevice));
-      HIPCHK(hipMemcpy(T.sc, sc.data(), slots * sizeof(int), hipMemcpyHostToDevice));
+      copy_htod(T.rp, rp.data(), (n + 1) * sizeof(int));
+      copy_htod(T.soff, so.data(), (ns + 1) * sizeof(int));
+      copy_htod(T.sc, sc.data(), slots * sizeof(int));
+      HIPCHK(hipMalloc(&T.win, 2 * (i64)nblocks(n, LP_WROWS) * sizeof(int)));
+      T.windowed = col_windows(n, T.rp, T.soff, T.sc, T.win);
       if (prec == 16) {
         float *tmp = nullptr;
         HIPCHK(hipMalloc(&tmp, slots * sizeof(float)));
-        HIPCHK(hipMemcpy(tmp, sv.data(), slots * sizeof(float), hipMemcpyHostToDevice));
+        copy_htod(tmp, sv.data(), slots * sizeof(float));
         MPK_LAUNCH(k_f2h, lp_grid(slots), 256, slots, (const float *)tmp, (__half *)T.sv);
         HIPCHK(hipStreamSynchronize(stream()));
         HIPCHK(hipFree(tmp));
       } else {
-        HIPCHK(hipMemcpy(T.sv, sv.data(), slots * sizeof(float), hipMemcpyHostToDevice));
+        copy_htod(T.sv, sv.data(), slots * sizeof(float));
       }
     }
     return !failed();
@@ -789,7 +893,7 @@ int64_t mpk_mixed_create(int64_t n, int64_t irow, int64_t icol, int64_t data, in
   }
   // low-precision matrix values in the sliced slots
   std::vector<int> so(((size_t)n + 63) / 64 + 1);
-  HIPCHK(hipMemcpy(so.data(), (const void *)soff, so.size() * sizeof(int), hipMemcpyDeviceToHost));
+  copy_dtoh(so.data(), (const void *)soff, so.size() * sizeof(int));
   const i64 slots = so.back() > 0 ? so.back() : 1, esz = prec == 16 ? 2 : 4;
   HIPCHK(hipMalloc(&S->lv, slots * esz));
   if (prec == 16) {
@@ -800,6 +904,8 @@ int64_t mpk_mixed_create(int64_t n, int64_t irow, int64_t icol, int64_t data, in
     MPK_LAUNCH(k_lp_sell<float>, lp_grid(n), 256, n, (const int *)irow, (const int *)soff, (const double *)sv,
                (float *)S->lv, (float *)nullptr);
   }
+  HIPCHK(hipMalloc(&S->winA, 2 * (i64)nblocks(n, LP_WROWS) * sizeof(int)));
+  S->winA_on = col_windows(n, (const int *)irow, (const int *)soff, (const int *)sc, S->winA);
   // low-precision preconditioner
   if (const char *plan = S->pc_plan()) {
     void **dst[2] = {&S->lc0, &S->lc1};
@@ -872,7 +978,7 @@ int64_t mpk_mixed_solve(int64_t h, int64_t b, int64_t x, int64_t cfg_) {
     if (failed() || rel <= rtol || outer >= max_outer || !(rn > 0.0)) break;
     // FP32 right-hand side scaled to unit norm; x += ||r|| d
     double sc[2] = {1.0 / rn, rn};
-    HIPCHK(hipMemcpy(S->st->red + 2, sc, sizeof sc, hipMemcpyHostToDevice));
+    copy_htod(S->st->red + 2, sc, sizeof sc);
     MPK_LAUNCH(k_lp_from_d, lp_grid(n), 256, n, (const double *)R, (const double *)(S->st->red + 2), rhs);
     inner += S->inner == LP_GMRES ? S->gmres(rhs, d, irtol, max_inner) : S->bcgs(rhs, d, irtol, max_inner);
     MPK_LAUNCH(k_lp_update, lp_grid(n), 256, n, X, (const double *)(S->st->red + 3), (const float *)d);

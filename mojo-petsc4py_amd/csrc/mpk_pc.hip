@@ -73,7 +73,7 @@ static bool describe_sweep(const Prog &prog, i64 n, std::vector<i64> &off, std::
   HIPCHK(hipMalloc(&col_d, mm * sizeof(int)));
   HIPCHK(hipMalloc(&coef_d, mm * sizeof(double)));
   if (!failed()) {
-    HIPCHK(hipMemcpy(off_d, off.data(), (n + 1) * sizeof(i64), hipMemcpyHostToDevice));
+    copy_htod(off_d, off.data(), (n + 1) * sizeof(i64));
     MPK_LAUNCH(k_sweep_describe<Prog>, nblocks(n, 256), 256, prog, n, 1, cnt_d, (const i64 *)off_d, col_d, coef_d);
     HIPCHK(hipMemcpyAsync(col.data(), col_d, mm * sizeof(int), hipMemcpyDeviceToHost, stream()));
     HIPCHK(hipMemcpyAsync(coef.data(), coef_d, mm * sizeof(double), hipMemcpyDeviceToHost, stream()));
@@ -106,7 +106,7 @@ static i64 plan_sweep(const Prog &prog, i64 n, bool forward, char *dst, i64 cap,
     set_error(5, "sweep plan larger than its bound");
     return -1;
   }
-  HIPCHK(hipMemcpy(dst, host.data(), bytes, hipMemcpyHostToDevice));
+  copy_htod(dst, host.data(), bytes);
   *depth = host[3];
   return failed() ? -1 : bytes;
 }
@@ -401,11 +401,11 @@ static bool build_icc_map(const int *ui, const int *uj, const int *udiag, i64 n,
   Ctx *c = ctx();
   if (!c) return false;
   std::vector<int> hui(n + 1), hud(n);
-  HIPCHK(hipMemcpy(hui.data(), ui, (n + 1) * sizeof(int), hipMemcpyDeviceToHost));
-  HIPCHK(hipMemcpy(hud.data(), udiag, n * sizeof(int), hipMemcpyDeviceToHost));
+  copy_dtoh(hui.data(), ui, (n + 1) * sizeof(int));
+  copy_dtoh(hud.data(), udiag, n * sizeof(int));
   i64 nnz = n > 0 ? hui[n] : 0;
   std::vector<int> huj(nnz > 0 ? nnz : 1);
-  if (nnz > 0) HIPCHK(hipMemcpy(huj.data(), uj, nnz * sizeof(int), hipMemcpyDeviceToHost));
+  if (nnz > 0) copy_dtoh(huj.data(), uj, nnz * sizeof(int));
   std::vector<int> trp(n + 1, 0);
   for (i64 i = 0; i < n; i++)
     for (int j = hui[i]; j < hud[i]; j++) trp[huj[j] + 1]++;
@@ -426,20 +426,20 @@ static bool build_icc_map(const int *ui, const int *uj, const int *udiag, i64 n,
     tmid[k] = j;
     late += trp[k + 1] - j;
   }
-  HIPCHK(hipMemcpy(trp_d, trp.data(), (n + 1) * sizeof(int), hipMemcpyHostToDevice));
+  copy_htod(trp_d, trp.data(), (n + 1) * sizeof(int));
   if (m > 0) {
-    HIPCHK(hipMemcpy(trow_d, trow.data(), m * sizeof(int), hipMemcpyHostToDevice));
-    HIPCHK(hipMemcpy(tpos_d, tpos.data(), m * sizeof(int), hipMemcpyHostToDevice));
+    copy_htod(trow_d, trow.data(), m * sizeof(int));
+    copy_htod(tpos_d, tpos.data(), m * sizeof(int));
   }
-  if (n > 0) HIPCHK(hipMemcpy(tmid_d, tmid.data(), n * sizeof(int), hipMemcpyHostToDevice));
+  if (n > 0) copy_htod(tmid_d, tmid.data(), n * sizeof(int));
   if (nlate) *nlate = late;
   return !failed();
 }
 
 i64 icc_map_count(const int *ui, const int *udiag, i64 n) {
   std::vector<int> hui(n + 1), hud(n);
-  HIPCHK(hipMemcpy(hui.data(), ui, (n + 1) * sizeof(int), hipMemcpyDeviceToHost));
-  HIPCHK(hipMemcpy(hud.data(), udiag, n * sizeof(int), hipMemcpyDeviceToHost));
+  copy_dtoh(hui.data(), ui, (n + 1) * sizeof(int));
+  copy_dtoh(hud.data(), udiag, n * sizeof(int));
   i64 m = 0;
   for (i64 i = 0; i < n; i++) m += hud[i] - hui[i];
   return m;

@@ -75,6 +75,17 @@ double *partials(size_t need) {
   return c->partials;
 }
 
+void copy_dtoh(void *dst, const void *src, size_t bytes) {
+  if (!bytes || !ctx()) return;
+  HIPCHK(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, stream()));
+  HIPCHK(hipStreamSynchronize(stream()));
+}
+void copy_htod(void *dst, const void *src, size_t bytes) {
+  if (!bytes || !ctx()) return;
+  HIPCHK(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, stream()));
+  HIPCHK(hipStreamSynchronize(stream()));
+}
+
 int *flag_buffer(size_t need) {
   Ctx *c = ctx();
   if (!c) return nullptr;
