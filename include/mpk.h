@@ -235,15 +235,19 @@ void mpk_mixed_free(int64_t handle);
  * No counterpart in the reference: many independent small systems solved by
  * one persistent launch, a workgroup per system at a time (largest first):
  * FP64 iterative refinement around an FP32 inner CG + Jacobi (kind 0) or
- * GMRES(20) + ILU(0) (kind 1, modified Gram-Schmidt), FP64 dots, scalars and
+ * GMRES(20) + ILU(0) (kind 1, classical Gram-Schmidt), FP64 dots, scalars and
  * least squares.  All arrays [D], concatenated over the systems (N rows in
  * all); sys int64[nsys][8] = {row0, n, kind, flv offset, forward levels, blv
- * offset, backward levels, 0}; order int32[nsys]; rp int32[N+1] and ci int32
+ * offset, backward levels, ring}; order int32[nsys]; rp int32[N+1] and ci int32
  * (global columns), v64 float64 / v32 float32 values; jd32 float32[N] Jacobi
  * inverse diagonal; the ILU(0) sweeps as single-block plans (see
  * mpk_sweep_plan): prow, cnt int32[N], dep int32[4N] (position slots), coef
  * float32[4N], level starts flv / blv int32, dinv32 float32[N] (inverted
- * diagonal).  nmax: the largest n.  The caller keeps the arrays alive.
+ * diagonal, in backward position order), f2b int32[N] (forward position ->
+ * backward position of the same row); sys[7] = 1 marks a system whose plans
+ * keep every read within 2048 positions of its level's end and whose dep
+ * slots are then position mod 2048.  nmax: the largest n.  The caller keeps
+ * the arrays alive.
  * mpk_batch_solve: x [D] from b [D], stats [D] float64[nsys][4] = {relres,
  * outer, inner, converged}; cfg [H] float64 {rtol, inner_rtol, max_outer,
  * max_inner, workgroup (128/256/512), grid, use_graph}: with use_graph the
@@ -251,7 +255,7 @@ void mpk_mixed_free(int64_t handle);
 int64_t mpk_batch_create(int64_t nsys, int64_t sys, int64_t order, int64_t rp, int64_t ci, int64_t v64, int64_t v32,
                          int64_t jd32, int64_t fprow, int64_t fcnt, int64_t fdep, int64_t fcoef, int64_t flv,
                          int64_t bprow, int64_t bcnt, int64_t bdep, int64_t bcoef, int64_t blv, int64_t dinv32,
-                         int64_t nmax);
+                         int64_t f2b, int64_t nmax);
 int64_t mpk_batch_solve(int64_t handle, int64_t b, int64_t x, int64_t stats, int64_t cfg);
 void mpk_batch_free(int64_t handle);
 

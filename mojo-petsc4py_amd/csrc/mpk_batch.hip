@@ -2,7 +2,7 @@
 // launch.  Every workgroup takes systems from a queue (largest first) and
 // solves each one alone: FP64 iterative refinement (true residual b - A x and
 // the update of x in FP64) around an FP32 inner solve -- CG with Jacobi, or
-// GMRES(m) with ILU(0) and modified Gram-Schmidt -- whose dot products,
+// GMRES(m) with ILU(0) and classical Gram-Schmidt -- whose dot products,
 // norms, scalars and Hessenberg least squares are FP64.  Everything a system
 // needs is synchronised by the workgroup's own barriers; workgroups never
 // wait on each other.
@@ -13,8 +13,9 @@
 // single-block plans (positions by level): for each of the forward and
 // backward sweep, prow int32[N] (local row of position), pcnt int32[N], 4
 // fixed entry slots per position (dep int32 = position slot, coef float32),
-// level starts int32 (per system, at the system's level offset), and dinv32
-// float32[N] (the factor's inverted diagonal).
+// level starts int32 (per system, at the system's level offset), dinv32
+// float32[N] (the factor's inverted diagonal, in backward position order) and
+// f2b int32[N] (forward position -> backward position of the same row).
 #include <cmath>
 #include <cstring>
 #include <vector>
@@ -27,7 +28,8 @@ namespace mpk {
 enum { BS_CG = 0, BS_GMRES = 1 };
 #define BS_M 20      // GMRES restart
 #define BS_W 4       // sweep entry slots per position
-#define BS_FIELDS 8  // per-system descriptor: row0, n, kind, lf0, nlf, lb0, nlb, 0
+#define BS_FIELDS 8  // per-system descriptor: row0, n, kind, lf0, nlf, lb0, nlb, ring
+#define BS_RING 2048 // LDS ring of sweep values (systems whose plans fit it)
 
 struct BatchArgs {
   i64 nsys;
@@ -38,6 +40,7 @@ struct BatchArgs {
   const float *v32, *jd32;
   const int *fprow, *fcnt, *fdep, *bprow, *bcnt, *bdep, *flv, *blv;
   const float *fcoef, *bcoef, *dinv32;
+  const int *f2b;  // forward position -> backward position of the same row
   const double *b;
   double *x;
   double *stats;  // [nsys][4]: relres, outer, inner, converged
@@ -49,23 +52,22 @@ struct BatchArgs {
   int max_outer, max_inner;
 };
 
-// Sum over the workgroup, result in every thread.  `sh` holds BS/64 doubles
-// plus one broadcast slot.
+// Sum over the workgroup, result in every thread: wave sums into one of two
+// LDS buffers (alternating per call, so one barrier suffices: a buffer is
+// written again only after every thread has passed the next call's
+// barrier), then every thread adds the wave sums in wave order.
 template <int BS>
-__device__ __forceinline__ double wg_sum(double v, double *sh) {
+__device__ __forceinline__ double wg_sum(double v, double *sh, int &par) {
 #pragma unroll
   for (int w = 32; w >= 1; w >>= 1) v += __shfl_down(v, w, 64);
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  double *buf = sh + (par ? BS / 64 : 0);
+  par ^= 1;
+  if (lane == 0) buf[wave] = v;
   __syncthreads();
-  if (lane == 0) sh[wave] = v;
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    double s = 0.0;
-    for (int k = 0; k < BS / 64; k++) s += sh[k];
-    sh[BS / 64] = s;
-  }
-  __syncthreads();
-  return sh[BS / 64];
+  double s = 0.0;
+  for (int k = 0; k < BS / 64; k++) s += buf[k];
+  return s;
 }
 
 template <int BS>
@@ -74,70 +76,301 @@ struct Sys {
   i64 row0, n;
   double *sh;
   int t;
-  __device__ double dot(const float *x, const float *y) const {
+  int par = 0;  // wg_sum's buffer
+  __device__ double wsum(double v) { return wg_sum<BS>(v, sh, par); }
+  // The thread's elements are i = t, t + BS, ...; every loop below takes
+  // them U at a time and issues all of a group's loads before it uses any,
+  // so a thread waits about one memory latency per group rather than one
+  // per element (the order of the arithmetic is the element order).
+  static constexpr int U = 8;
+  static constexpr int UG = 4;  // the same for the Gram-Schmidt passes (four basis vectors each)
+  __device__ double dot(const float *__restrict__ x, const float *__restrict__ y) {
     double s = 0.0;
-    for (i64 i = t; i < n; i += BS) s = fma((double)x[i], (double)y[i], s);
-    return wg_sum<BS>(s, sh);
+    for (i64 i0 = t; i0 < n; i0 += U * BS) {
+      float xv[U], yv[U];
+#pragma unroll
+      for (int u = 0; u < U; u++) {
+        const i64 i = i0 + (i64)u * BS;
+        xv[u] = i < n ? x[i] : 0.0f;
+        yv[u] = i < n ? y[i] : 0.0f;
+      }
+#pragma unroll
+      for (int u = 0; u < U; u++)
+        if (i0 + (i64)u * BS < n) s = fma((double)xv[u], (double)yv[u], s);
+    }
+    return wsum(s);
   }
-  // y = A x (FP32 values; x, y local vectors)
-  __device__ void spmv32(const float *x, float *y) const {
-    for (i64 i = t; i < n; i += BS) {
-      const int j0 = a.rp[row0 + i], j1 = a.rp[row0 + i + 1];
-      float s = 0.0f;
-      for (int j = j0; j < j1; j++) s = fmaf(a.v32[j], x[a.ci[j] - row0], s);
-      y[i] = s;
+  // y = A x (FP32 values; x, y local vectors): UR rows at a time, their
+  // entries KB at a time -- every index and value of a block loaded first,
+  // then every gathered x, so a block costs two memory round trips (each
+  // row's sum still in entry order)
+  static constexpr int UR = 4, KB = 4;
+  __device__ void spmv32(const float *__restrict__ x, float *__restrict__ y) const {
+    for (i64 i0 = t; i0 < n; i0 += UR * BS) {
+      int j0[UR], len[UR], lmax = 0;
+#pragma unroll
+      for (int u = 0; u < UR; u++) {
+        const i64 i = i0 + (i64)u * BS;
+        j0[u] = i < n ? a.rp[row0 + i] : 0;
+        len[u] = i < n ? a.rp[row0 + i + 1] - j0[u] : 0;
+        lmax = len[u] > lmax ? len[u] : lmax;
+      }
+      float sum[UR];
+#pragma unroll
+      for (int u = 0; u < UR; u++) sum[u] = 0.0f;
+      for (int k0 = 0; k0 < lmax; k0 += KB) {
+        int c[UR][KB];
+        float v[UR][KB], xv[UR][KB];
+#pragma unroll
+        for (int u = 0; u < UR; u++)
+#pragma unroll
+          for (int q = 0; q < KB; q++) {
+            const bool in = k0 + q < len[u];
+            c[u][q] = in ? a.ci[j0[u] + k0 + q] - (int)row0 : 0;
+            v[u][q] = in ? a.v32[j0[u] + k0 + q] : 0.0f;
+          }
+#pragma unroll
+        for (int u = 0; u < UR; u++)
+#pragma unroll
+          for (int q = 0; q < KB; q++) xv[u][q] = k0 + q < len[u] ? x[c[u][q]] : 0.0f;
+#pragma unroll
+        for (int u = 0; u < UR; u++)
+#pragma unroll
+          for (int q = 0; q < KB; q++)
+            if (k0 + q < len[u]) sum[u] = fmaf(v[u][q], xv[u][q], sum[u]);
+      }
+#pragma unroll
+      for (int u = 0; u < UR; u++)
+        if (i0 + (i64)u * BS < n) y[i0 + (i64)u * BS] = sum[u];
     }
     __syncthreads();
   }
   // r64 = b - A x (FP64), returns ||r64||
-  __device__ double residual(double *r64) const {
+  __device__ double residual(double *__restrict__ r64) {
     double s = 0.0;
-    for (i64 i = t; i < n; i += BS) {
-      const int j0 = a.rp[row0 + i], j1 = a.rp[row0 + i + 1];
-      double ax = 0.0;
-      for (int j = j0; j < j1; j++) ax = fma(a.v64[j], a.x[a.ci[j]], ax);
-      const double r = a.b[row0 + i] - ax;
-      r64[i] = r;
-      s = fma(r, r, s);
+    for (i64 i0 = t; i0 < n; i0 += U * BS) {
+      int j0[U], len[U], lmax = 0;
+#pragma unroll
+      for (int u = 0; u < U; u++) {
+        const i64 i = i0 + (i64)u * BS;
+        j0[u] = i < n ? a.rp[row0 + i] : 0;
+        len[u] = i < n ? a.rp[row0 + i + 1] - j0[u] : 0;
+        lmax = len[u] > lmax ? len[u] : lmax;
+      }
+      double ax[U];
+#pragma unroll
+      for (int u = 0; u < U; u++) ax[u] = 0.0;
+      for (int k = 0; k < lmax; k++) {
+        double v[U], xv[U];
+#pragma unroll
+        for (int u = 0; u < U; u++) {
+          const bool in = k < len[u];
+          v[u] = in ? a.v64[j0[u] + k] : 0.0;
+          xv[u] = in ? a.x[a.ci[j0[u] + k]] : 0.0;
+        }
+#pragma unroll
+        for (int u = 0; u < U; u++)
+          if (k < len[u]) ax[u] = fma(v[u], xv[u], ax[u]);
+      }
+#pragma unroll
+      for (int u = 0; u < U; u++) {
+        const i64 i = i0 + (i64)u * BS;
+        if (i < n) {
+          const double r = a.b[row0 + i] - ax[u];
+          r64[i] = r;
+          s = fma(r, r, s);
+        }
+      }
     }
-    return sqrt(wg_sum<BS>(s, sh));
+    return sqrt(wsum(s));
   }
-  // y = M^-1 x for ILU(0): forward sweep into val (by position), scatter F by
-  // row, backward sweep, y by row
-  __device__ void ilu(const float *x, float *y, float *val, float *F, i64 lf0, i64 nlf, i64 lb0, i64 nlb) const {
-    for (i64 l = 0; l < nlf; l++) {
-      const int pa = a.flv[lf0 + l], pb = a.flv[lf0 + l + 1];
-      for (int p = pa + t; p < pb; p += BS) {
-        const i64 gp = row0 + p;
-        const int i = a.fprow[gp];
-        float acc = x[i];
-        for (int e = 0; e < a.fcnt[gp]; e++) acc = fmaf(a.fcoef[gp * BS_W + e], val[a.fdep[gp * BS_W + e]], acc);
-        val[p] = acc;
-        F[i] = acc;
+  // Classical Gram-Schmidt over the basis V_0..V_k (V_j at V + j n):
+  // h_j = V_j . w for every j in one pass over w and the basis (four basis
+  // vectors per pass, each thread's elements in order), then the workgroup
+  // sums into hsh[0..k] (LDS; buf holds (BS_M + 1) BS/64 partial sums).
+  __device__ void mdot(const float *__restrict__ V, const float *__restrict__ w, int k, double *hsh,
+                       double *buf) const {
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    for (int j0 = 0; j0 <= k; j0 += 4) {
+      const float *vq[4];
+#pragma unroll
+      for (int q = 0; q < 4; q++) vq[q] = V + (i64)(j0 + q <= k ? j0 + q : k) * n;
+      double acc[4] = {0.0, 0.0, 0.0, 0.0};
+      for (i64 i0 = t; i0 < n; i0 += UG * BS) {
+        float wv[UG], vv[4][UG];
+#pragma unroll
+        for (int u = 0; u < UG; u++) {
+          const i64 i = i0 + (i64)u * BS;
+          wv[u] = i < n ? w[i] : 0.0f;
+#pragma unroll
+          for (int q = 0; q < 4; q++) vv[q][u] = i < n ? vq[q][i] : 0.0f;
+        }
+#pragma unroll
+        for (int q = 0; q < 4; q++)
+#pragma unroll
+          for (int u = 0; u < UG; u++)
+            if (i0 + (i64)u * BS < n) acc[q] = fma((double)vv[q][u], (double)wv[u], acc[q]);
       }
-      __syncthreads();
-    }
-    for (i64 l = 0; l < nlb; l++) {
-      const int pa = a.blv[lb0 + l], pb = a.blv[lb0 + l + 1];
-      for (int p = pa + t; p < pb; p += BS) {
-        const i64 gp = row0 + p;
-        const int i = a.bprow[gp];
-        float acc = F[i];
-        for (int e = 0; e < a.bcnt[gp]; e++) acc = fmaf(a.bcoef[gp * BS_W + e], val[a.bdep[gp * BS_W + e]], acc);
-        const float v = acc * a.dinv32[row0 + i];
-        val[p] = v;
-        y[i] = v;
+#pragma unroll
+      for (int q = 0; q < 4; q++) {
+        double v = acc[q];
+#pragma unroll
+        for (int o = 32; o >= 1; o >>= 1) v += __shfl_down(v, o, 64);
+        if (lane == 0 && j0 + q <= k) buf[(j0 + q) * (BS / 64) + wave] = v;
       }
-      __syncthreads();
     }
+    __syncthreads();
+    if (t <= k) {
+      double sum = 0.0;
+      for (int q = 0; q < BS / 64; q++) sum += buf[t * (BS / 64) + q];
+      hsh[t] = sum;
+    }
+    __syncthreads();
+  }
+  // w -= sum_j h_j V_j (j <= k, in j order for each element), four basis
+  // vectors per pass
+  __device__ void maxpy(const float *__restrict__ V, const double *hsh, int k, float *__restrict__ w) const {
+    for (int j0 = 0; j0 <= k; j0 += 4) {
+      const int nq = k - j0 + 1 < 4 ? k - j0 + 1 : 4;
+      float hq[4];
+      const float *vq[4];
+#pragma unroll
+      for (int q = 0; q < 4; q++) {
+        hq[q] = q < nq ? (float)hsh[j0 + q] : 0.0f;
+        vq[q] = V + (i64)(q < nq ? j0 + q : j0) * n;
+      }
+      for (i64 i0 = t; i0 < n; i0 += UG * BS) {
+        float wv[UG], vv[4][UG];
+#pragma unroll
+        for (int u = 0; u < UG; u++) {
+          const i64 i = i0 + (i64)u * BS;
+          wv[u] = i < n ? w[i] : 0.0f;
+#pragma unroll
+          for (int q = 0; q < 4; q++) vv[q][u] = i < n ? vq[q][i] : 0.0f;
+        }
+#pragma unroll
+        for (int u = 0; u < UG; u++) {
+          const i64 i = i0 + (i64)u * BS;
+          if (i < n) {
+            float x = wv[u];
+#pragma unroll
+            for (int q = 0; q < 4; q++)
+              if (q < nq) x = fmaf(-hq[q], vv[q][u], x);
+            w[i] = x;
+          }
+        }
+      }
+    }
+  }
+  // y = a x + y over the thread's elements (the axpy-shaped loops of the
+  // solvers), U at a time
+  __device__ void axpy(float al, const float *__restrict__ x, float *__restrict__ y) const {
+    for (i64 i0 = t; i0 < n; i0 += U * BS) {
+      float xv[U], yv[U];
+#pragma unroll
+      for (int u = 0; u < U; u++) {
+        const i64 i = i0 + (i64)u * BS;
+        xv[u] = i < n ? x[i] : 0.0f;
+        yv[u] = i < n ? y[i] : 0.0f;
+      }
+#pragma unroll
+      for (int u = 0; u < U; u++) {
+        const i64 i = i0 + (i64)u * BS;
+        if (i < n) y[i] = fmaf(al, xv[u], yv[u]);
+      }
+    }
+  }
+  // A position's inputs for a sweep, all stored in position order (entry
+  // count, slots, start value, inverted diagonal, output index), so each
+  // level's are loaded two levels ahead with no load waiting on another.
+  struct Row {
+    int p, o, cnt;
+    float x0, dv;
+    float c[BS_W];
+    int d[BS_W];
+  };
+  __device__ Row row_pos(const int *omap, const int *pcnt, const float *coef, const int *dep, const float *xpos,
+                         const float *dpos, int p, bool valid) const {
+    Row r;
+    r.p = valid ? p : -1;
+    if (!valid) return r;
+    const i64 gp = row0 + p;
+    r.o = omap[gp];
+    r.cnt = pcnt[gp];
+#pragma unroll
+    for (int e = 0; e < BS_W; e++) {
+      r.c[e] = coef[gp * BS_W + e];
+      r.d[e] = dep[gp * BS_W + e];
+    }
+    r.x0 = xpos[p];
+    r.dv = dpos ? dpos[gp] : 1.0f;
+    return r;
+  }
+  // One sweep over levels l0..l0+nl of a single-block plan, each thread's
+  // row of a level loaded two levels ahead.  Values by position in vb (the
+  // LDS ring when mask = BS_RING - 1, the slots then holding position mod
+  // BS_RING; else a global vector by position), and in out at each
+  // position's output index.
+  __device__ void sweep(const int *lv, i64 l0, i64 nl, const int *omap, const int *pcnt, const float *coef,
+                        const int *dep, const float *xpos, const float *dpos, float *vb, int mask, float *out) const {
+    if (nl <= 0) return;
+    int pa = lv[l0], pb = lv[l0 + 1], pc = nl > 1 ? lv[l0 + 2] : pb;
+    Row cur = row_pos(omap, pcnt, coef, dep, xpos, dpos, pa + t, pa + t < pb);
+    Row nxt = row_pos(omap, pcnt, coef, dep, xpos, dpos, pb + t, nl > 1 && pb + t < pc);
+    for (i64 l = 0; l < nl; l++) {
+      const int pd = l + 2 < nl ? lv[l0 + l + 3] : pc;
+      const Row nx2 = row_pos(omap, pcnt, coef, dep, xpos, dpos, pc + t, l + 2 < nl && pc + t < pd);
+      if (cur.p >= 0) finish(cur, vb, mask, out, dpos != nullptr);
+      for (int p = pa + t + BS; p < pb; p += BS)
+        finish(row_pos(omap, pcnt, coef, dep, xpos, dpos, p, true), vb, mask, out, dpos != nullptr);
+      __syncthreads();
+      cur = nxt;
+      nxt = nx2;
+      pa = pb;
+      pb = pc;
+      pc = pd;
+    }
+  }
+  __device__ void finish(const Row &r, float *vb, int mask, float *out, bool scale) const {
+    float acc = r.x0;
+#pragma unroll
+    for (int e = 0; e < BS_W; e++)
+      if (e < r.cnt) acc = fmaf(r.c[e], vb[r.d[e]], acc);
+    const float v = scale ? acc * r.dv : acc;
+    vb[r.p & mask] = v;
+    out[r.o] = v;
+  }
+  // y = M^-1 x for ILU(0).  The forward sweep's start values x[row] are
+  // first copied into position order (xs); the forward sweep writes F in the
+  // backward sweep's position order (f2b), which starts from it and writes y
+  // by row, times the inverted diagonal (dinv32, backward position order).
+  // ring: the system's plans fit the LDS ring sr, else values go through val
+  // (global, by position).
+  __device__ void ilu(const float *__restrict__ x, float *y, float *val, float *F, float *xs, i64 lf0, i64 nlf,
+                      i64 lb0, i64 nlb, bool ring, float *sr) const {
+    for (i64 p0 = t; p0 < n; p0 += U * BS) {
+      int iv[U];
+#pragma unroll
+      for (int u = 0; u < U; u++) iv[u] = p0 + (i64)u * BS < n ? a.fprow[row0 + p0 + (i64)u * BS] : 0;
+#pragma unroll
+      for (int u = 0; u < U; u++)
+        if (p0 + (i64)u * BS < n) xs[p0 + (i64)u * BS] = x[iv[u]];
+    }
+    __syncthreads();
+    float *vb = ring ? sr : val;
+    const int mask = ring ? BS_RING - 1 : -1;
+    sweep(a.flv, lf0, nlf, a.f2b, a.fcnt, a.fcoef, a.fdep, xs, nullptr, vb, mask, F);
+    sweep(a.blv, lb0, nlb, a.bprow, a.bcnt, a.bcoef, a.bdep, F, a.dinv32, vb, mask, y);
   }
 };
 
 template <int BS>
 __global__ void __launch_bounds__(BS) k_batch(BatchArgs a) {
-  __shared__ double sh[BS / 64 + 1];
+  __shared__ double sh[2 * (BS / 64) + 1];  // wg_sum's two buffers, then a broadcast slot
   __shared__ double H[(BS_M + 1) * BS_M], cs[BS_M], sn[BS_M], g[BS_M + 1], yv[BS_M];
+  __shared__ double hcol[BS_M + 1], mbuf[(BS_M + 1) * (BS / 64)];
   __shared__ int sys_sh;
+  __shared__ float sring[BS_RING];
   const int t = threadIdx.x;
   float *W = a.work + (i64)blockIdx.x * a.ws;
   double *r64 = a.work64 + (i64)blockIdx.x * a.ws64;
@@ -159,7 +392,7 @@ __global__ void __launch_bounds__(BS) k_batch(BatchArgs a) {
     __syncthreads();
     double bn = 0.0;
     for (i64 i = t; i < n; i += BS) bn = fma(a.b[row0 + i], a.b[row0 + i], bn);
-    bn = sqrt(wg_sum<BS>(bn, sh));
+    bn = sqrt(S.wsum(bn));
     int outer = 0, inner = 0;
     double rel = 0.0;
     for (;;) {
@@ -174,35 +407,65 @@ __global__ void __launch_bounds__(BS) k_batch(BatchArgs a) {
       __syncthreads();
       if (kind == BS_CG) {
         // CG + Jacobi on A d = rhs (||rhs|| = 1), zero guess
+        // (element-wise loops touch only the thread's own elements; a barrier
+        // is needed only before a product reads other threads' entries, and
+        // the reductions' own barrier provides it)
         for (i64 i = t; i < n; i += BS) {
           r[i] = rhs[i];
           z[i] = a.jd32[row0 + i] * rhs[i];
           p[i] = z[i];
         }
-        __syncthreads();
         double rz = S.dot(r, z);
         for (int it = 0; it < a.max_inner; it++) {
           S.spmv32(p, w);
           const double pq = S.dot(p, w);
           if (!(pq != 0.0)) break;
           const float al = (float)(rz / pq);
-          for (i64 i = t; i < n; i += BS) {
-            dv[i] = fmaf(al, p[i], dv[i]);
-            r[i] = fmaf(-al, w[i], r[i]);
-            z[i] = a.jd32[row0 + i] * r[i];
+          for (i64 i0 = t; i0 < n; i0 += S.U * BS) {
+            float pv[S.U], dvv[S.U], wv[S.U], rv[S.U], jv[S.U];
+#pragma unroll
+            for (int u = 0; u < S.U; u++) {
+              const i64 i = i0 + (i64)u * BS;
+              const bool in = i < n;
+              pv[u] = in ? p[i] : 0.0f;
+              dvv[u] = in ? dv[i] : 0.0f;
+              wv[u] = in ? w[i] : 0.0f;
+              rv[u] = in ? r[i] : 0.0f;
+              jv[u] = in ? a.jd32[row0 + i] : 0.0f;
+            }
+#pragma unroll
+            for (int u = 0; u < S.U; u++) {
+              const i64 i = i0 + (i64)u * BS;
+              if (i < n) {
+                dv[i] = fmaf(al, pv[u], dvv[u]);
+                const float rn = fmaf(-al, wv[u], rv[u]);
+                r[i] = rn;
+                z[i] = jv[u] * rn;
+              }
+            }
           }
-          __syncthreads();
           inner++;
           const double rr = S.dot(r, r);
           if (sqrt(rr) <= a.inner_rtol) break;
           const double rz2 = S.dot(r, z);
           const float be = (float)(rz2 / rz);
           rz = rz2;
-          for (i64 i = t; i < n; i += BS) p[i] = fmaf(be, p[i], z[i]);
+          for (i64 i0 = t; i0 < n; i0 += S.U * BS) {
+            float pv[S.U], zv[S.U];
+#pragma unroll
+            for (int u = 0; u < S.U; u++) {
+              const i64 i = i0 + (i64)u * BS;
+              pv[u] = i < n ? p[i] : 0.0f;
+              zv[u] = i < n ? z[i] : 0.0f;
+            }
+#pragma unroll
+            for (int u = 0; u < S.U; u++)
+              if (i0 + (i64)u * BS < n) p[i0 + (i64)u * BS] = fmaf(be, pv[u], zv[u]);
+          }
           __syncthreads();
         }
       } else {
-        // GMRES(m) + ILU(0), left preconditioned, MGS, zero guess
+        // GMRES(m) + ILU(0), left preconditioned, classical Gram-Schmidt, zero guess
         double target = -1.0;
         int its = 0;
         for (;;) {
@@ -210,7 +473,7 @@ __global__ void __launch_bounds__(BS) k_batch(BatchArgs a) {
           S.spmv32(dv, w);
           for (i64 i = t; i < n; i += BS) z[i] = rhs[i] - w[i];
           __syncthreads();
-          S.ilu(z, w, val, F, d[3], d[4], d[5], d[6]);
+          S.ilu(z, w, val, F, r, d[3], d[4], d[5], d[6], d[7] != 0, sring);
           const double beta = sqrt(S.dot(w, w));
           if (target < 0.0) target = a.inner_rtol * beta;
           if (!(beta > target)) break;
@@ -226,15 +489,12 @@ __global__ void __launch_bounds__(BS) k_batch(BatchArgs a) {
             const float *vk = V + (i64)k * n;
             float *vn = V + (i64)(k + 1) * n;
             S.spmv32(vk, z);
-            S.ilu(z, w, val, F, d[3], d[4], d[5], d[6]);
-            for (int j = 0; j <= k; j++) {
-              const float *vj = V + (i64)j * n;
-              const double h = S.dot(vj, w);
-              if (t == 0) H[k * (BS_M + 1) + j] = h;
-              const float hf = (float)h;
-              for (i64 i = t; i < n; i += BS) w[i] = fmaf(-hf, vj[i], w[i]);
-              __syncthreads();
-            }
+            S.ilu(z, w, val, F, r, d[3], d[4], d[5], d[6], d[7] != 0, sring);
+            // classical Gram-Schmidt (PETSc's default orthogonalisation)
+            S.mdot(V, w, k, hcol, mbuf);
+            if (t == 0)
+              for (int j = 0; j <= k; j++) H[k * (BS_M + 1) + j] = hcol[j];
+            S.maxpy(V, hcol, k, w);
             const double hn = sqrt(S.dot(w, w));
             for (i64 i = t; i < n; i += BS) vn[i] = hn > 0.0 ? (float)(w[i] / hn) : 0.0f;
             if (t == 0) {
@@ -252,10 +512,10 @@ __global__ void __launch_bounds__(BS) k_batch(BatchArgs a) {
               col[k + 1] = 0.0;
               g[k + 1] = -sn[k] * g[k];
               g[k] = cs[k] * g[k];
-              sh[BS / 64] = fabs(g[k + 1]);
+              sh[2 * (BS / 64)] = fabs(g[k + 1]);
             }
             __syncthreads();
-            const double res = sh[BS / 64];
+            const double res = sh[2 * (BS / 64)];
             __syncthreads();
             its++;
             inner++;
@@ -274,11 +534,7 @@ __global__ void __launch_bounds__(BS) k_batch(BatchArgs a) {
             }
           }
           __syncthreads();
-          for (i64 i = t; i < n; i += BS) {
-            float acc = dv[i];
-            for (int j = 0; j < kk; j++) acc = fmaf((float)yv[j], V[(i64)j * n + i], acc);
-            dv[i] = acc;
-          }
+          for (int j = 0; j < kk; j++) S.axpy((float)yv[j], V + (i64)j * n, dv);
           __syncthreads();
           if (done || its >= a.max_inner) break;
         }
@@ -359,7 +615,7 @@ extern "C" {
 int64_t mpk_batch_create(int64_t nsys, int64_t sys, int64_t order, int64_t rp, int64_t ci, int64_t v64,
                          int64_t v32, int64_t jd32, int64_t fprow, int64_t fcnt, int64_t fdep, int64_t fcoef,
                          int64_t flv, int64_t bprow, int64_t bcnt, int64_t bdep, int64_t bcoef, int64_t blv,
-                         int64_t dinv32, int64_t nmax) {
+                         int64_t dinv32, int64_t f2b, int64_t nmax) {
   if (nsys <= 0 || !ctx()) return 0;
   if (nmax <= 0) {
     set_error(7, "batch: nmax must be positive");
@@ -387,6 +643,7 @@ int64_t mpk_batch_create(int64_t nsys, int64_t sys, int64_t order, int64_t rp, i
   a.bcoef = (const float *)bcoef;
   a.blv = (const int *)blv;
   a.dinv32 = (const float *)dinv32;
+  a.f2b = (const int *)f2b;
   HIPCHK(hipMalloc(&B->queue, 64 * sizeof(unsigned)));
   a.queue = B->queue;
   if (failed()) {

@@ -102,7 +102,7 @@ SIGNATURES = {
     "mpk_sell_size": ([I, I], I),
     "mpk_sell_build": ([I] * 7, I),
     "mpk_sell_mult": ([I] * 8, None),
-    "mpk_batch_create": ([I] * 20, I),
+    "mpk_batch_create": ([I] * 21, I),
     "mpk_batch_solve": ([I] * 5, I),
     "mpk_batch_free": ([I], None),
     "mpk_mixed_create": ([I] * 13, I),

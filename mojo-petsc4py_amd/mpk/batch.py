@@ -8,7 +8,7 @@ launch.
 
 Each workgroup of the persistent kernel takes systems from a queue (largest
 first) and solves each alone: FP64 iterative refinement around an FP32 inner
-CG + Jacobi or GMRES(20) + ILU(0) (modified Gram-Schmidt), with FP64 dot
+CG + Jacobi or GMRES(20) + ILU(0) (classical Gram-Schmidt), with FP64 dot
 products, norms, scalars and least squares.  The launch is captured once into
 a HIP graph and replayed.  The library side is mpk_batch_create /
 mpk_batch_solve / mpk_batch_free.
@@ -94,6 +94,24 @@ def _plan(L, n, forward, off, col, coef):
     return lvl, prow, pcnt, edep, ecoef
 
 
+_RING = 2048  # LDS ring of sweep values in the batched kernel (BS_RING)
+
+
+def _ring_slots(lvl, pcnt, edep, n):
+    """A single-block plan's value slots (positions, _W per position) as slots
+    of the LDS ring (position mod _RING), or None when some entry reads a
+    position _RING or more behind the end of its row's level (its slot could
+    have been rewritten by then)."""
+    pos = np.arange(n)
+    lev = np.searchsorted(lvl, pos, side="right") - 1
+    end = np.asarray(lvl, np.int64)[lev + 1]
+    deps = edep.reshape(n, _W).astype(np.int64)
+    used = np.arange(_W)[None, :] < np.asarray(pcnt)[:, None]
+    if np.any((end[:, None] - deps)[used] > _RING):
+        return None
+    return np.where(used, deps % _RING, 0).astype(np.int32).ravel()
+
+
 def build_arrays(systems, L=None) -> dict:
     """The batch's device layout as host arrays (mpk_batch_create documents
     it): concatenated CSR, Jacobi inverse diagonal, ILU(0) sweep plans."""
@@ -123,6 +141,7 @@ def build_arrays(systems, L=None) -> dict:
     fcoef = np.zeros(N * _W, np.float32)
     bcoef = np.zeros(N * _W, np.float32)
     dinv = np.zeros(N, np.float32)
+    f2b = np.zeros(N, np.int32)
     flv, blv = [], []
     nflv = nblv = 0
     sysd = np.zeros((count, _FIELDS), dtype=np.int64)
@@ -168,7 +187,17 @@ def build_arrays(systems, L=None) -> dict:
                                         np.ascontiguousarray(bcf, dtype=np.float64))
             bprow[r0:r0 + n], bcnt[r0:r0 + n] = pr, pc_
             bdep[r0 * _W:(r0 + n) * _W], bcoef[r0 * _W:(r0 + n) * _W] = ed, ec
-            dinv[r0:r0 + n] = ba[bd[:n]]
+            rf = _ring_slots(lf, fcnt[r0:r0 + n], fdep[r0 * _W:(r0 + n) * _W], n)
+            rb = _ring_slots(lb, pc_, ed, n)
+            if rf is not None and rb is not None:
+                fdep[r0 * _W:(r0 + n) * _W], bdep[r0 * _W:(r0 + n) * _W] = rf, rb
+                sysd[s, 7] = 1
+            # by position: the backward sweep's inverted diagonal, and where
+            # each forward position's row sits in the backward order
+            bpos = np.empty(n, np.int64)
+            bpos[bprow[r0:r0 + n]] = np.arange(n)
+            f2b[r0:r0 + n] = bpos[fprow[r0:r0 + n]]
+            dinv[r0:r0 + n] = np.asarray(ba)[np.asarray(bd)[bprow[r0:r0 + n]]]
             sysd[s, 3], sysd[s, 4] = nflv, len(lf) - 1
             sysd[s, 5], sysd[s, 6] = nblv, len(lb) - 1
             flv.append(lf)
@@ -180,7 +209,7 @@ def build_arrays(systems, L=None) -> dict:
     order = np.argsort(-sizes, kind="stable").astype(np.int32)
     return dict(sizes=sizes, row0=row0, N=N, sys=sysd, order=order, rp=rp, ci=ci, v64=v64,
                 v32=v64.astype(np.float32), jd=jd, fprow=fprow, fcnt=fcnt, fdep=fdep, fcoef=fcoef, flv=flv,
-                bprow=bprow, bcnt=bcnt, bdep=bdep, bcoef=bcoef, blv=blv, dinv=dinv, b=b)
+                bprow=bprow, bcnt=bcnt, bdep=bdep, bcoef=bcoef, blv=blv, dinv=dinv, f2b=f2b, b=b)
 
 
 class BatchSolver:
@@ -197,7 +226,7 @@ class BatchSolver:
         self.kinds = [s[4] for s in systems]
         up = DeviceBuffer.from_array
         keys = ("sys", "order", "rp", "ci", "v64", "v32", "jd", "fprow", "fcnt", "fdep", "fcoef", "flv",
-                "bprow", "bcnt", "bdep", "bcoef", "blv", "dinv", "b")
+                "bprow", "bcnt", "bdep", "bcoef", "blv", "dinv", "f2b", "b")
         self._d = {k: up(H[k]) for k in keys}
         self._d["x"] = DeviceBuffer(8 * N)
         self._d["stats"] = DeviceBuffer(8 * 4 * self.count)
@@ -206,7 +235,7 @@ class BatchSolver:
                                      d["v64"].ptr, d["v32"].ptr, d["jd"].ptr, d["fprow"].ptr, d["fcnt"].ptr,
                                      d["fdep"].ptr, d["fcoef"].ptr, d["flv"].ptr, d["bprow"].ptr,
                                      d["bcnt"].ptr, d["bdep"].ptr, d["bcoef"].ptr, d["blv"].ptr,
-                                     d["dinv"].ptr, int(self.sizes.max()))
+                                     d["dinv"].ptr, d["f2b"].ptr, int(self.sizes.max()))
         check()
         if not self._h:
             raise RuntimeError("mpk_batch_create failed")

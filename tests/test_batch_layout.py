@@ -1,7 +1,8 @@
 """The batched solver's host-built layout (mpk.batch.build_arrays; no GPU
 needed): the concatenated CSR and Jacobi diagonal reproduce each system, and
 the ILU(0) sweep plans, run level by level here as the batched kernel runs
-them (in FP64 on the plans' FP32 coefficients), give the oracle's ILU(0)
+them (in FP64 on the plans' FP32 coefficients; through the LDS ring of
+position mod 2048 for the systems flagged for it), give the oracle's ILU(0)
 apply (ILUApplyFactor_SeqAIJ restated) for every GMRES system to within the
 FP32 rounding of the coefficients."""
 
@@ -17,14 +18,16 @@ def L():
     return _native.load(init=False)
 
 
-def _systems(count=12, seed=7):
-    return W.batch_systems(count=count, seed=seed, nmin=30, nmax=700)
+def _systems(count=12, seed=7, nmin=30, nmax=700):
+    return W.batch_systems(count=count, seed=seed, nmin=nmin, nmax=nmax)
 
 
 def _sweeps(H, s, x, coef_f, coef_b):
     """The kernel's ILU(0) apply for system s (FP64 here)."""
-    r0, n, kind, lf0, nlf, lb0, nlb, _ = (int(v) for v in H["sys"][s])
-    val = np.zeros(n)
+    from mpk.batch import _RING
+    r0, n, kind, lf0, nlf, lb0, nlb, ring = (int(v) for v in H["sys"][s])
+    slot = (lambda p: p % _RING) if ring else (lambda p: p)
+    val = np.full(_RING if ring else n, np.nan)
     F = np.zeros(n)
     for lev in range(nlf):
         for p in range(H["flv"][lf0 + lev], H["flv"][lf0 + lev + 1]):
@@ -33,18 +36,21 @@ def _sweeps(H, s, x, coef_f, coef_b):
             acc = x[i]
             for e in range(H["fcnt"][gp]):
                 acc = acc + coef_f[gp * 4 + e] * val[H["fdep"][gp * 4 + e]]
-            val[p] = acc
+            val[slot(p)] = acc
             F[i] = acc
     y = np.zeros(n)
+    # F by backward position, as the kernel's forward sweep writes it (f2b)
+    Fb = np.zeros(n)
+    Fb[H["f2b"][r0:r0 + n]] = F[H["fprow"][r0:r0 + n]]
     for lev in range(nlb):
         for p in range(H["blv"][lb0 + lev], H["blv"][lb0 + lev + 1]):
             gp = r0 + p
             i = H["bprow"][gp]
-            acc = F[i]
+            acc = Fb[p]
             for e in range(H["bcnt"][gp]):
                 acc = acc + coef_b[gp * 4 + e] * val[H["bdep"][gp * 4 + e]]
-            v = acc * H["dinv64"][r0 + i]
-            val[p] = v
+            v = acc * H["dinv64"][gp]
+            val[slot(p)] = v
             y[i] = v
     return y
 
@@ -83,14 +89,50 @@ def test_ilu_plans_reproduce_the_oracle_apply(L, oracle):
         bi, bj, ba, bd = _ilu0(L, ip, ix, va, n)
         r0 = int(H["row0"][s])
         H["dinv64"] = np.zeros(H["N"])
-        H["dinv64"][r0:r0 + n] = ba[bd[:n]]
-        assert np.array_equal(H["dinv"][r0:r0 + n], ba[bd[:n]].astype(np.float32))
+        H["dinv64"][r0:r0 + n] = ba[bd[H["bprow"][r0:r0 + n]]]  # backward position order
+        assert np.array_equal(H["dinv"][r0:r0 + n], H["dinv64"][r0:r0 + n].astype(np.float32))
         x = np.random.default_rng(s).standard_normal(n)
         want = pc.apply(x)
         got32 = _sweeps(H, s, x, H["fcoef"].astype(np.float64), H["bcoef"].astype(np.float64))
         assert np.allclose(got32, want, rtol=1e-5, atol=1e-6 * np.abs(want).max())
         checked += 1
     assert checked >= 4
+
+
+def test_ring_plans_for_grid_systems(L, oracle):
+    """Systems of 2-D grids up to 20000 rows read at most two levels back, so
+    all take the LDS ring; their ring-slot sweeps still give the oracle's
+    ILU(0) apply."""
+    from mpk.batch import build_arrays, _ilu0
+    systems = _systems(count=6, seed=3, nmin=9000, nmax=20000)
+    H = build_arrays(systems, L)
+    oracle.set_mode("ref")
+    for s, (ip, ix, va, b, kind) in enumerate(systems):
+        if kind != "gmres+ilu":
+            continue
+        assert H["sys"][s][7] == 1
+        n = len(ip) - 1
+        r0 = int(H["row0"][s])
+        bi, bj, ba, bd = _ilu0(L, ip, ix, va, n)
+        H["dinv64"] = np.zeros(H["N"])
+        H["dinv64"][r0:r0 + n] = ba[bd[H["bprow"][r0:r0 + n]]]
+        x = np.random.default_rng(s).standard_normal(n)
+        want = oracle.PCSetup(oracle.CSR(ip, ix, va), "ilu", 0).apply(x)
+        got = _sweeps(H, s, x, H["fcoef"].astype(np.float64), H["bcoef"].astype(np.float64))
+        assert np.allclose(got, want, rtol=1e-5, atol=1e-6 * np.abs(want).max())
+
+
+def test_ring_check_refuses_a_far_read():
+    """Plant the mistake on the CPU: a plan entry that reads a position more
+    than the ring behind the end of its level is refused."""
+    from mpk.batch import _ring_slots, _RING
+    n = _RING + 10
+    lvl = np.array([0, _RING + 5, n])
+    pcnt = np.zeros(n, np.int32)
+    edep = np.zeros(n * 4, np.int32)
+    assert _ring_slots(lvl, pcnt, edep, n) is not None
+    pcnt[n - 1] = 1  # the last row reads position 0, n - 0 > _RING behind its level's end
+    assert _ring_slots(lvl, pcnt, edep, n) is None
 
 
 def test_bad_kind_is_rejected(L):
