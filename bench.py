@@ -144,7 +144,27 @@ def _mat(csr):
     return A
 
 
-def bench_mixed(lib, which: str):
+def cpu_sample(csr, b, ksp: str, pc_kind: str, full_its: int, sample_its: int, **kw):
+    """The oracle's (the reference's single-threaded path, restated in C) FP64
+    solve on one core for `sample_its` iterations, extrapolated to the GPU's
+    `full_its`; its preconditioner set-up is timed apart."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as O
+    O.set_mode("ref")
+    A = O.CSR(*csr)
+    pc_kw = {k: kw.pop(k) for k in ("omega", "levels") if k in kw}
+    t0 = time.perf_counter()
+    pc = O.PCSetup(A, pc_kind, *([pc_kw.pop("levels")] if "levels" in pc_kw else []), **pc_kw)
+    t_setup = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    r = O.ksp_solve(A, b, ksp, pc, rtol=1e-10, max_it=sample_its, **kw)
+    per_it = (time.perf_counter() - t0) / max(r.its, 1)
+    return {"value": per_it * full_its, "unit": "s", "cores": 1, "kind": "port",
+            "sample": f"oracle {ksp}+{pc_kind}, {r.its} iterations on one core ({per_it * 1e3:.1f} ms/iteration),"
+                      f" extrapolated to the GPU's {full_its}; preconditioner set-up {t_setup:.2f} s apart"}
+
+
+def bench_mixed(lib, which: str, cpu: bool = False):
     """Mixed precision against plain FP64 on the GPU: the FP64 drop-in solve
     (time to rtol 1e-10 and its true residual), then the mixed solve to an
     FP64 relative residual of 1e-10 with exact sweeps and with Jacobi-pass
@@ -178,8 +198,14 @@ def bench_mixed(lib, which: str):
     x = bv.duplicate()
     t64 = timed_solves(ksp, bv, x, 1, 1)
     x64 = x.getArray().copy()
-    out = {"workload": desc, "fp64": {"seconds": t64, "its": ksp.getIterationNumber(),
+    its64 = ksp.getIterationNumber()
+    out = {"workload": desc, "fp64": {"seconds": t64, "its": its64,
                                       "reason": ksp.getConvergedReason(), "true_relres": true_relres(A, bv, x)}}
+    if cpu:
+        if which == "c3":
+            out["cpu_baseline"] = cpu_sample(csr, b, "gmres", "ilu", its64, 4, levels=0, restart=30)
+        else:
+            out["cpu_baseline"] = cpu_sample(csr, b, "bcgs", "sor", its64, 2, omega=omega)
     for prec, sweeps, irt in runs:
         ms = MixedSolver(A, pc, precision=prec, inner=inner, restart=30, sweeps=sweeps)
         xm = bv.duplicate()
@@ -198,7 +224,7 @@ def bench_mixed(lib, which: str):
     return out
 
 
-def bench_batched(lib, sample: int = 16):
+def bench_batched(lib, sample: int = 16, cpu: bool = False):
     """4096 independent 2-D diffusion systems (n log-uniform in [200, 20000],
     lognormal faces; even: cg + jacobi, odd: gmres(20) + ilu(0)), FP32 inner +
     FP64 refinement to 1e-8, one persistent launch replayed from a HIP graph
@@ -240,13 +266,33 @@ def bench_batched(lib, sample: int = 16):
         tt += time.perf_counter() - t1
     per = tt / sample
     bs.destroy()
+    cpu_line = None
+    if cpu:
+        # the oracle's FP64 solves of 8 of the sampled systems, one core
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import oracle as O
+        O.set_mode("ref")
+        tc = 0.0
+        for s in picks[:8]:
+            ip, ix, va, b, kind = systems[s]
+            A = O.CSR(ip, ix, va)
+            t0 = time.perf_counter()
+            if kind == "cg+jacobi":
+                O.ksp_solve(A, b, "cg", O.PCSetup(A, "jacobi"), rtol=1e-9, max_it=20000)
+            else:
+                O.ksp_solve(A, b, "gmres", O.PCSetup(A, "ilu", 0), rtol=1e-9, max_it=20000, restart=20)
+            tc += time.perf_counter() - t0
+        pc_ = tc / 8
+        cpu_line = {"value": 1.0 / pc_, "unit": "solves/s", "cores": 1, "kind": "port",
+                    "sample": f"oracle FP64 solves (with set-up) of 8 of the systems on one core, "
+                              f"{pc_ * 1e3:.1f} ms each"}
     return {"workload": "4096 x 2-D 5-point diffusion, n log-uniform [200, 20000], cg+jacobi / gmres(20)+ilu(0),"
                         " FP32 inner + FP64 refinement to 1e-8",
             "sum_n": int(sum(len(s_[0]) - 1 for s_ in systems)), "seconds": t, "solves_per_s": len(systems) / t,
             "converged": int(st[:, 3].sum()), "max_relres": float(st[:, 0].max()),
             "autotuned": {"workgroup": tune["workgroup"], "grid": tune["grid"]},
             "one_at_a_time_fp64": {"sample": sample, "ms_per_solve": per * 1e3, "solves_per_s": 1.0 / per},
-            "speedup": per * len(systems) / t}
+            "speedup": per * len(systems) / t, "cpu_baseline": cpu_line}
 
 
 def cpu_model() -> str:
@@ -258,6 +304,49 @@ def cpu_model() -> str:
     except OSError:
         pass
     return "unknown CPU"
+
+
+def kernels_only(lib, A, bv, csr, n, nnz, reps: int = 30):
+    """Each Vec / Mat / PC operation of the path, `reps` back-to-back calls on
+    the 128^3 operator and its vectors (for a kernel-trace run: the per-kernel
+    durations are the measurement), host time per call printed beside it."""
+    from mojopetsc4py import PETSc
+    x, y, z, w = (bv.duplicate() for _ in range(4))
+    for v, c in ((x, 1.0), (y, 0.5), (z, 0.25), (w, 0.125)):
+        v.set(c)
+    vecs = [bv.duplicate() for _ in range(8)]
+    for k, v in enumerate(vecs):
+        v.set(1.0 / (k + 1))
+    alphas = [0.5 ** k for k in range(8)]
+    ops = {
+        "mult": lambda: A.mult(x, y),
+        "multTranspose": lambda: A.multTranspose(x, y),
+        "dot": lambda: x.dot(y),
+        "norm": lambda: x.norm(),
+        "axpy": lambda: y.axpy(0.5, x),
+        "aypx": lambda: y.aypx(0.5, x),
+        "waxpy": lambda: w.waxpy(0.5, x, y),
+        "axpbypcz": lambda: z.axpbypcz(0.5, 0.25, 0.125, x, y),
+        "pointwiseMult": lambda: w.pointwiseMult(x, y),
+        "maxpy8": lambda: y.maxpy(alphas, vecs),
+    }
+    out = {}
+    for kind in ("jacobi", "sor", "icc", "ilu"):
+        pc = PETSc.PC().create()
+        pc.setType(kind)
+        pc.setOperators(A)
+        pc.setUp()
+        ops["pc_" + kind] = (lambda pc=pc: pc.apply(x, y))
+    for name, op in ops.items():
+        op()
+        lib.mpk_synchronize()
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            op()
+        lib.mpk_synchronize()
+        out[name] = (time.perf_counter() - t0) / reps * 1e6
+    print(json.dumps({"n": n, "nnz": nnz, "reps": reps, "host_us_per_call": out}))
+    return 0
 
 
 def main() -> int:
@@ -274,10 +363,12 @@ def main() -> int:
     ap.add_argument("--no-batched", action="store_true", help="skip the batched solve")
     ap.add_argument("--spmv-only", action="store_true",
                     help="only the SpMV timing loop and a copy of known size (for hardware counter runs)")
+    ap.add_argument("--kernels", action="store_true",
+                    help="only each Vec / Mat / PC operation, back to back (for kernel-trace runs)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print("bench.py runs on one GPU: this build has no multi-GPU path (see DESIGN.md for how it"
-              " would shard); run with --gpus 1", file=sys.stderr)
+        print("bench.py runs on one GPU: this build has no multi-GPU path (its sharding is a design"
+              " only); run with --gpus 1", file=sys.stderr)
         return 2
 
     from mpk import _native
@@ -291,6 +382,8 @@ def main() -> int:
     x = bv.duplicate()
     if args.spmv_only:
         return spmv_only(lib, A, bv, n, nnz, args.spmv_reps)
+    if args.kernels:
+        return kernels_only(lib, A, bv, csr, n, nnz)
 
     ksp = make_ksp(A, "jacobi")
     t_solve = timed_solves(ksp, bv, x, args.steps, args.warmup)
@@ -321,9 +414,9 @@ def main() -> int:
 
     mixed = {}
     if not args.no_mixed:
-        mixed["convection_diffusion_128"] = bench_mixed(lib, "c3")
-        mixed["random_spd_1e6"] = bench_mixed(lib, "c4")
-    batched = None if args.no_batched else bench_batched(lib)
+        mixed["convection_diffusion_128"] = bench_mixed(lib, "c3", cpu=not args.no_cpu)
+        mixed["random_spd_1e6"] = bench_mixed(lib, "c4", cpu=not args.no_cpu)
+    batched = None if args.no_batched else bench_batched(lib, cpu=not args.no_cpu)
 
     cpu = None
     if not args.no_cpu:
