@@ -81,10 +81,9 @@ class MatSORType:
 
 class _DevCSR:
     """The device copy: int32 row pointers and columns, float64 values, and
-    (built on first use) the sliced copy the products read and the transpose
-    map for A^T products."""
+    (built on first use) the sliced copies of A and of A^T the products read."""
 
-    __slots__ = ("rp", "ci", "v", "nnz", "nrows", "ncols", "tmap", "sellc")
+    __slots__ = ("rp", "ci", "v", "nnz", "nrows", "ncols", "sellc", "tsellc")
 
     def __init__(self, irow, icol, data, nrows, ncols):
         nnz = int(irow[nrows]) if nrows else 0
@@ -96,8 +95,8 @@ class _DevCSR:
         self.rp = DeviceBuffer.from_array(np.asarray(irow[: nrows + 1], dtype=np.int32))
         self.ci = DeviceBuffer.from_array(np.asarray(icol[: max(nnz, 1)], dtype=np.int32))
         self.v = DeviceBuffer.from_array(np.asarray(data[: max(nnz, 1)], dtype=np.float64))
-        self.tmap = None
         self.sellc = None
+        self.tsellc = None
 
     def sell(self):
         """(soff, sv, sc): rows in 64-row slices, column-major inside a slice,
@@ -114,16 +113,33 @@ class _DevCSR:
             self.sellc = (soff, sv, sc)
         return self.sellc
 
-    def transpose_map(self):
-        if self.tmap is None:
-            trp = DeviceBuffer(4 * (self.ncols + 1))
-            trow = DeviceBuffer(4 * max(self.nnz, 1))
-            tpos = DeviceBuffer(4 * max(self.nnz, 1))
-            lib().mpk_csr_transpose(self.rp.ptr, self.ci.ptr, self.nrows, self.ncols, trp.ptr, trow.ptr,
-                                    tpos.ptr)
+    def transpose_sell(self, irow, icol, data):
+        """(trp, soff, sv, sc): the sliced copy of A^T, built from the host CSR
+        the device copy was made from: column c's entries in increasing row,
+        the order MatMultTranspose_SeqAIJ accumulates them in, so a sliced
+        product of it gives the transpose product bit for bit."""
+        if self.tsellc is None:
+            L = lib()
+            nr, nc, nnz = self.nrows, self.ncols, self.nnz
+            ix = np.asarray(icol[:nnz], dtype=np.int64)
+            rows = np.repeat(np.arange(nr, dtype=np.int64), np.diff(np.asarray(irow[: nr + 1], dtype=np.int64)))
+            order = np.argsort(ix, kind="stable")
+            trp = np.zeros(nc + 1, dtype=np.int32)
+            trp[1:] = np.cumsum(np.bincount(ix, minlength=nc))
+            trp_d = DeviceBuffer.from_array(trp)
+            trow_d = DeviceBuffer.from_array(rows[order].astype(np.int32) if nnz else np.zeros(1, np.int32))
+            vt_d = DeviceBuffer.from_array(np.asarray(data[:nnz], dtype=np.float64)[order] if nnz
+                                           else np.zeros(1))
+            padded = L.mpk_sell_size(trp_d.ptr, nc)
             check()
-            self.tmap = (trp, trow, tpos)
-        return self.tmap
+            soff = DeviceBuffer(4 * ((nc + 63) // 64 + 1))
+            sv = DeviceBuffer(8 * max(padded, 1))
+            sc = DeviceBuffer(4 * max(padded, 1))
+            if L.mpk_sell_build(trp_d.ptr, trow_d.ptr, vt_d.ptr, nc, soff.ptr, sv.ptr, sc.ptr) != 0:
+                check()
+            L.mpk_synchronize()
+            self.tsellc = (trp_d, soff, sv, sc)
+        return self.tsellc
 
 
 class Mat:
@@ -527,8 +543,9 @@ class Mat:
         xa = x._d()
         if self._ncols:
             if self._nrows:
-                trp, trow, tpos = d.transpose_map()
-                lib().mpk_mat_mult_transpose_t(xa, out._dw(), d.v.ptr, self._ncols, trp.ptr, trow.ptr, tpos.ptr)
+                # the sliced copy of A^T (MatMultTranspose_SeqAIJ's order, ported.mojo:408-431)
+                trp, so, sv, sc = d.transpose_sell(self._irow, self._icol, self._data)
+                lib().mpk_sell_mult(xa, out._dw(), trp.ptr, so.ptr, sv.ptr, sc.ptr, self._ncols, 0)
             else:
                 out.zeroEntries()
         check()

@@ -194,3 +194,46 @@ def test_factor_apply_bitwise(native, O, name, kind, levels, ordering):
         native.mp_pc_icc_apply(db.ptr, dx.ptr, d["bi"].ptr, d["bj"].ptr, d["ba"].ptr, d["bdiag"].ptr,
                                d["r"].ptr, 0, n)
     assert same_bits(back(dx, np.float64, n), want)
+
+
+def _rect(m, n, seed):
+    """An m x n matrix with empty rows and columns no row touches."""
+    rng = np.random.default_rng(seed)
+    ip, ix = [0], []
+    for i in range(m):
+        cs = [] if i % 5 == 2 else sorted(set(rng.integers(0, max(n - 3, 1), int(rng.integers(1, 9))).tolist()))
+        ix.extend(cs)
+        ip.append(len(ix))
+    return (np.array(ip, np.int64), np.array(ix, np.int64), rng.normal(size=len(ix)))
+
+
+@pytest.mark.parametrize("shape", [(64, 64), (300, 170), (170, 300), (1, 5), (700, 700)])
+def test_mat_surface_products_bitwise(O, shape):
+    """Mat.mult / multTranspose / multTransposeAdd through the PETSc surface
+    against the oracle (MatMult_SeqAIJ, MatMultTranspose_SeqAIJ), bit for bit,
+    again after the values change (the sliced copies of A and A^T rebuilt)."""
+    from mojopetsc4py import PETSc
+    m, n = shape
+    ip, ix, va = _rect(m, n, m * 7 + n)
+    A = PETSc.Mat().createSeqAIJ((m, n), nnz=9)
+    A.setValuesCSR(ip, ix, va)
+    A.assemble()
+    for rnd in range(2):
+        xm, xn = rng_vec(m, 5 + rnd), rng_vec(n, 7 + rnd)
+        ym = A.createVecLeft()
+        A.mult(PETSc.Vec().createWithArray(xn), ym)
+        want = np.zeros(m)
+        O.lib().or_mat_mult(xn.ctypes.data, want.ctypes.data, ip.ctypes.data, ix.ctypes.data, va.ctypes.data, m)
+        assert same_bits(ym.getArray(), want)
+        yn = A.createVecRight()
+        A.multTranspose(PETSc.Vec().createWithArray(xm), yn)
+        want = np.zeros(n)
+        O.lib().or_mat_mult_transpose(xm.ctypes.data, want.ctypes.data, ip.ctypes.data, ix.ctypes.data,
+                                      va.ctypes.data, m, n)
+        assert same_bits(yn.getArray(), want)
+        v = rng_vec(n, 11)
+        zn = A.multTransposeAdd(PETSc.Vec().createWithArray(xm), PETSc.Vec().createWithArray(v))
+        assert same_bits(zn.getArray(), want + v)
+        va = va * 1.5 - 0.25
+        A.setValuesCSR(ip, ix, va)
+        A.assemble()
