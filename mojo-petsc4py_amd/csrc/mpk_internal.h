@@ -132,6 +132,52 @@ __device__ __forceinline__ unsigned xcd_block(unsigned b, unsigned nb) {
   return x < rem ? x * (per + 1) + k : rem * (per + 1) + (x - rem) * per + k;
 }
 
+// Loads of the sliced matrix stream and stores of the product: non-temporal
+// (NT) for matrices well past the last-level cache, so neither stream pushes
+// out of L2 the lines of x that rows a grid plane apart read again; ordinary
+// otherwise, so a matrix that fits the cache stays resident across products.
+template <bool NT, class T>
+__device__ __forceinline__ T ld_stream(const T *p) {
+  if constexpr (NT)
+    return __builtin_nontemporal_load(p);
+  else
+    return *p;
+}
+
+template <bool NT, class T>
+__device__ __forceinline__ void st_stream(T *p, T v) {
+  if constexpr (NT)
+    __builtin_nontemporal_store(v, p);
+  else
+    *p = v;
+}
+
+// One row of a sliced product: entries base + 64 k, k < len, summed in order
+// onto s (fma(v, x[c], s), k = 0, 1, ...).  The values and columns of four
+// entries are loaded before their gathers of x, so each lane keeps several
+// loads in flight; the arithmetic order is the serial one.
+template <bool NT>
+__device__ __forceinline__ double sell_row(const double *__restrict__ sv, const int *__restrict__ sc,
+                                           const double *__restrict__ x, int base, int len, double s) {
+  for (int k = 0; k < len; k += 4) {
+    int c[4];
+    double v[4], xv[4];
+#pragma unroll
+    for (int j = 0; j < 4; j++)
+      if (k + j < len) {
+        c[j] = ld_stream<NT>(sc + base + 64 * (k + j));
+        v[j] = ld_stream<NT>(sv + base + 64 * (k + j));
+      }
+#pragma unroll
+    for (int j = 0; j < 4; j++)
+      if (k + j < len) xv[j] = x[c[j]];
+#pragma unroll
+    for (int j = 0; j < 4; j++)
+      if (k + j < len) s = fma(v[j], xv[j], s);
+  }
+  return s;
+}
+
 // Halving tree over the 64 lanes of a wave: u[i] += u[i + w], w = 32..1.
 // Lane 0 ends with the wave's sum; the other lanes hold partial values.
 __device__ __forceinline__ double wave_tree(double u) {

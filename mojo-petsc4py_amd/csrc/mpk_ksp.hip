@@ -477,6 +477,7 @@ struct OpCgP {
 // the pair's leaf, which goes to LDS where the tile reduction keeps it (pair
 // 512 j + 2t is thread t's j-th); the leaves are then folded in that
 // reduction's order, so the partial is k_partials'.
+template <bool NT>
 __global__ void __launch_bounds__(4 * MPK_NT) k_cg_wdot(i64 n, const int *__restrict__ rp,
                                                         const int *__restrict__ soff, const double *__restrict__ sv,
                                                         const int *__restrict__ sc, const double *__restrict__ p,
@@ -490,18 +491,40 @@ __global__ void __launch_bounds__(4 * MPK_NT) k_cg_wdot(i64 n, const int *__rest
   const int len0 = e0 < n ? rp[e0 + 1] - rp[e0] : 0, len1 = e1 < n ? rp[e1 + 1] - rp[e1] : 0;
   const int q0 = e0 < n ? soff[e0 >> 6] + (int)(e0 & 63) : 0, q1 = e1 < n ? soff[e1 >> 6] + (int)(e1 & 63) : 0;
   const int len = len0 > len1 ? len0 : len1;
+  // both rows' sums in order (sell_row's), four entries of each in flight
   double w0 = 0.0, w1 = 0.0;
-  for (int k = 0; k < len; k++) {
-    if (k < len0) w0 = fma(sv[q0 + 64 * k], p[sc[q0 + 64 * k]], w0);
-    if (k < len1) w1 = fma(sv[q1 + 64 * k], p[sc[q1 + 64 * k]], w1);
+  for (int k = 0; k < len; k += 4) {
+    int c0[4], c1[4];
+    double v0[4], v1[4], x0[4], x1[4];
+#pragma unroll
+    for (int j = 0; j < 4; j++) {
+      if (k + j < len0) {
+        c0[j] = ld_stream<NT>(sc + q0 + 64 * (k + j));
+        v0[j] = ld_stream<NT>(sv + q0 + 64 * (k + j));
+      }
+      if (k + j < len1) {
+        c1[j] = ld_stream<NT>(sc + q1 + 64 * (k + j));
+        v1[j] = ld_stream<NT>(sv + q1 + 64 * (k + j));
+      }
+    }
+#pragma unroll
+    for (int j = 0; j < 4; j++) {
+      if (k + j < len0) x0[j] = p[c0[j]];
+      if (k + j < len1) x1[j] = p[c1[j]];
+    }
+#pragma unroll
+    for (int j = 0; j < 4; j++) {
+      if (k + j < len0) w0 = fma(v0[j], x0[j], w0);
+      if (k + j < len1) w1 = fma(v1[j], x1[j], w1);
+    }
   }
   double p0 = 0.0, p1 = 0.0;
   if (e0 < n) {
-    w[e0] = w0;
+    st_stream<NT>(w + e0, w0);
     p0 = p[e0];
   }
   if (e1 < n) {
-    w[e1] = w1;
+    st_stream<NT>(w + e1, w1);
     p1 = p[e1];
   }
   // pair (r, r + 1), r = T even, in each half: r = 512 j + 2t with j = 2 h + (T >= 512)
@@ -999,8 +1022,12 @@ void mp_ksp_solve_cg(int64_t n, int64_t irow, int64_t icol, int64_t data, int64_
         // the product and its reduction in one pass (k_cg_wdot)
         const unsigned T = ntiles(n);
         double *Pp = partials(T);
-        MPK_LAUNCH(k_cg_wdot, T, 4 * MPK_NT, n, S.A.rp, S.A.sell_off, S.A.sell_v, S.A.sell_c, (const double *)P, W,
-                   Pp, st);
+        if (sell_nt(n))
+          MPK_LAUNCH(k_cg_wdot<true>, T, 4 * MPK_NT, n, S.A.rp, S.A.sell_off, S.A.sell_v, S.A.sell_c,
+                     (const double *)P, W, Pp, st);
+        else
+          MPK_LAUNCH(k_cg_wdot<false>, T, 4 * MPK_NT, n, S.A.rp, S.A.sell_off, S.A.sell_v, S.A.sell_c,
+                     (const double *)P, W, Pp, st);
         MPK_LAUNCH(k_finalize<KspEpi>, 1, MPK_NT, (const double *)Pp, (int)T, 1, (double *)nullptr,
                    KspEpi{S.d, EPI_CG_DPI, 0}, st);
       } else {

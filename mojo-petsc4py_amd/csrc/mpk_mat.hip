@@ -6,6 +6,7 @@
 // arithmetic of MatMult_SeqAIJ (src/ported.mojo:380-405) -- so a row gives
 // the same bits as the CPU path.
 #include <algorithm>
+#include <cstdlib>
 #include <map>
 #include <tuple>
 #include <vector>
@@ -44,6 +45,7 @@ __global__ void __launch_bounds__(256) k_spmv_row(i64 nrow, const int *__restric
 // XCDs sweeps one contiguous range of rows, which keeps the x entries that
 // neighbouring rows share inside that XCD's L2.
 
+template <bool NT>
 __global__ void __launch_bounds__(256) k_spmv_sell(i64 nrow, const int *__restrict__ rp,
                                                    const int *__restrict__ soff, const double *__restrict__ sv,
                                                    const int *__restrict__ sc, const double *__restrict__ x,
@@ -54,12 +56,7 @@ __global__ void __launch_bounds__(256) k_spmv_sell(i64 nrow, const int *__restri
   if (i >= nrow) return;
   const int len = rp[i + 1] - rp[i];
   const int base = soff[i >> 6] + (int)(i & 63);
-  double s = add ? y[i] : 0.0;
-  for (int k = 0; k < len; k++) {
-    const int p = base + 64 * k;
-    s = fma(sv[p], x[sc[p]], s);
-  }
-  y[i] = s;
+  st_stream<NT>(y + i, sell_row<NT>(sv, sc, x, base, len, add ? y[i] : 0.0));
 }
 
 // scatter CSR into the slice layout (one lane per row)
@@ -92,11 +89,23 @@ i64 sell_offsets(const int *rp_d, i64 nrow, std::vector<int> &soff) {
   return total;
 }
 
+bool sell_nt(i64 nrow) {
+  static const int forced = [] {
+    const char *e = std::getenv("MPK_SELL_NT");
+    return e && (e[0] == '0' || e[0] == '1') ? e[0] - '0' : -1;
+  }();
+  return forced >= 0 ? forced == 1 : nrow >= 3000000;
+}
+
 void spmv(const DevCSR &A, const double *x, double *y, int add, const int64_t *stop) {
   if (A.nrow <= 0 || !ctx()) return;
   if (A.sell_off) {
-    MPK_LAUNCH(k_spmv_sell, nblocks(A.nrow, 256), 256, A.nrow, A.rp, A.sell_off, A.sell_v, A.sell_c, x, y, add,
-               stop);
+    if (sell_nt(A.nrow))
+      MPK_LAUNCH(k_spmv_sell<true>, nblocks(A.nrow, 256), 256, A.nrow, A.rp, A.sell_off, A.sell_v, A.sell_c, x, y,
+                 add, stop);
+    else
+      MPK_LAUNCH(k_spmv_sell<false>, nblocks(A.nrow, 256), 256, A.nrow, A.rp, A.sell_off, A.sell_v, A.sell_c, x, y,
+                 add, stop);
     return;
   }
   MPK_LAUNCH(k_spmv_row, nblocks(A.nrow, 256), 256, A.nrow, A.rp, A.ci, A.v, x, y, add, stop);
