@@ -150,6 +150,19 @@ void mp_icc_contiguous_cholesky(int64_t ujsrc, int64_t ui, int64_t udiag, int64_
                                 int64_t n);
 void mp_ordering_rcm(int64_t perm, int64_t level, int64_t visited, int64_t queue, int64_t irow, int64_t icol,
                      int64_t nrow);
+/* MatILUFactorNumeric_SeqAIJ (ported.mojo:957-1037) on the device, in place
+ * of mp_pc_ilu_numeric's host pass, with the same bits.  mpk_ilu_levels
+ * (host, needs no device): level sets of the factor's L part from bi, bj,
+ * bdiag [H] int64; lrow [H] int32[nrow] receives the rows level by level,
+ * lptr [H] int64[nlev + 1] the level bounds; returns nlev, or -2 when some
+ * row lacks its own diagonal slot or ascending L / U columns (then only the
+ * serial mp_pc_ilu_numeric reproduces the reference).  mpk_ilu_numeric: A as
+ * irow, icol [D] int32 and data [D] float64; r, ic (the row permutation and
+ * its inverse), bi, bj, bdiag and lrow [D] int32; lptr [H] int64; ba [D]
+ * float64[bdiag[0] + 1] receives the factor.  Returns 0, or -1 on error. */
+int64_t mpk_ilu_levels(int64_t bi, int64_t bj, int64_t bdiag, int64_t nrow, int64_t lrow, int64_t lptr);
+int64_t mpk_ilu_numeric(int64_t nrow, int64_t irow, int64_t icol, int64_t data, int64_t r, int64_t ic, int64_t bi,
+                        int64_t bj, int64_t bdiag, int64_t lrow, int64_t lptr, int64_t nlev, int64_t ba);
 /* No counterpart in the reference: the transpose of an IC factor's strict
  * part, which the device forward sweep gathers over.  [D] int32 arrays:
  * trp[nrow + 1], trow[m], tpos[m], tmid[nrow], m = mpk_icc_transpose_size().

@@ -100,6 +100,13 @@ struct FillList {
 
 }  // namespace
 
+// count entries from src to dst; a count below one copies nothing (the
+// reference's copies are `while k < count` loops, factor.mojo:31-67,335-381,
+// which a row without a diagonal drives negative)
+static inline void copy_ids(I *dst, const I *src, I count) {
+  if (count > 0) memcpy(dst, src, sizeof(I) * (size_t)count);
+}
+
 extern "C" {
 
 // factor.mojo:31-67 -- ILU(0), natural ordering: L = strict lower part of A,
@@ -113,14 +120,16 @@ int64_t mp_ilu_symbolic_ilu0(int64_t bi_, int64_t bj_, int64_t bdiag_, int64_t a
   for (I i = 0; i < nrow; i++) {
     I nl = adiag[i] - rp[i];
     bi[i + 1] = bi[i] + nl;
-    memcpy(bj + w, ci + rp[i], sizeof(I) * (size_t)nl);
-    w += nl;
+    copy_ids(bj + w, ci + rp[i], nl);
+    w += nl > 0 ? nl : 0;
   }
   bdiag[nrow] = bi[nrow] - 1;
   for (I i = nrow - 1; i >= 0; i--) {
+    // a row without a diagonal (adiag at the row end) has nu = -1: nothing
+    // is copied and its diagonal slot is the next row's (factor.mojo:57-66)
     I nu = rp[i + 1] - adiag[i] - 1;
-    memcpy(bj + w, ci + adiag[i] + 1, sizeof(I) * (size_t)nu);
-    w += nu;
+    copy_ids(bj + w, ci + adiag[i] + 1, nu);
+    w += nu > 0 ? nu : 0;
     bj[w++] = i;
     bdiag[i] = bdiag[i + 1] + nu + 1;
   }
@@ -280,7 +289,7 @@ void mp_ordering_rcm(int64_t perm_, int64_t level_, int64_t visited_, int64_t qu
       q[a] = q[b];
       q[b] = t;
     }
-    memcpy(perm + out, q, sizeof(I) * (size_t)tail);
+    copy_ids(perm + out, q, tail);
     out += tail;
   }
 }
@@ -298,12 +307,12 @@ void mp_ilu_contiguous_lu(int64_t bjsrc_, int64_t bi_, int64_t bdiag_, int64_t b
     bi[row] = row == 0 ? 0 : bi[row - 1] + prev_l;
     const I nl = bdiag[row];
     prev_l = nl;
-    memcpy(bj + bi[row], src + at, sizeof(I) * (size_t)nl);
+    copy_ids(bj + bi[row], src + at, nl);
     bdiag[row] = utop - 1;
     bj[utop - 1] = row;
     const I nu = len - nl;
     utop -= nu;
-    memcpy(bj + utop, src + at + nl + 1, sizeof(I) * (size_t)(nu - 1));
+    copy_ids(bj + utop, src + at + nl + 1, nu - 1);
     at += len;
   }
   if (n) {
@@ -354,6 +363,50 @@ int64_t mp_pc_ilu_numeric(int64_t ba_, int64_t rtmp_, int64_t irow_, int64_t ico
     ba[bd[i]] = 1.0 / w[i];
   }
   return bd[0] + 1;
+}
+
+// Level sets of an ILU factor's L part, for the level-by-level numeric
+// factorisation on the device: row i's level is one past the deepest level
+// among the rows its L entries name (0 with none), so every pivot row of a
+// level lies in an earlier one.  lrow [H] int32[n] lists the rows level by
+// level (ascending inside a level); lptr [H] int64[nlev + 1] bounds the
+// levels.  Returns the level count, or -2 when the layout is not one the
+// device pass reproduces: each row must hold its L columns strictly
+// ascending below i, its strict U columns strictly ascending above i, and a
+// diagonal slot of its own (bdiag[i] > bdiag[i + 1], bj[bdiag[i]] = i).  The
+// level-of-fill pattern of some nonsymmetric matrices leaves rows without
+// one; the reference's dense work row then carries values from row to row,
+// which only the serial pass (mp_pc_ilu_numeric) reproduces.
+int64_t mpk_ilu_levels(int64_t bi_, int64_t bj_, int64_t bdiag_, int64_t n, int64_t lrow_, int64_t lptr_) {
+  const I *bi = (const I *)bi_, *bj = (const I *)bj_, *bd = (const I *)bdiag_;
+  int32_t *lrow = (int32_t *)lrow_;
+  I *lptr = (I *)lptr_;
+  if (n <= 0) {
+    lptr[0] = 0;
+    return 0;
+  }
+  for (I i = 0; i < n; i++) {
+    if (bd[i] <= bd[i + 1] || bj[bd[i]] != i) return -2;
+    for (I j = bi[i]; j < bi[i + 1]; j++)
+      if (bj[j] < 0 || bj[j] >= i || (j > bi[i] && bj[j] <= bj[j - 1])) return -2;
+    for (I j = bd[i + 1] + 1; j < bd[i]; j++)
+      if (bj[j] <= i || bj[j] >= n || (j > bd[i + 1] + 1 && bj[j] <= bj[j - 1])) return -2;
+  }
+  std::vector<int32_t> lev(n);
+  int32_t top = 0;
+  for (I i = 0; i < n; i++) {
+    int32_t l = 0;
+    for (I j = bi[i]; j < bi[i + 1]; j++) l = std::max(l, lev[bj[j]] + 1);
+    lev[i] = l;
+    top = std::max(top, l);
+  }
+  const I nlev = (I)top + 1;
+  std::vector<I> cnt(nlev + 1, 0);
+  for (I i = 0; i < n; i++) cnt[lev[i] + 1]++;
+  for (I l = 0; l < nlev; l++) cnt[l + 1] += cnt[l];
+  for (I l = 0; l <= nlev; l++) lptr[l] = cnt[l];
+  for (I i = 0; i < n; i++) lrow[cnt[lev[i]]++] = (int32_t)i;
+  return nlev;
 }
 
 // ported.mojo:1040-1124 -- up-looking IC with the row chains of each column.

@@ -392,6 +392,67 @@ void PCView::apply_symmetric(const double *x, double *y, const int64_t *stop) co
     apply(x, y, stop);
 }
 
+// ------------------------------------------------------------ ILU numeric
+
+// Position of column c in row i of an ILU factor (the L part's columns
+// ascending in [l0, l1), the strict U part's ascending in [u0, dg), the
+// diagonal at dg), or -1 when the row's pattern has no such column.
+__device__ __forceinline__ int ilu_pos(const int *__restrict__ bj, int l0, int l1, int u0, int dg, int i, int c) {
+  if (c == i) return dg;
+  int lo = c < i ? l0 : u0;
+  const int end = c < i ? l1 : dg;
+  int hi = end;
+  while (lo < hi) {
+    const int mid = (lo + hi) >> 1;
+    if (bj[mid] < c)
+      lo = mid + 1;
+    else
+      hi = mid;
+  }
+  return lo < end && bj[lo] == c ? lo : -1;
+}
+
+// MatILUFactorNumeric_SeqAIJ (ported.mojo:957-1037) for the rows of one
+// level, a thread per row, in place in ba: the row's own slots stand in for
+// the dense work row (a column outside the row's pattern, which the
+// reference writes to and never reads, is skipped).  The same operations in
+// the same order: zero the row, load A's row r[i] through ic, then for each
+// L pivot p ascending with a nonzero multiplier, mult = w[p] * (1/u_pp),
+// w[c] = fma(-mult, u_pc, w[c]) along U's row p, and finally the inverted
+// diagonal.  Pivot rows lie in earlier levels, finished by earlier launches.
+__global__ void __launch_bounds__(256) k_ilu_numeric(i64 p0, i64 cnt, const int *__restrict__ lrow,
+                                                     const int *__restrict__ rp, const int *__restrict__ ci,
+                                                     const double *__restrict__ a, const int *__restrict__ r,
+                                                     const int *__restrict__ ic, const int *__restrict__ bi,
+                                                     const int *__restrict__ bj, const int *__restrict__ bd,
+                                                     double *ba) {
+  const i64 t = (i64)blockIdx.x * 256 + threadIdx.x;
+  if (t >= cnt) return;
+  const int i = lrow[p0 + t];
+  const int l0 = bi[i], l1 = bi[i + 1], dg = bd[i], u0 = bd[i + 1] + 1;
+  for (int j = l0; j < l1; j++) ba[j] = 0.0;
+  for (int j = u0; j <= dg; j++) ba[j] = 0.0;
+  const int ra = r[i];
+  for (int j = rp[ra]; j < rp[ra + 1]; j++) {
+    const int q = ilu_pos(bj, l0, l1, u0, dg, i, ic[ci[j]]);
+    if (q >= 0) ba[q] = a[j];
+  }
+  for (int j = l0; j < l1; j++) {
+    const int p = bj[j];
+    const double pc = ba[j];
+    if (pc != 0.0) {
+      const double mult = pc * ba[bd[p]];
+      ba[j] = mult;
+      const int qe = bd[p];
+      for (int q = bd[p + 1] + 1; q < qe; q++) {
+        const int w = ilu_pos(bj, l0, l1, u0, dg, i, bj[q]);
+        if (w >= 0) ba[w] = fma(-mult, ba[q], ba[w]);
+      }
+    }
+  }
+  ba[dg] = 1.0 / ba[dg];
+}
+
 // ------------------------------------------------------------ IC transpose map
 
 // Transpose of the strict upper part of an IC factor: for column k, the rows
@@ -638,6 +699,31 @@ int64_t mpk_icc_transpose(int64_t ui, int64_t uj, int64_t udiag, int64_t nrow, i
                        (int *)tpos, (int *)tmid, &nlate)
              ? nlate
              : -1;
+}
+
+// MatILUFactorNumeric_SeqAIJ on the device, one launch per level of the
+// factor's L part (lrow / lptr from mpk_ilu_levels).
+int64_t mpk_ilu_numeric(int64_t n, int64_t irow, int64_t icol, int64_t data, int64_t r, int64_t ic, int64_t bi,
+                        int64_t bj, int64_t bdiag, int64_t lrow, int64_t lptr, int64_t nlev, int64_t ba) {
+  if (n <= 0) return 0;
+  if (!ctx()) return -1;
+  const int64_t *lp = (const int64_t *)lptr;
+  if (nlev <= 0 || lp[0] != 0 || lp[nlev] != n) {
+    set_error(2, "mpk_ilu_numeric: the level bounds do not cover the rows");
+    return -1;
+  }
+  for (int64_t l = 0; l < nlev; l++) {
+    const i64 cnt = lp[l + 1] - lp[l];
+    if (cnt < 0) {
+      set_error(2, "mpk_ilu_numeric: level bounds out of order");
+      return -1;
+    }
+    if (cnt == 0) continue;
+    MPK_LAUNCH(k_ilu_numeric, nblocks(cnt, 256), 256, (i64)lp[l], cnt, (const int *)lrow, (const int *)irow,
+               (const int *)icol, (const double *)data, (const int *)r, (const int *)ic, (const int *)bi,
+               (const int *)bj, (const int *)bdiag, (double *)ba);
+  }
+  return failed() ? -1 : 0;
 }
 
 // Block plans of the descriptor's two sweeps (see include/mpk.h).

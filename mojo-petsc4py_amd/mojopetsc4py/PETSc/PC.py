@@ -357,7 +357,7 @@ class PC:
             self._build_ilu(m, n, r, ic, adiag, scratch)
         else:
             self._build_icc(m, n, r, ic, adiag, scratch)
-        self._upload_factor(n)
+        self._upload_factor(m, n)
 
     def _build_ilu(self, m, n, r, ic, adiag, scratch):
         L = lib()
@@ -380,13 +380,10 @@ class PC:
                     L.mp_ilu_contiguous_lu(_a(bjsrc), _a(bi), _a(bdiag), _a(bj), n)
                 break
             cap = -nnz + n
-        ba = np.zeros(max(int(bdiag[0]) + 1, 1), dtype=np.float64)
-        rtmp = np.zeros(n, dtype=np.float64)
-        L.mp_pc_ilu_numeric(_a(ba), _a(rtmp), _a(m.irow), _a(m.icol), _a(m.data), _a(bi), _a(bj),
-                            _a(bdiag), _a(r), _a(ic), n)
         # the factor lives in permuted coordinates; the solution is mapped back
-        # with the permutation itself (PC.py:436-440)
-        self._factor = {"bi": bi, "bj": bj, "ba": ba, "bdiag": bdiag, "r": r, "c": r}
+        # with the permutation itself (PC.py:436-440).  Its values are
+        # computed on the device (_ilu_numeric_device), level by level.
+        self._factor = {"bi": bi, "bj": bj, "ba": None, "bdiag": bdiag, "r": r, "c": r, "ic": ic}
 
     def _build_icc(self, m, n, r, ic, adiag, scratch):
         L = lib()
@@ -419,26 +416,70 @@ class PC:
                             _a(bj), _a(bdiag), _a(r), _a(rinv), n)
         self._factor = {"bi": bi, "bj": bj, "ba": ua, "bdiag": bdiag, "r": r}
 
-    def _upload_factor(self, n: int):
+    def _upload_factor(self, m: Mat, n: int):
         f = self._factor
         nnz = int(f["bi"][n]) if self._type == PCType.ICC else int(f["bdiag"][0]) + 1
         for key in ("bi", "bdiag", "r"):
             self._dbufs[key] = DeviceBuffer.from_array(f[key].astype(np.int32))
         self._dbufs["bj"] = DeviceBuffer.from_array(f["bj"][: max(nnz, 1)].astype(np.int32))
-        self._dbufs["ba"] = DeviceBuffer.from_array(f["ba"])
         if self._type == PCType.ILU:
             self._dbufs["c"] = self._dbufs["r"]
-        else:
-            L = lib()
-            bi, bj, bd = self._dbufs["bi"].ptr, self._dbufs["bj"].ptr, self._dbufs["bdiag"].ptr
-            mt = L.mpk_icc_transpose_size(bi, bd, n)
-            check()
-            trp = self._buf("trp", 4 * (n + 1))
-            trow = self._buf("trow", 4 * max(mt, 1))
-            tpos = self._buf("tpos", 4 * max(mt, 1))
-            tmid = self._buf("tmid", 4 * max(n, 1))
-            self._nlate = L.mpk_icc_transpose(bi, bj, bd, n, trp.ptr, trow.ptr, tpos.ptr, tmid.ptr)
-            check()
+            self._ilu_numeric_device(m, n, nnz)
+            return
+        self._dbufs["ba"] = DeviceBuffer.from_array(f["ba"])
+        L = lib()
+        bi, bj, bd = self._dbufs["bi"].ptr, self._dbufs["bj"].ptr, self._dbufs["bdiag"].ptr
+        mt = L.mpk_icc_transpose_size(bi, bd, n)
+        check()
+        trp = self._buf("trp", 4 * (n + 1))
+        trow = self._buf("trow", 4 * max(mt, 1))
+        tpos = self._buf("tpos", 4 * max(mt, 1))
+        tmid = self._buf("tmid", 4 * max(n, 1))
+        self._nlate = L.mpk_icc_transpose(bi, bj, bd, n, trp.ptr, trow.ptr, tpos.ptr, tmid.ptr)
+        check()
+
+    def _ilu_numeric_device(self, m: Mat, n: int, nnz: int):
+        """MatILUFactorNumeric_SeqAIJ (ported.mojo:957-1037) on the device:
+        the host lists the rows level by level (levels of the factor's L
+        part), the device eliminates one level per launch into ba.  A factor
+        layout the level-by-level pass cannot reproduce (a row without its
+        own diagonal slot, which level-of-fill leaves on some nonsymmetric
+        patterns) is factored by the serial pass on the host instead."""
+        L = lib()
+        f = self._factor
+        lrow = np.zeros(max(n, 1), dtype=np.int32)
+        lptr = np.zeros(n + 1, dtype=np.int64)
+        nlev = L.mpk_ilu_levels(_a(f["bi"]), _a(f["bj"]), _a(f["bdiag"]), n, _a(lrow), _a(lptr))
+        if nlev == -2:
+            ba = np.zeros(max(nnz, 1), dtype=np.float64)
+            rtmp = np.zeros(n, dtype=np.float64)
+            L.mp_pc_ilu_numeric(_a(ba), _a(rtmp), _a(m.irow), _a(m.icol), _a(m.data), _a(f["bi"]), _a(f["bj"]),
+                                _a(f["bdiag"]), _a(f["r"]), _a(f["ic"]), n)
+            f["ba"] = ba
+            self._dbufs["ba"] = DeviceBuffer.from_array(ba)
+            return
+        if nlev < 0:
+            raise RuntimeError("listing the ILU factor's levels failed")
+        d = m._dev()
+        ic = DeviceBuffer.from_array(f["ic"].astype(np.int32))
+        lrow_d = DeviceBuffer.from_array(lrow)
+        ba = self._buf("ba", 8 * max(nnz, 1))
+        b = self._dbufs
+        rc = L.mpk_ilu_numeric(n, d.rp.ptr, d.ci.ptr, d.v.ptr, b["r"].ptr, ic.ptr, b["bi"].ptr, b["bj"].ptr,
+                               b["bdiag"].ptr, lrow_d.ptr, _a(lptr), nlev, ba.ptr)
+        check()
+        if rc != 0:
+            raise RuntimeError("the device ILU factorisation failed")
+        L.mpk_synchronize()
+        f["nnz"] = nnz
+
+    def _host_factor(self) -> dict:
+        """The factor with its values on the host (ILU values are computed on
+        the device and downloaded on first request)."""
+        f = self._factor
+        if f is not None and f["ba"] is None:
+            f["ba"] = self._dbufs["ba"].to_numpy(np.float64, max(int(f["nnz"]), 1))
+        return f
 
     # ---------------------------------------------------------- descriptor
     def _sor_flag(self) -> int:
@@ -520,7 +561,7 @@ class PC:
 
     def getFactorMatrix(self) -> Mat:
         self.setUp()
-        f = self._factor
+        f = self._host_factor()
         if f is None:
             raise RuntimeError(f"PC type {self._type} has no factor matrix")
         if self._type == PCType.ILU:

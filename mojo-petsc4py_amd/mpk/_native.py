@@ -115,6 +115,8 @@ SIGNATURES = {
     "mpk_pc_apply": ([I] * 5, None),
     "mpk_icc_transpose_size": ([I, I, I], I),
     "mpk_icc_transpose": ([I] * 8, I),
+    "mpk_ilu_levels": ([I] * 6, I),
+    "mpk_ilu_numeric": ([I] * 13, I),
 }
 
 # exported names that only a device call can exercise

@@ -237,3 +237,70 @@ def test_mat_surface_products_bitwise(O, shape):
         va = va * 1.5 - 0.25
         A.setValuesCSR(ip, ix, va)
         A.assemble()
+
+
+ILU_DEV_MATS = dict(MATS)
+ILU_DEV_MATS["cd24"] = lambda: W.convection_diffusion_3d(24)[0]
+ILU_DEV_MATS["ragged300"] = lambda: _random_sparse(300, 14, 11)
+
+
+@pytest.mark.parametrize("name", sorted(ILU_DEV_MATS))
+@pytest.mark.parametrize("levels,ordering", [(0, "natural"), (1, "natural"), (2, "natural"), (0, "rcm"),
+                                             (2, "rcm")])
+def test_ilu_numeric_device_bitwise(native, O, name, levels, ordering):
+    """MatILUFactorNumeric_SeqAIJ on the device (mpk_ilu_levels, then
+    mpk_ilu_numeric level by level) gives the oracle's factor values bit for
+    bit, over levels of fill and orderings."""
+    ip, ix, va = ILU_DEV_MATS[name]()
+    A = O.CSR(ip, ix, va)
+    n = A.n
+    f = O.build_factor(A, "ilu", levels, ordering)
+    r, ic = O.permutations(A, ordering)
+    nnz = int(f["bdiag"][0]) + 1
+    lrow = np.zeros(n, np.int32)
+    lptr = np.zeros(n + 1, np.int64)
+    nlev = native.mpk_ilu_levels(f["bi"].ctypes.data, f["bj"].ctypes.data, f["bdiag"].ctypes.data, n,
+                                  lrow.ctypes.data, lptr.ctypes.data)
+    if nlev == -2:
+        # a level-of-fill layout with rows lacking their own diagonal slot:
+        # only the serial host pass reproduces it (the PC takes that path)
+        bd = f["bdiag"]
+        assert any(bd[i] <= bd[i + 1] or f["bj"][bd[i]] != i for i in range(n))
+        return
+    D = DevMat(ip, ix, va)
+    d = {k: dev(np.asarray(v).astype(np.int32)) for k, v in
+         (("r", r), ("ic", ic), ("bi", f["bi"]), ("bdiag", f["bdiag"]), ("bj", f["bj"][:nnz]), ("lrow", lrow))}
+    dba = dev(np.full(nnz, np.nan))
+    rc = native.mpk_ilu_numeric(n, D.rp.ptr, D.ci.ptr, D.v.ptr, d["r"].ptr, d["ic"].ptr, d["bi"].ptr,
+                                d["bj"].ptr, d["bdiag"].ptr, d["lrow"].ptr, lptr.ctypes.data, nlev, dba.ptr)
+    assert rc == 0
+    assert same_bits(back(dba, np.float64, nnz), f["ba"][:nnz])
+
+
+@pytest.mark.parametrize("mat,levels,ordering", [("cd16", 0, "natural"), ("cd16", 1, "rcm"),
+                                                 ("ragged300", 1, "natural")])
+def test_pc_ilu_set_up_on_device(O, mat, levels, ordering):
+    """PC.setUp's ILU factor through the PETSc surface (computed on the
+    device; ragged300 at level 1 has rows without their own diagonal slot and
+    is factored by the serial host pass) equals the oracle's, and so does its
+    apply."""
+    from mojopetsc4py import PETSc
+    ip, ix, va = W.convection_diffusion_3d(16)[0] if mat == "cd16" else ILU_DEV_MATS[mat]()
+    n = len(ip) - 1
+    A = PETSc.Mat().createSeqAIJ((n, n), nnz=7)
+    A.setValuesCSR(ip, ix, va)
+    A.assemble()
+    pc = PETSc.PC().create()
+    pc.setType("ilu")
+    pc.setFactorLevels(levels)
+    pc.setFactorOrdering(ordering)
+    pc.setOperators(A)
+    pc.setUp()
+    want = O.build_factor(O.CSR(ip, ix, va), "ilu", levels, ordering)
+    got = pc._host_factor()
+    nnz = int(want["bdiag"][0]) + 1
+    assert same_bits(got["ba"][:nnz], want["ba"][:nnz])
+    b = rng_vec(n, 4)
+    y = PETSc.Vec().createSeq(n)
+    pc.apply(PETSc.Vec().createWithArray(b), y)
+    assert same_bits(y.getArray(), O.PCSetup(O.CSR(ip, ix, va), "ilu", levels, ordering).apply(b))

@@ -34,7 +34,7 @@ def _sweep_entries(kind, pc, A):
         fwd = [[(ix[j], -va[j]) for j in range(ip[i], dm[i])] for i in range(n)]
         bwd = [[(ix[j], -va[j]) for j in range(dm[i] + 1, ip[i + 1])] for i in range(n)]
         return pack(fwd), pack(bwd)
-    f = pc._factor
+    f = pc._host_factor()
     bi, bj, ba, bd = f["bi"], f["bj"], f["ba"], f["bdiag"]
     if kind == "ilu":
         fwd = [[(bj[j], -ba[j]) for j in range(bi[i], bi[i + 1])] for i in range(n)]

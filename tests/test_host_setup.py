@@ -99,3 +99,109 @@ def test_rcm_bitwise(L, oracle):
     L.mp_ordering_rcm(*[a_(b) for b in bufs], a_(A.irow), a_(A.icol), n)
     r, _ = oracle.permutations(A, "rcm")
     assert np.array_equal(bufs[0], r)
+
+
+@pytest.mark.parametrize("name", sorted(MATS))
+@pytest.mark.parametrize("levels,ordering", [(0, "natural"), (2, "natural"), (1, "rcm")])
+def test_ilu_levels(L, oracle, name, levels, ordering):
+    """mpk_ilu_levels: every row's level is one past its deepest L pivot's
+    (brute force here), rows ascending inside a level, all rows once."""
+    A = oracle.CSR(*MATS[name]())
+    n = A.n
+    f = oracle.build_factor(A, "ilu", levels, ordering)
+    bi, bj = f["bi"], f["bj"]
+    want = np.zeros(n, np.int64)
+    for i in range(n):
+        piv = bj[bi[i]:bi[i + 1]]
+        assert np.all(piv < i)
+        want[i] = 1 + want[piv].max() if len(piv) else 0
+    lrow = np.zeros(n, np.int32)
+    lptr = np.zeros(n + 1, np.int64)
+    nlev = L.mpk_ilu_levels(a_(bi), a_(bj), a_(f["bdiag"]), n, a_(lrow), a_(lptr))
+    assert nlev == want.max() + 1 and lptr[0] == 0 and lptr[nlev] == n
+    for lev in range(nlev):
+        rows = lrow[lptr[lev]:lptr[lev + 1]]
+        assert np.all(np.diff(rows) > 0) and np.all(want[rows] == lev)
+    assert np.array_equal(np.sort(lrow), np.arange(n))
+
+
+def _random_nonsym(n, per_row, seed):
+    rng = np.random.default_rng(seed)
+    rows, cols = [], []
+    for i in range(n):
+        cs = set(rng.integers(0, n, per_row).tolist()) | {i}
+        for c in sorted(cs):
+            rows.append(i)
+            cols.append(c)
+    vals = rng.normal(size=len(rows))
+    return W._csr_from_coo(n, np.array(rows), np.array(cols), vals)
+
+
+def test_ilu_levels_refuses_rows_without_a_diagonal_slot(L, oracle):
+    """Level-of-fill on this random nonsymmetric pattern leaves rows that
+    share a later row's diagonal slot; the level listing refuses the layout
+    (the serial host pass factors it), and accepts the ILU(0) one."""
+    A = oracle.CSR(*_random_nonsym(300, 14, 11))
+    n = A.n
+    lrow, lptr = np.zeros(n, np.int32), np.zeros(n + 1, np.int64)
+    f = oracle.build_factor(A, "ilu", 1, "natural")
+    bd = f["bdiag"]
+    assert any(bd[i] <= bd[i + 1] for i in range(n))
+    assert L.mpk_ilu_levels(a_(f["bi"]), a_(f["bj"]), a_(bd), n, a_(lrow), a_(lptr)) == -2
+    f0 = oracle.build_factor(A, "ilu", 0, "natural")
+    assert L.mpk_ilu_levels(a_(f0["bi"]), a_(f0["bj"]), a_(f0["bdiag"]), n, a_(lrow), a_(lptr)) > 0
+    # plant a mistake: two L columns of a row out of order
+    bj = f0["bj"].copy()
+    i = next(i for i in range(n) if f0["bi"][i + 1] - f0["bi"][i] >= 2)
+    j = int(f0["bi"][i])
+    bj[j], bj[j + 1] = bj[j + 1], bj[j]
+    assert L.mpk_ilu_levels(a_(f0["bi"]), a_(bj), a_(f0["bdiag"]), n, a_(lrow), a_(lptr)) == -2
+
+
+def _missing_diagonals(n, seed):
+    """A nonsymmetric pattern where every fifth row has no diagonal entry."""
+    rng = np.random.default_rng(seed)
+    rows, cols = [], []
+    for i in range(n):
+        cs = set(rng.integers(0, n, 6).tolist())
+        if i % 5 == 0:
+            cs.discard(i)
+        else:
+            cs.add(i)
+        for c in sorted(cs):
+            rows.append(i)
+            cols.append(c)
+    return W._csr_from_coo(n, np.array(rows), np.array(cols), rng.normal(size=len(rows)) + 3.0)
+
+
+ODD = {
+    # level-of-fill outgrows the first capacity guess: the retry runs on the
+    # scratch the failed attempt left behind (PC.py:393-428), and some rows
+    # come out without their diagonal
+    "nonsym300": lambda: _random_nonsym(300, 14, 11),
+    "nodiag120": lambda: _missing_diagonals(120, 4),
+}
+
+
+@pytest.mark.parametrize("name", sorted(ODD))
+@pytest.mark.parametrize("levels,ordering", [(0, "natural"), (1, "natural"), (2, "natural"), (0, "rcm"),
+                                             (1, "rcm")])
+def test_ilu_set_up_on_degenerate_patterns(L, oracle, name, levels, ordering):
+    """Rows without a diagonal (in A, or left so by a retried symbolic pass)
+    give rows whose diagonal slot is the next row's; the product's host set up
+    still follows the reference's copy loops (which copy nothing for a
+    negative count) and matches the oracle entry for entry and bit for bit."""
+    A = oracle.CSR(*ODD[name]())
+    f = oracle.build_factor(A, "ilu", levels, ordering)
+    bi, bj, bdiag, ba, nnz = product_factor(L, oracle, A, "ilu", levels, ordering)
+    assert np.array_equal(bi, f["bi"]) and np.array_equal(bdiag, f["bdiag"])
+    assert np.array_equal(bj[:nnz], f["bj"][:nnz])
+    assert np.array_equal(ba[:nnz].view(np.int64), f["ba"][:nnz].view(np.int64))
+
+
+def test_degenerate_patterns_have_shared_diagonal_slots(oracle):
+    """The cases above do reach the degenerate layout."""
+    for name, lv in (("nonsym300", 1), ("nodiag120", 0)):
+        f = oracle.build_factor(oracle.CSR(*ODD[name]()), "ilu", lv, "natural")
+        bd = f["bdiag"]
+        assert any(bd[i] <= bd[i + 1] for i in range(len(bd) - 1)), name
