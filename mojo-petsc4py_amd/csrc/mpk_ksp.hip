@@ -72,7 +72,24 @@ struct KspState {
 
 // ------------------------------------------------------------ scalar steps
 
-// KSPConvergedDefault (ksp.mojo:96-152)
+// KSPConvergedDefault (ksp.mojo:96-152) once rnorm0 is seeded: reads the
+// state only.
+__device__ i64 conv_check(const KspState *s, i64 n, double rnorm) {
+  if (s->convmaxits && n >= s->max_it) return KSP_CONVERGED_ITS;
+  if (s->normtype == NORM_NONE) return 0;
+  if (n <= s->chknorm) return 0;
+  // MAX_REAL, not infinity, as the reference compares (ksp.mojo:85,138)
+  if (rnorm != rnorm || fabs(rnorm) == 1.7976931348623157e308) return KSP_DIVERGED_NANORINF;
+  if (n < s->min_it) return 0;
+  if (rnorm <= s->ttol) {
+    if (rnorm < s->atol) return KSP_CONVERGED_ATOL;
+    return KSP_CONVERGED_RTOL;
+  }
+  if (rnorm >= s->divtol * s->rnorm0) return KSP_DIVERGED_DTOL;
+  return 0;
+}
+
+// KSPConvergedDefault (ksp.mojo:96-152): seeds rnorm0 and ttol at n = 0.
 __device__ i64 conv_default(KspState *s, i64 n, double rnorm) {
   if (s->convmaxits && n >= s->max_it) return KSP_CONVERGED_ITS;
   if (s->normtype == NORM_NONE) return 0;
@@ -88,16 +105,7 @@ __device__ i64 conv_default(KspState *s, i64 n, double rnorm) {
     s->ttol = ttol > s->atol ? ttol : s->atol;
     s->seeded = 1;
   }
-  if (n <= s->chknorm) return 0;
-  // MAX_REAL, not infinity, as the reference compares (ksp.mojo:85,138)
-  if (rnorm != rnorm || fabs(rnorm) == 1.7976931348623157e308) return KSP_DIVERGED_NANORINF;
-  if (n < s->min_it) return 0;
-  if (rnorm <= s->ttol) {
-    if (rnorm < s->atol) return KSP_CONVERGED_ATOL;
-    return KSP_CONVERGED_RTOL;
-  }
-  if (rnorm >= s->divtol * s->rnorm0) return KSP_DIVERGED_DTOL;
-  return 0;
+  return conv_check(s, n, rnorm);
 }
 
 __device__ void finish(KspState *s, double rnorm, i64 its, i64 reason) {
@@ -177,6 +185,14 @@ __device__ i64 update_hessenberg(KspState *s, i64 it, i64 hapend, double *res) {
   return 0;
 }
 
+// CG's residual norm after iteration i from {z.z, r.r, z.r} (ksp.mojo:364-372)
+__device__ double cg_norm(const KspState *s, i64 i, const double *r) {
+  if (s->normtype == NORM_PRECONDITIONED && s->chknorm < i + 2) return sqrt(r[0]);
+  if (s->normtype == NORM_UNPRECONDITIONED && s->chknorm < i + 2) return sqrt(r[1]);
+  if (s->normtype == NORM_NATURAL) return sqrt(fabs(r[2]));
+  return 0.0;
+}
+
 __device__ void ksp_epilogue(KspState *s, int which, int arg, const double *r) {
   const i64 m = s->m;
   switch (which) {
@@ -217,15 +233,7 @@ __device__ void ksp_epilogue(KspState *s, int which, int arg, const double *r) {
     }
     case EPI_CG_NORM: {  // r = {z.z, r.r, z.r} (ksp.mojo:364-386)
       i64 i = s->i, its = i + 1;
-      double dp;
-      if (s->normtype == NORM_PRECONDITIONED && s->chknorm < i + 2)
-        dp = sqrt(r[0]);
-      else if (s->normtype == NORM_UNPRECONDITIONED && s->chknorm < i + 2)
-        dp = sqrt(r[1]);
-      else if (s->normtype == NORM_NATURAL)
-        dp = sqrt(fabs(r[2]));
-      else
-        dp = 0.0;
+      const double dp = cg_norm(s, i, r);
       s->dp = dp;
       s->hist[its] = dp;
       i64 reason = conv_default(s, its, dp);
@@ -558,27 +566,55 @@ struct OpCgXR {
     st2(r, e, n, al, D2{fma(-a, wv.a, rv.a), fma(-a, wv.b, rv.b)});
   }
 };
-// CG with Jacobi, one pass for the second half of an iteration:
+// CG with Jacobi, one pass for the second half of an iteration.  Every block
+// first folds the T tile partials of p.w (fold_partials_k, k_finalize's order)
+// and takes the p.w step from them (ksp.mojo:350-362): the breakdown test
+// and a = beta / p.w, reading only state that step leaves alone; the block
+// of tile 0 also runs the step on the state.  Unless the solve ends there:
 // x = x + a p, r = r - a w (OpCgXR), z = r .* diag (PCApply_Jacobi), and the
-// tile partials of z.z, r.r, z.r (k_finalize folds them).  Each thread holds
-// the elements the tile reduction gives it, so every value and every partial
-// is the one the separate kernels produce.  w may alias z (the reference's CG
+// tile partials of z.z, r.r, z.r (folded by k_cg_pz).  Each thread holds the
+// elements the tile reduction gives it, so every value and every partial is
+// the one the separate kernels produce.  w may alias z (the reference's CG
 // keeps W = Z).
 __global__ void __launch_bounds__(MPK_NT) k_cg_xrz(i64 n, double *x, double *r, double *z, const double *p,
-                                                   const double *w, const double *dg, const KspState *s, double *P,
-                                                   bool al, const int64_t *stop) {
+                                                   const double *w, const double *dg, const double *Pw, int T,
+                                                   KspState *s, double *P, bool al, const int64_t *stop) {
   if (stop && *stop) return;
   __shared__ double sh[4];
-  const double a = s->a;
+  __shared__ double res[1];
+  __shared__ double sa;
+  __shared__ int brk;
+  // the tile's operands are loaded first, so their latency overlaps the fold
   const i64 base = (i64)blockIdx.x * MPK_TILE + 2 * (i64)threadIdx.x;
+  D2 xv_[4], pv_[4], rv_[4], wv_[4], dv_[4];
+#pragma unroll
+  for (int j = 0; j < 4; j++) {
+    const i64 e = base + 512 * j;
+    if (e < n) {
+      xv_[j] = ld2(x, e, n, al);
+      pv_[j] = ld2(p, e, n, al);
+      rv_[j] = ld2(r, e, n, al);
+      wv_[j] = ld2(w, e, n, al);
+      dv_[j] = ld2(dg, e, n, al);
+    }
+  }
+  fold_partials_k<1>(Pw, T, res);
+  if (threadIdx.x == 0) {
+    const double dpi = res[0];
+    brk = dpi == 0.0 || (s->i > 0 && sign_real(dpi) * sign_real(s->dpiold) < 0);
+    sa = s->beta / dpi;
+    if (blockIdx.x == 0) ksp_epilogue(s, EPI_CG_DPI, 0, res);
+  }
+  __syncthreads();
+  if (brk) return;
+  const double a = sa;
   double lz[4], lr[4], lzr[4];
 #pragma unroll
   for (int j = 0; j < 4; j++) {
     const i64 e = base + 512 * j;
     D2 rn{0.0, 0.0}, zn{0.0, 0.0};
     if (e < n) {
-      const D2 xv = ld2(x, e, n, al), pv = ld2(p, e, n, al), rv = ld2(r, e, n, al), wv = ld2(w, e, n, al),
-               dv = ld2(dg, e, n, al);
+      const D2 xv = xv_[j], pv = pv_[j], rv = rv_[j], wv = wv_[j], dv = dv_[j];
       st2(x, e, n, al, D2{fma(a, pv.a, xv.a), fma(a, pv.b, xv.b)});
       rn = D2{fma(-a, wv.a, rv.a), fma(-a, wv.b, rv.b)};
       if (e + 1 >= n) rn.b = 0.0;
@@ -596,6 +632,60 @@ __global__ void __launch_bounds__(MPK_NT) k_cg_xrz(i64 n, double *x, double *r, 
   for (int k = 0; k < 3; k++) {
     const double t = block_tree(u[k], sh);
     if (threadIdx.x == 0) P[(i64)k * gridDim.x + blockIdx.x] = t;
+  }
+}
+
+// CG with Jacobi, the head of iteration i + 1 in one pass.  Every block folds
+// the tile partials of z.z, r.r, z.r from iteration i (k_finalize's order)
+// and takes the norm step's decisions from them (ksp.mojo:364-386 and the
+// checks of 337-350: convergence, max_it, beta = 0, a sign change of beta)
+// reading only state that step leaves alone; i comes from the host, which
+// queued the iterations.  The block of tile 0 also runs the step on the
+// state.  Unless the solve ends there, p = z + bcoef p over the block's tile
+// with OpCgP's arithmetic (VecAYPX, ported.mojo:285-294).
+__global__ void __launch_bounds__(MPK_NT) k_cg_pz(i64 n, double *p, const double *z, const double *Pn, int T,
+                                                  KspState *s, i64 i, bool al, const int64_t *stop) {
+  if (stop && *stop) return;
+  __shared__ double res[3];
+  __shared__ double sb;
+  __shared__ int brk;
+  // the tile's operands are loaded first, so their latency overlaps the fold
+  const i64 base = (i64)blockIdx.x * MPK_TILE + 2 * (i64)threadIdx.x;
+  D2 zv_[4], pv_[4];
+#pragma unroll
+  for (int j = 0; j < 4; j++) {
+    const i64 e = base + 512 * j;
+    if (e < n) {
+      zv_[j] = ld2(z, e, n, al);
+      pv_[j] = ld2(p, e, n, al);
+    }
+  }
+  fold_partials_k<3>(Pn, T, res);
+  if (threadIdx.x == 0) {
+    const double beta = res[2];
+    bool end = conv_check(s, i + 1, cg_norm(s, i, res)) != 0 || i + 1 >= s->max_it;
+    end = end || beta == 0.0 || beta * s->betaold < 0.0;
+    brk = end;
+    sb = end ? 0.0 : beta / s->betaold;
+    if (blockIdx.x == 0) ksp_epilogue(s, EPI_CG_NORM, 0, res);
+  }
+  __syncthreads();
+  if (brk) return;
+  const double b = sb;
+#pragma unroll
+  for (int j = 0; j < 4; j++) {
+    const i64 e = base + 512 * j;
+    if (e >= n) break;
+    const D2 zv = zv_[j];
+    if (b == 0.0) {
+      st2(p, e, n, al, zv);
+      continue;
+    }
+    const D2 pv = pv_[j];
+    if (b == 1.0)
+      st2(p, e, n, al, D2{fma(b, zv.a, pv.a), fma(b, zv.b, pv.b)});
+    else
+      st2(p, e, n, al, D2{fma(b, pv.a, zv.a), fma(b, pv.b, zv.b)});
   }
 }
 
@@ -1015,8 +1105,34 @@ void mp_ksp_solve_cg(int64_t n, int64_t irow, int64_t icol, int64_t data, int64_
   const int chunk = chunk_for(n);
   int slot = 0;
   bool pending = false;
+  // CG + Jacobi over the sliced copy: three launches an iteration, the p.w
+  // and norm steps folded into the kernels that use their results
+  const bool fused = S.A.sell_off && n > 0 && S.pc.type == PCT_JACOBI;
+  const unsigned nt = fused ? ntiles(n) : 0;
+  double *Pw = fused ? partials((size_t)4 * nt) : nullptr, *Pn = fused ? Pw + nt : nullptr;
+  if (fused && !Pw) return;
   for (i64 queued = 0;; queued += chunk) {
     for (int c = 0; c < chunk; c++) {
+      if (fused) {
+        const i64 it = queued + c;
+        const bool alp = aligned16(P) && aligned16(Z);
+        if (it == 0)
+          ew2(n, OpCgP{P, Z, S.d, alp}, st);
+        else
+          MPK_LAUNCH(k_cg_pz, nt, MPK_NT, n, P, (const double *)Z, (const double *)Pn, (int)nt, S.d, it - 1, alp,
+                     st);
+        if (sell_nt(n))
+          MPK_LAUNCH(k_cg_wdot<true>, nt, 4 * MPK_NT, n, S.A.rp, S.A.sell_off, S.A.sell_v, S.A.sell_c,
+                     (const double *)P, W, Pw, st);
+        else
+          MPK_LAUNCH(k_cg_wdot<false>, nt, 4 * MPK_NT, n, S.A.rp, S.A.sell_off, S.A.sell_v, S.A.sell_c,
+                     (const double *)P, W, Pw, st);
+        const bool al = aligned16(X) && aligned16(R) && aligned16(P) && aligned16(W) && aligned16(Z) &&
+                        aligned16(S.pc.jac_diag);
+        MPK_LAUNCH(k_cg_xrz, nt, MPK_NT, n, X, R, Z, (const double *)P, (const double *)W, S.pc.jac_diag,
+                   (const double *)Pw, (int)nt, S.d, Pn, al, st);
+        continue;
+      }
       ew2(n, OpCgP{P, Z, S.d, aligned16(P) && aligned16(Z)}, st);
       if (S.A.sell_off && n > 0) {
         // the product and its reduction in one pass (k_cg_wdot)
@@ -1035,19 +1151,9 @@ void mp_ksp_solve_cg(int64_t n, int64_t irow, int64_t icol, int64_t data, int64_
         S.reduce<1>(RedSet<1>{{{P, W, RK_DOT}}}, EPI_CG_DPI, 0, st);
       }
       const bool al = aligned16(X) && aligned16(R) && aligned16(P) && aligned16(W);
-      if (S.pc.type == PCT_JACOBI && n > 0) {
-        // x, r, the Jacobi apply and the three reductions in one pass
-        const unsigned T = ntiles(n);
-        double *Pp = partials((size_t)3 * T);
-        MPK_LAUNCH(k_cg_xrz, T, MPK_NT, n, X, R, Z, (const double *)P, (const double *)W, S.pc.jac_diag,
-                   (const KspState *)S.d, Pp, al && aligned16(Z) && aligned16(S.pc.jac_diag), st);
-        MPK_LAUNCH(k_finalize<KspEpi>, 1, MPK_NT, (const double *)Pp, (int)T, 3, (double *)nullptr,
-                   KspEpi{S.d, EPI_CG_NORM, 0}, st);
-      } else {
-        ew2(n, OpCgXR{X, R, P, W, S.d, al}, st);
-        S.pc.apply(R, Z, st);
-        S.reduce<3>(RedSet<3>{{{Z, Z, RK_SUMSQ}, {R, R, RK_SUMSQ}, {Z, R, RK_DOT}}}, EPI_CG_NORM, 0, st);
-      }
+      ew2(n, OpCgXR{X, R, P, W, S.d, al}, st);
+      S.pc.apply(R, Z, st);
+      S.reduce<3>(RedSet<3>{{{Z, Z, RK_SUMSQ}, {R, R, RK_SUMSQ}, {Z, R, RK_DOT}}}, EPI_CG_NORM, 0, st);
     }
     S.mark(slot);
     if (pending && S.done(slot ^ 1)) break;

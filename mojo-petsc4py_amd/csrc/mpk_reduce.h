@@ -122,6 +122,33 @@ __device__ __forceinline__ void fold_partials(const double *P, int T, int K, dou
   }
 }
 
+// fold_partials for K reductions at once, in the same order for each: every
+// thread issues all of its loads before it adds, and the K block trees share
+// one barrier, so a block that needs the results pays one round trip.
+template <int K>
+__device__ __forceinline__ void fold_partials_k(const double *P, int T, double *res) {
+  __shared__ double sh[K][4];
+  const int t = (int)threadIdx.x;
+  double u[K];
+#pragma unroll
+  for (int k = 0; k < K; k++) u[k] = t < T ? P[(i64)k * T + t] : 0.0;
+  for (int b = t + MPK_NT; b < T; b += MPK_NT) {
+#pragma unroll
+    for (int k = 0; k < K; k++) u[k] = u[k] + P[(i64)k * T + b];
+  }
+  const int lane = t & 63, wave = t >> 6;
+#pragma unroll
+  for (int k = 0; k < K; k++) {
+    u[k] = wave_tree(u[k]);
+    if (lane == 0) sh[k][wave] = u[k];
+  }
+  __syncthreads();
+  if (t == 0) {
+#pragma unroll
+    for (int k = 0; k < K; k++) res[k] = (sh[k][0] + sh[k][2]) + (sh[k][1] + sh[k][3]);
+  }
+}
+
 struct NoEpilogue {
   __device__ void operator()(const double *) const {}
 };
