@@ -163,6 +163,26 @@ void mp_ordering_rcm(int64_t perm, int64_t level, int64_t visited, int64_t queue
 int64_t mpk_ilu_levels(int64_t bi, int64_t bj, int64_t bdiag, int64_t nrow, int64_t lrow, int64_t lptr);
 int64_t mpk_ilu_numeric(int64_t nrow, int64_t irow, int64_t icol, int64_t data, int64_t r, int64_t ic, int64_t bi,
                         int64_t bj, int64_t bdiag, int64_t lrow, int64_t lptr, int64_t nlev, int64_t ba);
+/* MatICCFactorNumeric_SeqAIJ (ported.mojo:1040-1124) on the device, in place
+ * of mp_pc_icc_numeric's host pass, with the same bits.  mpk_icc_visits
+ * (host, needs no device): the pass's row chains replayed without
+ * arithmetic.  bi, bj [H] int64: the layout of mp_icc_contiguous_cholesky;
+ * irow, icol, rip, riip [H] int64: A and the permutation with its inverse.
+ * Out [H]: vptr int32[nrow + 1], vrow and vpos int32[bi[nrow]] (the rows
+ * that update row k, in order, with the position of their entry (i, k)),
+ * lrow int32[nrow] and lptr int64[nlev + 1] (level sets); returns nlev, or
+ * -2 when a row lacks ascending strict columns or a diagonal slot, or its
+ * result would depend on values left over from earlier rows (then only the
+ * serial mp_pc_icc_numeric reproduces the reference).
+ * mpk_icc_numeric: irow, icol [D] int32 and data [D] float64 (A); rip, riip,
+ * bi, bj, vptr, vrow, vpos, lrow [D] int32; lptr [H] int64; u0 [D]
+ * float64[bi[nrow]] scratch; ua [D] float64[bi[nrow]] receives the factor.
+ * Returns 0, or -1 on error. */
+int64_t mpk_icc_visits(int64_t bi, int64_t bj, int64_t irow, int64_t icol, int64_t rip, int64_t riip, int64_t nrow,
+                       int64_t vptr, int64_t vrow, int64_t vpos, int64_t lrow, int64_t lptr);
+int64_t mpk_icc_numeric(int64_t nrow, int64_t irow, int64_t icol, int64_t data, int64_t rip, int64_t riip,
+                        int64_t bi, int64_t bj, int64_t vptr, int64_t vrow, int64_t vpos, int64_t lrow, int64_t lptr,
+                        int64_t nlev, int64_t u0, int64_t ua);
 /* No counterpart in the reference: the transpose of an IC factor's strict
  * part, which the device forward sweep gathers over.  [D] int32 arrays:
  * trp[nrow + 1], trow[m], tpos[m], tmid[nrow], m = mpk_icc_transpose_size().

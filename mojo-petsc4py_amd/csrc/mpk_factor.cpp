@@ -459,6 +459,100 @@ void mp_pc_icc_numeric(int64_t ua_, int64_t rtmp_, int64_t il_, int64_t c2r_, in
   }
 }
 
+// The up-looking IC pass (MatICCFactorNumeric_SeqAIJ, ported.mojo:1040-1124)
+// replayed without arithmetic, for the level-by-level numeric pass on the
+// device: its row chains (il / c2r) decide which rows i < k hand row k an
+// update, and in what order; that order is integer work, done here.  bi, bj
+// [H] int64: the factor layout of mp_icc_contiguous_cholesky (row k's
+// strict columns, then its diagonal slot at bi[k + 1] - 1); irow, icol [H]
+// int64: A; rip, riip [H] int64: the permutation and its inverse.  Out
+// (host): vptr int32[n + 1], and vrow / vpos int32[bi[n]]: the visits of
+// row k in order, each a row i and the position of (i, k) in ua; lrow
+// int32[n] and lptr int64[nlev + 1]: rows level by level, a row one level
+// past the deepest row that updates it.  Returns nlev, or -2 when the layout
+// is not one the device pass reproduces: every row must hold its strict
+// columns strictly ascending (level-of-fill can place some below the
+// diagonal; the replay follows the chains through them as the reference
+// does) and a diagonal slot; A's entries on or above the diagonal of the
+// last rows must not run past the factor's end; and where the diagonal slot holds
+// another column than k (the unpermuted index, under a reordering) A's row
+// must hold the diagonal, which is then the only write to column k's cell
+// before it is read.
+int64_t mpk_icc_visits(int64_t bi_, int64_t bj_, int64_t irow_, int64_t icol_, int64_t rip_, int64_t riip_,
+                       int64_t n, int64_t vptr_, int64_t vrow_, int64_t vpos_, int64_t lrow_, int64_t lptr_) {
+  const I *bi = (const I *)bi_, *bj = (const I *)bj_, *rp = (const I *)irow_, *ci = (const I *)icol_,
+          *rip = (const I *)rip_, *riip = (const I *)riip_;
+  int32_t *vptr = (int32_t *)vptr_, *vrow = (int32_t *)vrow_, *vpos = (int32_t *)vpos_, *lrow = (int32_t *)lrow_;
+  I *lptr = (I *)lptr_;
+  if (n <= 0) {
+    vptr[0] = 0;
+    lptr[0] = 0;
+    return 0;
+  }
+  if (bi[n] >= ((I)1 << 31)) return -2;
+  for (I k = 0; k < n; k++) {
+    const I s0 = bi[k], dg = bi[k + 1] - 1;
+    if (dg < s0 || bj[dg] < 0 || bj[dg] >= n) return -2;
+    for (I j = s0; j < dg; j++)
+      if (bj[j] < 0 || bj[j] >= n || (j > s0 && bj[j] <= bj[j - 1])) return -2;
+    I up = 0, has_diag = 0;
+    for (I j = rp[rip[k]]; j < rp[rip[k] + 1]; j++) {
+      up += riip[ci[j]] >= k;
+      has_diag |= riip[ci[j]] == k;
+    }
+    // the reference zeroes one slot per such entry from the row's start; past
+    // the row's end those are later rows' slots, which they rewrite
+    if (s0 + up > bi[n]) return -2;
+    if (bj[dg] != k && !has_diag) {
+      bool slot = false;  // column k may still have a strict slot
+      for (I j = s0; j < dg; j++) slot |= bj[j] == k;
+      if (!slot) return -2;
+    }
+  }
+  std::vector<I> chain(n, n), first(n, 0);
+  std::vector<int32_t> lev(n, 0);
+  I nv = 0;
+  int32_t top = 0;
+  for (I k = 0; k < n; k++) {
+    vptr[k] = (int32_t)nv;
+    int32_t l = 0;
+    for (I i = chain[k]; i < k;) {
+      const I inext = chain[i];
+      const I f = first[i];
+      if (nv >= bi[n]) return -2;  // more visits than entries: a chain loops
+      vrow[nv] = (int32_t)i;
+      vpos[nv] = (int32_t)f;
+      nv++;
+      l = std::max(l, lev[i] + 1);
+      const I jmin = f + 1, jmax = bi[i + 1];
+      if (jmin < jmax) {
+        first[i] = jmin;
+        const I cj = bj[jmin];
+        chain[i] = chain[cj];
+        chain[cj] = i;
+      }
+      i = inext;
+    }
+    lev[k] = l;
+    top = std::max(top, l);
+    const I jmin = bi[k], jmax = bi[k + 1] - 1;
+    if (jmin < jmax) {
+      first[k] = jmin;
+      const I cj = bj[jmin];
+      chain[k] = chain[cj];
+      chain[cj] = k;
+    }
+  }
+  vptr[n] = (int32_t)nv;
+  const I nlev = (I)top + 1;
+  std::vector<I> cnt(nlev + 1, 0);
+  for (I k = 0; k < n; k++) cnt[lev[k] + 1]++;
+  for (I l = 0; l < nlev; l++) cnt[l + 1] += cnt[l];
+  for (I l = 0; l <= nlev; l++) lptr[l] = cnt[l];
+  for (I k = 0; k < n; k++) lrow[cnt[lev[k]]++] = (int32_t)k;
+  return nlev;
+}
+
 // Block plan of a triangular sweep (host part; the device part is
 // k_sweep_blk in mpk_sweep.h, which documents the layout and encoding).  off [H]
 // int64[n+1], col [H] int32 and coef [H] float64 hold every entry of every

@@ -453,6 +453,84 @@ __global__ void __launch_bounds__(256) k_ilu_numeric(i64 p0, i64 cnt, const int 
   ba[dg] = 1.0 / ba[dg];
 }
 
+// ------------------------------------------------------------ IC numeric
+
+// Cell of column c in row k's work row: the slot of c among the strict
+// columns (ascending in [s0, dg), some possibly below the diagonal), else the
+// diagonal slot dg when c is its column (the pattern stores there the row's
+// unpermuted index, which is k in the natural ordering), else -2 for column
+// k itself when the diagonal slot holds another column, else -1 (a column
+// outside the pattern).
+__device__ __forceinline__ int icc_cell(const int *__restrict__ bj, int s0, int dg, int k, int c) {
+  int lo = s0, hi = dg;
+  while (lo < hi) {
+    const int mid = (lo + hi) >> 1;
+    if (bj[mid] < c)
+      lo = mid + 1;
+    else
+      hi = mid;
+  }
+  if (lo < dg && bj[lo] == c) return lo;
+  if (c == bj[dg]) return dg;
+  return c == k ? -2 : -1;
+}
+
+// MatICCFactorNumeric_SeqAIJ (ported.mojo:1040-1124) for the rows of one
+// level, a thread per row.  The reference's dense work row becomes row k's
+// own slots of ua, plus a register for column k when the diagonal slot holds
+// another column (columns outside the pattern, which the reference writes to
+// and never reads, are skipped).  The same operations in the same order:
+// zero the row's slots, load A's row rip[k] on and above the diagonal, dk =
+// column k's value, then for each visit (i, f) in the chains' order
+// (mpk_icc_visits): uikdi = -u_f * (1/d_i), dk = fma(uikdi, u_f, dk), slot f
+// takes uikdi, and row i's later slots (its diagonal slot included, as the
+// reference's loop includes it) update the work row; finally the row's
+// values and 1/dk.  Visits read a row's values as it finished them from u0,
+// since rows of one level scale entries of a shared earlier row.
+__global__ void __launch_bounds__(256) k_icc_numeric(i64 p0, i64 cnt, const int *__restrict__ lrow,
+                                                     const int *__restrict__ rp, const int *__restrict__ ci,
+                                                     const double *__restrict__ a, const int *__restrict__ rip,
+                                                     const int *__restrict__ riip, const int *__restrict__ bi,
+                                                     const int *__restrict__ bj, const int *__restrict__ vptr,
+                                                     const int *__restrict__ vrow, const int *__restrict__ vpos,
+                                                     double *u0, double *ua) {
+  const i64 t = (i64)blockIdx.x * 256 + threadIdx.x;
+  if (t >= cnt) return;
+  const int k = lrow[p0 + t];
+  const int s0 = bi[k], dg = bi[k + 1] - 1;
+  for (int j = s0; j <= dg; j++) ua[j] = 0.0;
+  double wk = 0.0;  // column k's cell when it has no slot (A's diagonal is then present)
+  const int ra = rip[k];
+  for (int j = rp[ra]; j < rp[ra + 1]; j++) {
+    const int c = riip[ci[j]];
+    if (c >= k) {
+      const int q = icc_cell(bj, s0, dg, k, c);
+      if (q >= 0)
+        ua[q] = a[j];
+      else if (q == -2)
+        wk = a[j];
+    }
+  }
+  const int qk = icc_cell(bj, s0, dg, k, k);
+  double dk = qk >= 0 ? ua[qk] : wk;
+  for (int v = vptr[k]; v < vptr[k + 1]; v++) {
+    const int i = vrow[v], f = vpos[v];
+    const double uf = u0[f];
+    const int di = bi[i + 1] - 1;
+    const double uikdi = -uf * u0[di];
+    dk = fma(uikdi, uf, dk);
+    ua[f] = uikdi;
+    for (int j = f + 1; j <= di; j++) {
+      const int q = icc_cell(bj, s0, dg, k, bj[j]);
+      if (q >= 0) ua[q] = fma(uikdi, u0[j], ua[q]);
+    }
+  }
+  for (int j = s0; j < dg; j++) u0[j] = ua[j];
+  const double inv = 1.0 / dk;
+  u0[dg] = inv;
+  ua[dg] = inv;
+}
+
 // ------------------------------------------------------------ IC transpose map
 
 // Transpose of the strict upper part of an IC factor: for column k, the rows
@@ -722,6 +800,34 @@ int64_t mpk_ilu_numeric(int64_t n, int64_t irow, int64_t icol, int64_t data, int
     MPK_LAUNCH(k_ilu_numeric, nblocks(cnt, 256), 256, (i64)lp[l], cnt, (const int *)lrow, (const int *)irow,
                (const int *)icol, (const double *)data, (const int *)r, (const int *)ic, (const int *)bi,
                (const int *)bj, (const int *)bdiag, (double *)ba);
+  }
+  return failed() ? -1 : 0;
+}
+
+// MatICCFactorNumeric_SeqAIJ on the device, one launch per level of the
+// chains' visits (vptr / vrow / vpos, lrow / lptr from mpk_icc_visits); u0
+// [D] float64[bi[n]] is scratch for the values as each row finished them.
+int64_t mpk_icc_numeric(int64_t n, int64_t irow, int64_t icol, int64_t data, int64_t rip, int64_t riip, int64_t bi,
+                        int64_t bj, int64_t vptr, int64_t vrow, int64_t vpos, int64_t lrow, int64_t lptr,
+                        int64_t nlev, int64_t u0, int64_t ua) {
+  if (n <= 0) return 0;
+  if (!ctx()) return -1;
+  const int64_t *lp = (const int64_t *)lptr;
+  if (nlev <= 0 || lp[0] != 0 || lp[nlev] != n) {
+    set_error(2, "mpk_icc_numeric: the level bounds do not cover the rows");
+    return -1;
+  }
+  for (int64_t l = 0; l < nlev; l++) {
+    const i64 cnt = lp[l + 1] - lp[l];
+    if (cnt < 0) {
+      set_error(2, "mpk_icc_numeric: level bounds out of order");
+      return -1;
+    }
+    if (cnt == 0) continue;
+    MPK_LAUNCH(k_icc_numeric, nblocks(cnt, 256), 256, (i64)lp[l], cnt, (const int *)lrow, (const int *)irow,
+               (const int *)icol, (const double *)data, (const int *)rip, (const int *)riip, (const int *)bi,
+               (const int *)bj, (const int *)vptr, (const int *)vrow, (const int *)vpos, (double *)u0,
+               (double *)ua);
   }
   return failed() ? -1 : 0;
 }

@@ -408,13 +408,8 @@ class PC:
         bj = np.zeros(max(cap, int(ui[-1]) + 1), dtype=np.int64)
         bdiag[:n] = udiag[:n]
         L.mp_icc_contiguous_cholesky(_a(uj), _a(ui), _a(bdiag), _a(bi), _a(bdiag), _a(bj), n)
-        ua = np.zeros(max(int(ui[-1]), 1), dtype=np.float64)
-        rtmp = np.zeros(n, dtype=np.float64)
-        il = np.zeros(n, dtype=np.int64)
-        c2r = np.zeros(n, dtype=np.int64)
-        L.mp_pc_icc_numeric(_a(ua), _a(rtmp), _a(il), _a(c2r), _a(m.irow), _a(m.icol), _a(m.data), _a(bi),
-                            _a(bj), _a(bdiag), _a(r), _a(rinv), n)
-        self._factor = {"bi": bi, "bj": bj, "ba": ua, "bdiag": bdiag, "r": r}
+        # the values are computed on the device (_icc_numeric_device)
+        self._factor = {"bi": bi, "bj": bj, "ba": None, "bdiag": bdiag, "r": r, "rinv": rinv}
 
     def _upload_factor(self, m: Mat, n: int):
         f = self._factor
@@ -426,7 +421,7 @@ class PC:
             self._dbufs["c"] = self._dbufs["r"]
             self._ilu_numeric_device(m, n, nnz)
             return
-        self._dbufs["ba"] = DeviceBuffer.from_array(f["ba"])
+        self._icc_numeric_device(m, n, nnz)
         L = lib()
         bi, bj, bd = self._dbufs["bi"].ptr, self._dbufs["bj"].ptr, self._dbufs["bdiag"].ptr
         mt = L.mpk_icc_transpose_size(bi, bd, n)
@@ -473,8 +468,54 @@ class PC:
         L.mpk_synchronize()
         f["nnz"] = nnz
 
+    def _icc_numeric_device(self, m: Mat, n: int, nnz: int):
+        """MatICCFactorNumeric_SeqAIJ (ported.mojo:1040-1124) on the device:
+        the host replays the up-looking pass's row chains (which rows update
+        which, in what order, and the level sets that follow), the device
+        does the arithmetic one level per launch.  A layout whose result
+        depends on values the reference's dense work row carries from row to
+        row is factored by the serial pass on the host instead."""
+        L = lib()
+        f = self._factor
+        bi, bj = f["bi"], f["bj"]
+        mm = max(nnz, 1)
+        vptr = np.zeros(n + 1, dtype=np.int32)
+        vrow = np.zeros(mm, dtype=np.int32)
+        vpos = np.zeros(mm, dtype=np.int32)
+        lrow = np.zeros(max(n, 1), dtype=np.int32)
+        lptr = np.zeros(n + 1, dtype=np.int64)
+        nlev = L.mpk_icc_visits(_a(bi), _a(bj), _a(m.irow), _a(m.icol), _a(f["r"]), _a(f["rinv"]), n, _a(vptr),
+                                _a(vrow), _a(vpos), _a(lrow), _a(lptr))
+        if nlev == -2:
+            ua = np.zeros(mm, dtype=np.float64)
+            rtmp = np.zeros(n, dtype=np.float64)
+            il = np.zeros(n, dtype=np.int64)
+            c2r = np.zeros(n, dtype=np.int64)
+            L.mp_pc_icc_numeric(_a(ua), _a(rtmp), _a(il), _a(c2r), _a(m.irow), _a(m.icol), _a(m.data), _a(bi),
+                                _a(bj), _a(f["bdiag"]), _a(f["r"]), _a(f["rinv"]), n)
+            f["ba"] = ua
+            self._dbufs["ba"] = DeviceBuffer.from_array(ua)
+            return
+        if nlev < 0:
+            raise RuntimeError("replaying the IC factor's row chains failed")
+        d = m._dev()
+        b = self._dbufs
+        up = {k: DeviceBuffer.from_array(v) for k, v in
+              (("riip", f["rinv"].astype(np.int32)), ("vptr", vptr), ("vrow", vrow), ("vpos", vpos),
+               ("lrow", lrow))}
+        u0 = DeviceBuffer(8 * mm)
+        ua = self._buf("ba", 8 * mm)
+        rc = L.mpk_icc_numeric(n, d.rp.ptr, d.ci.ptr, d.v.ptr, b["r"].ptr, up["riip"].ptr, b["bi"].ptr,
+                               b["bj"].ptr, up["vptr"].ptr, up["vrow"].ptr, up["vpos"].ptr, up["lrow"].ptr,
+                               _a(lptr), nlev, u0.ptr, ua.ptr)
+        check()
+        if rc != 0:
+            raise RuntimeError("the device IC factorisation failed")
+        L.mpk_synchronize()
+        f["nnz"] = nnz
+
     def _host_factor(self) -> dict:
-        """The factor with its values on the host (ILU values are computed on
+        """The factor with its values on the host (the values are computed on
         the device and downloaded on first request)."""
         f = self._factor
         if f is not None and f["ba"] is None:

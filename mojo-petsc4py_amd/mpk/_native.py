@@ -117,6 +117,8 @@ SIGNATURES = {
     "mpk_icc_transpose": ([I] * 8, I),
     "mpk_ilu_levels": ([I] * 6, I),
     "mpk_ilu_numeric": ([I] * 13, I),
+    "mpk_icc_visits": ([I] * 12, I),
+    "mpk_icc_numeric": ([I] * 16, I),
 }
 
 # exported names that only a device call can exercise

@@ -304,3 +304,71 @@ def test_pc_ilu_set_up_on_device(O, mat, levels, ordering):
     y = PETSc.Vec().createSeq(n)
     pc.apply(PETSc.Vec().createWithArray(b), y)
     assert same_bits(y.getArray(), O.PCSetup(O.CSR(ip, ix, va), "ilu", levels, ordering).apply(b))
+
+
+ICC_DEV_MATS = {
+    "lap64": lambda: W.laplacian_2d(64),
+    "diff12": lambda: W.diffusion_3d(12)[0],
+    "tri33": lambda: W.tridiagonal(33),
+    "diff20": lambda: W.diffusion_3d(20)[0],
+}
+
+
+@pytest.mark.parametrize("name", sorted(ICC_DEV_MATS))
+@pytest.mark.parametrize("levels,ordering", [(0, "natural"), (1, "natural"), (2, "natural"), (0, "rcm"),
+                                             (2, "rcm")])
+def test_icc_numeric_device_bitwise(native, O, name, levels, ordering):
+    """MatICCFactorNumeric_SeqAIJ on the device (mpk_icc_visits' replay of
+    the row chains, then mpk_icc_numeric level by level) gives the oracle's
+    factor values bit for bit, including the layouts level-of-fill and the
+    reordering give (columns below the diagonal, the unpermuted index in the
+    diagonal slot)."""
+    ip, ix, va = ICC_DEV_MATS[name]()
+    A = O.CSR(ip, ix, va)
+    n = A.n
+    f = O.build_factor(A, "icc", levels, ordering)
+    bi, bj = f["bi"], f["bj"]
+    nnz = int(bi[n])
+    vptr, vrow, vpos = np.zeros(n + 1, np.int32), np.zeros(nnz, np.int32), np.zeros(nnz, np.int32)
+    lrow, lptr = np.zeros(n, np.int32), np.zeros(n + 1, np.int64)
+    c = lambda x: x.ctypes.data
+    nlev = native.mpk_icc_visits(c(bi), c(bj), c(A.irow), c(A.icol), c(f["r"]), c(f["rinv"]), n, c(vptr), c(vrow),
+                                 c(vpos), c(lrow), c(lptr))
+    assert nlev > 0
+    D = DevMat(ip, ix, va)
+    d = {k: dev(np.asarray(v).astype(np.int32)) for k, v in
+         (("r", f["r"]), ("riip", f["rinv"]), ("bi", bi), ("bj", bj[:nnz]), ("vptr", vptr), ("vrow", vrow),
+          ("vpos", vpos), ("lrow", lrow))}
+    du0 = dev(np.zeros(nnz))
+    dua = dev(np.full(nnz, np.nan))
+    rc = native.mpk_icc_numeric(n, D.rp.ptr, D.ci.ptr, D.v.ptr, d["r"].ptr, d["riip"].ptr, d["bi"].ptr,
+                                d["bj"].ptr, d["vptr"].ptr, d["vrow"].ptr, d["vpos"].ptr, d["lrow"].ptr, c(lptr),
+                                nlev, du0.ptr, dua.ptr)
+    assert rc == 0
+    assert same_bits(back(dua, np.float64, nnz), f["ba"][:nnz])
+
+
+@pytest.mark.parametrize("levels,ordering", [(0, "natural"), (2, "rcm")])
+def test_pc_icc_set_up_on_device(O, levels, ordering):
+    """PC.setUp's IC factor through the PETSc surface (computed on the
+    device) equals the oracle's, and so does its apply."""
+    from mojopetsc4py import PETSc
+    ip, ix, va = W.diffusion_3d(16)[0]
+    n = len(ip) - 1
+    A = PETSc.Mat().createSeqAIJ((n, n), nnz=7)
+    A.setValuesCSR(ip, ix, va)
+    A.assemble()
+    pc = PETSc.PC().create()
+    pc.setType("icc")
+    pc.setFactorLevels(levels)
+    pc.setFactorOrdering(ordering)
+    pc.setOperators(A)
+    pc.setUp()
+    want = O.build_factor(O.CSR(ip, ix, va), "icc", levels, ordering)
+    got = pc._host_factor()
+    nnz = int(want["bi"][n])
+    assert same_bits(got["ba"][:nnz], want["ba"][:nnz])
+    b = rng_vec(n, 4)
+    y = PETSc.Vec().createSeq(n)
+    pc.apply(PETSc.Vec().createWithArray(b), y)
+    assert same_bits(y.getArray(), O.PCSetup(O.CSR(ip, ix, va), "icc", levels, ordering).apply(b))

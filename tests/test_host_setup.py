@@ -205,3 +205,113 @@ def test_degenerate_patterns_have_shared_diagonal_slots(oracle):
         f = oracle.build_factor(oracle.CSR(*ODD[name]()), "ilu", lv, "natural")
         bd = f["bdiag"]
         assert any(bd[i] <= bd[i + 1] for i in range(len(bd) - 1)), name
+
+
+def _fma(a, b, c):
+    """fma(a, b, c) rounded once (exact product and sum, then to double)."""
+    from fractions import Fraction
+    return float(Fraction(a) * Fraction(b) + Fraction(c))
+
+
+def _icc_numeric_levels(A, f, vptr, vrow, vpos, lrow, lptr, nlev):
+    """The device IC numeric pass (k_icc_numeric) restated, level by level."""
+    n = A.n
+    bi, bj, r, rinv = f["bi"], f["bj"], f["r"], f["rinv"]
+    ua = np.zeros(max(int(bi[n]), 1))
+    u0 = np.zeros_like(ua)
+
+    def cell(k, c):
+        s0, dg = bi[k], bi[k + 1] - 1
+        hit = np.nonzero(bj[s0:dg] == c)[0]
+        if len(hit):
+            return s0 + hit[0]
+        if c == bj[dg]:
+            return dg
+        return -2 if c == k else -1
+
+    for lev in range(nlev):
+        for k in lrow[lptr[lev]:lptr[lev + 1]]:
+            s0, dg = bi[k], bi[k + 1] - 1
+            ua[s0:dg + 1] = 0.0
+            wk = 0.0
+            ra = r[k]
+            for j in range(A.irow[ra], A.irow[ra + 1]):
+                c = rinv[A.icol[j]]
+                if c >= k:
+                    q = cell(k, c)
+                    if q >= 0:
+                        ua[q] = A.data[j]
+                    elif q == -2:
+                        wk = A.data[j]
+            qk = cell(k, k)
+            dk = ua[qk] if qk >= 0 else wk
+            for v in range(vptr[k], vptr[k + 1]):
+                i, fp = vrow[v], vpos[v]
+                uf = u0[fp]
+                di = bi[i + 1] - 1
+                uikdi = -uf * u0[di]
+                dk = _fma(uikdi, uf, dk)
+                ua[fp] = uikdi
+                for j in range(fp + 1, di + 1):
+                    q = cell(k, bj[j])
+                    if q >= 0:
+                        ua[q] = _fma(uikdi, u0[j], ua[q])
+            u0[s0:dg] = ua[s0:dg]
+            u0[dg] = ua[dg] = 1.0 / dk
+    return ua
+
+
+ICC_MATS = {
+    "lap12": lambda: W.laplacian_2d(12),
+    "tri30": lambda: W.tridiagonal(30),
+    "diff5": lambda: W.diffusion_3d(5)[0],
+    "diff7": lambda: W.diffusion_3d(7)[0],
+    "lap20": lambda: W.laplacian_2d(20),
+    "cd4sym": lambda: _symmetrized(W.convection_diffusion_3d(4)[0]),
+}
+
+
+def _symmetrized(csr):
+    """A + A^T of a nonsymmetric operator (an SPD-shaped pattern with uneven
+    values)."""
+    import scipy.sparse as sp
+    ip, ix, va = csr
+    n = len(ip) - 1
+    M = sp.csr_matrix((va, ix, ip), shape=(n, n))
+    S = (M + M.T).tocsr()
+    S.sort_indices()
+    return S.indptr.astype(np.int64), S.indices.astype(np.int64), S.data.astype(np.float64)
+
+
+@pytest.mark.parametrize("name", sorted(ICC_MATS))
+@pytest.mark.parametrize("levels,ordering", [(0, "natural"), (1, "natural"), (2, "natural"), (0, "rcm"),
+                                             (2, "rcm")])
+def test_icc_visits_reproduce_the_oracle(L, oracle, name, levels, ordering):
+    """mpk_icc_visits replays the up-looking pass's row chains: the device
+    pass restated over its visits and level sets gives the oracle's IC factor
+    (MatICCFactorNumeric_SeqAIJ restated) bit for bit, and every row's
+    updaters sit in earlier levels."""
+    A = oracle.CSR(*ICC_MATS[name]())
+    n = A.n
+    f = oracle.build_factor(A, "icc", levels, ordering)
+    bi, bj = f["bi"], f["bj"]
+    m = max(int(bi[n]), 1)
+    vptr, vrow, vpos = np.zeros(n + 1, np.int32), np.zeros(m, np.int32), np.zeros(m, np.int32)
+    lrow, lptr = np.zeros(n, np.int32), np.zeros(n + 1, np.int64)
+    nlev = L.mpk_icc_visits(a_(bi), a_(bj), a_(A.irow), a_(A.icol), a_(f["r"]), a_(f["rinv"]), n, a_(vptr),
+                            a_(vrow), a_(vpos), a_(lrow), a_(lptr))
+    assert nlev > 0
+    lev = np.zeros(n, np.int64)
+    for l in range(nlev):
+        lev[lrow[lptr[l]:lptr[l + 1]]] = l
+    seen = set()
+    for k in range(n):
+        for v in range(vptr[k], vptr[k + 1]):
+            # the updater is finished first; no entry is visited twice (a
+            # visit scales its entry).  With columns below the diagonal the
+            # chains can hand a row over at another column's step.
+            assert lev[vrow[v]] < lev[k] and vpos[v] not in seen
+            seen.add(vpos[v])
+    got = _icc_numeric_levels(A, f, vptr, vrow, vpos, lrow, lptr, nlev)
+    nnz = int(bi[n])
+    assert np.array_equal(got[:nnz].view(np.int64), f["ba"][:nnz].view(np.int64))
