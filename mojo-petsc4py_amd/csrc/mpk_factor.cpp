@@ -644,6 +644,9 @@ int64_t mpk_sweep_plan(int64_t n, int64_t forward, int64_t off_, int64_t col_, i
       lev[i] = l;
       nl = std::max<int64_t>(nl, l + 1);
     }
+    // a ring needs the average level to hold at most kRingWidth rows: decided
+    // by the depth alone, before the latest levels are worked out
+    if (mode != 2 && n > kRingWidth * nl) goto blocks;
     std::fill(lev.begin(), lev.end(), (int)(nl - 1));
     for (int64_t s = n - 1; s >= 0; s--) {
       const int64_t i = forward ? s : n - 1 - s;
@@ -711,6 +714,7 @@ int64_t mpk_sweep_plan(int64_t n, int64_t forward, int64_t off_, int64_t col_, i
       return bytes;
     }
   }
+blocks:
   const int64_t B = plan_rows(n, bmax), nblk = n > 0 ? (n + B - 1) / B : 0;
   // local levels
   std::vector<int> lev(n > 0 ? n : 1, 0);

@@ -7,6 +7,7 @@
 // PCApply_SOR (src/ported.mojo:633-952) on the same factor.
 #include <map>
 #include <tuple>
+#include <memory>
 #include <vector>
 
 #include "mpk_internal.h"
@@ -99,25 +100,36 @@ static i64 plan_sweep(const Prog &prog, i64 n, bool forward, char *dst, i64 cap,
     set_error(5, "sweep plan buffer too small");
     return -1;
   }
-  std::vector<int64_t> host((size_t)(bound + 7) / 8);
+  // the planner writes every byte of its plan (it clears the buffer first)
+  std::unique_ptr<int64_t[]> host(new int64_t[(size_t)(bound + 7) / 8]);
   const i64 bytes = mpk_sweep_plan(n, forward ? 1 : 0, (int64_t)off.data(), (int64_t)col.data(),
-                                   (int64_t)coef.data(), MPK_BLK_ROWS, (int64_t)host.data(), bound);
+                                   (int64_t)coef.data(), MPK_BLK_ROWS, (int64_t)host.get(), bound);
   if (bytes < 0) {
     set_error(5, "sweep plan larger than its bound");
     return -1;
   }
-  copy_htod(dst, host.data(), bytes);
+  copy_htod(dst, host.get(), bytes);
   *depth = host[3];
   return failed() ? -1 : bytes;
 }
 
-// Entry count of one sweep (for the plan bound).
+// Entry count of one sweep (for the plan bound): the counting pass of
+// k_sweep_describe only.
 template <class Prog>
 static i64 sweep_entries(const Prog &prog, i64 n) {
-  std::vector<i64> off;
-  std::vector<int> col;
-  std::vector<double> coef;
-  return describe_sweep(prog, n, off, col, coef) ? off[n] : -1;
+  int *cnt_d = nullptr;
+  std::vector<int> cnt(n);
+  HIPCHK(hipMalloc(&cnt_d, n * sizeof(int)));
+  if (failed()) return -1;
+  MPK_LAUNCH(k_sweep_describe<Prog>, nblocks(n, 256), 256, prog, n, 0, cnt_d, (const i64 *)nullptr,
+             (int *)nullptr, (double *)nullptr);
+  HIPCHK(hipMemcpyAsync(cnt.data(), cnt_d, n * sizeof(int), hipMemcpyDeviceToHost, stream()));
+  HIPCHK(hipStreamSynchronize(stream()));
+  HIPCHK(hipFree(cnt_d));
+  if (failed()) return -1;
+  i64 m = 0;
+  for (i64 i = 0; i < n; i++) m += cnt[i];
+  return m;
 }
 
 // ------------------------------------------------------------ row programs
