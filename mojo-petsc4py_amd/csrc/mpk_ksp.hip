@@ -13,6 +13,7 @@
 // fma() where it contracts, so given the same reductions the iterates are
 // bitwise those of the CPU path.
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <memory>
 #include <vector>
@@ -566,8 +567,8 @@ struct OpCgXR {
     st2(r, e, n, al, D2{fma(-a, wv.a, rv.a), fma(-a, wv.b, rv.b)});
   }
 };
-// CG with Jacobi, one pass for the second half of an iteration.  Every block
-// first folds the T tile partials of p.w (fold_partials_k, k_finalize's order)
+// CG with Jacobi, one pass for the second half of an iteration.  With Pw,
+// every block first folds the T tile partials of p.w (fold_partials_k, k_finalize's order)
 // and takes the p.w step from them (ksp.mojo:350-362): the breakdown test
 // and a = beta / p.w, reading only state that step leaves alone; the block
 // of tile 0 also runs the step on the state.  Unless the solve ends there:
@@ -598,15 +599,20 @@ __global__ void __launch_bounds__(MPK_NT) k_cg_xrz(i64 n, double *x, double *r, 
       dv_[j] = ld2(dg, e, n, al);
     }
   }
-  fold_partials_k<1>(Pw, T, res);
-  if (threadIdx.x == 0) {
-    const double dpi = res[0];
-    brk = dpi == 0.0 || (s->i > 0 && sign_real(dpi) * sign_real(s->dpiold) < 0);
-    sa = s->beta / dpi;
-    if (blockIdx.x == 0) ksp_epilogue(s, EPI_CG_DPI, 0, res);
+  if (Pw) {
+    fold_partials_k<1>(Pw, T, res);
+    if (threadIdx.x == 0) {
+      const double dpi = res[0];
+      brk = dpi == 0.0 || (s->i > 0 && sign_real(dpi) * sign_real(s->dpiold) < 0);
+      sa = s->beta / dpi;
+      if (blockIdx.x == 0) ksp_epilogue(s, EPI_CG_DPI, 0, res);
+    }
+    __syncthreads();
+    if (brk) return;
+  } else if (threadIdx.x == 0) {
+    sa = s->a;  // the p.w step ran in a finalize (the stop flag covers its breakdown)
   }
-  __syncthreads();
-  if (brk) return;
+  if (!Pw) __syncthreads();
   const double a = sa;
   double lz[4], lr[4], lzr[4];
 #pragma unroll
@@ -1071,6 +1077,14 @@ struct Solve {
   }
 };
 
+// Tiles up to which CG + Jacobi folds its reductions' partials in every block
+// of the kernels that use them: 2048 (4,194,304 rows); MPK_CG_FOLD_TILES=k
+// overrides it (read at each solve; tuning and tests).
+static unsigned cg_fold_tiles() {
+  const char *e = std::getenv("MPK_CG_FOLD_TILES");
+  return e ? (unsigned)std::strtoul(e, nullptr, 10) : 2048u;
+}
+
 // Iterations queued between two reads of the stop flag.
 static int chunk_for(i64 n) { return n < 65536 ? 64 : (n < 1048576 ? 16 : 8); }
 
@@ -1106,8 +1120,11 @@ void mp_ksp_solve_cg(int64_t n, int64_t irow, int64_t icol, int64_t data, int64_
   int slot = 0;
   bool pending = false;
   // CG + Jacobi over the sliced copy: three launches an iteration, the p.w
-  // and norm steps folded into the kernels that use their results
-  const bool fused = S.A.sell_off && n > 0 && S.pc.type == PCT_JACOBI;
+  // and norm steps folded into the kernels that use their results.  Every
+  // block folding every partial costs blocks x partials loads, so past
+  // cg_fold_tiles() tiles the steps run in finalizes instead (five launches).
+  const bool jac = S.A.sell_off && n > 0 && S.pc.type == PCT_JACOBI;
+  const bool fused = jac && ntiles(n) <= cg_fold_tiles();
   const unsigned nt = fused ? ntiles(n) : 0;
   double *Pw = fused ? partials((size_t)4 * nt) : nullptr, *Pn = fused ? Pw + nt : nullptr;
   if (fused && !Pw) return;
@@ -1134,6 +1151,27 @@ void mp_ksp_solve_cg(int64_t n, int64_t irow, int64_t icol, int64_t data, int64_
         continue;
       }
       ew2(n, OpCgP{P, Z, S.d, aligned16(P) && aligned16(Z)}, st);
+      if (jac) {
+        // five launches: AYPX, the product with its partial, the p.w step,
+        // the update with its partials, the norm step
+        const unsigned T = ntiles(n);
+        double *Pp = partials((size_t)3 * T);
+        if (sell_nt(n))
+          MPK_LAUNCH(k_cg_wdot<true>, T, 4 * MPK_NT, n, S.A.rp, S.A.sell_off, S.A.sell_v, S.A.sell_c,
+                     (const double *)P, W, Pp, st);
+        else
+          MPK_LAUNCH(k_cg_wdot<false>, T, 4 * MPK_NT, n, S.A.rp, S.A.sell_off, S.A.sell_v, S.A.sell_c,
+                     (const double *)P, W, Pp, st);
+        MPK_LAUNCH(k_finalize<KspEpi>, 1, MPK_NT, (const double *)Pp, (int)T, 1, (double *)nullptr,
+                   KspEpi{S.d, EPI_CG_DPI, 0}, st);
+        const bool al = aligned16(X) && aligned16(R) && aligned16(P) && aligned16(W) && aligned16(Z) &&
+                        aligned16(S.pc.jac_diag);
+        MPK_LAUNCH(k_cg_xrz, T, MPK_NT, n, X, R, Z, (const double *)P, (const double *)W, S.pc.jac_diag,
+                   (const double *)nullptr, (int)T, S.d, Pp, al, st);
+        MPK_LAUNCH(k_finalize<KspEpi>, 1, MPK_NT, (const double *)Pp, (int)T, 3, (double *)nullptr,
+                   KspEpi{S.d, EPI_CG_NORM, 0}, st);
+        continue;
+      }
       if (S.A.sell_off && n > 0) {
         // the product and its reduction in one pass (k_cg_wdot)
         const unsigned T = ntiles(n);

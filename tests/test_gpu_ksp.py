@@ -180,3 +180,18 @@ def test_indefinite_cg(native, O):
 def test_tiny_divtol(native, O):
     g = _compare(O, DIFF16, DIFF16B, "gmres", "none", rtol=1e-10, divtol=1e-30, max_it=50)
     assert g[2] == -4 and g[1] == 0
+
+
+@pytest.mark.parametrize("normtype", [2, 3])
+def test_cg_jacobi_finalize_steps(native, O, monkeypatch, normtype):
+    """CG + Jacobi with its scalar steps in finalize launches (the form large
+    systems take, forced here by a zero tile threshold) gives the same bits as
+    the oracle, as the folded form does; also a max_it stop and a NaN."""
+    monkeypatch.setenv("MPK_CG_FOLD_TILES", "0")
+    _compare(O, DIFF16, DIFF16B, "cg", "jacobi", normtype=normtype, rtol=1e-10, max_it=300)
+    g = _compare(O, DIFF16, DIFF16B, "cg", "jacobi", rtol=1e-14, max_it=7)
+    assert g[1] == 7 and g[2] == 4
+    b = DIFF16B.copy()
+    b[3] = np.nan
+    g = _compare(O, DIFF16, b, "cg", "jacobi", rtol=1e-10, max_it=50)
+    assert g[2] == -9
