@@ -372,3 +372,38 @@ def test_pc_icc_set_up_on_device(O, levels, ordering):
     y = PETSc.Vec().createSeq(n)
     pc.apply(PETSc.Vec().createWithArray(b), y)
     assert same_bits(y.getArray(), O.PCSetup(O.CSR(ip, ix, va), "icc", levels, ordering).apply(b))
+
+
+def _edge_mats():
+    """Small matrices for the branches of the numeric passes: one row; a
+    stored zero below the diagonal (ILU skips a zero multiplier,
+    ported.mojo:1003); a pivot that eliminates to exactly zero (1/0 = inf);
+    and a 2 x 2 with a negative pivot."""
+    one = (np.array([0, 1]), np.array([0]), np.array([4.0]))
+    zmul = (np.array([0, 2, 5, 7]), np.array([0, 1, 0, 1, 2, 1, 2]),
+            np.array([2.0, 1.0, 0.0, 3.0, 1.0, 1.0, 5.0]))
+    # row 1: a10 = 2, a11 = 1 -> u11 = 1 - (2/2)*1 = 0 exactly
+    zpiv = (np.array([0, 2, 4]), np.array([0, 1, 0, 1]), np.array([2.0, 1.0, 2.0, 1.0]))
+    neg = (np.array([0, 2, 4]), np.array([0, 1, 0, 1]), np.array([-3.0, 1.0, 1.0, 2.0]))
+    return {"one": one, "zmul": zmul, "zpiv": zpiv, "neg": neg}
+
+
+@pytest.mark.parametrize("name", sorted(_edge_mats()))
+@pytest.mark.parametrize("kind", ["ilu", "icc"])
+def test_device_numeric_edge_cases(native, O, name, kind):
+    """The device numeric passes on the edge cases give the oracle's factor
+    bit for bit (inf where a pivot vanishes), through PC.setUp."""
+    from mojopetsc4py import PETSc
+    ip, ix, va = _edge_mats()[name]
+    n = len(ip) - 1
+    A = PETSc.Mat().createSeqAIJ((n, n), nnz=3)
+    A.setValuesCSR(ip, ix, va)
+    A.assemble()
+    pc = PETSc.PC().create()
+    pc.setType(kind)
+    pc.setOperators(A)
+    pc.setUp()
+    want = O.build_factor(O.CSR(ip, ix, va), kind, 0, "natural")
+    nnz = int(want["bi"][n]) if kind == "icc" else int(want["bdiag"][0]) + 1
+    got = pc._host_factor()
+    assert same_bits(got["ba"][:nnz], want["ba"][:nnz])
