@@ -90,11 +90,9 @@ i64 sell_offsets(const int *rp_d, i64 nrow, std::vector<int> &soff) {
 }
 
 bool sell_nt(i64 nrow) {
-  static const int forced = [] {
-    const char *e = std::getenv("MPK_SELL_NT");
-    return e && (e[0] == '0' || e[0] == '1') ? e[0] - '0' : -1;
-  }();
-  return forced >= 0 ? forced == 1 : nrow >= 3000000;
+  const char *e = std::getenv("MPK_SELL_NT");  // read at each launch (tuning and tests)
+  if (e && (e[0] == '0' || e[0] == '1') && !e[1]) return e[0] == '1';
+  return nrow >= 3000000;
 }
 
 void spmv(const DevCSR &A, const double *x, double *y, int add, const int64_t *stop) {

@@ -46,7 +46,8 @@ struct DevCSR {
 void spmv(const DevCSR &A, const double *x, double *y, int add, const int64_t *stop);
 // Whether the sliced products of an nrow-row matrix load the matrix stream
 // non-temporally: from 3,000,000 rows on (a 7-point operator there is about
-// 270 MB, past the 256 MB last-level cache); MPK_SELL_NT=0|1 overrides.
+// 270 MB, past the 256 MB last-level cache); MPK_SELL_NT=0|1 overrides
+// (read at each call).
 bool sell_nt(i64 nrow);
 
 // Builds a sliced copy of A for the lifetime of the object and attaches it

@@ -195,3 +195,14 @@ def test_cg_jacobi_finalize_steps(native, O, monkeypatch, normtype):
     b[3] = np.nan
     g = _compare(O, DIFF16, b, "cg", "jacobi", rtol=1e-10, max_it=50)
     assert g[2] == -9
+
+
+def test_cg_jacobi_nontemporal_product(native, O, monkeypatch):
+    """CG + Jacobi with the non-temporal product kernels (the default from
+    3,000,000 rows, forced by MPK_SELL_NT=1), in both the folded and the
+    finalize form of the scalar steps: the oracle's bits."""
+    monkeypatch.setenv("MPK_SELL_NT", "1")
+    _compare(O, DIFF16, DIFF16B, "cg", "jacobi", rtol=1e-10, max_it=300)
+    monkeypatch.setenv("MPK_CG_FOLD_TILES", "0")
+    _compare(O, DIFF16, DIFF16B, "cg", "jacobi", rtol=1e-10, max_it=300)
+    _compare(O, DIFF16, DIFF16B, "gmres", "jacobi", rtol=1e-10, max_it=300)

@@ -58,10 +58,15 @@ SELL_MATS = dict(MATS)
 SELL_MATS.update({f"ragged{n}": (lambda n=n: _ragged(n, n)) for n in (1, 63, 64, 65, 130, 1000)})
 
 
+@pytest.mark.parametrize("nt", ["default", "1"])
 @pytest.mark.parametrize("name", sorted(SELL_MATS))
-def test_sliced_spmv_bitwise(native, O, name):
+def test_sliced_spmv_bitwise(native, O, monkeypatch, name, nt):
     """The sliced (64-row, column-major) copy gives the CSR row sums bit for
-    bit, for y = A x and y = y + A x, including empty rows and partial slices."""
+    bit, for y = A x and y = y + A x, including empty rows and partial slices;
+    also in the non-temporal form (the default from 3,000,000 rows, forced
+    here by MPK_SELL_NT=1)."""
+    if nt != "default":
+        monkeypatch.setenv("MPK_SELL_NT", nt)
     ip, ix, va = SELL_MATS[name]()
     A = O.CSR(ip, ix, va)
     D = DevMat(ip, ix, va)
@@ -407,3 +412,4 @@ def test_device_numeric_edge_cases(native, O, name, kind):
     nnz = int(want["bi"][n]) if kind == "icc" else int(want["bdiag"][0]) + 1
     got = pc._host_factor()
     assert same_bits(got["ba"][:nnz], want["ba"][:nnz])
+
