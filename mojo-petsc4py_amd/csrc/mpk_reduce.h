@@ -160,7 +160,16 @@ __global__ void __launch_bounds__(MPK_NT) k_finalize(const double *P, int T, int
                                                      Epi epi, const int64_t *stop) {
   if (stop && *stop) return;
   __shared__ double res[256];
-  fold_partials(P, T, K, res);
+  // one round of loads for the common counts (the same sums, in the same
+  // order, as fold_partials)
+  if (K == 1)
+    fold_partials_k<1>(P, T, res);
+  else if (K == 2)
+    fold_partials_k<2>(P, T, res);
+  else if (K == 3)
+    fold_partials_k<3>(P, T, res);
+  else
+    fold_partials(P, T, K, res);
   if (threadIdx.x == 0) {
     if (out)
       for (int k = 0; k < K; k++) out[k] = res[k];
