@@ -169,7 +169,18 @@ __global__ void __launch_bounds__(MPK_NT) k_finalize(const double *P, int T, int
   else if (K == 3)
     fold_partials_k<3>(P, T, res);
   else
-    fold_partials(P, T, K, res);
+    for (int k0 = 0; k0 < K; k0 += 4) {  // four at a time (GMRES's Gram-Schmidt sums)
+      const int kk = K - k0 < 4 ? K - k0 : 4;
+      if (kk == 4)
+        fold_partials_k<4>(P + (i64)k0 * T, T, res + k0);
+      else if (kk == 3)
+        fold_partials_k<3>(P + (i64)k0 * T, T, res + k0);
+      else if (kk == 2)
+        fold_partials_k<2>(P + (i64)k0 * T, T, res + k0);
+      else
+        fold_partials_k<1>(P + (i64)k0 * T, T, res + k0);
+      __syncthreads();  // the shared trees are reused by the next four
+    }
   if (threadIdx.x == 0) {
     if (out)
       for (int k = 0; k < K; k++) out[k] = res[k];
