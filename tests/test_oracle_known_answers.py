@@ -8,7 +8,8 @@
   divtol 1e30, max_it 2000.  BCGS is held to +-20 as test_ksp_parity.py:264-283
   holds it.
 * The device reduction order gives the same counts and an iterate within
-  1e-12 relative of the reference order (the north star's bar).
+  1e-12 relative of the reference order (the FP64 tolerance these tests
+  hold the device path to).
 """
 
 import numpy as np
