@@ -42,7 +42,7 @@ HBM_PEAK_GBS = 8000.0
 # HBM bytes per launch of the roofline kernel, from rocprofv3 FETCH_SIZE and
 # WRITE_SIZE runs scaled by a copy kernel of known size
 # (profiles/spmv_sell_diffusion128_traffic.md), keyed by (kernel, n, nnz).
-MEASURED_TRAFFIC = {("k_spmv_sell", 2097152, 14581760): 219285052}
+MEASURED_TRAFFIC = {("k_spmv_sell", 2097152, 14581760): 219283289}
 
 
 def spmv_bytes(n: int, nnz: int) -> int:
