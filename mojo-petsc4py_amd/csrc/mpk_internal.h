@@ -100,7 +100,11 @@ __device__ __forceinline__ D2 ld2(const double *p, i64 e, i64 n, bool al) {
       r.b = p[e + 1];
     }
   } else {
-    r.a = e < n ? p[e] : 0.0;
+    // e == n - 1 or past the end.  A relaxed atomic load (a plain load at
+    // workgroup scope), so that the compiler cannot merge it with the other
+    // branch's load of p[e]: the merge would split the aligned 16-byte access
+    // in two.
+    r.a = e < n ? __hip_atomic_load(p + e, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP) : 0.0;
     r.b = 0.0;
   }
   return r;

@@ -73,9 +73,22 @@ struct KspState {
 
 // ------------------------------------------------------------ scalar steps
 
+// The fields of the state the convergence test and CG's norm step read,
+// loaded at once so their latency overlaps a kernel's other loads (none of
+// them changes once rnorm0 is seeded).
+struct ConvView {
+  i64 convmaxits, max_it, normtype, chknorm, min_it;
+  double ttol, atol, divtol, rnorm0;
+  __device__ static ConvView load(const KspState *s) {
+    return ConvView{s->convmaxits, s->max_it, s->normtype, s->chknorm, s->min_it,
+                    s->ttol,       s->atol,   s->divtol,   s->rnorm0};
+  }
+};
+
 // KSPConvergedDefault (ksp.mojo:96-152) once rnorm0 is seeded: reads the
-// state only.
-__device__ i64 conv_check(const KspState *s, i64 n, double rnorm) {
+// state (a KspState or a ConvView) only.
+template <class S>
+__device__ i64 conv_check(const S *s, i64 n, double rnorm) {
   if (s->convmaxits && n >= s->max_it) return KSP_CONVERGED_ITS;
   if (s->normtype == NORM_NONE) return 0;
   if (n <= s->chknorm) return 0;
@@ -187,7 +200,8 @@ __device__ i64 update_hessenberg(KspState *s, i64 it, i64 hapend, double *res) {
 }
 
 // CG's residual norm after iteration i from {z.z, r.r, z.r} (ksp.mojo:364-372)
-__device__ double cg_norm(const KspState *s, i64 i, const double *r) {
+template <class S>
+__device__ double cg_norm(const S *s, i64 i, const double *r) {
   if (s->normtype == NORM_PRECONDITIONED && s->chknorm < i + 2) return sqrt(r[0]);
   if (s->normtype == NORM_UNPRECONDITIONED && s->chknorm < i + 2) return sqrt(r[1]);
   if (s->normtype == NORM_NATURAL) return sqrt(fabs(r[2]));
@@ -491,14 +505,21 @@ __global__ void __launch_bounds__(4 * MPK_NT) k_cg_wdot(i64 n, const int *__rest
                                                         const int *__restrict__ soff, const double *__restrict__ sv,
                                                         const int *__restrict__ sc, const double *__restrict__ p,
                                                         double *__restrict__ w, double *P, const int64_t *stop) {
-  if (stop && *stop) return;
   __shared__ double lf[4 * MPK_NT];
   __shared__ double sh[4];
+  // the stop flag and the row bounds in one round of loads (indices clamped
+  // to row n - 1, n >= 1, so that no load waits on a branch), before the
+  // entries are read
+  const bool stopped = *stop != 0;  // never null here: the solve's own flag
   const unsigned tile = xcd_block(blockIdx.x, gridDim.x);
   const int T = (int)threadIdx.x;
   const i64 e0 = (i64)tile * MPK_TILE + T, e1 = e0 + MPK_TILE / 2;
-  const int len0 = e0 < n ? rp[e0 + 1] - rp[e0] : 0, len1 = e1 < n ? rp[e1 + 1] - rp[e1] : 0;
-  const int q0 = e0 < n ? soff[e0 >> 6] + (int)(e0 & 63) : 0, q1 = e1 < n ? soff[e1 >> 6] + (int)(e1 & 63) : 0;
+  const i64 c0 = e0 < n ? e0 : n - 1, c1 = e1 < n ? e1 : n - 1;
+  const int r00 = rp[c0], r01 = rp[c0 + 1], r10 = rp[c1], r11 = rp[c1 + 1];
+  const int o0 = soff[c0 >> 6], o1 = soff[c1 >> 6];
+  const int len0 = e0 < n ? r01 - r00 : 0, len1 = e1 < n ? r11 - r10 : 0;
+  const int q0 = e0 < n ? o0 + (int)(e0 & 63) : 0, q1 = e1 < n ? o1 + (int)(e1 & 63) : 0;
+  if (stopped) return;
   const int len = len0 > len1 ? len0 : len1;
   // both rows' sums in order (sell_row's), four entries of each in flight
   double w0 = 0.0, w1 = 0.0;
@@ -577,15 +598,20 @@ struct OpCgXR {
 // elements the tile reduction gives it, so every value and every partial is
 // the one the separate kernels produce.  w may alias z (the reference's CG
 // keeps W = Z).
+template <bool al>  // all six vectors 16-byte aligned (a constant, so every pair is one 16-byte access)
 __global__ void __launch_bounds__(MPK_NT) k_cg_xrz(i64 n, double *x, double *r, double *z, const double *p,
                                                    const double *w, const double *dg, const double *Pw, int T,
-                                                   KspState *s, double *P, bool al, const int64_t *stop) {
-  if (stop && *stop) return;
+                                                   KspState *s, double *P, const int64_t *stop) {
   __shared__ double sh[4];
   __shared__ double res[1];
   __shared__ double sa;
   __shared__ int brk;
-  // the tile's operands are loaded first, so their latency overlaps the fold
+  // every load is issued before any is waited on: the stop flag, the state
+  // the p.w step reads, the tile's operands, then the partials; a stopped
+  // solve returns before anything is written
+  const bool stopped = *stop != 0;  // never null here: the solve's own flag
+  const i64 si = s->i;
+  const double dpiold = s->dpiold, sbeta = s->beta, s_a = s->a;
   const i64 base = (i64)blockIdx.x * MPK_TILE + 2 * (i64)threadIdx.x;
   D2 xv_[4], pv_[4], rv_[4], wv_[4], dv_[4];
 #pragma unroll
@@ -599,21 +625,21 @@ __global__ void __launch_bounds__(MPK_NT) k_cg_xrz(i64 n, double *x, double *r, 
       dv_[j] = ld2(dg, e, n, al);
     }
   }
+  double a = s_a;  // without Pw the p.w step ran in a finalize (the stop flag covers its breakdown)
   if (Pw) {
     fold_partials_k<1>(Pw, T, res);
     if (threadIdx.x == 0) {
       const double dpi = res[0];
-      brk = dpi == 0.0 || (s->i > 0 && sign_real(dpi) * sign_real(s->dpiold) < 0);
-      sa = s->beta / dpi;
-      if (blockIdx.x == 0) ksp_epilogue(s, EPI_CG_DPI, 0, res);
+      brk = stopped || dpi == 0.0 || (si > 0 && sign_real(dpi) * sign_real(dpiold) < 0);
+      sa = sbeta / dpi;
+      if (!stopped && blockIdx.x == 0) ksp_epilogue(s, EPI_CG_DPI, 0, res);
     }
     __syncthreads();
     if (brk) return;
-  } else if (threadIdx.x == 0) {
-    sa = s->a;  // the p.w step ran in a finalize (the stop flag covers its breakdown)
+    a = sa;
+  } else if (stopped) {
+    return;
   }
-  if (!Pw) __syncthreads();
-  const double a = sa;
   double lz[4], lr[4], lzr[4];
 #pragma unroll
   for (int j = 0; j < 4; j++) {
@@ -649,13 +675,19 @@ __global__ void __launch_bounds__(MPK_NT) k_cg_xrz(i64 n, double *x, double *r, 
 // queued the iterations.  The block of tile 0 also runs the step on the
 // state.  Unless the solve ends there, p = z + bcoef p over the block's tile
 // with OpCgP's arithmetic (VecAYPX, ported.mojo:285-294).
+template <bool al>  // p and z 16-byte aligned
 __global__ void __launch_bounds__(MPK_NT) k_cg_pz(i64 n, double *p, const double *z, const double *Pn, int T,
-                                                  KspState *s, i64 i, bool al, const int64_t *stop) {
-  if (stop && *stop) return;
+                                                  KspState *s, i64 i, const int64_t *stop) {
   __shared__ double res[3];
   __shared__ double sb;
   __shared__ int brk;
-  // the tile's operands are loaded first, so their latency overlaps the fold
+  // every load is issued before any is waited on: the stop flag, the state
+  // the norm step's decisions read (none of it changes in that step), the
+  // tile's operands, then the partials; a stopped solve returns before
+  // anything is written
+  const bool stopped = *stop != 0;  // never null here: the solve's own flag
+  const ConvView cv = ConvView::load(s);
+  const double betaold = s->betaold;
   const i64 base = (i64)blockIdx.x * MPK_TILE + 2 * (i64)threadIdx.x;
   D2 zv_[4], pv_[4];
 #pragma unroll
@@ -669,11 +701,11 @@ __global__ void __launch_bounds__(MPK_NT) k_cg_pz(i64 n, double *p, const double
   fold_partials_k<3>(Pn, T, res);
   if (threadIdx.x == 0) {
     const double beta = res[2];
-    bool end = conv_check(s, i + 1, cg_norm(s, i, res)) != 0 || i + 1 >= s->max_it;
-    end = end || beta == 0.0 || beta * s->betaold < 0.0;
-    brk = end;
-    sb = end ? 0.0 : beta / s->betaold;
-    if (blockIdx.x == 0) ksp_epilogue(s, EPI_CG_NORM, 0, res);
+    bool end = conv_check(&cv, i + 1, cg_norm(&cv, i, res)) != 0 || i + 1 >= cv.max_it;
+    end = end || beta == 0.0 || beta * betaold < 0.0;
+    brk = stopped || end;
+    sb = end ? 0.0 : beta / betaold;
+    if (!stopped && blockIdx.x == 0) ksp_epilogue(s, EPI_CG_NORM, 0, res);
   }
   __syncthreads();
   if (brk) return;
@@ -1135,8 +1167,11 @@ void mp_ksp_solve_cg(int64_t n, int64_t irow, int64_t icol, int64_t data, int64_
         const bool alp = aligned16(P) && aligned16(Z);
         if (it == 0)
           ew2(n, OpCgP{P, Z, S.d, alp}, st);
+        else if (alp)
+          MPK_LAUNCH(k_cg_pz<true>, nt, MPK_NT, n, P, (const double *)Z, (const double *)Pn, (int)nt, S.d, it - 1,
+                     st);
         else
-          MPK_LAUNCH(k_cg_pz, nt, MPK_NT, n, P, (const double *)Z, (const double *)Pn, (int)nt, S.d, it - 1, alp,
+          MPK_LAUNCH(k_cg_pz<false>, nt, MPK_NT, n, P, (const double *)Z, (const double *)Pn, (int)nt, S.d, it - 1,
                      st);
         if (sell_nt(n))
           MPK_LAUNCH(k_cg_wdot<true>, nt, 4 * MPK_NT, n, S.A.rp, S.A.sell_off, S.A.sell_v, S.A.sell_c,
@@ -1146,8 +1181,12 @@ void mp_ksp_solve_cg(int64_t n, int64_t irow, int64_t icol, int64_t data, int64_
                      (const double *)P, W, Pw, st);
         const bool al = aligned16(X) && aligned16(R) && aligned16(P) && aligned16(W) && aligned16(Z) &&
                         aligned16(S.pc.jac_diag);
-        MPK_LAUNCH(k_cg_xrz, nt, MPK_NT, n, X, R, Z, (const double *)P, (const double *)W, S.pc.jac_diag,
-                   (const double *)Pw, (int)nt, S.d, Pn, al, st);
+        if (al)
+          MPK_LAUNCH(k_cg_xrz<true>, nt, MPK_NT, n, X, R, Z, (const double *)P, (const double *)W, S.pc.jac_diag,
+                     (const double *)Pw, (int)nt, S.d, Pn, st);
+        else
+          MPK_LAUNCH(k_cg_xrz<false>, nt, MPK_NT, n, X, R, Z, (const double *)P, (const double *)W, S.pc.jac_diag,
+                     (const double *)Pw, (int)nt, S.d, Pn, st);
         continue;
       }
       ew2(n, OpCgP{P, Z, S.d, aligned16(P) && aligned16(Z)}, st);
@@ -1166,8 +1205,12 @@ void mp_ksp_solve_cg(int64_t n, int64_t irow, int64_t icol, int64_t data, int64_
                    KspEpi{S.d, EPI_CG_DPI, 0}, st);
         const bool al = aligned16(X) && aligned16(R) && aligned16(P) && aligned16(W) && aligned16(Z) &&
                         aligned16(S.pc.jac_diag);
-        MPK_LAUNCH(k_cg_xrz, T, MPK_NT, n, X, R, Z, (const double *)P, (const double *)W, S.pc.jac_diag,
-                   (const double *)nullptr, (int)T, S.d, Pp, al, st);
+        if (al)
+          MPK_LAUNCH(k_cg_xrz<true>, T, MPK_NT, n, X, R, Z, (const double *)P, (const double *)W, S.pc.jac_diag,
+                     (const double *)nullptr, (int)T, S.d, Pp, st);
+        else
+          MPK_LAUNCH(k_cg_xrz<false>, T, MPK_NT, n, X, R, Z, (const double *)P, (const double *)W, S.pc.jac_diag,
+                     (const double *)nullptr, (int)T, S.d, Pp, st);
         MPK_LAUNCH(k_finalize<KspEpi>, 1, MPK_NT, (const double *)Pp, (int)T, 3, (double *)nullptr,
                    KspEpi{S.d, EPI_CG_NORM, 0}, st);
         continue;

@@ -123,18 +123,31 @@ __device__ __forceinline__ void fold_partials(const double *P, int T, int K, dou
 }
 
 // fold_partials for K reductions at once, in the same order for each: every
-// thread issues all of its loads before it adds, and the K block trees share
-// one barrier, so a block that needs the results pays one round trip.
+// thread issues its loads (all of them, up to 2304 partials) before it adds,
+// and the K block trees share one barrier, so a block that needs the results
+// pays one round trip.
 template <int K>
 __device__ __forceinline__ void fold_partials_k(const double *P, int T, double *res) {
   __shared__ double sh[K][4];
   const int t = (int)threadIdx.x;
+  // thread t's sum P[t] + P[t + 256] + ... in that order; the partials are
+  // loaded kFoldRound at a time, all of a round before any is added, so that
+  // up to 256 kFoldRound partials cost one round trip
+  constexpr int kFoldRound = 8;
   double u[K];
 #pragma unroll
   for (int k = 0; k < K; k++) u[k] = t < T ? P[(i64)k * T + t] : 0.0;
-  for (int b = t + MPK_NT; b < T; b += MPK_NT) {
+  for (int b0 = t + MPK_NT; b0 < T; b0 += kFoldRound * MPK_NT) {
+    double v[kFoldRound][K];
 #pragma unroll
-    for (int k = 0; k < K; k++) u[k] = u[k] + P[(i64)k * T + b];
+    for (int m = 0; m < kFoldRound; m++)
+#pragma unroll
+      for (int k = 0; k < K; k++) v[m][k] = b0 + m * MPK_NT < T ? P[(i64)k * T + b0 + m * MPK_NT] : 0.0;
+#pragma unroll
+    for (int m = 0; m < kFoldRound; m++)
+#pragma unroll
+      for (int k = 0; k < K; k++)
+        if (b0 + m * MPK_NT < T) u[k] = u[k] + v[m][k];
   }
   const int lane = t & 63, wave = t >> 6;
 #pragma unroll
