@@ -42,11 +42,17 @@ HBM_PEAK_GBS = 8000.0
 # HBM bytes per launch of the roofline kernel, from rocprofv3 FETCH_SIZE and
 # WRITE_SIZE runs scaled by a copy kernel of known size
 # (profiles/spmv_sell_diffusion128_traffic.md), keyed by (kernel, n, nnz).
-MEASURED_TRAFFIC = {("k_spmv_sell", 2097152, 14581760): 219283289}
+MEASURED_TRAFFIC = {("k_spmv_sell", 2097152, 14581760): 190142898}
 
 
-def spmv_bytes(n: int, nnz: int) -> int:
-    return 12 * nnz + 4 * (n + 1) + 16 * n
+def spmv_bytes(n: int, nnz: int, so=None) -> int:
+    """Algorithmic bytes of one sliced product: values, columns (2 bytes an
+    entry when the sliced copy `so` holds 16-bit offsets, marked by a negative
+    last slice offset; else 4), row pointers, x and y."""
+    cb = 4
+    if so is not None and int(so.to_numpy(np.int32, (n + 63) // 64 + 1)[-1]) < 0:
+        cb = 2
+    return (8 + cb) * nnz + 4 * (n + 1) + 16 * n
 
 
 def build_system(N: int):
@@ -129,7 +135,7 @@ def spmv_only(lib, A, bv, n, nnz, reps):
         lib.mp_vec_copy(src.ptr, dst.ptr, m)
     lib.mpk_synchronize()
     _native.check()
-    print(json.dumps({"spmv_ms": ms, "spmv_bytes": spmv_bytes(n, nnz), "launches": reps,
+    print(json.dumps({"spmv_ms": ms, "spmv_bytes": spmv_bytes(n, nnz, so), "launches": reps,
                       "copy_bytes_each_way": 8 * m, "copy_launches": 10}))
     return 0
 
@@ -149,7 +155,7 @@ def spmv_past_cache(lib, N: int = 256, reps: int = 20):
     ys = bv.duplicate()
     lib.mpk_time_spmv(xs._d(), ys._dw(), d.rp.ptr, d.ci.ptr, d.v.ptr, n, so.ptr, sv.ptr, sc.ptr, 3)
     ms = lib.mpk_time_spmv(xs._d(), ys._dw(), d.rp.ptr, d.ci.ptr, d.v.ptr, n, so.ptr, sv.ptr, sc.ptr, reps)
-    b = spmv_bytes(n, nnz)
+    b = spmv_bytes(n, nnz, so)
     gbs = b / (ms * 1e-3) / 1e9
     return {"workload": f"diffusion_3d {N}^3", "n": n, "nnz": nnz, "bytes_per_launch": b, "avg_launch_ms": ms,
             "achieved": gbs, "frac": gbs / HBM_PEAK_GBS, "unit": "GB/s"}
@@ -423,7 +429,7 @@ def main() -> int:
     spmv_ms = lib.mpk_time_spmv(xa, ya, d.rp.ptr, d.ci.ptr, d.v.ptr, n, so.ptr, sv.ptr, sc.ptr, args.spmv_reps)
     row_ms = lib.mpk_time_spmv(xa, ya, d.rp.ptr, d.ci.ptr, d.v.ptr, n, 0, 0, 0, args.spmv_reps)
     _native.check()
-    sbytes = spmv_bytes(n, nnz)
+    sbytes = spmv_bytes(n, nnz, so)
     spmv_gbs = sbytes / (spmv_ms * 1e-3) / 1e9
 
     extra = {}

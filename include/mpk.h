@@ -68,8 +68,11 @@ void mpk_mat_mult_transpose_t(int64_t x, int64_t y, int64_t data, int64_t ncols,
                               int64_t tpos);
 /* No counterpart in the reference: the sliced (64-row, column-major) copy of a
  * CSR matrix that the products read with coalesced loads; same row sums bit
- * for bit.  [D] soff[ceil(nrow/64) + 1] int32, sv[padded] float64,
- * sc[padded] int32, padded = mpk_sell_size(). */
+ * for bit.  [D] soff[ceil(nrow/64) + 1] int32, sv[padded] float64, sc of
+ * 6 * padded bytes, padded = mpk_sell_size().  sc holds the int32 column of
+ * each slot, then (when every entry lies within 32767 columns of its row)
+ * the same slots as int16 offsets from the row, which the products read;
+ * soff[ceil(nrow/64)] is then -1 - padded instead of padded. */
 int64_t mpk_sell_size(int64_t irow, int64_t nrow);
 int64_t mpk_sell_build(int64_t irow, int64_t icol, int64_t data, int64_t nrow, int64_t soff, int64_t sv,
                        int64_t sc);

@@ -156,20 +156,45 @@ __device__ __forceinline__ void st_stream(T *p, T v) {
     *p = v;
 }
 
-// One row of a sliced product: entries base + 64 k, k < len, summed in order
-// onto s (fma(v, x[c], s), k = 0, 1, ...).  The values and columns of four
-// entries are loaded before their gathers of x, so each lane keeps several
-// loads in flight; the arithmetic order is the serial one.
-template <bool NT>
-__device__ __forceinline__ double sell_row(const double *__restrict__ sv, const int *__restrict__ sc,
-                                           const double *__restrict__ x, int base, int len, double s) {
+// Columns of a sliced copy.  The column array holds the int32 column of every
+// slot; when every entry lies within 32767 columns of its row it is followed
+// by the same slots as 16-bit offsets from the row, and the last slice offset
+// soff[nslices] holds -1 - (padded slots) instead of the slot count.  The
+// products then read the offsets: 2 bytes a slot instead of 4.
+struct SellCols32 {
+  const int *c;
+  template <bool NT>
+  __device__ __forceinline__ int col(int q, int) const {
+    return ld_stream<NT>(c + q);
+  }
+};
+struct SellCols16 {
+  const short *d;
+  template <bool NT>
+  __device__ __forceinline__ int col(int q, int i) const {
+    return i + (int)ld_stream<NT>(d + q);
+  }
+};
+// The 16-bit offsets of a sliced copy with nrow rows, or null (int32 columns).
+__device__ __forceinline__ const short *sell_offsets16(const int *soff, const int *sc, i64 nrow) {
+  const int last = soff[(nrow + 63) >> 6];
+  return last < 0 ? reinterpret_cast<const short *>(sc + (-1 - (i64)last)) : nullptr;
+}
+
+// One row (row i) of a sliced product: entries base + 64 k, k < len, summed in
+// order onto s (fma(v, x[c], s), k = 0, 1, ...).  The values and columns of
+// four entries are loaded before their gathers of x, so each lane keeps
+// several loads in flight; the arithmetic order is the serial one.
+template <bool NT, class Cols>
+__device__ __forceinline__ double sell_row(const double *__restrict__ sv, const Cols &cols,
+                                           const double *__restrict__ x, int base, int len, double s, i64 i) {
   for (int k = 0; k < len; k += 4) {
     int c[4];
     double v[4], xv[4];
 #pragma unroll
     for (int j = 0; j < 4; j++)
       if (k + j < len) {
-        c[j] = ld_stream<NT>(sc + base + 64 * (k + j));
+        c[j] = cols.template col<NT>(base + 64 * (k + j), (int)i);
         v[j] = ld_stream<NT>(sv + base + 64 * (k + j));
       }
 #pragma unroll

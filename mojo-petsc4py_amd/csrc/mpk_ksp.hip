@@ -500,8 +500,41 @@ struct OpCgP {
 // the pair's leaf, which goes to LDS where the tile reduction keeps it (pair
 // 512 j + 2t is thread t's j-th); the leaves are then folded in that
 // reduction's order, so the partial is k_partials'.
+template <bool NT, class Cols>
+__device__ __forceinline__ void wdot_rows(const double *__restrict__ sv, const Cols &cols,
+                                          const double *__restrict__ p, int q0, int len0, int e0, int q1, int len1,
+                                          int e1, double &w0, double &w1) {
+  const int len = len0 > len1 ? len0 : len1;
+  for (int k = 0; k < len; k += 4) {
+    int c0[4], c1[4];
+    double v0[4], v1[4], x0[4], x1[4];
+#pragma unroll
+    for (int j = 0; j < 4; j++) {
+      if (k + j < len0) {
+        c0[j] = cols.template col<NT>(q0 + 64 * (k + j), e0);
+        v0[j] = ld_stream<NT>(sv + q0 + 64 * (k + j));
+      }
+      if (k + j < len1) {
+        c1[j] = cols.template col<NT>(q1 + 64 * (k + j), e1);
+        v1[j] = ld_stream<NT>(sv + q1 + 64 * (k + j));
+      }
+    }
+#pragma unroll
+    for (int j = 0; j < 4; j++) {
+      if (k + j < len0) x0[j] = p[c0[j]];
+      if (k + j < len1) x1[j] = p[c1[j]];
+    }
+#pragma unroll
+    for (int j = 0; j < 4; j++) {
+      if (k + j < len0) w0 = fma(v0[j], x0[j], w0);
+      if (k + j < len1) w1 = fma(v1[j], x1[j], w1);
+    }
+  }
+}
+
+// At most 64 VGPRs, so that two 1024-thread blocks fit a CU (8 waves a SIMD).
 template <bool NT>
-__global__ void __launch_bounds__(4 * MPK_NT) k_cg_wdot(i64 n, const int *__restrict__ rp,
+__global__ void __launch_bounds__(4 * MPK_NT) __attribute__((amdgpu_waves_per_eu(8))) k_cg_wdot(i64 n, const int *__restrict__ rp,
                                                         const int *__restrict__ soff, const double *__restrict__ sv,
                                                         const int *__restrict__ sc, const double *__restrict__ p,
                                                         double *__restrict__ w, double *P, const int64_t *stop) {
@@ -517,37 +550,17 @@ __global__ void __launch_bounds__(4 * MPK_NT) k_cg_wdot(i64 n, const int *__rest
   const i64 c0 = e0 < n ? e0 : n - 1, c1 = e1 < n ? e1 : n - 1;
   const int r00 = rp[c0], r01 = rp[c0 + 1], r10 = rp[c1], r11 = rp[c1 + 1];
   const int o0 = soff[c0 >> 6], o1 = soff[c1 >> 6];
+  const short *sd = sell_offsets16(soff, sc, n);
   const int len0 = e0 < n ? r01 - r00 : 0, len1 = e1 < n ? r11 - r10 : 0;
   const int q0 = e0 < n ? o0 + (int)(e0 & 63) : 0, q1 = e1 < n ? o1 + (int)(e1 & 63) : 0;
   if (stopped) return;
-  const int len = len0 > len1 ? len0 : len1;
-  // both rows' sums in order (sell_row's), four entries of each in flight
+  // both rows' sums in order (sell_row's), four entries of each in flight,
+  // over the 16-bit column offsets when the copy has them
   double w0 = 0.0, w1 = 0.0;
-  for (int k = 0; k < len; k += 4) {
-    int c0[4], c1[4];
-    double v0[4], v1[4], x0[4], x1[4];
-#pragma unroll
-    for (int j = 0; j < 4; j++) {
-      if (k + j < len0) {
-        c0[j] = ld_stream<NT>(sc + q0 + 64 * (k + j));
-        v0[j] = ld_stream<NT>(sv + q0 + 64 * (k + j));
-      }
-      if (k + j < len1) {
-        c1[j] = ld_stream<NT>(sc + q1 + 64 * (k + j));
-        v1[j] = ld_stream<NT>(sv + q1 + 64 * (k + j));
-      }
-    }
-#pragma unroll
-    for (int j = 0; j < 4; j++) {
-      if (k + j < len0) x0[j] = p[c0[j]];
-      if (k + j < len1) x1[j] = p[c1[j]];
-    }
-#pragma unroll
-    for (int j = 0; j < 4; j++) {
-      if (k + j < len0) w0 = fma(v0[j], x0[j], w0);
-      if (k + j < len1) w1 = fma(v1[j], x1[j], w1);
-    }
-  }
+  if (sd)
+    wdot_rows<NT>(sv, SellCols16{sd}, p, q0, len0, (int)e0, q1, len1, (int)e1, w0, w1);
+  else
+    wdot_rows<NT>(sv, SellCols32{sc}, p, q0, len0, (int)e0, q1, len1, (int)e1, w0, w1);
   double p0 = 0.0, p1 = 0.0;
   if (e0 < n) {
     st_stream<NT>(w + e0, w0);

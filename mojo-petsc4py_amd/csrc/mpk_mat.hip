@@ -54,9 +54,12 @@ __global__ void __launch_bounds__(256) k_spmv_sell(i64 nrow, const int *__restri
   const unsigned b = xcd_block(blockIdx.x, gridDim.x);
   const i64 i = (i64)b * 256 + threadIdx.x;
   if (i >= nrow) return;
+  const short *sd = sell_offsets16(soff, sc, nrow);
   const int len = rp[i + 1] - rp[i];
   const int base = soff[i >> 6] + (int)(i & 63);
-  st_stream<NT>(y + i, sell_row<NT>(sv, sc, x, base, len, add ? y[i] : 0.0));
+  const double s0 = add ? y[i] : 0.0;
+  st_stream<NT>(y + i, sd ? sell_row<NT>(sv, SellCols16{sd}, x, base, len, s0, i)
+                          : sell_row<NT>(sv, SellCols32{sc}, x, base, len, s0, i));
 }
 
 // scatter CSR into the slice layout (one lane per row)
@@ -70,6 +73,58 @@ __global__ void k_sell_fill(i64 nrow, const int *rp, const int *ci, const double
     sv[base + 64 * k] = v[j0 + k];
     sc[base + 64 * k] = ci[j0 + k];
   }
+}
+
+// The 16-bit column offsets of the slots (after the int32 columns, at sc +
+// padded slots; the same slot positions).
+__global__ void k_sell_fill16(i64 nrow, const int *rp, const int *ci, const int *soff, short *sd) {
+  i64 i = (i64)blockIdx.x * 256 + threadIdx.x;
+  if (i >= nrow) return;
+  const int j0 = rp[i], len = rp[i + 1] - j0;
+  const int base = soff[i >> 6] + (int)(i & 63);
+  for (int k = 0; k < len; k++) sd[base + 64 * k] = (short)(ci[j0 + k] - (int)i);
+}
+
+// The largest |column - row| over a CSR matrix's entries, into *out (which
+// starts at 0): a wave's maximum, then one atomic per wave.
+__global__ void k_csr_span(i64 nrow, const int *rp, const int *ci, int *out) {
+  const i64 i = (i64)blockIdx.x * 256 + threadIdx.x;
+  long long m = 0;
+  if (i < nrow)
+    for (int j = rp[i]; j < rp[i + 1]; j++) {
+      const long long d = (long long)ci[j] - i;
+      m = d < 0 ? (-d > m ? -d : m) : (d > m ? d : m);
+    }
+  int mi = m > 0x7fffffff ? 0x7fffffff : (int)m;
+#pragma unroll
+  for (int w = 32; w >= 1; w >>= 1) {
+    const int o = __shfl_down(mi, w, 64);
+    mi = o > mi ? o : mi;
+  }
+  if ((threadIdx.x & 63) == 0) atomicMax(out, mi);
+}
+
+// Fills a sliced copy (soff: the host offsets, copied to soff_d; sv; sc with
+// room for 6 bytes a slot): values and int32 columns, then, when every entry
+// lies within 32767 columns of its row, the 16-bit offsets, marked in the
+// last offset.  False when a launch failed.
+static bool sell_fill(const int *rp, const int *ci, const double *v, i64 nrow, std::vector<int> &soff,
+                      int *soff_d, double *sv, int *sc) {
+  const i64 ns = (nrow + 63) / 64, total = soff[ns];
+  int *span = reinterpret_cast<int *>(ctx()->dres);
+  const int zero = 0;
+  copy_htod(span, &zero, sizeof zero);
+  MPK_LAUNCH(k_csr_span, nblocks(nrow, 256), 256, nrow, rp, ci, span);
+  int width = 0x7fffffff;
+  copy_dtoh(&width, span, sizeof width);
+  if (failed()) return false;
+  if (width <= 32767) soff[ns] = (int)(-1 - total);
+  copy_htod(soff_d, soff.data(), soff.size() * sizeof(int));
+  MPK_LAUNCH(k_sell_fill, nblocks(nrow, 256), 256, nrow, rp, ci, v, (const int *)soff_d, sv, sc);
+  if (width <= 32767)
+    MPK_LAUNCH(k_sell_fill16, nblocks(nrow, 256), 256, nrow, rp, ci, (const int *)soff_d,
+               reinterpret_cast<short *>(sc + total));
+  return !failed();
 }
 
 // slice offsets from the row pointers; returns the padded entry count
@@ -117,10 +172,8 @@ SellTemp::SellTemp(DevCSR &A) {
   if (total >= (i64)1 << 31) return;  // int32 offsets: keep the row kernel
   HIPCHK(hipMalloc(&off, soff.size() * sizeof(int)));
   HIPCHK(hipMalloc(&v, (total > 0 ? total : 1) * sizeof(double)));
-  HIPCHK(hipMalloc(&c, (total > 0 ? total : 1) * sizeof(int)));
-  copy_htod(off, soff.data(), soff.size() * sizeof(int));
-  MPK_LAUNCH(k_sell_fill, nblocks(A.nrow, 256), 256, A.nrow, A.rp, A.ci, A.v, (const int *)off, v, c);
-  if (failed()) return;
+  HIPCHK(hipMalloc(&c, (total > 0 ? 6 * total : 4)));  // int32 columns, then 16-bit offsets
+  if (failed() || !sell_fill(A.rp, A.ci, A.v, A.nrow, soff, off, v, c)) return;
   A.sell_off = off;
   A.sell_v = v;
   A.sell_c = c;
@@ -329,10 +382,10 @@ int64_t mpk_sell_build(int64_t irow, int64_t icol, int64_t data, int64_t nrow, i
   if (nrow <= 0 || !ctx()) return 0;
   std::vector<int> off;
   sell_offsets((const int *)irow, nrow, off);
-  copy_htod((void *)soff, off.data(), off.size() * sizeof(int));
-  MPK_LAUNCH(k_sell_fill, nblocks(nrow, 256), 256, (i64)nrow, (const int *)irow, (const int *)icol,
-             (const double *)data, (const int *)soff, (double *)sv, (int *)sc);
-  return failed() ? -1 : 0;
+  return sell_fill((const int *)irow, (const int *)icol, (const double *)data, nrow, off, (int *)soff,
+                   (double *)sv, (int *)sc)
+             ? 0
+             : -1;
 }
 
 // y = A x (add = 0) or y = y + A x (add = 1) over the sliced copy.

@@ -894,7 +894,7 @@ int64_t mpk_mixed_create(int64_t n, int64_t irow, int64_t icol, int64_t data, in
   // low-precision matrix values in the sliced slots
   std::vector<int> so(((size_t)n + 63) / 64 + 1);
   copy_dtoh(so.data(), (const void *)soff, so.size() * sizeof(int));
-  const i64 slots = so.back() > 0 ? so.back() : 1, esz = prec == 16 ? 2 : 4;
+  const i64 total = sell_slots(so.back()), slots = total > 0 ? total : 1, esz = prec == 16 ? 2 : 4;
   HIPCHK(hipMalloc(&S->lv, slots * esz));
   if (prec == 16) {
     HIPCHK(hipMalloc(&S->rs, n * sizeof(float)));

@@ -107,7 +107,7 @@ class _DevCSR:
             check()
             soff = DeviceBuffer(4 * ((self.nrows + 63) // 64 + 1))
             sv = DeviceBuffer(8 * max(padded, 1))
-            sc = DeviceBuffer(4 * max(padded, 1))
+            sc = DeviceBuffer(6 * max(padded, 1))  # int32 columns, then int16 offsets
             if L.mpk_sell_build(self.rp.ptr, self.ci.ptr, self.v.ptr, self.nrows, soff.ptr, sv.ptr, sc.ptr) != 0:
                 check()
             self.sellc = (soff, sv, sc)
@@ -134,7 +134,7 @@ class _DevCSR:
             check()
             soff = DeviceBuffer(4 * ((nc + 63) // 64 + 1))
             sv = DeviceBuffer(8 * max(padded, 1))
-            sc = DeviceBuffer(4 * max(padded, 1))
+            sc = DeviceBuffer(6 * max(padded, 1))  # int32 columns, then int16 offsets
             if L.mpk_sell_build(trp_d.ptr, trow_d.ptr, vt_d.ptr, nc, soff.ptr, sv.ptr, sc.ptr) != 0:
                 check()
             L.mpk_synchronize()

@@ -38,3 +38,18 @@ class DevMat:
         self.rp = dev(np.asarray(indptr, dtype=np.int32))
         self.ci = dev(np.asarray(indices[:max(nnz, 1)], dtype=np.int32) if nnz else np.zeros(1, np.int32))
         self.v = dev(np.asarray(data[:max(nnz, 1)], dtype=np.float64) if nnz else np.zeros(1))
+
+
+def wide_band(n=40000, far=35000):
+    """SPD, diagonally dominant: 4 on the diagonal, -1 at distance 1 and -0.5
+    at distance `far` (so entries lie up to `far` columns from their row:
+    past 32767, where sliced copies keep int32 columns)."""
+    from mpk import workloads as W
+    rows, cols, vals = [], [], []
+    for d, v in ((0, 4.0), (1, -1.0), (-1, -1.0), (far, -0.5), (-far, -0.5)):
+        i = np.arange(max(0, -d), min(n, n - d))
+        rows.append(i)
+        cols.append(i + d)
+        vals.append(np.full(len(i), v))
+    return W._csr_from_coo(n, np.concatenate(rows).astype(np.int64), np.concatenate(cols).astype(np.int64),
+                           np.concatenate(vals))

@@ -5,7 +5,7 @@ bit for bit.  Driven through the PETSc surface, which calls mp_ksp_solve_*."""
 import numpy as np
 import pytest
 
-from _helpers import same_bits
+from _helpers import same_bits, wide_band
 from mpk import workloads as W
 
 pytestmark = pytest.mark.gpu
@@ -195,6 +195,21 @@ def test_cg_jacobi_finalize_steps(native, O, monkeypatch, normtype):
     b[3] = np.nan
     g = _compare(O, DIFF16, b, "cg", "jacobi", rtol=1e-10, max_it=50)
     assert g[2] == -9
+
+
+WIDE = wide_band()
+WIDE_B = np.random.default_rng(9).normal(size=40000)
+
+
+@pytest.mark.parametrize("fold", ["default", "0"])
+def test_cg_jacobi_int32_columns(native, O, monkeypatch, fold):
+    """CG + Jacobi on a matrix whose entries lie 35000 columns from their row,
+    so its sliced copy keeps int32 columns (the other systems here use 16-bit
+    offsets): the oracle's bits, in the folded and the finalize form."""
+    if fold != "default":
+        monkeypatch.setenv("MPK_CG_FOLD_TILES", fold)
+    g = _compare(O, WIDE, WIDE_B, "cg", "jacobi", rtol=1e-12, max_it=500)
+    assert g[2] == 2
 
 
 def test_cg_jacobi_nontemporal_product(native, O, monkeypatch):

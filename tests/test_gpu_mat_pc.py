@@ -4,7 +4,7 @@ GPU against the oracle, bit for bit, through the C ABI."""
 import numpy as np
 import pytest
 
-from _helpers import DevMat, back, dev, rng_vec, same_bits
+from _helpers import DevMat, back, dev, rng_vec, same_bits, wide_band
 from mpk import workloads as W
 
 pytestmark = pytest.mark.gpu
@@ -56,6 +56,7 @@ def _ragged(n, seed):
 
 SELL_MATS = dict(MATS)
 SELL_MATS.update({f"ragged{n}": (lambda n=n: _ragged(n, n)) for n in (1, 63, 64, 65, 130, 1000)})
+SELL_MATS["wide40000"] = wide_band  # entries 35000 columns from their row: int32 columns
 
 
 @pytest.mark.parametrize("nt", ["default", "1"])
@@ -64,7 +65,10 @@ def test_sliced_spmv_bitwise(native, O, monkeypatch, name, nt):
     """The sliced (64-row, column-major) copy gives the CSR row sums bit for
     bit, for y = A x and y = y + A x, including empty rows and partial slices;
     also in the non-temporal form (the default from 3,000,000 rows, forced
-    here by MPK_SELL_NT=1)."""
+    here by MPK_SELL_NT=1).  The copy holds the int32 columns, and 16-bit
+    offsets from the row when every entry is within 32767 columns of its row
+    (marked by -1 - padded in the last slice offset); the products read the
+    offsets then, and the columns otherwise (wide40000)."""
     if nt != "default":
         monkeypatch.setenv("MPK_SELL_NT", nt)
     ip, ix, va = SELL_MATS[name]()
@@ -76,8 +80,19 @@ def test_sliced_spmv_bitwise(native, O, monkeypatch, name, nt):
     assert padded == 64 * sum(widths)
     so = dev(np.zeros((n + 63) // 64 + 1, np.int32))
     sv = dev(np.zeros(max(padded, 1)))
-    sc = dev(np.zeros(max(padded, 1), np.int32))
+    sc = dev(np.zeros(3 * max(padded, 1) // 2 + 1, np.int32))  # 6 bytes a slot
     assert native.mpk_sell_build(D.rp.ptr, D.ci.ptr, D.v.ptr, n, so.ptr, sv.ptr, sc.ptr) == 0
+    span = max((abs(int(c) - i) for i in range(n) for c in ix[ip[i]:ip[i + 1]]), default=0)
+    offs = back(so, np.int32, (n + 63) // 64 + 1)
+    assert offs[-1] == (-1 - padded if span <= 32767 else padded)
+    raw = back(sc, np.int32, 3 * max(padded, 1) // 2 + 1)
+    cols, d16 = raw[:padded], raw.view(np.int16)[2 * padded:3 * padded]
+    for i in range(0, n, max(1, n // 97)):  # the slots of a spread of rows
+        for k in range(ip[i + 1] - ip[i]):
+            q = int(offs[i // 64]) + i % 64 + 64 * k
+            assert cols[q] == ix[ip[i] + k]
+            if span <= 32767:
+                assert int(d16[q]) == int(ix[ip[i] + k]) - i
     x = rng_vec(n, 1)
     y0 = rng_vec(n, 2)
     dx, dy = dev(x), dev(y0)
