@@ -118,6 +118,8 @@ static bool sell_fill(const int *rp, const int *ci, const double *v, i64 nrow, s
   int width = 0x7fffffff;
   copy_dtoh(&width, span, sizeof width);
   if (failed()) return false;
+  const char *cols = std::getenv("MPK_SELL_COLS");  // "32": int32 columns only (comparisons)
+  if (cols && cols[0] == '3' && cols[1] == '2' && !cols[2]) width = 0x7fffffff;
   if (width <= 32767) soff[ns] = (int)(-1 - total);
   copy_htod(soff_d, soff.data(), soff.size() * sizeof(int));
   MPK_LAUNCH(k_sell_fill, nblocks(nrow, 256), 256, nrow, rp, ci, v, (const int *)soff_d, sv, sc);

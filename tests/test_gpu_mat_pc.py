@@ -106,6 +106,33 @@ def test_sliced_spmv_bitwise(native, O, monkeypatch, name, nt):
     assert same_bits(back(dy, np.float64, n), h)
 
 
+def test_sliced_copy_int32_columns_on_request(native, O, monkeypatch):
+    """MPK_SELL_COLS=32 at build keeps int32 columns only (the form the bench
+    compares with): the last slice offset holds the slot count, and the row
+    sums are the same bits."""
+    ip, ix, va = W.diffusion_3d(12)[0]
+    A = O.CSR(ip, ix, va)
+    D = DevMat(ip, ix, va)
+    n = A.n
+    padded = native.mpk_sell_size(D.rp.ptr, n)
+    x = rng_vec(n, 1)
+    ys = []
+    for cols in ("32", None):
+        if cols:
+            monkeypatch.setenv("MPK_SELL_COLS", cols)
+        else:
+            monkeypatch.delenv("MPK_SELL_COLS", raising=False)
+        so = dev(np.zeros((n + 63) // 64 + 1, np.int32))
+        sv = dev(np.zeros(max(padded, 1)))
+        sc = dev(np.zeros(3 * max(padded, 1) // 2 + 1, np.int32))
+        assert native.mpk_sell_build(D.rp.ptr, D.ci.ptr, D.v.ptr, n, so.ptr, sv.ptr, sc.ptr) == 0
+        assert back(so, np.int32, (n + 63) // 64 + 1)[-1] == (padded if cols else -1 - padded)
+        dx, dy = dev(x), dev(np.zeros(n))
+        native.mpk_sell_mult(dx.ptr, dy.ptr, D.rp.ptr, so.ptr, sv.ptr, sc.ptr, n, 0)
+        ys.append(back(dy, np.float64, n))
+    assert same_bits(ys[0], A.mult(x)) and same_bits(ys[1], A.mult(x))
+
+
 @pytest.mark.parametrize("name", sorted(MATS))
 def test_spmv_family_bitwise(native, O, name):
     ip, ix, va = MATS[name]()
