@@ -45,6 +45,18 @@ void mpk_memcpy_dtoh(int64_t dst, int64_t src, int64_t bytes);  /* synchronous *
 void mpk_memcpy_dtod(int64_t dst, int64_t src, int64_t bytes);  /* queued */
 void mpk_memset(int64_t dst, int64_t value, int64_t bytes);     /* queued */
 void mpk_synchronize(void);
+/* The order of every floating-point reduction (dot, norms, sums, MDot) that
+ * the mp_* calls and the solvers compute.  0 (the default): a fixed
+ * tile-tree order, the same bits on every run.  1: the reference's SIMD
+ * order (src/simdvec.mojo:332-473 with SW = 4, BLK = 16, src/simd.mojo:18,
+ * 32-33; _mdot_basis, src/ksp.mojo:956-1062), computed by one workgroup as
+ * 16 serial chains per reduction -- the reference's bits, at the cost of a
+ * dependency chain of n / 16 steps per reduction.  The reference itself has
+ * no such switch: its order is the one its build host's SIMD width gives.
+ * MPK_REDUCTION=ref|dev sets the initial value.  Returns the previous order;
+ * any other value is an error (status 4) and leaves the order unchanged. */
+int64_t mpk_set_reduction_order(int64_t order);
+int64_t mpk_get_reduction_order(void);
 /* average ms of `reps` SpMV launches (over the sliced copy when soff != 0),
  * HIP events on the library stream */
 double mpk_time_spmv(int64_t x, int64_t y, int64_t irow, int64_t icol, int64_t data, int64_t nrow, int64_t soff,
@@ -72,8 +84,13 @@ void mpk_mat_mult_transpose_t(int64_t x, int64_t y, int64_t data, int64_t ncols,
  * 6 * padded bytes, padded = mpk_sell_size().  sc holds the int32 column of
  * each slot, then (when every entry lies within 32767 columns of its row)
  * the same slots as int16 offsets from the row, which the products read;
- * soff[ceil(nrow/64)] is then -1 - padded instead of padded. */
+ * soff[ceil(nrow/64)] is then -1 - padded instead of padded.  Offsets and
+ * slot indices are int32: mpk_sell_build refuses (status 4, returns -1) a
+ * copy of more than 2^31 - 1 padded slots, which the row kernel then
+ * multiplies.  mpk_sell_size_host: the padded count from [H] int64 row
+ * pointers, or -1 past that limit; needs no device. */
 int64_t mpk_sell_size(int64_t irow, int64_t nrow);
+int64_t mpk_sell_size_host(int64_t irow, int64_t nrow);
 int64_t mpk_sell_build(int64_t irow, int64_t icol, int64_t data, int64_t nrow, int64_t soff, int64_t sv,
                        int64_t sc);
 void mpk_sell_mult(int64_t x, int64_t y, int64_t irow, int64_t soff, int64_t sv, int64_t sc, int64_t nrow,

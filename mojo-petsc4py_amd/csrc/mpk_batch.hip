@@ -560,7 +560,8 @@ __global__ void k_batch_reset(unsigned *q) { *q = 0u; }
 struct Batch {
   BatchArgs a{};
   i64 nmax = 0;
-  int bs = 256, grid = 0;
+  int grid = 0;  // the workgroups the work buffers are sized for
+  int cap_bs = 0, cap_grid = 0;  // the launch shape the graph was captured with
   unsigned *queue = nullptr;
   float *work = nullptr;
   double *work64 = nullptr;
@@ -684,7 +685,7 @@ int64_t mpk_batch_solve(int64_t h, int64_t b, int64_t x, int64_t stats, int64_t 
   if (!B->size_work(g)) return -1;
   if (graph) {
     // re-capture when the launch shape or any argument changes
-    if (B->exec && (B->bs != bs || B->grid != g || std::memcmp(&B->captured, &a, sizeof a) != 0)) {
+    if (B->exec && (B->cap_bs != bs || B->cap_grid != g || std::memcmp(&B->captured, &a, sizeof a) != 0)) {
       HIPCHK(hipGraphExecDestroy(B->exec));
       HIPCHK(hipGraphDestroy(B->graph));
       B->exec = nullptr;
@@ -696,7 +697,8 @@ int64_t mpk_batch_solve(int64_t h, int64_t b, int64_t x, int64_t stats, int64_t 
       HIPCHK(hipStreamEndCapture(stream(), &B->graph));
       if (failed()) return -1;
       HIPCHK(hipGraphInstantiate(&B->exec, B->graph, nullptr, nullptr, 0));
-      B->bs = bs;
+      B->cap_bs = bs;
+      B->cap_grid = g;
       B->captured = a;
     }
     HIPCHK(hipGraphLaunch(B->exec, stream()));

@@ -64,6 +64,11 @@ inline unsigned ntiles(i64 n) { return nblocks(n, MPK_TILE); }
 // Synchronise the stream and record any asynchronous error.
 void sync();
 
+// Reduction order: false (the default) for the tile-tree order below, true
+// for the reference's SIMD order (mpk_reduce.h).  Set by
+// mpk_set_reduction_order(); MPK_REDUCTION=ref|dev gives the initial value.
+bool red_ref();
+
 }  // namespace mpk
 
 #define HIPCHK(x)                                                                   \

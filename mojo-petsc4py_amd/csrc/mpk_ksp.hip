@@ -1168,7 +1168,10 @@ void mp_ksp_solve_cg(int64_t n, int64_t irow, int64_t icol, int64_t data, int64_
   // and norm steps folded into the kernels that use their results.  Every
   // block folding every partial costs blocks x partials loads, so past
   // cg_fold_tiles() tiles the steps run in finalizes instead (five launches).
-  const bool jac = S.A.sell_off && n > 0 && S.pc.type == PCT_JACOBI;
+  // (the fused kernels reduce in the tile-tree order: the reference's order
+  // runs the product, the updates and the PC apply as separate launches)
+  const bool ref = red_ref();
+  const bool jac = !ref && S.A.sell_off && n > 0 && S.pc.type == PCT_JACOBI;
   const bool fused = jac && ntiles(n) <= cg_fold_tiles();
   const unsigned nt = fused ? ntiles(n) : 0;
   double *Pw = fused ? partials((size_t)4 * nt) : nullptr, *Pn = fused ? Pw + nt : nullptr;
@@ -1228,7 +1231,7 @@ void mp_ksp_solve_cg(int64_t n, int64_t irow, int64_t icol, int64_t data, int64_
                    KspEpi{S.d, EPI_CG_NORM, 0}, st);
         continue;
       }
-      if (S.A.sell_off && n > 0) {
+      if (!ref && S.A.sell_off && n > 0) {
         // the product and its reduction in one pass (k_cg_wdot)
         const unsigned T = ntiles(n);
         double *Pp = partials(T);
@@ -1330,16 +1333,26 @@ static void orthogonalize(Solve &S, double *vv, i64 it, double *vnew) {
   }
   // CGS: MDot, negate, MAXPY, subtract; refine when asked
   const int nv = (int)(it + 1);
-  unsigned T = ntiles(n);
+  const bool ref = red_ref();
+  // the reference's order: one partial per sum, from k_ref_mdot
+  unsigned T = ref ? 1u : ntiles(n);
   for (int pass = 0; pass < 2; pass++) {
     const int64_t *stp = pass == 0 ? inner : S.skipref;
-    double *Pp = partials((size_t)nv * T);
-    for (int g0 = 0; g0 < nv; g0 += 8) {
-      int g = nv - g0 < 8 ? nv - g0 : 8;
-      MPK_LAUNCH(k_partials_mdot<8>, T, MPK_NT, (const double *)vnew, (const double *)(vv + (i64)g0 * n), (i64)n,
-                 g, (i64)n, Pp + (size_t)g0 * T, stp);
+    // past MPK_FIN_LDS sums the finalize folds into the nv doubles after the partials
+    double *Pp = partials((size_t)nv * T + (size_t)nv);
+    if (!Pp) return;
+    double *fout = nv > MPK_FIN_LDS ? Pp + (size_t)nv * T : nullptr;
+    if (ref) {
+      MPK_LAUNCH(k_ref_mdot, (nv + 2) / 3, MPK_REF_NT, (const double *)vnew, (const double *)vv, (i64)n, nv,
+                 (i64)n, Pp, stp);
+    } else {
+      for (int g0 = 0; g0 < nv; g0 += 8) {
+        int g = nv - g0 < 8 ? nv - g0 : 8;
+        MPK_LAUNCH(k_partials_mdot<8>, T, MPK_NT, (const double *)vnew, (const double *)(vv + (i64)g0 * n),
+                   (i64)n, g, (i64)n, Pp + (size_t)g0 * T, stp);
+      }
     }
-    MPK_LAUNCH(k_finalize<KspEpi>, 1, MPK_NT, (const double *)Pp, (int)T, nv, (double *)nullptr,
+    MPK_LAUNCH(k_finalize<KspEpi>, 1, MPK_NT, (const double *)Pp, (int)T, nv, fout,
                KspEpi{S.d, pass == 0 ? EPI_GM_CGS_A : EPI_GM_CGS_B, 0}, stp);
     vmaxpy_dev(vnew, vv, n, S.h.LHH, nv, 1.0, n, false, stp);
     if (pass == 0 && S.h.cgstype == 1)

@@ -129,21 +129,29 @@ static bool sell_fill(const int *rp, const int *ci, const double *v, i64 nrow, s
   return !failed();
 }
 
-// slice offsets from the row pointers; returns the padded entry count
-i64 sell_offsets(const int *rp_d, i64 nrow, std::vector<int> &soff) {
-  std::vector<int> rp(nrow + 1);
-  copy_dtoh(rp.data(), rp_d, (nrow + 1) * sizeof(int));
+// slice offsets from the row pointers; returns the padded entry count.  The
+// offsets and every slot index are int32 (the last offset also encodes the
+// column form as -1 - count), so past INT32_MAX padded slots the offsets are
+// not filled in: callers check the count (sell_fits) and keep the row kernel.
+template <class Ix>
+static i64 sell_offsets_from(const Ix *rp, i64 nrow, std::vector<int> &soff) {
   i64 ns = (nrow + 63) / 64;
   soff.assign(ns + 1, 0);
   i64 total = 0;
   for (i64 s = 0; s < ns; s++) {
-    int w = 0;
-    for (i64 i = s * 64; i < nrow && i < s * 64 + 64; i++) w = rp[i + 1] - rp[i] > w ? rp[i + 1] - rp[i] : w;
-    soff[s] = (int)total;
-    total += 64 * (i64)w;
+    i64 w = 0;
+    for (i64 i = s * 64; i < nrow && i < s * 64 + 64; i++)
+      w = (i64)rp[i + 1] - (i64)rp[i] > w ? (i64)rp[i + 1] - (i64)rp[i] : w;
+    if (sell_fits(total)) soff[s] = (int)total;
+    total += 64 * w;
   }
-  soff[ns] = (int)total;
+  if (sell_fits(total)) soff[ns] = (int)total;
   return total;
+}
+i64 sell_offsets(const int *rp_d, i64 nrow, std::vector<int> &soff) {
+  std::vector<int> rp(nrow + 1);
+  copy_dtoh(rp.data(), rp_d, (nrow + 1) * sizeof(int));
+  return sell_offsets_from(rp.data(), nrow, soff);
 }
 
 bool sell_nt(i64 nrow) {
@@ -171,7 +179,7 @@ SellTemp::SellTemp(DevCSR &A) {
   if (A.nrow <= 0 || A.sell_off || !ctx()) return;
   std::vector<int> soff;
   i64 total = sell_offsets(A.rp, A.nrow, soff);
-  if (total >= (i64)1 << 31) return;  // int32 offsets: keep the row kernel
+  if (!sell_fits(total)) return;  // int32 offsets: keep the row kernel
   HIPCHK(hipMalloc(&off, soff.size() * sizeof(int)));
   HIPCHK(hipMalloc(&v, (total > 0 ? total : 1) * sizeof(double)));
   HIPCHK(hipMalloc(&c, (total > 0 ? 6 * total : 4)));  // int32 columns, then 16-bit offsets
@@ -379,11 +387,23 @@ int64_t mpk_sell_size(int64_t irow, int64_t nrow) {
   return sell_offsets((const int *)irow, nrow, soff);
 }
 
+int64_t mpk_sell_size_host(int64_t irow, int64_t nrow) {
+  if (nrow <= 0) return 0;
+  std::vector<int> soff;
+  const i64 total = sell_offsets_from((const int64_t *)irow, nrow, soff);
+  return sell_fits(total) ? total : -1;
+}
+
 int64_t mpk_sell_build(int64_t irow, int64_t icol, int64_t data, int64_t nrow, int64_t soff, int64_t sv,
                        int64_t sc) {
   if (nrow <= 0 || !ctx()) return 0;
   std::vector<int> off;
-  sell_offsets((const int *)irow, nrow, off);
+  const i64 total = sell_offsets((const int *)irow, nrow, off);
+  if (!sell_fits(total)) {
+    set_error(4, "sliced copy of %lld padded slots: past the int32 slot indices (at most %lld)", (long long)total,
+              (long long)kSellMaxSlots);
+    return -1;
+  }
   return sell_fill((const int *)irow, (const int *)icol, (const double *)data, nrow, off, (int *)soff,
                    (double *)sv, (int *)sc)
              ? 0

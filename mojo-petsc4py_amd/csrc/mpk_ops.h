@@ -49,6 +49,10 @@ void spmv(const DevCSR &A, const double *x, double *y, int add, const int64_t *s
 // 270 MB, past the 256 MB last-level cache); MPK_SELL_NT=0|1 overrides
 // (read at each call).
 bool sell_nt(i64 nrow);
+// The most padded slots a sliced copy can hold: its slice offsets and slot
+// indices are int32, and its last offset may read -1 - count.
+constexpr i64 kSellMaxSlots = 2147483647;
+inline bool sell_fits(i64 padded) { return padded >= 0 && padded <= kSellMaxSlots; }
 // The padded slot count of a sliced copy from its last slice offset, which
 // holds -1 - that count when the copy carries 16-bit column offsets.
 inline i64 sell_slots(int last) { return last < 0 ? -1 - (i64)last : (i64)last; }

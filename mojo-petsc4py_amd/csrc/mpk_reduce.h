@@ -167,12 +167,16 @@ struct NoEpilogue {
 };
 
 // One block: fold the partials, store the K results to `out` (if given) and
-// run the epilogue on them in thread 0.
+// run the epilogue on them in thread 0.  Up to MPK_FIN_LDS results are kept
+// in LDS; more (GMRES's Gram-Schmidt sums past a restart of 255) are folded
+// straight into `out`, which the caller must then provide (K doubles).
+#define MPK_FIN_LDS 256
 template <class Epi>
 __global__ void __launch_bounds__(MPK_NT) k_finalize(const double *P, int T, int K, double *out,
                                                      Epi epi, const int64_t *stop) {
   if (stop && *stop) return;
-  __shared__ double res[256];
+  __shared__ double sres[MPK_FIN_LDS];
+  double *const res = K <= MPK_FIN_LDS ? sres : out;
   // one round of loads for the common counts (the same sums, in the same
   // order, as fold_partials)
   if (K == 1)
@@ -195,17 +199,222 @@ __global__ void __launch_bounds__(MPK_NT) k_finalize(const double *P, int T, int
       __syncthreads();  // the shared trees are reused by the next four
     }
   if (threadIdx.x == 0) {
-    if (out)
+    if (out && res != out)
       for (int k = 0; k < K; k++) out[k] = res[k];
     epi(res);
   }
 }
 
+// ------------------------------------------------------------ the reference's order
+//
+// The order of the reference's SIMD loops (simdvec.mojo:332-473 with SW = 4,
+// BLK = 16, simd.mojo:18,32-33; _mdot_basis, ksp.mojo:956-1062), opt-in:
+//   the first 16 floor(n / 16) elements go to 16 accumulator chains, element
+//   e to chain e % 16, each chain a serial fma(x, y, acc) (an add for sums)
+//   in ascending e from +0.0; lane l of the SW-wide accumulator is then
+//   ((c_l + c_{4+l}) + c_{8+l}) + c_{12+l}; dot and the unary loops run
+//   SW-steps of four over the remainder into those lanes (_mdot_basis does
+//   not); the last elements go serially into a scalar s from +0.0; the result
+//   is s + ((acc_0 + acc_2) + (acc_1 + acc_3)), Mojo's reduce_add over four
+//   lanes.
+// A chain is a dependency sequence, so one workgroup runs the 16 chains of up
+// to four reductions in 16 lanes each of its first wave, from chunks of the
+// operands that all 1024 threads stage through LDS (double-buffered: the
+// loads of chunk c + 1 are in flight while the chains consume chunk c).  The
+// sums and abs-sums use fma(v, 1.0, acc), which is acc + v exactly, so every
+// lane runs the same instruction.
+#define MPK_REF_NT 1024
+#define MPK_REF_BUF 8192  // doubles in one chunk buffer (two buffers: 128 KB of LDS)
+
+struct RefJob {
+  const double *v[4];            // the distinct operands, staged
+  const double *sx[4], *sy[4];   // each reduction's operands (read by the tail)
+  unsigned code;                 // reduction k: byte k = xi | yi << 2 | kind << 4 (staged indices)
+  int nvec, nspec;               // nvec <= 4, nspec <= 4
+  int swsteps;                   // 1: dot / unary loops; 0: _mdot_basis (no SW-steps)
+  int al;                        // every operand 16-byte aligned
+};
+
+// The per-element factors of a chain step: acc = fma(f, g, acc).
+__device__ __forceinline__ void ref_terms(int kind, double x, double y, double &f, double &g) {
+  f = kind == RK_ASUM ? fabs(x) : x;
+  g = kind == RK_DOT ? y : (kind == RK_SUMSQ ? x : 1.0);
+}
+
+// Q double2 loads per thread and operand per chunk: Q * nvec <= 4, a chunk
+// holds PV = 2048 Q elements of each operand (128 Q blocks of 16).
+template <int Q>
+__device__ __forceinline__ void ref_run(const RefJob J, i64 n, double *out) {
+  __shared__ double lds[2 * MPK_REF_BUF];
+  __shared__ double chv[4][16];
+  constexpr int PV = 2048 * Q, CB = PV / 16, NVMAX = 4 / Q;
+  const int t = (int)threadIdx.x, lane = t & 63;
+  const i64 nb = n / 16, lim = 16 * nb;
+  const i64 nchunks = (nb + CB - 1) / CB;
+  const bool chain = t < 64 && lane < 16 * J.nspec;
+  const int k = lane >> 4, c = lane & 15;
+  // (shifts, not an indexed table: the job may sit in registers)
+  const unsigned code = (J.code >> (8 * k)) & 0xffu;
+  const int xi = (int)(code & 3u), yi = (int)((code >> 2) & 3u), kind = (int)(code >> 4);
+  double2 reg[4];
+  auto load = [&](i64 chunk) {
+#pragma unroll
+    for (int vi = 0; vi < NVMAX; vi++)
+#pragma unroll
+      for (int m = 0; m < Q; m++) {
+        double2 r = make_double2(0.0, 0.0);
+        const i64 e = chunk * PV + 2 * (i64)(t + MPK_REF_NT * m);
+        if (vi < J.nvec && e < lim) {  // e even and lim a multiple of 16: e + 1 < lim too
+          const double *p = J.v[vi] + e;
+          if (J.al) {
+            r = *reinterpret_cast<const double2 *>(p);
+          } else {
+            r.x = p[0];
+            r.y = p[1];
+          }
+        }
+        reg[vi * Q + m] = r;
+      }
+  };
+  auto store = [&](int buf) {
+#pragma unroll
+    for (int vi = 0; vi < NVMAX; vi++)
+#pragma unroll
+      for (int m = 0; m < Q; m++)
+        if (vi < J.nvec)
+          *reinterpret_cast<double2 *>(lds + buf * MPK_REF_BUF + vi * PV + 2 * (t + MPK_REF_NT * m)) =
+              reg[vi * Q + m];
+  };
+  double acc = 0.0;
+  if (nchunks > 0) {
+    load(0);
+    store(0);
+  }
+  __syncthreads();
+  for (i64 ch = 0; ch < nchunks; ch++) {
+    const int buf = (int)(ch & 1);
+    if (ch + 1 < nchunks) load(ch + 1);
+    if (chain) {
+      const double *xs = lds + buf * MPK_REF_BUF + xi * PV + c;
+      const double *ys = lds + buf * MPK_REF_BUF + yi * PV + c;
+      const i64 left = nb - ch * CB;
+      const int nblk = left < CB ? (int)left : CB;
+      int b = 0;
+      for (; b + 8 <= nblk; b += 8) {
+        double xv[8], yv[8];
+#pragma unroll
+        for (int u = 0; u < 8; u++) {
+          xv[u] = xs[16 * (b + u)];
+          yv[u] = ys[16 * (b + u)];
+        }
+#pragma unroll
+        for (int u = 0; u < 8; u++) {
+          double f, g;
+          ref_terms(kind, xv[u], yv[u], f, g);
+          acc = fma(f, g, acc);
+        }
+      }
+      for (; b < nblk; b++) {
+        double f, g;
+        ref_terms(kind, xs[16 * b], ys[16 * b], f, g);
+        acc = fma(f, g, acc);
+      }
+    }
+    if (ch + 1 < nchunks) store(buf ^ 1);
+    __syncthreads();
+  }
+  if (chain) chv[k][c] = acc;
+  __syncthreads();
+  // the tail of each reduction, in thread 0 (at most 15 elements each)
+  if (t == 0) {
+#pragma unroll
+    for (int r = 0; r < 4; r++) {
+      if (r >= J.nspec) break;
+      const int tkind = (int)((J.code >> (8 * r + 4)) & 0xfu);
+      const double *x = J.sx[r], *y = J.sy[r];
+      double a4[4];
+#pragma unroll
+      for (int l = 0; l < 4; l++) a4[l] = ((chv[r][l] + chv[r][4 + l]) + chv[r][8 + l]) + chv[r][12 + l];
+      i64 i = lim;
+      if (J.swsteps)
+        for (; i + 4 <= n; i += 4)
+#pragma unroll
+          for (int l = 0; l < 4; l++) {
+            double f, g;
+            ref_terms(tkind, x[i + l], y[i + l], f, g);
+            a4[l] = fma(f, g, a4[l]);
+          }
+      double s = 0.0;
+      for (; i < n; i++) {
+        double f, g;
+        ref_terms(tkind, x[i], y[i], f, g);
+        s = fma(f, g, s);
+      }
+      out[r] = s + ((a4[0] + a4[2]) + (a4[1] + a4[3]));
+    }
+  }
+}
+
+template <int Q>
+__global__ void __launch_bounds__(MPK_REF_NT) k_ref_red(RefJob J, i64 n, double *P, const int64_t *stop) {
+  if (stop && *stop) return;
+  ref_run<Q>(J, n, P);
+}
+
+// x . V_j, j < nv, in _mdot_basis's order: block b takes rows 3b .. 3b + 2
+// (a row's result does not depend on how the rows are grouped).
+__global__ void __launch_bounds__(MPK_REF_NT) k_ref_mdot(const double *x, const double *V, i64 ldv, int nv, i64 n,
+                                                         double *P, const int64_t *stop);
+
+// The job for a set of K <= 4 reductions: distinct operands staged once.
+template <int K>
+RefJob ref_job(const RedSet<K> &rs) {
+  static_assert(K <= 4, "at most four reductions in one pass");
+  RefJob J{};
+  J.swsteps = 1;
+  J.nspec = K;
+  J.al = 1;
+  auto slot = [&](const double *p) {
+    for (int i = 0; i < J.nvec; i++)
+      if (J.v[i] == p) return i;
+    if (J.nvec == 4) {  // no caller stages more than two
+      set_error(4, "reference-order reduction over more than four distinct vectors");
+      return 0;
+    }
+    J.v[J.nvec] = p;
+    J.al = J.al && aligned16(p);
+    return J.nvec++;
+  };
+  for (int k = 0; k < K; k++) {
+    const int kind = rs.s[k].kind;
+    const int xi = slot(rs.s[k].x);
+    const int yi = kind == RK_DOT ? slot(rs.s[k].y) : xi;
+    J.code |= (unsigned)(xi | yi << 2 | kind << 4) << (8 * k);
+    J.sx[k] = rs.s[k].x;
+    J.sy[k] = kind == RK_DOT ? rs.s[k].y : rs.s[k].x;
+  }
+  return J;
+}
+
+void ref_launch(const RefJob &J, i64 n, double *P, const int64_t *stop);
+
 // Host side: launch the partials for the K specs and the finalize with `epi`.
 // (A last-block finalize inside the partials launch needs a device-scope fence
 // per block, an L2 writeback on this GPU, which costs more than the launch.)
+// In the reference's order the one workgroup writes each result as a single
+// partial, which the finalize then folds with zeros (x + 0.0 is x: the
+// reference order never yields -0.0, every chain starting from +0.0).
 template <int K, class Epi>
 void reduce_launch(const RedSet<K> &rs, i64 n, double *out, Epi epi, const int64_t *stop) {
+  if (red_ref()) {
+    double *P = partials((size_t)K);
+    if (!P) return;
+    const RefJob J = ref_job<K>(rs);
+    if (failed()) return;
+    ref_launch(J, n, P, stop);
+    MPK_LAUNCH(k_finalize<Epi>, 1, MPK_NT, P, 1, K, out, epi, stop);
+    return;
+  }
   unsigned T = ntiles(n);
   if (T == 0) T = 1;  // n == 0 still yields a (zero) result
   double *P = partials((size_t)K * T);

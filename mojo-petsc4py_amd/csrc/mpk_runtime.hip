@@ -1,6 +1,8 @@
 // mpk_runtime.hip -- context, memory and error plumbing behind the C ABI.
+#include <atomic>
 #include <cstdarg>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <mutex>
 
@@ -123,6 +125,23 @@ void sync() {
   HIPCHK(hipStreamSynchronize(c->stream));
 }
 
+// -1: not read yet; 0: tile-tree order; 1: the reference's order
+static std::atomic<int> g_red_order{-1};
+
+static int red_order_from_env() {
+  const char *e = std::getenv("MPK_REDUCTION");
+  return e && std::strcmp(e, "ref") == 0 ? 1 : 0;
+}
+
+bool red_ref() {
+  int o = g_red_order.load(std::memory_order_relaxed);
+  if (o < 0) {
+    o = red_order_from_env();
+    g_red_order.store(o, std::memory_order_relaxed);
+  }
+  return o == 1;
+}
+
 }  // namespace mpk
 
 using namespace mpk;
@@ -203,5 +222,18 @@ void mpk_memset(int64_t dst, int64_t value, int64_t bytes) {
 }
 
 void mpk_synchronize(void) { sync(); }
+
+int64_t mpk_set_reduction_order(int64_t order) {
+  const int64_t old = red_ref() ? 1 : 0;
+  if (order != 0 && order != 1) {
+    set_error(4, "mpk_set_reduction_order: order %lld is neither 0 (tile tree) nor 1 (reference)",
+              (long long)order);
+    return old;
+  }
+  g_red_order.store((int)order, std::memory_order_relaxed);
+  return old;
+}
+
+int64_t mpk_get_reduction_order(void) { return red_ref() ? 1 : 0; }
 
 }  // extern "C"

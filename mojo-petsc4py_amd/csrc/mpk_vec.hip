@@ -285,6 +285,41 @@ void vmaxpy_dev(double *y, const double *base, i64 ld, const double *coef_dev, i
   }
 }
 
+// ------------------------------------------------------------ the reference's order
+
+void ref_launch(const RefJob &J, i64 n, double *P, const int64_t *stop) {
+  if (J.nvec <= 1)
+    MPK_LAUNCH(k_ref_red<4>, 1, MPK_REF_NT, J, n, P, stop);
+  else if (J.nvec == 2)
+    MPK_LAUNCH(k_ref_red<2>, 1, MPK_REF_NT, J, n, P, stop);
+  else
+    MPK_LAUNCH(k_ref_red<1>, 1, MPK_REF_NT, J, n, P, stop);
+}
+
+__global__ void __launch_bounds__(MPK_REF_NT) k_ref_mdot(const double *x, const double *V, i64 ldv, int nv, i64 n,
+                                                         double *P, const int64_t *stop) {
+  if (stop && *stop) return;
+  const int j0 = 3 * (int)blockIdx.x;
+  const int ns = nv - j0 < 3 ? nv - j0 : 3;
+  RefJob J;
+  J.swsteps = 0;
+  J.nspec = ns;
+  J.nvec = 1 + ns;
+  J.al = aligned16(x) && aligned16(V) && ldv % 2 == 0;
+  J.v[0] = x;
+  J.v[1] = V + (i64)j0 * ldv;
+  J.v[2] = V + (i64)(j0 + (ns > 1 ? 1 : 0)) * ldv;
+  J.v[3] = V + (i64)(j0 + (ns > 2 ? 2 : 0)) * ldv;
+  J.code = 0;
+#pragma unroll
+  for (int r = 0; r < 4; r++) {
+    J.code |= (unsigned)(0 | (r < 3 ? 1 + r : 3) << 2 | RK_DOT << 4) << (8 * r);
+    J.sx[r] = x;
+    J.sy[r] = J.v[r < 3 ? 1 + r : 3];
+  }
+  ref_run<1>(J, n, P + j0);
+}
+
 // ------------------------------------------------------------ reductions to host
 
 // Launch `K` tile-tree reductions and return the results on the host.

@@ -14,7 +14,7 @@ import math
 
 import numpy as np
 
-from mpk._native import check, lib
+from mpk._native import check, lib, reduction_order
 from mpk.device import DeviceBuffer
 
 from .Mat import Mat
@@ -298,8 +298,9 @@ class KSP:
         self._resolve_normtype()
         cfg, cfgi = self._state(n)
         d = self._amat._dev()
-        so, sv, sc = d.sell()
-        cfgi[13], cfgi[14], cfgi[15] = so.ptr, sv.ptr, sc.ptr
+        s = d.sell()  # None: the solver keeps the CSR row kernel
+        if s is not None:
+            cfgi[13], cfgi[14], cfgi[15] = s[0].ptr, s[1].ptr, s[2].ptr
         pci = self._pc._pci_block()
         pcf = self._pc._pcf_block()
         L = lib()
@@ -362,15 +363,23 @@ class KSP:
         return cfg, cfgi
 
     def _rhs_norm(self) -> float:
-        """KSPConvergedDefault's right-hand-side norm (KSP.py:496-507),
-        computed on the GPU."""
+        """KSPConvergedDefault's right-hand-side norm (KSP.py:496-507).  The
+        reference takes the two 2-norms with numpy on the host; in the
+        reference reduction order this does too, so that its bits follow;
+        in the tile-tree order they are the device norm."""
         b = self._rhs
         if b is None or self._normtype == NormType_.NORM_NONE:
             return 0.0
+
+        def two_norm(v):
+            if reduction_order() == "ref":
+                return float(np.linalg.norm(v.getArray()))
+            return v.norm()
+
         if self._normtype == NormType_.UNPRECONDITIONED or self._pc_side == 1:
-            return b.norm()
+            return two_norm(b)
         z = b.duplicate()
         self._pc.apply(b, z)
         if self._normtype == NormType_.PRECONDITIONED:
-            return z.norm()
+            return two_norm(z)
         return math.sqrt(abs(z.dot(b)))

@@ -18,6 +18,9 @@ from mpk.device import DeviceBuffer
 
 from .Vec import Vec, _f64
 
+# the most padded slots a sliced copy can hold (int32 slot indices)
+SELL_MAX_SLOTS = 2**31 - 1
+
 
 class MatType:
     """The matrix type."""
@@ -100,18 +103,23 @@ class _DevCSR:
 
     def sell(self):
         """(soff, sv, sc): rows in 64-row slices, column-major inside a slice,
-        so a wave's loads are contiguous; row sums stay in CSR order."""
+        so a wave's loads are contiguous; row sums stay in CSR order.  None
+        when the copy would pass the int32 slot indices (2^31 - 1 padded
+        slots): the products then run the CSR row kernel."""
         if self.sellc is None:
             L = lib()
             padded = L.mpk_sell_size(self.rp.ptr, self.nrows)
             check()
+            if padded > SELL_MAX_SLOTS:
+                self.sellc = False
+                return None
             soff = DeviceBuffer(4 * ((self.nrows + 63) // 64 + 1))
             sv = DeviceBuffer(8 * max(padded, 1))
             sc = DeviceBuffer(6 * max(padded, 1))  # int32 columns, then int16 offsets
             if L.mpk_sell_build(self.rp.ptr, self.ci.ptr, self.v.ptr, self.nrows, soff.ptr, sv.ptr, sc.ptr) != 0:
                 check()
             self.sellc = (soff, sv, sc)
-        return self.sellc
+        return self.sellc or None
 
     def transpose_sell(self, irow, icol, data):
         """(trp, soff, sv, sc): the sliced copy of A^T, built from the host CSR
@@ -512,8 +520,11 @@ class Mat:
             raise ValueError(f"incompatible sizes: matrix {self._ncols} columns, vector {x.getSize()}")
         d = self._dev()
         xa = x._d()
-        so, sv, sc = d.sell()
-        lib().mpk_sell_mult(xa, out._dw(), d.rp.ptr, so.ptr, sv.ptr, sc.ptr, self._nrows, 0)
+        s = d.sell()
+        if s is None:
+            lib().mp_mat_mult(xa, out._dw(), d.rp.ptr, d.ci.ptr, d.v.ptr, self._nrows)
+        else:
+            lib().mpk_sell_mult(xa, out._dw(), d.rp.ptr, s[0].ptr, s[1].ptr, s[2].ptr, self._nrows, 0)
         check()
         return out
 
@@ -528,8 +539,11 @@ class Mat:
             v.copy(out)
         d = self._dev()
         xa = x._d()
-        so, sv, sc = d.sell()
-        lib().mpk_sell_mult(xa, out._dw(), d.rp.ptr, so.ptr, sv.ptr, sc.ptr, self._nrows, 1)
+        s = d.sell()
+        if s is None:
+            lib().mp_mat_mult_add(xa, out._dw(), d.rp.ptr, d.ci.ptr, d.v.ptr, self._nrows)
+        else:
+            lib().mpk_sell_mult(xa, out._dw(), d.rp.ptr, s[0].ptr, s[1].ptr, s[2].ptr, self._nrows, 1)
         check()
         return out
 

@@ -5,4 +5,5 @@ ABI), `mpk.device` owns device memory, `mpk.workloads` builds the synthetic
 systems the tests and the benchmark use.
 """
 
-from ._native import NativeError, check, device_name, lib  # noqa: F401
+from ._native import (NativeError, check, device_name, lib, reduction_order,  # noqa: F401
+                      reduction_order_scope, set_reduction_order)

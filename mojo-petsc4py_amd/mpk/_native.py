@@ -95,11 +95,14 @@ SIGNATURES = {
     "mpk_memcpy_dtod": ([I, I, I], None),
     "mpk_memset": ([I, I, I], None),
     "mpk_synchronize": ([], None),
+    "mpk_set_reduction_order": ([I], I),
+    "mpk_get_reduction_order": ([], I),
     "mpk_csr_transpose": ([I] * 7, I),
     "mpk_mat_mult_transpose_t": ([I] * 7, None),
     "mpk_pc_icc_apply_t": ([I] * 13, I),
     "mpk_time_spmv": ([I] * 10, F),
     "mpk_sell_size": ([I, I], I),
+    "mpk_sell_size_host": ([I, I], I),
     "mpk_sell_build": ([I] * 7, I),
     "mpk_sell_mult": ([I] * 8, None),
     "mpk_batch_create": ([I] * 21, I),
@@ -171,6 +174,39 @@ def check() -> None:
         msg = last_error()
         _lib.mpk_clear_error()
         raise NativeError(msg)
+
+
+_ORDERS = {"dev": 0, "ref": 1}
+
+
+def set_reduction_order(order: str) -> str:
+    """Select the order of every reduction the library computes: "dev" (the
+    default, a fixed tile tree) or "ref" (the reference's SIMD order,
+    src/simdvec.mojo:332-473 with SW = 4, BLK = 16).  Returns the previous
+    order."""
+    if order not in _ORDERS:
+        raise ValueError(f"reduction order must be 'dev' or 'ref', not {order!r}")
+    old = lib().mpk_set_reduction_order(_ORDERS[order])
+    check()
+    return "ref" if old == 1 else "dev"
+
+
+def reduction_order() -> str:
+    return "ref" if lib().mpk_get_reduction_order() == 1 else "dev"
+
+
+class reduction_order_scope:
+    """`with reduction_order_scope("ref"): ...` -- the order inside the block."""
+
+    def __init__(self, order: str):
+        self.order = order
+
+    def __enter__(self):
+        self.old = set_reduction_order(self.order)
+        return self
+
+    def __exit__(self, *exc):
+        set_reduction_order(self.old)
 
 
 def device_name() -> str:

@@ -56,7 +56,10 @@ class MixedSolver:
             raise ValueError('sweeps must be "exact" or a positive number of Jacobi passes')
         self.sweeps = sweeps
         d = A._dev()
-        so, sv, sc = d.sell()
+        s = d.sell()
+        if s is None:
+            raise ValueError("the matrix is too large for the sliced copy the mixed solver reads")
+        so, sv, sc = s
         if pc is not None:
             pc.setUp()
             pci = pc._pci_block()

@@ -47,7 +47,8 @@ typedef int64_t I;
 #define SW 4
 #define BLK 16
 
-static int g_mode = 0;
+/* per thread, so that solves in both orders can run side by side */
+static _Thread_local int g_mode = 0;
 
 void or_set_reduction_mode(I mode) { g_mode = (int)mode; }
 I or_get_reduction_mode(void) { return g_mode; }

@@ -53,3 +53,77 @@ def wide_band(n=40000, far=35000):
         vals.append(np.full(len(i), v))
     return W._csr_from_coo(n, np.concatenate(rows).astype(np.int64), np.concatenate(cols).astype(np.int64),
                            np.concatenate(vals))
+
+
+# ---------------------------------------------------------------- whole solves
+
+def petsc_solve(csr, b, ksp_type, pc_type, levels=0, x0=None, **opt):
+    from mojopetsc4py import PETSc
+    ip, ix, va = csr
+    n = len(ip) - 1
+    A = PETSc.Mat().createSeqAIJ((n, n), nnz=7)
+    A.setValuesCSR(ip, ix, va)
+    A.assemble()
+    pc = PETSc.PC().create()
+    pc.setType(pc_type)
+    if levels:
+        pc.setFactorLevels(levels)
+    if "omega" in opt:
+        pc.setSOROmega(opt.pop("omega"))
+    if "ordering" in opt:
+        pc.setFactorOrdering(opt.pop("ordering"))
+    pc.setOperators(A)
+    ksp = PETSc.KSP().create()
+    ksp.setPC(pc)
+    ksp.setType(ksp_type)
+    ksp.setOperators(A)
+    tol = {k: opt.pop(k) for k in ("rtol", "atol", "divtol", "max_it") if k in opt}
+    if tol:
+        ksp.setTolerances(**tol)
+    if "restart" in opt:
+        ksp.setGMRESRestart(opt.pop("restart"))
+    if "orthog" in opt:
+        ksp.setGMRESOrthogonalization(opt.pop("orthog"))
+    if "cgstype" in opt:
+        ksp.setGMRESCGSRefinementType(opt.pop("cgstype"))
+    if "side" in opt:
+        ksp.setPCSide(opt.pop("side"))
+    if "normtype" in opt:
+        ksp.setNormType(opt.pop("normtype"))
+    assert not opt, opt
+    ksp.setConvergenceHistory()
+    bv = PETSc.Vec().createWithArray(b)
+    xv = PETSc.Vec().createSeq(n)
+    if x0 is not None:
+        xv.setArray(x0)
+        ksp.setInitialGuessNonzero(True)
+    ksp.solve(bv, xv)
+    its = ksp.getIterationNumber()
+    return (xv.getArray().copy(), its, ksp.getConvergedReason(), ksp.getResidualNorm(),
+            ksp.getConvergenceHistory()[: its + 1].copy())
+
+
+def oracle_solve(O, csr, b, ksp_type, pc_type, levels=0, x0=None, **opt):
+    A = O.CSR(*csr)
+    pcopt = {}
+    if "omega" in opt:
+        pcopt["omega"] = opt.pop("omega")
+    if "ordering" in opt:
+        pcopt["ordering"] = opt.pop("ordering")
+    pc = O.PCSetup(A, pc_type, levels, **pcopt)
+    kw = dict(opt)
+    if x0 is not None:
+        kw["guess_zero"] = 0
+    r = O.ksp_solve(A, b, ksp_type, pc, x0=x0, **kw)
+    return r.x, r.its, r.reason, r.rnorm, r.history
+
+
+def compare(O, csr, b, ksp_type, pc_type, levels=0, x0=None, **opt):
+    g = petsc_solve(csr, b, ksp_type, pc_type, levels, x0, **dict(opt))
+    o = oracle_solve(O, csr, b, ksp_type, pc_type, levels, x0, **dict(opt))
+    assert g[1] == o[1], ("its", g[1], o[1])
+    assert g[2] == o[2], ("reason", g[2], o[2])
+    assert same_bits(np.array([g[3]]), np.array([o[3]])), ("rnorm", g[3], o[3])
+    assert same_bits(g[4], o[4][: len(g[4])]), "history"
+    assert same_bits(g[0], o[0]), ("x", np.max(np.abs(g[0] - o[0])))
+    return g

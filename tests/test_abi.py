@@ -74,3 +74,24 @@ def test_no_device_fails_loudly():
         pytest.skip("a HIP device is present")
     with pytest.raises(_native.NativeError):
         _native.lib()
+
+
+def test_sell_slot_count_guard(lib):
+    """The sliced copy's padded slot count, host side: 64 slots per slice row
+    at the slice's longest row, and -1 once it passes the int32 slot
+    indices (2^31 - 1), where the products keep the CSR row kernel."""
+    import numpy as np
+    from mpk import workloads as W
+    ip, _, _ = W.laplacian_2d(20)
+    rp = np.asarray(ip, dtype=np.int64)
+    n = len(rp) - 1
+    lens = np.diff(rp)
+    want = sum(64 * int(lens[s:s + 64].max()) for s in range(0, n, 64))
+    assert lib.mpk_sell_size_host(rp.ctypes.data, n) == want
+    # one dense row: nnz fits int32, but its slice pads to 64 x the row
+    fits = np.array([0, (2**31 - 1) // 64], dtype=np.int64)
+    assert lib.mpk_sell_size_host(fits.ctypes.data, 1) == 64 * ((2**31 - 1) // 64)
+    past = np.array([0, 2**25], dtype=np.int64)
+    assert lib.mpk_sell_size_host(past.ctypes.data, 1) == -1
+    two = np.array([0, 2**24, 2**25 + 2**24], dtype=np.int64)  # the second row sets the slice's width
+    assert lib.mpk_sell_size_host(two.ctypes.data, 2) == -1

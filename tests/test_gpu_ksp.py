@@ -5,6 +5,7 @@ bit for bit.  Driven through the PETSc surface, which calls mp_ksp_solve_*."""
 import numpy as np
 import pytest
 
+from _helpers import compare as _compare
 from _helpers import same_bits, wide_band
 from mpk import workloads as W
 
@@ -16,78 +17,6 @@ def O(oracle):
     oracle.set_mode("dev")
     yield oracle
     oracle.set_mode("ref")
-
-
-def _petsc_solve(csr, b, ksp_type, pc_type, levels=0, x0=None, **opt):
-    from mojopetsc4py import PETSc
-    ip, ix, va = csr
-    n = len(ip) - 1
-    A = PETSc.Mat().createSeqAIJ((n, n), nnz=7)
-    A.setValuesCSR(ip, ix, va)
-    A.assemble()
-    pc = PETSc.PC().create()
-    pc.setType(pc_type)
-    if levels:
-        pc.setFactorLevels(levels)
-    if "omega" in opt:
-        pc.setSOROmega(opt.pop("omega"))
-    if "ordering" in opt:
-        pc.setFactorOrdering(opt.pop("ordering"))
-    pc.setOperators(A)
-    ksp = PETSc.KSP().create()
-    ksp.setPC(pc)
-    ksp.setType(ksp_type)
-    ksp.setOperators(A)
-    tol = {k: opt.pop(k) for k in ("rtol", "atol", "divtol", "max_it") if k in opt}
-    if tol:
-        ksp.setTolerances(**tol)
-    if "restart" in opt:
-        ksp.setGMRESRestart(opt.pop("restart"))
-    if "orthog" in opt:
-        ksp.setGMRESOrthogonalization(opt.pop("orthog"))
-    if "cgstype" in opt:
-        ksp.setGMRESCGSRefinementType(opt.pop("cgstype"))
-    if "side" in opt:
-        ksp.setPCSide(opt.pop("side"))
-    if "normtype" in opt:
-        ksp.setNormType(opt.pop("normtype"))
-    assert not opt, opt
-    ksp.setConvergenceHistory()
-    bv = PETSc.Vec().createWithArray(b)
-    xv = PETSc.Vec().createSeq(n)
-    if x0 is not None:
-        xv.setArray(x0)
-        ksp.setInitialGuessNonzero(True)
-    ksp.solve(bv, xv)
-    its = ksp.getIterationNumber()
-    return (xv.getArray().copy(), its, ksp.getConvergedReason(), ksp.getResidualNorm(),
-            ksp.getConvergenceHistory()[: its + 1].copy())
-
-
-def _oracle_solve(O, csr, b, ksp_type, pc_type, levels=0, x0=None, **opt):
-    A = O.CSR(*csr)
-    pcopt = {}
-    if "omega" in opt:
-        pcopt["omega"] = opt.pop("omega")
-    if "ordering" in opt:
-        pcopt["ordering"] = opt.pop("ordering")
-    pc = O.PCSetup(A, pc_type, levels, **pcopt)
-    kw = dict(opt)
-    if x0 is not None:
-        kw["guess_zero"] = 0
-    r = O.ksp_solve(A, b, ksp_type, pc, x0=x0, **kw)
-    return r.x, r.its, r.reason, r.rnorm, r.history
-
-
-def _compare(O, csr, b, ksp_type, pc_type, levels=0, x0=None, **opt):
-    g = _petsc_solve(csr, b, ksp_type, pc_type, levels, x0, **dict(opt))
-    o = _oracle_solve(O, csr, b, ksp_type, pc_type, levels, x0, **dict(opt))
-    assert g[1] == o[1], ("its", g[1], o[1])
-    assert g[2] == o[2], ("reason", g[2], o[2])
-    assert same_bits(np.array([g[3]]), np.array([o[3]])), ("rnorm", g[3], o[3])
-    assert same_bits(g[4], o[4][: len(g[4])]), "history"
-    assert same_bits(g[0], o[0]), ("x", np.max(np.abs(g[0] - o[0])))
-    return g
 
 
 LAP64 = W.laplacian_2d(64)
@@ -221,3 +150,21 @@ def test_cg_jacobi_nontemporal_product(native, O, monkeypatch):
     monkeypatch.setenv("MPK_CG_FOLD_TILES", "0")
     _compare(O, DIFF16, DIFF16B, "cg", "jacobi", rtol=1e-10, max_it=300)
     _compare(O, DIFF16, DIFF16B, "gmres", "jacobi", rtol=1e-10, max_it=300)
+
+
+def _wide_spectrum(n=400):
+    """Diagonal with eigenvalues log-uniform over [1, 1e6]: GMRES needs about
+    n iterations, so a restart of 300 fills its basis past 256 vectors."""
+    d = np.logspace(0, 6, n)
+    ip = np.arange(n + 1, dtype=np.int32)
+    return ip, np.arange(n, dtype=np.int32), d
+
+
+@pytest.mark.parametrize("ksp_type", ["gmres", "fgmres"])
+def test_gmres_restart_past_256(native, O, ksp_type):
+    """More Gram-Schmidt sums than the finalize keeps in LDS (the basis grows
+    to 301 vectors): the oracle's bits.  The reference accepts any restart."""
+    csr = _wide_spectrum()
+    b = np.random.default_rng(13).normal(size=400)
+    g = _compare(O, csr, b, ksp_type, "none", rtol=1e-13, max_it=320, restart=300)
+    assert g[1] > 257

@@ -350,3 +350,169 @@ def test_convergence_test_branches(oracle):
     assert run(3, 1e-9, cfgi_over={3: 5}) == 0     # below min_it
     assert run(3, 1e-9, cfgi_over={4: 3}) == 0     # chknorm defers the test
     assert run(3, 1.0, cfgi_over={0: 1}) == 0      # NORM_NONE never converges
+
+
+# ---------------------------------------------------------------- MAXPY, MDot, normalize, MultAdd, RCM
+
+def test_maxpy_sequential_fma(oracle):
+    """VecMAXPY (simdvec.mojo:503-518, ported.mojo:329-335): y_i takes
+    y_i + a_k v_k,i for k = 0, 1, ... one correctly rounded multiply-add at a
+    time.  Summing the terms first, or another k order, changes some bits
+    here (checked: the alternatives differ from the sequential result)."""
+    L = oracle.lib()
+    rng = np.random.default_rng(7)
+    n, nv = 41, 9
+    V = [rng.normal(size=n) * 10.0 ** rng.integers(-3, 4, size=n) for _ in range(nv)]
+    alpha = rng.normal(size=nv)
+    y0 = rng.normal(size=n)
+    y = y0.copy()
+    ptrs = np.array([v.ctypes.data for v in V], dtype=np.int64)
+    L.or_vec_maxpy(a_(y), nv, a_(alpha), a_(ptrs), n)
+    want = y0.copy()
+    for k in range(nv):
+        want = np.array([fma_exact(alpha[k], V[k][i], want[i]) for i in range(n)])
+    assert np.array_equal(y.view(np.int64), want.view(np.int64))
+    terms_first = np.array([float(Fr(y0[i]) + sum(Fr(alpha[k]) * Fr(V[k][i]) for k in range(nv))) for i in range(n)])
+    reversed_k = y0.copy()
+    for k in reversed(range(nv)):
+        reversed_k = np.array([fma_exact(alpha[k], V[k][i], reversed_k[i]) for i in range(n)])
+    assert not np.array_equal(want, terms_first) and not np.array_equal(want, reversed_k)
+
+
+@pytest.mark.parametrize("n", [0, 1, 15, 16, 17, 31, 33, 100, 2049])
+@pytest.mark.parametrize("nv", [1, 3, 4, 5, 8, 9])
+def test_mdot_against_fsum(O, n, nv):
+    """_mdot_basis (ksp.mojo:956-1062) in both orders: every x . V_j within
+    1e-13 of the exactly rounded sum, across the 4-row groups and the
+    16-element blocks."""
+    L = O.lib()
+    rng = np.random.default_rng(100 * n + nv)
+    x = rng.normal(size=n)
+    V = rng.normal(size=(nv, n))
+    res = np.zeros(nv)
+    L.or_red_mdot(a_(x), a_(np.ascontiguousarray(V)), nv, n, a_(res))
+    for j in range(nv):
+        exact = math.fsum(float(Fr(p) * Fr(q)) for p, q in zip(x, V[j])) if n else 0.0
+        scale = max(1.0, math.fsum(abs(p * q) for p, q in zip(x, V[j])))
+        assert abs(res[j] - exact) <= 1e-13 * scale, (j, res[j], exact)
+
+
+def _cancel_vector():
+    """1e16 at element 0, ones at 1..15, -1e16 at 16, ones at 17..20.  With
+    x = ones a sum of these n = 21 products shows its order: 1e16 + 1 rounds
+    back to 1e16 (a tie to even), so which ones meet the 1e16 before the
+    -1e16 cancels it decides the result."""
+    y = np.ones(21)
+    y[0], y[16] = 1e16, -1e16
+    return np.ones(21), y
+
+
+def test_reduction_orders_known_answers(oracle):
+    """Results worked out by hand from each order's rule; a mistake in the
+    chain of an element, the lane sums, the SW-steps, the tail or the tree
+    shape gives another number (the exact sum is 19, a serial sum 4).
+    REF dot (simdvec.mojo:332-353): chains c_e = e % 16 over e < 16; lane l
+    = c_l + c_{4+l} + c_{8+l} + c_{12+l}: 1e16 (three ones absorbed), 4, 4,
+    4; the SW-step over e = 16..19 adds -1e16, 1, 1, 1 to the lanes: 0, 5,
+    5, 5; the tail e = 20 gives s = 1; s + ((0 + 5) + (5 + 5)) = 16.
+    REF MDot (_mdot_basis, no SW-steps): the same lanes, (1e16 + 4) + 8 =
+    1e16 + 12, plus the tail -1e16 + 1 + 1 + 1 + 1 = -1e16 (ones absorbed):
+    12.  DEV (tile tree) on a 2048-element tile with 1e16 at 0, ones at 1
+    and 2, -1e16 at 512: thread 0's pairs are (1e16 + 1) -> 1e16 and -1e16,
+    summing to 0; thread 1's pair gives 1; the trees give 1 (REF: 2)."""
+    L = oracle.lib()
+    x, y = _cancel_vector()
+    res = np.zeros(1)
+    with oracle.mode("ref"):
+        assert L.or_red_dot(a_(x), a_(y), 21) == 16.0
+        L.or_red_mdot(a_(x), a_(y), 1, 21, a_(res))
+        assert res[0] == 12.0
+    with oracle.mode("dev"):
+        assert L.or_red_dot(a_(x), a_(y), 21) != 16.0
+    z = np.zeros(2048)
+    z[0], z[1], z[2], z[512] = 1e16, 1.0, 1.0, -1e16
+    ones = np.ones(2048)
+    with oracle.mode("dev"):
+        assert L.or_red_dot(a_(z), a_(ones), 2048) == 1.0
+        assert L.or_red_total(a_(z), 2048) == 1.0
+    with oracle.mode("ref"):
+        assert L.or_red_dot(a_(z), a_(ones), 2048) == 2.0
+        assert L.or_red_total(a_(z), 2048) == 2.0
+
+
+def test_normalize_is_a_divide(oracle):
+    """VecNormalize (ported.mojo:200-205, simdvec.mojo:296-304): x / ||x||_2,
+    a correctly rounded divide per element, not a multiply by the
+    reciprocal (which differs in some elements here); returns the norm."""
+    L = oracle.lib()
+    rng = np.random.default_rng(3)
+    x0 = rng.normal(size=257)
+    x = x0.copy()
+    v = np.zeros(1)
+    L.or_vec_normalize(a_(x), a_(v), len(x))
+    z = math.sqrt(L.or_red_sumsq(a_(x0), len(x0)))
+    assert v[0] == z
+    want = np.array([float(Fr(e) / Fr(z)) for e in x0])
+    assert np.array_equal(x.view(np.int64), want.view(np.int64))
+    assert np.any(x0 * (1.0 / z) != want)
+    zero = np.zeros(5)
+    L.or_vec_normalize(a_(zero), a_(v), 5)
+    assert v[0] == 0.0 and np.all(zero == 0.0)
+
+
+def test_mult_add_seeded_with_y(oracle):
+    """MatMultAdd_SeqAIJ (ported.mojo:394-405): y_i = y_i + sum_j a_ij x_j as
+    one serial multiply-add chain that starts from y_i.  Starting from 0 and
+    adding y_i at the end gives other bits here."""
+    L = oracle.lib()
+    ip, ix, va = W.diffusion_3d(4)[0]
+    A = oracle.CSR(ip, ix, va)
+    rng = np.random.default_rng(2)
+    x = rng.normal(size=A.n)
+    y0 = rng.normal(size=A.n) * 1e3
+    y = y0.copy()
+    L.or_mat_mult_add(a_(x), a_(y), a_(A.irow), a_(A.icol), a_(A.data), A.n)
+    seeded, late = [], []
+    for i in range(A.n):
+        s = y0[i]
+        t = 0.0
+        for j in range(ip[i], ip[i + 1]):
+            s = fma_exact(va[j], x[ix[j]], s)
+            t = fma_exact(va[j], x[ix[j]], t)
+        seeded.append(s)
+        late.append(t + y0[i])
+    assert np.array_equal(y, np.array(seeded))
+    assert not np.array_equal(np.array(seeded), np.array(late))
+
+
+def _rcm(oracle, n, edges):
+    """RCM of the symmetric pattern with the given edges plus the diagonal."""
+    rows = [[i] for i in range(n)]
+    for a, b in edges:
+        rows[a].append(b)
+        rows[b].append(a)
+    ip = np.zeros(n + 1, dtype=np.int64)
+    cols = []
+    for i in range(n):
+        c = sorted(set(rows[i]))
+        cols += c
+        ip[i + 1] = len(cols)
+    A = oracle.CSR(ip, np.array(cols, dtype=np.int64), np.ones(len(cols)))
+    r, _ = oracle.permutations(A, "rcm")
+    return r.tolist()
+
+
+def test_rcm_hand_derived(oracle):
+    """MatGetOrdering_RCM (factor.mojo:279-332): breadth first from each
+    unvisited node in index order, neighbours in increasing index, each
+    component reversed, components in the order found.  The permutations
+    below were derived by hand from that rule."""
+    # a path 0-1-2-3-4: visit order 0 1 2 3 4, reversed
+    assert _rcm(oracle, 5, [(0, 1), (1, 2), (2, 3), (3, 4)]) == [4, 3, 2, 1, 0]
+    # the 3 x 3 grid, row-major: visit order 0 1 3 2 4 6 5 7 8, reversed
+    A = oracle.CSR(*W.laplacian_2d(3))
+    assert oracle.permutations(A, "rcm")[0].tolist() == [8, 7, 5, 6, 4, 2, 3, 1, 0]
+    # two components {0, 4, 2} and {1, 5, 3}: 0 4 2 -> 2 4 0, then 1 5 3 -> 3 5 1
+    assert _rcm(oracle, 6, [(0, 4), (4, 2), (1, 5), (5, 3)]) == [2, 4, 0, 3, 5, 1]
+    # a star whose centre is the last node: 0 3, then 1 and 2 from 3 -> reversed 2 1 3 0
+    assert _rcm(oracle, 4, [(0, 3), (1, 3), (2, 3)]) == [2, 1, 3, 0]
