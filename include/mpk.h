@@ -106,6 +106,19 @@ void mp_vec_pointwise_max(int64_t w, int64_t x, int64_t y, int64_t n);
 void mp_vec_pointwise_max_abs(int64_t w, int64_t x, int64_t y, int64_t n);
 void mp_vec_norm(int64_t x, int64_t ntype, int64_t z, int64_t n);
 void mp_vec_dot(int64_t x, int64_t y, int64_t val, int64_t n);
+/* No counterpart in the reference: mp_vec_dot for callers that cannot
+ * allocate (code under HIP graph capture, tight serving loops, harnesses
+ * that forbid hipMalloc).  x, y [D]; the result goes to val [D] (one
+ * double), queued on the library's stream with no host wait.  Its reduction
+ * partials live in a fixed 512 KB device array (65,536 doubles) that each
+ * GPU holds from mpk_init until the process ends, so n is at most
+ * 65,536 x 2048 = 134,217,728 (status 4 past that, nothing queued).
+ * mpk_init must have run (status 4 otherwise): the call never initialises
+ * the library, allocates or frees.  Same bits as mp_vec_dot in either
+ * reduction order.  All calls on one GPU share the array: calls that
+ * overlap on it (from several host threads at once) race.  Returns 0, or -1
+ * on error. */
+int64_t mpk_vec_dot_noalloc(int64_t x, int64_t y, int64_t val, int64_t n);
 void mp_vec_tdot(int64_t x, int64_t y, int64_t val, int64_t n);
 void mp_vec_sum(int64_t x, int64_t val, int64_t n);
 void mp_vec_mean(int64_t x, int64_t val, int64_t n);

@@ -37,7 +37,13 @@ struct Ctx {
   unsigned *tickets = nullptr; // block tickets for the sync-free sweeps (64)
   double *dtmp = nullptr;      // device scratch vector, grown on demand
   size_t dtmp_cap = 0;
+  double *fixed = nullptr;     // the fixed scratch array of the no-allocation entry point
+  size_t fixed_cap = 0;        // in doubles
 };
+
+// The context if it is initialised already, else nullptr; never initialises
+// (so never allocates).
+Ctx *ctx_ready();
 
 // The process context, initialised on first use on the current device.
 // Returns nullptr (and records the error) when no HIP device is usable.

@@ -13,6 +13,17 @@ namespace mpk {
 
 static Ctx g_ctx;
 static bool g_init_done = false;
+
+// The scratch of the no-allocation entry point (mpk_vec_dot_noalloc): a
+// fixed array in the library's code object, 65,536 doubles (512 KB) on each
+// GPU the library runs on, held until the process ends.  Host code never uses
+// the name `scratch` as an address (on the host it is only a stand-in): init
+// looks up the device address once per device, with that device current, and
+// everything else uses that address.
+#define MPK_FIXED_SCRATCH 65536
+static __device__ __align__(256) double scratch[MPK_FIXED_SCRATCH];
+#define MPK_MAX_DEVICES 64
+static double *g_fixed_ptr[MPK_MAX_DEVICES] = {};
 static std::mutex g_mu;
 static int g_status = 0;
 static char g_msg[1024] = "";
@@ -45,11 +56,24 @@ static bool init_locked(int device) {
   HIPCHK(hipHostMalloc(&g_ctx.hres, 1024 * sizeof(double), hipHostMallocDefault));
   HIPCHK(hipMalloc(&g_ctx.tickets, 64 * sizeof(unsigned)));
   HIPCHK(hipMemset(g_ctx.tickets, 0, 64 * sizeof(unsigned)));
+  if (device >= 0 && device < MPK_MAX_DEVICES && !g_fixed_ptr[device]) {
+    void *p = nullptr;
+    hipError_t se = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (se != hipSuccess || !p) {
+      set_error(1, "hipGetSymbolAddress(scratch) on device %d: %s", device, hipGetErrorString(se));
+      return false;
+    }
+    g_fixed_ptr[device] = static_cast<double *>(p);
+  }
+  g_ctx.fixed = device >= 0 && device < MPK_MAX_DEVICES ? g_fixed_ptr[device] : nullptr;
+  g_ctx.fixed_cap = g_ctx.fixed ? MPK_FIXED_SCRATCH : 0;
   HIPCHK(hipDeviceSynchronize());
   if (failed()) return false;
   g_ctx.device = device;
   return true;
 }
+
+Ctx *ctx_ready() { return g_init_done && g_ctx.device >= 0 ? &g_ctx : nullptr; }
 
 Ctx *ctx() {
   if (g_init_done) return g_ctx.device >= 0 ? &g_ctx : nullptr;

@@ -501,6 +501,44 @@ void mp_vec_norm(int64_t x, int64_t ntype, int64_t z, int64_t n) {
   if (ntype == 4) DP(z)[1] = z1;
 }
 void mp_vec_dot(int64_t x, int64_t y, int64_t val, int64_t n) { *DP(val) = dot_host(CDP(x), CDP(y), n); }
+
+// mp_vec_dot's kernels over the fixed scratch array instead of the grown
+// partials, the result left in device memory: nothing is allocated, nothing
+// waits.  See include/mpk.h.
+int64_t mpk_vec_dot_noalloc(int64_t x, int64_t y, int64_t val, int64_t n) {
+  Ctx *c = ctx_ready();  // never initialises: initialising allocates
+  if (!c) {
+    set_error(4, "mpk_vec_dot_noalloc: the library is not initialised (call mpk_init first)");
+    return -1;
+  }
+  if (!c->fixed) {
+    set_error(4, "mpk_vec_dot_noalloc: no fixed scratch on this device");
+    return -1;
+  }
+  if (n < 0 || !val) {
+    set_error(4, "mpk_vec_dot_noalloc: n must be >= 0 and val non-null");
+    return -1;
+  }
+  const RedSet<1> rs{{{CDP(x), CDP(y), RK_DOT}}};
+  if (red_ref()) {
+    const RefJob J = ref_job<1>(rs);
+    if (failed()) return -1;
+    ref_launch(J, n, c->fixed, nullptr);
+    MPK_LAUNCH(k_finalize<NoEpilogue>, 1, MPK_NT, (const double *)c->fixed, 1, 1, DP(val), NoEpilogue{},
+               nullptr);
+    return failed() ? -1 : 0;
+  }
+  const unsigned T = n > 0 ? ntiles(n) : 1u;
+  if ((size_t)T > c->fixed_cap) {
+    set_error(4, "mpk_vec_dot_noalloc: n = %lld needs %u partials, more than the %zu of the fixed scratch "
+              "(at most %lld elements)", (long long)n, T, c->fixed_cap, (long long)c->fixed_cap * MPK_TILE);
+    return -1;
+  }
+  MPK_LAUNCH(k_partials<1>, T, MPK_NT, rs, n, c->fixed, nullptr);
+  MPK_LAUNCH(k_finalize<NoEpilogue>, 1, MPK_NT, (const double *)c->fixed, (int)T, 1, DP(val), NoEpilogue{},
+             nullptr);
+  return failed() ? -1 : 0;
+}
 void mp_vec_tdot(int64_t x, int64_t y, int64_t val, int64_t n) { *DP(val) = dot_host(CDP(y), CDP(x), n); }
 void mp_vec_dot_norm2(int64_t x, int64_t y, int64_t sval, int64_t rval, int64_t n) {
   RedSet<2> rs{{{CDP(x), CDP(y), RK_DOT}, {CDP(x), CDP(x), RK_SUMSQ}}};

@@ -34,6 +34,7 @@ SIGNATURES = {
     "mp_vec_pointwise_max_abs": ([I, I, I, I], None),
     "mp_vec_norm": ([I, I, I, I], None),
     "mp_vec_dot": ([I, I, I, I], None),
+    "mpk_vec_dot_noalloc": ([I, I, I, I], I),
     "mp_vec_tdot": ([I, I, I, I], None),
     "mp_vec_sum": ([I, I, I], None),
     "mp_vec_mean": ([I, I, I], None),
