@@ -217,13 +217,16 @@ __global__ void __launch_bounds__(MPK_NT) k_finalize(const double *P, int T, int
 //   not); the last elements go serially into a scalar s from +0.0; the result
 //   is s + ((acc_0 + acc_2) + (acc_1 + acc_3)), Mojo's reduce_add over four
 //   lanes.
-// A chain is a dependency sequence, so one workgroup runs the 16 chains of up
-// to four reductions in 16 lanes each of its first wave, from chunks of the
-// operands that all 1024 threads stage through LDS (double-buffered: the
-// loads of chunk c + 1 are in flight while the chains consume chunk c).  The
-// sums and abs-sums use fma(v, 1.0, acc), which is acc + v exactly, so every
-// lane runs the same instruction.
-#define MPK_REF_NT 1024
+// A chain is a dependency sequence, so one workgroup runs them all: wave k
+// runs the 16 chains of reduction k in its first 16 lanes (up to four
+// reductions, on the four SIMDs), from chunks of the operands that all 512
+// threads stage through LDS (eight 16-byte loads each), double-buffered: the loads of chunk c + 1 are in
+// flight while the chains consume chunk c.  In LDS a chunk of an operand is
+// stored transposed, chain c's elements in row c, so that a lane reads two
+// consecutive steps of its chain with one 16-byte load; the row length is
+// chosen so the 16 lanes' loads fall on distinct banks.
+#define MPK_REF_NT 512
+#define MPK_REF_IT 8  // 16-byte items per thread per chunk
 #define MPK_REF_BUF 8192  // doubles in one chunk buffer (two buffers: 128 KB of LDS)
 
 struct RefJob {
@@ -235,55 +238,125 @@ struct RefJob {
   int al;                        // every operand 16-byte aligned
 };
 
-// The per-element factors of a chain step: acc = fma(f, g, acc).
-__device__ __forceinline__ void ref_terms(int kind, double x, double y, double &f, double &g) {
-  f = kind == RK_ASUM ? fabs(x) : x;
-  g = kind == RK_DOT ? y : (kind == RK_SUMSQ ? x : 1.0);
+// Operand of index i (i < 4) without dynamic indexing.
+__device__ __forceinline__ const double *ref_vec(const RefJob &J, int i) {
+  const double *p = J.v[0];
+  p = i == 1 ? J.v[1] : p;
+  p = i == 2 ? J.v[2] : p;
+  p = i == 3 ? J.v[3] : p;
+  return p;
 }
 
-// Q double2 loads per thread and operand per chunk: Q * nvec <= 4, a chunk
-// holds PV = 2048 Q elements of each operand (128 Q blocks of 16).
-template <int Q>
+// One chain step: acc = fma(f, g, acc) with (f, g) = (x, y) for a dot, (x, x)
+// for a sum of squares, (x, 1) for a sum and (|x|, 1) for an abs-sum.
+template <int KIND>
+__device__ __forceinline__ double ref_step(double x, double y, double acc) {
+  if constexpr (KIND == RK_DOT) return fma(x, y, acc);
+  if constexpr (KIND == RK_SUMSQ) return fma(x, x, acc);
+  if constexpr (KIND == RK_ASUM) return fma(fabs(x), 1.0, acc);
+  return fma(x, 1.0, acc);
+}
+__device__ __forceinline__ double ref_step_rt(int kind, double x, double y, double acc) {
+  switch (kind) {
+    case RK_DOT: return ref_step<RK_DOT>(x, y, acc);
+    case RK_SUMSQ: return ref_step<RK_SUMSQ>(x, y, acc);
+    case RK_ASUM: return ref_step<RK_ASUM>(x, y, acc);
+    default: return ref_step<RK_SUM>(x, y, acc);
+  }
+}
+
+// nblk steps of one chain from its LDS rows xr, yr (16-byte aligned); the
+// next eight steps' loads are issued before the current eight are summed.
+template <int KIND>
+__device__ __forceinline__ double ref_chain(const double *xr, const double *yr, int nblk, double acc) {
+  int b = 0;
+  double2 xa[4], ya[4];
+  if (nblk >= 8)
+#pragma unroll
+    for (int u = 0; u < 4; u++) {
+      xa[u] = *reinterpret_cast<const double2 *>(xr + 2 * u);
+      if constexpr (KIND == RK_DOT) ya[u] = *reinterpret_cast<const double2 *>(yr + 2 * u);
+    }
+  for (; b + 8 <= nblk; b += 8) {
+    double2 xb[4], yb[4];
+    const bool more = b + 16 <= nblk;
+    if (more)
+#pragma unroll
+      for (int u = 0; u < 4; u++) {
+        xb[u] = *reinterpret_cast<const double2 *>(xr + b + 8 + 2 * u);
+        if constexpr (KIND == RK_DOT) yb[u] = *reinterpret_cast<const double2 *>(yr + b + 8 + 2 * u);
+      }
+#pragma unroll
+    for (int u = 0; u < 4; u++) {
+      double y0 = 0.0, y1 = 0.0;
+      if constexpr (KIND == RK_DOT) y0 = ya[u].x, y1 = ya[u].y;
+      acc = ref_step<KIND>(xa[u].x, y0, acc);
+      acc = ref_step<KIND>(xa[u].y, y1, acc);
+    }
+    if (more)
+#pragma unroll
+      for (int u = 0; u < 4; u++) {
+        xa[u] = xb[u];
+        if constexpr (KIND == RK_DOT) ya[u] = yb[u];
+      }
+  }
+  for (; b < nblk; b++) acc = ref_step<KIND>(xr[b], KIND == RK_DOT ? yr[b] : 0.0, acc);
+  return acc;
+}
+
 __device__ __forceinline__ void ref_run(const RefJob J, i64 n, double *out) {
-  __shared__ double lds[2 * MPK_REF_BUF];
+  __shared__ __align__(16) double lds[2 * MPK_REF_BUF];
   __shared__ double chv[4][16];
-  constexpr int PV = 2048 * Q, CB = PV / 16, NVMAX = 4 / Q;
-  const int t = (int)threadIdx.x, lane = t & 63;
+  const int t = (int)threadIdx.x, lane = t & 63, wave = t >> 6;
+  const int nvec = J.nvec;
+  // blocks per chunk = the row length, in doubles: nvec * 16 * cb <=
+  // MPK_REF_BUF (so also nvec * 8 * cb <= MPK_REF_IT * MPK_REF_NT), and
+  // cb = 2 mod 32 so that row c starts 4 c dwords (mod 64) into the banks and
+  // the 16 lanes' 16-byte reads hit 64 distinct banks
+  const int cb = ((MPK_REF_BUF / 16 / nvec - 2) & ~31) + 2, rw = cb, per = 8 * cb, items = nvec * per;
   const i64 nb = n / 16, lim = 16 * nb;
-  const i64 nchunks = (nb + CB - 1) / CB;
-  const bool chain = t < 64 && lane < 16 * J.nspec;
-  const int k = lane >> 4, c = lane & 15;
-  // (shifts, not an indexed table: the job may sit in registers)
-  const unsigned code = (J.code >> (8 * k)) & 0xffu;
+  const i64 nchunks = (nb + cb - 1) / cb;
+  const bool chain = wave < J.nspec && lane < 16;
+  const int c = lane & 15;
+  const unsigned code = (J.code >> (8 * (wave & 3))) & 0xffu;
   const int xi = (int)(code & 3u), yi = (int)((code >> 2) & 3u), kind = (int)(code >> 4);
-  double2 reg[4];
+  // each thread's MPK_REF_IT 16-byte items of a chunk, fixed across chunks:
+  // the operand, the pair's place in the chunk and its LDS offset
+  const double *src[MPK_REF_IT];
+  int el[MPK_REF_IT], roff[MPK_REF_IT];
+#pragma unroll
+  for (int j = 0; j < MPK_REF_IT; j++) {
+    const int idx = t + MPK_REF_NT * j;
+    const int vi = idx < items ? idx / per : 0, off = idx - vi * per;
+    el[j] = idx < items ? 2 * off : -1;  // chain el % 16 (even), block el / 16
+    roff[j] = vi * 16 * rw + ((2 * off) & 15) * rw + ((2 * off) >> 4);
+    src[j] = ref_vec(J, vi);
+  }
+  double2 reg[MPK_REF_IT];
   auto load = [&](i64 chunk) {
 #pragma unroll
-    for (int vi = 0; vi < NVMAX; vi++)
-#pragma unroll
-      for (int m = 0; m < Q; m++) {
-        double2 r = make_double2(0.0, 0.0);
-        const i64 e = chunk * PV + 2 * (i64)(t + MPK_REF_NT * m);
-        if (vi < J.nvec && e < lim) {  // e even and lim a multiple of 16: e + 1 < lim too
-          const double *p = J.v[vi] + e;
-          if (J.al) {
-            r = *reinterpret_cast<const double2 *>(p);
-          } else {
-            r.x = p[0];
-            r.y = p[1];
-          }
+    for (int j = 0; j < MPK_REF_IT; j++) {
+      double2 r = make_double2(0.0, 0.0);
+      const i64 e = chunk * 16 * cb + el[j];
+      if (el[j] >= 0 && e < lim) {  // e even and lim a multiple of 16: e + 1 < lim too
+        const double *p = src[j] + e;
+        if (J.al) {
+          r = *reinterpret_cast<const double2 *>(p);
+        } else {
+          r.x = p[0];
+          r.y = p[1];
         }
-        reg[vi * Q + m] = r;
       }
+      reg[j] = r;
+    }
   };
   auto store = [&](int buf) {
 #pragma unroll
-    for (int vi = 0; vi < NVMAX; vi++)
-#pragma unroll
-      for (int m = 0; m < Q; m++)
-        if (vi < J.nvec)
-          *reinterpret_cast<double2 *>(lds + buf * MPK_REF_BUF + vi * PV + 2 * (t + MPK_REF_NT * m)) =
-              reg[vi * Q + m];
+    for (int j = 0; j < MPK_REF_IT; j++)
+      if (el[j] >= 0) {
+        lds[buf * MPK_REF_BUF + roff[j]] = reg[j].x;
+        lds[buf * MPK_REF_BUF + roff[j] + rw] = reg[j].y;  // chain el % 16 + 1
+      }
   };
   double acc = 0.0;
   if (nchunks > 0) {
@@ -295,35 +368,21 @@ __device__ __forceinline__ void ref_run(const RefJob J, i64 n, double *out) {
     const int buf = (int)(ch & 1);
     if (ch + 1 < nchunks) load(ch + 1);
     if (chain) {
-      const double *xs = lds + buf * MPK_REF_BUF + xi * PV + c;
-      const double *ys = lds + buf * MPK_REF_BUF + yi * PV + c;
-      const i64 left = nb - ch * CB;
-      const int nblk = left < CB ? (int)left : CB;
-      int b = 0;
-      for (; b + 8 <= nblk; b += 8) {
-        double xv[8], yv[8];
-#pragma unroll
-        for (int u = 0; u < 8; u++) {
-          xv[u] = xs[16 * (b + u)];
-          yv[u] = ys[16 * (b + u)];
-        }
-#pragma unroll
-        for (int u = 0; u < 8; u++) {
-          double f, g;
-          ref_terms(kind, xv[u], yv[u], f, g);
-          acc = fma(f, g, acc);
-        }
-      }
-      for (; b < nblk; b++) {
-        double f, g;
-        ref_terms(kind, xs[16 * b], ys[16 * b], f, g);
-        acc = fma(f, g, acc);
+      const double *xr = lds + buf * MPK_REF_BUF + xi * 16 * rw + c * rw;
+      const double *yr = lds + buf * MPK_REF_BUF + yi * 16 * rw + c * rw;
+      const i64 left = nb - ch * cb;
+      const int nblk = left < cb ? (int)left : cb;
+      switch (kind) {  // uniform in the wave
+        case RK_DOT: acc = ref_chain<RK_DOT>(xr, yr, nblk, acc); break;
+        case RK_SUMSQ: acc = ref_chain<RK_SUMSQ>(xr, yr, nblk, acc); break;
+        case RK_ASUM: acc = ref_chain<RK_ASUM>(xr, yr, nblk, acc); break;
+        default: acc = ref_chain<RK_SUM>(xr, yr, nblk, acc); break;
       }
     }
     if (ch + 1 < nchunks) store(buf ^ 1);
     __syncthreads();
   }
-  if (chain) chv[k][c] = acc;
+  if (chain) chv[wave][c] = acc;
   __syncthreads();
   // the tail of each reduction, in thread 0 (at most 15 elements each)
   if (t == 0) {
@@ -339,27 +398,15 @@ __device__ __forceinline__ void ref_run(const RefJob J, i64 n, double *out) {
       if (J.swsteps)
         for (; i + 4 <= n; i += 4)
 #pragma unroll
-          for (int l = 0; l < 4; l++) {
-            double f, g;
-            ref_terms(tkind, x[i + l], y[i + l], f, g);
-            a4[l] = fma(f, g, a4[l]);
-          }
+          for (int l = 0; l < 4; l++) a4[l] = ref_step_rt(tkind, x[i + l], y[i + l], a4[l]);
       double s = 0.0;
-      for (; i < n; i++) {
-        double f, g;
-        ref_terms(tkind, x[i], y[i], f, g);
-        s = fma(f, g, s);
-      }
+      for (; i < n; i++) s = ref_step_rt(tkind, x[i], y[i], s);
       out[r] = s + ((a4[0] + a4[2]) + (a4[1] + a4[3]));
     }
   }
 }
 
-template <int Q>
-__global__ void __launch_bounds__(MPK_REF_NT) k_ref_red(RefJob J, i64 n, double *P, const int64_t *stop) {
-  if (stop && *stop) return;
-  ref_run<Q>(J, n, P);
-}
+__global__ void __launch_bounds__(MPK_REF_NT) k_ref_red(RefJob J, i64 n, double *P, const int64_t *stop);
 
 // x . V_j, j < nv, in _mdot_basis's order: block b takes rows 3b .. 3b + 2
 // (a row's result does not depend on how the rows are grouped).

@@ -287,13 +287,13 @@ void vmaxpy_dev(double *y, const double *base, i64 ld, const double *coef_dev, i
 
 // ------------------------------------------------------------ the reference's order
 
+__global__ void __launch_bounds__(MPK_REF_NT) k_ref_red(RefJob J, i64 n, double *P, const int64_t *stop) {
+  if (stop && *stop) return;
+  ref_run(J, n, P);
+}
+
 void ref_launch(const RefJob &J, i64 n, double *P, const int64_t *stop) {
-  if (J.nvec <= 1)
-    MPK_LAUNCH(k_ref_red<4>, 1, MPK_REF_NT, J, n, P, stop);
-  else if (J.nvec == 2)
-    MPK_LAUNCH(k_ref_red<2>, 1, MPK_REF_NT, J, n, P, stop);
-  else
-    MPK_LAUNCH(k_ref_red<1>, 1, MPK_REF_NT, J, n, P, stop);
+  MPK_LAUNCH(k_ref_red, 1, MPK_REF_NT, J, n, P, stop);
 }
 
 __global__ void __launch_bounds__(MPK_REF_NT) k_ref_mdot(const double *x, const double *V, i64 ldv, int nv, i64 n,
@@ -317,7 +317,7 @@ __global__ void __launch_bounds__(MPK_REF_NT) k_ref_mdot(const double *x, const 
     J.sx[r] = x;
     J.sy[r] = J.v[r < 3 ? 1 + r : 3];
   }
-  ref_run<1>(J, n, P + j0);
+  ref_run(J, n, P + j0);
 }
 
 // ------------------------------------------------------------ reductions to host
