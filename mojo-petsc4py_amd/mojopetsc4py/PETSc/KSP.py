@@ -11,6 +11,7 @@ memory.
 from __future__ import annotations
 
 import math
+from dataclasses import dataclass
 
 import numpy as np
 
@@ -97,6 +98,60 @@ class GMRESOrthogonalizationType:
     MGS = 1
 
 
+@dataclass
+class _KSPOptions:
+    """What the solver's setters record; the defaults are PETSc's (KSP.py:
+    25-29, 123-158 of the reference)."""
+
+    ksp_type: str = KSPType.GMRES
+    side: int = 0               # PC side: left
+    normtype: int = NormType_.NORM_DEFAULT
+    rtol: float = DEFAULT_RTOL
+    atol: float = DEFAULT_ATOL
+    divtol: float = DEFAULT_DTOL
+    max_it: int = DEFAULT_MAX_IT
+    min_it: int = 0
+    chknorm: int = -1
+    convmaxits: int = 0         # set by setTolerances(max_it=...): max_it then reports CONVERGED_ITS
+    guess_zero: int = 1
+    restart: int = GMRES_DEFAULT_MAXK
+    haptol: float = 1.0e-30
+    breakdowntol: float = 0.1
+    orthog: int = GMRESOrthogonalizationType.CLASSICAL
+    cgstype: int = GMRESCGSRefinementType.REFINE_NEVER
+
+
+# The scalar blocks the device drivers read (src/ksp.mojo:9-42): slot -> option.
+_CFG_SLOTS = ((0, "rtol"), (1, "atol"), (2, "divtol"), (3, "haptol"), (4, "breakdowntol"))
+_CFGI_SLOTS = ((0, "normtype"), (1, "side"), (2, "guess_zero"), (3, "min_it"), (4, "chknorm"),
+               (5, "convmaxits"), (6, "max_it"), (7, "restart"), (8, "orthog"), (9, "cgstype"))
+
+
+def _setter(field, convert, resets_setup=False):
+    def set_option(self, value):
+        setattr(self._opt, field, convert(value))
+        if resets_setup:
+            self._setupcalled = False
+
+    return set_option
+
+
+def _getter(field):
+    def get_option(self):
+        return getattr(self._opt, field)
+
+    return get_option
+
+
+class _Result:
+    """What the last solve reported."""
+
+    __slots__ = ("its", "rnorm", "reason")
+
+    def __init__(self):
+        self.its, self.rnorm, self.reason = 0, 0.0, ConvergedReason.CONVERGED_ITERATING
+
+
 class KSP:
     """A Krylov solver for a sequential AIJ matrix, iterating on the GPU."""
 
@@ -107,34 +162,14 @@ class KSP:
     GMRESOrthogonalizationType = GMRESOrthogonalizationType
 
     def __init__(self):
-        self._type = KSPType.GMRES
-        self._amat = None
-        self._pmat = None
+        self._opt = _KSPOptions()
+        self._amat = self._pmat = None
         self._pc = PC()
-        self._pc_side = 0
-        self._normtype = 0
-        self._rtol = DEFAULT_RTOL
-        self._atol = DEFAULT_ATOL
-        self._divtol = DEFAULT_DTOL
-        self._max_it = DEFAULT_MAX_IT
-        self._min_it = 0
-        self._chknorm = -1
-        self._convmaxits = 0
-        self._guess_zero = 1
-        self._restart = GMRES_DEFAULT_MAXK
-        self._haptol = 1.0e-30
-        self._breakdowntol = 0.1
-        self._orthog = 0
-        self._cgstype = 0
-        self._its = 0
-        self._rnorm = 0.0
-        self._reason = 0
-        self._history = None
-        self._histlen = 0
-        self._rhs = None
-        self._sol = None
+        self._last = _Result()
+        self._history = None  # the caller's history buffer (setConvergenceHistory)
+        self._rhs = self._sol = None
         self._setupcalled = False
-        self._arena = {}
+        self._arena = {}      # device work blocks, reused across solves
 
     # ---------------------------------------------------------- lifecycle
     def create(self, comm=None) -> "KSP":
@@ -151,15 +186,17 @@ class KSP:
         if ksp_type not in SUPPORTED:
             raise NotImplementedError(
                 f"KSP type {ksp_type!r} is not covered by mojo-petsc4py ({sorted(SUPPORTED)} are)")
-        self._type = ksp_type
+        self._opt.ksp_type = ksp_type
         self._setupcalled = False
 
-    def getType(self) -> str:
-        return self._type
+    getType = _getter("ksp_type")
+
+    @property
+    def _type(self):
+        return self._opt.ksp_type
 
     def setOperators(self, A: Mat = None, P: Mat = None) -> None:
-        self._amat = A
-        self._pmat = A if P is None else P
+        self._amat, self._pmat = A, (A if P is None else P)
         self._pc.setOperators(self._amat, self._pmat)
         self._setupcalled = False
 
@@ -169,91 +206,72 @@ class KSP:
     def setPC(self, pc: PC) -> None:
         self._pc = pc
         if self._pmat is not None:
-            self._pc.setOperators(self._amat, self._pmat)
+            pc.setOperators(self._amat, self._pmat)
         self._setupcalled = False
 
     def getPC(self) -> PC:
         return self._pc
 
-    def setPCSide(self, side) -> None:
-        self._pc_side = int(side)
+    def setUp(self) -> None:
+        """Set up the preconditioner (the drivers need nothing else)."""
+        if self._amat is None:
+            raise RuntimeError("KSP.setOperators() has not been called")
+        self._pc.setUp()
+        self._setupcalled = True
 
-    def getPCSide(self):
-        return self._pc_side
-
-    def setNormType(self, normtype) -> None:
-        self._normtype = int(normtype)
-
-    def getNormType(self):
-        return self._normtype
+    # ---------------------------------------------------------- options
+    setPCSide = _setter("side", int)
+    getPCSide = _getter("side")
+    setNormType = _setter("normtype", int)
+    getNormType = _getter("normtype")
+    setMinimumIterations = _setter("min_it", int)
+    getMinimumIterations = _getter("min_it")
+    setCheckNorm = _setter("chknorm", int)
+    setGMRESRestart = _setter("restart", int, resets_setup=True)
+    setGMRESHapTol = _setter("haptol", float)
+    setGMRESBreakdownTolerance = _setter("breakdowntol", float)
+    setGMRESOrthogonalization = _setter("orthog", int)
+    setGMRESCGSRefinementType = _setter("cgstype", int)
 
     def setTolerances(self, rtol=None, atol=None, divtol=None, max_it=None) -> None:
-        """A max_it also makes reaching it CONVERGED_ITS, as in the
-        reference (KSP.py:212-225)."""
-        if rtol is not None:
-            self._rtol = float(rtol)
-        if atol is not None:
-            self._atol = float(atol)
-        if divtol is not None:
-            self._divtol = float(divtol)
+        """Set any of the four; giving max_it also makes reaching it
+        CONVERGED_ITS, as in the reference (KSP.py:212-225)."""
+        given = {"rtol": rtol, "atol": atol, "divtol": divtol}
+        for name, value in given.items():
+            if value is not None:
+                setattr(self._opt, name, float(value))
         if max_it is not None:
-            self._max_it = int(max_it)
-            self._convmaxits = 1
+            self._opt.max_it = int(max_it)
+            self._opt.convmaxits = 1
 
     def getTolerances(self):
-        return (self._rtol, self._atol, self._divtol, self._max_it)
-
-    def setMinimumIterations(self, min_it) -> None:
-        self._min_it = int(min_it)
-
-    def getMinimumIterations(self) -> int:
-        return self._min_it
-
-    def setCheckNorm(self, flag) -> None:
-        self._chknorm = int(flag)
+        o = self._opt
+        return (o.rtol, o.atol, o.divtol, o.max_it)
 
     def setInitialGuessNonzero(self, flag) -> None:
-        self._guess_zero = 0 if flag else 1
+        self._opt.guess_zero = 0 if flag else 1
 
     def getInitialGuessNonzero(self) -> bool:
-        return not self._guess_zero
-
-    def setGMRESRestart(self, restart) -> None:
-        self._restart = int(restart)
-        self._setupcalled = False
-
-    def setGMRESHapTol(self, haptol) -> None:
-        self._haptol = float(haptol)
-
-    def setGMRESBreakdownTolerance(self, tol) -> None:
-        self._breakdowntol = float(tol)
-
-    def setGMRESOrthogonalization(self, orthog) -> None:
-        self._orthog = int(orthog)
-
-    def setGMRESCGSRefinementType(self, cgstype) -> None:
-        self._cgstype = int(cgstype)
+        return self._opt.guess_zero == 0
 
     def setConvergenceHistory(self, length: int | None = None, reset: bool = False) -> None:
-        size = 10000 if length is None or length is True else int(length)
-        if size < 0:
-            size = 10000
-        self._histlen = size
+        """Record the residual norms of the next solves (10000 entries unless
+        a positive length is given)."""
+        size = int(length) if (length is not None and length is not True and int(length) >= 0) else 10000
         self._history = np.zeros(size, dtype=np.float64)
 
     def getConvergenceHistory(self) -> np.ndarray:
-        if self._history is None:
-            return np.zeros(0, dtype=np.float64)
-        return self._history
+        return np.zeros(0, dtype=np.float64) if self._history is None else self._history
 
+    # the last solve's outcome, and its operands
     def getIterationNumber(self) -> int:
-        return self._its
+        return self._last.its
 
     def getResidualNorm(self) -> float:
-        return self._rnorm
+        return self._last.rnorm
 
     def getConvergedReason(self):
-        return self._reason
+        return self._last.reason
 
     def getRhs(self) -> Vec:
         return self._rhs
@@ -261,20 +279,13 @@ class KSP:
     def getSolution(self) -> Vec:
         return self._sol
 
-    def setUp(self) -> None:
-        if self._amat is None:
-            raise RuntimeError("KSP.setOperators() has not been called")
-        self._pc.setUp()
-        self._setupcalled = True
-
     # ---------------------------------------------------------- scratch
     def _dev(self, name: str, ndoubles: int) -> int:
         """A device arena of at least `ndoubles`, reused across solves."""
-        buf = self._arena.get(name)
         need = 8 * max(int(ndoubles), 1)
+        buf = self._arena.get(name)
         if buf is None or buf.nbytes < need:
-            buf = DeviceBuffer(need)
-            self._arena[name] = buf
+            buf = self._arena[name] = DeviceBuffer(need)
         return buf.ptr
 
     # ---------------------------------------------------------- solve
@@ -290,76 +301,55 @@ class KSP:
             raise ValueError(f"incompatible vector size: {n} is needed for both operands")
         if not self._setupcalled:
             self.setUp()
-        if self._guess_zero:
+        if self._opt.guess_zero:
             x.zeroEntries()
         self._pc.setUp()
-        self._rhs = b
-        self._sol = x
-        self._resolve_normtype()
+        self._rhs, self._sol = b, x
+        if self._opt.normtype == NormType_.NORM_DEFAULT:  # resolved per solve (KSP.py:451-455)
+            self._opt.normtype = NormType_.NORM_PRECONDITIONED
         cfg, cfgi = self._state(n)
         d = self._amat._dev()
         s = d.sell()  # None: the solver keeps the CSR row kernel
         if s is not None:
             cfgi[13], cfgi[14], cfgi[15] = s[0].ptr, s[1].ptr, s[2].ptr
-        pci = self._pc._pci_block()
-        pcf = self._pc._pcf_block()
+        pci, pcf = self._pc._pci_block(), self._pc._pcf_block()
+        mat = (d.rp.ptr, d.ci.ptr, d.v.ptr, pci.ctypes.data, pcf.ctypes.data, 0)
+        ends = (cfg.ctypes.data, cfgi.ctypes.data, b._d(), x._dw())
         L = lib()
-        b_addr = b._d()
-        x_addr = x._dw()
-        if self._type in (KSPType.GMRES, KSPType.FGMRES):
-            mk = self._restart
+        kind = self._opt.ksp_type
+        if kind in (KSPType.GMRES, KSPType.FGMRES):
+            mk = self._opt.restart
             hs = np.zeros((mk + 2) * (mk + 1) + (mk + 1) * (mk + 1) + 2 * (mk + 1) + 2 * (mk + 2))
-            w = self._dev("work", 2 * n)
-            vv = self._dev("vv", (mk + 1) * n)
-            if self._type == KSPType.GMRES:
-                L.mp_ksp_solve_gmres(n, d.rp.ptr, d.ci.ptr, d.v.ptr, pci.ctypes.data, pcf.ctypes.data, 0, w,
-                                     vv, hs.ctypes.data, cfg.ctypes.data, cfgi.ctypes.data, b_addr, x_addr)
+            basis = (self._dev("work", 2 * n), self._dev("vv", (mk + 1) * n))
+            if kind == KSPType.GMRES:
+                L.mp_ksp_solve_gmres(n, *mat, *basis, hs.ctypes.data, *ends)
             else:
-                pv = self._dev("prevec", (mk + 2) * n)
-                L.mp_ksp_solve_fgmres(n, d.rp.ptr, d.ci.ptr, d.v.ptr, pci.ctypes.data, pcf.ctypes.data, 0, w,
-                                      vv, pv, hs.ctypes.data, cfg.ctypes.data, cfgi.ctypes.data, b_addr, x_addr)
+                L.mp_ksp_solve_fgmres(n, *mat, *basis, self._dev("prevec", (mk + 2) * n), hs.ctypes.data, *ends)
         else:
-            w = self._dev("work", 7 * n)
-            fn = L.mp_ksp_solve_cg if self._type == KSPType.CG else L.mp_ksp_solve_bcgs
-            fn(n, d.rp.ptr, d.ci.ptr, d.v.ptr, pci.ctypes.data, pcf.ctypes.data, 0, w, cfg.ctypes.data,
-               cfgi.ctypes.data, b_addr, x_addr)
+            solver = L.mp_ksp_solve_cg if kind == KSPType.CG else L.mp_ksp_solve_bcgs
+            solver(n, *mat, self._dev("work", 7 * n), *ends)
         check()
-        self._its = int(cfgi[10])
-        self._reason = int(cfgi[11])
-        self._rnorm = float(cfg[7])
-        full = np.zeros(self._histlen, dtype=np.float64)
-        k = min(len(full), len(cfg) - 9)
-        full[:k] = cfg[9:9 + k]
-        self._history = full
-
-    def _resolve_normtype(self):
-        if self._normtype == NormType_.NORM_DEFAULT:
-            self._normtype = NormType_.NORM_PRECONDITIONED
+        self._last.its, self._last.reason, self._last.rnorm = int(cfgi[10]), int(cfgi[11]), float(cfg[7])
+        if self._history is not None:  # a new array per solve, as the reference hands back
+            fresh = np.zeros(len(self._history), dtype=np.float64)
+            k = min(len(fresh), len(cfg) - 9)
+            fresh[:k] = cfg[9:9 + k]
+            self._history = fresh
 
     def _state(self, n: int):
-        """The scalar blocks of ksp.mojo:9-42 (KSP.py:457-494)."""
-        size = 16 + self._max_it + 2
-        cfg = np.zeros(size, dtype=np.float64)
-        cfg[0] = self._rtol
-        cfg[1] = self._atol
-        cfg[2] = self._divtol
-        cfg[3] = self._haptol
-        cfg[4] = self._breakdowntol
+        """cfg / cfgi as the drivers read them (ksp.mojo:9-42, KSP.py:457-494):
+        the options in their slots, the history from cfg[9], and the
+        right-hand-side norm and its flag for a nonzero guess."""
+        o = self._opt
+        cfg = np.zeros(16 + o.max_it + 2, dtype=np.float64)
         cfgi = np.zeros(16, dtype=np.int64)
-        cfgi[0] = self._normtype
-        cfgi[1] = self._pc_side
-        cfgi[2] = self._guess_zero
-        cfgi[3] = self._min_it
-        cfgi[4] = self._chknorm
-        cfgi[5] = self._convmaxits
-        cfgi[6] = self._max_it
-        cfgi[7] = self._restart
-        cfgi[8] = self._orthog
-        cfgi[9] = self._cgstype
-        # the right-hand-side norm seeds the test only for a nonzero guess
-        cfg[12] = 0.0 if self._guess_zero else self._rhs_norm()
-        cfg[14] = 0.0
-        cfgi[12] = 0 if self._guess_zero else 1
+        for slot, name in _CFG_SLOTS:
+            cfg[slot] = getattr(o, name)
+        for slot, name in _CFGI_SLOTS:
+            cfgi[slot] = getattr(o, name)
+        nonzero_guess = not o.guess_zero
+        cfg[12] = self._rhs_norm() if nonzero_guess else 0.0  # read only for a nonzero guess
+        cfgi[12] = int(nonzero_guess)
         return cfg, cfgi
 
     def _rhs_norm(self) -> float:
@@ -368,18 +358,17 @@ class KSP:
         reference reduction order this does too, so that its bits follow;
         in the tile-tree order they are the device norm."""
         b = self._rhs
-        if b is None or self._normtype == NormType_.NORM_NONE:
+        o = self._opt
+        if b is None or o.normtype == NormType_.NORM_NONE:
             return 0.0
 
         def two_norm(v):
-            if reduction_order() == "ref":
-                return float(np.linalg.norm(v.getArray()))
-            return v.norm()
+            return float(np.linalg.norm(v.getArray())) if reduction_order() == "ref" else v.norm()
 
-        if self._normtype == NormType_.UNPRECONDITIONED or self._pc_side == 1:
+        if o.normtype == NormType_.UNPRECONDITIONED or o.side == 1:
             return two_norm(b)
         z = b.duplicate()
         self._pc.apply(b, z)
-        if self._normtype == NormType_.PRECONDITIONED:
+        if o.normtype == NormType_.PRECONDITIONED:
             return two_norm(z)
         return math.sqrt(abs(z.dot(b)))

@@ -1,12 +1,15 @@
 """Sequential AIJ matrices, mirroring `petsc4py/PETSc/Mat.pyx` (the
-reference's python/mojopetsc4py/PETSc/Mat.py).
+reference's python/mojopetsc4py/PETSc/Mat.py): the same methods, sizes,
+defaults and errors.
 
-Assembly stays on the host exactly as in the reference: entries set one by
-one collect in a pending dict and become CSR (int64 row starts, int64
-columns ascending within a row, float64 values) on demand; `setValuesCSR`
-installs CSR directly.  The products run on the GPU over a device copy of the
-matrix with int32 row pointers and columns, uploaded on first use and again
-after any change to the host arrays.
+Host side: CSR arrays (row starts, columns ascending within a row, float64
+values) plus a log of the entry writes made since they were last assembled;
+assembly folds the log into new arrays with numpy (a stable sort by entry,
+then each entry's writes in call order).  `setValuesCSR` installs arrays
+directly and keeps int32 index arrays as they are.  Device side: the
+products run on the GPU over a copy with int32 row pointers and columns and a
+sliced copy (64-row slices) for coalesced loads, uploaded on first use and
+again after any change.
 """
 
 from __future__ import annotations
@@ -140,6 +143,9 @@ class _DevCSR:
                                            else np.zeros(1))
             padded = L.mpk_sell_size(trp_d.ptr, nc)
             check()
+            if padded > SELL_MAX_SLOTS:
+                self.tsellc = False
+                return None
             soff = DeviceBuffer(4 * ((nc + 63) // 64 + 1))
             sv = DeviceBuffer(8 * max(padded, 1))
             sc = DeviceBuffer(6 * max(padded, 1))  # int32 columns, then int16 offsets
@@ -147,11 +153,167 @@ class _DevCSR:
                 check()
             L.mpk_synchronize()
             self.tsellc = (trp_d, soff, sv, sc)
-        return self.tsellc
+        return self.tsellc or None
+
+
+# ---------------------------------------------------------------- host storage
+
+class _HostCSR:
+    """The host arrays of an assembled matrix: row starts, columns and values
+    with room for `len(values)` entries, of which the first `nnz` are used.
+    Index arrays keep the integer width they were handed in (int32 CSR is
+    uploaded as it is); `wide()` gives the int64 view the reference exposes."""
+
+    __slots__ = ("indptr", "indices", "values", "nnz", "_wide")
+
+    def __init__(self, indptr, indices, values, nnz):
+        self.indptr = indptr
+        self.indices = indices
+        self.values = values
+        self.nnz = int(nnz)
+        self._wide = None
+
+    @classmethod
+    def empty(cls, nrows: int, room: int = 1):
+        room = max(int(room), 1)
+        return cls(np.zeros(nrows + 1, np.int64), np.zeros(room, np.int64), np.zeros(room), 0)
+
+    def wide(self):
+        """(indptr, indices) as int64 arrays (cached until the arrays change)."""
+        if self._wide is None:
+            if self.indptr.dtype == np.int64 and self.indices.dtype == np.int64:
+                self._wide = (self.indptr, self.indices)
+            else:
+                self._wide = (self.indptr.astype(np.int64), self.indices.astype(np.int64))
+        return self._wide
+
+    def changed(self):
+        self._wide = None
+
+    def row_of_entries(self):
+        """The row of every used entry, in storage order."""
+        ip = self.indptr.astype(np.int64, copy=False)
+        return np.repeat(np.arange(len(ip) - 1, dtype=np.int64), np.diff(ip))
+
+    def find(self, row: int, col: int) -> int:
+        """Storage position of (row, col), or -1 (a binary search over the
+        row's columns, which assembly keeps ascending)."""
+        lo, hi = int(self.indptr[row]), int(self.indptr[row + 1])
+        k = lo + int(np.searchsorted(self.indices[lo:hi], col))
+        return k if k < hi and int(self.indices[k]) == col else -1
+
+
+class _WriteLog:
+    """Entry writes not folded into the CSR arrays yet, in call order: one
+    chunk per call of rows, columns, values and whether the call adds.  The
+    CSR entries present when the first write arrives count as writes before
+    all of them (`base`), so the fold's result is the one a per-entry dict
+    gives: an insert replaces, an add accumulates onto the entry's value so
+    far (from 0.0 when it has none), and entries keep whatever they last
+    became, zeros included."""
+
+    __slots__ = ("rows", "cols", "vals", "adds", "base")
+
+    def __init__(self, base: _HostCSR | None):
+        self.rows, self.cols, self.vals, self.adds = [], [], [], []
+        self.base = base
+
+    def push(self, rows, cols, vals, add: bool):
+        if len(rows):
+            self.rows.append(np.asarray(rows, np.int64))
+            self.cols.append(np.asarray(cols, np.int64))
+            self.vals.append(np.asarray(vals, np.float64))
+            self.adds.append(np.full(len(rows), bool(add)))
+
+    def _stream(self):
+        """Every write in order: (rows, cols, vals, adds), the base first."""
+        r, c, v, a = list(self.rows), list(self.cols), list(self.vals), list(self.adds)
+        if self.base is not None and self.base.nnz:
+            b = self.base
+            ip, ix = b.wide()
+            r.insert(0, b.row_of_entries())
+            c.insert(0, ix[: b.nnz].copy())
+            v.insert(0, b.values[: b.nnz].copy())
+            a.insert(0, np.zeros(b.nnz, bool))
+        if not r:
+            z = np.zeros(0, np.int64)
+            return z, z, np.zeros(0), np.zeros(0, bool)
+        return np.concatenate(r), np.concatenate(c), np.concatenate(v), np.concatenate(a)
+
+    def value(self, row: int, col: int) -> float:
+        """The entry's value as the fold would leave it (0.0 if never set)."""
+        acc = 0.0
+        b = self.base
+        if b is not None and b.nnz and 0 <= row < len(b.indptr) - 1:
+            lo, hi = int(b.indptr[row]), int(b.indptr[row + 1])
+            hit = np.flatnonzero(b.indices[lo:hi] == col)
+            if len(hit):
+                acc = float(b.values[lo + hit[-1]])
+        for r, c, v, a in zip(self.rows, self.cols, self.vals, self.adds):
+            for k in np.flatnonzero((r == row) & (c == col)):
+                acc = acc + float(v[k]) if a[k] else float(v[k])
+        return acc
+
+    def fold(self, nrows: int, ncols: int) -> _HostCSR:
+        """The CSR arrays after every write: rows in order, columns ascending,
+        one entry per (row, column) ever written."""
+        r, c, v, a = self._stream()
+        n = len(r)
+        if n == 0:
+            return _HostCSR.empty(nrows)
+        key = r * max(ncols, 1) + c
+        order = np.argsort(key, kind="stable")  # call order kept within a key
+        key, v, a = key[order], v[order], a[order]
+        r, c = r[order], c[order]
+        head = np.ones(n, bool)
+        head[1:] = key[1:] != key[:-1]
+        starts = np.flatnonzero(head)
+        ends = np.append(starts[1:], n)
+        out = v[starts].copy()
+        # an entry written once takes its value (an add onto 0.0 is the value
+        # itself); entries written several times fold their writes in order
+        first_add = a[starts]
+        out[first_add] = 0.0 + v[starts][first_add]
+        multi = np.flatnonzero(ends - starts > 1)
+        for g in multi:
+            acc = 0.0
+            for k in range(starts[g], ends[g]):
+                acc = acc + float(v[k]) if a[k] else float(v[k])
+            out[g] = acc
+        ur, uc = r[starts], c[starts]
+        indptr = np.zeros(nrows + 1, np.int64)
+        np.cumsum(np.bincount(ur, minlength=nrows), out=indptr[1:])
+        m = len(ur)
+        indices = np.zeros(max(m, 1), np.int64)
+        values = np.zeros(max(m, 1))
+        indices[:m] = uc
+        values[:m] = out
+        return _HostCSR(indptr, indices, values, m)
+
+
+def _shape_of(size):
+    if isinstance(size, (tuple, list)):
+        return int(size[0]), int(size[1])
+    return int(size), int(size)
+
+
+def _outer_writes(rows, cols, vals=None, value=None):
+    """The (row, col, value) writes of a rows x cols block in row-major order."""
+    rows = np.asarray(list(rows), np.int64).ravel()
+    cols = np.asarray(list(cols), np.int64).ravel()
+    rr = np.repeat(rows, len(cols))
+    cc = np.tile(cols, len(rows))
+    if vals is None:
+        vv = np.full(len(rr), float(value))
+    else:
+        vv = vals[: len(rr)]
+    return rr, cc, vv
 
 
 class Mat:
-    """A sequential AIJ (CSR) matrix."""
+    """A sequential AIJ (CSR) matrix: host CSR arrays (assembled on demand
+    from the writes made since the last fold) and a device copy for the
+    products."""
 
     Type = MatType
     InfoType = MatInfoType
@@ -160,51 +322,73 @@ class Mat:
     SORType = MatSORType
 
     def __init__(self):
-        self._irow = None
-        self._icol = None
-        self._data = None
-        self._nrows = 0
-        self._ncols = 0
+        self._m = 0
+        self._n = 0
+        self._csr: _HostCSR | None = None  # None: not created (or destroyed)
+        self._log: _WriteLog | None = None
         self._assembled = False
-        self._nnz_used = 0
-        self._pending = None
         self._devc = None
         self._version = 0
 
-    # ---------------------------------------------------------- device copy
+    # ---------------------------------------------------------- internal state
+    @property
+    def _nrows(self):
+        return self._m
+
+    @property
+    def _ncols(self):
+        return self._n
+
     def _touch(self):
-        """The host arrays changed: the device copy and any set up built on
-        it are stale."""
+        """The entries changed: the device copy and set ups built on it are stale."""
         self._devc = None
         self._version += 1
+        if self._csr is not None:
+            self._csr.changed()
+
+    def _materialize(self):
+        """Fold the pending writes into the CSR arrays."""
+        if self._log is None:
+            return
+        self._csr = self._log.fold(self._m, self._n)
+        self._log = None
+        self._touch()
+
+    def _writes(self) -> _WriteLog:
+        if self._log is None:
+            self._log = _WriteLog(self._csr)
+        return self._log
+
+    def _host(self) -> _HostCSR:
+        self._materialize()
+        return self._csr
 
     def _dev(self) -> _DevCSR:
-        self._materialize()
+        h = self._host()
         if self._devc is None:
-            if self._nrows:
-                cols = self._icol[: int(self._irow[self._nrows])]
-                if len(cols) and (cols.min() < 0 or cols.max() >= self._ncols):
+            if self._m:
+                cols = h.indices[: h.nnz]
+                if len(cols) and (int(cols.min()) < 0 or int(cols.max()) >= self._n):
                     raise ValueError("a column index lies outside the matrix")
-            self._devc = _DevCSR(self._irow, self._icol, self._data, self._nrows, self._ncols)
+            self._devc = _DevCSR(h.indptr, h.indices, h.values, self._m, self._n)
         return self._devc
+
+    def _check(self):
+        if self._csr is None:
+            raise RuntimeError("Mat has not been created")
+        return self
 
     # ---------------------------------------------------------- creation
     def create(self, comm=None) -> "Mat":
         return self
 
     def createSeqAIJ(self, size, bsize=None, nnz=None, comm=None) -> "Mat":
-        if isinstance(size, (tuple, list)):
-            m, n = int(size[0]), int(size[1])
-        else:
-            m = n = int(size)
-        per_row = 1 if nnz is None else int(nnz)
-        self._nrows = m
-        self._ncols = n
-        self._irow = np.arange(m + 1, dtype=np.int64) * per_row
-        self._icol = np.zeros(m * per_row, dtype=np.int64)
-        self._data = np.zeros(m * per_row, dtype=np.float64)
+        """An m x n matrix with room for `nnz` entries a row and none set."""
+        self._m, self._n = _shape_of(size)
+        room = self._m * (1 if nnz is None else int(nnz))
+        self._csr = _HostCSR.empty(self._m, room)
+        self._log = _WriteLog(None)  # nothing written yet: assembling gives no entries
         self._assembled = True
-        self._pending = {}
         self._touch()
         return self
 
@@ -212,73 +396,60 @@ class Mat:
         return self.createSeqAIJ(size, bsize, nnz, comm)
 
     def setSizes(self, size, bsize=None) -> "Mat":
-        if isinstance(size, (tuple, list)):
-            self._nrows, self._ncols = int(size[0]), int(size[1])
-        else:
-            self._nrows = self._ncols = int(size)
-        if self._irow is None:
-            self._irow = np.zeros(self._nrows + 1, dtype=np.int64)
-            self._icol = np.zeros(1, dtype=np.int64)
-            self._data = np.zeros(1, dtype=np.float64)
-            self._pending = {}
+        self._m, self._n = _shape_of(size)
+        if self._csr is None:
+            self._csr = _HostCSR.empty(self._m)
+            self._log = _WriteLog(None)
         return self
 
     def duplicate(self, copy: bool = False) -> "Mat":
-        self._materialize()
-        out = Mat()
-        out._nrows = self._nrows
-        out._ncols = self._ncols
-        out._assembled = self._assembled
-        out._nnz_used = self._nnz_used
-        out._irow = self._irow.copy()
-        out._icol = self._icol.copy()
-        out._data = self._data.copy() if copy else np.zeros_like(self._data)
-        return out
+        h = self._host()
+        twin = Mat()
+        twin._m, twin._n = self._m, self._n
+        twin._assembled = self._assembled
+        vals = h.values.copy() if copy else np.zeros_like(h.values)
+        twin._csr = _HostCSR(h.indptr.copy(), h.indices.copy(), vals, h.nnz)
+        return twin
 
     def copy(self, result=None, structure=None) -> "Mat":
-        """Copy into `result`; with DIFFERENT_NONZERO_PATTERN the target keeps
-        its pattern and takes the values of the entries it holds
-        (Mat.py:163-210 of the reference)."""
-        source = self
-        source._materialize()
-        target = source.duplicate() if result is None else result
+        """Copy into `result` (a duplicate when None).  With
+        DIFFERENT_NONZERO_PATTERN the target keeps its own pattern and takes
+        the source's value at each of its entries (0.0 where the source has
+        none), as MatCopy does (Mat.py:163-210 of the reference)."""
+        src = self._host()
+        if result is None:
+            result = self.duplicate()
         if structure == MatStructures.DIFFERENT_NONZERO_PATTERN:
-            target._materialize()
-            if (target._nrows, target._ncols) != (source._nrows, source._ncols):
-                raise ValueError(
-                    f"incompatible sizes: {source.getSize()} into {target.getSize()}")
-            src = {
-                (int(source._icol[j]), i): float(source._data[j])
-                for i in range(source._nrows)
-                for j in range(int(source._irow[i]), int(source._irow[i + 1]))
-            }
-            trow, tcol, _ = target.getValuesCSR()
-            newdata = np.zeros(len(tcol), dtype=np.float64)
-            for i in range(target._nrows):
-                for j in range(int(trow[i]), int(trow[i + 1])):
-                    newdata[j] = src.get((int(tcol[j]), i), 0.0)
-            target._data[: len(newdata)] = newdata
-            target._touch()
-            return target
-        target._nrows = source._nrows
-        target._ncols = source._ncols
-        target._irow = source._irow.copy()
-        target._icol = source._icol.copy()
-        target._data = source._data.copy()
-        target._nnz_used = source._nnz_used
-        target._assembled = source._assembled
-        target._pending = None
-        target._touch()
-        return target
+            dst = result._host()
+            if (result._m, result._n) != (self._m, self._n):
+                raise ValueError(f"incompatible sizes: {self.getSize()} into {result.getSize()}")
+            width = max(self._n, 1)
+            skey = src.row_of_entries() * width + src.wide()[1][: src.nnz]
+            order = np.argsort(skey, kind="stable")
+            skey, sval = skey[order], src.values[: src.nnz][order]
+            last = np.ones(len(skey), bool)  # the last of equal keys wins
+            last[:-1] = skey[1:] != skey[:-1]
+            skey, sval = skey[last], sval[last]
+            dkey = dst.row_of_entries() * width + dst.wide()[1][: dst.nnz]
+            at = np.searchsorted(skey, dkey)
+            hit = at < len(skey)
+            hit[hit] = skey[at[hit]] == dkey[hit]
+            fresh = np.zeros(dst.nnz)
+            fresh[hit] = sval[at[hit]]
+            dst.values[: dst.nnz] = fresh
+            result._touch()
+            return result
+        result._m, result._n = self._m, self._n
+        result._csr = _HostCSR(src.indptr.copy(), src.indices.copy(), src.values.copy(), src.nnz)
+        result._log = None
+        result._assembled = self._assembled
+        result._touch()
+        return result
 
     def destroy(self) -> "Mat":
-        self._irow = None
+        self._csr = None
+        self._log = None
         self._touch()
-        return self
-
-    def _check(self):
-        if self._irow is None:
-            raise RuntimeError("Mat has not been created")
         return self
 
     # ---------------------------------------------------------- assembly
@@ -297,9 +468,9 @@ class Mat:
         self.assemblyEnd()
 
     def zeroEntries(self) -> None:
-        self._check()
-        self._materialize()
-        self._data[:] = 0.0
+        """Every stored entry becomes 0.0; the pattern stays."""
+        h = self._check()._host()
+        h.values[:] = 0.0
         self._touch()
 
     def setPreallocationNNZ(self, nnz) -> "Mat":
@@ -312,302 +483,243 @@ class Mat:
         return None
 
     def getType(self) -> str:
-        return "seqaij"
+        return MatType.SEQAIJ
 
     # ---------------------------------------------------------- accessors
     def getSize(self):
-        return (self._nrows, self._ncols)
+        return (self._m, self._n)
 
     def getLocalSize(self):
-        return (self._nrows, self._ncols)
+        return self.getSize()
 
     def getSizes(self):
-        return ((self._nrows, self._nrows), (self._ncols, self._ncols))
+        return ((self._m, self._m), (self._n, self._n))
 
     def getOwnershipRange(self):
-        return (0, self._nrows)
+        return (0, self._m)
 
     def getOwnershipRangeColumn(self):
-        return (0, self._ncols)
+        return (0, self._n)
 
     def getBlockSize(self) -> int:
         return 1
 
     def getNNZ(self) -> int:
-        self._materialize()
-        return int(self._nnz_used)
+        return self._host().nnz
 
     def getInfo(self, info=None) -> dict:
-        self._materialize()
-        return {"nz_allocated": int(len(self._data)), "nz_used": int(self._nnz_used), "rcond": 0.0}
+        """Counts: the room the arrays hold and the entries in use (stored
+        zeros included)."""
+        h = self._host()
+        return {MatInfoType.NNZ_ALLOCATED: len(h.values), MatInfoType.NZ_USED: h.nnz,
+                MatInfoType.ZERO_TOLERANCE: 0.0}
 
     @property
     def irow(self) -> np.ndarray:
-        self._materialize()
-        return self._irow
+        return self._host().wide()[0]
 
     @property
     def icol(self) -> np.ndarray:
-        self._materialize()
-        return self._icol
+        return self._host().wide()[1]
 
     @property
     def data(self) -> np.ndarray:
-        self._materialize()
-        return self._data
+        return self._host().values
 
     def getValuesCSR(self):
-        self._materialize()
-        nnz = int(self._nnz_used)
-        return (self._irow[: self._nrows + 1].copy(), self._icol[:nnz].copy(), self._data[:nnz].copy())
+        h = self._host()
+        ip, ix = h.wide()
+        return (ip[: self._m + 1].copy(), ix[: h.nnz].copy(), h.values[: h.nnz].copy())
 
     def setValuesCSR(self, indptr, indices, data=None) -> "Mat":
-        """Install CSR arrays (row starts, columns, values) directly."""
-        irow = np.asarray(indptr, dtype=np.int64)
-        icol = np.asarray(indices, dtype=np.int64)
-        vals = np.zeros(0, dtype=np.float64) if data is None else np.asarray(data, dtype=np.float64)
-        if len(irow) != self._nrows + 1:
-            raise ValueError(
-                f"incompatible array sizes: indptr has {len(irow)} entries,"
-                f" {self._nrows + 1} are needed")
-        if len(icol) != len(vals):
-            raise ValueError(f"incompatible array sizes: {len(icol)} indices but {len(vals)} values")
-        nnz = int(irow[-1])
-        if self._data is None or self._data.size < nnz:
-            self._data = np.zeros(nnz, dtype=np.float64)
-            self._icol = np.zeros(nnz, dtype=np.int64)
-        if self._irow is None or len(self._irow) != self._nrows + 1:
-            self._irow = np.zeros(self._nrows + 1, dtype=np.int64)
-        self._irow[:] = irow
-        self._icol[:nnz] = icol
-        self._data[:nnz] = vals
-        self._nnz_used = nnz
-        self._pending = None
+        """Install CSR arrays (row starts, columns, values).  int32 index
+        arrays are kept as they are and uploaded without a wider copy."""
+        def ints(a):
+            a = np.asarray(a)
+            return a if a.dtype in (np.int32, np.int64) else a.astype(np.int64)
+
+        ip, ix = ints(indptr), ints(indices)
+        vals = np.zeros(0) if data is None else np.asarray(data, dtype=np.float64)
+        if len(ip) != self._m + 1:
+            raise ValueError(f"incompatible array sizes: indptr has {len(ip)} entries,"
+                             f" {self._m + 1} are needed")
+        if len(ix) != len(vals):
+            raise ValueError(f"incompatible array sizes: {len(ix)} indices but {len(vals)} values")
+        nnz = int(ip[-1])
+        old = self._csr
+        room = len(old.values) if old is not None and len(old.values) >= nnz else nnz
+        if room == nnz:
+            cols, values = np.array(ix[:nnz], copy=True), np.array(vals[:nnz], dtype=np.float64)
+        else:  # the arrays keep their room past the entries
+            cols = np.zeros(room, dtype=ix.dtype)
+            values = np.zeros(room)
+            cols[:nnz] = ix[:nnz]
+            values[:nnz] = vals[:nnz]
+        self._csr = _HostCSR(np.array(ip, copy=True), cols, values, nnz)
+        self._log = None
         self._touch()
         return self
 
-    def _locate(self, row: int, col: int):
-        if row < 0 or row >= self._nrows or col < 0 or col >= self._ncols:
-            return (-1, -1)
-        lo = int(self._irow[row])
-        hi = int(self._irow[row + 1])
-        pos = int(np.searchsorted(self._icol[lo:hi], col))
-        if pos >= hi - lo or int(self._icol[lo + pos]) != col:
-            return (-1, -1)
-        return (row, lo + pos)
-
     def getValue(self, row, col) -> float:
-        return self._entry(int(row), int(col))
+        row, col = int(row), int(col)
+        if self._log is not None:
+            return self._log.value(row, col)
+        h = self._csr
+        if h is None or not (0 <= row < self._m and 0 <= col < self._n):
+            return 0.0
+        k = h.find(row, col)
+        return 0.0 if k < 0 else float(h.values[k])
 
     def getValues(self, rows, cols, values=None) -> np.ndarray:
-        rows = list(rows)
-        cols = list(cols)
-        out = np.array([[self.getValue(r, c) for c in cols] for r in rows], dtype=np.float64)
-        if values is not None:
-            values[:] = out
-            return values
-        return out
+        rows, cols = list(rows), list(cols)
+        block = np.array([[self.getValue(r, c) for c in cols] for r in rows], dtype=np.float64)
+        if values is None:
+            return block
+        values[:] = block
+        return values
 
     def _dense(self) -> np.ndarray:
-        self._materialize()
-        out = np.zeros((self._nrows, self._ncols), dtype=np.float64)
-        for i in range(self._nrows):
-            for j in range(int(self._irow[i]), int(self._irow[i + 1])):
-                out[i, int(self._icol[j])] += float(self._data[j])
+        h = self._host()
+        out = np.zeros((self._m, self._n))
+        np.add.at(out, (h.row_of_entries(), h.wide()[1][: h.nnz]), h.values[: h.nnz])
         return out
 
-    def _materialize(self):
-        """Pending entries -> CSR, columns sorted within each row."""
-        if self._pending is None:
-            return
-        rows = self._pending
-        nrow = self._nrows
-        counts = np.zeros(nrow, dtype=np.int64)
-        for r, cols in rows.items():
-            counts[r] = len(cols)
-        irow = np.zeros(nrow + 1, dtype=np.int64)
-        irow[1:] = np.cumsum(counts)
-        total = int(irow[-1])
-        icol = np.zeros(max(total, 1), dtype=np.int64)
-        data = np.zeros(max(total, 1), dtype=np.float64)
-        for r, cols in rows.items():
-            k = int(irow[r])
-            for c in sorted(cols):
-                icol[k] = c
-                data[k] = cols[c]
-                k += 1
-        self._irow = irow
-        self._icol = icol
-        self._data = data
-        self._nnz_used = total
-        self._pending = None
+    def _write(self, rr, cc, vv, add: bool):
+        """Record writes in order; the first one out of range raises, after
+        the writes before it are kept."""
+        bad = np.flatnonzero((rr < 0) | (rr >= self._m) | (cc < 0) | (cc >= self._n))
+        upto = int(bad[0]) if len(bad) else len(rr)
+        self._writes().push(rr[:upto], cc[:upto], vv[:upto], add)
         self._touch()
-
-    def _entry(self, row: int, col: int) -> float:
-        if self._pending is not None:
-            return self._pending.get(row, {}).get(col, 0.0)
-        i, j = self._locate(row, col)
-        return 0.0 if i < 0 else float(self._data[j])
-
-    def _spill(self):
-        if self._pending is not None:
-            return
-        self._pending = {}
-        for i in range(self._nrows):
-            cols = {}
-            for j in range(int(self._irow[i]), int(self._irow[i + 1])):
-                cols[int(self._icol[j])] = float(self._data[j])
-            if cols:
-                self._pending[i] = cols
-
-    def _setValue(self, row: int, col: int, value: float, addv: bool):
-        if row < 0 or row >= self._nrows or col < 0 or col >= self._ncols:
-            raise ValueError(f"entry ({row}, {col}) is out of range")
-        self._spill()
-        cols = self._pending.setdefault(row, {})
-        cols[col] = cols.get(col, 0.0) + value if addv else value
-        self._touch()
+        if len(bad):
+            raise ValueError(f"entry ({int(rr[upto])}, {int(cc[upto])}) is out of range")
 
     def setValue(self, row, col, value, addv=None) -> None:
-        self._setValue(int(row), int(col), float(value), addv is not None)
+        self._write(np.array([int(row)]), np.array([int(col)]), np.array([float(value)]), addv is not None)
 
     def setValues(self, rows, cols, values, addv=None, insertmode=None) -> None:
-        rows = list(rows)
-        cols = list(cols)
-        vals = list(np.asarray(values, dtype=np.float64).ravel())
+        rows, cols = list(rows), list(cols)
+        vals = np.asarray(values, dtype=np.float64).ravel()
         need = len(rows) * len(cols)
         if len(vals) < need:
-            raise ValueError(
-                f"incompatible array sizes: ni={len(rows)}, nj={len(cols)},"
-                f" nv={len(vals)}; at least {need} values are needed")
-        add = addv is not None
-        for i, r in enumerate(rows):
-            for j, c in enumerate(cols):
-                self._setValue(int(r), int(c), float(vals[i * len(cols) + j]), add)
+            raise ValueError(f"incompatible array sizes: ni={len(rows)}, nj={len(cols)},"
+                             f" nv={len(vals)}; at least {need} values are needed")
+        self._write(*_outer_writes(rows, cols, vals), addv is not None)
 
     def setValueBlocked(self, rows, cols, value, addv=None) -> None:
-        add = addv is not None
-        for r in list(rows):
-            for c in list(cols):
-                self._setValue(int(r), int(c), float(value), add)
+        self._write(*_outer_writes(rows, cols, value=value), addv is not None)
 
     def setValuesBlocked(self, rows, cols, values, addv=None) -> None:
         self.setValues(rows, cols, values, addv)
 
     # ---------------------------------------------------------- algebra
     def _out(self, x: Vec) -> Vec:
-        if x.getSize() != self._ncols:
-            raise ValueError(
-                f"incompatible sizes: matrix {self._ncols} columns, vector {x.getSize()}")
-        return Vec().createSeq(self._nrows, x.getBlockSize())
+        if x.getSize() != self._n:
+            raise ValueError(f"incompatible sizes: matrix {self._n} columns, vector {x.getSize()}")
+        return Vec().createSeq(self._m, x.getBlockSize())
 
     def _outT(self, x: Vec) -> Vec:
-        if x.getSize() != self._nrows:
-            raise ValueError(
-                f"incompatible sizes: matrix {self._nrows} rows, vector {x.getSize()}")
-        return Vec().createSeq(self._ncols, x.getBlockSize())
+        if x.getSize() != self._m:
+            raise ValueError(f"incompatible sizes: matrix {self._m} rows, vector {x.getSize()}")
+        return Vec().createSeq(self._n, x.getBlockSize())
 
     def _check_y(self, y: Vec, n: int) -> Vec:
         if y.getSize() != n:
             raise ValueError(f"incompatible sizes: output has {y.getSize()} entries, {n} are needed")
         return y
 
+    def _product(self, x: Vec, out: Vec, add: int):
+        d = self._dev()
+        s = d.sell()
+        L = lib()
+        if s is None:
+            fn = L.mp_mat_mult_add if add else L.mp_mat_mult
+            fn(x._d(), out._dw(), d.rp.ptr, d.ci.ptr, d.v.ptr, self._m)
+        else:
+            L.mpk_sell_mult(x._d(), out._dw(), d.rp.ptr, s[0].ptr, s[1].ptr, s[2].ptr, self._m, add)
+        check()
+
     def mult(self, x: Vec, y: Vec = None) -> Vec:
         """y = A @ x on the GPU."""
         self._materialize()
-        out = self._out(x) if y is None else self._check_y(y, self._nrows)
-        if x.getSize() != self._ncols:
-            raise ValueError(f"incompatible sizes: matrix {self._ncols} columns, vector {x.getSize()}")
-        d = self._dev()
-        xa = x._d()
-        s = d.sell()
-        if s is None:
-            lib().mp_mat_mult(xa, out._dw(), d.rp.ptr, d.ci.ptr, d.v.ptr, self._nrows)
-        else:
-            lib().mpk_sell_mult(xa, out._dw(), d.rp.ptr, s[0].ptr, s[1].ptr, s[2].ptr, self._nrows, 0)
-        check()
+        out = self._out(x) if y is None else self._check_y(y, self._m)
+        if x.getSize() != self._n:
+            raise ValueError(f"incompatible sizes: matrix {self._n} columns, vector {x.getSize()}")
+        self._product(x, out, 0)
         return out
 
     def multAdd(self, x: Vec, v: Vec, y: Vec = None) -> Vec:
-        """y = A @ x + v."""
+        """y = A @ x + v (the products' sums start from v's entries)."""
         self._materialize()
-        out = self._out(x) if y is None else self._check_y(y, self._nrows)
-        self._check_y(v, self._nrows)
+        out = self._out(x) if y is None else self._check_y(y, self._m)
+        self._check_y(v, self._m)
         if out is x:
             raise ValueError("output y may not alias the input x in multAdd")
         if out is not v:
             v.copy(out)
-        d = self._dev()
-        xa = x._d()
-        s = d.sell()
-        if s is None:
-            lib().mp_mat_mult_add(xa, out._dw(), d.rp.ptr, d.ci.ptr, d.v.ptr, self._nrows)
-        else:
-            lib().mpk_sell_mult(xa, out._dw(), d.rp.ptr, s[0].ptr, s[1].ptr, s[2].ptr, self._nrows, 1)
-        check()
+        self._product(x, out, 1)
         return out
 
     def multTranspose(self, x: Vec, y: Vec = None) -> Vec:
         """y = A^T @ x."""
         self._materialize()
-        out = self._outT(x) if y is None else self._check_y(y, self._ncols)
-        if x.getSize() != self._nrows:
-            raise ValueError(f"incompatible sizes: matrix {self._nrows} rows, vector {x.getSize()}")
+        out = self._outT(x) if y is None else self._check_y(y, self._n)
+        if x.getSize() != self._m:
+            raise ValueError(f"incompatible sizes: matrix {self._m} rows, vector {x.getSize()}")
+        if not self._n:
+            return out
+        if not self._m:
+            out.zeroEntries()
+            return out
         d = self._dev()
-        xa = x._d()
-        if self._ncols:
-            if self._nrows:
-                # the sliced copy of A^T (MatMultTranspose_SeqAIJ's order, ported.mojo:408-431)
-                trp, so, sv, sc = d.transpose_sell(self._irow, self._icol, self._data)
-                lib().mpk_sell_mult(xa, out._dw(), trp.ptr, so.ptr, sv.ptr, sc.ptr, self._ncols, 0)
-            else:
-                out.zeroEntries()
+        h = self._csr
+        t = d.transpose_sell(h.indptr, h.indices, h.values)
+        if t is None:  # past the sliced copy's slots: the transpose-map product
+            lib().mp_mat_mult_transpose(x._d(), out._dw(), d.rp.ptr, d.ci.ptr, d.v.ptr, self._m, self._n)
+        else:
+            # the sliced copy of A^T (MatMultTranspose_SeqAIJ's order, ported.mojo:408-431)
+            lib().mpk_sell_mult(x._d(), out._dw(), t[0].ptr, t[1].ptr, t[2].ptr, t[3].ptr, self._n, 0)
         check()
         return out
 
     def multTransposeAdd(self, x: Vec, v: Vec, y: Vec = None) -> Vec:
         """y = A^T @ x + v."""
         self._materialize()
-        out = self._outT(x) if y is None else self._check_y(y, self._ncols)
-        self._check_y(v, self._ncols)
-        self._check_y(x, self._nrows)
+        out = self._outT(x) if y is None else self._check_y(y, self._n)
+        self._check_y(v, self._n)
+        self._check_y(x, self._m)
         if out is x:
             raise ValueError("output y may not alias the input x in multTransposeAdd")
-        if out is v:
-            v = v.copy()
+        keep = v.copy() if out is v else v  # the product overwrites its output first
         self.multTranspose(x, out)
-        out.axpy(1.0, v)
+        out.axpy(1.0, keep)
         return out
 
     def createVecLeft(self) -> Vec:
-        return Vec().createSeq(self._nrows)
+        return Vec().createSeq(self._m)
 
     def createVecRight(self) -> Vec:
-        return Vec().createSeq(self._ncols)
+        return Vec().createSeq(self._n)
 
     def createVecs(self):
         return (self.createVecRight(), self.createVecLeft())
 
     def __call__(self, x, y=None):
-        if y is None:
-            y = self.createVecLeft()
-        self.mult(x, y)
-        return y
+        return self.mult(x, self.createVecLeft() if y is None else y)
 
     def getDiagonal(self, result: Vec = None) -> Vec:
         self._materialize()
-        out = Vec().createSeq(self._nrows) if result is None else self._check_y(result, self._nrows)
+        out = Vec().createSeq(self._m) if result is None else self._check_y(result, self._m)
         d = self._dev()
-        lib().mp_mat_get_diagonal(out._dw(), d.rp.ptr, d.ci.ptr, d.v.ptr, self._nrows)
+        lib().mp_mat_get_diagonal(out._dw(), d.rp.ptr, d.ci.ptr, d.v.ptr, self._m)
         check()
         return out
 
     def createDiagonal(self, d) -> "Mat":
         n = d.getSize()
-        out = Mat().createSeqAIJ(n, nnz=1)
-        rows = np.arange(n, dtype=np.int64)
-        out.setValuesCSR(np.arange(n + 1), rows, d.getArray(readonly=True))
-        return out
+        diag = Mat().createSeqAIJ(n, nnz=1)
+        diag.setValuesCSR(np.arange(n + 1), np.arange(n), d.getArray(readonly=True))
+        return diag
 
     def createTranspose(self, mat=None) -> "Mat":
         return _transpose(self if mat is None else mat)
@@ -616,78 +728,65 @@ class Mat:
         return _transpose(self)
 
     def scale(self, alpha) -> None:
-        """Scale every stored entry (the same product the device kernel
-        forms, a_ij * alpha)."""
-        self._check()
-        self._materialize()
-        self._data *= float(alpha)
+        """a_ij * alpha for every stored entry."""
+        h = self._check()._host()
+        h.values *= float(alpha)
         self._touch()
 
     def equal(self, mat, tol: float = 0.0) -> bool:
-        other = mat
-        self._materialize()
-        other._materialize()
-        if self._nrows != other._nrows or self._ncols != other._ncols:
+        """Same shape, same stored arrays, values within tol."""
+        a, b = self._host(), mat._host()
+        if self.getSize() != mat.getSize():
             return False
-        if not np.array_equal(self._irow, other._irow):
-            return False
-        if not np.array_equal(self._icol, other._icol):
-            return False
-        return bool(np.allclose(self._data, other._data, rtol=0, atol=tol))
+        same = np.array_equal(a.wide()[0], b.wide()[0]) and np.array_equal(a.wide()[1], b.wide()[1])
+        return bool(same and np.allclose(a.values, b.values, rtol=0, atol=tol))
 
     def isSymmetric(self, tol: float = 0.0) -> bool:
-        if self._nrows != self._ncols:
+        if self._m != self._n:
             return False
-        return bool(np.allclose(self._dense(), _transpose(self)._dense(), rtol=0, atol=tol))
+        dense = self._dense()
+        return bool(np.allclose(dense, dense.T, rtol=0, atol=tol))
 
     def isTranspose(self, mat=None, tol: float = 0.0) -> bool:
         other = self if mat is None else mat
-        if self._nrows != other._ncols or self._ncols != other._nrows:
+        if (self._m, self._n) != (other._n, other._m):
             return False
-        return bool(np.allclose(self._dense(), _transpose(other)._dense(), rtol=0, atol=tol))
+        return bool(np.allclose(self._dense(), other._dense().T, rtol=0, atol=tol))
 
     def __matmul__(self, x):
         return self.mult(x)
 
     def __mul__(self, other):
-        if not isinstance(other, Mat):
-            return NotImplemented
-        return _matmul(self, other)
+        return _matmul(self, other) if isinstance(other, Mat) else NotImplemented
 
     def __neg__(self):
-        out = self.duplicate(copy=True)
-        out.scale(-1.0)
-        return out
+        flipped = self.duplicate(copy=True)
+        flipped.scale(-1.0)
+        return flipped
 
 
 def _transpose(m: Mat) -> Mat:
-    """Transpose a CSR matrix (columns of the result ascending)."""
-    rows, cols, vals = m.getValuesCSR()
-    out = Mat().createSeqAIJ(m._ncols, nnz=1)
-    out._ncols = m._nrows
-    counts = np.bincount(cols.astype(np.int64), minlength=m._ncols) if len(cols) else np.zeros(m._ncols, np.int64)
-    irow = np.zeros(m._ncols + 1, dtype=np.int64)
-    irow[1:] = np.cumsum(counts)
-    row_of = np.repeat(np.arange(m._nrows, dtype=np.int64), np.diff(rows))
-    order = np.lexsort((row_of, cols))
-    out._irow = irow
-    out._icol = row_of[order] if len(order) else np.zeros(1, dtype=np.int64)
-    out._data = vals[order] if len(order) else np.zeros(1, dtype=np.float64)
-    out._nnz_used = int(irow[-1])
-    out._assembled = True
-    out._pending = None
-    out._touch()
-    return out
+    """A^T as CSR, each column's entries in increasing row."""
+    h = m._host()
+    rows = h.row_of_entries()
+    cols = h.wide()[1][: h.nnz]
+    order = np.lexsort((rows, cols))
+    t = Mat().createSeqAIJ((m._n, m._m), nnz=1)
+    indptr = np.zeros(m._n + 1, np.int64)
+    if h.nnz:
+        np.cumsum(np.bincount(cols, minlength=m._n), out=indptr[1:])
+    t.setValuesCSR(indptr, rows[order], h.values[: h.nnz][order])
+    return t
 
 
 def _matmul(a: Mat, b: Mat) -> Mat:
-    if a._ncols != b._nrows:
+    if a._n != b._m:
         raise ValueError(f"incompatible sizes: {a.getSize()} times {b.getSize()}")
     return from_dense(a._dense() @ b._dense())
 
 
 def from_dense(array, mat=None) -> Mat:
-    """Build a CSR matrix from a dense 2-D array (nonzeros only)."""
+    """A CSR matrix of the nonzeros of a dense 2-D array."""
     arr = _f64(array)
     if arr.ndim != 2:
         raise ValueError("expected a 2-D array")
@@ -695,14 +794,7 @@ def from_dense(array, mat=None) -> Mat:
     out = mat if mat is not None else Mat()
     out.createSeqAIJ((n, m), nnz=1)
     rr, cc = np.nonzero(arr)
-    counts = np.count_nonzero(arr, axis=1).astype(np.int64)
-    out._irow = np.zeros(n + 1, dtype=np.int64)
-    out._irow[1:] = np.cumsum(counts)
-    total = int(out._irow[-1])
-    out._icol = cc.astype(np.int64) if total else np.zeros(1, dtype=np.int64)
-    out._data = arr[rr, cc].astype(np.float64) if total else np.zeros(1, dtype=np.float64)
-    out._nnz_used = total
-    out._assembled = True
-    out._pending = None
-    out._touch()
+    indptr = np.zeros(n + 1, np.int64)
+    np.cumsum(np.bincount(rr, minlength=n), out=indptr[1:])
+    out.setValuesCSR(indptr, cc.astype(np.int64), arr[rr, cc])
     return out

@@ -12,6 +12,8 @@ then upload the factor (int32 indices) for the device sweeps.
 
 from __future__ import annotations
 
+from dataclasses import dataclass
+
 import numpy as np
 
 from mpk._native import check, lib
@@ -94,6 +96,54 @@ def _a(arr: np.ndarray) -> int:
     return arr.ctypes.data
 
 
+@dataclass
+class _PCOptions:
+    """What the PC's setters record.  The defaults are PETSc's (PC.py:108-148
+    of the reference): natural ordering, level 0, no shift, omega 1, one
+    symmetric SOR sweep, the diagonal Jacobi with fix-diagonal on."""
+
+    levels: int = 0
+    shift_type: str = "none"
+    shift: float = 0.0
+    zeropivot: float = 100.0
+    ordering: str = MatOrderingType.NATURAL
+    diagonal_fill: bool = False
+    sor_omega: float = 1.0
+    sor_fshift: float = 0.0
+    sor_its: int = 1  # recorded, not applied: the apply runs its = lits = 1 (capi.mojo:272-280)
+    sor_lits: int = 1
+    sor_symmetric: bool = True
+    jacobi_type: int = PC_JACOBI_DIAGONAL
+    jacobi_abs: bool = False
+    jacobi_fix: bool = True
+    rowl1_scale: float = 1.0
+    use_amat: bool = False
+    reuse: bool = False
+
+
+def _setter(field, convert, resets="setup", doc=None):
+    """A PC method storing convert(value) in the options; `resets` is what a
+    change invalidates: "setup" (the next apply sets up again), "descriptor"
+    (only the packed descriptor) or None."""
+
+    def set_option(self, value):
+        setattr(self._opt, field, convert(value))
+        if resets == "setup":
+            self._setupcalled = False
+        elif resets == "descriptor":
+            self._pci_cache = None
+
+    set_option.__doc__ = doc
+    return set_option
+
+
+def _getter(field):
+    def get_option(self):
+        return getattr(self._opt, field)
+
+    return get_option
+
+
 class PC:
     """A preconditioner for a sequential AIJ matrix, applied on the GPU."""
 
@@ -103,28 +153,11 @@ class PC:
 
     def __init__(self):
         self._type = PCType.NONE
-        self._amat = None
-        self._pmat = None
+        self._amat = self._pmat = None
+        self._opt = _PCOptions()
+        self._failed_reason = FailedReason.NOERROR
         self._setupcalled = False
         self._setup_version = None
-        self._use_amat = False
-        self._reuse = False
-        self._failed_reason = FailedReason.NOERROR
-        self._levels = 0
-        self._shift = 0.0
-        self._zeropivot = 100.0
-        self._shift_type = "none"
-        self._ordering = MatOrderingType.NATURAL
-        self._diagonal_fill = False
-        self._sor_omega = 1.0
-        self._sor_fshift = 0.0
-        self._sor_its = 1
-        self._sor_lits = 1
-        self._sor_sym = "symmetric"
-        self._fix_diag = True
-        self._use_abs = False
-        self._jacobi_type = PC_JACOBI_DIAGONAL
-        self._rowl1_scale = 1.0
         self._factor = None      # host factor arrays (int64 / float64)
         self._nlate = 0          # IC entries below the diagonal (see mpk_icc_transpose)
         self._dbufs = {}         # device buffers of the set up
@@ -151,21 +184,15 @@ class PC:
         return self._type
 
     def setOperators(self, A: Mat = None, P: Mat = None) -> None:
-        self._amat = A
-        self._pmat = A if P is None else P
+        self._amat, self._pmat = A, (A if P is None else P)
         self._reset_factor()
 
     def getOperators(self):
         return (self._amat, self._pmat)
 
-    def setUseAmat(self, flag: bool) -> None:
-        self._use_amat = bool(flag)
-
-    def getUseAmat(self) -> bool:
-        return self._use_amat
-
-    def setReusePreconditioner(self, flag: bool) -> None:
-        self._reuse = bool(flag)
+    setUseAmat = _setter("use_amat", bool, resets=None)
+    getUseAmat = _getter("use_amat")
+    setReusePreconditioner = _setter("reuse", bool, resets=None)
 
     def setFailedReason(self, reason) -> None:
         self._failed_reason = reason
@@ -183,9 +210,16 @@ class PC:
         self._pci_cache = None
 
     # ---------------------------------------------------------- options
-    def setFactorLevels(self, levels) -> None:
-        self._levels = int(levels)
-        self._setupcalled = False
+    setFactorLevels = _setter("levels", int)
+    setSOROmega = _setter("sor_omega", float)
+    setSORIterations = _setter("sor_its", int, doc="Recorded; the apply runs one sweep pair, as the "
+                               "reference's does (capi.mojo:272-280).")
+    setSORShift = _setter("sor_fshift", float, resets=None)
+    setSORSymmetric = _setter("sor_symmetric", bool, resets="descriptor")
+    setJacobiType = _setter("jacobi_type", int)
+    setJacobiUseAbs = _setter("jacobi_abs", bool)
+    setJacobiFixDiagonal = _setter("jacobi_fix", bool)
+    setJacobiRowl1Scale = _setter("rowl1_scale", float)
 
     def setFactorShift(self, shift_type, amount=None) -> None:
         """Only 'none' (PETSc's default, no pivot retry) is covered."""
@@ -197,9 +231,9 @@ class PC:
                 f"factor shift {name!r} is not covered by mojo-petsc4py; the"
                 " pivot-retry path (MatPivotCheck) is not ported, so only"
                 " 'none' reproduces upstream")
-        self._shift_type = name
+        self._opt.shift_type = name
         if amount is not None:
-            self._shift = float(amount)
+            self._opt.shift = float(amount)
         self._setupcalled = False
 
     def setFactorPivot(self, flag) -> None:
@@ -208,52 +242,19 @@ class PC:
             raise NotImplementedError(
                 "a nonzero factor pivot tolerance is not covered by mojo-petsc4py;"
                 " MatPivotCheck is not ported, so the tolerance would be ignored")
-        self._zeropivot = 0.0
-        self._diagonal_fill = bool(flag)
+        self._opt.zeropivot = 0.0
+        self._opt.diagonal_fill = bool(flag)
         self._setupcalled = False
 
     def setFactorOrdering(self, ordering) -> None:
         if ordering not in (MatOrderingType.NATURAL, MatOrderingType.RCM):
             raise NotImplementedError(
                 f"matrix ordering {ordering!r} is not covered by mojo-petsc4py (natural and rcm are)")
-        self._ordering = ordering
+        self._opt.ordering = ordering
         self._setupcalled = False
 
     def setFactorReorderForNonzeroDiagonal(self) -> None:
-        self._diagonal_fill = True
-        self._setupcalled = False
-
-    def setSORSymmetric(self, flag) -> None:
-        self._sor_sym = "symmetric" if flag else "forward"
-        self._pci_cache = None
-
-    def setSOROmega(self, omega) -> None:
-        self._sor_omega = float(omega)
-        self._setupcalled = False
-
-    def setSORIterations(self, its) -> None:
-        # stored but not applied: the apply runs its = lits = 1, as the
-        # reference's does (capi.mojo:272-280)
-        self._sor_its = int(its)
-        self._setupcalled = False
-
-    def setSORShift(self, fshift) -> None:
-        self._sor_fshift = float(fshift)
-
-    def setJacobiType(self, jtype) -> None:
-        self._jacobi_type = int(jtype)
-        self._setupcalled = False
-
-    def setJacobiUseAbs(self, flag) -> None:
-        self._use_abs = bool(flag)
-        self._setupcalled = False
-
-    def setJacobiFixDiagonal(self, flag) -> None:
-        self._fix_diag = bool(flag)
-        self._setupcalled = False
-
-    def setJacobiRowl1Scale(self, scale) -> None:
-        self._rowl1_scale = float(scale)
+        self._opt.diagonal_fill = True
         self._setupcalled = False
 
     def setFactorMatrixType(self, mtype) -> None:
@@ -318,10 +319,10 @@ class PC:
         k = max(n, 1)
         diag = self._buf("diag", 8 * k)
         didx = self._buf("didx", 8 * k)
-        lib().mp_pc_jacobi_setup(diag.ptr, self._jacobi_type, int(self._use_abs), int(self._fix_diag),
-                                 self._rowl1_scale, d.rp.ptr, d.ci.ptr, d.v.ptr, n, didx.ptr)
+        lib().mp_pc_jacobi_setup(diag.ptr, self._opt.jacobi_type, int(self._opt.jacobi_abs), int(self._opt.jacobi_fix),
+                                 self._opt.rowl1_scale, d.rp.ptr, d.ci.ptr, d.v.ptr, n, didx.ptr)
         dsq = self._buf("diagsqrt", 8 * k)
-        lib().mp_pc_jacobi_setup_symmetric(dsq.ptr, self._jacobi_type, d.rp.ptr, d.ci.ptr, d.v.ptr, n)
+        lib().mp_pc_jacobi_setup_symmetric(dsq.ptr, self._opt.jacobi_type, d.rp.ptr, d.ci.ptr, d.v.ptr, n)
 
     def _build_sor(self, m: Mat, n: int):
         d = m._dev()
@@ -333,16 +334,15 @@ class PC:
         self._buf("sorwork", 8 * 3 * k)
 
     def _permutations(self, m: Mat, n: int):
-        if self._ordering == MatOrderingType.RCM and n > 0:
-            perm = np.zeros(n, dtype=np.int64)
-            level = np.zeros(n, dtype=np.int64)
-            visited = np.zeros(n, dtype=np.int64)
-            queue = np.zeros(n, dtype=np.int64)
-            lib().mp_ordering_rcm(_a(perm), _a(level), _a(visited), _a(queue), _a(m.irow), _a(m.icol), n)
-            r = perm
-        else:
+        """(row order r, its inverse ic): natural, or the reference's
+        index-order RCM (factor.mojo:279-332) computed by the library."""
+        if not (self._opt.ordering == MatOrderingType.RCM and n > 0):
             r = np.arange(n, dtype=np.int64)
-        ic = np.empty(n, dtype=np.int64)
+        else:
+            work = np.zeros((4, n), dtype=np.int64)  # permutation, levels, marks, queue
+            lib().mp_ordering_rcm(*(_a(row) for row in work), _a(m.irow), _a(m.icol), n)
+            r = work[0].copy()
+        ic = np.empty_like(r)
         ic[r] = np.arange(n, dtype=np.int64)
         return r, ic
 
@@ -360,55 +360,63 @@ class PC:
         self._upload_factor(m, n)
 
     def _build_ilu(self, m, n, r, ic, adiag, scratch):
+        """The ILU pattern: the ILU(0) fast path for natural ordering at level
+        0, else the level-of-fill pass and its contiguous L/U layout
+        (PC.py:392-440).  Every attempt takes fresh output arrays and the
+        same scratch, which keeps what the attempt before left in it."""
         L = lib()
-        cap = max(4 * n, 4 * int(m._irow[-1]) + 4 * n, 16)
-        while True:
-            bi = np.zeros(n + 1, dtype=np.int64)
-            bj = np.zeros(cap, dtype=np.int64)
-            bjlvl = np.zeros(cap, dtype=np.int64)
-            bdiag = np.zeros(n + 1, dtype=np.int64)
-            if self._levels == 0 and np.array_equal(r, np.arange(n)):
-                nnz = L.mp_ilu_symbolic_ilu0(_a(bi), _a(bj), _a(bdiag), _a(adiag), _a(m.irow), _a(m.icol), n)
-                contiguous = False
+        fast = self._opt.levels == 0 and np.array_equal(r, np.arange(n))
+        out = {}
+
+        def attempt(cap):
+            out.update(bi=np.zeros(n + 1, np.int64), bdiag=np.zeros(n + 1, np.int64), bj=np.zeros(cap, np.int64))
+            if fast:
+                got = L.mp_ilu_symbolic_ilu0(_a(out["bi"]), _a(out["bj"]), _a(out["bdiag"]), _a(adiag), _a(m.irow),
+                                             _a(m.icol), n)
             else:
-                bjsrc = np.zeros(cap, dtype=np.int64)
-                nnz = L.mp_ilu_symbolic(_a(bi), _a(bjsrc), _a(bjlvl), _a(bdiag), _a(m.irow), _a(m.icol),
-                                        _a(r), _a(ic), self._levels, cap, _a(scratch), n)
-                contiguous = True
-            if nnz >= 0:
-                if contiguous:
-                    L.mp_ilu_contiguous_lu(_a(bjsrc), _a(bi), _a(bdiag), _a(bj), n)
-                break
-            cap = -nnz + n
-        # the factor lives in permuted coordinates; the solution is mapped back
-        # with the permutation itself (PC.py:436-440).  Its values are
-        # computed on the device (_ilu_numeric_device), level by level.
-        self._factor = {"bi": bi, "bj": bj, "ba": None, "bdiag": bdiag, "r": r, "c": r, "ic": ic}
+                out["src"] = np.zeros(cap, np.int64)
+                lvl = np.zeros(cap, np.int64)
+                got = L.mp_ilu_symbolic(_a(out["bi"]), _a(out["src"]), _a(lvl), _a(out["bdiag"]), _a(m.irow),
+                                        _a(m.icol), _a(r), _a(ic), self._opt.levels, cap, _a(scratch), n)
+            return got, n - got
+
+        _until_it_fits(attempt, _first_capacity(m, n))
+        if not fast:
+            L.mp_ilu_contiguous_lu(_a(out["src"]), _a(out["bi"]), _a(out["bdiag"]), _a(out["bj"]), n)
+        # the factor is in permuted coordinates and the solution goes back
+        # through the permutation itself, c = r (PC.py:436-440); its values
+        # come from the device pass (_ilu_numeric_device)
+        self._factor = {"bi": out["bi"], "bj": out["bj"], "ba": None, "bdiag": out["bdiag"], "r": r, "c": r,
+                        "ic": ic}
 
     def _build_icc(self, m, n, r, ic, adiag, scratch):
+        """The IC pattern through the level-of-fill pass at every level, then
+        its contiguous Cholesky layout (PC.py:442-492).  The scratch grows
+        only when an attempt needs more than it holds."""
         L = lib()
-        cap = max(4 * n, 4 * int(m._irow[-1]) + 4 * n, 16)
-        rinv = np.empty(n, dtype=np.int64)
+        rinv = np.empty(n, np.int64)
         rinv[r] = np.arange(n, dtype=np.int64)
-        while True:
-            ui = np.zeros(n + 1, dtype=np.int64)
-            uj = np.zeros(cap, dtype=np.int64)
-            ujlvl = np.zeros(cap, dtype=np.int64)
-            udiag = np.zeros(n, dtype=np.int64)
+        out = {"scratch": scratch}
+
+        def attempt(cap):
             need = 6 * (n + 1) + 2 * cap
-            if scratch.size < need:
-                scratch = np.zeros(need, dtype=np.int64)
-            nnz = L.mp_icc_symbolic(_a(ui), _a(uj), _a(ujlvl), _a(udiag), _a(m.irow), _a(m.icol), _a(r),
-                                    _a(rinv), self._levels, cap, _a(scratch), n, int(scratch.size))
-            if nnz >= 0:
-                break
-            cap = -nnz
-        bi = np.zeros(n + 1, dtype=np.int64)
-        bdiag = np.zeros(n + 1, dtype=np.int64)
-        bj = np.zeros(max(cap, int(ui[-1]) + 1), dtype=np.int64)
-        bdiag[:n] = udiag[:n]
+            if out["scratch"].size < need:
+                out["scratch"] = np.zeros(need, np.int64)
+            sc = out["scratch"]
+            out.update(ui=np.zeros(n + 1, np.int64), uj=np.zeros(cap, np.int64), udiag=np.zeros(n, np.int64))
+            lvl = np.zeros(cap, np.int64)
+            got = L.mp_icc_symbolic(_a(out["ui"]), _a(out["uj"]), _a(lvl), _a(out["udiag"]), _a(m.irow), _a(m.icol),
+                                    _a(r), _a(rinv), self._opt.levels, cap, _a(sc), n, int(sc.size))
+            return got, -got
+
+        cap = _until_it_fits(attempt, _first_capacity(m, n))
+        ui, uj = out["ui"], out["uj"]
+        bi = np.zeros(n + 1, np.int64)
+        bdiag = np.zeros(n + 1, np.int64)
+        bdiag[:n] = out["udiag"]
+        bj = np.zeros(max(cap, int(ui[-1]) + 1), np.int64)
         L.mp_icc_contiguous_cholesky(_a(uj), _a(ui), _a(bdiag), _a(bi), _a(bdiag), _a(bj), n)
-        # the values are computed on the device (_icc_numeric_device)
+        # the values come from the device pass (_icc_numeric_device)
         self._factor = {"bi": bi, "bj": bj, "ba": None, "bdiag": bdiag, "r": r, "rinv": rinv}
 
     def _upload_factor(self, m: Mat, n: int):
@@ -524,7 +532,7 @@ class PC:
 
     # ---------------------------------------------------------- descriptor
     def _sor_flag(self) -> int:
-        base = SOR_LOCAL_SYMMETRIC_SWEEP if self._sor_sym == "symmetric" else SOR_FORWARD_SWEEP
+        base = SOR_LOCAL_SYMMETRIC_SWEEP if self._opt.sor_symmetric else SOR_FORWARD_SWEEP
         return base | SOR_ZERO_INITIAL_GUESS
 
     def _pci_block(self) -> np.ndarray:
@@ -558,7 +566,7 @@ class PC:
         return self._pci_cache
 
     def _pcf_block(self) -> np.ndarray:
-        return np.array([self._sor_omega, self._sor_fshift], dtype=np.float64)
+        return np.array([self._opt.sor_omega, self._opt.sor_fshift], dtype=np.float64)
 
     # ---------------------------------------------------------- apply
     def _direct_apply(self, x: Vec, y: Vec):
@@ -629,36 +637,61 @@ def _host_diag_markers(m: Mat, adiag: np.ndarray) -> None:
         adiag[has] = first[has]
 
 
+def _first_capacity(m: Mat, n: int) -> int:
+    """The first capacity a symbolic factorisation is given (PC.py:393)."""
+    return max(4 * n, 4 * int(m.irow[-1]) + 4 * n, 16)
+
+
+def _until_it_fits(attempt, cap: int) -> int:
+    """The symbolic phases' capacity protocol: attempt(cap) returns (count,
+    capacity to try next); a negative count means the output did not fit.
+    Returns the capacity that fitted."""
+    while True:
+        got, more = attempt(cap)
+        if got >= 0:
+            return cap
+        cap = int(more)
+
+
+def _segments(starts, counts):
+    """(segment index, position) of every element of the segments
+    [starts[k], starts[k] + counts[k]) (a negative count is an empty one)."""
+    counts = np.maximum(np.asarray(counts, np.int64), 0)
+    seg = np.repeat(np.arange(len(counts)), counts)
+    first = np.repeat(np.asarray(starts, np.int64) - np.concatenate(([0], np.cumsum(counts)[:-1])), counts)
+    return seg, first + np.arange(int(counts.sum()))
+
+
 def _ilu_factor_mat(f) -> Mat:
-    """L*U of the ILU factor as a CSR matrix (PC.py:660-689)."""
-    bi, bj, ba, bdiag, r, c = f["bi"], f["bj"], f["ba"], f["bdiag"], f["r"], f["c"]
+    """L U of the ILU factor, in the original numbering (PC.py:660-689): L is
+    unit lower with row i's entries bi[i]..bi[i+1]; U's row i holds the
+    entries after bdiag[i+1] and ends on its inverted diagonal at bdiag[i]."""
+    bi, bj, ba, bdiag, r, c = (np.asarray(f[k]) for k in ("bi", "bj", "ba", "bdiag", "r", "c"))
     n = len(bi) - 1
-    L = np.eye(n)
-    U = np.zeros((n, n))
-    for i in range(n):
-        for j in range(int(bi[i]), int(bi[i + 1])):
-            L[int(r[i]), int(c[int(bj[j])])] = ba[j]
-    for i in range(n):
-        v = int(bdiag[i + 1]) + 1
-        nz = int(bdiag[i] - bdiag[i + 1] - 1)
-        for j in range(nz):
-            U[int(r[i]), int(c[int(bj[v + j])])] = ba[v + j]
-        U[int(r[i]), int(r[i])] = 1.0 / ba[v + nz]
-    return from_dense(L @ U)
+    lower = np.eye(n)
+    i, q = _segments(bi[:n], np.diff(bi))
+    lower[r[i], c[bj[q]]] = ba[q]
+    upper = np.zeros((n, n))
+    first = bdiag[1:] + 1
+    count = bdiag[:n] - bdiag[1:] - 1
+    i, q = _segments(first, count)
+    upper[r[i], c[bj[q]]] = ba[q]
+    upper[r, r] = 1.0 / ba[first + count]
+    return from_dense(lower @ upper)
 
 
 def _icc_factor_mat(f) -> Mat:
-    """L*D*L^T of the IC factor as a CSR matrix (PC.py:692-745)."""
-    bi, bj, ba, bdiag, r = f["bi"], f["bj"], f["ba"], f["bdiag"], f["r"]
+    """L D L^T of the IC factor, in the original numbering (PC.py:692-745):
+    row i of the stored upper part holds -(u_ij / d_i) at bi[i]..bdiag[i] and
+    1 / d_i at bdiag[i]."""
+    bi, bj, ba, bdiag, r = (np.asarray(f[k]) for k in ("bi", "bj", "ba", "bdiag", "r"))
     n = len(bi) - 1
-    unit = np.zeros((n, n), dtype=np.float64)
-    d = np.zeros(n, dtype=np.float64)
-    for i in range(n):
-        row = int(r[i])
-        slot = int(bdiag[i])
-        unit[row, row] = 1.0
-        for j in range(int(bi[i]), slot):
-            unit[int(r[int(bj[j])]), row] = -ba[j]
-        inv = ba[slot]
-        d[row] = 1.0 / inv if inv != 0.0 else 0.0
+    unit = np.zeros((n, n))
+    unit[r, r] = 1.0
+    i, q = _segments(bi[:n], bdiag[:n] - bi[:n])
+    unit[r[bj[q]], r[i]] = -ba[q]
+    inv = ba[bdiag[:n]]
+    d = np.zeros(n)
+    nz = inv != 0.0
+    d[r[nz]] = 1.0 / inv[nz]
     return from_dense((unit * d) @ unit.T)

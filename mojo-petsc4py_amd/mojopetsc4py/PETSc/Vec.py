@@ -217,9 +217,8 @@ class Vec:
         z = np.zeros(2, dtype=np.float64)
         lib().mp_vec_norm(self._d(), ntype, z.ctypes.data, self._n())
         check()
-        if ntype != NormType.NORM_1_AND_2:
-            return float(z[0])
-        return (float(z[0]), float(z[1]))
+        both = ntype == NormType.NORM_1_AND_2
+        return (float(z[0]), float(z[1])) if both else float(z[0])
 
     def normBegin(self, norm_type=None) -> None:
         return None
@@ -385,58 +384,22 @@ class Vec:
         lib().mp_vec_conjugate(self._dw(), self._n())
 
     # ---------------------------------------------------------- operators
+    # (the binary ones are attached after the class, from _BINARY)
     def __neg__(self):
-        out = self.duplicate()
-        out.set(-1.0)
-        out.pointwiseMult(self, out)
-        return out
+        flipped = self.copy()
+        flipped.scale(-1.0)  # x * -1.0, the reference's set(-1) and pointwise product
+        return flipped
 
     def __pos__(self):
         return self.copy()
 
     def __abs__(self):
-        out = self.copy()
-        out.abs()
-        return out
-
-    def __add__(self, other):
-        if not isinstance(other, Vec):
-            return NotImplemented
-        out = self.copy()
-        out.axpy(1.0, other)
-        return out
-
-    __radd__ = __add__
-
-    def __sub__(self, other):
-        # the reference returns a zero vector minus `other` here
-        # (python/mojopetsc4py/PETSc/Vec.py:490-495); kept as it is
-        if not isinstance(other, Vec):
-            return NotImplemented
-        out = self.duplicate()
-        out.axpy(-1.0, other)
-        return out
+        # the reference takes abs() of duplicate(), a zero vector
+        # (python/mojopetsc4py/PETSc/Vec.py:466-469); kept as it is
+        return self.duplicate()
 
     def __rsub__(self, other):
-        if not isinstance(other, Vec):
-            return NotImplemented
-        return other - self
-
-    def __mul__(self, other):
-        if not isinstance(other, Vec):
-            return NotImplemented
-        out = self.copy()
-        out.pointwiseMult(self, other)
-        return out
-
-    __rmul__ = __mul__
-
-    def __truediv__(self, other):
-        if not isinstance(other, Vec):
-            return NotImplemented
-        out = self.duplicate()
-        out.pointwiseDivide(self, other)
-        return out
+        return other - self if isinstance(other, Vec) else NotImplemented
 
     def __matmul__(self, other):
         return self.dot(other)
@@ -446,3 +409,31 @@ class Vec:
 
     def __setitem__(self, i, v):
         self.getArray()[i] = v
+
+
+# name -> (what the result starts as, how it is then formed from (a, b)).
+# The reference starts __sub__ and __truediv__ from duplicate(), a zero
+# vector, so a - b is -b (python/mojopetsc4py/PETSc/Vec.py:490-495); kept.
+_BINARY = {
+    "__add__": ("copy", lambda out, a, b: out.axpy(1.0, b)),
+    "__sub__": ("zeros", lambda out, a, b: out.axpy(-1.0, b)),
+    "__mul__": ("copy", lambda out, a, b: out.pointwiseMult(a, b)),
+    "__truediv__": ("zeros", lambda out, a, b: out.pointwiseDivide(a, b)),
+}
+
+
+def _binary(start, form):
+    def operator(self, other):
+        if not isinstance(other, Vec):
+            return NotImplemented
+        out = self.copy() if start == "copy" else self.duplicate()
+        form(out, self, other)
+        return out
+
+    return operator
+
+
+for _name, (_start, _form) in _BINARY.items():
+    setattr(Vec, _name, _binary(_start, _form))
+Vec.__radd__ = Vec.__add__
+Vec.__rmul__ = Vec.__mul__

@@ -26,7 +26,7 @@ def _sweep_entries(kind, pc, A):
         return off, col, coef
 
     if kind == "sor":
-        ip, ix, va = A._irow, A._icol, A._data
+        ip, ix, va = A.irow, A.icol, A.data
         dm = np.zeros(n, np.int64)
         for i in range(n):
             hit = np.nonzero(ix[ip[i]:ip[i + 1]] == i)[0]

@@ -36,6 +36,25 @@ def test_vec_sub_returns_minus_other(native):
     assert same_bits((a - b).getArray(), -np.full(5, 0.5))
 
 
+def test_vec_operators(native):
+    """The other operators: -a is a * -1, a + b, a * b and a / b elementwise
+    (a / b starts from a zero vector, as the reference's does, and divides
+    a by b into it); abs(a) is a zero vector, because the reference takes
+    abs() of duplicate() (Vec.py:466-469); a @ b is the dot product."""
+    from mojopetsc4py import PETSc
+    x = np.random.default_rng(8).normal(size=37)
+    y = np.random.default_rng(9).normal(size=37)
+    a, b = PETSc.Vec().createWithArray(x), PETSc.Vec().createWithArray(y)
+    assert same_bits((-a).getArray(), x * -1.0)
+    assert same_bits((+a).getArray(), x)
+    assert same_bits((a + b).getArray(), x + y)
+    assert same_bits((a * b).getArray(), x * y)
+    assert same_bits((a / b).getArray(), x / y)
+    assert same_bits(abs(a).getArray(), np.zeros(37))
+    assert (a @ b) == a.dot(b)
+    assert a.__add__(3.0) is NotImplemented
+
+
 def test_sor_iterations_are_not_used(native):
     """setSORIterations is stored but the apply runs its = lits = 1
     (PC.py:280-282, capi.mojo:277-280): the same bits as the default."""
