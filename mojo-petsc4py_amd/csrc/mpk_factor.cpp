@@ -107,6 +107,33 @@ static inline void copy_ids(I *dst, const I *src, I count) {
   if (count > 0) memcpy(dst, src, sizeof(I) * (size_t)count);
 }
 
+// hdr[7] = 1 marks a narrow plan, which the device runs in one workgroup
+// with its rows staged a few levels ahead (k_sweep_narrow): one block (a
+// single-block plan or a ring), 4 or 8 fixed slots, at most 256 rows a level
+// and 4096 levels, and no entry reading another block's published value (d
+// in [-n, 0)).  MPK_SWEEP_NARROW=0 leaves every plan unmarked.
+static void mark_narrow(char *out, int64_t n) {
+  int64_t *hdr = (int64_t *)out;
+  hdr[7] = 0;
+  const char *e = std::getenv("MPK_SWEEP_NARROW");
+  if (e && e[0] == '0' && !e[1]) return;
+  const int64_t nblk = hdr[0], nlvl = hdr[2], w = hdr[5];
+  if (nblk != 1 || (w != 4 && w != 8) || nlvl > 4096) return;
+  const int *blk_pos = (const int *)(out + 64), *blk_lev = blk_pos + nblk + 1, *lvl = blk_lev + nblk + 1;
+  for (int64_t l = blk_lev[0]; l + 1 < blk_lev[1]; l++)
+    if (lvl[l + 1] - lvl[l] > 256) return;
+  const int64_t ints = 2 * (nblk + 1) + nlvl + 3 * n + 1;
+  const double *ecoef = (const double *)(out + 64 + ((4 * ints + 7) & ~(int64_t)7));
+  const int *edep = (const int *)(ecoef + w * n);
+  const int *pcnt = lvl + nlvl + n;
+  for (int64_t p = 0; p < n; p++)
+    for (int64_t k = 0; k < pcnt[p]; k++) {
+      const int d = edep[k * n + p];
+      if (d < 0 && d >= -n) return;
+    }
+  hdr[7] = 1;
+}
+
 extern "C" {
 
 // factor.mojo:31-67 -- ILU(0), natural ordering: L = strict lower part of A,
@@ -711,6 +738,7 @@ int64_t mpk_sweep_plan(int64_t n, int64_t forward, int64_t off_, int64_t col_, i
         }
       }
       pent[n] = w ? 0 : (int)e_out;
+      mark_narrow(out, n);
       return bytes;
     }
   }
@@ -793,6 +821,7 @@ blocks:
   pent[n] = (int)(w ? 0 : e_out);
   if (w)
     for (int64_t p = 0; p <= n; p++) pent[p] = 0;  // unused with fixed slots
+  mark_narrow(out, n);
   return bytes;
 }
 

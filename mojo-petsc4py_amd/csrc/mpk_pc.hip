@@ -6,8 +6,9 @@
 // gives the same bits as ILUApplyFactor_SeqAIJ / ICCApplyFactor_SeqAIJ /
 // PCApply_SOR (src/ported.mojo:633-952) on the same factor.
 #include <map>
-#include <tuple>
 #include <memory>
+#include <mutex>
+#include <tuple>
 #include <vector>
 
 #include "mpk_internal.h"
@@ -32,13 +33,46 @@ __global__ void k_sweep_prepare(i64 n, double *p, unsigned *ticket, const int64_
   if (i < n) p[i] = __longlong_as_double((long long)MPK_SENTINEL);
 }
 
+// Which sweeps of a PC's plans are narrow and their slot count (hdr[7] and
+// hdr[5] of each: bits 0-3 the first sweep, 4-7 the second; 0 when not
+// narrow), recorded on the host by PCView::schedule when it writes them, so
+// an apply picks the kernel without reading the device.  A plan this
+// process did not write counts as not narrow.
+static std::mutex g_narrow_mu;
+static std::map<const char *, int> &narrow_plans() {
+  static std::map<const char *, int> m;
+  return m;
+}
+static void note_narrow(const char *plan, int bits) {
+  std::lock_guard<std::mutex> lk(g_narrow_mu);
+  narrow_plans()[plan] = bits;
+}
+// The slot count (4 or 8) of a narrow sweep, else 0.  bits: 4 bits a sweep.
+static int plan_narrow(const char *plan, int second) {
+  std::lock_guard<std::mutex> lk(g_narrow_mu);
+  auto it = narrow_plans().find(plan);
+  return it == narrow_plans().end() ? 0 : (it->second >> (second ? 4 : 0)) & 15;
+}
+
 // One sweep: over its block plan when the PC has one (`plan`, and `second`
-// for the PC's second sweep), else in row order.
+// for the PC's second sweep), else in row order.  A narrow plan needs no
+// sentinel fill (k_sweep_narrow polls nothing): two launches.
 template <class Prog>
 static void run_sweep(const Prog &prog, i64 n, bool forward, double *pub, const int64_t *stop,
                       const char *plan = nullptr, int second = 0, Staged *st = nullptr) {
   Ctx *c = ctx();
   if (!c || n <= 0) return;
+  const int narrow = plan ? plan_narrow(plan, second) : 0;  // the plan's slot count, or 0
+  if (narrow) {
+    MPK_LAUNCH((k_sweep_stage<Prog, Staged>), nblocks(n, 256), 256, prog, n, plan, second, st, stop);
+    if (narrow == 4)
+      MPK_LAUNCH((k_sweep_narrow<Prog, 4>), 1, 2 * MPK_NARROW_ROWS, prog, n, plan, second,
+                 (const Staged *)st, pub, stop);
+    else
+      MPK_LAUNCH((k_sweep_narrow<Prog, 8>), 1, 2 * MPK_NARROW_ROWS, prog, n, plan, second,
+                 (const Staged *)st, pub, stop);
+    return;
+  }
   MPK_LAUNCH(k_sweep_prepare, nblocks(n, 256), 256, n, pub, c->tickets, stop);
   if (plan) {
     MPK_LAUNCH((k_sweep_stage<Prog, Staged>), nblocks(n, 256), 256, prog, n, plan, second, st, stop);
@@ -90,7 +124,7 @@ static bool describe_sweep(const Prog &prog, i64 n, std::vector<i64> &off, std::
 // The block plan of one sweep into dst [D] (cap bytes, at least the bound);
 // returns its size in bytes and the deepest block's level count, or -1.
 template <class Prog>
-static i64 plan_sweep(const Prog &prog, i64 n, bool forward, char *dst, i64 cap, i64 *depth) {
+static i64 plan_sweep(const Prog &prog, i64 n, bool forward, char *dst, i64 cap, i64 *depth, int *narrow) {
   std::vector<i64> off;
   std::vector<int> col;
   std::vector<double> coef;
@@ -110,6 +144,7 @@ static i64 plan_sweep(const Prog &prog, i64 n, bool forward, char *dst, i64 cap,
   }
   copy_htod(dst, host.get(), bytes);
   *depth = host[3];
+  *narrow = host[7] == 1 ? (int)host[5] : 0;  // narrow: its slot count
   return failed() ? -1 : bytes;
 }
 
@@ -380,18 +415,22 @@ i64 PCView::schedule_size() const {
 i64 PCView::schedule(char *plan, i64 cap) const {
   i64 used = 0, depth = 0;
   bool bad = false;
+  int bits = 0, which = 0;
   const bool any = with_sweeps(*this, [&](auto prog, bool forward) {
     if (bad) return;
     i64 d = 0;
-    const i64 bytes = plan_sweep(prog, n, forward, plan + used, cap - used, &d);
+    int narrow = 0;
+    const i64 bytes = plan_sweep(prog, n, forward, plan + used, cap - used, &d, &narrow);
     if (bytes < 0) {
       bad = true;
       return;
     }
+    bits |= narrow << (4 * which++);
     used += bytes;
     depth = d > depth ? d : depth;
   });
   if (!any) return 0;
+  note_narrow(plan, bad ? 0 : bits);
   return bad ? -1 : depth;
 }
 

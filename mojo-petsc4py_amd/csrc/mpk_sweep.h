@@ -526,6 +526,157 @@ __global__ void __launch_bounds__(256) k_sweep_blk(Prog prog, i64 n, const char 
     plan_levels<MPK_PLAN_PF, Tr>(prog, coef, n, P, st, b, val, pub, nullptr);
 }
 
+// Narrow plans (the planner marks them in hdr[7]): a single block, or a ring,
+// whose levels hold at most MPK_NARROW_ROWS rows and number at most
+// MPK_NARROW_LVLS, rows in w = 4 or 8 fixed slots, and no entry reading
+// `pub` (every value a row reads is in LDS).  One workgroup runs every level.
+// Waves 0-3 compute a level, a row per thread, from LDS only: the row's
+// entries and staged values (written one level ahead into one of two LDS
+// buffers) and the values of earlier rows.  Waves 4-7 stage: each keeps the
+// loads of the next three levels' rows in flight (at most 60 loads, under the
+// 63 a wave can have outstanding) and, each level, writes the oldest of them
+// to LDS.  Every staging load is unconditional (a
+// spare thread loads a valid position it then ignores), so each wait is for
+// exactly the oldest level's loads; the computing waves wait on LDS only.
+// Nothing is polled, so `pub` needs no sentinel fill.
+#define MPK_NARROW_ROWS 256
+#define MPK_NARROW_LVLS 4096
+
+template <int SL>
+struct NarrowStage {  // one level's rows, structure of arrays
+  int cnt[MPK_NARROW_ROWS], row[MPK_NARROW_ROWS];
+  double2 st0[MPK_NARROW_ROWS], st1[MPK_NARROW_ROWS];  // a Staged record as two pairs
+  double c[SL][MPK_NARROW_ROWS];
+  int d[SL][MPK_NARROW_ROWS];
+};
+
+template <int SL>
+struct NarrowLoad {  // a stager's loads of one position
+  int cnt, row;
+  double2 st0, st1;
+  double c[SL];
+  int d[SL];
+};
+
+template <int SL>
+__device__ __forceinline__ NarrowLoad<SL> narrow_load(const PlanView &P, const Staged *st, int p) {
+  NarrowLoad<SL> r;
+  r.cnt = P.pcnt[p];
+  r.row = P.prow[p];
+  // the 32-byte record as two 16-byte loads (a struct copy goes through scratch)
+  const double2 *q = reinterpret_cast<const double2 *>(st + p);
+  r.st0 = q[0];
+  r.st1 = q[1];
+#pragma unroll
+  for (int e = 0; e < SL; e++) {
+    r.c[e] = P.ecoef[(i64)e * P.n + p];
+    r.d[e] = P.edep[(i64)e * P.n + p];
+  }
+  return r;
+}
+
+template <int SL, class Prog>
+__device__ __forceinline__ void narrow_levels(const Prog &prog, i64 n, const PlanView &P, const Staged *st, double *pub,
+                                              double *val, int *lv, NarrowStage<SL> *buf) {
+  const int t = (int)threadIdx.x;
+  const bool stager = t >= MPK_NARROW_ROWS;
+  const int s = t - MPK_NARROW_ROWS;  // a stager's lane in the level
+  const int l0 = P.blk_lev[0], l1 = P.blk_lev[1] - 1;  // levels l0..l1-1; lvl[l1] ends the block
+  const int nl = l1 - l0, p0 = P.blk_pos[0];
+  // position of stager s in level l (a valid one past the level's end or the
+  // last level: its loads are then ignored)
+  // (the helpers capture by value: a by-reference capture would move the
+  // plan and the program to scratch memory)
+  auto spos = [=](int l) {
+    const int k = l < nl ? l : nl - 1;
+    const int p = lv[k] + s;
+    return p < lv[k + 1] ? p : lv[k];
+  };
+  auto put = [=](const NarrowLoad<SL> &r, int l) {  // level l's row of stager s into its buffer
+    if (l < nl && lv[l] + s < lv[l + 1]) {
+      NarrowStage<SL> &b = buf[l & 1];
+      b.cnt[s] = r.cnt;
+      b.row[s] = r.row;
+      b.st0[s] = r.st0;
+      b.st1[s] = r.st1;
+#pragma unroll
+      for (int e = 0; e < SL; e++) {
+        b.c[e][s] = r.c[e];
+        b.d[e][s] = r.d[e];
+      }
+    }
+  };
+  auto compute = [=](int l) {  // level l from buffer l & 1, thread t's row
+    const NarrowStage<SL> &b = buf[l & 1];
+    if (lv[l] + t >= lv[l + 1]) return;
+    const int cnt = b.cnt[t];
+    const i64 i = b.row[t];
+    const double2 a0 = b.st0[t], a1 = b.st1[t];
+    const Staged sr{a0.x, a0.y, a1.x, (i64)__double_as_longlong(a1.y)};
+    double c[SL], v[SL];
+#pragma unroll
+    for (int e = 0; e < SL; e++)
+      if (e < cnt) {
+        const int de = b.d[e][t];
+        c[e] = b.c[e][t];
+        v[e] = de >= 0 ? val[de] : prog.stale(-(i64)de - n - 1);  // no -n <= de < 0 in a narrow plan
+      }
+    double acc = row_start(sr);
+#pragma unroll
+    for (int e = 0; e < SL; e++)
+      if (e < cnt) acc = fma(c[e], v[e], acc);
+    const double out = prog.fin(i, acc, sr);
+    val[(lv[l] + t - p0) & P.mask] = out;
+    store_pub(pub, i, out);
+  };
+  // stagers: the loads of levels 0..2 in flight, level 0 in LDS, then 3
+  // loading in the set level 0 left
+  NarrowLoad<SL> g0, g1, g2;
+  if (stager) {
+    g0 = narrow_load<SL>(P, st, spos(0));
+    g1 = narrow_load<SL>(P, st, spos(1));
+    g2 = narrow_load<SL>(P, st, spos(2));
+    put(g0, 0);
+    g0 = narrow_load<SL>(P, st, spos(3));
+  }
+  __syncthreads();
+  // level l: computing waves run it from buffer l & 1; stagers write level
+  // l + 1 (loaded three levels earlier, in set r) to the other buffer and
+  // reload r with level l + 4.  Unrolled by three so that no set is copied
+  // (a copy would wait for the loads still in flight).
+  auto level = [=](NarrowLoad<SL> &r, int l) {
+    if (l >= nl) return;  // uniform
+    if (stager) {
+      put(r, l + 1);
+      r = narrow_load<SL>(P, st, spos(l + 4));
+    } else {
+      compute(l);
+    }
+    __syncthreads();
+  };
+  for (int l = 0; l < nl; l += 3) {
+    level(g1, l);
+    level(g2, l + 1);
+    level(g0, l + 2);
+  }
+}
+
+template <class Prog, int SL>
+__global__ void __launch_bounds__(2 * MPK_NARROW_ROWS) k_sweep_narrow(Prog prog, i64 n, const char *plan, int second,
+                                                                      const Staged *st, double *pub,
+                                                                      const int64_t *stop) {
+  __shared__ double val[MPK_BLK_ROWS];
+  __shared__ int lv[MPK_NARROW_LVLS];
+  __shared__ NarrowStage<SL> buf[2];
+  if (stop && *stop) return;
+  if (second) plan += reinterpret_cast<const i64 *>(plan)[4];
+  const PlanView P = plan_view(plan, n);
+  const int l0 = P.blk_lev[0], l1 = P.blk_lev[1] - 1;
+  for (int l = (int)threadIdx.x; l <= l1 - l0; l += 2 * MPK_NARROW_ROWS) lv[l] = P.lvl[l0 + l];
+  __syncthreads();
+  narrow_levels<SL>(prog, n, P, st, pub, val, lv, buf);
+}
+
 // Entries of a sweep in the rows' own order, for the host-side planner:
 // count[i] (pass 0), then col / coef at off[i].. (pass 1).
 template <class Prog>

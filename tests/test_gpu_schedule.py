@@ -133,12 +133,24 @@ def test_plan_matches_host_planner(native, plan_mode, case):
 
 
 @pytest.mark.parametrize("plan_mode,case", MODES, ids=IDS, indirect=["plan_mode"])
-def test_scheduled_apply_bitwise(native, oracle, plan_mode, case):
+@pytest.mark.parametrize("narrow", ["default", "0"])
+def test_scheduled_apply_bitwise(native, oracle, monkeypatch, plan_mode, case, narrow):
+    """Applies through the plans (and, where the planner marks a plan
+    narrow, through the one-launch narrow kernel; MPK_SWEEP_NARROW=0 keeps
+    the block engine everywhere) give the oracle's bits."""
     from mojopetsc4py import PETSc
     name, kind, levels, ordering = case
+    if narrow == "0":
+        monkeypatch.setenv("MPK_SWEEP_NARROW", "0")
     oracle.set_mode("dev")
     try:
         csr, A, pc = _setup(name, kind, levels, ordering)
+        raw = back(pc._dbufs["sched"], np.uint8, 64)
+        marked = int(raw.view(np.int64)[7])
+        if narrow == "0":
+            assert marked == 0
+        elif name in ("lap32", "tri50", "lap100", "diff8") and plan_mode == "auto" and levels == 0:
+            assert marked == 1, "a small or narrow sweep takes the narrow kernel"
         n = A.getSize()[0]
         Ao = oracle.CSR(*csr)
         if kind == "sor":

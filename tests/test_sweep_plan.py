@@ -365,3 +365,43 @@ def test_check_catches_a_ring_read_outside_the_window(L, monkeypatch):
     bad["ring"] = 16
     bad["edep"] = np.where(plan["edep"] >= 0, plan["edep"] % 16, plan["edep"])
     assert not _check(bad, n, True, off, col, coef)
+
+
+def _narrow_rule(raw, n):
+    """The narrow mark restated: one block, 4 or 8 fixed slots, at most 256
+    rows a level and 4096 levels, and no read of a published value."""
+    h = raw[:64].view(np.int64)
+    p = _unpack(raw, n)
+    if p["nblk"] != 1 or p["w"] not in (4, 8) or len(p["lvl"]) > 4096:
+        return 0
+    lv = p["lvl"][p["blk_lev"][0]:p["blk_lev"][1]]
+    if len(lv) > 1 and np.diff(lv).max() > 256:
+        return 0
+    d = p["edep"]
+    return 0 if np.any((d < 0) & (d >= -n)) else 1
+
+
+@pytest.mark.parametrize("name,want", [("lap20", 1), ("diff7", 1), ("cd6", 1), ("lap100", 1), ("diff20", 0),
+                                       ("lap300", 0)])
+@pytest.mark.parametrize("forward", [True, False])
+def test_narrow_mark(L, monkeypatch, name, want, forward):
+    """hdr[7] marks the plans the one-workgroup narrow kernel runs (single
+    block or ring, narrow levels, every read in LDS): small systems and
+    2-D grids up to 256 wide take it; a 20^3 operator in blocks (reads across
+    blocks) and a 300-wide grid (levels of 300 rows) do not."""
+    mats = dict(SMALL, lap100=lambda: W.laplacian_2d(100), diff20=lambda: W.diffusion_3d(20)[0],
+                lap300=lambda: W.laplacian_2d(300))
+    off, col, coef = _lower(*mats[name](), forward)
+    n = len(off) - 1
+    m = int(off[n])
+    cap = L.mpk_sweep_plan_bound(n, m, 8192)
+    buf = np.zeros((cap + 7) // 8, np.int64)
+    nb = L.mpk_sweep_plan(n, 1 if forward else 0, off.ctypes.data, col.ctypes.data, coef.ctypes.data, 8192,
+                          buf.ctypes.data, cap)
+    raw = buf.view(np.uint8)[:nb]
+    assert int(raw[:64].view(np.int64)[7]) == _narrow_rule(raw, n) == want
+    monkeypatch.setenv("MPK_SWEEP_NARROW", "0")
+    nb = L.mpk_sweep_plan(n, 1 if forward else 0, off.ctypes.data, col.ctypes.data, coef.ctypes.data, 8192,
+                          buf.ctypes.data, cap)
+    assert int(buf[7]) == 0
+
