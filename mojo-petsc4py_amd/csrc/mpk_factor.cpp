@@ -107,31 +107,75 @@ static inline void copy_ids(I *dst, const I *src, I count) {
   if (count > 0) memcpy(dst, src, sizeof(I) * (size_t)count);
 }
 
-// hdr[7] = 1 marks a narrow plan, which the device runs in one workgroup
-// with its rows staged a few levels ahead (k_sweep_narrow): one block (a
-// single-block plan or a ring), 4 or 8 fixed slots, at most 256 rows a level
-// and 4096 levels, and no entry reading another block's published value (d
-// in [-n, 0)).  MPK_SWEEP_NARROW=0 leaves every plan unmarked.
-static void mark_narrow(char *out, int64_t n) {
+// hdr[7] marks the plans the device runs in one workgroup, every level in
+// turn, without waits on other workgroups:
+//   1, narrow (k_sweep_narrow): one block (a single-block plan or a ring), 4
+//      or 8 fixed slots, at most 256 rows a level and 4095 levels, and no
+//      entry reading another block's published value (d in [-n, 0));
+//   2, wave (k_sweep_wave): as narrow, with at most 64 rows a level and
+//      12288 levels and no stale entry (d < -n); one wave runs every level,
+//      a row per lane.  Its records follow the plan (at plan_bytes, a
+//      multiple of 16): per position, the w coefficients (-0.0 past the
+//      row's count), the w LDS slots (kWaveZero past the count, a slot that
+//      holds 0.0, so that a padded entry adds fma(-0.0, 0.0, acc) == acc
+//      exactly), the row and the count, in wave_rec_bytes(w) bytes; then a
+//      byte per level, its longest row's count.
+// MPK_SWEEP_NARROW=0 leaves every plan unmarked; MPK_SWEEP_WAVE=0 marks no
+// plan 2.  Returns the plan's size with the records.
+static constexpr int kWaveZero = 8192;
+static int64_t wave_rec_bytes(int64_t w) { return (12 * w + 8 + 15) & ~(int64_t)15; }
+static bool env_off(const char *name) {
+  const char *e = std::getenv(name);
+  return e && e[0] == '0' && !e[1];
+}
+static int64_t mark_plan(char *out, int64_t n, int64_t bytes, int64_t cap) {
   int64_t *hdr = (int64_t *)out;
   hdr[7] = 0;
-  const char *e = std::getenv("MPK_SWEEP_NARROW");
-  if (e && e[0] == '0' && !e[1]) return;
+  if (env_off("MPK_SWEEP_NARROW")) return bytes;
   const int64_t nblk = hdr[0], nlvl = hdr[2], w = hdr[5];
-  if (nblk != 1 || (w != 4 && w != 8) || nlvl > 4096) return;
+  if (nblk != 1 || (w != 4 && w != 8)) return bytes;
   const int *blk_pos = (const int *)(out + 64), *blk_lev = blk_pos + nblk + 1, *lvl = blk_lev + nblk + 1;
-  for (int64_t l = blk_lev[0]; l + 1 < blk_lev[1]; l++)
-    if (lvl[l + 1] - lvl[l] > 256) return;
+  int widest = 0;
+  for (int64_t l = blk_lev[0]; l + 1 < blk_lev[1]; l++) widest = std::max(widest, lvl[l + 1] - lvl[l]);
+  const int64_t levels = blk_lev[1] - 1 - blk_lev[0];
   const int64_t ints = 2 * (nblk + 1) + nlvl + 3 * n + 1;
   const double *ecoef = (const double *)(out + 64 + ((4 * ints + 7) & ~(int64_t)7));
   const int *edep = (const int *)(ecoef + w * n);
-  const int *pcnt = lvl + nlvl + n;
+  const int *prow = lvl + nlvl, *pcnt = prow + n;
+  bool stale = false;
   for (int64_t p = 0; p < n; p++)
     for (int64_t k = 0; k < pcnt[p]; k++) {
       const int d = edep[k * n + p];
-      if (d < 0 && d >= -n) return;
+      if (d < 0 && d >= -n) return bytes;
+      stale |= d < -n;
     }
-  hdr[7] = 1;
+  const bool narrow = widest <= 256 && levels < 4096;  // its kernel holds levels + 1 bounds
+  const int64_t rb = wave_rec_bytes(w), wbytes = bytes + n * rb + ((levels + 15) & ~(int64_t)15);
+  const bool wave = widest <= 64 && levels <= 12288 && !stale && wbytes <= cap && !env_off("MPK_SWEEP_WAVE");
+  if (!wave) {
+    hdr[7] = narrow ? 1 : 0;
+    return bytes;
+  }
+  std::memset(out + bytes, 0, (size_t)(wbytes - bytes));  // the padding too
+  for (int64_t p = 0; p < n; p++) {
+    char *r = out + bytes + p * rb;
+    double *c = (double *)r;
+    int *d = (int *)(c + w);
+    for (int64_t e = 0; e < w; e++) {
+      const bool used = e < pcnt[p];
+      c[e] = used ? ecoef[e * n + p] : -0.0;
+      d[e] = used ? edep[e * n + p] : kWaveZero;
+    }
+    d[w] = prow[p];
+    d[w + 1] = pcnt[p];
+  }
+  unsigned char *lmax = (unsigned char *)(out + bytes + n * rb);
+  for (int64_t l = 0; l < levels; l++)
+    for (int p = lvl[blk_lev[0] + l]; p < lvl[blk_lev[0] + l + 1]; p++)
+      lmax[l] = (unsigned char)std::max<int>(lmax[l], pcnt[p]);
+  hdr[4] = wbytes;
+  hdr[7] = 2;
+  return wbytes;
 }
 
 extern "C" {
@@ -610,15 +654,18 @@ static int64_t plan_w(int64_t widest, int64_t n, int64_t m) {
   w = (widest + 7) / 8 * 8;
   return w <= 64 && n * w <= 4 * m ? w : 0;
 }
+// (a multiple of 16, so that plans placed one after another keep the wave
+// records 16-byte aligned)
 static int64_t plan_bytes(int64_t n, int64_t nblk, int64_t nlvl, int64_t slots) {
   const int64_t ints = 2 * (nblk + 1) + nlvl + 3 * n + 1;
-  return 64 + ((4 * ints + 7) & ~(int64_t)7) + ((12 * slots + 7) & ~(int64_t)7);
+  return (64 + ((4 * ints + 7) & ~(int64_t)7) + ((12 * slots + 7) & ~(int64_t)7) + 15) & ~(int64_t)15;
 }
 
-// Upper bound of the plan size in bytes for n rows and m entries.
+// Upper bound of the plan size in bytes for n rows and m entries (with the
+// records of a wave plan).
 int64_t mpk_sweep_plan_bound(int64_t n, int64_t m, int64_t bmax) {
   const int64_t B = plan_rows(n, bmax), nblk = n > 0 ? (n + B - 1) / B : 0;
-  return plan_bytes(n, nblk, n + nblk, std::max(4 * m, 32 * n));
+  return plan_bytes(n, nblk, n + nblk, std::max(4 * m, 32 * n)) + n * wave_rec_bytes(8) + ((n + 16) & ~(int64_t)15);
 }
 
 // Ring plans.  A sweep too long for one block whose DAG is deep and narrow
@@ -738,8 +785,7 @@ int64_t mpk_sweep_plan(int64_t n, int64_t forward, int64_t off_, int64_t col_, i
         }
       }
       pent[n] = w ? 0 : (int)e_out;
-      mark_narrow(out, n);
-      return bytes;
+      return mark_plan(out, n, bytes, cap);
     }
   }
 blocks:
@@ -821,8 +867,7 @@ blocks:
   pent[n] = (int)(w ? 0 : e_out);
   if (w)
     for (int64_t p = 0; p <= n; p++) pent[p] = 0;  // unused with fixed slots
-  mark_narrow(out, n);
-  return bytes;
+  return mark_plan(out, n, bytes, cap);
 }
 
 }  // extern "C"

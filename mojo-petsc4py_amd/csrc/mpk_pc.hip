@@ -33,11 +33,11 @@ __global__ void k_sweep_prepare(i64 n, double *p, unsigned *ticket, const int64_
   if (i < n) p[i] = __longlong_as_double((long long)MPK_SENTINEL);
 }
 
-// Which sweeps of a PC's plans are narrow and their slot count (hdr[7] and
-// hdr[5] of each: bits 0-3 the first sweep, 4-7 the second; 0 when not
-// narrow), recorded on the host by PCView::schedule when it writes them, so
-// an apply picks the kernel without reading the device.  A plan this
-// process did not write counts as not narrow.
+// Which sweeps of a PC's plans run in one workgroup and how (hdr[7] and
+// hdr[5] of each: bits 0-7 the first sweep, 8-15 the second; a sweep's byte
+// is its slot count, plus 16 for a wave plan, or 0), recorded on the host by
+// PCView::schedule when it writes them, so an apply picks the kernel without
+// reading the device.  A plan this process did not write counts as neither.
 static std::mutex g_narrow_mu;
 static std::map<const char *, int> &narrow_plans() {
   static std::map<const char *, int> m;
@@ -47,29 +47,38 @@ static void note_narrow(const char *plan, int bits) {
   std::lock_guard<std::mutex> lk(g_narrow_mu);
   narrow_plans()[plan] = bits;
 }
-// The slot count (4 or 8) of a narrow sweep, else 0.  bits: 4 bits a sweep.
+// A sweep's byte of the record above.
 static int plan_narrow(const char *plan, int second) {
   std::lock_guard<std::mutex> lk(g_narrow_mu);
   auto it = narrow_plans().find(plan);
-  return it == narrow_plans().end() ? 0 : (it->second >> (second ? 4 : 0)) & 15;
+  return it == narrow_plans().end() ? 0 : (it->second >> (second ? 8 : 0)) & 255;
 }
 
 // One sweep: over its block plan when the PC has one (`plan`, and `second`
-// for the PC's second sweep), else in row order.  A narrow plan needs no
-// sentinel fill (k_sweep_narrow polls nothing): two launches.
+// for the PC's second sweep), else in row order.  Narrow and wave plans need
+// no sentinel fill (their kernels poll nothing): two launches for a narrow
+// plan, one for a wave plan (its kernel stages the rows itself).
 template <class Prog>
 static void run_sweep(const Prog &prog, i64 n, bool forward, double *pub, const int64_t *stop,
                       const char *plan = nullptr, int second = 0, Staged *st = nullptr) {
   Ctx *c = ctx();
   if (!c || n <= 0) return;
-  const int narrow = plan ? plan_narrow(plan, second) : 0;  // the plan's slot count, or 0
+  const int form = plan ? plan_narrow(plan, second) : 0;
+  const int narrow = form & 15;  // the plan's slot count, or 0
+  if (form & 16) {
+    if (narrow == 4)
+      MPK_LAUNCH((k_sweep_wave<Prog, 4>), 1, MPK_WAVE_THREADS, prog, n, plan, second, st, pub, stop);
+    else
+      MPK_LAUNCH((k_sweep_wave<Prog, 8>), 1, MPK_WAVE_THREADS, prog, n, plan, second, st, pub, stop);
+    return;
+  }
   if (narrow) {
     MPK_LAUNCH((k_sweep_stage<Prog, Staged>), nblocks(n, 256), 256, prog, n, plan, second, st, stop);
     if (narrow == 4)
-      MPK_LAUNCH((k_sweep_narrow<Prog, 4>), 1, 2 * MPK_NARROW_ROWS, prog, n, plan, second,
+      MPK_LAUNCH((k_sweep_narrow<Prog, 4>), 1, MPK_NARROW_THREADS, prog, n, plan, second,
                  (const Staged *)st, pub, stop);
     else
-      MPK_LAUNCH((k_sweep_narrow<Prog, 8>), 1, 2 * MPK_NARROW_ROWS, prog, n, plan, second,
+      MPK_LAUNCH((k_sweep_narrow<Prog, 8>), 1, MPK_NARROW_THREADS, prog, n, plan, second,
                  (const Staged *)st, pub, stop);
     return;
   }
@@ -144,7 +153,8 @@ static i64 plan_sweep(const Prog &prog, i64 n, bool forward, char *dst, i64 cap,
   }
   copy_htod(dst, host.get(), bytes);
   *depth = host[3];
-  *narrow = host[7] == 1 ? (int)host[5] : 0;  // narrow: its slot count
+  // narrow: its slot count; wave: the count plus 16
+  *narrow = host[7] == 1 ? (int)host[5] : host[7] == 2 ? (int)host[5] | 16 : 0;
   return failed() ? -1 : bytes;
 }
 
@@ -425,7 +435,7 @@ i64 PCView::schedule(char *plan, i64 cap) const {
       bad = true;
       return;
     }
-    bits |= narrow << (4 * which++);
+    bits |= narrow << (8 * which++);
     used += bytes;
     depth = d > depth ? d : depth;
   });

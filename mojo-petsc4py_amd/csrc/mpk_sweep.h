@@ -178,6 +178,7 @@ struct PlanView {
   int mask;  // LDS slot of position p in block b: (p - blk_pos[b]) & mask
   const int *blk_pos, *blk_lev, *lvl, *prow, *pcnt, *pent, *edep;
   const double *ecoef;
+  const char *wrec;  // a wave plan's records (hdr[7] = 2)
 };
 
 // A ring plan (hdr[6] = W > 0) is one block over the sweep's global levels
@@ -210,6 +211,7 @@ __host__ __device__ inline PlanView plan_view(const char *base, i64 n) {
   const i64 ints = 2 * (v.nblk + 1) + nlvl + 3 * n + 1;
   v.ecoef = reinterpret_cast<const double *>(base + 64 + plan_align8(4 * ints));
   v.edep = reinterpret_cast<const int *>(v.ecoef + slots);
+  v.wrec = base + ((64 + plan_align8(4 * ints) + plan_align8(12 * slots) + 15) & ~(i64)15);
   return v;
 }
 
@@ -529,18 +531,20 @@ __global__ void __launch_bounds__(256) k_sweep_blk(Prog prog, i64 n, const char 
 // Narrow plans (the planner marks them in hdr[7]): a single block, or a ring,
 // whose levels hold at most MPK_NARROW_ROWS rows and number at most
 // MPK_NARROW_LVLS, rows in w = 4 or 8 fixed slots, and no entry reading
-// `pub` (every value a row reads is in LDS).  One workgroup runs every level.
-// Waves 0-3 compute a level, a row per thread, from LDS only: the row's
-// entries and staged values (written one level ahead into one of two LDS
-// buffers) and the values of earlier rows.  Waves 4-7 stage: each keeps the
-// loads of the next three levels' rows in flight (at most 60 loads, under the
-// 63 a wave can have outstanding) and, each level, writes the oldest of them
-// to LDS.  Every staging load is unconditional (a
-// spare thread loads a valid position it then ignores), so each wait is for
-// exactly the oldest level's loads; the computing waves wait on LDS only.
+// `pub` (every value a row reads is in LDS).  One workgroup runs every level,
+// one barrier per level.  Waves 0-3 compute a level, a row per thread, from
+// LDS only: the row's record (written one level ahead into one of two LDS
+// buffers) and the values of earlier rows.  The other waves stage, in
+// MPK_NARROW_GROUPS groups of four: group g holds the loads of the levels
+// l = g (mod MPK_NARROW_GROUPS), issued that many levels before the level
+// runs, and writes them to LDS the level before.  A stager wave thus keeps
+// one level's loads in flight and waits only for its own; every staging load
+// is unconditional (a spare lane loads a valid position it then ignores).
 // Nothing is polled, so `pub` needs no sentinel fill.
 #define MPK_NARROW_ROWS 256
 #define MPK_NARROW_LVLS 4096
+#define MPK_NARROW_GROUPS 3
+#define MPK_NARROW_THREADS ((1 + MPK_NARROW_GROUPS) * MPK_NARROW_ROWS)
 
 template <int SL>
 struct NarrowStage {  // one level's rows, structure of arrays
@@ -552,119 +556,131 @@ struct NarrowStage {  // one level's rows, structure of arrays
 
 template <int SL>
 struct NarrowLoad {  // a stager's loads of one position
-  int cnt, row;
-  double2 st0, st1;
+  double s[4];  // the Staged record's four words
   double c[SL];
   int d[SL];
+  int cnt, row;
 };
 
+// (loaded into the fields in place: a returned or assigned record is copied
+// through scratch memory)
 template <int SL>
-__device__ __forceinline__ NarrowLoad<SL> narrow_load(const PlanView &P, const Staged *st, int p) {
-  NarrowLoad<SL> r;
+__device__ __forceinline__ void narrow_load(NarrowLoad<SL> &r, const PlanView &P, const Staged *st, int p) {
   r.cnt = P.pcnt[p];
   r.row = P.prow[p];
   // the 32-byte record as two 16-byte loads (a struct copy goes through scratch)
   const double2 *q = reinterpret_cast<const double2 *>(st + p);
-  r.st0 = q[0];
-  r.st1 = q[1];
+  const double2 a = q[0], b = q[1];
+  r.s[0] = a.x;
+  r.s[1] = a.y;
+  r.s[2] = b.x;
+  r.s[3] = b.y;
 #pragma unroll
   for (int e = 0; e < SL; e++) {
     r.c[e] = P.ecoef[(i64)e * P.n + p];
     r.d[e] = P.edep[(i64)e * P.n + p];
   }
-  return r;
+}
+
+template <int SL>
+__device__ __forceinline__ void narrow_put(NarrowStage<SL> &b, int s, const NarrowLoad<SL> &r) {
+  b.cnt[s] = r.cnt;
+  b.row[s] = r.row;
+  b.st0[s] = make_double2(r.s[0], r.s[1]);
+  b.st1[s] = make_double2(r.s[2], r.s[3]);
+#pragma unroll
+  for (int e = 0; e < SL; e++) {
+    b.c[e][s] = r.c[e];
+    b.d[e][s] = r.d[e];
+  }
+}
+
+// Position of stager lane s in level l of a narrow block (lv: the block's
+// level bounds, nl levels): a valid position also past the level's end or
+// past the last level, whose record is then loaded and never read.
+__device__ __forceinline__ int narrow_spos(const int *lv, int nl, int s, int l) {
+  const int k = l < nl ? l : nl - 1;
+  const int p = lv[k] + s;
+  return p < lv[k + 1] ? p : lv[k];
 }
 
 template <int SL, class Prog>
 __device__ __forceinline__ void narrow_levels(const Prog &prog, i64 n, const PlanView &P, const Staged *st, double *pub,
-                                              double *val, int *lv, NarrowStage<SL> *buf) {
+                                              double *val, const int *lv, NarrowStage<SL> *buf) {
+  constexpr int G = MPK_NARROW_GROUPS;
   const int t = (int)threadIdx.x;
-  const bool stager = t >= MPK_NARROW_ROWS;
-  const int s = t - MPK_NARROW_ROWS;  // a stager's lane in the level
   const int l0 = P.blk_lev[0], l1 = P.blk_lev[1] - 1;  // levels l0..l1-1; lvl[l1] ends the block
   const int nl = l1 - l0, p0 = P.blk_pos[0];
-  // position of stager s in level l (a valid one past the level's end or the
-  // last level: its loads are then ignored)
-  // (the helpers capture by value: a by-reference capture would move the
-  // plan and the program to scratch memory)
-  auto spos = [=](int l) {
-    const int k = l < nl ? l : nl - 1;
-    const int p = lv[k] + s;
-    return p < lv[k + 1] ? p : lv[k];
-  };
-  auto put = [=](const NarrowLoad<SL> &r, int l) {  // level l's row of stager s into its buffer
-    if (l < nl && lv[l] + s < lv[l + 1]) {
-      NarrowStage<SL> &b = buf[l & 1];
-      b.cnt[s] = r.cnt;
-      b.row[s] = r.row;
-      b.st0[s] = r.st0;
-      b.st1[s] = r.st1;
-#pragma unroll
-      for (int e = 0; e < SL; e++) {
-        b.c[e][s] = r.c[e];
-        b.d[e][s] = r.d[e];
-      }
-    }
-  };
-  auto compute = [=](int l) {  // level l from buffer l & 1, thread t's row
-    const NarrowStage<SL> &b = buf[l & 1];
-    if (lv[l] + t >= lv[l + 1]) return;
-    const int cnt = b.cnt[t];
-    const i64 i = b.row[t];
-    const double2 a0 = b.st0[t], a1 = b.st1[t];
-    const Staged sr{a0.x, a0.y, a1.x, (i64)__double_as_longlong(a1.y)};
-    double c[SL], v[SL];
-#pragma unroll
-    for (int e = 0; e < SL; e++)
-      if (e < cnt) {
-        const int de = b.d[e][t];
-        c[e] = b.c[e][t];
-        v[e] = de >= 0 ? val[de] : prog.stale(-(i64)de - n - 1);  // no -n <= de < 0 in a narrow plan
-      }
-    double acc = row_start(sr);
-#pragma unroll
-    for (int e = 0; e < SL; e++)
-      if (e < cnt) acc = fma(c[e], v[e], acc);
-    const double out = prog.fin(i, acc, sr);
-    val[(lv[l] + t - p0) & P.mask] = out;
-    store_pub(pub, i, out);
-  };
-  // stagers: the loads of levels 0..2 in flight, level 0 in LDS, then 3
-  // loading in the set level 0 left
-  NarrowLoad<SL> g0, g1, g2;
-  if (stager) {
-    g0 = narrow_load<SL>(P, st, spos(0));
-    g1 = narrow_load<SL>(P, st, spos(1));
-    g2 = narrow_load<SL>(P, st, spos(2));
-    put(g0, 0);
-    g0 = narrow_load<SL>(P, st, spos(3));
-  }
-  __syncthreads();
-  // level l: computing waves run it from buffer l & 1; stagers write level
-  // l + 1 (loaded three levels earlier, in set r) to the other buffer and
-  // reload r with level l + 4.  Unrolled by three so that no set is copied
-  // (a copy would wait for the loads still in flight).
-  auto level = [=](NarrowLoad<SL> &r, int l) {
-    if (l >= nl) return;  // uniform
-    if (stager) {
-      put(r, l + 1);
-      r = narrow_load<SL>(P, st, spos(l + 4));
-    } else {
-      compute(l);
+  // the roles run separate loops with the same barriers, on a branch that is
+  // uniform per wave
+  const int role = __builtin_amdgcn_readfirstlane(t) / MPK_NARROW_ROWS;  // 0: compute, 1 + g: group g
+  if (role > 0) {
+    const int g = role - 1, s = t - role * MPK_NARROW_ROWS;
+    NarrowLoad<SL> r;
+    narrow_load<SL>(r, P, st, narrow_spos(lv, nl, s, g));
+    int next = g;  // the level r holds
+    if (g == 0) {
+      narrow_put<SL>(buf[0], s, r);
+      narrow_load<SL>(r, P, st, narrow_spos(lv, nl, s, G));
+      next = G;
     }
     __syncthreads();
-  };
-  for (int l = 0; l < nl; l += 3) {
-    level(g1, l);
-    level(g2, l + 1);
-    level(g0, l + 2);
+    for (int l = 0; l < nl; l++) {
+      if (next == l + 1) {  // level l + 1 into the buffer level l is not reading
+        narrow_put<SL>(buf[(l + 1) & 1], s, r);
+        narrow_load<SL>(r, P, st, narrow_spos(lv, nl, s, l + 1 + G));
+        next = l + 1 + G;
+      }
+      __syncthreads();
+    }
+    return;
+  }
+  // computing waves: level l from buffer l & 1, thread t's row.  The gather
+  // has no branches: an entry past the row's count reads slot 0 and is
+  // dropped by a select, so a row's value reads all issue at once.
+  __syncthreads();
+  // level l's positions lo..hi-1, uniform; the next bound is read a level
+  // ahead and taken into a scalar register only at the level's end
+  int lo = __builtin_amdgcn_readfirstlane(lv[0]), hi = __builtin_amdgcn_readfirstlane(lv[1]);
+  for (int l = 0; l < nl; l++) {
+    const int hi_next = lv[l + 2 <= nl ? l + 2 : nl];
+    if (lo + t < hi) {
+      const NarrowStage<SL> &b = buf[l & 1];
+      const int cnt = b.cnt[t];
+      const i64 i = b.row[t];
+      const double2 a0 = b.st0[t], a1 = b.st1[t];
+      const Staged sr{a0.x, a0.y, a1.x, (i64)__double_as_longlong(a1.y)};
+      int de[SL];
+      double c[SL], v[SL];
+#pragma unroll
+      for (int e = 0; e < SL; e++) {  // every read in one round, then the selects
+        de[e] = b.d[e][t];
+        c[e] = b.c[e][t];
+      }
+#pragma unroll
+      for (int e = 0; e < SL; e++) de[e] = e < cnt ? de[e] : 0;
+#pragma unroll
+      for (int e = 0; e < SL; e++) v[e] = val[de[e] >= 0 ? de[e] : 0];
+#pragma unroll
+      for (int e = 0; e < SL; e++)  // no -n <= de < 0 in a narrow plan
+        if (de[e] < 0) v[e] = prog.stale(-(i64)de[e] - n - 1);
+      double acc = row_start(sr);
+#pragma unroll
+      for (int e = 0; e < SL; e++) acc = e < cnt ? fma(c[e], v[e], acc) : acc;
+      const double out = prog.fin(i, acc, sr);
+      val[(lo + t - p0) & P.mask] = out;
+      store_pub(pub, i, out);
+    }
+    lo = hi;
+    hi = __builtin_amdgcn_readfirstlane(hi_next);
+    __syncthreads();
   }
 }
 
 template <class Prog, int SL>
-__global__ void __launch_bounds__(2 * MPK_NARROW_ROWS) k_sweep_narrow(Prog prog, i64 n, const char *plan, int second,
-                                                                      const Staged *st, double *pub,
-                                                                      const int64_t *stop) {
+__global__ void __launch_bounds__(MPK_NARROW_THREADS) k_sweep_narrow(Prog prog, i64 n, const char *plan, int second,
+                                                                     const Staged *st, double *pub,
+                                                                     const int64_t *stop) {
   __shared__ double val[MPK_BLK_ROWS];
   __shared__ int lv[MPK_NARROW_LVLS];
   __shared__ NarrowStage<SL> buf[2];
@@ -672,9 +688,168 @@ __global__ void __launch_bounds__(2 * MPK_NARROW_ROWS) k_sweep_narrow(Prog prog,
   if (second) plan += reinterpret_cast<const i64 *>(plan)[4];
   const PlanView P = plan_view(plan, n);
   const int l0 = P.blk_lev[0], l1 = P.blk_lev[1] - 1;
-  for (int l = (int)threadIdx.x; l <= l1 - l0; l += 2 * MPK_NARROW_ROWS) lv[l] = P.lvl[l0 + l];
+  for (int l = (int)threadIdx.x; l <= l1 - l0; l += MPK_NARROW_THREADS) lv[l] = P.lvl[l0 + l];
   __syncthreads();
   narrow_levels<SL>(prog, n, P, st, pub, val, lv, buf);
+}
+
+// Wave plans (the planner marks them hdr[7] = 2): one block (a single-block
+// plan or a ring) whose levels hold at most 64 rows, every read in LDS and
+// none stale.  All threads first stage every position (prog.stage into st)
+// and copy the level bounds and each level's longest row count into LDS;
+// then one wave runs every level, position lo + lane of a level on lane
+// `lane`.  A wave's LDS operations take effect in issue order, so a level
+// reads what the level before it wrote with no barrier.  The wave holds the
+// records of the next kWaveAhead levels in registers, each loaded that many
+// levels before its level runs (unrolled, so that no set is copied).  Padded
+// entries (coefficient -0.0, slot MPK_WAVE_ZERO holding 0.0) leave a sum
+// unchanged and entries past the level's longest row are skipped on a
+// uniform branch: each row's sum is the same chain of fused multiply-adds as
+// in the other engines.
+#define MPK_WAVE_LVLS 12288
+#define MPK_WAVE_ZERO MPK_BLK_ROWS
+#define MPK_WAVE_THREADS 512
+
+template <int SL>
+struct WaveSet {  // this lane's records of one level
+  static constexpr int kBytes = (12 * SL + 8 + 15) & ~15;  // the plan's record
+  double2 c[SL / 2];  // coefficients
+  int4 d[SL / 4];     // LDS slots
+  int row;
+  double2 s0, s1;     // the Staged record
+  int pos, m, valid;  // position, the level's longest row (uniform), row present
+};
+
+// Loads level l's records for this lane (levels past the last: the last
+// level's, with no row present).  Only the words used are loaded: a load
+// whose result is partly unused leaves registers the compiler may reuse
+// while the load is still in flight, which costs a wait on every load.
+template <int SL>
+__device__ __forceinline__ void wave_fetch(WaveSet<SL> &w, const PlanView &P, const Staged *st, int lo, int hi, int m,
+                                           bool real, int lane) {
+  const int p = lo + lane < hi ? lo + lane : lo;
+  w.pos = p;
+  w.m = __builtin_amdgcn_readfirstlane(m);
+  w.valid = real && lo + lane < hi;
+  const char *r = P.wrec + (i64)p * WaveSet<SL>::kBytes;
+  const double2 *cq = reinterpret_cast<const double2 *>(r);
+  const int4 *dq = reinterpret_cast<const int4 *>(r + 8 * SL);
+#pragma unroll
+  for (int k = 0; k < SL / 2; k++) w.c[k] = cq[k];
+#pragma unroll
+  for (int k = 0; k < SL / 4; k++) w.d[k] = dq[k];
+  w.row = *reinterpret_cast<const int *>(r + 12 * SL);
+  const double2 *sq = reinterpret_cast<const double2 *>(st + p);
+  w.s0 = sq[0];
+  w.s1 = sq[1];
+}
+
+template <int SL>
+__device__ __forceinline__ double wave_coef(const WaveSet<SL> &w, int e) {
+  return (e & 1) ? w.c[e >> 1].y : w.c[e >> 1].x;
+}
+template <int SL>
+__device__ __forceinline__ int wave_slot(const WaveSet<SL> &w, int e) {
+  const int4 q = w.d[e >> 2];
+  return (e & 3) == 0 ? q.x : (e & 3) == 1 ? q.y : (e & 3) == 2 ? q.z : q.w;
+}
+// The first E entries' sum: the reads in one round, then the chain.
+template <int E, int SL>
+__device__ __forceinline__ double wave_sum(const WaveSet<SL> &w, const double *val) {
+  double v[E];
+#pragma unroll
+  for (int e = 0; e < E; e++) v[e] = val[wave_slot(w, e)];
+  double acc = w.s0.x;  // row_start
+#pragma unroll
+  for (int e = 0; e < E; e++) acc = fma(wave_coef(w, e), v[e], acc);
+  return acc;
+}
+
+template <int SL, class Prog>
+__device__ __forceinline__ void wave_step(const Prog &prog, const WaveSet<SL> &w, double *val, double *pub, int p0,
+                                          int mask) {
+  // one straight-line variant per range of the level's longest row (a
+  // uniform branch; entries past a row's own count are padded)
+  double acc;
+  if (w.m <= 2)
+    acc = wave_sum<2>(w, val);
+  else if (SL == 4 || w.m <= 4)
+    acc = wave_sum<4>(w, val);
+  else
+    acc = wave_sum<SL>(w, val);
+  if (w.valid) {
+    const Staged sr{w.s0.x, w.s0.y, w.s1.x, (i64)__double_as_longlong(w.s1.y)};
+    const double out = prog.fin(w.row, acc, sr);
+    val[(w.pos - p0) & mask] = out;
+    store_pub(pub, w.row, out);
+  }
+}
+
+__device__ __forceinline__ void wave_bounds(const int *lv, const unsigned char *lvm, int nl, int l, int &lo, int &hi,
+                                            int &m) {
+  const int k = l < nl ? l : nl - 1;
+  lo = lv[k];
+  hi = lv[k + 1];
+  m = lvm[k];
+}
+
+template <class Prog, int SL>
+__global__ void __launch_bounds__(MPK_WAVE_THREADS) k_sweep_wave(Prog prog, i64 n, const char *plan, int second,
+                                                                 Staged *st, double *pub, const int64_t *stop) {
+  constexpr int kWaveAhead = SL == 4 ? 8 : 5;  // at most 63 loads and stores in flight
+  constexpr int kStageUnroll = 8;
+  __shared__ double val[MPK_BLK_ROWS + 1];
+  __shared__ int lv[MPK_WAVE_LVLS + 1];
+  __shared__ unsigned char lvm[MPK_WAVE_LVLS];
+  if (stop && *stop) return;
+  if (second) plan += reinterpret_cast<const i64 *>(plan)[4];
+  const PlanView P = plan_view(plan, n);
+  const int l0 = P.blk_lev[0], nl = P.blk_lev[1] - 1 - l0;  // levels l0..; lvl[l0 + nl] ends the block
+  const int p0 = P.blk_pos[0], p1 = P.blk_pos[1];
+  const int t = (int)threadIdx.x;
+  const unsigned char *lmax = reinterpret_cast<const unsigned char *>(P.wrec + (i64)n * WaveSet<SL>::kBytes);
+#pragma unroll 4
+  for (int l = t; l <= nl; l += MPK_WAVE_THREADS) lv[l] = P.lvl[l0 + l];
+#pragma unroll 4
+  for (int l = t; l < nl; l += MPK_WAVE_THREADS) lvm[l] = lmax[l];
+  // every position's Staged record, kStageUnroll of them a thread at a time
+  // (their loads all issue before the first store)
+  for (int b = p0 + t; b < p1; b += kStageUnroll * MPK_WAVE_THREADS) {
+    Staged tmp[kStageUnroll];
+#pragma unroll
+    for (int u = 0; u < kStageUnroll; u++) {
+      const int p = b + u * MPK_WAVE_THREADS;
+      if (p < p1) tmp[u] = prog.stage(P.prow[p]);
+    }
+#pragma unroll
+    for (int u = 0; u < kStageUnroll; u++) {
+      const int p = b + u * MPK_WAVE_THREADS;
+      if (p < p1) st[p] = tmp[u];
+    }
+  }
+  if (t == 0) val[MPK_WAVE_ZERO] = 0.0;
+  __syncthreads();  // st, lv, lvm (workgroup scope: the stores to st are seen)
+  if (t >= 64) return;
+  // the first sets in level order: the compiler may not reorder their loads
+  // (its waits in the loop are placed for loads issued in that order)
+  WaveSet<SL> w[kWaveAhead];
+#pragma unroll
+  for (int k = 0; k < kWaveAhead; k++) {
+    int lo, hi, m;
+    wave_bounds(lv, lvm, nl, k, lo, hi, m);
+    wave_fetch<SL>(w[k], P, st, lo, hi, m, k < nl, t);
+    __builtin_amdgcn_sched_barrier(0);
+  }
+  for (int l = 0; l < nl; l += kWaveAhead) {
+#pragma unroll
+    for (int k = 0; k < kWaveAhead; k++) {
+      // the bounds of the level to fetch are read with this level's values
+      int lo, hi, m;
+      wave_bounds(lv, lvm, nl, l + k + kWaveAhead, lo, hi, m);
+      wave_step<SL>(prog, w[k], val, pub, p0, P.mask);
+      wave_fetch<SL>(w[k], P, st, lo, hi, m, l + k + kWaveAhead < nl, t);
+    }
+  }
 }
 
 // Entries of a sweep in the rows' own order, for the host-side planner:

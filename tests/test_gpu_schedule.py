@@ -132,26 +132,47 @@ def test_plan_matches_host_planner(native, plan_mode, case):
     assert pc._sweep_depth == depth
 
 
+def _marks(pc, n):
+    """Each sweep's hdr[7] in the device plans, with the sweep's plan bytes."""
+    raw = back(pc._dbufs["sched"], np.uint8, pc._dbufs["sched"].nbytes)
+    out, at = [], 0
+    while at < len(raw):
+        h = raw[at:at + 64].view(np.int64)
+        nb = int(h[4])
+        if nb <= 0:
+            break
+        out.append((int(h[7]), raw[at:at + nb]))
+        at += nb
+        if len(out) == 2:
+            break
+    return out
+
+
 @pytest.mark.parametrize("plan_mode,case", MODES, ids=IDS, indirect=["plan_mode"])
-@pytest.mark.parametrize("narrow", ["default", "0"])
+@pytest.mark.parametrize("narrow", ["default", "nowave", "0"])
 def test_scheduled_apply_bitwise(native, oracle, monkeypatch, plan_mode, case, narrow):
-    """Applies through the plans (and, where the planner marks a plan
-    narrow, through the one-launch narrow kernel; MPK_SWEEP_NARROW=0 keeps
-    the block engine everywhere) give the oracle's bits."""
+    """Applies through the plans give the oracle's bits: through the one-wave
+    kernel where the planner marks a plan 2, the staged one-workgroup kernel
+    where it marks it 1 (every plan it can take with MPK_SWEEP_WAVE=0), the
+    block engine elsewhere (everywhere with MPK_SWEEP_NARROW=0)."""
     from mojopetsc4py import PETSc
+    from test_sweep_plan import _narrow_rule
     name, kind, levels, ordering = case
     if narrow == "0":
         monkeypatch.setenv("MPK_SWEEP_NARROW", "0")
+    if narrow == "nowave":
+        monkeypatch.setenv("MPK_SWEEP_WAVE", "0")
     oracle.set_mode("dev")
     try:
         csr, A, pc = _setup(name, kind, levels, ordering)
-        raw = back(pc._dbufs["sched"], np.uint8, 64)
-        marked = int(raw.view(np.int64)[7])
-        if narrow == "0":
-            assert marked == 0
-        elif name in ("lap32", "tri50", "lap100", "diff8") and plan_mode == "auto" and levels == 0:
-            assert marked == 1, "a small or narrow sweep takes the narrow kernel"
         n = A.getSize()[0]
+        marks = _marks(pc, n)
+        assert marks
+        for mark, raw in marks:
+            want = 0 if narrow == "0" else _narrow_rule(raw, n, wave=narrow == "default")
+            assert mark == want
+        if plan_mode == "auto" and (name, kind, levels) in (("lap32", "ilu", 0), ("tri50", "ilu", 0)):
+            assert [m for m, _ in marks] == [{"default": 2, "nowave": 1, "0": 0}[narrow]] * 2
         Ao = oracle.CSR(*csr)
         if kind == "sor":
             po = oracle.PCSetup(Ao, "sor", omega=1.2)

@@ -236,9 +236,9 @@ def _random(n, per_row, seed):
     return W._csr_from_coo(n, np.array(rows), np.array(cols), rng.normal(size=len(rows)))
 
 
-def _lower(ip, ix, va, forward):
-    """The part of each row a sweep reads: strictly earlier columns, plus a
-    stray later column in a few rows to exercise the stale path."""
+def _lower(ip, ix, va, forward, stray=True):
+    """The part of each row a sweep reads: strictly earlier columns, plus
+    (stray) a later column in a few rows to exercise the stale path."""
     n = len(ip) - 1
     rows, cols, vals = [], [], []
     for i in range(n):
@@ -248,7 +248,7 @@ def _lower(ip, ix, va, forward):
                 rows.append(i)
                 cols.append(k)
                 vals.append(va[j])
-        if i % 97 == 5 and 0 < i < n - 1:
+        if stray and i % 97 == 5 and 0 < i < n - 1:
             rows.append(i)
             cols.append(i + 1 if forward else i - 1)
             vals.append(0.5)
@@ -367,31 +367,68 @@ def test_check_catches_a_ring_read_outside_the_window(L, monkeypatch):
     assert not _check(bad, n, True, off, col, coef)
 
 
-def _narrow_rule(raw, n):
-    """The narrow mark restated: one block, 4 or 8 fixed slots, at most 256
-    rows a level and 4096 levels, and no read of a published value."""
-    h = raw[:64].view(np.int64)
+def _narrow_rule(raw, n, wave=True):
+    """The one-workgroup marks restated: one block, 4 or 8 fixed slots, no
+    read of a published value; then 2 (wave) with at most 64 rows a level,
+    12288 levels and no stale read, else 1 (narrow) with at most 256 rows a
+    level and 4095 levels."""
     p = _unpack(raw, n)
-    if p["nblk"] != 1 or p["w"] not in (4, 8) or len(p["lvl"]) > 4096:
+    d = p["edep"]
+    if p["nblk"] != 1 or p["w"] not in (4, 8) or np.any((d < 0) & (d >= -n)):
         return 0
     lv = p["lvl"][p["blk_lev"][0]:p["blk_lev"][1]]
-    if len(lv) > 1 and np.diff(lv).max() > 256:
-        return 0
-    d = p["edep"]
-    return 0 if np.any((d < 0) & (d >= -n)) else 1
+    widest, levels = (np.diff(lv).max() if len(lv) > 1 else 0), len(lv) - 1
+    if wave and widest <= 64 and levels <= 12288 and not np.any(d < -n):
+        return 2
+    return 1 if widest <= 256 and levels < 4096 else 0
 
 
-@pytest.mark.parametrize("name,want", [("lap20", 1), ("diff7", 1), ("cd6", 1), ("lap100", 1), ("diff20", 0),
+def _wave_records(raw, n):
+    """A wave plan's records restated from its entries: per position the w
+    coefficients (-0.0 past the row's count), the w LDS slots (8192 past the
+    count), the row and the count, 16-byte aligned; then each level's longest
+    row count, a byte a level."""
+    p = _unpack(raw, n)
+    h = raw[:64].view(np.int64)
+    nblk, nlvl, w = int(h[0]), int(h[2]), int(h[5])
+    ints = 2 * (nblk + 1) + nlvl + 3 * n + 1
+    at = (64 + (4 * ints + 7) // 8 * 8 + (12 * w * n + 7) // 8 * 8 + 15) // 16 * 16
+    rb = (12 * w + 8 + 15) // 16 * 16
+    pcnt = np.diff(p["pent"])
+    recs = np.zeros((n, rb), np.uint8)
+    for q in range(n):
+        k = int(pcnt[q])
+        c = np.full(w, -0.0)
+        dd = np.full(w + 2, 8192, np.int32)
+        c[:k] = p["ecoef"][p["pent"][q]:p["pent"][q + 1]]
+        dd[:k] = p["edep"][p["pent"][q]:p["pent"][q + 1]]
+        dd[w], dd[w + 1] = p["prow"][q], k
+        recs[q, :12 * w + 8] = np.concatenate([c.view(np.uint8), dd.view(np.uint8)])
+    lv = p["lvl"][p["blk_lev"][0]:p["blk_lev"][1]]
+    lmax = np.array([pcnt[lv[i]:lv[i + 1]].max() for i in range(len(lv) - 1)], np.uint8)
+    tail = np.zeros((len(lmax) + 15) // 16 * 16, np.uint8)
+    tail[:len(lmax)] = lmax
+    return at, np.concatenate([recs.ravel(), tail])
+
+
+@pytest.mark.parametrize("name,want", [("lap20", 2), ("diff7", 2), ("cd6", 2), ("lap100", 1), ("diff20", 0),
                                        ("lap300", 0)])
 @pytest.mark.parametrize("forward", [True, False])
-def test_narrow_mark(L, monkeypatch, name, want, forward):
-    """hdr[7] marks the plans the one-workgroup narrow kernel runs (single
-    block or ring, narrow levels, every read in LDS): small systems and
-    2-D grids up to 256 wide take it; a 20^3 operator in blocks (reads across
-    blocks) and a 300-wide grid (levels of 300 rows) do not."""
+@pytest.mark.parametrize("stray", [False, True])
+def test_narrow_mark(L, monkeypatch, name, want, forward, stray):
+    """hdr[7] marks the plans one workgroup runs (single block or ring,
+    narrow levels, every read in LDS): 2 for one wave (levels of at most 64
+    rows: a 20^2 grid, small 3-D operators) with its records after the plan,
+    1 for the staged kernel (a 100^2 grid), 0 for a 20^3 operator in blocks
+    (reads across blocks) and a 300-wide grid (levels of 300 rows).
+    A stale read (a stray later column) keeps a plan out of the wave form.
+    MPK_SWEEP_WAVE=0 leaves the wave plans narrow, MPK_SWEEP_NARROW=0 every
+    plan unmarked."""
     mats = dict(SMALL, lap100=lambda: W.laplacian_2d(100), diff20=lambda: W.diffusion_3d(20)[0],
                 lap300=lambda: W.laplacian_2d(300))
-    off, col, coef = _lower(*mats[name](), forward)
+    off, col, coef = _lower(*mats[name](), forward, stray)
+    if stray and want == 2:
+        want = 1
     n = len(off) - 1
     m = int(off[n])
     cap = L.mpk_sweep_plan_bound(n, m, 8192)
@@ -400,6 +437,15 @@ def test_narrow_mark(L, monkeypatch, name, want, forward):
                           buf.ctypes.data, cap)
     raw = buf.view(np.uint8)[:nb]
     assert int(raw[:64].view(np.int64)[7]) == _narrow_rule(raw, n) == want
+    if want == 2:
+        at, want_bytes = _wave_records(raw, n)
+        assert nb == at + len(want_bytes) and nb % 16 == 0
+        assert np.array_equal(raw[at:], want_bytes)
+    monkeypatch.setenv("MPK_SWEEP_WAVE", "0")
+    nb = L.mpk_sweep_plan(n, 1 if forward else 0, off.ctypes.data, col.ctypes.data, coef.ctypes.data, 8192,
+                          buf.ctypes.data, cap)
+    raw = buf.view(np.uint8)[:nb]
+    assert int(buf[7]) == _narrow_rule(raw, n, wave=False) == (1 if want else 0)
     monkeypatch.setenv("MPK_SWEEP_NARROW", "0")
     nb = L.mpk_sweep_plan(n, 1 if forward else 0, off.ctypes.data, col.ctypes.data, coef.ctypes.data, 8192,
                           buf.ctypes.data, cap)
