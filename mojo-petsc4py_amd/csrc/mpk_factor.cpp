@@ -118,8 +118,7 @@ static inline void copy_ids(I *dst, const I *src, I count) {
 //      multiple of 16): per position, the w coefficients (-0.0 past the
 //      row's count), the w LDS slots (kWaveZero past the count, a slot that
 //      holds 0.0, so that a padded entry adds fma(-0.0, 0.0, acc) == acc
-//      exactly), the row and the count, in wave_rec_bytes(w) bytes; then a
-//      byte per level, its longest row's count.
+//      exactly), the row and the count, in wave_rec_bytes(w) bytes.
 // MPK_SWEEP_NARROW=0 leaves every plan unmarked; MPK_SWEEP_WAVE=0 marks no
 // plan 2.  Returns the plan's size with the records.
 static constexpr int kWaveZero = 8192;
@@ -150,13 +149,13 @@ static int64_t mark_plan(char *out, int64_t n, int64_t bytes, int64_t cap) {
       stale |= d < -n;
     }
   const bool narrow = widest <= 256 && levels < 4096;  // its kernel holds levels + 1 bounds
-  const int64_t rb = wave_rec_bytes(w), wbytes = bytes + n * rb + ((levels + 15) & ~(int64_t)15);
+  const int64_t rb = wave_rec_bytes(w), wbytes = bytes + n * rb;
   const bool wave = widest <= 64 && levels <= 12288 && !stale && wbytes <= cap && !env_off("MPK_SWEEP_WAVE");
   if (!wave) {
     hdr[7] = narrow ? 1 : 0;
     return bytes;
   }
-  std::memset(out + bytes, 0, (size_t)(wbytes - bytes));  // the padding too
+  std::memset(out + bytes, 0, (size_t)(wbytes - bytes));  // the records' padding too
   for (int64_t p = 0; p < n; p++) {
     char *r = out + bytes + p * rb;
     double *c = (double *)r;
@@ -169,10 +168,6 @@ static int64_t mark_plan(char *out, int64_t n, int64_t bytes, int64_t cap) {
     d[w] = prow[p];
     d[w + 1] = pcnt[p];
   }
-  unsigned char *lmax = (unsigned char *)(out + bytes + n * rb);
-  for (int64_t l = 0; l < levels; l++)
-    for (int p = lvl[blk_lev[0] + l]; p < lvl[blk_lev[0] + l + 1]; p++)
-      lmax[l] = (unsigned char)std::max<int>(lmax[l], pcnt[p]);
   hdr[4] = wbytes;
   hdr[7] = 2;
   return wbytes;
@@ -665,7 +660,7 @@ static int64_t plan_bytes(int64_t n, int64_t nblk, int64_t nlvl, int64_t slots) 
 // records of a wave plan).
 int64_t mpk_sweep_plan_bound(int64_t n, int64_t m, int64_t bmax) {
   const int64_t B = plan_rows(n, bmax), nblk = n > 0 ? (n + B - 1) / B : 0;
-  return plan_bytes(n, nblk, n + nblk, std::max(4 * m, 32 * n)) + n * wave_rec_bytes(8) + ((n + 16) & ~(int64_t)15);
+  return plan_bytes(n, nblk, n + nblk, std::max(4 * m, 32 * n)) + n * wave_rec_bytes(8);
 }
 
 // Ring plans.  A sweep too long for one block whose DAG is deep and narrow

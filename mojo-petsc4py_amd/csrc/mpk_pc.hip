@@ -35,7 +35,8 @@ __global__ void k_sweep_prepare(i64 n, double *p, unsigned *ticket, const int64_
 
 // Which sweeps of a PC's plans run in one workgroup and how (hdr[7] and
 // hdr[5] of each: bits 0-7 the first sweep, 8-15 the second; a sweep's byte
-// is its slot count, plus 16 for a wave plan, or 0), recorded on the host by
+// is its slot count, plus 16 for a wave plan and 32 more when none of its
+// rows has over 2 entries, or 0), recorded on the host by
 // PCView::schedule when it writes them, so an apply picks the kernel without
 // reading the device.  A plan this process did not write counts as neither.
 static std::mutex g_narrow_mu;
@@ -66,10 +67,12 @@ static void run_sweep(const Prog &prog, i64 n, bool forward, double *pub, const 
   const int form = plan ? plan_narrow(plan, second) : 0;
   const int narrow = form & 15;  // the plan's slot count, or 0
   if (form & 16) {
-    if (narrow == 4)
-      MPK_LAUNCH((k_sweep_wave<Prog, 4>), 1, MPK_WAVE_THREADS, prog, n, plan, second, st, pub, stop);
+    if (narrow == 4 && (form & 32))
+      MPK_LAUNCH((k_sweep_wave<Prog, 4, 2>), 1, MPK_WAVE_THREADS, prog, n, plan, second, st, pub, stop);
+    else if (narrow == 4)
+      MPK_LAUNCH((k_sweep_wave<Prog, 4, 4>), 1, MPK_WAVE_THREADS, prog, n, plan, second, st, pub, stop);
     else
-      MPK_LAUNCH((k_sweep_wave<Prog, 8>), 1, MPK_WAVE_THREADS, prog, n, plan, second, st, pub, stop);
+      MPK_LAUNCH((k_sweep_wave<Prog, 8, 8>), 1, MPK_WAVE_THREADS, prog, n, plan, second, st, pub, stop);
     return;
   }
   if (narrow) {
@@ -153,8 +156,14 @@ static i64 plan_sweep(const Prog &prog, i64 n, bool forward, char *dst, i64 cap,
   }
   copy_htod(dst, host.get(), bytes);
   *depth = host[3];
-  // narrow: its slot count; wave: the count plus 16
-  *narrow = host[7] == 1 ? (int)host[5] : host[7] == 2 ? (int)host[5] | 16 : 0;
+  // narrow: its slot count; wave: the count plus 16, plus 32 when no row has
+  // over 2 entries
+  int widest = 0;
+  if (host[7] == 2) {
+    const int *pcnt = reinterpret_cast<const int *>(host.get() + 8) + 2 * (host[0] + 1) + host[2] + n;
+    for (i64 p = 0; p < n; p++) widest = pcnt[p] > widest ? pcnt[p] : widest;
+  }
+  *narrow = host[7] == 1 ? (int)host[5] : host[7] == 2 ? ((int)host[5] | 16 | (widest <= 2 ? 32 : 0)) : 0;
   return failed() ? -1 : bytes;
 }
 

@@ -696,16 +696,22 @@ __global__ void __launch_bounds__(MPK_NARROW_THREADS) k_sweep_narrow(Prog prog, 
 // Wave plans (the planner marks them hdr[7] = 2): one block (a single-block
 // plan or a ring) whose levels hold at most 64 rows, every read in LDS and
 // none stale.  All threads first stage every position (prog.stage into st)
-// and copy the level bounds and each level's longest row count into LDS;
-// then one wave runs every level, position lo + lane of a level on lane
-// `lane`.  A wave's LDS operations take effect in issue order, so a level
-// reads what the level before it wrote with no barrier.  The wave holds the
-// records of the next kWaveAhead levels in registers, each loaded that many
-// levels before its level runs (unrolled, so that no set is copied).  Padded
-// entries (coefficient -0.0, slot MPK_WAVE_ZERO holding 0.0) leave a sum
-// unchanged and entries past the level's longest row are skipped on a
-// uniform branch: each row's sum is the same chain of fused multiply-adds as
-// in the other engines.
+// and copy the level bounds into LDS; then one wave runs every level,
+// position lo + lane of a level on lane `lane`.  A wave's LDS operations
+// take effect in issue order, so a level reads what the level before it
+// wrote with no barrier.  The wave holds the records of the next kWaveAhead
+// levels in registers, each loaded that many levels before its level runs
+// (unrolled, so that no set is copied).  A level's step has no branches:
+//  - every row sums E entries (E = 2 when no row of the sweep has more,
+//    else the slot count); padded entries (coefficient -0.0, slot
+//    MPK_WAVE_ZERO holding 0.0) leave a sum unchanged, so each row's sum is
+//    the same chain of fused multiply-adds as in the other engines;
+//  - a lane past its level's end takes the level's first position, and the
+//    levels past the last repeat the last level: such a lane computes that
+//    row again from the same values and stores the same bits to the same
+//    places.
+// (Loads or stores under a branch cost the compiler's count of the loads in
+// flight its precision, and the prefetch with it.)
 #define MPK_WAVE_LVLS 12288
 #define MPK_WAVE_ZERO MPK_BLK_ROWS
 #define MPK_WAVE_THREADS 512
@@ -715,22 +721,18 @@ struct WaveSet {  // this lane's records of one level
   static constexpr int kBytes = (12 * SL + 8 + 15) & ~15;  // the plan's record
   double2 c[SL / 2];  // coefficients
   int4 d[SL / 4];     // LDS slots
-  int row;
-  double2 s0, s1;     // the Staged record
-  int pos, m, valid;  // position, the level's longest row (uniform), row present
+  int row, pos;
+  double2 s0, s1;  // the Staged record
 };
 
-// Loads level l's records for this lane (levels past the last: the last
-// level's, with no row present).  Only the words used are loaded: a load
-// whose result is partly unused leaves registers the compiler may reuse
-// while the load is still in flight, which costs a wait on every load.
+// Loads the records of the level at lo..hi-1 for this lane.  Only the words
+// used are loaded: a load whose result is partly unused leaves registers the
+// compiler may reuse while the load is in flight, at the cost of a wait.
 template <int SL>
-__device__ __forceinline__ void wave_fetch(WaveSet<SL> &w, const PlanView &P, const Staged *st, int lo, int hi, int m,
-                                           bool real, int lane) {
+__device__ __forceinline__ void wave_fetch(WaveSet<SL> &w, const PlanView &P, const Staged *st, int lo, int hi,
+                                           int lane) {
   const int p = lo + lane < hi ? lo + lane : lo;
   w.pos = p;
-  w.m = __builtin_amdgcn_readfirstlane(m);
-  w.valid = real && lo + lane < hi;
   const char *r = P.wrec + (i64)p * WaveSet<SL>::kBytes;
   const double2 *cq = reinterpret_cast<const double2 *>(r);
   const int4 *dq = reinterpret_cast<const int4 *>(r + 8 * SL);
@@ -753,65 +755,29 @@ __device__ __forceinline__ int wave_slot(const WaveSet<SL> &w, int e) {
   const int4 q = w.d[e >> 2];
   return (e & 3) == 0 ? q.x : (e & 3) == 1 ? q.y : (e & 3) == 2 ? q.z : q.w;
 }
-// The first E entries' sum: the reads in one round, then the chain.
-template <int E, int SL>
-__device__ __forceinline__ double wave_sum(const WaveSet<SL> &w, const double *val) {
-  double v[E];
-#pragma unroll
-  for (int e = 0; e < E; e++) v[e] = val[wave_slot(w, e)];
-  double acc = w.s0.x;  // row_start
-#pragma unroll
-  for (int e = 0; e < E; e++) acc = fma(wave_coef(w, e), v[e], acc);
-  return acc;
-}
 
-template <int SL, class Prog>
-__device__ __forceinline__ void wave_step(const Prog &prog, const WaveSet<SL> &w, double *val, double *pub, int p0,
-                                          int mask) {
-  // one straight-line variant per range of the level's longest row (a
-  // uniform branch; entries past a row's own count are padded)
-  double acc;
-  if (w.m <= 2)
-    acc = wave_sum<2>(w, val);
-  else if (SL == 4 || w.m <= 4)
-    acc = wave_sum<4>(w, val);
-  else
-    acc = wave_sum<SL>(w, val);
-  if (w.valid) {
-    const Staged sr{w.s0.x, w.s0.y, w.s1.x, (i64)__double_as_longlong(w.s1.y)};
-    const double out = prog.fin(w.row, acc, sr);
-    val[(w.pos - p0) & mask] = out;
-    store_pub(pub, w.row, out);
-  }
-}
-
-__device__ __forceinline__ void wave_bounds(const int *lv, const unsigned char *lvm, int nl, int l, int &lo, int &hi,
-                                            int &m) {
+// Level l's bounds (levels past the last: the last level's).
+__device__ __forceinline__ void wave_bounds(const int *lv, int nl, int l, int &lo, int &hi) {
   const int k = l < nl ? l : nl - 1;
   lo = lv[k];
   hi = lv[k + 1];
-  m = lvm[k];
 }
 
-template <class Prog, int SL>
+template <class Prog, int SL, int E>
 __global__ void __launch_bounds__(MPK_WAVE_THREADS) k_sweep_wave(Prog prog, i64 n, const char *plan, int second,
                                                                  Staged *st, double *pub, const int64_t *stop) {
-  constexpr int kWaveAhead = SL == 4 ? 8 : 5;  // at most 63 loads and stores in flight
+  constexpr int kWaveAhead = SL == 4 ? 8 : 5;  // loads in flight within the 63 a wave can count
   constexpr int kStageUnroll = 8;
   __shared__ double val[MPK_BLK_ROWS + 1];
   __shared__ int lv[MPK_WAVE_LVLS + 1];
-  __shared__ unsigned char lvm[MPK_WAVE_LVLS];
   if (stop && *stop) return;
   if (second) plan += reinterpret_cast<const i64 *>(plan)[4];
   const PlanView P = plan_view(plan, n);
   const int l0 = P.blk_lev[0], nl = P.blk_lev[1] - 1 - l0;  // levels l0..; lvl[l0 + nl] ends the block
   const int p0 = P.blk_pos[0], p1 = P.blk_pos[1];
   const int t = (int)threadIdx.x;
-  const unsigned char *lmax = reinterpret_cast<const unsigned char *>(P.wrec + (i64)n * WaveSet<SL>::kBytes);
 #pragma unroll 4
   for (int l = t; l <= nl; l += MPK_WAVE_THREADS) lv[l] = P.lvl[l0 + l];
-#pragma unroll 4
-  for (int l = t; l < nl; l += MPK_WAVE_THREADS) lvm[l] = lmax[l];
   // every position's Staged record, kStageUnroll of them a thread at a time
   // (their loads all issue before the first store)
   for (int b = p0 + t; b < p1; b += kStageUnroll * MPK_WAVE_THREADS) {
@@ -828,28 +794,53 @@ __global__ void __launch_bounds__(MPK_WAVE_THREADS) k_sweep_wave(Prog prog, i64 
     }
   }
   if (t == 0) val[MPK_WAVE_ZERO] = 0.0;
-  __syncthreads();  // st, lv, lvm (workgroup scope: the stores to st are seen)
+  __syncthreads();  // st and lv (workgroup scope: the stores to st are seen)
   if (t >= 64) return;
   // the first sets in level order: the compiler may not reorder their loads
   // (its waits in the loop are placed for loads issued in that order)
   WaveSet<SL> w[kWaveAhead];
 #pragma unroll
   for (int k = 0; k < kWaveAhead; k++) {
-    int lo, hi, m;
-    wave_bounds(lv, lvm, nl, k, lo, hi, m);
-    wave_fetch<SL>(w[k], P, st, lo, hi, m, k < nl, t);
+    int lo, hi;
+    wave_bounds(lv, nl, k, lo, hi);
+    wave_fetch<SL>(w[k], P, st, lo, hi, t);
     __builtin_amdgcn_sched_barrier(0);
   }
+  // A step: the value reads; while they are in flight, the step before's
+  // deferred work (its published value, the refetch of its set); then the
+  // chain, the finish and the LDS write the next step reads.  The first step
+  // refetches the last set's own level (the same values).
+  double pend_out = 0.0;
+  int pend_row = 0;
+  int flo, fhi;
+  wave_bounds(lv, nl, kWaveAhead - 1, flo, fhi);
   for (int l = 0; l < nl; l += kWaveAhead) {
 #pragma unroll
     for (int k = 0; k < kWaveAhead; k++) {
-      // the bounds of the level to fetch are read with this level's values
-      int lo, hi, m;
-      wave_bounds(lv, lvm, nl, l + k + kWaveAhead, lo, hi, m);
-      wave_step<SL>(prog, w[k], val, pub, p0, P.mask);
-      wave_fetch<SL>(w[k], P, st, lo, hi, m, l + k + kWaveAhead < nl, t);
+      const WaveSet<SL> &cur = w[k];
+      WaveSet<SL> &prev = w[(k + kWaveAhead - 1) % kWaveAhead];
+      int lo, hi;  // the level this set is refetched for, read with its values
+      wave_bounds(lv, nl, l + k + kWaveAhead, lo, hi);
+      double v[E];
+#pragma unroll
+      for (int e = 0; e < E; e++) v[e] = val[wave_slot(cur, e)];
+      __builtin_amdgcn_sched_barrier(0);
+      if (l + k > 0) store_pub(pub, pend_row, pend_out);  // uniform
+      wave_fetch<SL>(prev, P, st, flo, fhi, t);
+      __builtin_amdgcn_sched_barrier(0);
+      double acc = cur.s0.x;  // row_start
+#pragma unroll
+      for (int e = 0; e < E; e++) acc = fma(wave_coef(cur, e), v[e], acc);
+      const Staged sr{cur.s0.x, cur.s0.y, cur.s1.x, (i64)__double_as_longlong(cur.s1.y)};
+      const double out = prog.fin(cur.row, acc, sr);
+      val[(cur.pos - p0) & P.mask] = out;
+      pend_out = out;
+      pend_row = cur.row;
+      flo = lo;
+      fhi = hi;
     }
   }
+  store_pub(pub, pend_row, pend_out);
 }
 
 // Entries of a sweep in the rows' own order, for the host-side planner:

@@ -386,8 +386,7 @@ def _narrow_rule(raw, n, wave=True):
 def _wave_records(raw, n):
     """A wave plan's records restated from its entries: per position the w
     coefficients (-0.0 past the row's count), the w LDS slots (8192 past the
-    count), the row and the count, 16-byte aligned; then each level's longest
-    row count, a byte a level."""
+    count), the row and the count, 16-byte aligned."""
     p = _unpack(raw, n)
     h = raw[:64].view(np.int64)
     nblk, nlvl, w = int(h[0]), int(h[2]), int(h[5])
@@ -404,11 +403,7 @@ def _wave_records(raw, n):
         dd[:k] = p["edep"][p["pent"][q]:p["pent"][q + 1]]
         dd[w], dd[w + 1] = p["prow"][q], k
         recs[q, :12 * w + 8] = np.concatenate([c.view(np.uint8), dd.view(np.uint8)])
-    lv = p["lvl"][p["blk_lev"][0]:p["blk_lev"][1]]
-    lmax = np.array([pcnt[lv[i]:lv[i + 1]].max() for i in range(len(lv) - 1)], np.uint8)
-    tail = np.zeros((len(lmax) + 15) // 16 * 16, np.uint8)
-    tail[:len(lmax)] = lmax
-    return at, np.concatenate([recs.ravel(), tail])
+    return at, recs.ravel()
 
 
 @pytest.mark.parametrize("name,want", [("lap20", 2), ("diff7", 2), ("cd6", 2), ("lap100", 1), ("diff20", 0),
