@@ -57,6 +57,13 @@ void mpk_synchronize(void);
  * any other value is an error (status 4) and leaves the order unchanged. */
 int64_t mpk_set_reduction_order(int64_t order);
 int64_t mpk_get_reduction_order(void);
+/* The number of Krylov solves since the library loaded that ran as one
+ * launch of one workgroup (small systems in the tile-tree order: n <= 16384
+ * with wave-plan sweeps, n <= 4096 with a preconditioner without sweeps;
+ * MPK_SMALL_ROWS=k lowers the limit, 0 routes every solve to the general
+ * path).  A diagnostic with no counterpart in the reference: both paths give
+ * the same bits. */
+int64_t mpk_small_solves(void);
 /* average ms of `reps` SpMV launches (over the sliced copy when soff != 0),
  * HIP events on the library stream */
 double mpk_time_spmv(int64_t x, int64_t y, int64_t irow, int64_t icol, int64_t data, int64_t nrow, int64_t soff,

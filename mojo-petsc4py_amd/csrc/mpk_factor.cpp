@@ -112,9 +112,10 @@ static inline void copy_ids(I *dst, const I *src, I count) {
 //   1, narrow (k_sweep_narrow): one block (a single-block plan or a ring), 4
 //      or 8 fixed slots, at most 256 rows a level and 4095 levels, and no
 //      entry reading another block's published value (d in [-n, 0));
-//   2, wave (k_sweep_wave): as narrow, with at most 64 rows a level and
-//      12288 levels and no stale entry (d < -n); one wave runs every level,
-//      a row per lane.  Its records follow the plan (at plan_bytes, a
+//   2, wave (k_sweep_wave): as narrow, with 12288 levels at most and no
+//      stale entry (d < -n); one to four waves run every level, a row per
+//      lane (a wave per 64 rows of the widest level).  Its records follow
+//      the plan (at plan_bytes, a
 //      multiple of 16): per position, the w coefficients (-0.0 past the
 //      row's count), the w LDS slots (kWaveZero past the count, a slot that
 //      holds 0.0, so that a padded entry adds fma(-0.0, 0.0, acc) == acc
@@ -150,7 +151,7 @@ static int64_t mark_plan(char *out, int64_t n, int64_t bytes, int64_t cap) {
     }
   const bool narrow = widest <= 256 && levels < 4096;  // its kernel holds levels + 1 bounds
   const int64_t rb = wave_rec_bytes(w), wbytes = bytes + n * rb;
-  const bool wave = widest <= 64 && levels <= 12288 && !stale && wbytes <= cap && !env_off("MPK_SWEEP_WAVE");
+  const bool wave = widest <= 256 && levels <= 12288 && !stale && wbytes <= cap && !env_off("MPK_SWEEP_WAVE");
   if (!wave) {
     hdr[7] = narrow ? 1 : 0;
     return bytes;

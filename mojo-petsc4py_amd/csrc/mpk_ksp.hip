@@ -12,6 +12,7 @@
 // Scalar arithmetic follows the reference expression by expression, with
 // fma() where it contracts, so given the same reductions the iterates are
 // bitwise those of the CPU path.
+#include <atomic>
 #include <cmath>
 #include <cstdlib>
 #include <cstring>
@@ -21,6 +22,7 @@
 #include "mpk_internal.h"
 #include "mpk_ops.h"
 #include "mpk_pc.h"
+#include "mpk_progs.h"
 #include "mpk_reduce.h"
 #include "../../include/mpk.h"
 
@@ -1122,6 +1124,207 @@ struct Solve {
   }
 };
 
+// ------------------------------------------------------------ small systems
+//
+// A CG solve of a small system (n <= MPK_SMALL_ROWS) in one launch of one
+// workgroup of MPK_WAVE_THREADS threads.  The steps of an iteration run one
+// after another, separated by barriers, with the arithmetic of the general
+// path's launches: the same pair operations (OpCgP, OpCgXR, the Jacobi
+// product), serial row sums, tile-tree reductions (threads 0-255 and 256-511
+// are two 256-thread blocks, each taking tiles as k_partials' blocks do,
+// then threads 0-255 fold the partials as k_finalize does), the same
+// epilogues in thread 0, and the sweeps of wave plans (wave_sweep) -- so its
+// iterates are the general path's bit for bit.  The loop ends when an
+// epilogue sets the stop flag (at the latest at max_it).
+
+#define MPK_SMALL_ROWS 16384
+#define MPK_SMALL_TILES (MPK_SMALL_ROWS / MPK_TILE)
+
+struct SmallLds {
+  double val[MPK_BLK_ROWS + 1];  // the sweeps' values
+  int lv[MPK_WAVE_LVLS + 1];     // the sweeps' level bounds
+  double wsum[3][MPK_SMALL_TILES][4];  // per tile, its four wave sums
+  double fsum[3][4];                   // the fold's wave sums
+  double res[3];
+};
+
+struct SmallCG {
+  i64 n, max_it;
+  const int *rp, *ci;
+  const double *v;
+  double *X, *R, *Z, *P, *W;  // W may be Z
+  KspState *s;
+  const double *jac;  // Jacobi: the inverted diagonal
+  IluFactor ilu;
+  IccFactor icc;
+  int form0, form1;  // the sweeps' wave forms
+  double *S;         // the sweeps' scratch
+  Staged *st;
+};
+
+// K reductions (RedSet) of length n <= MPK_SMALL_ROWS into L.res, in the
+// tile-tree order (mpk_reduce.h).  Every thread calls it.
+template <int K>
+__device__ __forceinline__ void small_reduce(const RedSet<K> &rs, i64 n, SmallLds &L) {
+  const int t = (int)threadIdx.x, tl = t & 255, lane = t & 63, wv = (t >> 6) & 3;
+  const int T = (int)((n + MPK_TILE - 1) / MPK_TILE);
+  bool al = true;
+#pragma unroll
+  for (int k = 0; k < K; k++) al = al && aligned16(rs.s[k].x) && (rs.s[k].kind != RK_DOT || aligned16(rs.s[k].y));
+  for (int j = t >> 8; j < T; j += 2) {  // block t >> 8 takes every other tile
+#pragma unroll
+    for (int k = 0; k < K; k++) {
+      const i64 base = (i64)j * MPK_TILE + 2 * tl;
+      const double p0 = pair_leaf(rs.s[k], base, n, al), p1 = pair_leaf(rs.s[k], base + 512, n, al);
+      const double p2 = pair_leaf(rs.s[k], base + 1024, n, al), p3 = pair_leaf(rs.s[k], base + 1536, n, al);
+      const double u = wave_tree((p0 + p1) + (p2 + p3));
+      if (lane == 0) L.wsum[k][j][wv] = u;
+    }
+  }
+  __syncthreads();
+  if (t < 256) {  // the fold: thread t's partial P_t (T <= 256), then the block tree
+#pragma unroll
+    for (int k = 0; k < K; k++) {
+      const double u = t < T ? (L.wsum[k][t][0] + L.wsum[k][t][2]) + (L.wsum[k][t][1] + L.wsum[k][t][3]) : 0.0;
+      const double r = wave_tree(u);
+      if (lane == 0) L.fsum[k][wv] = r;
+    }
+  }
+  __syncthreads();
+  if (t == 0) {
+#pragma unroll
+    for (int k = 0; k < K; k++) L.res[k] = (L.fsum[k][0] + L.fsum[k][2]) + (L.fsum[k][1] + L.fsum[k][3]);
+  }
+}
+
+// y = A x, each row's serial sum from 0.0 in CSR order (MatMult_SeqAIJ,
+// ported.mojo:380-391).  A thread takes 2 rows at a time and issues the loads
+// of up to 8 entries of each before any product, so that the rows' loads
+// overlap; longer rows finish in a loop.
+__device__ __forceinline__ void small_spmv(const SmallCG &a, const double *x, double *y) {
+  constexpr int R = 2, E = 8;
+  const int t = (int)threadIdx.x;
+  for (i64 i0 = t; i0 < a.n; i0 += R * MPK_WAVE_THREADS) {
+    int j0[R], j1[R];
+#pragma unroll
+    for (int r = 0; r < R; r++) {
+      const i64 i = i0 + (i64)r * MPK_WAVE_THREADS;
+      j0[r] = i < a.n ? a.rp[i] : 0;
+      j1[r] = i < a.n ? a.rp[i + 1] : 0;
+    }
+    int c[R][E];
+    double v[R][E], xv[R][E], acc[R];
+#pragma unroll
+    for (int r = 0; r < R; r++)
+#pragma unroll
+      for (int e = 0; e < E; e++)
+        if (j0[r] + e < j1[r]) {
+          c[r][e] = a.ci[j0[r] + e];
+          v[r][e] = a.v[j0[r] + e];
+        }
+#pragma unroll
+    for (int r = 0; r < R; r++)
+#pragma unroll
+      for (int e = 0; e < E; e++)
+        if (j0[r] + e < j1[r]) xv[r][e] = x[c[r][e]];
+#pragma unroll
+    for (int r = 0; r < R; r++) {
+      acc[r] = 0.0;
+#pragma unroll
+      for (int e = 0; e < E; e++)
+        if (j0[r] + e < j1[r]) acc[r] = fma(v[r][e], xv[r][e], acc[r]);
+      for (int j = j0[r] + E; j < j1[r]; j++) acc[r] = fma(a.v[j], x[a.ci[j]], acc[r]);
+      const i64 i = i0 + (i64)r * MPK_WAVE_THREADS;
+      if (i < a.n) y[i] = acc[r];
+    }
+  }
+}
+
+// z = M^-1 r for the PC type PCT (PCView::apply's arithmetic), ending with a
+// barrier.
+template <int PCT>
+__device__ __forceinline__ void small_pc(const SmallCG &a, const double *r, double *z, SmallLds &L) {
+  const i64 n = a.n;
+  const int t = (int)threadIdx.x;
+  if (PCT == PCT_JACOBI) {
+    for (i64 i = t; i < n; i += MPK_WAVE_THREADS) z[i] = r[i] * a.jac[i];
+  } else if (PCT == PCT_ILU) {
+    double *F = a.S, *B = a.S + n;
+    const IluFactor &f = a.ilu;
+    wave_sweep_form(a.form0, IluFwd{f.bi, f.bj, f.r, f.ba, r}, n, f.plan, 0, a.st, F, L.val, L.lv);
+    __syncthreads();
+    wave_sweep_form(a.form1, IluBwd{f.bj, f.bdiag, f.c, f.ba, F, z}, n, f.plan, 1, a.st, B, L.val, L.lv);
+  } else if (PCT == PCT_ICC) {
+    double *Y = a.S, *F = a.S + n, *B = a.S + 2 * n;
+    const IccFactor &f = a.icc;
+    wave_sweep_form(a.form0, IccFwd{f.trp, f.trow, f.tpos, f.tmid, f.udiag, f.c, f.ua, r, F}, n, f.plan, 0, a.st,
+                    Y, L.val, L.lv);
+    __syncthreads();
+    if (f.nlate) {
+      for (i64 k = t; k < n; k += MPK_WAVE_THREADS) icc_late_row(k, f.trp, f.trow, f.tpos, f.tmid, f.ua, Y, F);
+      __syncthreads();
+    }
+    wave_sweep_form(a.form1, IccBwd{f.ui, f.uj, f.udiag, f.c, f.ua, F, z, n - 1}, n, f.plan, 1, a.st, B, L.val,
+                    L.lv);
+  } else if (r != z) {
+    for (i64 i = t; i < n; i += MPK_WAVE_THREADS) z[i] = r[i];
+  }
+  __syncthreads();
+}
+
+__device__ __forceinline__ bool small_stopped(const KspState *s) {
+  return *reinterpret_cast<const volatile int64_t *>(&s->stop) != 0;
+}
+
+// The iterations of KSPSolve_CG (ksp.mojo:337-389) after the initial
+// residual, its PC apply and EPI_CG_INIT, as the general path queues them.
+template <int PCT>
+__global__ void __launch_bounds__(MPK_WAVE_THREADS) k_cg_small(SmallCG a) {
+  __shared__ SmallLds L;
+  KspState *s = a.s;
+  const i64 n = a.n;
+  const int t = (int)threadIdx.x;
+  const bool al_p = aligned16(a.P) && aligned16(a.Z);
+  const bool al_x = aligned16(a.X) && aligned16(a.R) && aligned16(a.P) && aligned16(a.W);
+  for (i64 it = 0; it <= a.max_it + 1; it++) {
+    if (small_stopped(s)) break;  // the flag is read after a barrier: uniform
+    for (i64 e = 2 * (i64)t; e < n; e += 2 * MPK_WAVE_THREADS) OpCgP{a.P, a.Z, s, al_p}(e, n);
+    __syncthreads();
+    small_spmv(a, a.P, a.W);
+    __syncthreads();
+    small_reduce<1>(RedSet<1>{{{a.P, a.W, RK_DOT}}}, n, L);
+    if (t == 0) ksp_epilogue(s, EPI_CG_DPI, 0, L.res);
+    __syncthreads();
+    if (small_stopped(s)) break;
+    for (i64 e = 2 * (i64)t; e < n; e += 2 * MPK_WAVE_THREADS) OpCgXR{a.X, a.R, a.P, a.W, s, al_x}(e, n);
+    __syncthreads();
+    small_pc<PCT>(a, a.R, a.Z, L);
+    small_reduce<3>(RedSet<3>{{{a.Z, a.Z, RK_SUMSQ}, {a.R, a.R, RK_SUMSQ}, {a.Z, a.R, RK_DOT}}}, n, L);
+    if (t == 0) ksp_epilogue(s, EPI_CG_NORM, 0, L.res);
+    __syncthreads();
+  }
+}
+
+// Whether a solve goes to the one-workgroup drivers: the tile-tree order (not
+// MPK_REDUCTION=ref), n <= MPK_SMALL_ROWS (MPK_SMALL_ROWS=k lowers it; 0
+// switches the drivers off), and a PC with two wave plans, or a PC without
+// sweeps and n <= MPK_SMALL_PLAIN (past it the general path's fused
+// launches, spread over the chip, are faster).
+#define MPK_SMALL_PLAIN 4096
+static std::atomic<int64_t> g_small_solves{0};
+static bool small_ok(const PCView &pc, i64 n, bool ref, int *f0, int *f1) {
+  const char *e = std::getenv("MPK_SMALL_ROWS");
+  const i64 lim = e ? std::strtoll(e, nullptr, 10) : MPK_SMALL_ROWS;
+  if (ref || n <= 0 || n > lim || n > MPK_SMALL_ROWS) return false;
+  *f0 = *f1 = -1;
+  if (pc.type == PCT_NONE || pc.type == PCT_JACOBI) return n <= MPK_SMALL_PLAIN;
+  const char *plan = pc.type == PCT_ILU ? pc.ilu.plan : pc.type == PCT_ICC ? pc.icc.plan : nullptr;
+  if (!plan) return false;
+  *f0 = plan_wave_form(plan, 0);
+  *f1 = plan_wave_form(plan, 1);
+  return *f0 >= 0 && *f1 >= 0;
+}
+
 // Tiles up to which CG + Jacobi folds its reductions' partials in every block
 // of the kernels that use them: 2048 (4,194,304 rows); MPK_CG_FOLD_TILES=k
 // overrides it (read at each solve; tuning and tests).
@@ -1140,6 +1343,8 @@ using namespace mpk;
 #define P64(a) reinterpret_cast<const int64_t *>(a)
 
 extern "C" {
+
+int64_t mpk_small_solves(void) { return g_small_solves.load(); }
 
 // ksp.mojo:271-389 KSPSolve_CG
 void mp_ksp_solve_cg(int64_t n, int64_t irow, int64_t icol, int64_t data, int64_t pci, int64_t pcf,
@@ -1161,6 +1366,33 @@ void mp_ksp_solve_cg(int64_t n, int64_t irow, int64_t icol, int64_t data, int64_
   }
   S.pc.apply(R, Z, nullptr);
   S.reduce<3>(RedSet<3>{{{Z, Z, RK_SUMSQ}, {R, R, RK_SUMSQ}, {Z, R, RK_DOT}}}, EPI_CG_INIT, 0, nullptr);
+  int f0, f1;
+  if (small_ok(S.pc, n, red_ref(), &f0, &f1)) {
+    // the iterations in one launch of one workgroup
+    SmallCG a{n, S.h.max_it, S.A.rp, S.A.ci, S.A.v, X, R, Z, P, W, S.d, S.pc.jac_diag, S.pc.ilu, S.pc.icc, f0, f1,
+              nullptr, nullptr};
+    if (S.pc.type == PCT_ILU || S.pc.type == PCT_ICC) {
+      a.S = pc_sweep_scratch(n);
+      if (!a.S) return;
+      a.st = static_cast<Staged *>(pc_staged_rows(a.S, n));
+    }
+    switch (S.pc.type) {
+      case PCT_JACOBI:
+        MPK_LAUNCH(k_cg_small<PCT_JACOBI>, 1, MPK_WAVE_THREADS, a);
+        break;
+      case PCT_ILU:
+        MPK_LAUNCH(k_cg_small<PCT_ILU>, 1, MPK_WAVE_THREADS, a);
+        break;
+      case PCT_ICC:
+        MPK_LAUNCH(k_cg_small<PCT_ICC>, 1, MPK_WAVE_THREADS, a);
+        break;
+      default:
+        MPK_LAUNCH(k_cg_small<PCT_NONE>, 1, MPK_WAVE_THREADS, a);
+    }
+    g_small_solves++;
+    S.finish_out((double *)cfg, (int64_t *)cfgi);
+    return;
+  }
   const int chunk = chunk_for(n);
   int slot = 0;
   bool pending = false;

@@ -72,6 +72,15 @@ struct PCView {
   i64 schedule(char *plan, i64 cap) const;
 };
 
+// The one-workgroup form of sweep `second` of a PC's plans, as the process
+// recorded it when it wrote them (wave_form in mpk_sweep.h: 0..2 for a wave
+// plan, -1 otherwise or for a plan it did not write).
+int plan_wave_form(const char *plan, int second);
+// The scratch of a PC's planned sweeps (published values, then the Staged
+// records): 7 n + 1 doubles, and where its Staged records start.
+double *pc_sweep_scratch(i64 n);
+void *pc_staged_rows(double *S, i64 n);
+
 // Every entry of sweep `which` (0: forward / first, 1: second) of a PC with
 // sweeps, in each row's own order, on the host: off[n+1], col, coef.
 bool pc_sweep_entries(const PCView &pc, int which, std::vector<i64> &off, std::vector<int> &col,

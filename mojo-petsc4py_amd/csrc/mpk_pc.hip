@@ -15,6 +15,7 @@
 #include "mpk_ops.h"
 #include "mpk_pc.h"
 #include "mpk_sweep.h"
+#include "mpk_progs.h"
 #include "../../include/mpk.h"
 
 namespace mpk {
@@ -35,8 +36,8 @@ __global__ void k_sweep_prepare(i64 n, double *p, unsigned *ticket, const int64_
 
 // Which sweeps of a PC's plans run in one workgroup and how (hdr[7] and
 // hdr[5] of each: bits 0-7 the first sweep, 8-15 the second; a sweep's byte
-// is its slot count, plus 16 for a wave plan and 32 more when none of its
-// rows has over 2 entries, or 0), recorded on the host by
+// is its slot count, plus for a wave plan 16, 32 when none of its rows has
+// over 2 entries and 64 times log2 of its waves, or 0), recorded on the host by
 // PCView::schedule when it writes them, so an apply picks the kernel without
 // reading the device.  A plan this process did not write counts as neither.
 static std::mutex g_narrow_mu;
@@ -67,12 +68,13 @@ static void run_sweep(const Prog &prog, i64 n, bool forward, double *pub, const 
   const int form = plan ? plan_narrow(plan, second) : 0;
   const int narrow = form & 15;  // the plan's slot count, or 0
   if (form & 16) {
+    const int nw = 1 << ((form >> 6) & 3);
     if (narrow == 4 && (form & 32))
-      MPK_LAUNCH((k_sweep_wave<Prog, 4, 2>), 1, MPK_WAVE_THREADS, prog, n, plan, second, st, pub, stop);
+      MPK_LAUNCH((k_sweep_wave<Prog, 4, 2>), 1, MPK_WAVE_THREADS, prog, n, plan, second, st, pub, nw, stop);
     else if (narrow == 4)
-      MPK_LAUNCH((k_sweep_wave<Prog, 4, 4>), 1, MPK_WAVE_THREADS, prog, n, plan, second, st, pub, stop);
+      MPK_LAUNCH((k_sweep_wave<Prog, 4, 4>), 1, MPK_WAVE_THREADS, prog, n, plan, second, st, pub, nw, stop);
     else
-      MPK_LAUNCH((k_sweep_wave<Prog, 8, 8>), 1, MPK_WAVE_THREADS, prog, n, plan, second, st, pub, stop);
+      MPK_LAUNCH((k_sweep_wave<Prog, 8, 8>), 1, MPK_WAVE_THREADS, prog, n, plan, second, st, pub, nw, stop);
     return;
   }
   if (narrow) {
@@ -156,14 +158,19 @@ static i64 plan_sweep(const Prog &prog, i64 n, bool forward, char *dst, i64 cap,
   }
   copy_htod(dst, host.get(), bytes);
   *depth = host[3];
-  // narrow: its slot count; wave: the count plus 16, plus 32 when no row has
-  // over 2 entries
-  int widest = 0;
+  // narrow: its slot count; wave: the count plus 16, 32 when no row has over
+  // 2 entries, and 64 log2 of its waves (one per 64 rows of its widest level)
+  int longest = 0, widest = 0;
   if (host[7] == 2) {
-    const int *pcnt = reinterpret_cast<const int *>(host.get() + 8) + 2 * (host[0] + 1) + host[2] + n;
-    for (i64 p = 0; p < n; p++) widest = pcnt[p] > widest ? pcnt[p] : widest;
+    const int *ip = reinterpret_cast<const int *>(host.get() + 8);
+    const int *lvl = ip + 2 * (host[0] + 1), *pcnt = lvl + host[2] + n;
+    for (i64 p = 0; p < n; p++) longest = pcnt[p] > longest ? pcnt[p] : longest;
+    for (i64 l = 0; l + 1 < host[2]; l++) widest = lvl[l + 1] - lvl[l] > widest ? lvl[l + 1] - lvl[l] : widest;
   }
-  *narrow = host[7] == 1 ? (int)host[5] : host[7] == 2 ? ((int)host[5] | 16 | (widest <= 2 ? 32 : 0)) : 0;
+  const int lw = widest <= 64 ? 0 : widest <= 128 ? 1 : 2;
+  *narrow = host[7] == 1   ? (int)host[5]
+            : host[7] == 2 ? ((int)host[5] | 16 | (longest <= 2 ? 32 : 0) | (lw << 6))
+                           : 0;
   return failed() ? -1 : bytes;
 }
 
@@ -186,128 +193,16 @@ static i64 sweep_entries(const Prog &prog, i64 n) {
   return m;
 }
 
-// ------------------------------------------------------------ row programs
+// ------------------------------------------------------------ row programs (mpk_progs.h)
 
-struct IluFwd {  // tmp_i = b[r_i] - sum_{L row i} ba_j tmp[bj_j]
-  const int *bi, *bj, *r;
-  const double *ba, *b;
-  __device__ Staged stage(i64 i) const { return Staged{b[r[i]], 0.0, 0.0, 0}; }
-  __device__ int begin(i64 i) const { return bi[i]; }
-  __device__ int end(i64 i) const { return bi[i + 1]; }
-  __device__ i64 col(int j) const { return bj[j]; }
-  __device__ double coef(int j) const { return -ba[j]; }
-  __device__ double fin(i64, double acc, const Staged &) const { return acc; }
-  __device__ double stale(i64) const { return 0.0; }
-};
-struct IluBwd {  // sum = tmp_i - sum_{U row i} ba tmp; x[c_i] = sum * ba[bdiag_i]
-  const int *bj, *bdiag, *c;
-  const double *ba, *F;
-  double *x;
-  __device__ Staged stage(i64 i) const { return Staged{F[i], ba[bdiag[i]], 0.0, c[i]}; }
-  __device__ int begin(i64 i) const { return bdiag[i + 1] + 1; }
-  __device__ int end(i64 i) const { return bdiag[i]; }
-  __device__ i64 col(int j) const { return bj[j]; }
-  __device__ double coef(int j) const { return -ba[j]; }
-  __device__ double fin(i64, double acc, const Staged &st) const {
-    double r = acc * st.s1;
-    x[st.o] = r;
-    return r;
-  }
-  __device__ double stale(i64) const { return 0.0; }
-};
-// IC forward: the reference scatters s_k += ua_ik y_i for i ascending; here
-// row k gathers over the transpose in the same ascending i, up to the first
-// row after k (the rest arrive after row k is final: k_icc_late).
-struct IccFwd {
-  const int *trp, *trow, *tpos, *tmid, *udiag, *c;
-  const double *ua, *b;
-  double *F;  // scaled values for the backward sweep
-  __device__ Staged stage(i64 k) const { return Staged{b[c[k]], ua[udiag[k]], 0.0, 0}; }
-  __device__ int begin(i64 k) const { return trp[k]; }
-  __device__ int end(i64 k) const { return tmid[k]; }
-  __device__ i64 col(int j) const { return trow[j]; }
-  __device__ double coef(int j) const { return ua[tpos[j]]; }
-  __device__ double fin(i64 k, double acc, const Staged &st) const {
-    F[k] = acc * st.s1;
-    return acc;
-  }
-  __device__ double stale(i64) const { return 0.0; }
-};
-// IC backward (ported.mojo:902-952): rows n-2 down to 0 add their strict
-// part; the last row is copied as it is, so a strict part it holds (late
-// entries, columns below the diagonal) is not applied.
-struct IccBwd {
-  const int *ui, *uj, *udiag, *c;
-  const double *ua, *F;
-  double *x;
-  i64 last;  // n - 1
-  __device__ Staged stage(i64 i) const { return Staged{F[i], 0.0, 0.0, c[i]}; }
-  __device__ int begin(i64 i) const { return ui[i]; }
-  __device__ int end(i64 i) const { return i == last ? ui[i] : udiag[i]; }
-  __device__ i64 col(int j) const { return uj[j]; }
-  __device__ double coef(int j) const { return ua[j]; }
-  __device__ double fin(i64, double acc, const Staged &st) const {
-    x[st.o] = acc;
-    return acc;
-  }
-  // a column below the diagonal: the backward loop has not reached it yet,
-  // so the reference reads its forward value
-  __device__ double stale(i64 k) const { return F[k]; }
-};
-
-// The late forward contributions: F[k] += ua_ik y_i for the rows i > k whose
-// strict part holds k, in ascending i, after row k was scaled.  A row whose
-// strict part holds its own column (trow == k, which the reference's staging
-// can produce) adds to itself before the scaling overwrites it: no effect.
+// k_icc_late: icc_late_row for every row.
 __global__ void k_icc_late(i64 n, const int *trp, const int *trow, const int *tpos, const int *tmid,
                            const double *ua, const double *Y, double *F, const int64_t *stop) {
   if (stop && *stop) return;
   i64 k = (i64)blockIdx.x * 256 + threadIdx.x;
   if (k >= n) return;
-  int j0 = tmid[k], j1 = trp[k + 1];
-  if (j0 == j1) return;
-  double f = F[k];
-  for (int j = j0; j < j1; j++)
-    if (trow[j] != k) f = fma(ua[tpos[j]], Y[trow[j]], f);
-  F[k] = f;
+  icc_late_row(k, trp, trow, tpos, tmid, ua, Y, F);
 }
-// SOR, zero initial guess (ported.mojo:655-683)
-struct SorFwd {
-  const int *rp, *ci, *dmark;
-  const double *a, *x, *idiag;
-  double *t, *yout;  // yout: final y when no backward sweep follows (else null)
-  const double *yin;
-  __device__ Staged stage(i64 i) const { return Staged{x[i], idiag[i], 0.0, 0}; }
-  __device__ int begin(i64 i) const { return rp[i]; }
-  __device__ int end(i64 i) const { return dmark[i]; }
-  __device__ i64 col(int j) const { return ci[j]; }
-  __device__ double coef(int j) const { return -a[j]; }
-  __device__ double fin(i64 i, double acc, const Staged &st) const {
-    t[i] = acc;
-    double r = acc * st.s1;
-    if (yout) yout[i] = r;
-    return r;
-  }
-  __device__ double stale(i64 k) const { return yin[k]; }
-};
-struct SorBwd {
-  const int *rp, *ci, *dmark;
-  const double *a, *src, *idiag, *yf;  // src: t after a forward sweep, else x
-  double *y;
-  double omega;
-  int use_t;
-  __device__ Staged stage(i64 i) const { return Staged{src[i], idiag[i], use_t ? yf[i] : 0.0, 0}; }
-  __device__ int begin(i64 i) const { return dmark[i] + 1; }
-  __device__ int end(i64 i) const { return rp[i + 1]; }
-  __device__ i64 col(int j) const { return ci[j]; }
-  __device__ double coef(int j) const { return -a[j]; }
-  __device__ double fin(i64 i, double acc, const Staged &st) const {
-    double r = use_t ? fma(1.0 - omega, st.s2, acc * st.s1) : acc * st.s1;
-    y[i] = r;
-    return r;
-  }
-  __device__ double stale(i64 k) const { return y[k]; }
-};
 
 // MatInvertDiagonalForSOR_SeqAIJ (ported.mojo:610-630)
 __global__ void k_sor_idiag(i64 n, const double *a, const int *dmark, double *idiag, double *mdiag,
@@ -328,6 +223,9 @@ static double *sweep_scratch(i64 n, bool planned) {
   return scratch_vector((size_t)(planned ? 7 : 3) * (size_t)n + 1);
 }
 static Staged *staged_rows(double *S, i64 n) { return reinterpret_cast<Staged *>(S + 3 * n + (n & 1)); }
+double *pc_sweep_scratch(i64 n) { return sweep_scratch(n, true); }
+void *pc_staged_rows(double *S, i64 n) { return staged_rows(S, n); }
+int plan_wave_form(const char *plan, int second) { return plan ? wave_form(plan_narrow(plan, second)) : -1; }
 
 void ilu_apply(const IluFactor &f, const double *b, double *x, const int64_t *stop) {
   i64 n = f.n;

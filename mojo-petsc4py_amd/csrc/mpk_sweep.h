@@ -694,22 +694,24 @@ __global__ void __launch_bounds__(MPK_NARROW_THREADS) k_sweep_narrow(Prog prog, 
 }
 
 // Wave plans (the planner marks them hdr[7] = 2): one block (a single-block
-// plan or a ring) whose levels hold at most 64 rows, every read in LDS and
+// plan or a ring) whose levels hold at most 256 rows, every read in LDS and
 // none stale.  All threads first stage every position (prog.stage into st)
-// and copy the level bounds into LDS; then one wave runs every level,
-// position lo + lane of a level on lane `lane`.  A wave's LDS operations
-// take effect in issue order, so a level reads what the level before it
-// wrote with no barrier.  The wave holds the records of the next kWaveAhead
-// levels in registers, each loaded that many levels before its level runs
-// (unrolled, so that no set is copied).  A level's step has no branches:
+// and copy the level bounds into LDS; then nw = 1, 2 or 4 waves (one per 64
+// rows of the widest level) run every level, position lo + lane of a level
+// on thread `lane`.  A wave's LDS operations take effect in issue order, so
+// with one wave a level reads what the level before it wrote with no
+// barrier; more waves meet at an LDS-only barrier after each level.  Each
+// wave holds the records of its rows of the next kWaveAhead levels in
+// registers, each loaded that many levels before its level runs (unrolled,
+// so that no set is copied).  A level's step has no branches:
 //  - every row sums E entries (E = 2 when no row of the sweep has more,
 //    else the slot count); padded entries (coefficient -0.0, slot
 //    MPK_WAVE_ZERO holding 0.0) leave a sum unchanged, so each row's sum is
 //    the same chain of fused multiply-adds as in the other engines;
-//  - a lane past its level's end takes the level's first position, and the
-//    levels past the last repeat the last level: such a lane computes that
-//    row again from the same values and stores the same bits to the same
-//    places.
+//  - a thread past its level's end takes the level's first position, and
+//    the levels past the last repeat the last level: such a thread computes
+//    that row again from the same values and stores the same bits to the
+//    same places.
 // (Loads or stores under a branch cost the compiler's count of the loads in
 // flight its precision, and the prefetch with it.)
 #define MPK_WAVE_LVLS 12288
@@ -763,14 +765,26 @@ __device__ __forceinline__ void wave_bounds(const int *lv, int nl, int l, int &l
   hi = lv[k + 1];
 }
 
+// A barrier that orders LDS only: the wait for this wave's LDS writes and
+// the barrier, no wait for its loads in flight (a full __syncthreads may
+// also wait for its global stores, and with them for every load).
+__device__ __forceinline__ void lds_barrier() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+  __builtin_amdgcn_s_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+}
+
+// One wave plan's sweep by a whole workgroup of MPK_WAVE_THREADS threads:
+// staging by every thread, the levels by waves 0..nw-1.  val (MPK_BLK_ROWS +
+// 1 doubles) and lv (MPK_WAVE_LVLS + 1 ints) are the caller's LDS.  Every
+// thread must call it; the caller's barrier after it orders the results
+// (st, pub and the finish's outputs, all in global memory) before any
+// thread reads them.
 template <class Prog, int SL, int E>
-__global__ void __launch_bounds__(MPK_WAVE_THREADS) k_sweep_wave(Prog prog, i64 n, const char *plan, int second,
-                                                                 Staged *st, double *pub, const int64_t *stop) {
+__device__ __forceinline__ void wave_sweep(const Prog &prog, i64 n, const char *plan, int second, Staged *st,
+                                           double *pub, double *val, int *lv, int nw) {
   constexpr int kWaveAhead = SL == 4 ? 8 : 5;  // loads in flight within the 63 a wave can count
   constexpr int kStageUnroll = 8;
-  __shared__ double val[MPK_BLK_ROWS + 1];
-  __shared__ int lv[MPK_WAVE_LVLS + 1];
-  if (stop && *stop) return;
   if (second) plan += reinterpret_cast<const i64 *>(plan)[4];
   const PlanView P = plan_view(plan, n);
   const int l0 = P.blk_lev[0], nl = P.blk_lev[1] - 1 - l0;  // levels l0..; lvl[l0 + nl] ends the block
@@ -795,7 +809,15 @@ __global__ void __launch_bounds__(MPK_WAVE_THREADS) k_sweep_wave(Prog prog, i64 
   }
   if (t == 0) val[MPK_WAVE_ZERO] = 0.0;
   __syncthreads();  // st and lv (workgroup scope: the stores to st are seen)
-  if (t >= 64) return;
+  if (t >= 64 * nw) {
+    // the other waves meet the level barriers too: a barrier counts every
+    // wave of the workgroup still running, and the caller's barrier after
+    // the sweep must not pair with them
+    if (nw > 1)
+      for (int l = 0; l < nl; l += kWaveAhead)
+        for (int k = 0; k < kWaveAhead; k++) lds_barrier();
+    return;
+  }
   // the first sets in level order: the compiler may not reorder their loads
   // (its waits in the loop are placed for loads issued in that order)
   WaveSet<SL> w[kWaveAhead];
@@ -838,9 +860,42 @@ __global__ void __launch_bounds__(MPK_WAVE_THREADS) k_sweep_wave(Prog prog, i64 
       pend_row = cur.row;
       flo = lo;
       fhi = hi;
+      if (nw > 1) lds_barrier();  // uniform
     }
   }
   store_pub(pub, pend_row, pend_out);
+}
+
+// A wave plan's sweep as its own launch (one workgroup).
+template <class Prog, int SL, int E>
+__global__ void __launch_bounds__(MPK_WAVE_THREADS) k_sweep_wave(Prog prog, i64 n, const char *plan, int second,
+                                                                 Staged *st, double *pub, int nw,
+                                                                 const int64_t *stop) {
+  __shared__ double val[MPK_BLK_ROWS + 1];
+  __shared__ int lv[MPK_WAVE_LVLS + 1];
+  if (stop && *stop) return;
+  wave_sweep<Prog, SL, E>(prog, n, plan, second, st, pub, val, lv, nw);
+}
+
+// A wave plan's kernel form from the registry's byte (slot count in bits
+// 0-3, 16: a wave plan, 32: no row over 2 entries, bits 6-7: log2 of the
+// waves): 0 (4 slots, 2 entries), 1 (4 slots), 2 (8 slots) plus 4 times
+// log2 of the waves; -1 for another plan.
+__host__ __device__ inline int wave_form(int byte) {
+  if (!(byte & 16)) return -1;
+  return ((byte & 15) == 4 ? ((byte & 32) ? 0 : 1) : 2) + 4 * ((byte >> 6) & 3);
+}
+// The sweep in its form, by a whole workgroup (wave_sweep).
+template <class Prog>
+__device__ __forceinline__ void wave_sweep_form(int form, const Prog &prog, i64 n, const char *plan, int second,
+                                                Staged *st, double *pub, double *val, int *lv) {
+  const int nw = 1 << (form >> 2);
+  if ((form & 3) == 0)
+    wave_sweep<Prog, 4, 2>(prog, n, plan, second, st, pub, val, lv, nw);
+  else if ((form & 3) == 1)
+    wave_sweep<Prog, 4, 4>(prog, n, plan, second, st, pub, val, lv, nw);
+  else
+    wave_sweep<Prog, 8, 8>(prog, n, plan, second, st, pub, val, lv, nw);
 }
 
 // Entries of a sweep in the rows' own order, for the host-side planner:

@@ -369,16 +369,16 @@ def test_check_catches_a_ring_read_outside_the_window(L, monkeypatch):
 
 def _narrow_rule(raw, n, wave=True):
     """The one-workgroup marks restated: one block, 4 or 8 fixed slots, no
-    read of a published value; then 2 (wave) with at most 64 rows a level,
-    12288 levels and no stale read, else 1 (narrow) with at most 256 rows a
-    level and 4095 levels."""
+    read of a published value, at most 256 rows a level; then 2 (wave) with
+    at most 12288 levels and no stale read, else 1 (narrow) with at most 4095
+    levels."""
     p = _unpack(raw, n)
     d = p["edep"]
     if p["nblk"] != 1 or p["w"] not in (4, 8) or np.any((d < 0) & (d >= -n)):
         return 0
     lv = p["lvl"][p["blk_lev"][0]:p["blk_lev"][1]]
     widest, levels = (np.diff(lv).max() if len(lv) > 1 else 0), len(lv) - 1
-    if wave and widest <= 64 and levels <= 12288 and not np.any(d < -n):
+    if wave and widest <= 256 and levels <= 12288 and not np.any(d < -n):
         return 2
     return 1 if widest <= 256 and levels < 4096 else 0
 
@@ -406,16 +406,16 @@ def _wave_records(raw, n):
     return at, recs.ravel()
 
 
-@pytest.mark.parametrize("name,want", [("lap20", 2), ("diff7", 2), ("cd6", 2), ("lap100", 1), ("diff20", 0),
+@pytest.mark.parametrize("name,want", [("lap20", 2), ("diff7", 2), ("cd6", 2), ("lap100", 2), ("diff20", 0),
                                        ("lap300", 0)])
 @pytest.mark.parametrize("forward", [True, False])
 @pytest.mark.parametrize("stray", [False, True])
 def test_narrow_mark(L, monkeypatch, name, want, forward, stray):
     """hdr[7] marks the plans one workgroup runs (single block or ring,
-    narrow levels, every read in LDS): 2 for one wave (levels of at most 64
-    rows: a 20^2 grid, small 3-D operators) with its records after the plan,
-    1 for the staged kernel (a 100^2 grid), 0 for a 20^3 operator in blocks
-    (reads across blocks) and a 300-wide grid (levels of 300 rows).
+    levels of at most 256 rows, every read in LDS): 2 for the wave kernel (a
+    20^2 or 100^2 grid, small 3-D operators) with its records after the plan,
+    0 for a 20^3 operator in blocks (reads across blocks) and a 300-wide grid
+    (levels of 300 rows).
     A stale read (a stray later column) keeps a plan out of the wave form.
     MPK_SWEEP_WAVE=0 leaves the wave plans narrow, MPK_SWEEP_NARROW=0 every
     plan unmarked."""
