@@ -323,12 +323,12 @@ def bench_batched(lib, sample: int = 16, cpu: bool = False):
     bs.destroy()
     cpu_line = None
     if cpu:
-        # the oracle's FP64 solves of 8 of the sampled systems, one core
+        # the oracle's FP64 solves of the same sampled systems, one core
         sys.path.insert(0, os.path.join(ROOT, "oracle"))
         import oracle as O
         O.set_mode("ref")
         tc = 0.0
-        for s in picks[:8]:
+        for s in picks:
             ip, ix, va, b, kind = systems[s]
             A = O.CSR(ip, ix, va)
             t0 = time.perf_counter()
@@ -337,10 +337,11 @@ def bench_batched(lib, sample: int = 16, cpu: bool = False):
             else:
                 O.ksp_solve(A, b, "gmres", O.PCSetup(A, "ilu", 0), rtol=1e-9, max_it=20000, restart=20)
             tc += time.perf_counter() - t0
-        pc_ = tc / 8
+        pc_ = tc / len(picks)
         cpu_line = {"value": 1.0 / pc_, "unit": "solves/s", "cores": 1, "kind": "port",
-                    "sample": f"oracle FP64 solves (with set-up) of 8 of the systems on one core, "
-                              f"{pc_ * 1e3:.1f} ms each"}
+                    "sample": f"oracle FP64 solves (with set-up) of the {len(picks)} systems the one-at-a-time"
+                              f" line solves, on one core, {pc_ * 1e3:.1f} ms each",
+                    "vs_one_at_a_time": pc_ / per}
     return {"workload": "4096 x 2-D 5-point diffusion, n log-uniform [200, 20000], cg+jacobi / gmres(20)+ilu(0),"
                         " FP32 inner + FP64 refinement to 1e-8",
             "sum_n": int(sum(len(s_[0]) - 1 for s_ in systems)), "seconds": t, "solves_per_s": len(systems) / t,
@@ -348,6 +349,93 @@ def bench_batched(lib, sample: int = 16, cpu: bool = False):
             "autotuned": {"workgroup": tune["workgroup"], "grid": tune["grid"]},
             "one_at_a_time_fp64": {"sample": sample, "ms_per_solve": per * 1e3, "solves_per_s": 1.0 / per},
             "speedup": per * len(systems) / t, "cpu_baseline": cpu_line}
+
+
+# The reference's published small-system solve times (mojo-petsc4py column,
+# README.md:302-322; b = normal(seed 5), x0 = 0, rtol 1e-10, atol 1e-50,
+# divtol 1e30, max_it 2000, bench/bench.py:210-228), in ms, with its
+# iteration counts.
+PUBLISHED_SMALL = {
+    ("cg", "none", 32): (0.895, 113), ("cg", "none", 64): (7.597, 225), ("cg", "none", 128): (65.590, 434),
+    ("gmres", "none", 32): (3.580, 232), ("gmres", "none", 64): (57.955, 626),
+    ("gmres", "none", 128): (720.996, 2000),
+    ("fgmres", "none", 32): (3.233, 232), ("fgmres", "none", 64): (47.752, 626),
+    ("fgmres", "none", 128): (693.121, 2000),
+    ("bcgs", "none", 32): (1.249, 85), ("bcgs", "none", 64): (10.906, 162), ("bcgs", "none", 128): (91.673, 316),
+    ("cg", "ilu", 32): (0.784, 40), ("cg", "ilu", 64): (6.067, 77), ("cg", "ilu", 128): (48.259, 146),
+    ("cg", "icc", 32): (0.828, 40), ("cg", "icc", 64): (6.008, 77), ("cg", "icc", 128): (48.678, 146),
+    ("gmres", "jacobi", 32): (4.485, 232), ("gmres", "jacobi", 64): (51.261, 626),
+}
+
+
+def bench_small(lib, cpu: bool = False, reps: int = 3):
+    """Small systems: config 1 (64^2 5-point Laplacian, CG + ILU(0) natural,
+    rtol 1e-10, max_it 1000) with b = ones and b = normal(seed 5) -- GPU time
+    (the mean of `reps` solves after a first one, which is timed with the
+    set-up apart), its, reason, the iterate against the oracle in both
+    reduction orders -- then the reference's published 32^2 / 64^2 / 128^2
+    solves timed the same way."""
+    from mojopetsc4py import PETSc
+    from mpk import workloads as W
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as O
+
+    def solve(k, ksp_t, pc_t, b, tol):
+        csr = W.laplacian_2d(k)
+        A = _mat(csr)
+        pc = PETSc.PC().create()
+        pc.setType(pc_t)
+        pc.setOperators(A)
+        ksp = PETSc.KSP().create()
+        ksp.setPC(pc)
+        ksp.setType(ksp_t)
+        ksp.setOperators(A)
+        ksp.setTolerances(**tol)
+        t0 = time.perf_counter()
+        ksp.setUp()
+        bv = PETSc.Vec().createWithArray(b)
+        x = bv.duplicate()
+        ksp.solve(bv, x)
+        lib.mpk_synchronize()
+        first = time.perf_counter() - t0
+        best = timed_solves(ksp, bv, x, reps, 0)
+        return csr, x.getArray().copy(), ksp.getIterationNumber(), ksp.getConvergedReason(), best, first
+
+    small0 = lib.mpk_small_solves()
+    out = {"config1": {}, "published": []}
+    for rhs in ("ones", "normal5"):
+        n = 64 * 64
+        b = np.ones(n) if rhs == "ones" else np.random.default_rng(5).normal(size=n)
+        csr, x, its, reason, best, first = solve(64, "cg", "ilu", b, dict(rtol=1e-10, max_it=1000))
+        line = {"rhs": rhs, "gpu_ms": best * 1e3, "first_solve_with_setup_ms": first * 1e3, "its": its,
+                "reason": reason}
+        if rhs == "normal5":
+            line["published_ms"] = PUBLISHED_SMALL[("cg", "ilu", 64)][0]
+            line["vs_published"] = line["published_ms"] / line["gpu_ms"]
+        A_o = O.CSR(*csr)
+        O.set_mode("dev")
+        r = O.ksp_solve(A_o, b, "cg", O.PCSetup(A_o, "ilu", 0), rtol=1e-10, max_it=1000)
+        line["bitwise_vs_oracle_dev_order"] = bool(np.array_equal(r.x.view(np.int64), x.view(np.int64))
+                                                   and r.its == its and r.reason == reason)
+        O.set_mode("ref")
+        t0 = time.perf_counter()
+        r = O.ksp_solve(A_o, b, "cg", O.PCSetup(A_o, "ilu", 0), rtol=1e-10, max_it=1000)
+        t_ref = time.perf_counter() - t0
+        line["oracle_ref_order"] = {"its": r.its, "reason": r.reason,
+                                    "iterate_gap": float(np.max(np.abs(r.x - x)) / np.max(np.abs(r.x)))}
+        if cpu:
+            line["cpu_baseline"] = {"value": t_ref * 1e3, "unit": "ms", "cores": 1, "kind": "port",
+                                    "sample": "the oracle's full CG + ILU(0) solve (reference order, set-up"
+                                              " included) on one core"}
+        out["config1"][rhs] = line
+    tol = dict(rtol=1e-10, atol=1e-50, divtol=1e30, max_it=2000)
+    for (ksp_t, pc_t, k), (pub_ms, pub_its) in PUBLISHED_SMALL.items():
+        b = np.random.default_rng(5).normal(size=k * k)
+        _, _, its, reason, best, _ = solve(k, ksp_t, pc_t, b, tol)
+        out["published"].append({"case": f"{ksp_t}+{pc_t} {k}x{k}", "gpu_ms": best * 1e3, "its": its,
+                                 "published_ms": pub_ms, "published_its": pub_its, "speedup": pub_ms / (best * 1e3)})
+    out["one_launch_solves"] = lib.mpk_small_solves() - small0
+    return out
 
 
 def cpu_model() -> str:
@@ -416,6 +504,8 @@ def main() -> int:
     ap.add_argument("--no-icc", action="store_true")
     ap.add_argument("--no-mixed", action="store_true", help="skip the mixed-precision comparisons")
     ap.add_argument("--no-batched", action="store_true", help="skip the batched solve")
+    ap.add_argument("--no-small", action="store_true", help="skip the small-system solves (config 1 and the"
+                    " reference's published 32^2-128^2 table)")
     ap.add_argument("--no-large", action="store_true", help="skip the SpMV timing at 256^3 (past the cache)")
     ap.add_argument("--spmv-only", action="store_true",
                     help="only the SpMV timing loop and a copy of known size (for hardware counter runs)")
@@ -475,6 +565,7 @@ def main() -> int:
         mixed["convection_diffusion_128"] = bench_mixed(lib, "c3", cpu=not args.no_cpu)
         mixed["random_spd_1e6"] = bench_mixed(lib, "c4", cpu=not args.no_cpu)
     batched = None if args.no_batched else bench_batched(lib, cpu=not args.no_cpu)
+    small = None if args.no_small else bench_small(lib, cpu=not args.no_cpu)
 
     past = None
     if not args.no_large and N == 128:
@@ -519,6 +610,7 @@ def main() -> int:
         "solves": extra,
         "mixed_precision": mixed,
         "batched": batched,
+        "small_systems": small,
         "setup_s": time.perf_counter() - t_build,
     }
     print(json.dumps(out))
