@@ -107,21 +107,25 @@ static inline void copy_ids(I *dst, const I *src, I count) {
   if (count > 0) memcpy(dst, src, sizeof(I) * (size_t)count);
 }
 
-// hdr[7] marks the plans the device runs in one workgroup, every level in
-// turn, without waits on other workgroups:
+// hdr[7] marks the plans the device runs with one workgroup per block,
+// every level of a block in turn:
 //   1, narrow (k_sweep_narrow): one block (a single-block plan or a ring), 4
 //      or 8 fixed slots, at most 256 rows a level and 4095 levels, and no
 //      entry reading another block's published value (d in [-n, 0));
 //   2, wave (k_sweep_wave): as narrow, with 12288 levels at most and no
 //      stale entry (d < -n); one to four waves run every level, a row per
-//      lane (a wave per 64 rows of the widest level).  Its records follow
-//      the plan (at plan_bytes, a
-//      multiple of 16): per position, the w coefficients (-0.0 past the
-//      row's count), the w LDS slots (kWaveZero past the count, a slot that
-//      holds 0.0, so that a padded entry adds fma(-0.0, 0.0, acc) == acc
-//      exactly), the row and the count, in wave_rec_bytes(w) bytes.
+//      lane (a wave per 64 rows of the widest level);
+//   3, wave blocks (k_sweep_wblk): as wave, for plans of several blocks or
+//      with entries reading published values (earlier blocks' rows, or a
+//      ring's old ones): the wave kernel per block, with those values loaded
+//      ahead.
+// Plans 2 and 3 carry records after the plan (at plan_bytes, a multiple of
+// 16): per position, the w coefficients (-0.0 past the row's count), the w
+// dependencies (edep, kWaveZero past the count: an LDS slot that holds 0.0,
+// so that a padded entry adds fma(-0.0, 0.0, acc) == acc exactly), the row
+// and the count, in wave_rec_bytes(w) bytes.
 // MPK_SWEEP_NARROW=0 leaves every plan unmarked; MPK_SWEEP_WAVE=0 marks no
-// plan 2.  Returns the plan's size with the records.
+// plan 2 or 3.  Returns the plan's size with the records.
 static constexpr int kWaveZero = 8192;
 static int64_t wave_rec_bytes(int64_t w) { return (12 * w + 8 + 15) & ~(int64_t)15; }
 static bool env_off(const char *name) {
@@ -133,23 +137,26 @@ static int64_t mark_plan(char *out, int64_t n, int64_t bytes, int64_t cap) {
   hdr[7] = 0;
   if (env_off("MPK_SWEEP_NARROW")) return bytes;
   const int64_t nblk = hdr[0], nlvl = hdr[2], w = hdr[5];
-  if (nblk != 1 || (w != 4 && w != 8)) return bytes;
+  if (nblk < 1 || (w != 4 && w != 8)) return bytes;
   const int *blk_pos = (const int *)(out + 64), *blk_lev = blk_pos + nblk + 1, *lvl = blk_lev + nblk + 1;
   int widest = 0;
-  for (int64_t l = blk_lev[0]; l + 1 < blk_lev[1]; l++) widest = std::max(widest, lvl[l + 1] - lvl[l]);
-  const int64_t levels = blk_lev[1] - 1 - blk_lev[0];
+  int64_t levels = 0;  // a block's most
+  for (int64_t b = 0; b < nblk; b++) {
+    for (int64_t l = blk_lev[b]; l + 1 < blk_lev[b + 1]; l++) widest = std::max(widest, lvl[l + 1] - lvl[l]);
+    levels = std::max<int64_t>(levels, blk_lev[b + 1] - 1 - blk_lev[b]);
+  }
   const int64_t ints = 2 * (nblk + 1) + nlvl + 3 * n + 1;
   const double *ecoef = (const double *)(out + 64 + ((4 * ints + 7) & ~(int64_t)7));
   const int *edep = (const int *)(ecoef + w * n);
   const int *prow = lvl + nlvl, *pcnt = prow + n;
-  bool stale = false;
+  bool stale = false, published = false;
   for (int64_t p = 0; p < n; p++)
     for (int64_t k = 0; k < pcnt[p]; k++) {
       const int d = edep[k * n + p];
-      if (d < 0 && d >= -n) return bytes;
+      published |= d < 0 && d >= -n;
       stale |= d < -n;
     }
-  const bool narrow = widest <= 256 && levels < 4096;  // its kernel holds levels + 1 bounds
+  const bool narrow = nblk == 1 && !published && widest <= 256 && levels < 4096;  // levels + 1 bounds held
   const int64_t rb = wave_rec_bytes(w), wbytes = bytes + n * rb;
   const bool wave = widest <= 256 && levels <= 12288 && !stale && wbytes <= cap && !env_off("MPK_SWEEP_WAVE");
   if (!wave) {
@@ -170,7 +177,7 @@ static int64_t mark_plan(char *out, int64_t n, int64_t bytes, int64_t cap) {
     d[w + 1] = pcnt[p];
   }
   hdr[4] = wbytes;
-  hdr[7] = 2;
+  hdr[7] = nblk > 1 || published ? 3 : 2;
   return wbytes;
 }
 

@@ -34,12 +34,14 @@ __global__ void k_sweep_prepare(i64 n, double *p, unsigned *ticket, const int64_
   if (i < n) p[i] = __longlong_as_double((long long)MPK_SENTINEL);
 }
 
-// Which sweeps of a PC's plans run in one workgroup and how (hdr[7] and
-// hdr[5] of each: bits 0-7 the first sweep, 8-15 the second; a sweep's byte
-// is its slot count, plus for a wave plan 16, 32 when none of its rows has
-// over 2 entries and 64 times log2 of its waves, or 0), recorded on the host by
-// PCView::schedule when it writes them, so an apply picks the kernel without
-// reading the device.  A plan this process did not write counts as neither.
+// Which sweeps of a PC's plans run on the one-workgroup-per-block kernels
+// and how (hdr[7] and hdr[5] of each: bits 0-15 the first sweep, 16-31 the
+// second; a sweep's word is its slot count, plus for a wave or wave-block
+// plan 16, 32 when none of its rows has over 2 entries, 64 times log2 of its
+// waves, and 256 for wave blocks; 0 for other plans), recorded on the host
+// by PCView::schedule when it writes them, so an apply picks the kernel
+// without reading the device.  A plan this process did not write counts as
+// neither.
 static std::mutex g_narrow_mu;
 static std::map<const char *, int> &narrow_plans() {
   static std::map<const char *, int> m;
@@ -49,11 +51,11 @@ static void note_narrow(const char *plan, int bits) {
   std::lock_guard<std::mutex> lk(g_narrow_mu);
   narrow_plans()[plan] = bits;
 }
-// A sweep's byte of the record above.
+// A sweep's word of the record above.
 static int plan_narrow(const char *plan, int second) {
   std::lock_guard<std::mutex> lk(g_narrow_mu);
   auto it = narrow_plans().find(plan);
-  return it == narrow_plans().end() ? 0 : (it->second >> (second ? 8 : 0)) & 255;
+  return it == narrow_plans().end() ? 0 : (it->second >> (second ? 16 : 0)) & 0xffff;
 }
 
 // One sweep: over its block plan when the PC has one (`plan`, and `second`
@@ -67,6 +69,23 @@ static void run_sweep(const Prog &prog, i64 n, bool forward, double *pub, const 
   if (!c || n <= 0) return;
   const int form = plan ? plan_narrow(plan, second) : 0;
   const int narrow = form & 15;  // the plan's slot count, or 0
+  if (form & 256) {  // wave blocks: published values, so the sentinel fill
+    const int nw = 1 << ((form >> 6) & 3);
+    // blocks hold at least min(n, 1024) rows (mpk_sweep_plan), so this grid
+    // covers every block; the spare workgroups find no block and return
+    const unsigned nblk = nblocks(n, 1024);
+    MPK_LAUNCH(k_sweep_prepare, nblocks(n, 256), 256, n, pub, c->tickets, stop);
+    if (narrow == 4 && (form & 32))
+      MPK_LAUNCH((k_sweep_wblk<Prog, 4, 2>), nblk, MPK_WAVE_THREADS, prog, n, plan, second, st, pub, c->tickets,
+                 nw, stop);
+    else if (narrow == 4)
+      MPK_LAUNCH((k_sweep_wblk<Prog, 4, 4>), nblk, MPK_WAVE_THREADS, prog, n, plan, second, st, pub, c->tickets,
+                 nw, stop);
+    else
+      MPK_LAUNCH((k_sweep_wblk<Prog, 8, 8>), nblk, MPK_WAVE_THREADS, prog, n, plan, second, st, pub, c->tickets,
+                 nw, stop);
+    return;
+  }
   if (form & 16) {
     const int nw = 1 << ((form >> 6) & 3);
     if (narrow == 4 && (form & 32))
@@ -158,18 +177,19 @@ static i64 plan_sweep(const Prog &prog, i64 n, bool forward, char *dst, i64 cap,
   }
   copy_htod(dst, host.get(), bytes);
   *depth = host[3];
-  // narrow: its slot count; wave: the count plus 16, 32 when no row has over
-  // 2 entries, and 64 log2 of its waves (one per 64 rows of its widest level)
+  // narrow: its slot count; wave and wave blocks: the count plus 16, 32 when
+  // no row has over 2 entries, 64 log2 of its waves (one per 64 rows of its
+  // widest level), and 256 for wave blocks
   int longest = 0, widest = 0;
-  if (host[7] == 2) {
+  if (host[7] == 2 || host[7] == 3) {
     const int *ip = reinterpret_cast<const int *>(host.get() + 8);
     const int *lvl = ip + 2 * (host[0] + 1), *pcnt = lvl + host[2] + n;
     for (i64 p = 0; p < n; p++) longest = pcnt[p] > longest ? pcnt[p] : longest;
     for (i64 l = 0; l + 1 < host[2]; l++) widest = lvl[l + 1] - lvl[l] > widest ? lvl[l + 1] - lvl[l] : widest;
   }
   const int lw = widest <= 64 ? 0 : widest <= 128 ? 1 : 2;
-  *narrow = host[7] == 1   ? (int)host[5]
-            : host[7] == 2 ? ((int)host[5] | 16 | (longest <= 2 ? 32 : 0) | (lw << 6))
+  *narrow = host[7] == 1 ? (int)host[5]
+            : host[7] >= 2 ? ((int)host[5] | 16 | (longest <= 2 ? 32 : 0) | (lw << 6) | (host[7] == 3 ? 256 : 0))
                            : 0;
   return failed() ? -1 : bytes;
 }
@@ -342,7 +362,7 @@ i64 PCView::schedule(char *plan, i64 cap) const {
       bad = true;
       return;
     }
-    bits |= narrow << (8 * which++);
+    bits |= narrow << (16 * which++);
     used += bytes;
     depth = d > depth ? d : depth;
   });

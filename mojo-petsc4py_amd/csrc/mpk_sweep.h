@@ -774,42 +774,91 @@ __device__ __forceinline__ void lds_barrier() {
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
 }
 
-// One wave plan's sweep by a whole workgroup of MPK_WAVE_THREADS threads:
-// staging by every thread, the levels by waves 0..nw-1.  val (MPK_BLK_ROWS +
-// 1 doubles) and lv (MPK_WAVE_LVLS + 1 ints) are the caller's LDS.  Every
-// thread must call it; the caller's barrier after it orders the results
-// (st, pub and the finish's outputs, all in global memory) before any
-// thread reads them.
-template <class Prog, int SL, int E>
-__device__ __forceinline__ void wave_sweep(const Prog &prog, i64 n, const char *plan, int second, Staged *st,
+// The published words a set's entries read, loaded ahead (an entry reading
+// LDS loads row 0's word and ignores it: every load is unconditional).
+template <int E>
+struct WavePub {
+  unsigned long long b[E];
+};
+template <int SL, int E>
+__device__ __forceinline__ void wave_pub_issue(const WaveSet<SL> &w, const double *pub, WavePub<E> &q) {
+#pragma unroll
+  for (int e = 0; e < E; e++) {
+    const int d = wave_slot(w, e);
+    q.b[e] = load_pub(pub, d < 0 ? -(i64)d - 1 : 0);
+  }
+}
+
+// Block b of a wave plan (a wave-block plan when Pub) swept by a whole
+// workgroup of MPK_WAVE_THREADS threads: staging by every thread, the levels
+// by waves 0..nw-1.  val (MPK_BLK_ROWS + 1 doubles) and lv (MPK_WAVE_LVLS +
+// 1 ints) are the caller's LDS.  Every thread must call it; the caller's
+// barrier after it orders the results (st, pub and the finish's outputs,
+// all in global memory) before any thread reads them.
+// With Pub, an entry may read another block's row (or a ring's old row)
+// from pub: its word is loaded kPubAhead levels before its level runs, and
+// a word still holding the sentinel then is polled until its row is
+// published (a uniform branch the wave takes only when its producer lags);
+// each row is published as soon as it is computed, since other blocks wait
+// on it (without Pub the store is deferred a step, off the level chain).
+template <class Prog, int SL, int E, bool Pub>
+__device__ __forceinline__ void wave_sweep(const Prog &prog, i64 n, const char *plan, int second, i64 b, Staged *st,
                                            double *pub, double *val, int *lv, int nw) {
-  constexpr int kWaveAhead = SL == 4 ? 8 : 5;  // loads in flight within the 63 a wave can count
+  // loads and stores in flight within the 63 a wave can count
+  constexpr int kWaveAhead = Pub ? (SL == 4 ? 6 : 4) : (SL == 4 ? 8 : 5);
+  constexpr int kPubAhead = Pub ? 2 : 0;
   constexpr int kStageUnroll = 8;
   if (second) plan += reinterpret_cast<const i64 *>(plan)[4];
   const PlanView P = plan_view(plan, n);
-  const int l0 = P.blk_lev[0], nl = P.blk_lev[1] - 1 - l0;  // levels l0..; lvl[l0 + nl] ends the block
-  const int p0 = P.blk_pos[0], p1 = P.blk_pos[1];
+  const int l0 = P.blk_lev[b], nl = P.blk_lev[b + 1] - 1 - l0;  // levels l0..; lvl[l0 + nl] ends the block
+  const int p0 = P.blk_pos[b], p1 = P.blk_pos[b + 1];
+  // Staging: every position's Staged record, kStageUnroll of them a thread
+  // at a time (their loads all issue before the first store).  With one
+  // sweeping wave and Pub, the whole workgroup stages only the first kStage0
+  // positions before the sweep starts; the other waves stage the rest in
+  // position (level) order while it runs, each publishing in LDS the rounds
+  // it has finished (after a release fence), and the sweeping wave waits
+  // for a round only if it gets there first.
+  constexpr int kHelpers = MPK_WAVE_THREADS / 64 - 1;  // waves
+  constexpr int kStage0 = kStageUnroll * MPK_WAVE_THREADS;
+  constexpr int kRound = kStageUnroll * 64 * kHelpers;  // positions a helper round stages
+  __shared__ int stage_rounds[kHelpers];
+  const bool helpers = Pub && nw == 1;
   const int t = (int)threadIdx.x;
+  const int s0 = helpers && p1 - p0 > kStage0 ? p0 + kStage0 : p1;  // staged before the sweep
+  auto stage_span = [=](int q0, int q1, int th, int nth) {
+    for (int q = q0 + th; q < q1; q += kStageUnroll * nth) {
+      Staged tmp[kStageUnroll];
+#pragma unroll
+      for (int u = 0; u < kStageUnroll; u++) {
+        const int p = q + u * nth;
+        if (p < q1) tmp[u] = prog.stage(P.prow[p]);
+      }
+#pragma unroll
+      for (int u = 0; u < kStageUnroll; u++) {
+        const int p = q + u * nth;
+        if (p < q1) st[p] = tmp[u];
+      }
+    }
+  };
 #pragma unroll 4
   for (int l = t; l <= nl; l += MPK_WAVE_THREADS) lv[l] = P.lvl[l0 + l];
-  // every position's Staged record, kStageUnroll of them a thread at a time
-  // (their loads all issue before the first store)
-  for (int b = p0 + t; b < p1; b += kStageUnroll * MPK_WAVE_THREADS) {
-    Staged tmp[kStageUnroll];
-#pragma unroll
-    for (int u = 0; u < kStageUnroll; u++) {
-      const int p = b + u * MPK_WAVE_THREADS;
-      if (p < p1) tmp[u] = prog.stage(P.prow[p]);
-    }
-#pragma unroll
-    for (int u = 0; u < kStageUnroll; u++) {
-      const int p = b + u * MPK_WAVE_THREADS;
-      if (p < p1) st[p] = tmp[u];
-    }
-  }
+  stage_span(p0, s0, t, MPK_WAVE_THREADS);
+  if (t < kHelpers) stage_rounds[t] = 0;
   if (t == 0) val[MPK_WAVE_ZERO] = 0.0;
   __syncthreads();  // st and lv (workgroup scope: the stores to st are seen)
   if (t >= 64 * nw) {
+    if (helpers) {
+      const int h = (t >> 6) - 1, th = t - 64;
+      int r = 0;
+      for (int q = s0; q < p1; q += kRound) {
+        stage_span(q, q + kRound < p1 ? q + kRound : p1, th, 64 * kHelpers);
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");  // this wave's stores, seen in the workgroup
+        r++;
+        if ((t & 63) == 0) __hip_atomic_store(&stage_rounds[h], r, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+      }
+      return;
+    }
     // the other waves meet the level barriers too: a barrier counts every
     // wave of the workgroup still running, and the caller's barrier after
     // the sweep must not pair with them
@@ -818,9 +867,27 @@ __device__ __forceinline__ void wave_sweep(const Prog &prog, i64 n, const char *
         for (int k = 0; k < kWaveAhead; k++) lds_barrier();
     return;
   }
+  // positions below `staged` are staged (uniform); waits until `upto` is
+  int staged = s0;
+  auto wait_staged = [&](int upto) {
+    for (;;) {
+      int m = 1 << 30;
+#pragma unroll
+      for (int h = 0; h < kHelpers; h++) {
+        const int v = __hip_atomic_load(&stage_rounds[h], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+        m = v < m ? v : m;
+      }
+      const i64 s = (i64)s0 + (i64)m * kRound;
+      staged = s < p1 ? (int)s : p1;
+      if (staged >= upto) break;
+      __builtin_amdgcn_s_sleep(1);
+    }
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+  };
   // the first sets in level order: the compiler may not reorder their loads
   // (its waits in the loop are placed for loads issued in that order)
   WaveSet<SL> w[kWaveAhead];
+  WavePub<E> q[kWaveAhead];
 #pragma unroll
   for (int k = 0; k < kWaveAhead; k++) {
     int lo, hi;
@@ -828,10 +895,15 @@ __device__ __forceinline__ void wave_sweep(const Prog &prog, i64 n, const char *
     wave_fetch<SL>(w[k], P, st, lo, hi, t);
     __builtin_amdgcn_sched_barrier(0);
   }
+  if (Pub) {
+#pragma unroll
+    for (int k = 0; k < kPubAhead; k++) wave_pub_issue<SL, E>(w[k], pub, q[k]);
+  }
   // A step: the value reads; while they are in flight, the step before's
-  // deferred work (its published value, the refetch of its set); then the
-  // chain, the finish and the LDS write the next step reads.  The first step
-  // refetches the last set's own level (the same values).
+  // deferred work (its published value, the refetch of its set) and the
+  // published words of the set kPubAhead levels on; then the chain, the
+  // finish and the LDS write the next step reads.  The first step refetches
+  // the last set's own level (the same values).
   double pend_out = 0.0;
   int pend_row = 0;
   int flo, fhi;
@@ -845,17 +917,50 @@ __device__ __forceinline__ void wave_sweep(const Prog &prog, i64 n, const char *
       wave_bounds(lv, nl, l + k + kWaveAhead, lo, hi);
       double v[E];
 #pragma unroll
-      for (int e = 0; e < E; e++) v[e] = val[wave_slot(cur, e)];
+      for (int e = 0; e < E; e++) {
+        const int d = wave_slot(cur, e);
+        v[e] = val[Pub && d < 0 ? MPK_WAVE_ZERO : d];
+      }
       __builtin_amdgcn_sched_barrier(0);
-      if (l + k > 0) store_pub(pub, pend_row, pend_out);  // uniform
+      if (!Pub && l + k > 0) store_pub(pub, pend_row, pend_out);  // uniform
+      if (helpers && fhi > staged) wait_staged(fhi);                // uniform, rare
       wave_fetch<SL>(prev, P, st, flo, fhi, t);
+      if (Pub) {
+        // the set kPubAhead levels on (a level past the last reloads the
+        // last level's words: the same, already published)
+        const int j = (k + kPubAhead) % kWaveAhead;
+        wave_pub_issue<SL, E>(w[j], pub, q[j]);
+      }
       __builtin_amdgcn_sched_barrier(0);
+      if (Pub) {
+        WavePub<E> &qc = q[k];
+        bool lag = false;
+#pragma unroll
+        for (int e = 0; e < E; e++) lag |= wave_slot(cur, e) < 0 && qc.b[e] == MPK_SENTINEL;
+        if (__ballot(lag)) {  // uniform; a producer still behind
+#pragma unroll
+          for (int e = 0; e < E; e++) {
+            const int d = wave_slot(cur, e);
+            if (d < 0)
+              while (qc.b[e] == MPK_SENTINEL) qc.b[e] = load_pub(pub, -(i64)d - 1);
+          }
+          // the next sets' words were loaded before this one was
+          // published, so likely before theirs were too: load them again
+#pragma unroll
+          for (int j = 1; j <= kPubAhead; j++)
+            wave_pub_issue<SL, E>(w[(k + j) % kWaveAhead], pub, q[(k + j) % kWaveAhead]);
+        }
+#pragma unroll
+        for (int e = 0; e < E; e++)
+          if (wave_slot(cur, e) < 0) v[e] = __longlong_as_double((long long)qc.b[e]);
+      }
       double acc = cur.s0.x;  // row_start
 #pragma unroll
       for (int e = 0; e < E; e++) acc = fma(wave_coef(cur, e), v[e], acc);
       const Staged sr{cur.s0.x, cur.s0.y, cur.s1.x, (i64)__double_as_longlong(cur.s1.y)};
       const double out = prog.fin(cur.row, acc, sr);
       val[(cur.pos - p0) & P.mask] = out;
+      if (Pub) store_pub(pub, cur.row, out);  // at once: other blocks wait on it
       pend_out = out;
       pend_row = cur.row;
       flo = lo;
@@ -863,7 +968,7 @@ __device__ __forceinline__ void wave_sweep(const Prog &prog, i64 n, const char *
       if (nw > 1) lds_barrier();  // uniform
     }
   }
-  store_pub(pub, pend_row, pend_out);
+  if (!Pub) store_pub(pub, pend_row, pend_out);
 }
 
 // A wave plan's sweep as its own launch (one workgroup).
@@ -874,15 +979,34 @@ __global__ void __launch_bounds__(MPK_WAVE_THREADS) k_sweep_wave(Prog prog, i64 
   __shared__ double val[MPK_BLK_ROWS + 1];
   __shared__ int lv[MPK_WAVE_LVLS + 1];
   if (stop && *stop) return;
-  wave_sweep<Prog, SL, E>(prog, n, plan, second, st, pub, val, lv, nw);
+  wave_sweep<Prog, SL, E, false>(prog, n, plan, second, 0, st, pub, val, lv, nw);
 }
 
-// A wave plan's kernel form from the registry's byte (slot count in bits
+// A wave-block plan's sweep: one workgroup per block, blocks taken in
+// sweep order by ticket (so the oldest unfinished block reads only finished
+// ones), pub sentinel-filled and the ticket zeroed before the launch.
+template <class Prog, int SL, int E>
+__global__ void __launch_bounds__(MPK_WAVE_THREADS) k_sweep_wblk(Prog prog, i64 n, const char *plan, int second,
+                                                                 Staged *st, double *pub, unsigned *ticket, int nw,
+                                                                 const int64_t *stop) {
+  __shared__ double val[MPK_BLK_ROWS + 1];
+  __shared__ int lv[MPK_WAVE_LVLS + 1];
+  __shared__ unsigned tk_sh;
+  if (threadIdx.x == 0) tk_sh = atomicAdd(ticket, 1u);
+  __syncthreads();
+  if (stop && *stop) return;
+  const i64 nblk = reinterpret_cast<const i64 *>(second ? plan + reinterpret_cast<const i64 *>(plan)[4] : plan)[0];
+  if ((i64)tk_sh >= nblk) return;
+  wave_sweep<Prog, SL, E, true>(prog, n, plan, second, tk_sh, st, pub, val, lv, nw);
+}
+
+// A wave plan's kernel form from the registry's word (slot count in bits
 // 0-3, 16: a wave plan, 32: no row over 2 entries, bits 6-7: log2 of the
-// waves): 0 (4 slots, 2 entries), 1 (4 slots), 2 (8 slots) plus 4 times
-// log2 of the waves; -1 for another plan.
+// waves, 256: a wave-block plan): 0 (4 slots, 2 entries), 1 (4 slots), 2 (8
+// slots) plus 4 times log2 of the waves; -1 for another plan, wave blocks
+// included (one workgroup cannot run them).
 __host__ __device__ inline int wave_form(int byte) {
-  if (!(byte & 16)) return -1;
+  if (!(byte & 16) || (byte & 256)) return -1;
   return ((byte & 15) == 4 ? ((byte & 32) ? 0 : 1) : 2) + 4 * ((byte >> 6) & 3);
 }
 // The sweep in its form, by a whole workgroup (wave_sweep).
@@ -891,11 +1015,11 @@ __device__ __forceinline__ void wave_sweep_form(int form, const Prog &prog, i64 
                                                 Staged *st, double *pub, double *val, int *lv) {
   const int nw = 1 << (form >> 2);
   if ((form & 3) == 0)
-    wave_sweep<Prog, 4, 2>(prog, n, plan, second, st, pub, val, lv, nw);
+    wave_sweep<Prog, 4, 2, false>(prog, n, plan, second, 0, st, pub, val, lv, nw);
   else if ((form & 3) == 1)
-    wave_sweep<Prog, 4, 4>(prog, n, plan, second, st, pub, val, lv, nw);
+    wave_sweep<Prog, 4, 4, false>(prog, n, plan, second, 0, st, pub, val, lv, nw);
   else
-    wave_sweep<Prog, 8, 8>(prog, n, plan, second, st, pub, val, lv, nw);
+    wave_sweep<Prog, 8, 8, false>(prog, n, plan, second, 0, st, pub, val, lv, nw);
 }
 
 // Entries of a sweep in the rows' own order, for the host-side planner:

@@ -368,19 +368,25 @@ def test_check_catches_a_ring_read_outside_the_window(L, monkeypatch):
 
 
 def _narrow_rule(raw, n, wave=True):
-    """The one-workgroup marks restated: one block, 4 or 8 fixed slots, no
-    read of a published value, at most 256 rows a level; then 2 (wave) with
-    at most 12288 levels and no stale read, else 1 (narrow) with at most 4095
-    levels."""
+    """The one-workgroup-per-block marks restated: 4 or 8 fixed slots, every
+    block's levels of at most 256 rows; then 3 (wave blocks: several blocks,
+    or reads of published values) or 2 (wave) with at most 12288 levels a
+    block and no stale read, else 1 (narrow: one block, no published read,
+    at most 4095 levels)."""
     p = _unpack(raw, n)
     d = p["edep"]
-    if p["nblk"] != 1 or p["w"] not in (4, 8) or np.any((d < 0) & (d >= -n)):
+    if p["w"] not in (4, 8):
         return 0
-    lv = p["lvl"][p["blk_lev"][0]:p["blk_lev"][1]]
-    widest, levels = (np.diff(lv).max() if len(lv) > 1 else 0), len(lv) - 1
-    if wave and widest <= 256 and levels <= 12288 and not np.any(d < -n):
-        return 2
-    return 1 if widest <= 256 and levels < 4096 else 0
+    published, stale = bool(np.any((d < 0) & (d >= -n))), bool(np.any(d < -n))
+    widest = levels = 0
+    for b in range(p["nblk"]):
+        lv = p["lvl"][p["blk_lev"][b]:p["blk_lev"][b + 1]]
+        if len(lv) > 1:
+            widest = max(widest, int(np.diff(lv).max()))
+        levels = max(levels, len(lv) - 1)
+    if wave and widest <= 256 and levels <= 12288 and not stale:
+        return 3 if p["nblk"] > 1 or published else 2
+    return 1 if p["nblk"] == 1 and not published and widest <= 256 and levels < 4096 else 0
 
 
 def _wave_records(raw, n):
@@ -406,24 +412,30 @@ def _wave_records(raw, n):
     return at, recs.ravel()
 
 
-@pytest.mark.parametrize("name,want", [("lap20", 2), ("diff7", 2), ("cd6", 2), ("lap100", 2), ("diff20", 0),
-                                       ("lap300", 0)])
+@pytest.mark.parametrize("name,want", [("lap20", 2), ("diff7", 2), ("cd6", 2), ("lap100", 2), ("lap128", 2),
+                                       ("lap128-blocks", 3), ("diff20", 0), ("lap300", 0)])
 @pytest.mark.parametrize("forward", [True, False])
 @pytest.mark.parametrize("stray", [False, True])
 def test_narrow_mark(L, monkeypatch, name, want, forward, stray):
-    """hdr[7] marks the plans one workgroup runs (single block or ring,
-    levels of at most 256 rows, every read in LDS): 2 for the wave kernel (a
-    20^2 or 100^2 grid, small 3-D operators) with its records after the plan,
-    0 for a 20^3 operator in blocks (reads across blocks) and a 300-wide grid
-    (levels of 300 rows).
+    """hdr[7] marks the plans whose blocks one workgroup each runs (levels of
+    at most 256 rows): 2 for the wave kernel (a single block or ring with
+    every read in LDS: 20^2 to 128^2 grids, small 3-D operators), 3 for the
+    wave-block kernel (the 128^2 grid in two blocks, the second reading the
+    first's published values), both with their records after the plan; 0 for
+    a 20^3 operator (levels of over 256 rows) and a 300-wide grid.
     A stale read (a stray later column) keeps a plan out of the wave form.
     MPK_SWEEP_WAVE=0 leaves the wave plans narrow, MPK_SWEEP_NARROW=0 every
     plan unmarked."""
-    mats = dict(SMALL, lap100=lambda: W.laplacian_2d(100), diff20=lambda: W.diffusion_3d(20)[0],
-                lap300=lambda: W.laplacian_2d(300))
+    mats = dict(SMALL, lap100=lambda: W.laplacian_2d(100), lap128=lambda: W.laplacian_2d(128),
+                diff20=lambda: W.diffusion_3d(20)[0], lap300=lambda: W.laplacian_2d(300))
+    mats["lap128-blocks"] = mats["lap128"]
+    if name.endswith("-blocks"):
+        monkeypatch.setenv("MPK_SWEEP_PLAN", "blocks")
     off, col, coef = _lower(*mats[name](), forward, stray)
     if stray and want == 2:
         want = 1
+    if stray and want == 3:
+        want = 0
     n = len(off) - 1
     m = int(off[n])
     cap = L.mpk_sweep_plan_bound(n, m, 8192)
@@ -432,7 +444,7 @@ def test_narrow_mark(L, monkeypatch, name, want, forward, stray):
                           buf.ctypes.data, cap)
     raw = buf.view(np.uint8)[:nb]
     assert int(raw[:64].view(np.int64)[7]) == _narrow_rule(raw, n) == want
-    if want == 2:
+    if want >= 2:
         at, want_bytes = _wave_records(raw, n)
         assert nb == at + len(want_bytes) and nb % 16 == 0
         assert np.array_equal(raw[at:], want_bytes)
@@ -440,7 +452,7 @@ def test_narrow_mark(L, monkeypatch, name, want, forward, stray):
     nb = L.mpk_sweep_plan(n, 1 if forward else 0, off.ctypes.data, col.ctypes.data, coef.ctypes.data, 8192,
                           buf.ctypes.data, cap)
     raw = buf.view(np.uint8)[:nb]
-    assert int(buf[7]) == _narrow_rule(raw, n, wave=False) == (1 if want else 0)
+    assert int(buf[7]) == _narrow_rule(raw, n, wave=False) == (1 if want in (1, 2) else 0)
     monkeypatch.setenv("MPK_SWEEP_NARROW", "0")
     nb = L.mpk_sweep_plan(n, 1 if forward else 0, off.ctypes.data, col.ctypes.data, coef.ctypes.data, 8192,
                           buf.ctypes.data, cap)
