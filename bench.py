@@ -101,19 +101,22 @@ def true_relres(A, bv, x):
     return r.norm() / bv.norm()
 
 
-def cpu_baseline(csr, b, gpu_its: int, sample_its: int):
-    """The oracle's CG + Jacobi on one core for `sample_its` iterations,
-    extrapolated to `gpu_its` iterations."""
+def cpu_baseline(csr, b, x_gpu, max_it: int = 20000):
+    """The oracle's full CG + Jacobi solve to rtol 1e-10 on one core, in the
+    reference's reduction order: its time, iterations and reason, and the
+    gap between its iterate and the GPU's (which computes the tile-tree
+    order)."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle as O
     O.set_mode("ref")
     A = O.CSR(*csr)
     pc = O.PCSetup(A, "jacobi")
     t0 = time.perf_counter()
-    r = O.ksp_solve(A, b, "cg", pc, rtol=1e-10, max_it=sample_its)
+    r = O.ksp_solve(A, b, "cg", pc, rtol=1e-10, max_it=max_it)
     dt = time.perf_counter() - t0
-    per_it = dt / max(r.its, 1)
-    return per_it * gpu_its, per_it, r.its
+    gap2 = float(np.linalg.norm(r.x - x_gpu) / np.linalg.norm(r.x))
+    gapm = float(np.max(np.abs(r.x - x_gpu)) / np.max(np.abs(r.x)))
+    return dt, r.its, r.reason, r.rnorm, gap2, gapm
 
 
 def spmv_only(lib, A, bv, n, nnz, reps):
@@ -185,8 +188,14 @@ def spmv_past_cache(lib, N: int = 256, reps: int = 20):
     ms = lib.mpk_time_spmv(xs._d(), ys._dw(), d.rp.ptr, d.ci.ptr, d.v.ptr, n, so.ptr, sv.ptr, sc.ptr, reps)
     b = spmv_bytes(n, nnz, so)
     gbs = b / (ms * 1e-3) / 1e9
+    # the CG iteration's dominant kernel: the same product with its p.w
+    # partials (the same algorithmic bytes: the partials are 8 bytes a tile)
+    lib.mpk_time_cg_wdot(xs._d(), ys._dw(), d.rp.ptr, n, so.ptr, sv.ptr, sc.ptr, 3)
+    ms_w = lib.mpk_time_cg_wdot(xs._d(), ys._dw(), d.rp.ptr, n, so.ptr, sv.ptr, sc.ptr, reps)
+    gbs_w = b / (ms_w * 1e-3) / 1e9
     return {"workload": f"diffusion_3d {N}^3", "n": n, "nnz": nnz, "bytes_per_launch": b, "avg_launch_ms": ms,
-            "achieved": gbs, "frac": gbs / HBM_PEAK_GBS, "unit": "GB/s"}
+            "achieved": gbs, "frac": gbs / HBM_PEAK_GBS, "unit": "GB/s",
+            "k_cg_wdot": {"avg_launch_ms": ms_w, "achieved": gbs_w, "frac": gbs_w / HBM_PEAK_GBS}}
 
 
 def _mat(csr):
@@ -488,7 +497,26 @@ def kernels_only(lib, A, bv, csr, n, nnz, reps: int = 30):
             op()
         lib.mpk_synchronize()
         out[name] = (time.perf_counter() - t0) / reps * 1e6
-    print(json.dumps({"n": n, "nnz": nnz, "reps": reps, "host_us_per_call": out}))
+    # the fused CG + Jacobi kernels: 30 iterations of the solve
+    ksp = make_ksp(A, "jacobi")
+    ksp.setTolerances(rtol=1e-10, max_it=30)
+    t0 = time.perf_counter()
+    ksp.solve(bv, x)
+    lib.mpk_synchronize()
+    out["cg_jacobi_30_its"] = (time.perf_counter() - t0) * 1e6
+    # 10 copies of 2^24 doubles (128 MiB read, 128 MiB written each), the
+    # kernel of known traffic a counter run scales its readings by
+    from mpk import _native
+    from mpk.device import DeviceBuffer
+    m = 1 << 24
+    src, dst = DeviceBuffer(8 * m), DeviceBuffer(8 * m)
+    lib.mp_vec_set(src.ptr, 1.0, m)
+    for _ in range(10):
+        lib.mp_vec_copy(src.ptr, dst.ptr, m)
+    lib.mpk_synchronize()
+    _native.check()
+    print(json.dumps({"n": n, "nnz": nnz, "reps": reps, "host_us_per_call": out, "copy_bytes_each_way": 8 * m,
+                      "copy_launches": 10}))
     return 0
 
 
@@ -499,7 +527,6 @@ def main() -> int:
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--n", type=int, default=128, help="cells per dimension")
     ap.add_argument("--spmv-reps", type=int, default=50)
-    ap.add_argument("--cpu-its", type=int, default=60, help="iterations in the CPU sample")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-icc", action="store_true")
     ap.add_argument("--no-mixed", action="store_true", help="skip the mixed-precision comparisons")
@@ -574,10 +601,12 @@ def main() -> int:
 
     cpu = None
     if not args.no_cpu:
-        est, per_it, sample = cpu_baseline(csr, b, its, args.cpu_its)
-        cpu = {"value": est, "unit": "s", "cores": 1, "kind": "port",
-               "sample": f"oracle CG+Jacobi, {sample} iterations at {N}^3 on one core of {cpu_model()}"
-                         f" ({per_it * 1e3:.1f} ms/iteration), extrapolated to the GPU's {its} iterations"}
+        dt, c_its, c_reason, c_rnorm, gap2, gapm = cpu_baseline(csr, b, x.getArray().copy())
+        cpu = {"value": dt, "unit": "s", "cores": 1, "kind": "port",
+               "sample": f"the oracle's full CG+Jacobi solve to rtol 1e-10 at {N}^3 in the reference's reduction"
+                         f" order, on one core of {cpu_model()} ({dt / max(c_its, 1) * 1e3:.1f} ms/iteration)",
+               "its": c_its, "reason": c_reason, "rnorm": c_rnorm, "gpu_its": its, "gpu_reason": reason,
+               "iterate_gap_vs_gpu": gap2, "iterate_gap_vs_gpu_max_norm": gapm}
 
     out = {
         "metric": "time to rel. residual 1e-10 + SpMV HBM GB/s; mixed/batched vs plain FP64",

@@ -64,6 +64,12 @@ int64_t mpk_get_reduction_order(void);
  * path).  A diagnostic with no counterpart in the reference: both paths give
  * the same bits. */
 int64_t mpk_small_solves(void);
+/* average ms of `reps` launches of the fused CG product (w = A p over the
+ * sliced copy with the tile partials of p.w, k_cg_wdot), HIP events on the
+ * library stream; -1 without a sliced copy.  A measurement entry with no
+ * counterpart in the reference. */
+double mpk_time_cg_wdot(int64_t p, int64_t w, int64_t irow, int64_t nrow, int64_t soff, int64_t sv, int64_t sc,
+                        int64_t reps);
 /* average ms of `reps` SpMV launches (over the sliced copy when soff != 0),
  * HIP events on the library stream */
 double mpk_time_spmv(int64_t x, int64_t y, int64_t irow, int64_t icol, int64_t data, int64_t nrow, int64_t soff,

@@ -1346,6 +1346,36 @@ extern "C" {
 
 int64_t mpk_small_solves(void) { return g_small_solves.load(); }
 
+// average ms of `reps` launches of k_cg_wdot (w = A p over the sliced copy,
+// with the tile partials of p.w), HIP events on the library stream
+double mpk_time_cg_wdot(int64_t p, int64_t w, int64_t irow, int64_t nrow, int64_t soff, int64_t sv, int64_t sc,
+                        int64_t reps) {
+  Ctx *c = ctx();
+  if (!c || reps <= 0 || nrow <= 0 || !soff) return -1.0;
+  const unsigned T = ntiles(nrow);
+  double *Pp = partials(T);
+  if (!Pp) return -1.0;
+  hipEvent_t e0, e1;
+  HIPCHK(hipEventCreate(&e0));
+  HIPCHK(hipEventCreate(&e1));
+  HIPCHK(hipEventRecord(e0, c->stream));
+  for (int64_t r = 0; r < reps; r++) {
+    if (sell_nt(nrow))
+      MPK_LAUNCH(k_cg_wdot<true>, T, 4 * MPK_NT, (i64)nrow, (const int *)irow, (const int *)soff,
+                 (const double *)sv, (const int *)sc, (const double *)p, (double *)w, Pp, (const int64_t *)nullptr);
+    else
+      MPK_LAUNCH(k_cg_wdot<false>, T, 4 * MPK_NT, (i64)nrow, (const int *)irow, (const int *)soff,
+                 (const double *)sv, (const int *)sc, (const double *)p, (double *)w, Pp, (const int64_t *)nullptr);
+  }
+  HIPCHK(hipEventRecord(e1, c->stream));
+  HIPCHK(hipEventSynchronize(e1));
+  float ms = 0.0f;
+  HIPCHK(hipEventElapsedTime(&ms, e0, e1));
+  HIPCHK(hipEventDestroy(e0));
+  HIPCHK(hipEventDestroy(e1));
+  return failed() ? -1.0 : (double)ms / (double)reps;
+}
+
 // ksp.mojo:271-389 KSPSolve_CG
 void mp_ksp_solve_cg(int64_t n, int64_t irow, int64_t icol, int64_t data, int64_t pci, int64_t pcf,
                      int64_t pstmp, int64_t work, int64_t cfg, int64_t cfgi, int64_t b, int64_t x) {

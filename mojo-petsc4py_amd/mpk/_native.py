@@ -103,6 +103,7 @@ SIGNATURES = {
     "mpk_mat_mult_transpose_t": ([I] * 7, None),
     "mpk_pc_icc_apply_t": ([I] * 13, I),
     "mpk_time_spmv": ([I] * 10, F),
+    "mpk_time_cg_wdot": ([I] * 8, F),
     "mpk_sell_size": ([I, I], I),
     "mpk_sell_size_host": ([I, I], I),
     "mpk_sell_build": ([I] * 7, I),
