@@ -42,7 +42,7 @@ HBM_PEAK_GBS = 8000.0
 # HBM bytes per launch of the roofline kernel, from rocprofv3 FETCH_SIZE and
 # WRITE_SIZE runs scaled by a copy kernel of known size
 # (profiles/spmv_sell_diffusion128_traffic.md), keyed by (kernel, n, nnz).
-MEASURED_TRAFFIC = {("k_spmv_sell", 2097152, 14581760): 190142898}
+MEASURED_TRAFFIC = {("k_spmv_sell", 2097152, 14581760): 190142898, ("k_cg_wdot", 2097152, 14581760): 198584162}
 
 
 def spmv_bytes(n: int, nnz: int, so=None) -> int:
@@ -120,9 +120,9 @@ def cpu_baseline(csr, b, x_gpu, max_it: int = 20000):
 
 
 def spmv_only(lib, A, bv, n, nnz, reps):
-    """The SpMV timing loop alone, then 10 copies of 2^24 doubles (a kernel
-    whose HBM traffic is known: 128 MiB read, 128 MiB written per launch), so a
-    counter run can scale its readings."""
+    """The SpMV timing loop and the CG product's (k_cg_wdot), then 10 copies
+    of 2^24 doubles (a kernel whose HBM traffic is known: 128 MiB read, 128 MiB
+    written per launch), so a counter run can scale its readings."""
     from mpk import _native
     from mpk.device import DeviceBuffer
     d = A._dev()
@@ -130,7 +130,11 @@ def spmv_only(lib, A, bv, n, nnz, reps):
     xs.set(1.0)
     ys = bv.duplicate()
     so, sv, sc = d.sell()
+    # the first launches of each kernel (code object loading) stay untimed
+    lib.mpk_time_spmv(xs._d(), ys._dw(), d.rp.ptr, d.ci.ptr, d.v.ptr, n, so.ptr, sv.ptr, sc.ptr, 5)
     ms = lib.mpk_time_spmv(xs._d(), ys._dw(), d.rp.ptr, d.ci.ptr, d.v.ptr, n, so.ptr, sv.ptr, sc.ptr, reps)
+    lib.mpk_time_cg_wdot(xs._d(), ys._dw(), d.rp.ptr, n, so.ptr, sv.ptr, sc.ptr, 5)
+    ms_w = lib.mpk_time_cg_wdot(xs._d(), ys._dw(), d.rp.ptr, n, so.ptr, sv.ptr, sc.ptr, reps)
     m = 1 << 24
     src, dst = DeviceBuffer(8 * m), DeviceBuffer(8 * m)
     lib.mp_vec_set(src.ptr, 1.0, m)
@@ -138,7 +142,7 @@ def spmv_only(lib, A, bv, n, nnz, reps):
         lib.mp_vec_copy(src.ptr, dst.ptr, m)
     lib.mpk_synchronize()
     _native.check()
-    print(json.dumps({"spmv_ms": ms, "spmv_bytes": spmv_bytes(n, nnz, so), "launches": reps,
+    print(json.dumps({"spmv_ms": ms, "cg_wdot_ms": ms_w, "spmv_bytes": spmv_bytes(n, nnz, so), "launches": reps,
                       "copy_bytes_each_way": 8 * m, "copy_launches": 10}))
     return 0
 
@@ -573,9 +577,14 @@ def main() -> int:
     lib.mpk_time_spmv(xa, ya, d.rp.ptr, d.ci.ptr, d.v.ptr, n, so.ptr, sv.ptr, sc.ptr, 5)
     spmv_ms = lib.mpk_time_spmv(xa, ya, d.rp.ptr, d.ci.ptr, d.v.ptr, n, so.ptr, sv.ptr, sc.ptr, args.spmv_reps)
     row_ms = lib.mpk_time_spmv(xa, ya, d.rp.ptr, d.ci.ptr, d.v.ptr, n, 0, 0, 0, args.spmv_reps)
+    # the solve's dominant kernel: the CG product with its p.w partials, 681
+    # launches a solve; the same algorithmic bytes as the product
+    lib.mpk_time_cg_wdot(xa, ya, d.rp.ptr, n, so.ptr, sv.ptr, sc.ptr, 5)
+    wdot_ms = lib.mpk_time_cg_wdot(xa, ya, d.rp.ptr, n, so.ptr, sv.ptr, sc.ptr, args.spmv_reps)
     _native.check()
     sbytes = spmv_bytes(n, nnz, so)
     spmv_gbs = sbytes / (spmv_ms * 1e-3) / 1e9
+    wdot_gbs = sbytes / (wdot_ms * 1e-3) / 1e9
     cols32 = spmv_cols32(lib, A, bv, n, nnz, args.spmv_reps)
     _native.check()
 
@@ -624,15 +633,19 @@ def main() -> int:
         "config": {"workload": f"diffusion_3d {N}^3 7-point lognormal(0,1), CG+Jacobi rtol 1e-10, FP64 int32 CSR",
                    "n": n, "nnz": nnz, "its": its, "reason": reason, "true_relres": relres,
                    "device": dev_name},
-        "roofline": {"bound": "hbm", "kernel": "k_spmv_sell", "achieved": spmv_gbs, "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": spmv_gbs / HBM_PEAK_GBS,
-                     "traffic": MEASURED_TRAFFIC.get(("k_spmv_sell", n, nnz)),
-                     "bytes_per_launch": sbytes, "avg_launch_ms": spmv_ms,
-                     "row_csr_kernel_ms": row_ms,
-                     "note": f"the {sbytes / 1e6:.0f} MB working set stays in the 256 MB last-level cache across"
-                             " back-to-back launches; the entries are 10 bytes when the sliced copy carries"
-                             " 16-bit column offsets (cols32: the same kernel over int32 columns); past_cache is"
-                             " the same kernel on a 1.74 GB matrix",
+        "roofline": {"bound": "hbm", "kernel": "k_cg_wdot", "achieved": wdot_gbs, "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": wdot_gbs / HBM_PEAK_GBS,
+                     "traffic": MEASURED_TRAFFIC.get(("k_cg_wdot", n, nnz)),
+                     "bytes_per_launch": sbytes, "avg_launch_ms": wdot_ms,
+                     "note": f"the CG product (w = A p with the tile partials of p.w), the dominant kernel of the"
+                             f" solve, timed over back-to-back launches; its {sbytes / 1e6:.0f} MB working set stays"
+                             " in the 256 MB last-level cache across them (past_cache: the same kernels on a"
+                             " 1.74 GB matrix); the entries are 10 bytes when the sliced copy carries 16-bit column"
+                             " offsets (cols32: the product over int32 columns)",
+                     "spmv": {"kernel": "k_spmv_sell", "avg_launch_ms": spmv_ms, "achieved": spmv_gbs,
+                              "frac": spmv_gbs / HBM_PEAK_GBS,
+                              "traffic": MEASURED_TRAFFIC.get(("k_spmv_sell", n, nnz)),
+                              "row_csr_kernel_ms": row_ms},
                      "cols32": cols32,
                      "past_cache": past},
         "cpu_baseline": cpu,

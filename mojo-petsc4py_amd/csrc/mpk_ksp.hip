@@ -17,6 +17,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <memory>
+#include <mutex>
 #include <vector>
 
 #include "mpk_internal.h"
@@ -545,7 +546,7 @@ __global__ void __launch_bounds__(4 * MPK_NT) __attribute__((amdgpu_waves_per_eu
   // the stop flag and the row bounds in one round of loads (indices clamped
   // to row n - 1, n >= 1, so that no load waits on a branch), before the
   // entries are read
-  const bool stopped = *stop != 0;  // never null here: the solve's own flag
+  const bool stopped = stop && *stop != 0;  // the solve's own flag (null: never stopped)
   const unsigned tile = xcd_block(blockIdx.x, gridDim.x);
   const int T = (int)threadIdx.x;
   const i64 e0 = (i64)tile * MPK_TILE + T, e1 = e0 + MPK_TILE / 2;
@@ -624,7 +625,7 @@ __global__ void __launch_bounds__(MPK_NT) k_cg_xrz(i64 n, double *x, double *r, 
   // every load is issued before any is waited on: the stop flag, the state
   // the p.w step reads, the tile's operands, then the partials; a stopped
   // solve returns before anything is written
-  const bool stopped = *stop != 0;  // never null here: the solve's own flag
+  const bool stopped = stop && *stop != 0;  // the solve's own flag (null: never stopped)
   const i64 si = s->i;
   const double dpiold = s->dpiold, sbeta = s->beta, s_a = s->a;
   const i64 base = (i64)blockIdx.x * MPK_TILE + 2 * (i64)threadIdx.x;
@@ -700,7 +701,7 @@ __global__ void __launch_bounds__(MPK_NT) k_cg_pz(i64 n, double *p, const double
   // the norm step's decisions read (none of it changes in that step), the
   // tile's operands, then the partials; a stopped solve returns before
   // anything is written
-  const bool stopped = *stop != 0;  // never null here: the solve's own flag
+  const bool stopped = stop && *stop != 0;  // the solve's own flag (null: never stopped)
   const ConvView cv = ConvView::load(s);
   const double betaold = s->betaold;
   const i64 base = (i64)blockIdx.x * MPK_TILE + 2 * (i64)threadIdx.x;
@@ -1347,11 +1348,27 @@ extern "C" {
 int64_t mpk_small_solves(void) { return g_small_solves.load(); }
 
 // average ms of `reps` launches of k_cg_wdot (w = A p over the sliced copy,
-// with the tile partials of p.w), HIP events on the library stream
+// with the tile partials of p.w), HIP events on the library stream.  The
+// kernel reads its stop flag with its first loads, so it gets a device flag
+// holding 0 (kept for the process).
 double mpk_time_cg_wdot(int64_t p, int64_t w, int64_t irow, int64_t nrow, int64_t soff, int64_t sv, int64_t sc,
                         int64_t reps) {
   Ctx *c = ctx();
   if (!c || reps <= 0 || nrow <= 0 || !soff) return -1.0;
+  static std::mutex mu;
+  static int64_t *no_stop = nullptr;
+  {
+    std::lock_guard<std::mutex> lk(mu);
+    if (!no_stop) {
+      int64_t *f = nullptr;
+      HIPCHK(hipMalloc(&f, sizeof(int64_t)));
+      if (failed() || !f) return -1.0;
+      HIPCHK(hipMemset(f, 0, sizeof(int64_t)));
+      HIPCHK(hipDeviceSynchronize());
+      if (failed()) return -1.0;
+      no_stop = f;
+    }
+  }
   const unsigned T = ntiles(nrow);
   double *Pp = partials(T);
   if (!Pp) return -1.0;
@@ -1362,10 +1379,10 @@ double mpk_time_cg_wdot(int64_t p, int64_t w, int64_t irow, int64_t nrow, int64_
   for (int64_t r = 0; r < reps; r++) {
     if (sell_nt(nrow))
       MPK_LAUNCH(k_cg_wdot<true>, T, 4 * MPK_NT, (i64)nrow, (const int *)irow, (const int *)soff,
-                 (const double *)sv, (const int *)sc, (const double *)p, (double *)w, Pp, (const int64_t *)nullptr);
+                 (const double *)sv, (const int *)sc, (const double *)p, (double *)w, Pp, (const int64_t *)no_stop);
     else
       MPK_LAUNCH(k_cg_wdot<false>, T, 4 * MPK_NT, (i64)nrow, (const int *)irow, (const int *)soff,
-                 (const double *)sv, (const int *)sc, (const double *)p, (double *)w, Pp, (const int64_t *)nullptr);
+                 (const double *)sv, (const int *)sc, (const double *)p, (double *)w, Pp, (const int64_t *)no_stop);
   }
   HIPCHK(hipEventRecord(e1, c->stream));
   HIPCHK(hipEventSynchronize(e1));
