@@ -38,6 +38,12 @@ sys.path[:0] = [os.path.join(ROOT, "mojo-petsc4py_amd"), ROOT]
 import numpy as np  # noqa: E402
 
 HBM_PEAK_GBS = 8000.0
+_T0 = time.perf_counter()
+
+
+def _progress(msg: str) -> None:
+    """One line on stderr as each leg of the benchmark ends."""
+    print(f"[bench {time.perf_counter() - _T0:7.1f} s] {msg}", file=sys.stderr, flush=True)
 
 # HBM bytes per launch of the roofline kernel, from rocprofv3 FETCH_SIZE and
 # WRITE_SIZE runs scaled by a copy kernel of known size
@@ -566,6 +572,7 @@ def main() -> int:
     t_solve = timed_solves(ksp, bv, x, args.steps, args.warmup)
     its, reason = ksp.getIterationNumber(), ksp.getConvergedReason()
     relres = true_relres(A, bv, x)
+    _progress(f"cg+jacobi {N}^3: {t_solve * 1e3:.2f} ms, {its} its")
 
     # SpMV: average launch time, HIP events on the library stream
     d = A._dev()
@@ -587,6 +594,7 @@ def main() -> int:
     wdot_gbs = sbytes / (wdot_ms * 1e-3) / 1e9
     cols32 = spmv_cols32(lib, A, bv, n, nnz, args.spmv_reps)
     _native.check()
+    _progress(f"spmv {spmv_ms * 1e3:.1f} us, cg product {wdot_ms * 1e3:.1f} us")
 
     extra = {}
     if not args.no_icc:
@@ -595,18 +603,24 @@ def main() -> int:
         t_icc = timed_solves(ksp2, bv, x2, 1, 1)
         extra["cg_icc0"] = {"seconds": t_icc, "its": ksp2.getIterationNumber(),
                             "reason": ksp2.getConvergedReason(), "true_relres": true_relres(A, bv, x2)}
+        _progress(f"cg+icc(0): {t_icc * 1e3:.2f} ms")
 
     mixed = {}
     if not args.no_mixed:
         mixed["convection_diffusion_128"] = bench_mixed(lib, "c3", cpu=not args.no_cpu)
+        _progress("mixed precision: convection-diffusion done")
         mixed["random_spd_1e6"] = bench_mixed(lib, "c4", cpu=not args.no_cpu)
+        _progress("mixed precision: random SPD done")
     batched = None if args.no_batched else bench_batched(lib, cpu=not args.no_cpu)
+    _progress("batched done")
     small = None if args.no_small else bench_small(lib, cpu=not args.no_cpu)
+    _progress("small systems done")
 
     past = None
     if not args.no_large and N == 128:
         del d, xs, ys
         past = spmv_past_cache(lib)
+        _progress("256^3 product done")
 
     cpu = None
     if not args.no_cpu:

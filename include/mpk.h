@@ -57,6 +57,20 @@ void mpk_synchronize(void);
  * any other value is an error (status 4) and leaves the order unchanged. */
 int64_t mpk_set_reduction_order(int64_t order);
 int64_t mpk_get_reduction_order(void);
+/* Bounded sweep waits (no counterpart in the reference, whose sweeps are
+ * serial loops that never wait).  A triangular or SOR sweep on the device
+ * waits for rows other workgroups publish; each such wait gives up after
+ * `polls` polls (default 2^24, at least about a second), or within 256 polls
+ * once mpk_sweep_abort(1) has raised the abort word, takes a NaN for the value
+ * and lets its kernel run to the end.  The library then reports status 10 at
+ * its next synchronisation (mpk_synchronize, mpk_status, any solve).  A
+ * well-formed plan never gives up.  mpk_set_sweep_wait_limit returns the
+ * previous limit (a value outside 1..2^32-1 is error 4); mpk_sweep_abort(0)
+ * lowers the word again, returns 0, or -1 without a device.  The abort word
+ * is host memory the running kernels read, so another host thread may raise
+ * it while a sweep runs. */
+int64_t mpk_set_sweep_wait_limit(int64_t polls);
+int64_t mpk_sweep_abort(int64_t on);
 /* The number of Krylov solves since the library loaded that ran as one
  * launch of one workgroup (small systems in the tile-tree order: n <= 16384
  * with wave-plan sweeps, n <= 4096 with a preconditioner without sweeps;

@@ -39,7 +39,14 @@ struct Ctx {
   size_t dtmp_cap = 0;
   double *fixed = nullptr;     // the fixed scratch array of the no-allocation entry point
   size_t fixed_cap = 0;        // in doubles
+  unsigned *watch = nullptr;   // fine-grained host words of the sweep waits: [0] abort, [1] fault
 };
+
+// The sweep waits' watch: its host words, the polls a wait makes before it
+// gives up, and a generation that changes whenever either changes (each
+// source file that launches polling kernels copies it to its device side
+// when the generation moved; mpk_sweep.h).
+void sweep_watch_state(unsigned **word, unsigned *limit, unsigned *gen);
 
 // The context if it is initialised already, else nullptr; never initialises
 // (so never allocates).

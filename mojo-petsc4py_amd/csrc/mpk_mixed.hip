@@ -621,6 +621,7 @@ struct MixedSolver {
       return;
     }
     // the planned sweep on the FP32 engine: stage, then the block or ring plan
+    watch_install();
     MPK_LAUNCH(k_lp_prepare, lp_grid(n), 256, n, pub, ticket);
     const void *lc = second ? lc1 : lc0;
     MPK_LAUNCH((k_sweep_stage<Prog, LpStaged>), lp_grid(n), 256, prog, n, pc_plan(), second, lst,

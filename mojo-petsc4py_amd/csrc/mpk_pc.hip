@@ -67,6 +67,7 @@ static void run_sweep(const Prog &prog, i64 n, bool forward, double *pub, const 
                       const char *plan = nullptr, int second = 0, Staged *st = nullptr) {
   Ctx *c = ctx();
   if (!c || n <= 0) return;
+  watch_install();
   const int form = plan ? plan_narrow(plan, second) : 0;
   const int narrow = form & 15;  // the plan's slot count, or 0
   if (form & 256) {  // wave blocks: published values, so the sentinel fill

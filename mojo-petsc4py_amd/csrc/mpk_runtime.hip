@@ -27,6 +27,12 @@ static double *g_fixed_ptr[MPK_MAX_DEVICES] = {};
 static std::mutex g_mu;
 static int g_status = 0;
 static char g_msg[1024] = "";
+// Polls a sweep's wait makes before it gives up: at least ~1 s at one poll
+// per sleep-and-load round trip, far beyond any wait of a well-formed plan
+// (a whole sweep takes milliseconds).
+#define MPK_WATCH_DEFAULT_POLLS (1u << 24)
+static std::atomic<unsigned> g_watch_limit{MPK_WATCH_DEFAULT_POLLS};
+static std::atomic<unsigned> g_watch_gen{1};
 
 void set_error(int code, const char *fmt, ...) {
   if (g_status != 0) return;  // keep the first error
@@ -56,6 +62,10 @@ static bool init_locked(int device) {
   HIPCHK(hipHostMalloc(&g_ctx.hres, 1024 * sizeof(double), hipHostMallocDefault));
   HIPCHK(hipMalloc(&g_ctx.tickets, 64 * sizeof(unsigned)));
   HIPCHK(hipMemset(g_ctx.tickets, 0, 64 * sizeof(unsigned)));
+  // fine-grained (coherent) host memory: the host's abort store reaches a
+  // running kernel, and a kernel's fault store reaches the host
+  HIPCHK(hipHostMalloc(&g_ctx.watch, 4 * sizeof(unsigned), hipHostMallocCoherent));
+  if (g_ctx.watch) std::memset(g_ctx.watch, 0, 4 * sizeof(unsigned));
   if (device >= 0 && device < MPK_MAX_DEVICES && !g_fixed_ptr[device]) {
     void *p = nullptr;
     hipError_t se = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
@@ -143,10 +153,31 @@ double *scratch_vector(size_t need) {
   return c->dtmp;
 }
 
+// Records an error when a kernel has raised the sweep watch's fault word
+// since the last check (and clears the word).
+static void watch_check() {
+  Ctx *c = ctx_ready();
+  if (!c || !c->watch) return;
+  volatile unsigned *w = c->watch;
+  if (w[1]) {
+    w[1] = 0u;
+    set_error(10, "a sweep's wait for a published row gave up (after %u polls, or on the abort word): "
+                  "the sweep's results are not valid", g_watch_limit.load());
+  }
+}
+
 void sync() {
   Ctx *c = ctx();
   if (!c) return;
   HIPCHK(hipStreamSynchronize(c->stream));
+  watch_check();
+}
+
+void sweep_watch_state(unsigned **word, unsigned *limit, unsigned *gen) {
+  Ctx *c = ctx_ready();
+  *word = c ? c->watch : nullptr;
+  *limit = g_watch_limit.load();
+  *gen = c && c->watch ? g_watch_gen.load() : 0u;
 }
 
 // -1: not read yet; 0: tile-tree order; 1: the reference's order
@@ -177,7 +208,10 @@ int64_t mpk_init(int64_t device) {
   return init_locked((int)device) ? 0 : (g_status ? g_status : 2);
 }
 
-int64_t mpk_status(void) { return g_status; }
+int64_t mpk_status(void) {
+  watch_check();
+  return g_status;
+}
 
 int64_t mpk_last_error(int64_t buf, int64_t len) {
   if (buf && len > 0) {
@@ -259,5 +293,24 @@ int64_t mpk_set_reduction_order(int64_t order) {
 }
 
 int64_t mpk_get_reduction_order(void) { return red_ref() ? 1 : 0; }
+
+int64_t mpk_set_sweep_wait_limit(int64_t polls) {
+  const int64_t old = g_watch_limit.load();
+  if (polls < 1 || polls > 0xffffffffLL) {
+    set_error(4, "mpk_set_sweep_wait_limit: %lld polls is outside 1..2^32-1", (long long)polls);
+    return old;
+  }
+  g_watch_limit.store((unsigned)polls);
+  g_watch_gen.fetch_add(1u);
+  return old;
+}
+
+int64_t mpk_sweep_abort(int64_t on) {
+  Ctx *c = ctx();
+  if (!c || !c->watch) return -1;
+  volatile unsigned *w = c->watch;
+  w[0] = on ? 1u : 0u;
+  return 0;
+}
 
 }  // extern "C"

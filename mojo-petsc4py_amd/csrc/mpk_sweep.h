@@ -40,12 +40,57 @@ __device__ __forceinline__ void store_pub_f(float *pub, i64 k, float v) {
   __hip_atomic_store(reinterpret_cast<unsigned *>(pub + k), b, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
+// Bounded waits.  A wait for a row another workgroup publishes counts its
+// polls; it gives up after the watch's limit, or at its next 256th poll once
+// the host raises the abort word (mpk_sweep_abort), and then takes a quiet
+// NaN for the value.  The kernel so runs to its end (the rows downstream of
+// the wait come out NaN) and, as its last act, raises the fault word, which
+// the library reports as an error at its next synchronisation.  Both words
+// sit in fine-grained host memory.  A well-formed plan never gives up: its
+// blocks take tickets in sweep order, so every awaited row is being computed.
+// `gave` is kept in a register by the loops and reported once at the end, so
+// no store sits in a loop's branches.
+struct SweepWatch {
+  unsigned *word;  // [0] abort (set by the host), [1] fault (set by kernels)
+  unsigned limit;  // polls before a wait gives up
+};
+static __device__ SweepWatch g_sweep_watch;  // this file's copy (watch_install)
+
+// Counts one poll; true when the wait must give up.
+__device__ __forceinline__ bool watch_expired(unsigned &polls) {
+  ++polls;
+  const SweepWatch w = g_sweep_watch;
+  if (!w.word) return false;
+  if (polls >= w.limit) return true;
+  return (polls & 255u) == 0 && __hip_atomic_load(w.word, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM) != 0u;
+}
+__device__ __forceinline__ void watch_report(bool gave) {
+  if (!gave) return;
+  unsigned *w = g_sweep_watch.word;
+  if (w) __hip_atomic_store(w + 1, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+}
+
+// Host side: copies the process's watch (words and limit) into this file's
+// g_sweep_watch when it changed since the last copy; every launch of a
+// kernel that polls goes after it.
+static unsigned g_sweep_watch_gen = 0;
+static inline void watch_install() {
+  unsigned *word = nullptr;
+  unsigned limit = 0, gen = 0;
+  sweep_watch_state(&word, &limit, &gen);
+  if (gen == g_sweep_watch_gen) return;
+  const SweepWatch w{word, limit};
+  HIPCHK(hipMemcpyToSymbol(HIP_SYMBOL(g_sweep_watch), &w, sizeof w, 0, hipMemcpyHostToDevice));
+  if (!failed()) g_sweep_watch_gen = gen;
+}
+
 // Value types of a planned sweep: FP64 (the drop-in preconditioners) or FP32
 // (the inner solves of the mixed-precision solver).
 struct SweepF64 {
   using V = double;
   using Bits = unsigned long long;
   static constexpr Bits kSentinel = MPK_SENTINEL;
+  static constexpr Bits kQuiet = MPK_QNAN;
   __device__ static Bits load(const V *pub, i64 k) { return load_pub(pub, k); }
   __device__ static void store(V *pub, i64 k, V v) { store_pub(pub, k, v); }
   __device__ static V value(Bits b) { return __longlong_as_double((long long)b); }
@@ -55,6 +100,7 @@ struct SweepF32 {
   using V = float;
   using Bits = unsigned;
   static constexpr Bits kSentinel = MPK_SENTINEL_F;
+  static constexpr Bits kQuiet = MPK_QNAN_F;
   __device__ static Bits load(const V *pub, i64 k) { return load_pub_f(pub, k); }
   __device__ static void store(V *pub, i64 k, V v) { store_pub_f(pub, k, v); }
   __device__ static V value(Bits b) { return __uint_as_float(b); }
@@ -107,6 +153,8 @@ __global__ void __launch_bounds__(64) k_sweep(Prog prog, i64 n, int forward, uns
     je = prog.end(i);
   }
   unsigned long long mask = 0;  // lanes of this wave that have published
+  unsigned polls = 0;
+  bool gave = false;  // the wave's wait gave up: unready values read as NaN
   for (;;) {
     bool moved = false;
     if (!done) {
@@ -120,11 +168,18 @@ __global__ void __launch_bounds__(64) k_sweep(Prog prog, i64 n, int forward, uns
           // without its diagonal) reaches this; it reads the pre-sweep value
           v = prog.stale(k);
         } else if (Lk >= L0) {
-          if (!((mask >> (Lk - L0)) & 1ull)) break;
-          v = lv[Lk - L0];
+          if (!((mask >> (Lk - L0)) & 1ull)) {
+            if (!gave) break;
+            v = __longlong_as_double((long long)MPK_QNAN);
+          } else {
+            v = lv[Lk - L0];
+          }
         } else {
           unsigned long long b = load_pub(pub, k);
-          if (b == MPK_SENTINEL) break;
+          if (b == MPK_SENTINEL) {
+            if (!gave) break;
+            b = MPK_QNAN;
+          }
           v = __longlong_as_double((long long)b);
         }
         acc = sweep_step(prog, j, v, acc);
@@ -142,9 +197,13 @@ __global__ void __launch_bounds__(64) k_sweep(Prog prog, i64 n, int forward, uns
     const unsigned long long newmask = __ballot(done);
     if (newmask == ~0ull) break;
     // nothing new in this wave: the rows it waits on belong to earlier waves
-    if (newmask == mask && __ballot(moved) == 0ull) __builtin_amdgcn_s_sleep(2);
+    if (newmask == mask && __ballot(moved) == 0ull) {
+      __builtin_amdgcn_s_sleep(2);
+      if (watch_expired(polls)) gave = true;  // uniform: polls and the watch are
+    }
     mask = newmask;
   }
+  if (lane == 0) watch_report(gave);
 }
 
 // Block schedule ("plan") of a sweep.  The rows are cut into blocks of
@@ -297,7 +356,7 @@ __device__ __forceinline__ void gather_issue(i64 n, const typename Tr::V *val, c
 
 template <int G, class Tr, class Prog>
 __device__ __forceinline__ void gather_resolve(const Prog &prog, i64 n, const typename Tr::V *pub, const int *d,
-                                               int cnt, const Gather<G, Tr> &g, typename Tr::V *v) {
+                                               int cnt, const Gather<G, Tr> &g, typename Tr::V *v, bool &gave) {
 #pragma unroll
   for (int e = 0; e < G; e++) {
     if (e < cnt) {
@@ -306,8 +365,14 @@ __device__ __forceinline__ void gather_resolve(const Prog &prog, i64 n, const ty
         v[e] = g.lv[e];
       } else if (de >= -n) {
         typename Tr::Bits b = g.gb[e];
+        unsigned polls = 0;
         while (b == Tr::kSentinel) {
           __builtin_amdgcn_s_sleep(1);
+          if (watch_expired(polls)) {
+            gave = true;
+            b = Tr::kQuiet;
+            break;
+          }
           b = Tr::load(pub, -(i64)de - 1);
         }
         v[e] = Tr::value(b);
@@ -323,13 +388,13 @@ __device__ __forceinline__ void gather_resolve(const Prog &prog, i64 n, const ty
 template <int PF, class Tr, class Prog, class Coef, class S>
 __device__ __forceinline__ void plan_finish(const Prog &prog, const Coef &coef, i64 n, const PlanView &P,
                                             const PlanRow<PF, Tr, S> &r, const Gather<PF, Tr> &g,
-                                            typename Tr::V *val, int p0, typename Tr::V *pub) {
+                                            typename Tr::V *val, int p0, typename Tr::V *pub, bool &gave) {
   using V = typename Tr::V;
   V acc = row_start(r.s);
   int j_rest = 0;  // entries still to do (beyond the registers, or CSR form)
   if (P.w) {
     V v[PF];
-    gather_resolve<PF, Tr>(prog, n, pub, r.d, r.cnt, g, v);
+    gather_resolve<PF, Tr>(prog, n, pub, r.d, r.cnt, g, v, gave);
 #pragma unroll
     for (int e = 0; e < PF; e++)
       if (e < r.cnt) acc = Tr::madd(r.c[e], v[e], acc);
@@ -348,7 +413,7 @@ __device__ __forceinline__ void plan_finish(const Prog &prog, const Coef &coef, 
     }
     Gather<8, Tr> g8;
     gather_issue<8, Tr>(n, val, pub, d, k, g8);
-    gather_resolve<8, Tr>(prog, n, pub, d, k, g8, v);
+    gather_resolve<8, Tr>(prog, n, pub, d, k, g8, v, gave);
 #pragma unroll
     for (int e = 0; e < 8; e++)
       if (e < k) acc = Tr::madd(c[e], v[e], acc);
@@ -361,10 +426,10 @@ __device__ __forceinline__ void plan_finish(const Prog &prog, const Coef &coef, 
 template <int PF, class Tr, class Prog, class Coef, class S>
 __device__ __forceinline__ void plan_compute(const Prog &prog, const Coef &coef, i64 n, const PlanView &P,
                                              const PlanRow<PF, Tr, S> &r, typename Tr::V *val, int p0,
-                                             typename Tr::V *pub) {
+                                             typename Tr::V *pub, bool &gave) {
   Gather<PF, Tr> g;
   if (P.w) gather_issue<PF, Tr>(n, val, pub, r.d, r.cnt, g);
-  plan_finish<PF, Tr>(prog, coef, n, P, r, g, val, p0, pub);
+  plan_finish<PF, Tr>(prog, coef, n, P, r, g, val, p0, pub, gave);
 }
 
 // Ring plans run in workgroup 0 while workgroups 8, 16, .., 8 MPK_RING_PF
@@ -405,15 +470,22 @@ __device__ __forceinline__ void ring_prefetch(const PlanView &P, const S *st, co
   const int t = (int)threadIdx.x;
   const int nl = P.blk_lev[1] - 1;
   unsigned acc = 0;
+  bool gave = false;
   for (int l = k - 1; l < nl; l += MPK_RING_PF) {
     if (t == 0) {
       unsigned pr = ring_progress(progress);
+      unsigned polls = 0;
       while (pr != MPK_RING_DONE && l > (int)pr + MPK_RING_AHEAD) {
         if (stop && *stop) {
           pr = MPK_RING_DONE;
           break;
         }
         __builtin_amdgcn_s_sleep(2);
+        if (watch_expired(polls)) {
+          gave = true;
+          pr = MPK_RING_DONE;
+          break;
+        }
         pr = ring_progress(progress);
       }
       done_sh = pr == MPK_RING_DONE;
@@ -448,6 +520,7 @@ __device__ __forceinline__ void ring_prefetch(const PlanView &P, const S *st, co
     __syncthreads();
   }
   if (acc == 0x9E3779B9u) *sink = acc;  // keeps the loads; the value is never used
+  if (t == 0) watch_report(gave);
 }
 
 // The levels of block b, holding PF slots of each row in registers.  With a
@@ -456,7 +529,7 @@ __device__ __forceinline__ void ring_prefetch(const PlanView &P, const S *st, co
 template <int PF, class Tr, class Prog, class Coef, class S>
 __device__ __forceinline__ void plan_levels(const Prog &prog, const Coef &coef, i64 n, const PlanView &P,
                                             const S *st, i64 b, typename Tr::V *val, typename Tr::V *pub,
-                                            unsigned *progress) {
+                                            unsigned *progress, bool &gave) {
   const int t = (int)threadIdx.x;
   const int p0 = P.blk_pos[b];
   const int l0 = P.blk_lev[b], l1 = P.blk_lev[b + 1] - 1;  // levels l0..l1-1; lvl[l1] ends the block
@@ -473,9 +546,9 @@ __device__ __forceinline__ void plan_levels(const Prog &prog, const Coef &coef, 
     Gather<PF, Tr> g;
     if (cur.p >= 0 && P.w) gather_issue<PF, Tr>(n, val, pub, cur.d, cur.cnt, g);
     const PlanRow<PF, Tr, S> nxt = plan_row<PF, Tr>(P, coef, st, pb + t, more && pb + t < pc);
-    if (cur.p >= 0) plan_finish<PF, Tr>(prog, coef, n, P, cur, g, val, p0, pub);
+    if (cur.p >= 0) plan_finish<PF, Tr>(prog, coef, n, P, cur, g, val, p0, pub, gave);
     for (int p = pa + t + 256; p < pb; p += 256)
-      plan_compute<PF, Tr>(prog, coef, n, P, plan_row<PF, Tr>(P, coef, st, p, true), val, p0, pub);
+      plan_compute<PF, Tr>(prog, coef, n, P, plan_row<PF, Tr>(P, coef, st, p, true), val, p0, pub, gave);
     __syncthreads();
     cur = nxt;
     pa = pb;
@@ -501,16 +574,18 @@ __global__ void __launch_bounds__(256) k_sweep_blk(Prog prog, i64 n, const char 
   if (second) plan += reinterpret_cast<const i64 *>(plan)[4];
   const PlanView P = plan_view(plan, n);
   const Coef coef = Coef::make(P, lc);
+  bool gave = false;
   if (P.mask >= 0) {  // ring plan
     const unsigned g = blockIdx.x;
     if (g == 0) {
       if (!(stop && *stop)) {
         if (P.w == 4)
-          plan_levels<4, Tr>(prog, coef, n, P, st, 0, val, pub, ticket + 1);
+          plan_levels<4, Tr>(prog, coef, n, P, st, 0, val, pub, ticket + 1, gave);
         else
-          plan_levels<MPK_PLAN_PF, Tr>(prog, coef, n, P, st, 0, val, pub, ticket + 1);
+          plan_levels<MPK_PLAN_PF, Tr>(prog, coef, n, P, st, 0, val, pub, ticket + 1, gave);
       }
       if (threadIdx.x == 0) ring_publish(ticket + 1, MPK_RING_DONE);
+      watch_report(gave);
     } else if (g % 8 == 0 && g / 8 <= MPK_RING_PF) {
       ring_prefetch(P, st, reinterpret_cast<const char *>(coef.c), Coef::kBytes, (int)(g / 8), ticket + 1,
                     ticket + 2, stop);
@@ -523,9 +598,10 @@ __global__ void __launch_bounds__(256) k_sweep_blk(Prog prog, i64 n, const char 
   const i64 b = tk_sh;
   if (b >= P.nblk) return;
   if (P.w == 4)
-    plan_levels<4, Tr>(prog, coef, n, P, st, b, val, pub, nullptr);
+    plan_levels<4, Tr>(prog, coef, n, P, st, b, val, pub, nullptr, gave);
   else
-    plan_levels<MPK_PLAN_PF, Tr>(prog, coef, n, P, st, b, val, pub, nullptr);
+    plan_levels<MPK_PLAN_PF, Tr>(prog, coef, n, P, st, b, val, pub, nullptr, gave);
+  watch_report(gave);
 }
 
 // Narrow plans (the planner marks them in hdr[7]): a single block, or a ring,
@@ -803,7 +879,7 @@ __device__ __forceinline__ void wave_pub_issue(const WaveSet<SL> &w, const doubl
 // on it (without Pub the store is deferred a step, off the level chain).
 template <class Prog, int SL, int E, bool Pub>
 __device__ __forceinline__ void wave_sweep(const Prog &prog, i64 n, const char *plan, int second, i64 b, Staged *st,
-                                           double *pub, double *val, int *lv, int nw) {
+                                           double *pub, double *val, int *lv, int nw, bool &gave) {
   // loads and stores in flight within the 63 a wave can count
   constexpr int kWaveAhead = Pub ? (SL == 4 ? 6 : 4) : (SL == 4 ? 8 : 5);
   constexpr int kPubAhead = Pub ? 2 : 0;
@@ -941,8 +1017,17 @@ __device__ __forceinline__ void wave_sweep(const Prog &prog, i64 n, const char *
 #pragma unroll
           for (int e = 0; e < E; e++) {
             const int d = wave_slot(cur, e);
-            if (d < 0)
-              while (qc.b[e] == MPK_SENTINEL) qc.b[e] = load_pub(pub, -(i64)d - 1);
+            if (d < 0) {
+              unsigned polls = 0;
+              while (qc.b[e] == MPK_SENTINEL) {
+                if (watch_expired(polls)) {
+                  gave = true;
+                  qc.b[e] = MPK_QNAN;
+                  break;
+                }
+                qc.b[e] = load_pub(pub, -(i64)d - 1);
+              }
+            }
           }
           // the next sets' words were loaded before this one was
           // published, so likely before theirs were too: load them again
@@ -979,7 +1064,8 @@ __global__ void __launch_bounds__(MPK_WAVE_THREADS) k_sweep_wave(Prog prog, i64 
   __shared__ double val[MPK_BLK_ROWS + 1];
   __shared__ int lv[MPK_WAVE_LVLS + 1];
   if (stop && *stop) return;
-  wave_sweep<Prog, SL, E, false>(prog, n, plan, second, 0, st, pub, val, lv, nw);
+  bool gave = false;  // no waits without published reads
+  wave_sweep<Prog, SL, E, false>(prog, n, plan, second, 0, st, pub, val, lv, nw, gave);
 }
 
 // A wave-block plan's sweep: one workgroup per block, blocks taken in
@@ -997,7 +1083,9 @@ __global__ void __launch_bounds__(MPK_WAVE_THREADS) k_sweep_wblk(Prog prog, i64 
   if (stop && *stop) return;
   const i64 nblk = reinterpret_cast<const i64 *>(second ? plan + reinterpret_cast<const i64 *>(plan)[4] : plan)[0];
   if ((i64)tk_sh >= nblk) return;
-  wave_sweep<Prog, SL, E, true>(prog, n, plan, second, tk_sh, st, pub, val, lv, nw);
+  bool gave = false;
+  wave_sweep<Prog, SL, E, true>(prog, n, plan, second, tk_sh, st, pub, val, lv, nw, gave);
+  watch_report(gave);
 }
 
 // A wave plan's kernel form from the registry's word (slot count in bits
@@ -1014,12 +1102,13 @@ template <class Prog>
 __device__ __forceinline__ void wave_sweep_form(int form, const Prog &prog, i64 n, const char *plan, int second,
                                                 Staged *st, double *pub, double *val, int *lv) {
   const int nw = 1 << (form >> 2);
+  bool gave = false;  // no waits without published reads
   if ((form & 3) == 0)
-    wave_sweep<Prog, 4, 2, false>(prog, n, plan, second, 0, st, pub, val, lv, nw);
+    wave_sweep<Prog, 4, 2, false>(prog, n, plan, second, 0, st, pub, val, lv, nw, gave);
   else if ((form & 3) == 1)
-    wave_sweep<Prog, 4, 4, false>(prog, n, plan, second, 0, st, pub, val, lv, nw);
+    wave_sweep<Prog, 4, 4, false>(prog, n, plan, second, 0, st, pub, val, lv, nw, gave);
   else
-    wave_sweep<Prog, 8, 8, false>(prog, n, plan, second, 0, st, pub, val, lv, nw);
+    wave_sweep<Prog, 8, 8, false>(prog, n, plan, second, 0, st, pub, val, lv, nw, gave);
 }
 
 // Entries of a sweep in the rows' own order, for the host-side planner:

@@ -98,6 +98,8 @@ SIGNATURES = {
     "mpk_synchronize": ([], None),
     "mpk_set_reduction_order": ([I], I),
     "mpk_get_reduction_order": ([], I),
+    "mpk_set_sweep_wait_limit": ([I], I),
+    "mpk_sweep_abort": ([I], I),
     "mpk_small_solves": ([], I),
     "mpk_csr_transpose": ([I] * 7, I),
     "mpk_mat_mult_transpose_t": ([I] * 7, None),

@@ -95,3 +95,20 @@ def test_sell_slot_count_guard(lib):
     assert lib.mpk_sell_size_host(past.ctypes.data, 1) == -1
     two = np.array([0, 2**24, 2**25 + 2**24], dtype=np.int64)  # the second row sets the slice's width
     assert lib.mpk_sell_size_host(two.ctypes.data, 2) == -1
+
+
+def test_sweep_wait_limit_arguments(lib):
+    """The sweep waits' poll limit: a value outside 1..2^32-1 is error 4 and
+    leaves the limit as it was; a valid one returns the previous limit (the
+    default 2^24).  No device is needed."""
+    lib.mpk_clear_error()
+    old = lib.mpk_set_sweep_wait_limit(0)
+    assert lib.mpk_status() == 4
+    lib.mpk_clear_error()
+    assert lib.mpk_set_sweep_wait_limit(2**32) == old
+    assert lib.mpk_status() == 4
+    lib.mpk_clear_error()
+    assert old == 2**24
+    assert lib.mpk_set_sweep_wait_limit(5) == old
+    assert lib.mpk_set_sweep_wait_limit(old) == 5
+    assert lib.mpk_status() == 0
