@@ -242,7 +242,8 @@ def bench_mixed(lib, which: str, cpu: bool = False):
     """Mixed precision against plain FP64 on the GPU: the FP64 drop-in solve
     (time to rtol 1e-10 and its true residual), then the mixed solve to an
     FP64 relative residual of 1e-10 with exact sweeps and with Jacobi-pass
-    sweeps, each with its error against the FP64 solution."""
+    sweeps, each with its error against the FP64 solution and beside the
+    same solver run in FP64 (its control)."""
     from mojopetsc4py import PETSc
     from mpk import workloads as W
     from mpk.mixed import MixedSolver
@@ -280,7 +281,7 @@ def bench_mixed(lib, which: str, cpu: bool = False):
             out["cpu_baseline"] = cpu_sample(csr, b, "gmres", "ilu", its64, 4, levels=0, restart=30)
         else:
             out["cpu_baseline"] = cpu_sample(csr, b, "bcgs", "sor", its64, 2, omega=omega)
-    for prec, sweeps, irt in runs:
+    def run(prec, sweeps, irt):
         ms = MixedSolver(A, pc, precision=prec, inner=inner, restart=30, sweeps=sweeps)
         xm = bv.duplicate()
         ms.solve(bv, xm, rtol=1e-10, inner_rtol=irt)
@@ -290,11 +291,24 @@ def bench_mixed(lib, which: str, cpu: bool = False):
         lib.mpk_synchronize()
         t = time.perf_counter() - t0
         err = float(np.linalg.norm(xm.getArray() - x64) / np.linalg.norm(x64))
-        key = f"{prec}_{inner}_sweeps_{sweeps}"
-        out[key] = {"seconds": t, "speedup_vs_fp64": t64 / t, "outer": st["outer"], "inner": st["inner"],
-                    "converged": st["converged"], "fp64_relres": st["relres"], "true_relres": true_relres(A, bv, xm),
-                    "err_vs_fp64_solution": err, "inner_rtol": irt}
         ms.destroy()
+        return {"seconds": t, "speedup_vs_fp64": t64 / t, "outer": st["outer"], "inner": st["inner"],
+                "converged": st["converged"], "fp64_relres": st["relres"], "true_relres": true_relres(A, bv, xm),
+                "err_vs_fp64_solution": err, "inner_rtol": irt}
+
+    # the FP64 control of each preconditioner form: the same solver (inner
+    # Krylov method, sweeps, refinement, inner tolerance) on FP64 vectors and
+    # values, so that its time over a low-precision run's is the precision's
+    # own gain ("speedup_vs_fp64" compares with the plain FP64 drop-in solve)
+    controls = {}
+    for prec, sweeps, irt in runs:
+        if (sweeps, irt) not in controls:
+            controls[(sweeps, irt)] = run("fp64", sweeps, irt)
+            out[f"fp64_{inner}_sweeps_{sweeps}"] = controls[(sweeps, irt)]
+        r = run(prec, sweeps, irt)
+        r["fp64_same_solver_seconds"] = controls[(sweeps, irt)]["seconds"]
+        r["speedup_vs_fp64_same_solver"] = controls[(sweeps, irt)]["seconds"] / r["seconds"]
+        out[f"{prec}_{inner}_sweeps_{sweeps}"] = r
     return out
 
 

@@ -1,7 +1,9 @@
 // mpk_mixed.hip -- mixed-precision solves: the inner Krylov iteration, its
 // SpMV and its preconditioner run on FP32 vectors (matrix values in FP32, or in
 // FP16 with one FP32 scale per row, FP32 accumulation), inside FP64 iterative
-// refinement.  FP64 stays where the accuracy is decided: the true residual
+// refinement.  The same solver also runs with FP64 vectors and values (the
+// precision's own control: identical inner solver, preconditioner form and
+// refinement, so the time ratio of the two is the gain of the precision).  FP64 stays where the accuracy is decided: the true residual
 // b - A x, the solution update, every dot product and norm (FP32 data summed
 // in FP64), the scalars they feed and the GMRES Hessenberg / Givens least
 // squares.  This is a form beside the drop-in path; it does not reproduce the
@@ -17,6 +19,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <type_traits>
 #include <vector>
 
 #include "mpk_internal.h"
@@ -27,18 +30,36 @@
 
 namespace mpk {
 
-__device__ __forceinline__ float lp_f(float v) { return v; }
-__device__ __forceinline__ float lp_f(__half v) { return __half2float(v); }
+// Stored values (CT: __half, float or double) as the inner solve's type VT
+// (float, or double with double values), and back.
+template <class VT>
+__device__ __forceinline__ VT lp_val(float v) {
+  return (VT)v;
+}
+template <class VT>
+__device__ __forceinline__ VT lp_val(__half v) {
+  return (VT)__half2float(v);
+}
+template <class VT>
+__device__ __forceinline__ VT lp_val(double v) {
+  return (VT)v;
+}
 template <class CT>
-__device__ __forceinline__ CT lp_from(float v);
+__device__ __forceinline__ CT lp_from(double v);
 template <>
-__device__ __forceinline__ float lp_from<float>(float v) {
+__device__ __forceinline__ double lp_from<double>(double v) {
   return v;
 }
 template <>
-__device__ __forceinline__ __half lp_from<__half>(float v) {
-  return __float2half(v);
+__device__ __forceinline__ float lp_from<float>(double v) {
+  return (float)v;
 }
+template <>
+__device__ __forceinline__ __half lp_from<__half>(double v) {
+  return __float2half((float)v);
+}
+__device__ __forceinline__ float lp_fma(float a, float b, float c) { return fmaf(a, b, c); }
+__device__ __forceinline__ double lp_fma(double a, double b, double c) { return fma(a, b, c); }
 
 // ------------------------------------------------------------ matrix
 
@@ -56,17 +77,22 @@ __global__ void k_lp_sell(i64 n, const int *rp, const int *soff, const double *s
     s = m > 0.0 ? m : 1.0;
     rs[i] = (float)s;
   }
-  for (int k = 0; k < len; k++) lv[base + 64 * k] = lp_from<CT>((float)(sv[base + 64 * k] / s));
+  for (int k = 0; k < len; k++) lv[base + 64 * k] = lp_from<CT>(sv[base + 64 * k] / s);
 }
 
 // Windowed products.  The sliced matrices of the inner solve are read in
 // blocks of LP_WROWS rows.  When the columns one block reads span fewer than
-// LP_XWIN entries of x (matrices whose entries stay near the diagonal), the
+// lp_xwin entries of x (matrices whose entries stay near the diagonal), the
 // block first copies that window of x into LDS with coalesced loads and
 // gathers from there; otherwise it gathers from x directly.  The values
 // gathered, and so the results, are the same either way.
 #define LP_WROWS 512
-#define LP_XWIN 12288
+#define LP_XWIN_BYTES 49152
+// entries of x a block's window holds: 12288 FP32, 6144 FP64
+template <class VT>
+__host__ __device__ constexpr int lp_xwin() {
+  return LP_XWIN_BYTES / (int)sizeof(VT);
+}
 
 // Column window of each block: win[2b], win[2b+1] = the smallest and largest
 // column read by block b's rows (0, -1 when they read none).
@@ -101,11 +127,12 @@ __global__ void __launch_bounds__(LP_WROWS) k_col_window(i64 n, const int *rp, c
 
 // Copies block b's window of x into xs when it fits; returns whether it did
 // (every thread of the block calls this).
-__device__ __forceinline__ bool stage_window(const float *__restrict__ x, const int *__restrict__ win, unsigned b,
-                                             float *xs, int &lo) {
+template <class VT>
+__device__ __forceinline__ bool stage_window(const VT *__restrict__ x, const int *__restrict__ win, unsigned b,
+                                             VT *xs, int &lo) {
   lo = win[2 * b];
   const int hi = win[2 * b + 1];
-  const bool fit = hi - lo < LP_XWIN;
+  const bool fit = hi - lo < lp_xwin<VT>();
   if (fit)
     for (int c = lo + (int)threadIdx.x; c <= hi; c += LP_WROWS) xs[c - lo] = x[c];
   __syncthreads();
@@ -114,101 +141,108 @@ __device__ __forceinline__ bool stage_window(const float *__restrict__ x, const 
 
 // Row i's serial sum over its sliced entries, four entries' loads issued
 // before their values are gathered (from xs when the window is staged).
-template <class CT>
-__device__ __forceinline__ float sliced_row(const int *__restrict__ sc, const CT *__restrict__ sv, int base, int len,
-                                            const float *__restrict__ x, const float *xs, bool fit, int lo) {
-  float s = 0.0f;
+template <class CT, class VT>
+__device__ __forceinline__ VT sliced_row(const int *__restrict__ sc, const CT *__restrict__ sv, int base, int len,
+                                         const VT *__restrict__ x, const VT *xs, bool fit, int lo) {
+  VT s = VT(0);
   for (int k0 = 0; k0 < len; k0 += 4) {
     int c[4];
-    float v[4];
+    VT v[4];
 #pragma unroll
     for (int e = 0; e < 4; e++) {
       const int p = base + 64 * (k0 + e);
       c[e] = k0 + e < len ? sc[p] : 0;
-      v[e] = k0 + e < len ? lp_f(sv[p]) : 0.0f;
+      v[e] = k0 + e < len ? lp_val<VT>(sv[p]) : VT(0);
     }
 #pragma unroll
     for (int e = 0; e < 4; e++)
-      if (k0 + e < len) s = fmaf(v[e], fit ? xs[c[e] - lo] : x[c[e]], s);
+      if (k0 + e < len) s = lp_fma(v[e], fit ? xs[c[e] - lo] : x[c[e]], s);
   }
   return s;
 }
 
 // WIN: the kernel stages windows (matrices whose blocks' windows fit LDS);
 // else it gathers from x with no LDS at all.
-template <class CT, bool WIN>
+template <class CT, class VT, bool WIN>
 __global__ void __launch_bounds__(LP_WROWS) k_spmv_lp(i64 n, const int *__restrict__ rp, const int *__restrict__ soff,
                                                       const int *__restrict__ sc, const CT *__restrict__ sv,
                                                       const float *__restrict__ rs, const int *__restrict__ win,
-                                                      const float *__restrict__ x, float *__restrict__ y) {
-  __shared__ float xs[WIN ? LP_XWIN : 1];
+                                                      const VT *__restrict__ x, VT *__restrict__ y) {
+  __shared__ VT xs[WIN ? lp_xwin<VT>() : 1];
   const unsigned b = xcd_block(blockIdx.x, gridDim.x);
   int lo = 0;
   const bool fit = WIN ? stage_window(x, win, b, xs, lo) : false;
   const i64 i = (i64)b * LP_WROWS + threadIdx.x;
   if (i >= n) return;
-  const float s = sliced_row(sc, sv, soff[i >> 6] + (int)(i & 63), rp[i + 1] - rp[i], x, xs, fit, lo);
-  y[i] = rs ? s * rs[i] : s;
+  const VT s = sliced_row<CT, VT>(sc, sv, soff[i >> 6] + (int)(i & 63), rp[i + 1] - rp[i], x, xs, fit, lo);
+  y[i] = rs ? s * (VT)rs[i] : s;
 }
 
-// ------------------------------------------------------------ FP32 vectors
+// ------------------------------------------------------------ inner vectors (VT)
 
 static unsigned lp_grid(i64 n) {
   const i64 g = (n + 255) / 256;
   return (unsigned)(g < 1 ? 1 : g);
 }
 
-// out = (float)(r * scale[0])
-__global__ void k_lp_from_d(i64 n, const double *r, const double *scale, float *out) {
+// out = (VT)(r * scale[0])
+template <class VT>
+__global__ void k_lp_from_d(i64 n, const double *r, const double *scale, VT *out) {
   const i64 i = (i64)blockIdx.x * 256 + threadIdx.x;
-  if (i < n) out[i] = (float)(r[i] * scale[0]);
+  if (i < n) out[i] = (VT)(r[i] * scale[0]);
 }
 // x += scale[0] * d (FP64)
-__global__ void k_lp_update(i64 n, double *x, const double *scale, const float *d) {
+template <class VT>
+__global__ void k_lp_update(i64 n, double *x, const double *scale, const VT *d) {
   const i64 i = (i64)blockIdx.x * 256 + threadIdx.x;
   if (i < n) x[i] = fma(scale[0], (double)d[i], x[i]);
 }
 // y = x * s[0]
-__global__ void k_lp_scale(i64 n, const float *x, const double *s, float *y, const int *skip) {
+template <class VT>
+__global__ void k_lp_scale(i64 n, const VT *x, const double *s, VT *y, const int *skip) {
   if (skip && *skip) return;
   const i64 i = (i64)blockIdx.x * 256 + threadIdx.x;
-  if (i < n) y[i] = x[i] * (float)s[0];
+  if (i < n) y[i] = x[i] * (VT)s[0];
 }
 // y += sum_j coef[j] V_j, j < nv (V with leading dimension ld)
-__global__ void k_lp_maxpy(i64 n, float *y, const float *V, i64 ld, const double *coef, int nv, const int *skip) {
+template <class VT>
+__global__ void k_lp_maxpy(i64 n, VT *y, const VT *V, i64 ld, const double *coef, int nv, const int *skip) {
   if (skip && *skip) return;
   const i64 i = (i64)blockIdx.x * 256 + threadIdx.x;
   if (i >= n) return;
-  float a = y[i];
-  for (int j = 0; j < nv; j++) a = fmaf((float)coef[j], V[(i64)j * ld + i], a);
+  VT a = y[i];
+  for (int j = 0; j < nv; j++) a = lp_fma((VT)coef[j], V[(i64)j * ld + i], a);
   y[i] = a;
 }
-__global__ void k_lp_zero(i64 n, float *y) {
+template <class VT>
+__global__ void k_lp_zero(i64 n, VT *y) {
   const i64 i = (i64)blockIdx.x * 256 + threadIdx.x;
-  if (i < n) y[i] = 0.0f;
+  if (i < n) y[i] = VT(0);
 }
 // z = x - z
-__global__ void k_lp_rsub(i64 n, const float *x, float *z) {
+template <class VT>
+__global__ void k_lp_rsub(i64 n, const VT *x, VT *z) {
   const i64 i = (i64)blockIdx.x * 256 + threadIdx.x;
   if (i < n) z[i] = x[i] - z[i];
 }
-__global__ void k_lp_copy(i64 n, const float *x, float *y) {
+template <class VT>
+__global__ void k_lp_copy(i64 n, const VT *x, VT *y) {
   const i64 i = (i64)blockIdx.x * 256 + threadIdx.x;
   if (i < n) y[i] = x[i];
 }
 
 // ------------------------------------------------------------ reductions
-// FP32 data summed in FP64: per-block partials, then one block per result.
+// VT data summed in FP64: per-block partials, then one block per result.
 #define LP_RBLK 512
 
 // P[k * LP_RBLK + b] = sum over block b's share of x_k . y_k (y_k null: x_k . x_k)
-template <int K>
+template <int K, class VT>
 struct LpDots {
-  const float *x[K];
-  const float *y[K];
+  const VT *x[K];
+  const VT *y[K];
 };
-template <int K>
-__global__ void __launch_bounds__(256) k_lp_dots(LpDots<K> d, i64 n, double *P) {
+template <int K, class VT>
+__global__ void __launch_bounds__(256) k_lp_dots(LpDots<K, VT> d, i64 n, double *P) {
   __shared__ double sh[4];
   double a[K];
 #pragma unroll
@@ -227,7 +261,8 @@ __global__ void __launch_bounds__(256) k_lp_dots(LpDots<K> d, i64 n, double *P) 
   }
 }
 // P[j * LP_RBLK + b] = block b's share of V_j . w, j < nv
-__global__ void __launch_bounds__(256) k_lp_mdot(const float *V, i64 ld, int nv, const float *w, i64 n, double *P,
+template <class VT>
+__global__ void __launch_bounds__(256) k_lp_mdot(const VT *V, i64 ld, int nv, const VT *w, i64 n, double *P,
                                                  const int *skip) {
   if (skip && *skip) return;
   __shared__ double sh[4];
@@ -262,82 +297,111 @@ __global__ void __launch_bounds__(256) k_lp_fold(const double *P, int nb, double
 
 // Sentinel-fills a sweep's published values and zeroes its block ticket and
 // ring progress word.
-__global__ void k_lp_prepare(i64 n, float *pub, unsigned *ticket) {
+__device__ __forceinline__ float lp_sentinel(float) { return __uint_as_float(MPK_SENTINEL_F); }
+__device__ __forceinline__ double lp_sentinel(double) { return __longlong_as_double((long long)MPK_SENTINEL); }
+template <class VT>
+__global__ void k_lp_prepare(i64 n, VT *pub, unsigned *ticket) {
   const i64 i = (i64)blockIdx.x * 256 + threadIdx.x;
   if (i == 0) {
     ticket[0] = 0u;
     ticket[1] = 0u;
   }
-  if (i < n) pub[i] = __uint_as_float(MPK_SENTINEL_F);
+  if (i < n) pub[i] = lp_sentinel(VT(0));
 }
 
+template <class VT>
 struct alignas(8) LpStaged {
-  float init, s, d1, d2;
+  VT init, s, d1, d2;
   i64 o;
 };
-// The low-precision row programs add their sum to the staged start in fin.
-__device__ __forceinline__ float row_start(const LpStaged &) { return 0.0f; }
+// The row programs of the inner solve add their sum to the staged start in fin.
+template <class VT>
+__device__ __forceinline__ VT row_start(const LpStaged<VT> &) {
+  return VT(0);
+}
 
-// Plan coefficients from the solver's low-precision copy (slot-major like the
-// plan's own, divided by the row scale where there is one).
-template <class CT>
+// The sweep engine's value type for the inner solve's vectors.
+template <class VT>
+struct LpSweep;
+template <>
+struct LpSweep<float> {
+  using T = SweepF32;
+};
+template <>
+struct LpSweep<double> {
+  using T = SweepF64;
+};
+
+// Plan coefficients from the solver's copy (slot-major like the plan's own,
+// divided by the row scale where there is one).
+template <class CT, class VT>
 struct LpCoef {
   const CT *c;
   static constexpr int kBytes = sizeof(CT);
   __device__ static LpCoef make(const PlanView &, const void *lc) { return LpCoef{(const CT *)lc}; }
-  __device__ float operator()(i64 q) const { return lp_f(c[q]); }
+  __device__ VT operator()(i64 q) const { return lp_val<VT>(c[q]); }
 };
 
-// The row programs of the low-precision sweeps: acc = sum of coef * value in
-// FP32 (coefficients divided by the row scale where there is one), then fin.
+// The row programs of the inner solve's sweeps: acc = sum of coef * value in
+// VT (coefficients divided by the row scale where there is one), then fin.
+template <class VT>
 struct LpIluFwd {
   const int *r;
-  const float *b;
-  __device__ LpStaged stage(i64 i) const { return LpStaged{b[r[i]], 1.0f, 0.0f, 0.0f, 0}; }
-  __device__ float fin(i64, float acc, const LpStaged &s) const { return s.init + acc; }
-  __device__ float stale(i64) const { return 0.0f; }
+  const VT *b;
+  __device__ LpStaged<VT> stage(i64 i) const { return LpStaged<VT>{b[r[i]], VT(1), VT(0), VT(0), 0}; }
+  __device__ VT fin(i64, VT acc, const LpStaged<VT> &s) const { return s.init + acc; }
+  __device__ VT stale(i64) const { return VT(0); }
 };
+template <class VT>
 struct LpIluBwd {
   const int *c;
-  const float *F, *dinv;
-  float *x;
-  __device__ LpStaged stage(i64 i) const { return LpStaged{F[i], 1.0f, dinv[i], 0.0f, c[i]}; }
-  __device__ float fin(i64, float acc, const LpStaged &s) const {
-    const float v = (s.init + acc) * s.d1;
+  const VT *F, *dinv;
+  VT *x;
+  __device__ LpStaged<VT> stage(i64 i) const { return LpStaged<VT>{F[i], VT(1), dinv[i], VT(0), c[i]}; }
+  __device__ VT fin(i64, VT acc, const LpStaged<VT> &s) const {
+    const VT v = (s.init + acc) * s.d1;
     x[s.o] = v;
     return v;
   }
-  __device__ float stale(i64) const { return 0.0f; }
+  __device__ VT stale(i64) const { return VT(0); }
 };
+template <class VT>
 struct LpSorFwd {
-  const float *x, *rs, *idiag;
-  float *t, *yout;
-  const float *yin;
-  __device__ LpStaged stage(i64 i) const { return LpStaged{x[i], rs ? rs[i] : 1.0f, idiag[i], 0.0f, 0}; }
-  __device__ float fin(i64 i, float acc, const LpStaged &s) const {
-    const float tv = fmaf(s.s, acc, s.init);
+  const VT *x;
+  const float *rs;
+  const VT *idiag;
+  VT *t, *yout;
+  const VT *yin;
+  __device__ LpStaged<VT> stage(i64 i) const {
+    return LpStaged<VT>{x[i], rs ? (VT)rs[i] : VT(1), idiag[i], VT(0), 0};
+  }
+  __device__ VT fin(i64 i, VT acc, const LpStaged<VT> &s) const {
+    const VT tv = lp_fma(s.s, acc, s.init);
     t[i] = tv;
-    const float v = tv * s.d1;
+    const VT v = tv * s.d1;
     if (yout) yout[i] = v;
     return v;
   }
-  __device__ float stale(i64 k) const { return yin[k]; }
+  __device__ VT stale(i64 k) const { return yin[k]; }
 };
+template <class VT>
 struct LpSorBwd {
-  const float *src, *rs, *idiag, *yf;
-  float *y;
-  float omega;
+  const VT *src;
+  const float *rs;
+  const VT *idiag, *yf;
+  VT *y;
+  VT omega;
   int use_t;
-  __device__ LpStaged stage(i64 i) const {
-    return LpStaged{src[i], rs ? rs[i] : 1.0f, idiag[i], use_t ? yf[i] : 0.0f, 0};
+  __device__ LpStaged<VT> stage(i64 i) const {
+    return LpStaged<VT>{src[i], rs ? (VT)rs[i] : VT(1), idiag[i], use_t ? yf[i] : VT(0), 0};
   }
-  __device__ float fin(i64 i, float acc, const LpStaged &s) const {
-    const float tv = fmaf(s.s, acc, s.init);
-    const float v = use_t ? fmaf(1.0f - omega, s.d2, tv * s.d1) : tv * s.d1;
+  __device__ VT fin(i64 i, VT acc, const LpStaged<VT> &s) const {
+    const VT tv = lp_fma(s.s, acc, s.init);
+    const VT v = use_t ? lp_fma(VT(1) - omega, s.d2, tv * s.d1) : tv * s.d1;
     y[i] = v;
     return v;
   }
-  __device__ float stale(i64 k) const { return y[k]; }
+  __device__ VT stale(i64 k) const { return y[k]; }
 };
 
 // Approximate sweeps: k Jacobi passes over a sweep's entries (a truncated
@@ -345,20 +409,21 @@ struct LpSorBwd {
 // 1 (u_in null) keeps only the row's own term; pass j adds the entries' values
 // from pass j - 1.  Every pass is one bandwidth-bound sweep over the sliced
 // entries, with no ordering between rows.
-template <class Prog, class CT, bool WIN>
+template <class Prog, class CT, class VT, bool WIN>
 __global__ void __launch_bounds__(LP_WROWS) k_tri_jacobi(Prog prog, i64 n, const int *__restrict__ rp,
                                                          const int *__restrict__ soff, const int *__restrict__ sc,
                                                          const CT *__restrict__ sv, const int *__restrict__ win,
-                                                         const float *__restrict__ u_in, float *__restrict__ u_out) {
-  __shared__ float xs[WIN ? LP_XWIN : 1];
+                                                         const VT *__restrict__ u_in, VT *__restrict__ u_out) {
+  __shared__ VT xs[WIN ? lp_xwin<VT>() : 1];
   const unsigned b = xcd_block(blockIdx.x, gridDim.x);
   int lo = 0;
   const bool fit = WIN && u_in ? stage_window(u_in, win, b, xs, lo) : false;  // uniform over the block
   const i64 i = (i64)b * LP_WROWS + threadIdx.x;
   if (i >= n) return;
-  const LpStaged st = prog.stage(i);
-  const float acc =
-      u_in ? sliced_row(sc, sv, soff[i >> 6] + (int)(i & 63), rp[i + 1] - rp[i], u_in, xs, fit, lo) : 0.0f;
+  const LpStaged<VT> st = prog.stage(i);
+  const VT acc = u_in ? sliced_row<CT, VT>(sc, sv, soff[i >> 6] + (int)(i & 63), rp[i + 1] - rp[i], u_in, xs, fit,
+                                           lo)
+                      : VT(0);
   u_out[i] = prog.fin(i, acc, st);
 }
 __global__ void k_f2h(i64 m, const float *x, __half *y) {
@@ -374,19 +439,20 @@ struct TriSell {
 };
 
 // Column windows of a sliced matrix into win [D]; returns whether the
-// windowed kernels pay: at least 7 in 8 blocks' windows fit LDS.
-static bool col_windows(i64 n, const int *rp, const int *soff, const int *sc, int *win) {
+// windowed kernels pay: at least 7 in 8 blocks' windows fit LDS (xwin
+// entries of x).
+static bool col_windows(i64 n, const int *rp, const int *soff, const int *sc, int *win, int xwin) {
   const unsigned nb = nblocks(n, LP_WROWS);
   MPK_LAUNCH(k_col_window, nb, LP_WROWS, n, rp, soff, sc, win);
   std::vector<int> h(2 * (size_t)nb);
   copy_dtoh(h.data(), win, h.size() * sizeof(int));
   if (failed()) return false;
   i64 fit = 0;
-  for (unsigned b = 0; b < nb; b++) fit += h[2 * b + 1] - h[2 * b] < LP_XWIN;
+  for (unsigned b = 0; b < nb; b++) fit += h[2 * b + 1] - h[2 * b] < xwin;
   return 8 * fit >= 7 * (i64)nb;
 }
 
-// Low-precision coefficients of one sweep's plan: lc[slot] = coef / row scale.
+// The inner solve's coefficients of one sweep's plan: lc[slot] = coef / row scale.
 template <class CT>
 __global__ void k_lp_plan_coef(i64 n, const char *plan, int second, const float *rs, CT *lc) {
   if (second) plan += reinterpret_cast<const i64 *>(plan)[4];
@@ -397,21 +463,22 @@ __global__ void k_lp_plan_coef(i64 n, const char *plan, int second, const float 
   const i64 base = P.w ? p : P.pent[p], stride = P.w ? n : 1;
   for (int j = 0; j < P.pcnt[p]; j++) {
     const i64 q = base + (i64)j * stride;
-    lc[q] = lp_from<CT>((float)(P.ecoef[q] / s));
+    lc[q] = lp_from<CT>(P.ecoef[q] / s);
   }
 }
 
-// FP32 inverted diagonals: ILU's stored 1/u_ii; SOR's as MatInvertDiagonalForSOR.
-__global__ void k_lp_ilu_dinv(i64 n, const double *ba, const int *bdiag, float *dinv) {
+// Inverted diagonals in VT: ILU's stored 1/u_ii; SOR's as MatInvertDiagonalForSOR.
+template <class VT>
+__global__ void k_lp_ilu_dinv(i64 n, const double *ba, const int *bdiag, VT *dinv) {
   const i64 i = (i64)blockIdx.x * 256 + threadIdx.x;
-  if (i < n) dinv[i] = (float)ba[bdiag[i]];
+  if (i < n) dinv[i] = (VT)ba[bdiag[i]];
 }
-__global__ void k_lp_sor_idiag(i64 n, const double *a, const int *dmark, double omega, double fshift,
-                               float *idiag) {
+template <class VT>
+__global__ void k_lp_sor_idiag(i64 n, const double *a, const int *dmark, double omega, double fshift, VT *idiag) {
   const i64 i = (i64)blockIdx.x * 256 + threadIdx.x;
   if (i >= n) return;
   const double d = a[dmark[i]];
-  idiag[i] = (float)((omega == 1.0 && fshift <= 0.0) ? 1.0 / d : omega / (fshift + d));
+  idiag[i] = (VT)((omega == 1.0 && fshift <= 0.0) ? 1.0 / d : omega / (fshift + d));
 }
 
 static i64 plan_slots(const char *plan_d, int second, i64 n) {
@@ -529,44 +596,56 @@ __global__ void k_bc_omega(LpState *s) {  // red[0] = t . s, red[1] = t . t
 __global__ void k_bc_res(LpState *s) { s->res = sqrt(s->red[0]); }
 
 // p = r + c0 p + c1 v
-__global__ void k_bc_p(i64 n, float *p, const float *r, const float *v, const double *c) {
+template <class VT>
+__global__ void k_bc_p(i64 n, VT *p, const VT *r, const VT *v, const double *c) {
   const i64 i = (i64)blockIdx.x * 256 + threadIdx.x;
-  if (i < n) p[i] = fmaf((float)c[1], v[i], fmaf((float)c[0], p[i], r[i]));
+  if (i < n) p[i] = lp_fma((VT)c[1], v[i], lp_fma((VT)c[0], p[i], r[i]));
 }
 // y = x + c v
-__global__ void k_lp_waxpy(i64 n, float *y, const float *x, const float *v, const double *c) {
+template <class VT>
+__global__ void k_lp_waxpy(i64 n, VT *y, const VT *x, const VT *v, const double *c) {
   const i64 i = (i64)blockIdx.x * 256 + threadIdx.x;
-  if (i < n) y[i] = fmaf((float)c[0], v[i], x[i]);
+  if (i < n) y[i] = lp_fma((VT)c[0], v[i], x[i]);
 }
 // d += c0 a + c1 b
-__global__ void k_lp_axpby_add(i64 n, float *d, const float *a, const float *b, const double *c) {
+template <class VT>
+__global__ void k_lp_axpby_add(i64 n, VT *d, const VT *a, const VT *b, const double *c) {
   const i64 i = (i64)blockIdx.x * 256 + threadIdx.x;
-  if (i < n) d[i] = fmaf((float)c[1], b[i], fmaf((float)c[0], a[i], d[i]));
+  if (i < n) d[i] = lp_fma((VT)c[1], b[i], lp_fma((VT)c[0], a[i], d[i]));
 }
 
-struct MixedSolver {
+// The solver behind a handle: the refinement loop in FP64 around the inner
+// solve in VT (float: FP32 or FP16 values; double: FP64 values).
+struct MixedBase {
+  virtual ~MixedBase() {}
+  virtual int solve(const double *B, double *X, double *cfg) = 0;
+};
+
+template <class VT>
+struct MixedSolver : MixedBase {
+  using Tr = typename LpSweep<VT>::T;
   i64 n = 0;
   int prec = 32, inner = LP_GMRES, m = 30, refine = 1;
   DevCSR A64{};
   const int *soff = nullptr, *sc = nullptr;
-  void *lv = nullptr;  // low-precision matrix values (float or __half)
-  float *rs = nullptr;
+  void *lv = nullptr;  // the inner solve's matrix values (__half, float or double)
+  float *rs = nullptr;  // FP16: one scale per row
   PCView pc;
-  void *lc0 = nullptr, *lc1 = nullptr;  // low-precision sweep coefficients
-  float *dinv = nullptr, *idiag = nullptr;
+  void *lc0 = nullptr, *lc1 = nullptr;  // the inner solve's sweep coefficients
+  VT *dinv = nullptr, *idiag = nullptr;
   int tri_iters = 0;  // > 0: approximate sweeps by this many Jacobi passes
   TriSell tri[2];
-  float *work = nullptr;  // FP32 vectors
+  VT *work = nullptr;     // the inner solve's vectors
   double *r64 = nullptr;  // FP64 residual
   double *P = nullptr;    // reduction partials
   LpState *st = nullptr;
   double *hst = nullptr;  // pinned: res, breakdown
   unsigned *ticket = nullptr;
-  LpStaged *lst = nullptr;  // staged rows of the planned sweeps
-  int *winA = nullptr;      // column windows of the low-precision matrix
-  bool winA_on = false;     // products stage them (col_windows)
+  LpStaged<VT> *lst = nullptr;  // staged rows of the planned sweeps
+  int *winA = nullptr;          // column windows of the inner solve's matrix
+  bool winA_on = false;         // products stage them (col_windows)
 
-  ~MixedSolver() {
+  ~MixedSolver() override {
     void *bufs[] = {lv,          rs,          lc0,         lc1,       dinv,      idiag,    work,
                     r64,         P,           st,          ticket,    lst,       winA,     tri[0].rp,
                     tri[0].soff, tri[0].sc,   tri[0].sv,   tri[0].win, tri[1].rp, tri[1].soff, tri[1].sc,
@@ -577,75 +656,86 @@ struct MixedSolver {
     if (hst) hipHostFree(hst);
   }
 
-  template <class CT>
-  void spmv_t(const float *x, float *y) {
-    if (winA_on)
-      MPK_LAUNCH((k_spmv_lp<CT, true>), nblocks(n, LP_WROWS), LP_WROWS, n, A64.rp, soff, sc, (const CT *)lv,
-                 (const float *)rs, (const int *)winA, x, y);
-    else
-      MPK_LAUNCH((k_spmv_lp<CT, false>), nblocks(n, LP_WROWS), LP_WROWS, n, A64.rp, soff, sc, (const CT *)lv,
-                 (const float *)rs, (const int *)winA, x, y);
+  // Calls f with a null pointer of the stored value type (CT) of this
+  // precision: __half (16), float (32) or double (64).
+  template <class F>
+  void with_ct(F &&f) const {
+    if constexpr (std::is_same<VT, double>::value) {
+      f((const double *)nullptr);
+    } else {
+      if (prec == 16)
+        f((const __half *)nullptr);
+      else
+        f((const float *)nullptr);
+    }
   }
-  void spmv(const float *x, float *y) {
-    if (prec == 16)
-      spmv_t<__half>(x, y);
-    else
-      spmv_t<float>(x, y);
-  }
+  i64 esz() const { return prec == 16 ? 2 : prec == 32 ? 4 : 8; }
 
-  template <class Prog, class CT>
-  void tri_pass(const Prog &prog, const TriSell &T, const float *in, float *out) {
-    if (T.windowed)
-      MPK_LAUNCH((k_tri_jacobi<Prog, CT, true>), nblocks(n, LP_WROWS), LP_WROWS, prog, n, (const int *)T.rp,
-                 (const int *)T.soff, (const int *)T.sc, (const CT *)T.sv, (const int *)T.win, in, out);
-    else
-      MPK_LAUNCH((k_tri_jacobi<Prog, CT, false>), nblocks(n, LP_WROWS), LP_WROWS, prog, n, (const int *)T.rp,
-                 (const int *)T.soff, (const int *)T.sc, (const CT *)T.sv, (const int *)T.win, in, out);
+  void spmv(const VT *x, VT *y) {
+    with_ct([&](auto tag) {
+      using CT = std::remove_cv_t<std::remove_pointer_t<decltype(tag)>>;
+      if (winA_on)
+        MPK_LAUNCH((k_spmv_lp<CT, VT, true>), nblocks(n, LP_WROWS), LP_WROWS, n, A64.rp, soff, sc, (const CT *)lv,
+                   (const float *)rs, (const int *)winA, x, y);
+      else
+        MPK_LAUNCH((k_spmv_lp<CT, VT, false>), nblocks(n, LP_WROWS), LP_WROWS, n, A64.rp, soff, sc, (const CT *)lv,
+                   (const float *)rs, (const int *)winA, x, y);
+    });
   }
 
   template <class Prog>
-  void sweep(const Prog &prog, int second, float *pub) {
+  void tri_pass(const Prog &prog, const TriSell &T, const VT *in, VT *out) {
+    with_ct([&](auto tag) {
+      using CT = std::remove_cv_t<std::remove_pointer_t<decltype(tag)>>;
+      if (T.windowed)
+        MPK_LAUNCH((k_tri_jacobi<Prog, CT, VT, true>), nblocks(n, LP_WROWS), LP_WROWS, prog, n, (const int *)T.rp,
+                   (const int *)T.soff, (const int *)T.sc, (const CT *)T.sv, (const int *)T.win, in, out);
+      else
+        MPK_LAUNCH((k_tri_jacobi<Prog, CT, VT, false>), nblocks(n, LP_WROWS), LP_WROWS, prog, n, (const int *)T.rp,
+                   (const int *)T.soff, (const int *)T.sc, (const CT *)T.sv, (const int *)T.win, in, out);
+    });
+  }
+
+  template <class Prog>
+  void sweep(const Prog &prog, int second, VT *pub) {
     if (tri_iters > 0) {
       // Jacobi passes, ping-ponging through the 4th PC vector so that the last
       // pass lands in pub
-      float *tmp = pc_scratch() + 3 * n;
+      VT *tmp = pc_scratch() + 3 * n;
       const TriSell &T = tri[second];
       for (int j = 0; j < tri_iters; j++) {
-        float *out = ((tri_iters - 1 - j) % 2 == 0) ? pub : tmp;
-        const float *in = j == 0 ? nullptr : (((tri_iters - j) % 2 == 0) ? pub : tmp);
-        if (prec == 16)
-          tri_pass<Prog, __half>(prog, T, in, out);
-        else
-          tri_pass<Prog, float>(prog, T, in, out);
+        VT *out = ((tri_iters - 1 - j) % 2 == 0) ? pub : tmp;
+        const VT *in = j == 0 ? nullptr : (((tri_iters - j) % 2 == 0) ? pub : tmp);
+        tri_pass(prog, T, in, out);
       }
       return;
     }
-    // the planned sweep on the FP32 engine: stage, then the block or ring plan
+    // the planned sweep on the block engine: stage, then the block or ring plan
     watch_install();
-    MPK_LAUNCH(k_lp_prepare, lp_grid(n), 256, n, pub, ticket);
+    MPK_LAUNCH(k_lp_prepare<VT>, lp_grid(n), 256, n, pub, ticket);
     const void *lc = second ? lc1 : lc0;
-    MPK_LAUNCH((k_sweep_stage<Prog, LpStaged>), lp_grid(n), 256, prog, n, pc_plan(), second, lst,
+    MPK_LAUNCH((k_sweep_stage<Prog, LpStaged<VT>>), lp_grid(n), 256, prog, n, pc_plan(), second, lst,
                (const int64_t *)nullptr);
-    if (prec == 16)
-      MPK_LAUNCH((k_sweep_blk<Prog, SweepF32, LpCoef<__half>, LpStaged>), nblocks(n, 1024), 256, prog, n, pc_plan(),
-                 second, (const LpStaged *)lst, lc, ticket, pub, (const int64_t *)nullptr);
-    else
-      MPK_LAUNCH((k_sweep_blk<Prog, SweepF32, LpCoef<float>, LpStaged>), nblocks(n, 1024), 256, prog, n, pc_plan(),
-                 second, (const LpStaged *)lst, lc, ticket, pub, (const int64_t *)nullptr);
+    with_ct([&](auto tag) {
+      using CT = std::remove_cv_t<std::remove_pointer_t<decltype(tag)>>;
+      MPK_LAUNCH((k_sweep_blk<Prog, Tr, LpCoef<CT, VT>, LpStaged<VT>>), nblocks(n, 1024), 256, prog, n, pc_plan(),
+                 second, (const LpStaged<VT> *)lst, lc, ticket, pub, (const int64_t *)nullptr);
+    });
   }
   const char *pc_plan() const {
     return pc.type == PCT_ILU ? pc.ilu.plan : pc.type == PCT_SOR ? pc.sor.plan : nullptr;
   }
-  // FP32 vectors: the inner solver's (GMRES: V[m+1], w, z, d; BiCGStab: r,
-  // rhat, p, v, s, t, phat, shat, d -- d last), then 4 for the
+  // The inner solve's vectors: the inner solver's (GMRES: V[m+1], w, z, d;
+  // BiCGStab: r, rhat, p, v, s, t, phat, shat, d -- d last), then 4 for the
   // preconditioner's sweeps, then the right-hand side
   i64 nsolver() const { return inner == LP_GMRES ? m + 4 : 9; }
-  float *d_vec() { return work + (nsolver() - 1) * n; }
-  float *pc_scratch() { return work + nsolver() * n; }
-  float *rhs_vec() { return work + (nsolver() + 4) * n; }
+  VT *d_vec() { return work + (nsolver() - 1) * n; }
+  VT *pc_scratch() { return work + nsolver() * n; }
+  VT *rhs_vec() { return work + (nsolver() + 4) * n; }
 
-  // The sweeps' entries as sliced matrices in low precision (coefficients
-  // divided by the row scale where the matrix has one: SOR at FP16).
+  // The sweeps' entries as sliced matrices in the inner solve's value type
+  // (coefficients divided by the row scale where the matrix has one: SOR at
+  // FP16).
   bool build_tri() {
     std::vector<float> hrs;
     if (pc.type == PCT_SOR && rs) {
@@ -670,65 +760,70 @@ struct MixedSolver {
       so[ns] = (int)tot;
       const i64 slots = tot > 0 ? tot : 1;
       std::vector<int> sc(slots, 0);
-      std::vector<float> sv(slots, 0.0f);
+      std::vector<double> sv(slots, 0.0);
       for (i64 i = 0; i < n; i++) {
         const double sdiv = hrs.empty() ? 1.0 : (double)hrs[i];
         const i64 base = so[i >> 6] + (i & 63);
         for (i64 e = off[i]; e < off[i + 1]; e++) {
           sc[base + 64 * (e - off[i])] = col[e];
-          sv[base + 64 * (e - off[i])] = (float)(coef[e] / sdiv);
+          sv[base + 64 * (e - off[i])] = coef[e] / sdiv;
         }
       }
       TriSell &T = tri[w];
       HIPCHK(hipMalloc(&T.rp, (n + 1) * sizeof(int)));
       HIPCHK(hipMalloc(&T.soff, (ns + 1) * sizeof(int)));
       HIPCHK(hipMalloc(&T.sc, slots * sizeof(int)));
-      HIPCHK(hipMalloc(&T.sv, slots * (prec == 16 ? 2 : 4)));
+      HIPCHK(hipMalloc(&T.sv, slots * esz()));
       if (failed()) return false;
       copy_htod(T.rp, rp.data(), (n + 1) * sizeof(int));
       copy_htod(T.soff, so.data(), (ns + 1) * sizeof(int));
       copy_htod(T.sc, sc.data(), slots * sizeof(int));
       HIPCHK(hipMalloc(&T.win, 2 * (i64)nblocks(n, LP_WROWS) * sizeof(int)));
-      T.windowed = col_windows(n, T.rp, T.soff, T.sc, T.win);
-      if (prec == 16) {
-        float *tmp = nullptr;
-        HIPCHK(hipMalloc(&tmp, slots * sizeof(float)));
-        copy_htod(tmp, sv.data(), slots * sizeof(float));
-        MPK_LAUNCH(k_f2h, lp_grid(slots), 256, slots, (const float *)tmp, (__half *)T.sv);
-        HIPCHK(hipStreamSynchronize(stream()));
-        HIPCHK(hipFree(tmp));
+      T.windowed = col_windows(n, T.rp, T.soff, T.sc, T.win, lp_xwin<VT>());
+      if (prec == 64) {
+        copy_htod(T.sv, sv.data(), slots * sizeof(double));
       } else {
-        copy_htod(T.sv, sv.data(), slots * sizeof(float));
+        std::vector<float> sf(sv.begin(), sv.end());  // rounded to FP32
+        if (prec == 16) {
+          float *tmp = nullptr;
+          HIPCHK(hipMalloc(&tmp, slots * sizeof(float)));
+          copy_htod(tmp, sf.data(), slots * sizeof(float));
+          MPK_LAUNCH(k_f2h, lp_grid(slots), 256, slots, (const float *)tmp, (__half *)T.sv);
+          HIPCHK(hipStreamSynchronize(stream()));
+          HIPCHK(hipFree(tmp));
+        } else {
+          copy_htod(T.sv, sf.data(), slots * sizeof(float));
+        }
       }
     }
     return !failed();
   }
 
-  // y = M^-1 x in FP32
-  void apply_pc(const float *x, float *y) {
-    float *S = pc_scratch();
+  // y = M^-1 x in VT
+  void apply_pc(const VT *x, VT *y) {
+    VT *S = pc_scratch();
     if (pc.type == PCT_ILU) {
-      float *F = S, *B = S + n;
-      sweep(LpIluFwd{pc.ilu.r, x}, 0, F);
-      sweep(LpIluBwd{pc.ilu.c, F, dinv, y}, 1, B);
+      VT *F = S, *B = S + n;
+      sweep(LpIluFwd<VT>{pc.ilu.r, x}, 0, F);
+      sweep(LpIluBwd<VT>{pc.ilu.c, F, dinv, y}, 1, B);
     } else if (pc.type == PCT_SOR) {
-      float *YF = S, *YB = S + n, *T = S + 2 * n;
+      VT *YF = S, *YB = S + n, *T = S + 2 * n;
       const int flag = pc.sor.flag;
       const bool fwd = (flag & (1 | 4)) != 0, bwd = (flag & (2 | 8)) != 0;
-      if (fwd) sweep(LpSorFwd{x, rs, idiag, T, bwd ? nullptr : y, y}, 0, YF);
-      if (bwd) sweep(LpSorBwd{fwd ? T : x, rs, idiag, YF, y, (float)pc.sor.omega, fwd ? 1 : 0}, 1, YB);
+      if (fwd) sweep(LpSorFwd<VT>{x, rs, idiag, T, bwd ? nullptr : y, y}, 0, YF);
+      if (bwd) sweep(LpSorBwd<VT>{fwd ? T : x, rs, idiag, YF, y, (VT)pc.sor.omega, fwd ? 1 : 0}, 1, YB);
     } else {
-      MPK_LAUNCH(k_lp_copy, lp_grid(n), 256, n, x, y);
+      MPK_LAUNCH(k_lp_copy<VT>, lp_grid(n), 256, n, x, y);
     }
   }
 
   // FP64 results of K dot products into st->red
   template <int K>
-  void dots(LpDots<K> d) {
-    MPK_LAUNCH(k_lp_dots<K>, LP_RBLK, 256, d, n, P);
+  void dots(LpDots<K, VT> d) {
+    MPK_LAUNCH((k_lp_dots<K, VT>), LP_RBLK, 256, d, n, P);
     MPK_LAUNCH(k_lp_fold, K, 256, (const double *)P, LP_RBLK, st->red, (const int *)nullptr);
   }
-  LpDots<1> dot1(const float *x, const float *y) { return LpDots<1>{{x}, {y}}; }
+  LpDots<1, VT> dot1(const VT *x, const VT *y) { return LpDots<1, VT>{{x}, {y}}; }
 
   double read_res(int *breakdown) {
     HIPCHK(hipMemcpyAsync(hst, &st->res, sizeof(double), hipMemcpyDeviceToHost, stream()));
@@ -740,11 +835,11 @@ struct MixedSolver {
 
   // Inner GMRES(m), left preconditioned, zero guess: d ~ A^-1 rhs to a
   // relative preconditioned residual rtol (or max_it steps).  Returns steps.
-  i64 gmres(const float *rhs, float *d, double rtol, i64 max_it) {
-    float *V = work, *w = work + (i64)(m + 1) * n, *z = w + n;
+  i64 gmres(const VT *rhs, VT *d, double rtol, i64 max_it) {
+    VT *V = work, *w = work + (i64)(m + 1) * n, *z = w + n;
     const i64 ld = n;
-    MPK_LAUNCH(k_lp_zero, lp_grid(n), 256, n, d);
-    MPK_LAUNCH(k_lp_copy, lp_grid(n), 256, n, rhs, z);
+    MPK_LAUNCH(k_lp_zero<VT>, lp_grid(n), 256, n, d);
+    MPK_LAUNCH(k_lp_copy<VT>, lp_grid(n), 256, n, rhs, z);
     i64 its = 0;
     double target = -1.0;
     while (its < max_it) {
@@ -752,7 +847,7 @@ struct MixedSolver {
       apply_pc(z, w);
       dots(dot1(w, nullptr));
       MPK_LAUNCH(k_gm_begin, 1, 1, st);
-      MPK_LAUNCH(k_lp_scale, lp_grid(n), 256, n, (const float *)w, (const double *)&st->inv, V,
+      MPK_LAUNCH(k_lp_scale<VT>, lp_grid(n), 256, n, (const VT *)w, (const double *)&st->inv, V,
                  (const int *)nullptr);
       int brk = 0;
       const double beta = read_res(&brk);
@@ -760,7 +855,7 @@ struct MixedSolver {
       if (brk || beta <= target) break;
       int kk = 0;
       for (int k = 0; k < m && its < max_it; k++) {
-        float *vk = V + (i64)k * ld, *vn = V + (i64)(k + 1) * ld;
+        VT *vk = V + (i64)k * ld, *vn = V + (i64)(k + 1) * ld;
         spmv(vk, z);
         apply_pc(z, w);
         const int nv = k + 1;
@@ -768,22 +863,21 @@ struct MixedSolver {
         // (h2 = V^T w, w -= V h2, h += h2) when k_gm_decide asks for it; the
         // device skips the second pass's kernels otherwise
         const int *nore = &st->norefine;
-        MPK_LAUNCH(k_lp_mdot, LP_RBLK, 256, (const float *)V, ld, nv, (const float *)w, n, P,
-                   (const int *)nullptr);
+        MPK_LAUNCH(k_lp_mdot<VT>, LP_RBLK, 256, (const VT *)V, ld, nv, (const VT *)w, n, P, (const int *)nullptr);
         MPK_LAUNCH(k_lp_fold, nv, 256, (const double *)P, LP_RBLK, st->h, (const int *)nullptr);
         MPK_LAUNCH(k_gm_neg, 1, 1, st, nv, (const double *)st->h, (const int *)nullptr);
-        MPK_LAUNCH(k_lp_maxpy, lp_grid(n), 256, n, w, (const float *)V, ld, (const double *)st->coef, nv,
+        MPK_LAUNCH(k_lp_maxpy<VT>, lp_grid(n), 256, n, w, (const VT *)V, ld, (const double *)st->coef, nv,
                    (const int *)nullptr);
         dots(dot1(w, nullptr));
         MPK_LAUNCH(k_gm_decide, 1, 1, st, nv, refine);
-        MPK_LAUNCH(k_lp_mdot, LP_RBLK, 256, (const float *)V, ld, nv, (const float *)w, n, P, nore);
+        MPK_LAUNCH(k_lp_mdot<VT>, LP_RBLK, 256, (const VT *)V, ld, nv, (const VT *)w, n, P, nore);
         MPK_LAUNCH(k_lp_fold, nv, 256, (const double *)P, LP_RBLK, st->h2, nore);
         MPK_LAUNCH(k_gm_neg, 1, 1, st, nv, (const double *)st->h2, nore);
-        MPK_LAUNCH(k_lp_maxpy, lp_grid(n), 256, n, w, (const float *)V, ld, (const double *)st->coef, nv, nore);
-        MPK_LAUNCH(k_lp_dots<1>, LP_RBLK, 256, dot1(w, nullptr), n, P);
+        MPK_LAUNCH(k_lp_maxpy<VT>, lp_grid(n), 256, n, w, (const VT *)V, ld, (const double *)st->coef, nv, nore);
+        MPK_LAUNCH((k_lp_dots<1, VT>), LP_RBLK, 256, dot1(w, nullptr), n, P);
         MPK_LAUNCH(k_lp_fold, 1, 256, (const double *)P, LP_RBLK, st->red, nore);
         MPK_LAUNCH(k_gm_column, 1, 1, st, k);
-        MPK_LAUNCH(k_lp_scale, lp_grid(n), 256, n, (const float *)w, (const double *)&st->inv, vn,
+        MPK_LAUNCH(k_lp_scale<VT>, lp_grid(n), 256, n, (const VT *)w, (const double *)&st->inv, vn,
                    (const int *)nullptr);
         its++;
         kk = k + 1;
@@ -791,26 +885,26 @@ struct MixedSolver {
         if (brk || res <= target) break;
       }
       MPK_LAUNCH(k_gm_solve, 1, 1, st, kk);
-      MPK_LAUNCH(k_lp_maxpy, lp_grid(n), 256, n, d, (const float *)V, ld, (const double *)st->y, kk,
+      MPK_LAUNCH(k_lp_maxpy<VT>, lp_grid(n), 256, n, d, (const VT *)V, ld, (const double *)st->y, kk,
                  (const int *)nullptr);
       const double res = read_res(&brk);
       if (brk || res <= target || its >= max_it) break;
       // restart: z = rhs - A d
       spmv(d, z);
-      MPK_LAUNCH(k_lp_rsub, lp_grid(n), 256, n, rhs, z);
+      MPK_LAUNCH(k_lp_rsub<VT>, lp_grid(n), 256, n, rhs, z);
     }
     return its;
   }
 
   // Inner BiCGStab, right preconditioned, zero guess, to a relative residual
   // rtol (or max_it steps).  Returns steps.
-  i64 bcgs(const float *rhs, float *d, double rtol, i64 max_it) {
-    float *r = work, *rh = r + n, *p = rh + n, *v = p + n, *sv = v + n, *t = sv + n, *ph = t + n, *sh = ph + n;
-    MPK_LAUNCH(k_lp_zero, lp_grid(n), 256, n, d);
-    MPK_LAUNCH(k_lp_copy, lp_grid(n), 256, n, rhs, r);
-    MPK_LAUNCH(k_lp_copy, lp_grid(n), 256, n, rhs, rh);
-    MPK_LAUNCH(k_lp_zero, lp_grid(n), 256, n, p);
-    MPK_LAUNCH(k_lp_zero, lp_grid(n), 256, n, v);
+  i64 bcgs(const VT *rhs, VT *d, double rtol, i64 max_it) {
+    VT *r = work, *rh = r + n, *p = rh + n, *v = p + n, *sv = v + n, *t = sv + n, *ph = t + n, *sh = ph + n;
+    MPK_LAUNCH(k_lp_zero<VT>, lp_grid(n), 256, n, d);
+    MPK_LAUNCH(k_lp_copy<VT>, lp_grid(n), 256, n, rhs, r);
+    MPK_LAUNCH(k_lp_copy<VT>, lp_grid(n), 256, n, rhs, rh);
+    MPK_LAUNCH(k_lp_zero<VT>, lp_grid(n), 256, n, p);
+    MPK_LAUNCH(k_lp_zero<VT>, lp_grid(n), 256, n, v);
     dots(dot1(r, nullptr));
     MPK_LAUNCH(k_bc_init, 1, 1, st);
     int brk = 0;
@@ -820,20 +914,20 @@ struct MixedSolver {
     while (!brk && its < max_it) {
       dots(dot1(rh, r));
       MPK_LAUNCH(k_bc_rho, 1, 1, st);
-      MPK_LAUNCH(k_bc_p, lp_grid(n), 256, n, p, (const float *)r, (const float *)v, (const double *)st->coef);
+      MPK_LAUNCH(k_bc_p<VT>, lp_grid(n), 256, n, p, (const VT *)r, (const VT *)v, (const double *)st->coef);
       apply_pc(p, ph);
       spmv(ph, v);
       dots(dot1(rh, v));
       MPK_LAUNCH(k_bc_alpha, 1, 1, st);
-      MPK_LAUNCH(k_lp_waxpy, lp_grid(n), 256, n, sv, (const float *)r, (const float *)v,
+      MPK_LAUNCH(k_lp_waxpy<VT>, lp_grid(n), 256, n, sv, (const VT *)r, (const VT *)v,
                  (const double *)(st->coef + 2));
       apply_pc(sv, sh);
       spmv(sh, t);
-      dots(LpDots<2>{{t, t}, {sv, nullptr}});
+      dots(LpDots<2, VT>{{t, t}, {sv, nullptr}});
       MPK_LAUNCH(k_bc_omega, 1, 1, st);
-      MPK_LAUNCH(k_lp_axpby_add, lp_grid(n), 256, n, d, (const float *)ph, (const float *)sh,
+      MPK_LAUNCH(k_lp_axpby_add<VT>, lp_grid(n), 256, n, d, (const VT *)ph, (const VT *)sh,
                  (const double *)(st->coef + 3));
-      MPK_LAUNCH(k_lp_waxpy, lp_grid(n), 256, n, r, (const float *)sv, (const float *)t,
+      MPK_LAUNCH(k_lp_waxpy<VT>, lp_grid(n), 256, n, r, (const VT *)sv, (const VT *)t,
                  (const double *)(st->coef + 5));
       dots(dot1(r, nullptr));
       MPK_LAUNCH(k_bc_res, 1, 1, st);
@@ -842,6 +936,112 @@ struct MixedSolver {
       if (res <= target) break;
     }
     return its;
+  }
+
+  // Builds the inner solve's copies of the matrix and preconditioner.
+  bool create(int64_t irow, int64_t icol, int64_t data, int64_t soff_, int64_t sv, int64_t sc_, int64_t pci,
+              int64_t pcf) {
+    A64 = DevCSR{n, n, (const int *)irow, (const int *)icol, (const double *)data, (const int *)soff_,
+                 (const double *)sv, (const int *)sc_};
+    soff = (const int *)soff_;
+    sc = (const int *)sc_;
+    if (!pc.parse((const int64_t *)pci, (const double *)pcf, n)) return false;
+    if (pc.type != PCT_ILU && pc.type != PCT_SOR && pc.type != PCT_NONE) {
+      set_error(6, "mixed solve: the preconditioner must be ilu, sor or none");
+      return false;
+    }
+    if ((pc.type == PCT_ILU || pc.type == PCT_SOR) && !pc_plan()) {
+      set_error(6, "mixed solve: the preconditioner has no sweep plans (mpk_pc_schedule)");
+      return false;
+    }
+    // the inner solve's matrix values in the sliced slots
+    std::vector<int> so(((size_t)n + 63) / 64 + 1);
+    copy_dtoh(so.data(), (const void *)soff, so.size() * sizeof(int));
+    const i64 total = sell_slots(so.back()), slots = total > 0 ? total : 1;
+    HIPCHK(hipMalloc(&lv, slots * esz()));
+    if (prec == 16) {
+      HIPCHK(hipMalloc(&rs, n * sizeof(float)));
+      MPK_LAUNCH(k_lp_sell<__half>, lp_grid(n), 256, n, (const int *)irow, soff, (const double *)sv, (__half *)lv,
+                 rs);
+    } else if (prec == 32) {
+      MPK_LAUNCH(k_lp_sell<float>, lp_grid(n), 256, n, (const int *)irow, soff, (const double *)sv, (float *)lv,
+                 (float *)nullptr);
+    } else {
+      MPK_LAUNCH(k_lp_sell<double>, lp_grid(n), 256, n, (const int *)irow, soff, (const double *)sv,
+                 (double *)lv, (float *)nullptr);
+    }
+    HIPCHK(hipMalloc(&winA, 2 * (i64)nblocks(n, LP_WROWS) * sizeof(int)));
+    winA_on = col_windows(n, (const int *)irow, soff, sc, winA, lp_xwin<VT>());
+    // the inner solve's preconditioner
+    if (const char *plan = pc_plan()) {
+      void **dst[2] = {&lc0, &lc1};
+      for (int sw = 0; sw < 2; sw++) {
+        const i64 ns = plan_slots(plan, sw, n);
+        HIPCHK(hipMalloc(dst[sw], (ns > 0 ? ns : 1) * esz()));
+        const float *rsw = pc.type == PCT_SOR ? rs : nullptr;
+        if (prec == 16)
+          MPK_LAUNCH(k_lp_plan_coef<__half>, lp_grid(n), 256, n, plan, sw, rsw, (__half *)*dst[sw]);
+        else if (prec == 32)
+          MPK_LAUNCH(k_lp_plan_coef<float>, lp_grid(n), 256, n, plan, sw, rsw, (float *)*dst[sw]);
+        else
+          MPK_LAUNCH(k_lp_plan_coef<double>, lp_grid(n), 256, n, plan, sw, rsw, (double *)*dst[sw]);
+      }
+      if (tri_iters > 0 && !build_tri()) return false;
+      if (pc.type == PCT_ILU) {
+        HIPCHK(hipMalloc(&dinv, n * sizeof(VT)));
+        MPK_LAUNCH(k_lp_ilu_dinv<VT>, lp_grid(n), 256, n, pc.ilu.ba, pc.ilu.bdiag, dinv);
+      } else {
+        HIPCHK(hipMalloc(&idiag, n * sizeof(VT)));
+        MPK_LAUNCH(k_lp_sor_idiag<VT>, lp_grid(n), 256, n, pc.sor.a, pc.sor.dmark, pc.sor.omega, pc.sor.fshift,
+                   idiag);
+      }
+    }
+    const i64 nvec = nsolver() + 5;
+    HIPCHK(hipMalloc(&work, nvec * n * sizeof(VT)));
+    HIPCHK(hipMalloc(&r64, n * sizeof(double)));
+    HIPCHK(hipMalloc(&P, (LP_MMAX + 1) * LP_RBLK * sizeof(double)));
+    HIPCHK(hipMalloc(&st, sizeof(LpState)));
+    HIPCHK(hipHostMalloc(&hst, 4 * sizeof(double), hipHostMallocDefault));
+    HIPCHK(hipMalloc(&ticket, 64 * sizeof(unsigned)));
+    HIPCHK(hipMalloc(&lst, (n > 0 ? n : 1) * sizeof(LpStaged<VT>)));
+    if (!failed()) HIPCHK(hipMemset(st, 0, sizeof(LpState)));
+    sync();
+    return !failed();
+  }
+
+  // Iterative refinement (mpk_mixed_solve).
+  int solve(const double *B, double *X, double *cfg) override {
+    const double rtol = cfg[0], irtol = cfg[1];
+    const i64 max_outer = (i64)cfg[2], max_inner = (i64)cfg[3];
+    refine = (int)cfg[4];
+    const bool zero = cfg[5] != 0.0;
+    double *R = r64;
+    const double bn = norm2_host(B, n);
+    cfg[11] = bn;
+    VT *rhs = rhs_vec(), *d = d_vec();
+    if (zero) vset(X, 0.0, n, nullptr);
+    i64 outer = 0, ninner = 0;
+    double rel = 0.0;
+    for (;;) {
+      mpk::spmv(A64, X, R, 0, nullptr);  // R = b - A x in FP64
+      vaypx(R, -1.0, B, n, nullptr);
+      const double rn = norm2_host(R, n);
+      rel = bn > 0.0 ? rn / bn : rn;
+      if (failed() || rel <= rtol || outer >= max_outer || !(rn > 0.0)) break;
+      // right-hand side scaled to unit norm; x += ||r|| d
+      double scl[2] = {1.0 / rn, rn};
+      copy_htod(st->red + 2, scl, sizeof scl);
+      MPK_LAUNCH(k_lp_from_d<VT>, lp_grid(n), 256, n, (const double *)R, (const double *)(st->red + 2), rhs);
+      ninner += inner == LP_GMRES ? gmres(rhs, d, irtol, max_inner) : bcgs(rhs, d, irtol, max_inner);
+      MPK_LAUNCH(k_lp_update<VT>, lp_grid(n), 256, n, X, (const double *)(st->red + 3), (const VT *)d);
+      outer++;
+    }
+    sync();
+    cfg[8] = rel;
+    cfg[9] = (double)outer;
+    cfg[10] = (double)ninner;
+    cfg[12] = rel <= rtol ? 1.0 : 0.0;
+    return failed() ? -1 : 0;
   }
 };
 
@@ -856,8 +1056,8 @@ int64_t mpk_mixed_create(int64_t n, int64_t irow, int64_t icol, int64_t data, in
                          int64_t sc, int64_t pci, int64_t pcf, int64_t prec, int64_t inner, int64_t restart,
                          int64_t tri_iters) {
   if (n <= 0 || !ctx()) return 0;
-  if (prec != 32 && prec != 16) {
-    set_error(6, "mixed solve: precision must be 32 or 16, got %lld", (long long)prec);
+  if (prec != 64 && prec != 32 && prec != 16) {
+    set_error(6, "mixed solve: precision must be 64, 32 or 16, got %lld", (long long)prec);
     return 0;
   }
   if (inner != LP_GMRES && inner != LP_BCGS) {
@@ -868,85 +1068,19 @@ int64_t mpk_mixed_create(int64_t n, int64_t irow, int64_t icol, int64_t data, in
     set_error(6, "mixed solve: restart must be in 1..%d", LP_MMAX);
     return 0;
   }
-  MixedSolver *S = new MixedSolver();
-  S->n = n;
-  S->prec = (int)prec;
-  S->inner = (int)inner;
-  S->m = (int)restart;
-  S->tri_iters = tri_iters > 0 ? (int)tri_iters : 0;
-  S->A64 = DevCSR{n, n, (const int *)irow, (const int *)icol, (const double *)data, (const int *)soff,
-                  (const double *)sv, (const int *)sc};
-  S->soff = (const int *)soff;
-  S->sc = (const int *)sc;
-  if (!S->pc.parse((const int64_t *)pci, (const double *)pcf, n)) {
-    delete S;
-    return 0;
-  }
-  if (S->pc.type != PCT_ILU && S->pc.type != PCT_SOR && S->pc.type != PCT_NONE) {
-    set_error(6, "mixed solve: the preconditioner must be ilu, sor or none");
-    delete S;
-    return 0;
-  }
-  if ((S->pc.type == PCT_ILU || S->pc.type == PCT_SOR) && !S->pc_plan()) {
-    set_error(6, "mixed solve: the preconditioner has no sweep plans (mpk_pc_schedule)");
-    delete S;
-    return 0;
-  }
-  // low-precision matrix values in the sliced slots
-  std::vector<int> so(((size_t)n + 63) / 64 + 1);
-  copy_dtoh(so.data(), (const void *)soff, so.size() * sizeof(int));
-  const i64 total = sell_slots(so.back()), slots = total > 0 ? total : 1, esz = prec == 16 ? 2 : 4;
-  HIPCHK(hipMalloc(&S->lv, slots * esz));
-  if (prec == 16) {
-    HIPCHK(hipMalloc(&S->rs, n * sizeof(float)));
-    MPK_LAUNCH(k_lp_sell<__half>, lp_grid(n), 256, n, (const int *)irow, (const int *)soff, (const double *)sv,
-               (__half *)S->lv, S->rs);
-  } else {
-    MPK_LAUNCH(k_lp_sell<float>, lp_grid(n), 256, n, (const int *)irow, (const int *)soff, (const double *)sv,
-               (float *)S->lv, (float *)nullptr);
-  }
-  HIPCHK(hipMalloc(&S->winA, 2 * (i64)nblocks(n, LP_WROWS) * sizeof(int)));
-  S->winA_on = col_windows(n, (const int *)irow, (const int *)soff, (const int *)sc, S->winA);
-  // low-precision preconditioner
-  if (const char *plan = S->pc_plan()) {
-    void **dst[2] = {&S->lc0, &S->lc1};
-    for (int sw = 0; sw < 2; sw++) {
-      const i64 ns = plan_slots(plan, sw, n);
-      HIPCHK(hipMalloc(dst[sw], (ns > 0 ? ns : 1) * esz));
-      const float *rsw = S->pc.type == PCT_SOR ? S->rs : nullptr;
-      if (prec == 16)
-        MPK_LAUNCH(k_lp_plan_coef<__half>, lp_grid(n), 256, n, plan, sw, rsw, (__half *)*dst[sw]);
-      else
-        MPK_LAUNCH(k_lp_plan_coef<float>, lp_grid(n), 256, n, plan, sw, rsw, (float *)*dst[sw]);
-    }
-    if (S->tri_iters > 0 && !S->build_tri()) {
+  auto make = [&](auto *S) -> int64_t {
+    S->n = n;
+    S->prec = (int)prec;
+    S->inner = (int)inner;
+    S->m = (int)restart;
+    S->tri_iters = tri_iters > 0 ? (int)tri_iters : 0;
+    if (!S->create(irow, icol, data, soff, sv, sc, pci, pcf)) {
       delete S;
       return 0;
     }
-    if (S->pc.type == PCT_ILU) {
-      HIPCHK(hipMalloc(&S->dinv, n * sizeof(float)));
-      MPK_LAUNCH(k_lp_ilu_dinv, lp_grid(n), 256, n, S->pc.ilu.ba, S->pc.ilu.bdiag, S->dinv);
-    } else {
-      HIPCHK(hipMalloc(&S->idiag, n * sizeof(float)));
-      MPK_LAUNCH(k_lp_sor_idiag, lp_grid(n), 256, n, S->pc.sor.a, S->pc.sor.dmark, S->pc.sor.omega,
-                 S->pc.sor.fshift, S->idiag);
-    }
-  }
-  const i64 nvec = S->nsolver() + 5;
-  HIPCHK(hipMalloc(&S->work, nvec * n * sizeof(float)));
-  HIPCHK(hipMalloc(&S->r64, n * sizeof(double)));
-  HIPCHK(hipMalloc(&S->P, (LP_MMAX + 1) * LP_RBLK * sizeof(double)));
-  HIPCHK(hipMalloc(&S->st, sizeof(LpState)));
-  HIPCHK(hipHostMalloc(&S->hst, 4 * sizeof(double), hipHostMallocDefault));
-  HIPCHK(hipMalloc(&S->ticket, 64 * sizeof(unsigned)));
-  HIPCHK(hipMalloc(&S->lst, (n > 0 ? n : 1) * sizeof(LpStaged)));
-  if (!failed()) HIPCHK(hipMemset(S->st, 0, sizeof(LpState)));
-  sync();
-  if (failed()) {
-    delete S;
-    return 0;
-  }
-  return (int64_t)S;
+    return (int64_t) static_cast<MixedBase *>(S);
+  };
+  return prec == 64 ? make(new MixedSolver<double>()) : make(new MixedSolver<float>());
 }
 
 // Iterative refinement: x (FP64, [D]) from b (FP64, [D]).  cfg [H] float64:
@@ -955,44 +1089,11 @@ int64_t mpk_mixed_create(int64_t n, int64_t irow, int64_t icol, int64_t data, in
 // out {cfg[8] final relative residual, cfg[9] outer steps, cfg[10] inner
 //      steps, cfg[11] ||b||, cfg[12] converged (1) or not (0)}.
 int64_t mpk_mixed_solve(int64_t h, int64_t b, int64_t x, int64_t cfg_) {
-  MixedSolver *S = (MixedSolver *)h;
-  double *cfg = (double *)cfg_;
+  MixedBase *S = (MixedBase *)h;
   if (!S || !ctx()) return -1;
-  const i64 n = S->n;
-  const double rtol = cfg[0], irtol = cfg[1];
-  const i64 max_outer = (i64)cfg[2], max_inner = (i64)cfg[3];
-  S->refine = (int)cfg[4];
-  const bool zero = cfg[5] != 0.0;
-  double *X = (double *)x, *R = S->r64;
-  const double *B = (const double *)b;
-  const double bn = norm2_host(B, n);
-  cfg[11] = bn;
-  float *rhs = S->rhs_vec(), *d = S->d_vec();
-  if (zero) vset(X, 0.0, n, nullptr);
-  i64 outer = 0, inner = 0;
-  double rel = 0.0;
-  for (;;) {
-    spmv(S->A64, X, R, 0, nullptr);  // R = b - A x in FP64
-    vaypx(R, -1.0, B, n, nullptr);
-    const double rn = norm2_host(R, n);
-    rel = bn > 0.0 ? rn / bn : rn;
-    if (failed() || rel <= rtol || outer >= max_outer || !(rn > 0.0)) break;
-    // FP32 right-hand side scaled to unit norm; x += ||r|| d
-    double sc[2] = {1.0 / rn, rn};
-    copy_htod(S->st->red + 2, sc, sizeof sc);
-    MPK_LAUNCH(k_lp_from_d, lp_grid(n), 256, n, (const double *)R, (const double *)(S->st->red + 2), rhs);
-    inner += S->inner == LP_GMRES ? S->gmres(rhs, d, irtol, max_inner) : S->bcgs(rhs, d, irtol, max_inner);
-    MPK_LAUNCH(k_lp_update, lp_grid(n), 256, n, X, (const double *)(S->st->red + 3), (const float *)d);
-    outer++;
-  }
-  sync();
-  cfg[8] = rel;
-  cfg[9] = (double)outer;
-  cfg[10] = (double)inner;
-  cfg[12] = rel <= rtol ? 1.0 : 0.0;
-  return failed() ? -1 : 0;
+  return S->solve((const double *)b, (double *)x, (double *)cfg_);
 }
 
-void mpk_mixed_free(int64_t h) { delete (MixedSolver *)h; }
+void mpk_mixed_free(int64_t h) { delete (MixedBase *)h; }
 
 }  // extern "C"

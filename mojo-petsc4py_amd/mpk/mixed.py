@@ -5,7 +5,9 @@ preconditioner (ILU or SOR, from a set-up `PETSc.PC`) run on FP32 vectors with
 matrix and preconditioner values in FP32, or in FP16 with one FP32 scale per
 row; dot products and norms sum in FP64, and the GMRES least squares is FP64.
 An FP64 iterative refinement around it computes the true residual b - A x and
-updates x in FP64 until ||b - A x|| / ||b|| <= rtol.
+updates x in FP64 until ||b - A x|| / ||b|| <= rtol.  precision="fp64" runs the
+same inner solver, preconditioner form and refinement on FP64 vectors and
+values: the control against which the lower precisions' gain is measured.
 
     from mpk.mixed import MixedSolver
     ms = MixedSolver(A, pc, precision="fp32", inner="gmres", restart=30)
@@ -20,14 +22,14 @@ import numpy as np
 
 from ._native import check, lib
 
-_PREC = {"fp32": 32, "fp16": 16}
+_PREC = {"fp64": 64, "fp32": 32, "fp16": 16}
 _INNER = {"gmres": 0, "bcgs": 1}
 _REFINE = {"never": 0, "ifneeded": 1, "always": 2}
 
 
 class MixedSolver:
-    """FP32 / FP16 inner solves under FP64 iterative refinement for one matrix
-    and preconditioner.  The matrix and PC must stay unchanged while the
+    """FP32 / FP16 (or FP64) inner solves under FP64 iterative refinement for
+    one matrix and preconditioner.  The matrix and PC must stay unchanged while the
     solver is used (it keeps low-precision copies of their values)."""
 
     def __init__(self, A, pc=None, precision: str = "fp32", inner: str = "gmres", restart: int = 30,

@@ -7,9 +7,9 @@
   1e-12 relative of the oracle in the reference's order -- except cg + icc(0),
   whose two orders drift apart (2.8e-10 in the max norm) and which therefore
   also runs in the reference's order, bit for bit with that oracle;
-* the mixed-precision solves of the two mixed workloads: FP64 relative
-  residual <= 1e-10 recomputed on the host, and the solution within 1e-8 of
-  the FP64 oracle's;
+* the mixed-precision solves of the two mixed workloads, and their FP64
+  controls: FP64 relative residual <= 1e-10 recomputed on the host, and the
+  solution within 1e-8 of the FP64 oracle's;
 * the 4096 batched systems: every one at FP64 relative residual <= 1e-8,
   a sample against the oracle's FP64 solves.
 
@@ -70,6 +70,9 @@ MIXED = [
     ("convdiff128-fp32-exact", "convdiff_128", "ilu", 1.0, "gmres", "fp32", "exact", 1e-3, "gmres_ilu0"),
     ("spd1e6-fp16-jac2", "random_spd_1e6", "sor", 1.2, "bcgs", "fp16", 2, 1e-2, "bcgs_sor"),
     ("spd1e6-fp32-jac2", "random_spd_1e6", "sor", 1.2, "bcgs", "fp32", 2, 1e-2, "bcgs_sor"),
+    # the FP64 controls (the same inner solver and preconditioner forms)
+    ("convdiff128-fp64-jac8", "convdiff_128", "ilu", 1.0, "gmres", "fp64", 8, 1e-2, "gmres_ilu0"),
+    ("spd1e6-fp64-jac2", "random_spd_1e6", "sor", 1.2, "bcgs", "fp64", 2, 1e-2, "bcgs_sor"),
 ]
 
 

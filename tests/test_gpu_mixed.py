@@ -1,7 +1,8 @@
 """Mixed-precision solves (FP32 / FP16 inner iterations under FP64 iterative
-refinement): they reach the FP64 relative residual asked for, checked here
-with an FP64 residual computed on the host, and their solution agrees with the
-oracle's FP64 solve of the same system to within what that residual allows."""
+refinement, and the same solver with FP64 inner iterations as their control):
+they reach the FP64 relative residual asked for, checked here with an FP64
+residual computed on the host, and their solution agrees with the oracle's
+FP64 solve of the same system to within what that residual allows."""
 
 import numpy as np
 import pytest
@@ -63,6 +64,13 @@ CASES = [
     # over 8192 rows with a deep, narrow DAG: ring-planned sweeps
     ("spd20k-sor-bcgs-fp32", lambda: W.random_spd(20000, band=3000), "sor", 1.2, "bcgs", "fp32", "exact"),
     ("lap120-ilu-gmres-fp16", lambda: (W.laplacian_2d(120), None), "ilu", 1.0, "gmres", "fp16", "exact"),
+    # the FP64 control: the same inner solver and preconditioner forms
+    ("cd12-ilu-gmres-fp64", lambda: W.convection_diffusion_3d(12), "ilu", 1.0, "gmres", "fp64", "exact"),
+    ("cd12-ilu-gmres-fp64-jac3", lambda: W.convection_diffusion_3d(12), "ilu", 1.0, "gmres", "fp64", 3),
+    ("spd3000-sor-bcgs-fp64", lambda: W.random_spd(3000, band=400), "sor", 1.2, "bcgs", "fp64", "exact"),
+    ("spd3000-sor-bcgs-fp64-jac3", lambda: W.random_spd(3000, band=400), "sor", 1.2, "bcgs", "fp64", 3),
+    ("spd20k-sor-bcgs-fp64", lambda: W.random_spd(20000, band=3000), "sor", 1.2, "bcgs", "fp64", "exact"),
+    ("lap40-none-gmres-fp64", lambda: (W.laplacian_2d(40), None), "none", 1.0, "gmres", "fp64", "exact"),
 ]
 
 
