@@ -139,24 +139,29 @@ __device__ __forceinline__ bool stage_window(const VT *__restrict__ x, const int
   return fit;
 }
 
-// Row i's serial sum over its sliced entries, four entries' loads issued
-// before their values are gathered (from xs when the window is staged).
+// Row i's serial sum over its sliced entries, eight entries' loads issued
+// before their values are gathered (from xs when the window is staged), and
+// all eight gathers before the first product: a row of up to eight entries
+// costs two dependent rounds of loads.
 template <class CT, class VT>
 __device__ __forceinline__ VT sliced_row(const int *__restrict__ sc, const CT *__restrict__ sv, int base, int len,
                                          const VT *__restrict__ x, const VT *xs, bool fit, int lo) {
+  constexpr int B = 8;
   VT s = VT(0);
-  for (int k0 = 0; k0 < len; k0 += 4) {
-    int c[4];
-    VT v[4];
+  for (int k0 = 0; k0 < len; k0 += B) {
+    int c[B];
+    VT v[B], xv[B];
 #pragma unroll
-    for (int e = 0; e < 4; e++) {
+    for (int e = 0; e < B; e++) {
       const int p = base + 64 * (k0 + e);
       c[e] = k0 + e < len ? sc[p] : 0;
       v[e] = k0 + e < len ? lp_val<VT>(sv[p]) : VT(0);
     }
 #pragma unroll
-    for (int e = 0; e < 4; e++)
-      if (k0 + e < len) s = lp_fma(v[e], fit ? xs[c[e] - lo] : x[c[e]], s);
+    for (int e = 0; e < B; e++) xv[e] = k0 + e < len ? (fit ? xs[c[e] - lo] : x[c[e]]) : VT(0);
+#pragma unroll
+    for (int e = 0; e < B; e++)
+      if (k0 + e < len) s = lp_fma(v[e], xv[e], s);
   }
   return s;
 }
@@ -260,22 +265,60 @@ __global__ void __launch_bounds__(256) k_lp_dots(LpDots<K, VT> d, i64 n, double 
     if (threadIdx.x == 0) P[(i64)k * gridDim.x + blockIdx.x] = r;
   }
 }
-// P[j * LP_RBLK + b] = block b's share of V_j . w, j < nv
+// Four consecutive elements of a VT vector (16-byte aligned) in one or two
+// 16-byte loads, as doubles.
+struct Q4 {
+  double x, y, z, w;
+};
+__device__ __forceinline__ Q4 load_q4(const float *p) {
+  const float4 v = *reinterpret_cast<const float4 *>(p);
+  return Q4{(double)v.x, (double)v.y, (double)v.z, (double)v.w};
+}
+__device__ __forceinline__ Q4 load_q4(const double *p) {
+  const double2 a = *reinterpret_cast<const double2 *>(p), b = *reinterpret_cast<const double2 *>(p + 2);
+  return Q4{a.x, a.y, b.x, b.y};
+}
+
+// P[j * LP_RBLK + b] = block b's share of V_j . w, j < nv.  Block b takes one
+// contiguous span of the vectors, in groups of 4 elements (V_j at V + j ld,
+// ld and the vectors 16-byte aligned: ld a multiple of 8); the last n % 4
+// elements go to block 0.  Eight vectors at a time: a thread loads its
+// group of w and of all eight (vectors past nv repeat the last one and are
+// not stored) before any product, so every load of a step is in flight at
+// once.
 template <class VT>
 __global__ void __launch_bounds__(256) k_lp_mdot(const VT *V, i64 ld, int nv, const VT *w, i64 n, double *P,
                                                  const int *skip) {
   if (skip && *skip) return;
   __shared__ double sh[4];
+  const i64 n4 = n / 4, per = (n4 + gridDim.x - 1) / gridDim.x;
+  const i64 g0 = (i64)blockIdx.x * per, g1 = g0 + per < n4 ? g0 + per : n4;
   for (int j0 = 0; j0 < nv; j0 += 8) {
+    const VT *Vj[8];
+#pragma unroll
+    for (int k = 0; k < 8; k++) Vj[k] = V + (i64)(j0 + k < nv ? j0 + k : nv - 1) * ld;
     double a[8];
 #pragma unroll
     for (int k = 0; k < 8; k++) a[k] = 0.0;
-    for (i64 i = (i64)blockIdx.x * 256 + threadIdx.x; i < n; i += (i64)gridDim.x * 256) {
-      const double wv = w[i];
+    for (i64 g = g0 + threadIdx.x; g < g1; g += 256) {
+      const Q4 wv = load_q4(w + 4 * g);
+      Q4 v[8];
 #pragma unroll
-      for (int k = 0; k < 8; k++)
-        if (j0 + k < nv) a[k] = fma((double)V[(i64)(j0 + k) * ld + i], wv, a[k]);
+      for (int k = 0; k < 8; k++) v[k] = load_q4(Vj[k] + 4 * g);
+#pragma unroll
+      for (int k = 0; k < 8; k++) {
+        a[k] = fma(v[k].x, wv.x, a[k]);
+        a[k] = fma(v[k].y, wv.y, a[k]);
+        a[k] = fma(v[k].z, wv.z, a[k]);
+        a[k] = fma(v[k].w, wv.w, a[k]);
+      }
     }
+    if (blockIdx.x == 0)
+      for (i64 i = 4 * n4 + threadIdx.x; i < n; i += 256) {
+        const double wi = (double)w[i];
+#pragma unroll
+        for (int k = 0; k < 8; k++) a[k] = fma((double)Vj[k][i], wi, a[k]);
+      }
 #pragma unroll
     for (int k = 0; k < 8; k++) {
       const double r = block_tree(a[k], sh);
@@ -701,7 +744,7 @@ struct MixedSolver : MixedBase {
     if (tri_iters > 0) {
       // Jacobi passes, ping-ponging through the 4th PC vector so that the last
       // pass lands in pub
-      VT *tmp = pc_scratch() + 3 * n;
+      VT *tmp = pc_scratch() + 3 * vs();
       const TriSell &T = tri[second];
       for (int j = 0; j < tri_iters; j++) {
         VT *out = ((tri_iters - 1 - j) % 2 == 0) ? pub : tmp;
@@ -729,9 +772,12 @@ struct MixedSolver : MixedBase {
   // BiCGStab: r, rhat, p, v, s, t, phat, shat, d -- d last), then 4 for the
   // preconditioner's sweeps, then the right-hand side
   i64 nsolver() const { return inner == LP_GMRES ? m + 4 : 9; }
-  VT *d_vec() { return work + (nsolver() - 1) * n; }
-  VT *pc_scratch() { return work + nsolver() * n; }
-  VT *rhs_vec() { return work + (nsolver() + 4) * n; }
+  // the stride of the work buffer's vectors: n rounded up to 8 entries, so
+  // every vector starts 16-byte aligned (k_lp_mdot's loads)
+  i64 vs() const { return (n + 7) & ~(i64)7; }
+  VT *d_vec() { return work + (nsolver() - 1) * vs(); }
+  VT *pc_scratch() { return work + nsolver() * vs(); }
+  VT *rhs_vec() { return work + (nsolver() + 4) * vs(); }
 
   // The sweeps' entries as sliced matrices in the inner solve's value type
   // (coefficients divided by the row scale where the matrix has one: SOR at
@@ -802,12 +848,13 @@ struct MixedSolver : MixedBase {
   // y = M^-1 x in VT
   void apply_pc(const VT *x, VT *y) {
     VT *S = pc_scratch();
+    const i64 ld = vs();
     if (pc.type == PCT_ILU) {
-      VT *F = S, *B = S + n;
+      VT *F = S, *B = S + ld;
       sweep(LpIluFwd<VT>{pc.ilu.r, x}, 0, F);
       sweep(LpIluBwd<VT>{pc.ilu.c, F, dinv, y}, 1, B);
     } else if (pc.type == PCT_SOR) {
-      VT *YF = S, *YB = S + n, *T = S + 2 * n;
+      VT *YF = S, *YB = S + ld, *T = S + 2 * ld;
       const int flag = pc.sor.flag;
       const bool fwd = (flag & (1 | 4)) != 0, bwd = (flag & (2 | 8)) != 0;
       if (fwd) sweep(LpSorFwd<VT>{x, rs, idiag, T, bwd ? nullptr : y, y}, 0, YF);
@@ -836,8 +883,8 @@ struct MixedSolver : MixedBase {
   // Inner GMRES(m), left preconditioned, zero guess: d ~ A^-1 rhs to a
   // relative preconditioned residual rtol (or max_it steps).  Returns steps.
   i64 gmres(const VT *rhs, VT *d, double rtol, i64 max_it) {
-    VT *V = work, *w = work + (i64)(m + 1) * n, *z = w + n;
-    const i64 ld = n;
+    const i64 ld = vs();
+    VT *V = work, *w = work + (i64)(m + 1) * ld, *z = w + ld;
     MPK_LAUNCH(k_lp_zero<VT>, lp_grid(n), 256, n, d);
     MPK_LAUNCH(k_lp_copy<VT>, lp_grid(n), 256, n, rhs, z);
     i64 its = 0;
@@ -899,7 +946,8 @@ struct MixedSolver : MixedBase {
   // Inner BiCGStab, right preconditioned, zero guess, to a relative residual
   // rtol (or max_it steps).  Returns steps.
   i64 bcgs(const VT *rhs, VT *d, double rtol, i64 max_it) {
-    VT *r = work, *rh = r + n, *p = rh + n, *v = p + n, *sv = v + n, *t = sv + n, *ph = t + n, *sh = ph + n;
+    const i64 ld = vs();
+    VT *r = work, *rh = r + ld, *p = rh + ld, *v = p + ld, *sv = v + ld, *t = sv + ld, *ph = t + ld, *sh = ph + ld;
     MPK_LAUNCH(k_lp_zero<VT>, lp_grid(n), 256, n, d);
     MPK_LAUNCH(k_lp_copy<VT>, lp_grid(n), 256, n, rhs, r);
     MPK_LAUNCH(k_lp_copy<VT>, lp_grid(n), 256, n, rhs, rh);
@@ -997,7 +1045,7 @@ struct MixedSolver : MixedBase {
       }
     }
     const i64 nvec = nsolver() + 5;
-    HIPCHK(hipMalloc(&work, nvec * n * sizeof(VT)));
+    HIPCHK(hipMalloc(&work, nvec * vs() * sizeof(VT)));
     HIPCHK(hipMalloc(&r64, n * sizeof(double)));
     HIPCHK(hipMalloc(&P, (LP_MMAX + 1) * LP_RBLK * sizeof(double)));
     HIPCHK(hipMalloc(&st, sizeof(LpState)));
