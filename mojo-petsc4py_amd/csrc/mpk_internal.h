@@ -225,11 +225,28 @@ __device__ __forceinline__ double sell_row(const double *__restrict__ sv, const 
   return s;
 }
 
+// Lane i + n's value within lane i's row of 16 (DPP row_shl:n; lanes whose
+// i + n leaves the row read 0.0), both halves of the double.
+template <int N>
+__device__ __forceinline__ double row_shl_d(double v) {
+  const long long b = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_update_dpp(0, (int)b, 0x100 + N, 0xF, 0xF, true);
+  const int hi = __builtin_amdgcn_update_dpp(0, (int)(b >> 32), 0x100 + N, 0xF, 0xF, true);
+  return __longlong_as_double((long long)(((unsigned long long)(unsigned)hi << 32) | (unsigned)lo));
+}
+
 // Halving tree over the 64 lanes of a wave: u[i] += u[i + w], w = 32..1.
-// Lane 0 ends with the wave's sum; the other lanes hold partial values.
+// Lane 0 ends with the wave's sum; the other lanes hold partial values.  At
+// step w only lanes i < 2w still feed lane 0, so for w <= 8 (i + w < 16) a
+// row shift gives every lane that matters its operand: the lane-crossing
+// permute is needed for w = 32 and 16 only.
 __device__ __forceinline__ double wave_tree(double u) {
-#pragma unroll
-  for (int w = 32; w >= 1; w >>= 1) u = u + __shfl_down(u, w, 64);
+  u = u + __shfl_down(u, 32, 64);
+  u = u + __shfl_down(u, 16, 64);
+  u = u + row_shl_d<8>(u);
+  u = u + row_shl_d<4>(u);
+  u = u + row_shl_d<2>(u);
+  u = u + row_shl_d<1>(u);
   return u;
 }
 

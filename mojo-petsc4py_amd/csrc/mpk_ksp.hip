@@ -863,8 +863,7 @@ static void ew2(i64 n, const Op &op, const int64_t *stop) {
 // Back substitution of KSPGMRESBuildSoln / _fgmres_buildsoln (ksp.mojo:632-658,
 // 886-900) in one thread.  Sets nbuild = it + 1 vectors to combine, or
 // skip_build when there is nothing to combine.
-__global__ void k_gmres_backsub(KspState *s, int flexible, const int64_t *stop) {
-  if (stop && *stop) return;
+__device__ void gmres_backsub(KspState *s, int flexible) {
   const i64 m = s->m;
   i64 it = s->it - 1;
   double *hh = s->HH, *rs = s->RS;
@@ -894,9 +893,13 @@ __global__ void k_gmres_backsub(KspState *s, int flexible, const int64_t *stop) 
   s->nbuild = it + 1;
   s->skip_build = 0;
 }
+__global__ void k_gmres_backsub(KspState *s, int flexible, const int64_t *stop) {
+  if (stop && *stop) return;
+  gmres_backsub(s, flexible);
+}
 
 // The tail of a GMRES cycle (ksp.mojo:756-768) / FGMRES cycle (861-867).
-__global__ void k_gmres_cycle_end(KspState *s, int flexible) {
+__device__ void gmres_cycle_end(KspState *s, int flexible) {
   if (s->stop) return;
   if (!flexible) {
     if (s->reason == 0 && s->breason != 0) s->reason = s->breason;
@@ -917,6 +920,7 @@ __global__ void k_gmres_cycle_end(KspState *s, int flexible) {
     if (s->reason != 0) finish(s, s->res, s->its, s->reason);
   }
 }
+__global__ void k_gmres_cycle_end(KspState *s, int flexible) { gmres_cycle_end(s, flexible); }
 
 // Combination for BuildSoln: y = sum_k rs[k] v_k, k < nbuild (device count).
 struct OpBuild {
@@ -1139,15 +1143,18 @@ struct Solve {
 // epilogue sets the stop flag (at the latest at max_it).
 
 #define MPK_SMALL_ROWS 16384
+#define MPK_SMALL_PLAIN 4096  // the limit for a PC without sweeps, and for GMRES / FGMRES / BiCGStab
 #define MPK_SMALL_TILES (MPK_SMALL_ROWS / MPK_TILE)
 
-struct SmallLds {
+template <int LVLS>
+struct SmallLdsT {
   double val[MPK_BLK_ROWS + 1];  // the sweeps' values
-  int lv[MPK_WAVE_LVLS + 1];     // the sweeps' level bounds
-  double wsum[3][MPK_SMALL_TILES][4];  // per tile, its four wave sums
-  double fsum[3][4];                   // the fold's wave sums
-  double res[3];
+  int lv[LVLS + 1];              // the sweeps' level bounds
+  double wsum[8][MPK_SMALL_TILES][4];  // per tile, its four wave sums
+  double fsum[8][4];                   // the fold's wave sums
+  double res[8];
 };
+using SmallLds = SmallLdsT<MPK_WAVE_LVLS>;
 
 struct SmallCG {
   i64 n, max_it;
@@ -1165,21 +1172,27 @@ struct SmallCG {
 
 // K reductions (RedSet) of length n <= MPK_SMALL_ROWS into L.res, in the
 // tile-tree order (mpk_reduce.h).  Every thread calls it.
-template <int K>
-__device__ __forceinline__ void small_reduce(const RedSet<K> &rs, i64 n, SmallLds &L) {
+template <int K, class LD>
+__device__ __forceinline__ void small_reduce(const RedSet<K> &rs, i64 n, LD &L) {
   const int t = (int)threadIdx.x, tl = t & 255, lane = t & 63, wv = (t >> 6) & 3;
   const int T = (int)((n + MPK_TILE - 1) / MPK_TILE);
   bool al = true;
 #pragma unroll
   for (int k = 0; k < K; k++) al = al && aligned16(rs.s[k].x) && (rs.s[k].kind != RK_DOT || aligned16(rs.s[k].y));
   for (int j = t >> 8; j < T; j += 2) {  // block t >> 8 takes every other tile
+    // every sum's leaves (and so all their loads) before any wave tree
+    const i64 base = (i64)j * MPK_TILE + 2 * tl;
+    double u[K];
 #pragma unroll
     for (int k = 0; k < K; k++) {
-      const i64 base = (i64)j * MPK_TILE + 2 * tl;
       const double p0 = pair_leaf(rs.s[k], base, n, al), p1 = pair_leaf(rs.s[k], base + 512, n, al);
       const double p2 = pair_leaf(rs.s[k], base + 1024, n, al), p3 = pair_leaf(rs.s[k], base + 1536, n, al);
-      const double u = wave_tree((p0 + p1) + (p2 + p3));
-      if (lane == 0) L.wsum[k][j][wv] = u;
+      u[k] = (p0 + p1) + (p2 + p3);
+    }
+#pragma unroll
+    for (int k = 0; k < K; k++) {
+      const double w = wave_tree(u[k]);
+      if (lane == 0) L.wsum[k][j][wv] = w;
     }
   }
   __syncthreads();
@@ -1243,8 +1256,8 @@ __device__ __forceinline__ void small_spmv(const SmallCG &a, const double *x, do
 
 // z = M^-1 r for the PC type PCT (PCView::apply's arithmetic), ending with a
 // barrier.
-template <int PCT>
-__device__ __forceinline__ void small_pc(const SmallCG &a, const double *r, double *z, SmallLds &L) {
+template <int PCT, class LD>
+__device__ __forceinline__ void small_pc(const SmallCG &a, const double *r, double *z, LD &L) {
   const i64 n = a.n;
   const int t = (int)threadIdx.x;
   if (PCT == PCT_JACOBI) {
@@ -1306,12 +1319,391 @@ __global__ void __launch_bounds__(MPK_WAVE_THREADS) k_cg_small(SmallCG a) {
   }
 }
 
+// ------------------------------------------------------------ small GMRES / FGMRES / BiCGStab
+//
+// The same one-workgroup form for KSPSolve_GMRES (ksp.mojo:667-769),
+// KSPSolve_FGMRES (775-868) and the iterations of KSPSolve_BCGS (437-505):
+// each step of the general path's launch sequence in order, with its
+// arithmetic (the pair operations, serial row sums, tile-tree reductions and
+// the epilogues in thread 0), and a step the general path would skip on a
+// flag (stop, inner_stop, skip_ref, skip_build) skipped on the same flag,
+// read after the barrier that follows the epilogue setting it.
+
+#define MPK_SMALL_M 31  // the largest restart of the one-workgroup GMRES
+
+// The LDS of the GMRES / FGMRES / BiCGStab drivers: the sweeps' and
+// reductions' space (levels up to the rows, n <= MPK_SMALL_PLAIN), the
+// Gram-Schmidt sums of a step, and the solve's state with its Hessenberg,
+// Givens and right-hand-side arrays, moved here for the kernel's length so
+// that the epilogues' serial steps in thread 0 touch LDS, not global memory.
+struct SmallKspLds : SmallLdsT<MPK_SMALL_PLAIN> {
+  double mres[MPK_SMALL_M + 1];
+  KspState ks;
+  double HH[(MPK_SMALL_M + 2) * (MPK_SMALL_M + 1)], HES[(MPK_SMALL_M + 1) * (MPK_SMALL_M + 1)];
+  double CC[MPK_SMALL_M + 1], SS[MPK_SMALL_M + 1], RS[MPK_SMALL_M + 2], LHH[MPK_SMALL_M + 1];
+};
+
+// The state into LDS (with the GMRES arrays when `arrays`: m <= MPK_SMALL_M);
+// returns the LDS copy, which the kernel then uses throughout.  Every thread
+// calls it.
+__device__ __forceinline__ KspState *small_state_in(KspState *g, SmallKspLds &L, bool arrays) {
+  const int t = (int)threadIdx.x;
+  if (arrays) {
+    const i64 m = g->m;
+    for (i64 i = t; i < (m + 2) * (m + 1); i += MPK_WAVE_THREADS) L.HH[i] = g->HH[i];
+    for (i64 i = t; i < (m + 1) * (m + 1); i += MPK_WAVE_THREADS) L.HES[i] = g->HES[i];
+    for (i64 i = t; i <= m; i += MPK_WAVE_THREADS) {
+      L.CC[i] = g->CC[i];
+      L.SS[i] = g->SS[i];
+      L.LHH[i] = g->LHH[i];
+    }
+    for (i64 i = t; i < m + 2; i += MPK_WAVE_THREADS) L.RS[i] = g->RS[i];
+  }
+  if (t == 0) {
+    L.ks = *g;
+    if (arrays) {
+      L.ks.HH = L.HH, L.ks.HES = L.HES, L.ks.CC = L.CC;
+      L.ks.SS = L.SS, L.ks.RS = L.RS, L.ks.LHH = L.LHH;
+    }
+  }
+  __syncthreads();
+  return &L.ks;
+}
+// The LDS state back into g (g keeps its own array pointers).  Every thread
+// calls it.
+__device__ __forceinline__ void small_state_out(KspState *g, SmallKspLds &L, bool arrays) {
+  __syncthreads();
+  const int t = (int)threadIdx.x;
+  if (arrays) {
+    const i64 m = L.ks.m;
+    for (i64 i = t; i < (m + 2) * (m + 1); i += MPK_WAVE_THREADS) g->HH[i] = L.HH[i];
+    for (i64 i = t; i < (m + 1) * (m + 1); i += MPK_WAVE_THREADS) g->HES[i] = L.HES[i];
+    for (i64 i = t; i <= m; i += MPK_WAVE_THREADS) {
+      g->CC[i] = L.CC[i];
+      g->SS[i] = L.SS[i];
+      g->LHH[i] = L.LHH[i];
+    }
+    for (i64 i = t; i < m + 2; i += MPK_WAVE_THREADS) g->RS[i] = L.RS[i];
+  }
+  if (t == 0) {
+    KspState o = L.ks;
+    o.HH = g->HH, o.HES = g->HES, o.CC = g->CC;
+    o.SS = g->SS, o.RS = g->RS, o.LHH = g->LHH;
+    *g = o;
+  }
+}
+
+struct SmallKsp {
+  SmallCG c;  // the matrix, the PC, the state, c.X the solution
+  int side, flexible;
+  i64 m, orthog, cgstype, guess_zero;
+  double *vv, *prevec, *TEMP, *TMP;  // GMRES / FGMRES: the bases (leading dimension n), work
+  const double *B;
+  double *R, *RP, *V, *T, *Sv, *P;  // BiCGStab
+};
+
+__device__ __forceinline__ bool small_flag(const i64 *f) { return *reinterpret_cast<const volatile i64 *>(f) != 0; }
+
+// A pair operation (Op(e, n) on elements e, e + 1) over the vector, as k_ew2
+// applies it; no barrier.
+template <class Op>
+__device__ __forceinline__ void small_ew(const Op &op, i64 n) {
+  for (i64 e = 2 * (i64)threadIdx.x; e < n; e += 2 * MPK_WAVE_THREADS) op(e, n);
+}
+__device__ __forceinline__ void small_copy(const double *x, double *y, i64 n) {
+  if (x == y) return;
+  for (i64 i = threadIdx.x; i < n; i += MPK_WAVE_THREADS) y[i] = x[i];
+}
+// y = alpha x + y, VecAXPY's fma(alpha, x, y)
+__device__ __forceinline__ void small_axpy(double *y, double alpha, const double *x, i64 n) {
+  for (i64 i = threadIdx.x; i < n; i += MPK_WAVE_THREADS) y[i] = fma(alpha, x[i], y[i]);
+}
+
+// KSP_PCApplyBAorAB (ksp.mojo:246-265) for the left and right sides, w the
+// work vector; ends with a barrier.
+template <int PCT>
+__device__ __forceinline__ void small_ba(const SmallKsp &a, const double *x, double *y, double *w, SmallKspLds &L) {
+  if (a.side == PC_RIGHT) {
+    small_pc<PCT>(a.c, x, w, L);
+    small_spmv(a.c, w, y);
+  } else {
+    small_spmv(a.c, x, w);
+    __syncthreads();
+    small_pc<PCT>(a.c, w, y, L);
+  }
+  __syncthreads();
+}
+
+// x . V_j for G vectors from v0 (leading dimension ld) into L.mres + j0:
+// k_partials_mdot's tile partials (x's pairs loaded once for the G vectors)
+// and small_reduce's fold, which is k_finalize's.  Every thread calls it.
+template <int G>
+__device__ __forceinline__ void small_mdot_g(const double *x, const double *v0, i64 ld, int j0, i64 n,
+                                             SmallKspLds &L) {
+  const int t = (int)threadIdx.x, tl = t & 255, lane = t & 63, wv = (t >> 6) & 3;
+  const int T = (int)((n + MPK_TILE - 1) / MPK_TILE);
+  const bool al = aligned16(x) && aligned16(v0) && ld % 2 == 0;
+  for (int j = t >> 8; j < T; j += 2) {  // block t >> 8 takes every other tile
+    const i64 base = (i64)j * MPK_TILE + 2 * tl;
+    D2 xs[4];
+#pragma unroll
+    for (int q = 0; q < 4; q++) xs[q] = ld2(x, base + 512 * q, n, al);
+    double u[G];
+#pragma unroll
+    for (int g = 0; g < G; g++) {
+      const double *v = v0 + (i64)g * ld;
+      double p[4];
+#pragma unroll
+      for (int q = 0; q < 4; q++) {
+        const D2 vv = ld2(v, base + 512 * q, n, al);
+        p[q] = fma(xs[q].a, vv.a, xs[q].b * vv.b);
+      }
+      u[g] = (p[0] + p[1]) + (p[2] + p[3]);
+    }
+#pragma unroll
+    for (int g = 0; g < G; g++) {
+      const double w = wave_tree(u[g]);
+      if (lane == 0) L.wsum[g][j][wv] = w;
+    }
+  }
+  __syncthreads();
+  if (t < 256) {
+#pragma unroll
+    for (int g = 0; g < G; g++) {
+      const double u = t < T ? (L.wsum[g][t][0] + L.wsum[g][t][2]) + (L.wsum[g][t][1] + L.wsum[g][t][3]) : 0.0;
+      const double r = wave_tree(u);
+      if (lane == 0) L.fsum[g][wv] = r;
+    }
+  }
+  __syncthreads();
+  if (t == 0) {
+#pragma unroll
+    for (int g = 0; g < G; g++) L.mres[j0 + g] = (L.fsum[g][0] + L.fsum[g][2]) + (L.fsum[g][1] + L.fsum[g][3]);
+  }
+}
+// x . V_j, j < nv, into L.mres, four sums at a time; ends with a barrier.
+__device__ __forceinline__ void small_mdot(const double *x, const double *V, i64 ld, int nv, i64 n,
+                                           SmallKspLds &L) {
+  int j0 = 0;
+  for (; j0 + 4 <= nv; j0 += 4) small_mdot_g<4>(x, V + (i64)j0 * ld, ld, j0, n, L);
+  const double *v0 = V + (i64)j0 * ld;
+  if (nv - j0 == 3)
+    small_mdot_g<3>(x, v0, ld, j0, n, L);
+  else if (nv - j0 == 2)
+    small_mdot_g<2>(x, v0, ld, j0, n, L);
+  else if (nv - j0 == 1)
+    small_mdot_g<1>(x, v0, ld, j0, n, L);
+  __syncthreads();
+}
+
+// One Arnoldi orthogonalisation of vnew against v_0..v_it (orthogonalize's
+// steps); ends with a barrier.
+__device__ __forceinline__ void small_orthogonalize(const SmallKsp &a, KspState *s, i64 it, double *vnew,
+                                                    SmallKspLds &L) {
+  const i64 n = a.c.n;
+  const int t = (int)threadIdx.x;
+  if (a.orthog == 1) {  // MGS
+    for (i64 k = 0; k <= it; k++) {
+      const double *vk = a.vv + k * n;
+      small_reduce<1>(RedSet<1>{{{vnew, vk, RK_DOT}}}, n, L);
+      if (t == 0) ksp_epilogue(s, EPI_GM_MGS, (int)k, L.res);
+      __syncthreads();
+      small_ew(OpAxpyDev{vnew, vk, s->LHH + k, aligned16(vnew) && aligned16(vk)}, n);
+      __syncthreads();
+    }
+    return;
+  }
+  const int nv = (int)(it + 1);
+  for (int pass = 0; pass < 2; pass++) {
+    if (pass == 1 && small_flag(&s->skip_ref)) break;
+    small_mdot(vnew, a.vv, n, nv, n, L);
+    if (t == 0) ksp_epilogue(s, pass == 0 ? EPI_GM_CGS_A : EPI_GM_CGS_B, 0, L.mres);
+    __syncthreads();
+    // vmaxpy_dev(vnew, vv, n, LHH, nv, 1.0): each element's chain over the nv
+    // vectors, eight vectors' loads issued before their multiply-adds
+    const double *coef = s->LHH;
+    for (i64 i = t; i < n; i += MPK_WAVE_THREADS) {
+      double acc = vnew[i];
+      int k = 0;
+      for (; k + 8 <= nv; k += 8) {
+        double v[8];
+#pragma unroll
+        for (int u = 0; u < 8; u++) v[u] = a.vv[(i64)(k + u) * n + i];
+#pragma unroll
+        for (int u = 0; u < 8; u++) acc = fma(1.0 * coef[k + u], v[u], acc);
+      }
+      for (; k < nv; k++) acc = fma(1.0 * coef[k], a.vv[(i64)k * n + i], acc);
+      vnew[i] = acc;
+    }
+    __syncthreads();
+    if (pass == 0 && a.cgstype == 1) {
+      small_reduce<1>(RedSet<1>{{{vnew, vnew, RK_SUMSQ}}}, n, L);
+      if (t == 0) ksp_epilogue(s, EPI_GM_CGS_NORM, 0, L.res);
+      __syncthreads();
+    }
+    if (pass == 0 && a.cgstype == 0) break;
+  }
+}
+
+// KSPSolve_GMRES's restart cycles (mp_ksp_solve_gmres's launch sequence).
+template <int PCT>
+__global__ void __launch_bounds__(MPK_WAVE_THREADS) k_gmres_small(SmallKsp a) {
+  __shared__ SmallKspLds L;
+  KspState *s = small_state_in(a.c.s, L, true);
+  const i64 n = a.c.n, m = a.m;
+  const int t = (int)threadIdx.x;
+  double *X = a.c.X, *vv = a.vv, *TEMP = a.TEMP, *TMP = a.TMP;
+  i64 guess_zero = a.guess_zero;
+  for (i64 cycle = 0;; cycle++) {
+    // KSPInitialResidual(guess_zero, X, TEMP, TMP, vv, B)
+    if (!guess_zero) {
+      small_spmv(a.c, X, TEMP);
+      small_copy(a.B, TMP, n);
+      __syncthreads();
+      small_axpy(TMP, -1.0, TEMP, n);
+    } else {
+      small_copy(a.B, TMP, n);
+    }
+    __syncthreads();
+    if (a.side == PC_LEFT) {
+      small_pc<PCT>(a.c, TMP, vv, L);
+    } else {
+      small_copy(TMP, vv, n);
+      __syncthreads();
+    }
+    small_reduce<1>(RedSet<1>{{{vv, vv, RK_SUMSQ}}}, n, L);
+    if (t == 0) ksp_epilogue(s, EPI_GM_START, 0, L.res);
+    __syncthreads();
+    if (!small_flag(&s->stop)) small_ew(OpDivDev{vv, &s->res, aligned16(vv)}, n);
+    __syncthreads();
+    for (i64 k = 0; k < m; k++) {
+      if (small_flag(&s->inner_stop)) break;
+      double *vk = vv + k * n, *vnew = vv + (k + 1) * n;
+      small_ba<PCT>(a, vk, vnew, TMP, L);
+      small_orthogonalize(a, s, k, vnew, L);
+      small_reduce<1>(RedSet<1>{{{vnew, vnew, RK_SUMSQ}}}, n, L);
+      if (t == 0) ksp_epilogue(s, EPI_GM_STEP, 0, L.res);
+      __syncthreads();
+      if (!small_flag(&s->inner_stop)) small_ew(OpDivDev{vnew, &s->tt, aligned16(vnew)}, n);
+      __syncthreads();
+    }
+    // KSPGMRESBuildSoln
+    if (t == 0 && !small_flag(&s->stop)) gmres_backsub(s, 0);
+    __syncthreads();
+    if (!small_flag(&s->skip_build)) {
+      small_ew(OpBuild{TEMP, vv, n, s, aligned16(TEMP) && aligned16(vv) && n % 2 == 0}, n);
+      __syncthreads();
+      // KSPUnwindPreconditioner(TEMP, TMP)
+      small_copy(TEMP, TMP, n);
+      __syncthreads();
+      if (a.side == PC_RIGHT) {
+        small_pc<PCT>(a.c, TEMP, TMP, L);
+        small_copy(TMP, TEMP, n);
+        __syncthreads();
+      }
+      small_axpy(X, 1.0, TEMP, n);
+      __syncthreads();
+    }
+    if (t == 0) gmres_cycle_end(s, 0);
+    __syncthreads();
+    if (small_flag(&s->stop)) break;
+    guess_zero = 0;
+    if (cycle > a.c.max_it + 2) break;
+  }
+  small_state_out(a.c.s, L, true);
+}
+
+// KSPSolve_FGMRES's cycles after the first residual (mp_ksp_solve_fgmres's
+// launch sequence).
+template <int PCT>
+__global__ void __launch_bounds__(MPK_WAVE_THREADS) k_fgmres_small(SmallKsp a) {
+  __shared__ SmallKspLds L;
+  KspState *s = small_state_in(a.c.s, L, true);
+  const i64 n = a.c.n, m = a.m;
+  const int t = (int)threadIdx.x;
+  double *X = a.c.X, *vv = a.vv, *TEMP = a.TEMP;
+  for (i64 cycle = 0;; cycle++) {
+    small_reduce<1>(RedSet<1>{{{vv, vv, RK_SUMSQ}}}, n, L);
+    if (t == 0) ksp_epilogue(s, EPI_FGM_START, 0, L.res);
+    __syncthreads();
+    if (!small_flag(&s->inner_stop)) small_ew(OpScaleRecipDev{vv, &s->res, nullptr, aligned16(vv)}, n);
+    __syncthreads();
+    for (i64 k = 0; k < m; k++) {
+      if (small_flag(&s->inner_stop)) break;
+      double *vk = vv + k * n, *vnew = vv + (k + 1) * n, *pk = a.prevec + k * n;
+      small_pc<PCT>(a.c, vk, pk, L);
+      small_spmv(a.c, pk, vnew);
+      __syncthreads();
+      small_orthogonalize(a, s, k, vnew, L);
+      small_reduce<1>(RedSet<1>{{{vnew, vnew, RK_SUMSQ}}}, n, L);
+      if (t == 0) ksp_epilogue(s, EPI_FGM_STEP, 0, L.res);
+      __syncthreads();
+      if (!small_flag(&s->inner_stop)) small_ew(OpScaleRecipDev{vnew, &s->tt, &s->hapend, aligned16(vnew)}, n);
+      __syncthreads();
+    }
+    if (t == 0 && !small_flag(&s->stop)) gmres_backsub(s, 1);
+    __syncthreads();
+    if (!small_flag(&s->skip_build)) {
+      small_ew(OpBuild{TEMP, a.prevec, n, s, aligned16(TEMP) && aligned16(a.prevec) && n % 2 == 0}, n);
+      __syncthreads();
+      small_axpy(X, 1.0, TEMP, n);
+      __syncthreads();
+    }
+    if (t == 0) gmres_cycle_end(s, 1);
+    __syncthreads();
+    if (small_flag(&s->stop)) break;
+    // the next cycle's residual: vv = -1 (A x) + b
+    small_spmv(a.c, X, TEMP);
+    __syncthreads();
+    for (i64 i = t; i < n; i += MPK_WAVE_THREADS) vv[i] = fma(-1.0, TEMP[i], a.B[i]);
+    __syncthreads();
+    if (cycle > a.c.max_it + 2) break;
+  }
+  small_state_out(a.c.s, L, true);
+}
+
+// KSPSolve_BCGS's iterations (mp_ksp_solve_bcgs's loop).
+template <int PCT>
+__global__ void __launch_bounds__(MPK_WAVE_THREADS) k_bcgs_small(SmallKsp a) {
+  __shared__ SmallKspLds L;
+  KspState *s = small_state_in(a.c.s, L, false);
+  const i64 n = a.c.n;
+  const int t = (int)threadIdx.x;
+  double *X = a.c.X, *R = a.R, *RP = a.RP, *V = a.V, *T = a.T, *Sv = a.Sv, *P = a.P;
+  for (i64 it = 0; it <= a.c.max_it + 1; it++) {
+    if (small_flag(&s->stop)) break;
+    small_reduce<1>(RedSet<1>{{{R, RP, RK_DOT}}}, n, L);
+    if (t == 0) ksp_epilogue(s, EPI_BCGS_RHO, 0, L.res);
+    __syncthreads();
+    small_ew(OpBcgsP{P, R, V, s, aligned16(P) && aligned16(R) && aligned16(V)}, n);
+    __syncthreads();
+    small_ba<PCT>(a, P, V, T, L);
+    small_reduce<1>(RedSet<1>{{{V, RP, RK_DOT}}}, n, L);
+    if (t == 0) ksp_epilogue(s, EPI_BCGS_D1, 0, L.res);
+    __syncthreads();
+    if (small_flag(&s->stop)) break;
+    small_ew(OpWaxpyDev{Sv, V, R, &s->alpha, aligned16(Sv) && aligned16(V) && aligned16(R)}, n);
+    __syncthreads();
+    small_ba<PCT>(a, Sv, T, R, L);
+    small_reduce<3>(RedSet<3>{{{Sv, T, RK_DOT}, {T, T, RK_SUMSQ}, {Sv, Sv, RK_SUMSQ}}}, n, L);
+    if (t == 0) ksp_epilogue(s, EPI_BCGS_D2, 0, L.res);
+    __syncthreads();
+    if (small_flag(&s->stop)) break;
+    small_ew(OpBcgsX{X, P, Sv, s, aligned16(X) && aligned16(P) && aligned16(Sv)}, n);
+    small_ew(OpWaxpyDev{R, T, Sv, &s->omega, aligned16(R) && aligned16(T) && aligned16(Sv)}, n);
+    __syncthreads();
+    small_reduce<1>(RedSet<1>{{{R, R, RK_SUMSQ}}}, n, L);
+    if (t == 0) ksp_epilogue(s, EPI_BCGS_DP, 0, L.res);
+    __syncthreads();
+  }
+  small_state_out(a.c.s, L, false);
+}
+
 // Whether a solve goes to the one-workgroup drivers: the tile-tree order (not
 // MPK_REDUCTION=ref), n <= MPK_SMALL_ROWS (MPK_SMALL_ROWS=k lowers it; 0
 // switches the drivers off), and a PC with two wave plans, or a PC without
 // sweeps and n <= MPK_SMALL_PLAIN (past it the general path's fused
 // launches, spread over the chip, are faster).
-#define MPK_SMALL_PLAIN 4096
 static std::atomic<int64_t> g_small_solves{0};
 static bool small_ok(const PCView &pc, i64 n, bool ref, int *f0, int *f1) {
   const char *e = std::getenv("MPK_SMALL_ROWS");
@@ -1325,6 +1717,48 @@ static bool small_ok(const PCView &pc, i64 n, bool ref, int *f0, int *f1) {
   *f1 = plan_wave_form(plan, 1);
   return *f0 >= 0 && *f1 >= 0;
 }
+
+// Whether a GMRES / FGMRES (gmres) or BiCGStab solve goes to its
+// one-workgroup driver: small_ok's rules, at most MPK_SMALL_PLAIN rows (past
+// them one workgroup's share of the chip's bandwidth makes the Gram-Schmidt
+// sums slower than the general path's launches), the left or right side, and
+// for GMRES a restart of at most MPK_SMALL_M.
+static bool small_ksp_ok(const PCView &pc, const KspState &h, i64 n, bool gmres, int *f0, int *f1) {
+  if (n > MPK_SMALL_PLAIN || h.side == PC_SYMMETRIC) return false;
+  if (gmres && (h.m < 1 || h.m > MPK_SMALL_M)) return false;
+  return small_ok(pc, n, red_ref(), f0, f1);
+}
+
+// The matrix, PC and state part of a one-workgroup solve (the sweeps'
+// scratch for ILU / IC); false when that scratch cannot be had.
+static bool small_args(SmallCG &a, i64 n, i64 max_it, const DevCSR &A, const PCView &pc, KspState *d, double *X,
+                       int f0, int f1) {
+  a = SmallCG{n, max_it, A.rp, A.ci, A.v, X, nullptr, nullptr, nullptr, nullptr, d, pc.jac_diag, pc.ilu, pc.icc,
+              f0, f1, nullptr, nullptr};
+  if (pc.type == PCT_ILU || pc.type == PCT_ICC) {
+    a.S = pc_sweep_scratch(n);
+    if (!a.S) return false;
+    a.st = static_cast<Staged *>(pc_staged_rows(a.S, n));
+  }
+  return true;
+}
+
+#define MPK_SMALL_LAUNCH(kern, pct, arg)                           \
+  do {                                                             \
+    switch (pct) {                                                 \
+      case PCT_JACOBI:                                             \
+        MPK_LAUNCH(kern<PCT_JACOBI>, 1, MPK_WAVE_THREADS, arg);    \
+        break;                                                     \
+      case PCT_ILU:                                                \
+        MPK_LAUNCH(kern<PCT_ILU>, 1, MPK_WAVE_THREADS, arg);       \
+        break;                                                     \
+      case PCT_ICC:                                                \
+        MPK_LAUNCH(kern<PCT_ICC>, 1, MPK_WAVE_THREADS, arg);       \
+        break;                                                     \
+      default:                                                     \
+        MPK_LAUNCH(kern<PCT_NONE>, 1, MPK_WAVE_THREADS, arg);      \
+    }                                                              \
+  } while (0)
 
 // Tiles up to which CG + Jacobi folds its reductions' partials in every block
 // of the kernels that use them: 2048 (4,194,304 rows); MPK_CG_FOLD_TILES=k
@@ -1567,6 +2001,15 @@ void mp_ksp_solve_bcgs(int64_t n, int64_t irow, int64_t icol, int64_t data, int6
   const int chunk = chunk_for(n);
   int slot = 0;
   bool pending = false;
+  int f0, f1;
+  SmallKsp a{};
+  if (small_ksp_ok(S.pc, S.h, n, false, &f0, &f1) && small_args(a.c, n, S.h.max_it, S.A, S.pc, S.d, X, f0, f1)) {
+    // the iterations in one launch of one workgroup
+    a.side = (int)S.h.side;
+    a.R = R, a.RP = RP, a.V = V, a.T = T, a.Sv = Sv, a.P = P;
+    MPK_SMALL_LAUNCH(k_bcgs_small, S.pc.type, a);
+    g_small_solves++;
+  } else
   for (i64 queued = 0;; queued += chunk) {
     for (int c = 0; c < chunk; c++) {
       S.reduce<1>(RedSet<1>{{{R, RP, RK_DOT}}}, EPI_BCGS_RHO, 0, st);
@@ -1659,6 +2102,18 @@ void mp_ksp_solve_gmres(int64_t n, int64_t irow, int64_t icol, int64_t data, int
   const i64 m = S.h.m;
   const int64_t *st = S.stop, *inner = S.inner;
   i64 guess_zero = S.h.guess_zero;
+  int f0, f1;
+  SmallKsp a{};
+  if (small_ksp_ok(S.pc, S.h, n, true, &f0, &f1) && small_args(a.c, n, S.h.max_it, S.A, S.pc, S.d, X, f0, f1)) {
+    // the restart cycles in one launch of one workgroup
+    a.side = (int)S.h.side;
+    a.m = m, a.orthog = S.h.orthog, a.cgstype = S.h.cgstype, a.guess_zero = guess_zero;
+    a.vv = vv, a.TEMP = TEMP, a.TMP = TEMP_MATOP, a.B = B;
+    MPK_SMALL_LAUNCH(k_gmres_small, S.pc.type, a);
+    g_small_solves++;
+    S.finish_out((double *)cfg, (int64_t *)cfgi);
+    return;
+  }
   for (i64 cycle = 0;; cycle++) {
     S.initial_residual(guess_zero, X, TEMP, TEMP_MATOP, vv, B, st);
     S.reduce<1>(RedSet<1>{{{vv, vv, RK_SUMSQ}}}, EPI_GM_START, 0, st);
@@ -1704,6 +2159,18 @@ void mp_ksp_solve_fgmres(int64_t n, int64_t irow, int64_t icol, int64_t data, in
     vwaxpy(vv, -1.0, TEMP, B, n, nullptr);
   } else {
     vcopy(B, vv, n);
+  }
+  int f0, f1;
+  SmallKsp a{};
+  if (small_ksp_ok(S.pc, S.h, n, true, &f0, &f1) && small_args(a.c, n, S.h.max_it, S.A, S.pc, S.d, X, f0, f1)) {
+    // the restart cycles in one launch of one workgroup
+    a.side = (int)S.h.side, a.flexible = 1;
+    a.m = m, a.orthog = S.h.orthog, a.cgstype = S.h.cgstype;
+    a.vv = vv, a.prevec = prevec, a.TEMP = TEMP, a.B = B;
+    MPK_SMALL_LAUNCH(k_fgmres_small, S.pc.type, a);
+    g_small_solves++;
+    S.finish_out((double *)cfg, (int64_t *)cfgi);
+    return;
   }
   for (i64 cycle = 0;; cycle++) {
     S.reduce<1>(RedSet<1>{{{vv, vv, RK_SUMSQ}}}, EPI_FGM_START, 0, st);
