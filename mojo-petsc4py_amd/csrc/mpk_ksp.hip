@@ -1338,6 +1338,7 @@ __global__ void __launch_bounds__(MPK_WAVE_THREADS) k_cg_small(SmallCG a) {
 // that the epilogues' serial steps in thread 0 touch LDS, not global memory.
 struct SmallKspLds : SmallLdsT<MPK_SMALL_PLAIN> {
   double mres[MPK_SMALL_M + 1];
+  double mw[MPK_SMALL_M + 1][2][4];  // the Gram-Schmidt sums' wave sums per tile (n <= 4096: two tiles)
   KspState ks;
   double HH[(MPK_SMALL_M + 2) * (MPK_SMALL_M + 1)], HES[(MPK_SMALL_M + 1) * (MPK_SMALL_M + 1)];
   double CC[MPK_SMALL_M + 1], SS[MPK_SMALL_M + 1], RS[MPK_SMALL_M + 2], LHH[MPK_SMALL_M + 1];
@@ -1434,16 +1435,18 @@ __device__ __forceinline__ void small_ba(const SmallKsp &a, const double *x, dou
   __syncthreads();
 }
 
-// x . V_j for G vectors from v0 (leading dimension ld) into L.mres + j0:
-// k_partials_mdot's tile partials (x's pairs loaded once for the G vectors)
-// and small_reduce's fold, which is k_finalize's.  Every thread calls it.
+// The tile sums of x . V_j for G vectors from v0 (leading dimension ld)
+// into L.mw[j0..]: k_partials_mdot's per-thread pairs (x's pairs loaded once
+// for the G vectors) and wave trees.
+// `h` is the half of the workgroup (256 threads, a block of the general
+// path) that takes the tiles: both halves when they split the tiles, one
+// when they split the groups.
 template <int G>
-__device__ __forceinline__ void small_mdot_g(const double *x, const double *v0, i64 ld, int j0, i64 n,
-                                             SmallKspLds &L) {
+__device__ __forceinline__ void small_mdot_tiles(const double *x, const double *v0, i64 ld, int j0, i64 n, int T,
+                                                 int h0, int hs, SmallKspLds &L) {
   const int t = (int)threadIdx.x, tl = t & 255, lane = t & 63, wv = (t >> 6) & 3;
-  const int T = (int)((n + MPK_TILE - 1) / MPK_TILE);
   const bool al = aligned16(x) && aligned16(v0) && ld % 2 == 0;
-  for (int j = t >> 8; j < T; j += 2) {  // block t >> 8 takes every other tile
+  for (int j = (t >> 8) - h0; j >= 0 && j < T; j += hs) {
     const i64 base = (i64)j * MPK_TILE + 2 * tl;
     D2 xs[4];
 #pragma unroll
@@ -1463,36 +1466,44 @@ __device__ __forceinline__ void small_mdot_g(const double *x, const double *v0, 
 #pragma unroll
     for (int g = 0; g < G; g++) {
       const double w = wave_tree(u[g]);
-      if (lane == 0) L.wsum[g][j][wv] = w;
+      if (lane == 0) L.mw[j0 + g][j][wv] = w;
     }
-  }
-  __syncthreads();
-  if (t < 256) {
-#pragma unroll
-    for (int g = 0; g < G; g++) {
-      const double u = t < T ? (L.wsum[g][t][0] + L.wsum[g][t][2]) + (L.wsum[g][t][1] + L.wsum[g][t][3]) : 0.0;
-      const double r = wave_tree(u);
-      if (lane == 0) L.fsum[g][wv] = r;
-    }
-  }
-  __syncthreads();
-  if (t == 0) {
-#pragma unroll
-    for (int g = 0; g < G; g++) L.mres[j0 + g] = (L.fsum[g][0] + L.fsum[g][2]) + (L.fsum[g][1] + L.fsum[g][3]);
   }
 }
-// x . V_j, j < nv, into L.mres, four sums at a time; ends with a barrier.
+// x . V_j, j < nv, into L.mres, n <= MPK_SMALL_PLAIN (at most two tiles):
+// every sum's tile partials (block_tree's (W0 + W2) + (W1 + W3) of its wave
+// sums), then k_finalize's fold of the T partials -- thread t < 256 holds
+// P_t (0.0 past T), each wave sums its lanes by the halving tree, the result
+// is (F0 + F2) + (F1 + F3).  With T <= 64 the waves past the first sum
+// zeros (F1 = F2 = F3 = +0.0), so a sum's fold is one wave's tree of
+// P_0, .., P_{T-1} and (F0 + 0.0) + (0.0 + 0.0): the same arithmetic, done
+// by one wave per sum, the waves taking different sums.  Ends with a barrier.
 __device__ __forceinline__ void small_mdot(const double *x, const double *V, i64 ld, int nv, i64 n,
                                            SmallKspLds &L) {
+  const int T = (int)((n + MPK_TILE - 1) / MPK_TILE);
+  // one tile: the halves take alternate groups of four; two: one tile each
+  const int half = (int)threadIdx.x >> 8;
   int j0 = 0;
-  for (; j0 + 4 <= nv; j0 += 4) small_mdot_g<4>(x, V + (i64)j0 * ld, ld, j0, n, L);
+  for (int gi = 0; j0 + 4 <= nv; j0 += 4, gi++)
+    if (T == 2 || (gi & 1) == half) small_mdot_tiles<4>(x, V + (i64)j0 * ld, ld, j0, n, T, T == 2 ? 0 : half, T == 2 ? 2 : 1, L);
   const double *v0 = V + (i64)j0 * ld;
-  if (nv - j0 == 3)
-    small_mdot_g<3>(x, v0, ld, j0, n, L);
-  else if (nv - j0 == 2)
-    small_mdot_g<2>(x, v0, ld, j0, n, L);
-  else if (nv - j0 == 1)
-    small_mdot_g<1>(x, v0, ld, j0, n, L);
+  const int gl = nv / 4;  // the tail's group index
+  if (T == 2 || (gl & 1) == half) {
+    const int h0 = T == 2 ? 0 : half, hs = T == 2 ? 2 : 1;
+    if (nv - j0 == 3)
+      small_mdot_tiles<3>(x, v0, ld, j0, n, T, h0, hs, L);
+    else if (nv - j0 == 2)
+      small_mdot_tiles<2>(x, v0, ld, j0, n, T, h0, hs, L);
+    else if (nv - j0 == 1)
+      small_mdot_tiles<1>(x, v0, ld, j0, n, T, h0, hs, L);
+  }
+  __syncthreads();
+  const int lane = (int)threadIdx.x & 63;
+  for (int g = (int)threadIdx.x >> 6; g < nv; g += MPK_WAVE_THREADS / 64) {
+    const double u = lane < T ? (L.mw[g][lane][0] + L.mw[g][lane][2]) + (L.mw[g][lane][1] + L.mw[g][lane][3]) : 0.0;
+    const double f0 = wave_tree(u);
+    if (lane == 0) L.mres[g] = (f0 + 0.0) + (0.0 + 0.0);
+  }
   __syncthreads();
 }
 
@@ -1517,8 +1528,36 @@ __device__ __forceinline__ void small_orthogonalize(const SmallKsp &a, KspState 
   for (int pass = 0; pass < 2; pass++) {
     if (pass == 1 && small_flag(&s->skip_ref)) break;
     small_mdot(vnew, a.vv, n, nv, n, L);
-    if (t == 0) ksp_epilogue(s, pass == 0 ? EPI_GM_CGS_A : EPI_GM_CGS_B, 0, L.mres);
-    __syncthreads();
+    // EPI_GM_CGS_A / EPI_GM_CGS_B with each entry j <= it on thread j: the
+    // same operations per entry (HH(j) = 0.0 - LHH(j) in the first pass, as
+    // zeroing then subtracting gives), then the serial norm in thread 0
+    {
+      const i64 itc = s->it, m = s->m;
+      if (t <= itc) {
+        const double l = -L.mres[t];
+        s->LHH[t] = l;
+        double *hh = s->HH + itc * (m + 2) + t, *he = s->HES + itc * (m + 1) + t;
+        if (pass == 0) {
+          *hh = 0.0 - l;
+          *he = 0.0 - l;
+        } else {
+          *hh = *hh - l;
+          *he = *he - l;
+        }
+      }
+      __syncthreads();
+      if (t == 0 && pass == 0) {
+        if (s->cgstype == 1) {
+          double hnrm = 0.0;
+          for (i64 j = 0; j <= itc; j++) hnrm = fma(s->LHH[j], s->LHH[j], hnrm);
+          s->hnrm = sqrt(hnrm);
+          s->skip_ref = 1;  // decided after the norm of the projected vector
+        } else {
+          s->skip_ref = s->cgstype == 2 ? 0 : 1;
+        }
+      }
+      __syncthreads();
+    }
     // vmaxpy_dev(vnew, vv, n, LHH, nv, 1.0): each element's chain over the nv
     // vectors, eight vectors' loads issued before their multiply-adds
     const double *coef = s->LHH;
