@@ -323,7 +323,9 @@ def bench_batched(lib, sample: int = 16, cpu: bool = False):
     from mpk.batch import BatchSolver
     systems = W.batch_systems(count=4096)
     bs = BatchSolver(systems)
-    tune = bs.autotune(candidates=((256, 1024), (512, 256), (512, 512)))
+    # workgroup and grid, then the tiles (rows per wavefront, product variant,
+    # reduction tiling) at the fastest workgroup size, timed on every 4th system
+    tune = bs.autotune(candidates=((256, 512), (256, 1024), (512, 256), (512, 512)), sample_every=4)
     bs.solve(graph=True)
     t0 = time.perf_counter()
     _, st = bs.solve(graph=True)
@@ -379,7 +381,8 @@ def bench_batched(lib, sample: int = 16, cpu: bool = False):
                         " FP32 inner + FP64 refinement to 1e-8",
             "sum_n": int(sum(len(s_[0]) - 1 for s_ in systems)), "seconds": t, "solves_per_s": len(systems) / t,
             "converged": int(st[:, 3].sum()), "max_relres": float(st[:, 0].max()),
-            "autotuned": {"workgroup": tune["workgroup"], "grid": tune["grid"]},
+            "autotuned": {k: tune[k] for k in ("workgroup", "grid", "rows_per_wavefront", "spmv_variant",
+                                               "reduction_tiling", "sampled_systems", "timings")},
             "one_at_a_time_fp64": {"sample": sample, "ms_per_solve": per * 1e3, "solves_per_s": 1.0 / per},
             "speedup": per * len(systems) / t, "cpu_baseline": cpu_line}
 

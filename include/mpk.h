@@ -343,14 +343,24 @@ void mpk_mixed_free(int64_t handle);
  * the arrays alive.
  * mpk_batch_solve: x [D] from b [D], stats [D] float64[nsys][4] = {relres,
  * outer, inner, converged}; cfg [H] float64 {rtol, inner_rtol, max_outer,
- * max_inner, workgroup (128/256/512), grid, use_graph}: with use_graph the
- * launch is captured once into a HIP graph and replayed.  0 or -1. */
+ * max_inner, workgroup (128/256/512), grid, use_graph, variant}: variant an
+ * index into mpk_batch_variants' list whose workgroup is cfg[4], or -1 for
+ * the default tiles at that workgroup; with use_graph the launch is captured
+ * once into a HIP graph and replayed (again when the variant, grid or
+ * arguments change).  0 or -1.
+ * mpk_batch_variants: the compiled launch variants, four int64 each into out
+ * (at most cap of them): workgroup size, rows a thread takes per pass of the
+ * product (a wave covers 64 times that), the product's entries loaded per
+ * round, and the elements a thread takes per step of the Gram-Schmidt sums;
+ * returns their count.  No counterpart in the reference (its solves are
+ * serial CPU loops); the autotuner (mpk/batch.py) times them. */
 int64_t mpk_batch_create(int64_t nsys, int64_t sys, int64_t order, int64_t rp, int64_t ci, int64_t v64, int64_t v32,
                          int64_t jd32, int64_t fprow, int64_t fcnt, int64_t fdep, int64_t fcoef, int64_t flv,
                          int64_t bprow, int64_t bcnt, int64_t bdep, int64_t bcoef, int64_t blv, int64_t dinv32,
                          int64_t f2b, int64_t nmax);
 int64_t mpk_batch_solve(int64_t handle, int64_t b, int64_t x, int64_t stats, int64_t cfg);
 void mpk_batch_free(int64_t handle);
+int64_t mpk_batch_variants(int64_t out, int64_t cap);
 
 #ifdef __cplusplus
 }

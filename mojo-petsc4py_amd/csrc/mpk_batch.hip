@@ -70,7 +70,14 @@ __device__ __forceinline__ double wg_sum(double v, double *sh, int &par) {
   return s;
 }
 
-template <int BS>
+// A system's solve by one workgroup of BS threads.  The tile parameters are
+// the autotuner's choices: UR, the rows a thread takes per pass of the FP32
+// product (a wave covers 64 UR rows a pass); KB, the product's variant --
+// the entries of a row loaded per round before their gathers (a 5-point row
+// takes two rounds at 4, one at 8); UG, the reduction tiling of the
+// Gram-Schmidt sums and updates -- the elements a thread takes per step for
+// four basis vectors at once.
+template <int BS, int UR_, int KB_, int UG_>
 struct Sys {
   const BatchArgs &a;
   i64 row0, n;
@@ -83,7 +90,7 @@ struct Sys {
   // so a thread waits about one memory latency per group rather than one
   // per element (the order of the arithmetic is the element order).
   static constexpr int U = 8;
-  static constexpr int UG = 4;  // the same for the Gram-Schmidt passes (four basis vectors each)
+  static constexpr int UG = UG_;  // the same for the Gram-Schmidt passes (four basis vectors each)
   __device__ double dot(const float *__restrict__ x, const float *__restrict__ y) {
     double s = 0.0;
     for (i64 i0 = t; i0 < n; i0 += U * BS) {
@@ -104,7 +111,7 @@ struct Sys {
   // entries KB at a time -- every index and value of a block loaded first,
   // then every gathered x, so a block costs two memory round trips (each
   // row's sum still in entry order)
-  static constexpr int UR = 4, KB = 4;
+  static constexpr int UR = UR_, KB = KB_;
   __device__ void spmv32(const float *__restrict__ x, float *__restrict__ y) const {
     for (i64 i0 = t; i0 < n; i0 += UR * BS) {
       int j0[UR], len[UR], lmax = 0;
@@ -364,7 +371,7 @@ struct Sys {
   }
 };
 
-template <int BS>
+template <int BS, int UR, int KB, int UG>
 __global__ void __launch_bounds__(BS) k_batch(BatchArgs a) {
   __shared__ double sh[2 * (BS / 64) + 1];  // wg_sum's two buffers, then a broadcast slot
   __shared__ double H[(BS_M + 1) * BS_M], cs[BS_M], sn[BS_M], g[BS_M + 1], yv[BS_M];
@@ -382,7 +389,7 @@ __global__ void __launch_bounds__(BS) k_batch(BatchArgs a) {
     if (q >= a.nsys) return;
     const i64 s = a.order[q];
     const i64 *d = a.sys + s * BS_FIELDS;
-    Sys<BS> S{a, d[0], d[1], sh, t};
+    Sys<BS, UR, KB, UG> S{a, d[0], d[1], sh, t};
     const i64 n = S.n, row0 = S.row0;
     const int kind = (int)d[2];
     // local FP32 vectors
@@ -557,11 +564,38 @@ __global__ void __launch_bounds__(BS) k_batch(BatchArgs a) {
 
 __global__ void k_batch_reset(unsigned *q) { *q = 0u; }
 
+// The compiled launch variants (workgroup, UR, KB, UG), in the order
+// mpk_batch_variants lists them: every tile choice at 512 threads (the
+// fastest workgroup size on the batched workload), the default tiles
+// (4, 4, 4) at 128 and 256.
+#define MPK_BATCH_VARIANTS(X)                                                                                   \
+  X(512, 4, 4, 4) X(512, 2, 4, 4) X(512, 8, 4, 4) X(512, 2, 8, 4) X(512, 4, 8, 4) X(512, 8, 8, 4)            \
+  X(512, 2, 4, 2) X(512, 4, 4, 2) X(512, 8, 4, 2) X(512, 2, 8, 2) X(512, 4, 8, 2) X(512, 8, 8, 2)            \
+  X(512, 2, 4, 8) X(512, 4, 4, 8) X(512, 8, 4, 8) X(512, 2, 8, 8) X(512, 4, 8, 8) X(512, 8, 8, 8)            \
+  X(128, 4, 4, 4) X(256, 4, 4, 4)
+struct BatchVariant {
+  int bs, ur, kb, ug;
+};
+static const BatchVariant kBatchVariants[] = {
+#define MPK_BV_ROW(bs, ur, kb, ug) {bs, ur, kb, ug},
+    MPK_BATCH_VARIANTS(MPK_BV_ROW)
+#undef MPK_BV_ROW
+};
+static const int kNumBatchVariants = (int)(sizeof kBatchVariants / sizeof kBatchVariants[0]);
+// The variant with the default tiles at workgroup size bs, or -1.
+static int batch_default_variant(int bs) {
+  for (int v = 0; v < kNumBatchVariants; v++)
+    if (kBatchVariants[v].bs == bs && kBatchVariants[v].ur == 4 && kBatchVariants[v].kb == 4 &&
+        kBatchVariants[v].ug == 4)
+      return v;
+  return -1;
+}
+
 struct Batch {
   BatchArgs a{};
   i64 nmax = 0;
   int grid = 0;  // the workgroups the work buffers are sized for
-  int cap_bs = 0, cap_grid = 0;  // the launch shape the graph was captured with
+  int cap_variant = -1, cap_grid = 0;  // the launch the graph was captured with
   unsigned *queue = nullptr;
   float *work = nullptr;
   double *work64 = nullptr;
@@ -591,18 +625,16 @@ struct Batch {
     a.work64 = work64;
     return !failed();
   }
-  void launch(int bs_, int g) {
+  void launch(int variant, int g) {
     MPK_LAUNCH(k_batch_reset, 1, 1, queue);
-    switch (bs_) {
-      case 128:
-        MPK_LAUNCH(k_batch<128>, g, 128, a);
-        break;
-      case 512:
-        MPK_LAUNCH(k_batch<512>, g, 512, a);
-        break;
-      default:
-        MPK_LAUNCH(k_batch<256>, g, 256, a);
-    }
+    int v = 0;
+#define MPK_BV_LAUNCH(bs, ur, kb, ug)                            \
+  if (variant == v++) {                                          \
+    MPK_LAUNCH((k_batch<bs, ur, kb, ug>), g, bs, a);             \
+    return;                                                      \
+  }
+    MPK_BATCH_VARIANTS(MPK_BV_LAUNCH)
+#undef MPK_BV_LAUNCH
   }
 };
 
@@ -654,10 +686,24 @@ int64_t mpk_batch_create(int64_t nsys, int64_t sys, int64_t order, int64_t rp, i
   return (int64_t)B;
 }
 
+// See include/mpk.h.
+int64_t mpk_batch_variants(int64_t out, int64_t cap) {
+  int64_t *o = (int64_t *)out;
+  for (int v = 0; v < kNumBatchVariants && v < cap; v++) {
+    o[4 * v] = kBatchVariants[v].bs;
+    o[4 * v + 1] = kBatchVariants[v].ur;
+    o[4 * v + 2] = kBatchVariants[v].kb;
+    o[4 * v + 3] = kBatchVariants[v].ug;
+  }
+  return kNumBatchVariants;
+}
+
 // One batched solve: x [D] (concatenated) from b [D]; stats [D] float64
 // [nsys][4] = {relres, outer, inner, converged}.  cfg [H] float64 in {rtol,
 // inner_rtol, max_outer, max_inner, workgroup size (128/256/512), grid
-// (workgroups), use_graph}.  With use_graph the queue reset and the
+// (workgroups), use_graph, variant}: variant an index into
+// mpk_batch_variants' list (its workgroup size must be cfg[4]), or -1 for
+// the default tiles at that size.  With use_graph the queue reset and the
 // persistent kernel are captured once into a HIP graph and replayed.
 // Returns 0, or -1 on error.
 int64_t mpk_batch_solve(int64_t h, int64_t b, int64_t x, int64_t stats, int64_t cfg_) {
@@ -678,6 +724,11 @@ int64_t mpk_batch_solve(int64_t h, int64_t b, int64_t x, int64_t stats, int64_t 
     set_error(7, "batch: workgroup size must be 128, 256 or 512");
     return -1;
   }
+  const int variant = cfg[7] < 0.0 ? batch_default_variant(bs) : (int)cfg[7];
+  if (variant < 0 || variant >= kNumBatchVariants || kBatchVariants[variant].bs != bs) {
+    set_error(7, "batch: variant %d is not a compiled variant at workgroup size %d", variant, bs);
+    return -1;
+  }
   if (g < 1 || g > 65536) {
     set_error(7, "batch: grid must be 1..65536");
     return -1;
@@ -685,7 +736,7 @@ int64_t mpk_batch_solve(int64_t h, int64_t b, int64_t x, int64_t stats, int64_t 
   if (!B->size_work(g)) return -1;
   if (graph) {
     // re-capture when the launch shape or any argument changes
-    if (B->exec && (B->cap_bs != bs || B->cap_grid != g || std::memcmp(&B->captured, &a, sizeof a) != 0)) {
+    if (B->exec && (B->cap_variant != variant || B->cap_grid != g || std::memcmp(&B->captured, &a, sizeof a) != 0)) {
       HIPCHK(hipGraphExecDestroy(B->exec));
       HIPCHK(hipGraphDestroy(B->graph));
       B->exec = nullptr;
@@ -693,17 +744,17 @@ int64_t mpk_batch_solve(int64_t h, int64_t b, int64_t x, int64_t stats, int64_t 
     }
     if (!B->exec) {
       HIPCHK(hipStreamBeginCapture(stream(), hipStreamCaptureModeThreadLocal));
-      B->launch(bs, g);
+      B->launch(variant, g);
       HIPCHK(hipStreamEndCapture(stream(), &B->graph));
       if (failed()) return -1;
       HIPCHK(hipGraphInstantiate(&B->exec, B->graph, nullptr, nullptr, 0));
-      B->cap_bs = bs;
+      B->cap_variant = variant;
       B->cap_grid = g;
       B->captured = a;
     }
     HIPCHK(hipGraphLaunch(B->exec, stream()));
   } else {
-    B->launch(bs, g);
+    B->launch(variant, g);
   }
   sync();
   return failed() ? -1 : 0;

@@ -113,6 +113,7 @@ SIGNATURES = {
     "mpk_batch_create": ([I] * 21, I),
     "mpk_batch_solve": ([I] * 5, I),
     "mpk_batch_free": ([I], None),
+    "mpk_batch_variants": ([I, I], I),
     "mpk_mixed_create": ([I] * 13, I),
     "mpk_mixed_solve": ([I] * 4, I),
     "mpk_mixed_free": ([I], None),

@@ -3,7 +3,7 @@ launch.
 
     from mpk.batch import BatchSolver
     bs = BatchSolver(systems)            # [(indptr, indices, data, b, "cg+jacobi" | "gmres+ilu"), ...]
-    cfg = bs.autotune()                  # workgroup size and grid, timed on this batch
+    cfg = bs.autotune()                  # workgroup, grid and tiles, timed on (a sample of) this batch
     x, stats = bs.solve(rtol=1e-8)
 
 Each workgroup of the persistent kernel takes systems from a queue (largest
@@ -217,6 +217,7 @@ class BatchSolver:
 
     def __init__(self, systems):
         L = lib()
+        self._systems = systems
         H = build_arrays(systems, L)
         self.count = len(systems)
         self.sizes = H["sizes"]
@@ -239,15 +240,36 @@ class BatchSolver:
         check()
         if not self._h:
             raise RuntimeError("mpk_batch_create failed")
-        self.config = {"workgroup": 256, "grid": 512}
+        self.config = {"workgroup": 256, "grid": 512, "variant": -1}
+
+    @staticmethod
+    def variants() -> list:
+        """The compiled launch variants: dicts of workgroup, rows_per_thread
+        (the product's rows a thread takes per pass; a wavefront covers 64
+        times that), spmv_block (the product's entries loaded per round) and
+        gs_tiling (elements a thread takes per step of the Gram-Schmidt
+        sums), indexed as solve(variant=...) takes them."""
+        L = lib()
+        buf = np.zeros(4 * 64, dtype=np.int64)
+        k = L.mpk_batch_variants(_a(buf), 64)
+        return [{"workgroup": int(buf[4 * v]), "rows_per_thread": int(buf[4 * v + 1]),
+                 "spmv_block": int(buf[4 * v + 2]), "gs_tiling": int(buf[4 * v + 3])} for v in range(k)]
 
     def solve(self, rtol: float = 1e-8, inner_rtol: float = 1e-3, max_outer: int = 30, max_inner: int = 4000,
-              workgroup: int | None = None, grid: int | None = None, graph: bool = True):
+              workgroup: int | None = None, grid: int | None = None, graph: bool = True,
+              variant: int | None = None):
         """Solves every system; returns (x, stats) with x the concatenated
-        FP64 solutions and stats[s] = (relres, outer, inner, converged)."""
-        wg = workgroup or self.config["workgroup"]
+        FP64 solutions and stats[s] = (relres, outer, inner, converged).
+        variant: an index into variants() (its workgroup size is used), or
+        -1 for the default tiles at the workgroup size."""
+        if variant is None:
+            variant = self.config["variant"] if workgroup is None else -1
+        if variant >= 0:
+            wg = self.variants()[variant]["workgroup"]
+        else:
+            wg = workgroup or self.config["workgroup"]
         g = grid or self.config["grid"]
-        cfg = np.array([rtol, inner_rtol, max_outer, max_inner, wg, g, 1.0 if graph else 0.0, 0.0])
+        cfg = np.array([rtol, inner_rtol, max_outer, max_inner, wg, g, 1.0 if graph else 0.0, float(variant)])
         d = self._d
         rc = lib().mpk_batch_solve(self._h, d["b"].ptr, d["x"].ptr, d["stats"].ptr, _a(cfg))
         check()
@@ -259,20 +281,50 @@ class BatchSolver:
         return x[self.row0[s]:self.row0[s + 1]]
 
     def autotune(self, candidates=((128, 512), (256, 256), (256, 512), (256, 1024), (512, 256), (512, 512)),
-                 reps: int = 1, **solve_kw) -> dict:
-        """Times each (workgroup size, grid) on this batch and keeps the
-        fastest; returns {"workgroup", "grid", "timings"}."""
-        timings = {}
-        for wg, g in candidates:
-            self.solve(workgroup=wg, grid=g, **solve_kw)  # warm-up (and graph capture)
+                 reps: int = 1, tiles: bool = True, sample_every: int = 1, **solve_kw) -> dict:
+        """Chooses the launch for this batch by timing solves: first each
+        (workgroup size, grid) of `candidates` with the default tiles, then,
+        at the fastest of those, every compiled tile variant for that
+        workgroup size -- the rows a thread (so a wavefront) takes per pass
+        of the product, the product's entry block (its variant), and the
+        Gram-Schmidt reduction tiling.  With sample_every = k the timing runs
+        on every k-th system (a smaller batch of the same mix) and the choice
+        is applied to this one.  Returns the choice and every timing."""
+        tb = self if sample_every <= 1 else BatchSolver(self._systems[::sample_every])
+        var = self.variants()
+
+        def timed(**kw):
+            tb.solve(**kw, **solve_kw)  # warm-up (and graph capture)
             t0 = time.perf_counter()
             for _ in range(reps):
-                self.solve(workgroup=wg, grid=g, **solve_kw)
-            timings[(wg, g)] = (time.perf_counter() - t0) / reps
-        best = min(timings, key=timings.get)
-        self.config = {"workgroup": best[0], "grid": best[1]}
-        return {"workgroup": best[0], "grid": best[1],
-                "timings": {f"{k[0]}x{k[1]}": v for k, v in timings.items()}}
+                tb.solve(**kw, **solve_kw)
+            return (time.perf_counter() - t0) / reps
+
+        try:
+            timings = {}
+            for wg, g in candidates:
+                timings[f"{wg}x{g}"] = timed(workgroup=wg, grid=g, variant=-1)
+            best_wg, best_g = min(candidates, key=lambda c: timings[f"{c[0]}x{c[1]}"])
+            best_t, best_v = timings[f"{best_wg}x{best_g}"], -1
+            if tiles:
+                for v, d in enumerate(var):
+                    if d["workgroup"] != best_wg or (d["rows_per_thread"], d["spmv_block"], d["gs_tiling"]) == (4, 4, 4):
+                        continue
+                    t = timed(grid=best_g, variant=v)
+                    timings[f"{best_wg}x{best_g}/ur{d['rows_per_thread']}-kb{d['spmv_block']}-ug{d['gs_tiling']}"] = t
+                    if t < best_t:
+                        best_t, best_v = t, v
+        finally:
+            if tb is not self:
+                tb.destroy()
+        chosen = var[best_v] if best_v >= 0 else {"workgroup": best_wg, "rows_per_thread": 4, "spmv_block": 4,
+                                                   "gs_tiling": 4}
+        self.config = {"workgroup": best_wg, "grid": best_g, "variant": best_v}
+        return {"workgroup": best_wg, "grid": best_g, "variant": best_v,
+                "rows_per_wavefront": 64 * chosen["rows_per_thread"],
+                "spmv_variant": f"{chosen['spmv_block']} entries per round",
+                "reduction_tiling": chosen["gs_tiling"], "timings": timings,
+                "sampled_systems": tb.count if tb is not self else self.count}
 
     def destroy(self) -> None:
         if getattr(self, "_h", None):

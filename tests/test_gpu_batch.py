@@ -69,10 +69,50 @@ def test_graph_replays_match_eager(batch):
 
 def test_autotune_picks_a_timed_candidate(batch):
     systems, bs = batch
-    cfg = bs.autotune(candidates=((128, 32), (256, 32)))
+    cfg = bs.autotune(candidates=((128, 32), (256, 32)), tiles=False)
     assert (cfg["workgroup"], cfg["grid"]) in ((128, 32), (256, 32))
     assert len(cfg["timings"]) == 2
     x, st = bs.solve(rtol=1e-8)
+    assert np.all(st[:, 3] == 1.0)
+
+
+def test_tile_variants_give_the_same_bits(batch):
+    """The tile variants change only how loads are grouped (rows per pass,
+    entries per round, elements per Gram-Schmidt step), never the order of
+    any sum: at one workgroup size and grid every variant gives the default
+    tiles' bits, and each system meets the tolerance."""
+    from mpk.batch import BatchSolver
+    systems, bs = batch
+    var = BatchSolver.variants()
+    assert len(var) >= 20
+    assert {(d["rows_per_thread"], d["spmv_block"], d["gs_tiling"]) for d in var if d["workgroup"] == 512} == {
+        (ur, kb, ug) for ur in (2, 4, 8) for kb in (4, 8) for ug in (2, 4, 8)}
+    ref, st0 = bs.solve(rtol=1e-8, workgroup=512, grid=32, graph=False, variant=-1)
+    assert np.all(st0[:, 3] == 1.0)
+    for v, d in enumerate(var):
+        if d["workgroup"] != 512:
+            continue
+        print(f"[batch] variant {v}: {d}", flush=True)
+        x, st = bs.solve(rtol=1e-8, grid=32, graph=False, variant=v)
+        assert np.array_equal(x.view(np.int64), ref.view(np.int64)), d
+        assert np.array_equal(st, st0), d
+    for v, d in enumerate(var):
+        if d["workgroup"] != 512:
+            x, st = bs.solve(rtol=1e-8, grid=16, graph=True, variant=v)
+            assert np.all(st[:, 3] == 1.0) and np.max(st[:, 0]) <= 1e-8, d
+
+
+def test_autotune_chooses_tiles(batch):
+    """The autotuner times every tile variant of the fastest workgroup size
+    (on a sample of the systems) and reports the three choices."""
+    systems, bs = batch
+    cfg = bs.autotune(candidates=((512, 32),), sample_every=2)
+    assert cfg["workgroup"] == 512 and cfg["grid"] == 32
+    assert len(cfg["timings"]) == 18 and cfg["sampled_systems"] == 24
+    assert cfg["rows_per_wavefront"] in (128, 256, 512)
+    assert cfg["reduction_tiling"] in (2, 4, 8)
+    assert cfg["spmv_variant"] in ("4 entries per round", "8 entries per round")
+    x, st = bs.solve(rtol=1e-8)  # with the chosen variant
     assert np.all(st[:, 3] == 1.0)
 
 
@@ -81,3 +121,5 @@ def test_bad_config_is_rejected(batch):
     systems, bs = batch
     with pytest.raises((RuntimeError, NativeError)):
         bs.solve(workgroup=100, grid=4)
+    with pytest.raises((RuntimeError, NativeError, IndexError)):
+        bs.solve(grid=4, variant=1000)
