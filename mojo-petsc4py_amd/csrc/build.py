@@ -27,6 +27,7 @@ SOURCES = [
     "mpk_vec.hip",
     "mpk_mat.hip",
     "mpk_pc.hip",
+    "mpk_plan.hip",
     "mpk_ksp.hip",
     "mpk_mixed.hip",
     "mpk_batch.hip",

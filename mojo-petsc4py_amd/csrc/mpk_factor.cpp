@@ -874,3 +874,19 @@ blocks:
 }
 
 }  // extern "C"
+
+namespace mpk {
+
+// The planner's shape rules, shared with the device planner (mpk_plan.hip),
+// which must lay plans out exactly as mpk_sweep_plan does.
+int64_t plan_rows_rule(int64_t n, int64_t bmax) { return plan_rows(n, bmax); }
+int64_t plan_w_rule(int64_t widest, int64_t n, int64_t m) { return plan_w(widest, n, m); }
+int64_t plan_bytes_rule(int64_t n, int64_t nblk, int64_t nlvl, int64_t slots) {
+  return plan_bytes(n, nblk, nlvl, slots);
+}
+int plan_mode_rule() { return plan_mode(); }
+int64_t plan_ring_width_rule() { return kRingWidth; }
+bool plan_env_off(const char *name) { return env_off(name); }
+int64_t wave_rec_bytes_rule(int64_t w) { return wave_rec_bytes(w); }
+
+}  // namespace mpk

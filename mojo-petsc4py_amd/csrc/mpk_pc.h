@@ -81,6 +81,35 @@ int plan_wave_form(const char *plan, int second);
 double *pc_sweep_scratch(i64 n);
 void *pc_staged_rows(double *S, i64 n);
 
+// The host planner's shape rules (mpk_factor.cpp): rows per block, fixed
+// slots per position, plan bytes, the MPK_SWEEP_PLAN mode (0 auto, 1 blocks,
+// 2 ring), the ring's level-width limit, an environment switch set to "0",
+// and a wave record's bytes.
+int64_t plan_rows_rule(int64_t n, int64_t bmax);
+int64_t plan_w_rule(int64_t widest, int64_t n, int64_t m);
+int64_t plan_bytes_rule(int64_t n, int64_t nblk, int64_t nlvl, int64_t slots);
+int plan_mode_rule();
+int64_t plan_ring_width_rule();
+bool plan_env_off(const char *name);
+int64_t wave_rec_bytes_rule(int64_t w);
+
+// A sweep's entries on the device, in each row's own order: off [n+1] int64,
+// col int32, coef float64 (m entries), and the longest row.
+struct DevEntries {
+  i64 n, m;
+  const i64 *off;
+  const int *col;
+  const double *coef;
+  i64 widest;
+};
+// The plan of one sweep built on the device into dst [D] (cap bytes),
+// byte for byte mpk_sweep_plan's, for the block form.  Returns its bytes
+// (and depth, longest row, widest level, hdr[7] through the pointers), 0
+// when the sweep takes a ring plan (which the host planner builds), -1 on
+// error.
+i64 device_sweep_plan(const DevEntries &E, bool forward, i64 bmax, char *dst, i64 cap, i64 *depth,
+                      int *longest, int *widest_level, int *mark);
+
 // Every entry of sweep `which` (0: forward / first, 1: second) of a PC with
 // sweeps, in each row's own order, on the host: off[n+1], col, coef.
 bool pc_sweep_entries(const PCView &pc, int which, std::vector<i64> &off, std::vector<int> &col,

@@ -155,10 +155,80 @@ static bool describe_sweep(const Prog &prog, i64 n, std::vector<i64> &off, std::
   return !failed();
 }
 
+// The entries of one sweep on the device (k_sweep_describe; the offsets are
+// scanned on the host from the counts), into buffers the caller frees.
+template <class Prog>
+static bool describe_sweep_dev(const Prog &prog, i64 n, DevEntries &E, std::vector<void *> &keep) {
+  int *cnt_d = nullptr, *col_d = nullptr;
+  i64 *off_d = nullptr;
+  double *coef_d = nullptr;
+  HIPCHK(hipMalloc(&cnt_d, n * sizeof(int)));
+  if (failed()) return false;
+  keep.push_back(cnt_d);
+  MPK_LAUNCH(k_sweep_describe<Prog>, nblocks(n, 256), 256, prog, n, 0, cnt_d, (const i64 *)nullptr,
+             (int *)nullptr, (double *)nullptr);
+  std::vector<int> cnt(n);
+  copy_dtoh(cnt.data(), cnt_d, n * sizeof(int));
+  std::vector<i64> off(n + 1, 0);
+  i64 widest = 0;
+  for (i64 i = 0; i < n; i++) {
+    off[i + 1] = off[i] + cnt[i];
+    widest = cnt[i] > widest ? cnt[i] : widest;
+  }
+  const i64 m = off[n], mm = m > 0 ? m : 1;
+  HIPCHK(hipMalloc(&off_d, (n + 1) * sizeof(i64)));
+  HIPCHK(hipMalloc(&col_d, mm * sizeof(int)));
+  HIPCHK(hipMalloc(&coef_d, mm * sizeof(double)));
+  for (void *q : {(void *)off_d, (void *)col_d, (void *)coef_d})
+    if (q) keep.push_back(q);
+  if (failed()) return false;
+  copy_htod(off_d, off.data(), (n + 1) * sizeof(i64));
+  MPK_LAUNCH(k_sweep_describe<Prog>, nblocks(n, 256), 256, prog, n, 1, cnt_d, (const i64 *)off_d, col_d, coef_d);
+  E = DevEntries{n, m, off_d, col_d, coef_d, widest};
+  return !failed();
+}
+
+// The narrow word of a plan from its header's w (hdr[5]) and mark (hdr[7]),
+// its longest row and widest level: narrow: the slot count; wave and wave
+// blocks: the count plus 16, 32 when no row has over 2 entries, 64 log2 of
+// its waves (one per 64 rows of its widest level), and 256 for wave blocks.
+static int narrow_word(i64 w, i64 mark, int longest, int widest) {
+  const int lw = widest <= 64 ? 0 : widest <= 128 ? 1 : 2;
+  return mark == 1 ? (int)w
+         : mark >= 2 ? ((int)w | 16 | (longest <= 2 ? 32 : 0) | (lw << 6) | (mark == 3 ? 256 : 0))
+                     : 0;
+}
+
 // The block plan of one sweep into dst [D] (cap bytes, at least the bound);
 // returns its size in bytes and the deepest block's level count, or -1.
+// Block plans are built on the device (device_sweep_plan, byte for byte the
+// host planner's); ring plans, and every plan with MPK_SWEEP_DEVICE_PLAN=0,
+// by the host planner.
 template <class Prog>
 static i64 plan_sweep(const Prog &prog, i64 n, bool forward, char *dst, i64 cap, i64 *depth, int *narrow) {
+  if (!plan_env_off("MPK_SWEEP_DEVICE_PLAN")) {
+    DevEntries E{};
+    std::vector<void *> keep;
+    i64 bytes = -1, d = 0;
+    int longest = 0, widest = 0, mark = 0;
+    if (describe_sweep_dev(prog, n, E, keep)) {
+      const i64 bound = mpk_sweep_plan_bound(n, E.m, MPK_BLK_ROWS);
+      if (bound > cap)
+        set_error(5, "sweep plan buffer too small");
+      else
+        bytes = device_sweep_plan(E, forward, MPK_BLK_ROWS, dst, cap, &d, &longest, &widest, &mark);
+    }
+    HIPCHK(hipStreamSynchronize(stream()));
+    for (void *q : keep) HIPCHK(hipFree(q));
+    if (bytes < 0 || failed()) return -1;
+    if (bytes > 0) {
+      i64 w = 0;
+      copy_dtoh(&w, dst + 5 * sizeof(i64), sizeof w);
+      *depth = d;
+      *narrow = narrow_word(w, mark, longest, widest);
+      return failed() ? -1 : bytes;
+    }
+  }
   std::vector<i64> off;
   std::vector<int> col;
   std::vector<double> coef;
@@ -188,10 +258,7 @@ static i64 plan_sweep(const Prog &prog, i64 n, bool forward, char *dst, i64 cap,
     for (i64 p = 0; p < n; p++) longest = pcnt[p] > longest ? pcnt[p] : longest;
     for (i64 l = 0; l + 1 < host[2]; l++) widest = lvl[l + 1] - lvl[l] > widest ? lvl[l + 1] - lvl[l] : widest;
   }
-  const int lw = widest <= 64 ? 0 : widest <= 128 ? 1 : 2;
-  *narrow = host[7] == 1 ? (int)host[5]
-            : host[7] >= 2 ? ((int)host[5] | 16 | (longest <= 2 ? 32 : 0) | (lw << 6) | (host[7] == 3 ? 256 : 0))
-                           : 0;
+  *narrow = narrow_word(host[5], host[7], longest, widest);
   return failed() ? -1 : bytes;
 }
 

@@ -350,8 +350,7 @@ class PC:
         if n == 0:
             return
         r, ic = self._permutations(m, n)
-        adiag = np.zeros(n, dtype=np.int64)
-        _host_diag_markers(m, adiag)
+        adiag = _device_diag_markers(m, n)
         scratch = np.zeros(7 * n + 8, dtype=np.int64)
         if self._type == PCType.ILU:
             self._build_ilu(m, n, r, ic, adiag, scratch)
@@ -618,23 +617,15 @@ class PC:
         return _icc_factor_mat(f)
 
 
-def _host_diag_markers(m: Mat, adiag: np.ndarray) -> None:
-    """MatGetDiagonalMarkers_SeqAIJ on the host CSR (ported.mojo:354-377):
-    the position of a_ii in row i, or the row end."""
-    n = m._nrows
-    irow, icol = m.irow, m.icol
-    rows = np.repeat(np.arange(n, dtype=np.int64), np.diff(irow[: n + 1]))
-    nnz = len(rows)
-    hit = np.nonzero(icol[:nnz] == rows)[0]
-    adiag[:] = irow[1: n + 1]
-    if len(hit):
-        # first match per row
-        first = np.full(n, -1, dtype=np.int64)
-        rr = rows[hit]
-        keep = np.r_[True, rr[1:] != rr[:-1]]
-        first[rr[keep]] = hit[keep]
-        has = first >= 0
-        adiag[has] = first[has]
+def _device_diag_markers(m: Mat, n: int) -> np.ndarray:
+    """MatGetDiagonalMarkers_SeqAIJ (ported.mojo:354-377) on the device copy
+    of the matrix (k_diag_markers), as the int64 array the symbolic phases
+    take: the position of a_ii in row i, or the row end."""
+    d = m._dev()
+    dm = DeviceBuffer(4 * max(n, 1))
+    lib().mp_mat_get_diagonal_markers(d.rp.ptr, d.ci.ptr, dm.ptr, n)
+    check()
+    return dm.to_numpy(np.int32, n).astype(np.int64)
 
 
 def _first_capacity(m: Mat, n: int) -> int:
