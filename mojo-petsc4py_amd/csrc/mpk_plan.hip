@@ -13,7 +13,13 @@
 //     from a counting sort by the same wave (k_plan_pos);
 //   - entries, dependencies and wave records one position per thread
 //     (k_plan_entries, k_plan_records).
-// Sweeps that take a ring plan are left to the host planner.
+// Ring plans likewise: the latest levels from the same level sweep run in
+// reverse sweep order over the dependencies transposed (each row's readers,
+// k_plan_tcount / k_plan_tfill); the positions from a stable radix sort of
+// the rows by level (sweep order within a level); the window test and the
+// entries per row / position.
+#include <hipcub/hipcub.hpp>
+
 #include <algorithm>
 #include <cstring>
 #include <vector>
@@ -295,19 +301,131 @@ __global__ void k_plan_counts(i64 n, const i64 *off, const int *prow, int *c) {
   if (p < n) c[p] = (int)(off[prow[p] + 1] - off[prow[p]]);
 }
 
+// The readers of each row, for the latest levels: tcnt[k] += 1 for every
+// dependency k of every row (tcnt zeroed before, n + 1 long so that its
+// exclusive scan ends with the total), then tcol lists row k's readers from
+// toff[k] on (cursor a copy of toff; the order within a row does not matter,
+// only the largest level is taken).
+__global__ void k_plan_tcount(i64 n, int forward, const i64 *off, const int *col, unsigned long long *tcnt) {
+  const i64 i = (i64)blockIdx.x * 256 + threadIdx.x;
+  if (i >= n) return;
+  const i64 si = plan_spos(i, n, forward);
+  for (i64 j = off[i]; j < off[i + 1]; j++)
+    if (plan_spos(col[j], n, forward) < si) atomicAdd(tcnt + col[j], 1ull);
+}
+__global__ void k_plan_tfill(i64 n, int forward, const i64 *off, const int *col, unsigned long long *cursor,
+                             int *tcol) {
+  const i64 i = (i64)blockIdx.x * 256 + threadIdx.x;
+  if (i >= n) return;
+  const i64 si = plan_spos(i, n, forward);
+  for (i64 j = off[i]; j < off[i + 1]; j++)
+    if (plan_spos(col[j], n, forward) < si) tcol[atomicAdd(cursor + col[j], 1ull)] = (int)i;
+}
+
+// Latest levels from the reverse sweep's earliest levels: a row's latest
+// level is nl - 1 less the longest chain of readers after it, which is its
+// earliest level in the reverse sweep over the readers.
+__global__ void k_plan_late_out(i64 n, int nl1, const int *rlev, int *late) {
+  const i64 i = (i64)blockIdx.x * 256 + threadIdx.x;
+  if (i < n) late[i] = nl1 - rlev[i];
+}
+
+// Sort inputs: each sweep position's row and its latest level.
+__global__ void k_plan_keys(i64 n, int forward, const int *late, int *key, int *row) {
+  const i64 s = (i64)blockIdx.x * 256 + threadIdx.x;
+  if (s >= n) return;
+  const i64 i = plan_spos(s, n, forward);
+  key[s] = late[i];
+  row[s] = (int)i;
+}
+
+// After the sort (rows by level, sweep order within one): pos[row] = p, and
+// the level starts lvl[l] (l <= nl; lvl[nl] = n) from the sorted levels.
+__global__ void k_plan_ring_pos(i64 n, i64 nl, const int *skey, const int *prow, int *pos, int *lvl) {
+  const i64 p = (i64)blockIdx.x * 256 + threadIdx.x;
+  if (p > n) return;
+  const int lo = p == 0 ? -1 : skey[p - 1], hi = p == n ? (int)nl : skey[p];
+  for (int l = lo + 1; l <= hi; l++) lvl[l] = (int)p;  // levels starting here (every level if none is empty)
+  if (p < n) pos[prow[p]] = (int)p;
+}
+
+// Entries that read a row W or more positions before the end of the
+// reader's level ("old": their ring slot may have been rewritten).
+__global__ void k_plan_old(i64 n, int forward, i64 W, const i64 *off, const int *col, const int *late,
+                           const int *pos, const int *lvl, unsigned long long *old) {
+  const i64 i = (i64)blockIdx.x * 256 + threadIdx.x;
+  unsigned long long c = 0;
+  if (i < n) {
+    const i64 si = plan_spos(i, n, forward), end = lvl[late[i] + 1];
+    for (i64 j = off[i]; j < off[i + 1]; j++) {
+      const int k = col[j];
+      if (plan_spos(k, n, forward) < si && !(end - pos[k] <= W)) c++;
+    }
+  }
+  if (c) atomicAdd(old, c);
+}
+
+// A ring plan's entries, per position: as k_plan_entries, the dependency
+// slot being the LDS ring slot (pos mod W) of a row less than W positions
+// behind the end of the reader's level, else the published row.
+__global__ void k_plan_ring_entries(i64 n, int forward, i64 W, i64 w, const i64 *off, const int *col,
+                                    const double *coef, const int *prow, const int *pos, const int *late,
+                                    const int *lvl, const i64 *eout, int *pcnt, int *pent, double *ecoef, int *edep,
+                                    unsigned *flags) {
+  const i64 p = (i64)blockIdx.x * 256 + threadIdx.x;
+  unsigned f = 0;
+  if (p < n) {
+    const i64 i = prow[p], si = plan_spos(i, n, forward), o = off[i], cnt = off[i + 1] - o, end = lvl[late[i] + 1];
+    pcnt[p] = (int)cnt;
+    pent[p] = w ? 0 : (int)eout[p];
+    const i64 q0 = w ? p : eout[p], qs = w ? n : 1;
+    for (i64 e = 0; e < cnt; e++) {
+      const i64 k = col[o + e];
+      const i64 q = q0 + e * qs;
+      ecoef[q] = coef[o + e];
+      int d;
+      if (plan_spos(k, n, forward) >= si) {
+        d = (int)(-k - n - 1);
+        f |= 2u;
+      } else if (end - pos[k] <= W) {
+        d = (int)(pos[k] & (W - 1));
+      } else {
+        d = (int)(-k - 1);
+        f |= 1u;
+      }
+      edep[q] = d;
+    }
+  }
+  const unsigned long long any1 = __ballot(f & 1u), any2 = __ballot(f & 2u);
+  if ((threadIdx.x & 63) == 0 && (any1 | any2)) atomicOr(flags, (any1 ? 1u : 0u) | (any2 ? 2u : 0u));
+}
+
 // Device scratch of the planner, grown on demand and kept.
 struct PlanScratch {
   int *lev = nullptr, *pos = nullptr, *nlev = nullptr, *wlev = nullptr, *maxlev = nullptr, *cnt = nullptr;
+  int *late = nullptr, *key = nullptr, *skey = nullptr, *row = nullptr;
   i64 *li = nullptr, *eout = nullptr;
   unsigned *flags = nullptr;
-  i64 cap_n = 0, cap_b = 0;
+  unsigned long long *old = nullptr, *lw = nullptr;
+  i64 *toff = nullptr;
+  int *tcol = nullptr;
+  void *sort_tmp = nullptr;
+  size_t sort_cap = 0;
+  i64 cap_n = 0, cap_b = 0, cap_t = 0;
   bool ensure(i64 n, i64 nblk) {
     if (n > cap_n) {
-      for (void *q : {(void *)lev, (void *)pos, (void *)eout, (void *)cnt})
+      for (void *q : {(void *)lev, (void *)pos, (void *)eout, (void *)cnt, (void *)late, (void *)key, (void *)skey,
+                      (void *)row, (void *)lw, (void *)toff})
         if (q) HIPCHK(hipFree(q));
+      HIPCHK(hipMalloc(&lw, (n + 1) * sizeof(unsigned long long)));
+      HIPCHK(hipMalloc(&toff, (n + 1) * sizeof(i64)));
       HIPCHK(hipMalloc(&lev, n * sizeof(int)));
       HIPCHK(hipMalloc(&pos, n * sizeof(int)));
       HIPCHK(hipMalloc(&cnt, n * sizeof(int)));
+      HIPCHK(hipMalloc(&late, n * sizeof(int)));
+      HIPCHK(hipMalloc(&key, n * sizeof(int)));
+      HIPCHK(hipMalloc(&skey, n * sizeof(int)));
+      HIPCHK(hipMalloc(&row, n * sizeof(int)));
       HIPCHK(hipMalloc(&eout, (n + 1) * sizeof(i64)));
       cap_n = n;
     }
@@ -322,10 +440,147 @@ struct PlanScratch {
     if (!maxlev) {
       HIPCHK(hipMalloc(&maxlev, 4 * sizeof(int)));
       HIPCHK(hipMalloc(&flags, 4 * sizeof(unsigned)));
+      HIPCHK(hipMalloc(&old, sizeof(unsigned long long)));
     }
     return !failed();
   }
+  bool ensure_t(i64 n, i64 m) {  // after ensure(n, ...)
+    (void)n;
+    if (m > cap_t) {
+      if (tcol) HIPCHK(hipFree(tcol));
+      HIPCHK(hipMalloc(&tcol, (m > 0 ? m : 1) * sizeof(int)));
+      cap_t = m;
+    }
+    return !failed();
+  }
+  bool ensure_sort(size_t bytes) {
+    if (bytes <= sort_cap && sort_tmp) return true;
+    if (sort_tmp) HIPCHK(hipFree(sort_tmp));
+    HIPCHK(hipMalloc(&sort_tmp, bytes > 0 ? bytes : 16));
+    sort_cap = bytes;
+    return !failed();
+  }
 };
+
+// The CSR form's entry offsets in position order (w = 0): eout [D] and
+// pent[n] (the entry count).
+static bool plan_eout(i64 n, const i64 *off, const int *prow, int *pent, PlanScratch &S) {
+  MPK_LAUNCH(k_plan_counts, nblocks(n, 256), 256, n, off, prow, S.cnt);
+  std::vector<int> pc(n);
+  copy_dtoh(pc.data(), S.cnt, n * sizeof(int));
+  std::vector<i64> eo(n + 1, 0);
+  for (i64 p = 0; p < n; p++) eo[p + 1] = eo[p] + pc[p];
+  copy_htod(S.eout, eo.data(), (n + 1) * sizeof(i64));
+  const int last = (int)eo[n];
+  copy_htod(pent + n, &last, sizeof last);
+  return !failed();
+}
+
+// mark_plan's choice for a plan of nblk blocks, w slots, its deepest
+// block's levels and widest level: the wave records (appended when the
+// wave form takes it, written by k_plan_records) and hdr[7], hdr[4].
+static i64 plan_mark(i64 n, i64 nblk, i64 w, i64 levels, int widest, bool published, bool stale, i64 bytes,
+                     i64 cap, const double *ecoef, const int *edep, const int *prow, const int *pcnt, char *dst,
+                     int64_t *hdr) {
+  const bool wave_w = w == 4 || w == 8;
+  if (plan_env_off("MPK_SWEEP_NARROW") || nblk < 1 || !wave_w) return bytes;
+  const i64 rb = wave_rec_bytes_rule(w), wbytes = bytes + n * rb;
+  const bool narrow = nblk == 1 && !published && widest <= 256 && levels < 4096;
+  const bool wave = widest <= 256 && levels <= 12288 && !stale && wbytes <= cap && !plan_env_off("MPK_SWEEP_WAVE");
+  if (!wave) {
+    hdr[7] = narrow ? 1 : 0;
+    return bytes;
+  }
+  MPK_LAUNCH(k_plan_records, nblocks(n, 256), 256, n, w, rb, ecoef, edep, prow, pcnt, dst + bytes);
+  hdr[4] = wbytes;
+  hdr[7] = nblk > 1 || published ? 3 : 2;
+  return wbytes;
+}
+
+// The ring plan of a sweep of depth nl (earliest levels already in S.lev),
+// or 0 when the host planner's test sends the sweep to blocks instead.
+static i64 device_ring_plan(const DevEntries &E, bool forward, i64 bmax, i64 nl, int mode, char *dst, i64 cap,
+                            i64 *depth_out, int *widest_level, int *mark, PlanScratch &S) {
+  Ctx *c = ctx();
+  const i64 n = E.n, m = E.m;
+  const int fw = forward ? 1 : 0;
+  const i64 w = plan_w_rule(E.widest, n, m), slots = w ? w * n : m;
+  i64 W = 1;
+  while (2 * W <= bmax) W *= 2;
+  // latest levels: the readers of each row (the dependencies transposed),
+  // then the reverse sweep's earliest levels over them
+  if (!S.ensure_t(n, m)) return -1;
+  HIPCHK(hipMemsetAsync(S.lw, 0, (n + 1) * sizeof(unsigned long long), stream()));
+  MPK_LAUNCH(k_plan_tcount, nblocks(n, 256), 256, n, fw, E.off, E.col, S.lw);
+  size_t sb = 0;
+  HIPCHK(hipcub::DeviceScan::ExclusiveSum(nullptr, sb, S.lw, S.toff, (int)(n + 1), stream()));
+  if (!S.ensure_sort(sb)) return -1;
+  HIPCHK(hipcub::DeviceScan::ExclusiveSum(S.sort_tmp, sb, S.lw, S.toff, (int)(n + 1), stream()));
+  HIPCHK(hipMemcpyAsync(S.lw, S.toff, n * sizeof(i64), hipMemcpyDeviceToDevice, stream()));
+  MPK_LAUNCH(k_plan_tfill, nblocks(n, 256), 256, n, fw, E.off, E.col, S.lw, S.tcol);
+  HIPCHK(hipMemsetAsync(S.lev, 0xff, n * sizeof(int), stream()));
+  HIPCHK(hipMemsetAsync(S.maxlev + 1, 0xff, sizeof(int), stream()));
+  HIPCHK(hipMemsetAsync(c->tickets, 0, sizeof(unsigned), stream()));
+  MPK_LAUNCH(k_plan_glev, nblocks(n, 64), 64, n, 1 - fw, (const i64 *)S.toff, (const int *)S.tcol, S.lev,
+             c->tickets, S.maxlev + 1);
+  MPK_LAUNCH(k_plan_late_out, nblocks(n, 256), 256, n, (int)(nl - 1), (const int *)S.lev, S.late);
+  // positions: rows by latest level, sweep order within a level
+  MPK_LAUNCH(k_plan_keys, nblocks(n, 256), 256, n, fw, (const int *)S.late, S.key, S.row);
+  const i64 nlvl = nl + 1, bytes = plan_bytes_rule(n, 1, nlvl, slots);
+  if (bytes > cap) {
+    set_error(5, "sweep plan buffer too small");
+    return -1;
+  }
+  int bits = 1;
+  while (((i64)1 << bits) <= nl) bits++;
+  int *blk_pos = reinterpret_cast<int *>(dst + 64), *blk_lev = blk_pos + 2, *lvl = blk_lev + 2;
+  int *prow = lvl + nlvl, *pcnt = prow + n, *pent = pcnt + n;
+  size_t tb = 0;
+  HIPCHK(hipcub::DeviceRadixSort::SortPairs(nullptr, tb, S.key, S.skey, S.row, S.pos, (int)n, 0, bits, stream()));
+  if (!S.ensure_sort(tb)) return -1;
+  // the sorted rows go to the plan's prow once the buffer is cleared; sort
+  // into scratch first (S.pos, rewritten below as row -> position)
+  HIPCHK(hipcub::DeviceRadixSort::SortPairs(S.sort_tmp, tb, S.key, S.skey, S.row, S.lev, (int)n, 0, bits, stream()));
+  const i64 rb = wave_rec_bytes_rule(w), wbytes = bytes + ((w == 4 || w == 8) ? n * rb : 0);
+  HIPCHK(hipMemsetAsync(dst, 0, (size_t)std::min(cap, wbytes), stream()));
+  HIPCHK(hipMemcpyAsync(prow, S.lev, n * sizeof(int), hipMemcpyDeviceToDevice, stream()));
+  MPK_LAUNCH(k_plan_ring_pos, nblocks(n + 1, 256), 256, n, nl, (const int *)S.skey, (const int *)prow, S.pos, lvl);
+  // the host planner's test: few enough old reads
+  HIPCHK(hipMemsetAsync(S.old, 0, sizeof(unsigned long long), stream()));
+  MPK_LAUNCH(k_plan_old, nblocks(n, 256), 256, n, fw, W, E.off, E.col, (const int *)S.late, (const int *)S.pos,
+             (const int *)lvl, S.old);
+  unsigned long long old = 0;
+  copy_dtoh(&old, S.old, sizeof old);
+  if (failed()) return -1;
+  if (!(mode == 2 || (n <= plan_ring_width_rule() * nl && 64 * (i64)old <= m))) return 0;
+  // the ring plan
+  const int hb[4] = {0, (int)n, 0, (int)nlvl};
+  copy_htod(blk_pos, hb, sizeof hb);
+  const i64 ints = 2 * 2 + nlvl + 3 * n + 1;
+  double *ecoef = reinterpret_cast<double *>(dst + 64 + ((4 * ints + 7) & ~(i64)7));
+  int *edep = reinterpret_cast<int *>(ecoef + slots);
+  if (!w && !plan_eout(n, E.off, prow, pent, S)) return -1;
+  HIPCHK(hipMemsetAsync(S.flags, 0, sizeof(unsigned), stream()));
+  MPK_LAUNCH(k_plan_ring_entries, nblocks(n, 256), 256, n, fw, W, w, E.off, E.col, E.coef, (const int *)prow,
+             (const int *)S.pos, (const int *)S.late, (const int *)lvl, (const i64 *)S.eout, pcnt, pent, ecoef, edep,
+             S.flags);
+  std::vector<int> lv(nlvl);
+  copy_dtoh(lv.data(), lvl, nlvl * sizeof(int));
+  unsigned flags = 0;
+  copy_dtoh(&flags, S.flags, sizeof flags);
+  if (failed()) return -1;
+  int widest = 0;
+  for (i64 l = 0; l + 1 < nlvl; l++) widest = std::max(widest, lv[l + 1] - lv[l]);
+  int64_t hdr[8] = {1, m, nlvl, nl, bytes, w, W, 0};
+  const i64 total = plan_mark(n, 1, w, nl, widest, flags & 1u, flags & 2u, bytes, cap, ecoef, edep, prow, pcnt,
+                              dst, hdr);
+  copy_htod(dst, hdr, sizeof hdr);
+  if (failed()) return -1;
+  *depth_out = nl;
+  *widest_level = widest;
+  *mark = (int)hdr[7];
+  return total;
+}
 static PlanScratch &plan_scratch() {
   static PlanScratch s;
   return s;
@@ -339,7 +594,6 @@ i64 device_sweep_plan(const DevEntries &E, bool forward, i64 bmax, char *dst, i6
   const int fw = forward ? 1 : 0;
   const i64 w = plan_w_rule(E.widest, n, m), slots = w ? w * n : m;
   const int mode = plan_mode_rule();
-  if (mode == 2 && n > bmax) return 0;  // a forced ring: the host planner's
   const i64 B = plan_rows_rule(n, bmax), nblk = (n + B - 1) / B;
   PlanScratch &S = plan_scratch();
   if (!S.ensure(n, nblk)) return -1;
@@ -355,7 +609,13 @@ i64 device_sweep_plan(const DevEntries &E, bool forward, i64 bmax, char *dst, i6
     copy_dtoh(&ml, S.maxlev, sizeof ml);
     if (failed()) return -1;
     const i64 nl = (i64)ml + 1;
-    if (!(n > plan_ring_width_rule() * nl)) return 0;
+    if (mode == 2 || !(n > plan_ring_width_rule() * nl)) {
+      const i64 r = device_ring_plan(E, forward, bmax, nl, mode, dst, cap, depth_out, widest_level, mark, S);
+      if (r != 0) {
+        *longest = (int)E.widest;
+        return r;
+      }
+    }
   }
   // blocks: local levels, then the sizes the layout needs
   MPK_LAUNCH(k_plan_local, nblk, 64, n, fw, B, E.off, E.col, S.lev, S.nlev);
@@ -386,18 +646,7 @@ i64 device_sweep_plan(const DevEntries &E, bool forward, i64 bmax, char *dst, i6
   int *edep = reinterpret_cast<int *>(ecoef + slots);
   MPK_LAUNCH(k_plan_pos, nblk, 64, n, fw, B, nblk, nlvl, (const int *)S.lev, (const i64 *)S.li,
              (const int *)S.nlev, blk_pos, blk_lev, lvl, prow, S.pos, S.wlev);
-  if (!w) {
-    // CSR form: each position's entries start at the running count in
-    // position order
-    MPK_LAUNCH(k_plan_counts, nblocks(n, 256), 256, n, E.off, (const int *)prow, S.cnt);
-    std::vector<int> pc(n);
-    copy_dtoh(pc.data(), S.cnt, n * sizeof(int));
-    std::vector<i64> eo(n + 1, 0);
-    for (i64 p = 0; p < n; p++) eo[p + 1] = eo[p] + pc[p];
-    copy_htod(S.eout, eo.data(), (n + 1) * sizeof(i64));
-    const int last = (int)eo[n];
-    copy_htod(pent + n, &last, sizeof last);
-  }
+  if (!w && !plan_eout(n, E.off, prow, pent, S)) return -1;  // CSR form's offsets
   HIPCHK(hipMemsetAsync(S.flags, 0, sizeof(unsigned), stream()));
   MPK_LAUNCH(k_plan_entries, nblocks(n, 256), 256, n, fw, B, w, E.off, E.col, E.coef, (const int *)prow,
              (const int *)S.pos, (const int *)blk_pos, (const i64 *)S.eout, pcnt, pent, ecoef, edep, S.flags);
@@ -409,22 +658,9 @@ i64 device_sweep_plan(const DevEntries &E, bool forward, i64 bmax, char *dst, i6
   int widest = 0;
   for (i64 b = 0; b < nblk; b++) widest = std::max(widest, wl[b]);
   const bool published = flags & 1u, stale = flags & 2u;
-  // mark_plan's choice
   int64_t hdr[8] = {nblk, m, nlvl, depth, bytes, w, 0, 0};
-  i64 total = bytes;
-  if (!plan_env_off("MPK_SWEEP_NARROW") && nblk >= 1 && wave_w) {
-    const bool narrow = nblk == 1 && !published && widest <= 256 && levels < 4096;
-    const bool wave = widest <= 256 && levels <= 12288 && !stale && wbytes <= cap && !plan_env_off("MPK_SWEEP_WAVE");
-    if (wave) {
-      MPK_LAUNCH(k_plan_records, nblocks(n, 256), 256, n, w, rb, (const double *)ecoef, (const int *)edep,
-                 (const int *)prow, (const int *)pcnt, dst + bytes);
-      hdr[4] = wbytes;
-      hdr[7] = nblk > 1 || published ? 3 : 2;
-      total = wbytes;
-    } else {
-      hdr[7] = narrow ? 1 : 0;
-    }
-  }
+  const i64 total = plan_mark(n, nblk, w, levels, widest, published, stale, bytes, cap, ecoef, edep, prow, pcnt, dst,
+                              hdr);
   copy_htod(dst, hdr, sizeof hdr);
   if (failed()) return -1;
   *depth_out = depth;
