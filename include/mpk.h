@@ -251,6 +251,36 @@ int64_t mpk_icc_numeric(int64_t nrow, int64_t irow, int64_t icol, int64_t data, 
 int64_t mpk_icc_transpose_size(int64_t ui, int64_t udiag, int64_t nrow);
 int64_t mpk_icc_transpose(int64_t ui, int64_t uj, int64_t udiag, int64_t nrow, int64_t trp, int64_t trow,
                           int64_t tpos, int64_t tmid);
+/* No counterpart in the reference: the IC(0) analysis with the natural
+ * ordering on the device, giving exactly the arrays of mp_icc_symbolic at
+ * level 0 + mp_icc_contiguous_cholesky, mpk_icc_transpose and mpk_icc_visits
+ * (whose replay of the row chains becomes a sort by a closed-form key).
+ * irow, icol [D] int32: A.  mpk_ic0_size returns bi[nrow] (each factor row
+ * holds A's columns above the diagonal and its diagonal slot), or -2 when a
+ * row of A is not strictly ascending within 0..nrow-1, a column of the
+ * strict part holds over 256 entries, or bi[nrow] passes int32 (then the
+ * host passes apply), or -1 on error.  mpk_ic0_analyse writes, with
+ * m = bi[nrow] - nrow, [D] int32: bi[nrow + 1], bj[bi[nrow]], bdiag[nrow + 1]
+ * (the contiguous Cholesky layout); trp[nrow + 1], trow[m], tpos[m],
+ * tmid[nrow] (the transpose, no late entries); vrow[m], vpos[m] (the visits,
+ * vptr being trp); lrow[nrow]; and lptr [H] int64[nlev + 1] (room for
+ * nrow + 1).  Returns nlev, or -1. */
+int64_t mpk_ic0_size(int64_t irow, int64_t icol, int64_t nrow);
+/* No counterpart in the reference: the ILU(0) analysis with the natural
+ * ordering on the device, giving exactly the arrays of mp_ilu_symbolic_ilu0
+ * and mpk_ilu_levels.  irow, icol [D] int32: A.  mpk_ilu0_size returns the
+ * factor's entry count (bdiag[0] + 1), or -2 when a row of A is not strictly
+ * ascending within 0..nrow-1 or lacks its diagonal, or the count passes int32
+ * (then the host passes apply), or -1 on error.  mpk_ilu0_analyse writes
+ * [D] int32 bi[nrow + 1], bj[count], bdiag[nrow + 1] (the contiguous LU
+ * layout) and lrow[nrow], and lptr [H] int64[nlev + 1] (room for nrow + 1).
+ * Returns nlev, or -1. */
+int64_t mpk_ilu0_size(int64_t irow, int64_t icol, int64_t nrow);
+int64_t mpk_ilu0_analyse(int64_t irow, int64_t icol, int64_t nrow, int64_t bi, int64_t bj, int64_t bdiag,
+                         int64_t lrow, int64_t lptr);
+int64_t mpk_ic0_analyse(int64_t irow, int64_t icol, int64_t nrow, int64_t bi, int64_t bj, int64_t bdiag, int64_t trp,
+                        int64_t trow, int64_t tpos, int64_t tmid, int64_t vrow, int64_t vpos, int64_t lrow,
+                        int64_t lptr);
 /* No counterpart in the reference: block plans for the two sweeps of the PC a
  * descriptor describes (SOR, ILU, IC).  A sweep's rows are cut into blocks of
  * consecutive rows; a workgroup runs a block level by level (levels of the

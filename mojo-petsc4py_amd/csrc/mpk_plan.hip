@@ -670,4 +670,381 @@ i64 device_sweep_plan(const DevEntries &E, bool forward, i64 bmax, char *dst, i6
   return total;
 }
 
+// ------------------------------------------------------- IC(0) analysis
+// The IC(0) factor's pattern with the natural ordering, and everything the
+// host computes from it (mp_icc_symbolic at level 0, mp_icc_contiguous_
+// cholesky, mpk_icc_transpose, mpk_icc_visits), built on the device:
+//   - row k of the factor holds A's columns above k, ascending, then k
+//     itself (level 0 admits no fill; the symbolic pass sorts the strict
+//     columns and puts the diagonal slot last, present in A or not);
+//   - column k's entries (i, k), i < k, are the rows that update row k; the
+//     transpose lists them by ascending i;
+//   - the up-looking pass (ported.mojo:1040-1124) visits them in the order
+//     its row chains hold them.  Row i joins column k's chain while the pass
+//     is at row q, the column before k in row i (or i itself when k is its
+//     first strict column), in the order it left q's chain, itself after
+//     every row that was in q's chain; each join goes to the chain's head.
+//     So column k's visits come by q descending, ties (same q) in the
+//     reverse of their order at q, row q itself first.  Unrolled, row i's
+//     key is its columns before k in reverse order, then i, then an end
+//     mark; keys compare element by element, descending at even depths and
+//     ascending at odd ones, the end mark last at even depths and first at
+//     odd ones (k_ic0_order);
+//   - a row's level is one past its deepest visitor's (k_plan_glev over the
+//     transpose), the level lists by a stable radix sort of the rows.
+// Rows of A must hold strictly ascending columns within 0..n-1, and no
+// column may gather more than kIc0MaxCol entries (each column is sorted by
+// one thread); mpk_ic0_size reports -2 otherwise, and the host passes apply.
+
+static const int kIc0MaxCol = 256;
+
+// Per row: the check, the factor row's length (strict columns + 1), and the
+// column counts of the strict part.  flags[0] bit 0: a row out of order or
+// a column out of range; flags[1]: the longest column (atomicMax).
+__global__ void k_ic0_count(i64 n, const int *rp, const int *ci, unsigned long long *rlen, int *ccnt,
+                            unsigned *flags) {
+  const i64 i = (i64)blockIdx.x * 256 + threadIdx.x;
+  if (i >= n) return;
+  int prev = -1;
+  unsigned long long s = 1;
+  bool bad = false;
+  for (int j = rp[i]; j < rp[i + 1]; j++) {
+    const int c = ci[j];
+    bad |= c <= prev || c >= n;
+    prev = c;
+    if (!bad && c > i) {
+      s++;
+      const int q = atomicAdd(ccnt + c, 1) + 1;
+      if (q > kIc0MaxCol) atomicMax(flags + 1, (unsigned)q);
+    }
+  }
+  rlen[i] = s;
+  if (bad) atomicOr(flags, 1u);
+}
+
+// The factor rows (bi, bj, bdiag in the contiguous Cholesky layout) and the
+// transpose's entries, unordered within a column (cursor: a copy of trp).
+__global__ void k_ic0_fill(i64 n, const int *rp, const int *ci, const i64 *b64, int *bi, int *bj, int *bdiag,
+                           int *cursor, int *trow, int *tpos) {
+  const i64 i = (i64)blockIdx.x * 256 + threadIdx.x;
+  if (i >= n) return;
+  const int b = (int)b64[i], e = (int)b64[i + 1];
+  bi[i] = b;
+  bdiag[i] = e - 1;
+  int t = b;
+  for (int j = rp[i]; j < rp[i + 1]; j++) {
+    const int c = ci[j];
+    if (c > i) {
+      bj[t] = c;
+      const int p = atomicAdd(cursor + c, 1);
+      trow[p] = (int)i;
+      tpos[p] = t;
+      t++;
+    }
+  }
+  bj[e - 1] = (int)i;
+  if (i == n - 1) {
+    bi[n] = e;
+    bdiag[n] = e;
+  }
+}
+
+// Element d of the key of row i's entry at position p (bj[p] = k): the
+// columns before k backwards, then i, then the end mark (-1).
+__device__ __forceinline__ int ic0_key(const int *bi, const int *bj, int i, int p, int d) {
+  const int q = p - 1 - d, b = bi[i];
+  return q >= b ? bj[q] : (q == b - 1 ? i : -1);
+}
+// True when the entry (ia, pa) comes before (ib, pb) in the column's visits.
+__device__ __forceinline__ bool ic0_before(const int *bi, const int *bj, int ia, int pa, int ib, int pb) {
+  for (int d = 0;; d++) {
+    const int ea = ic0_key(bi, bj, ia, pa, d), eb = ic0_key(bi, bj, ib, pb, d);
+    if (ea == eb) {
+      if (ea < 0) return false;  // the same entry
+      continue;
+    }
+    if ((d & 1) == 0) return eb < 0 || (ea >= 0 && ea > eb);
+    return ea < 0 || (eb >= 0 && ea < eb);
+  }
+}
+
+// One thread per column: the transpose by ascending row, the visits in the
+// chains' order, tmid (every entry is above the diagonal: the column's end).
+__global__ void k_ic0_order(i64 n, const int *trp, int *trow, int *tpos, const int *bi, const int *bj, int *vrow,
+                            int *vpos, int *tmid) {
+  const i64 k = (i64)blockIdx.x * 256 + threadIdx.x;
+  if (k >= n) return;
+  const int s0 = trp[k], s1 = trp[k + 1];
+  for (int a = s0 + 1; a < s1; a++) {
+    const int r = trow[a], p = tpos[a];
+    int b = a - 1;
+    while (b >= s0 && trow[b] > r) {
+      trow[b + 1] = trow[b];
+      tpos[b + 1] = tpos[b];
+      b--;
+    }
+    trow[b + 1] = r;
+    tpos[b + 1] = p;
+  }
+  for (int a = s0; a < s1; a++) {
+    const int r = trow[a], p = tpos[a];
+    int b = a - 1;
+    while (b >= s0 && ic0_before(bi, bj, r, p, vrow[b], vpos[b])) {
+      vrow[b + 1] = vrow[b];
+      vpos[b + 1] = vpos[b];
+      b--;
+    }
+    vrow[b + 1] = r;
+    vpos[b + 1] = p;
+  }
+  tmid[k] = s1;
+}
+
+__global__ void k_ic0_widen(i64 n, const int *a, i64 *b) {
+  const i64 i = (i64)blockIdx.x * 256 + threadIdx.x;
+  if (i <= n) b[i] = a[i];
+}
+__global__ void k_ic0_iota(i64 n, int *a) {
+  const i64 i = (i64)blockIdx.x * 256 + threadIdx.x;
+  if (i < n) a[i] = (int)i;
+}
+// Level starts from the rows sorted by level (every level 0..nlev-1 holds a
+// row: a row's deepest visitor is one level up).
+__global__ void k_ic0_lptr(i64 n, i64 nlev, const int *slev, int *lp) {
+  const i64 p = (i64)blockIdx.x * 256 + threadIdx.x;
+  if (p < n && (p == 0 || slev[p] != slev[p - 1])) lp[slev[p]] = (int)p;
+  if (p == 0) lp[nlev] = (int)n;
+}
+
+struct Ic0Scratch {
+  unsigned long long *rlen = nullptr, *ucnt = nullptr;
+  i64 *b64 = nullptr, *t64 = nullptr;
+  int *ccnt = nullptr, *cursor = nullptr, *lev = nullptr, *slev = nullptr, *iota = nullptr, *lp = nullptr;
+  unsigned *flags = nullptr;
+  int *maxlev = nullptr;
+  void *tmp = nullptr;
+  size_t tmp_cap = 0;
+  i64 cap = 0;
+  bool ensure(i64 n) {
+    if (n > cap) {
+      for (void *q : {(void *)rlen, (void *)ucnt, (void *)b64, (void *)t64, (void *)ccnt, (void *)cursor, (void *)lev,
+                      (void *)slev, (void *)iota, (void *)lp})
+        if (q) HIPCHK(hipFree(q));
+      HIPCHK(hipMalloc(&rlen, (n + 1) * sizeof(unsigned long long)));
+      HIPCHK(hipMalloc(&ucnt, (n + 1) * sizeof(unsigned long long)));
+      HIPCHK(hipMalloc(&b64, (n + 1) * sizeof(i64)));
+      HIPCHK(hipMalloc(&t64, (n + 1) * sizeof(i64)));
+      for (int **q : {&ccnt, &cursor, &lev, &slev, &iota, &lp}) HIPCHK(hipMalloc(q, (n + 1) * sizeof(int)));
+      cap = n;
+    }
+    if (!flags) {
+      HIPCHK(hipMalloc(&flags, 4 * sizeof(unsigned)));
+      HIPCHK(hipMalloc(&maxlev, 4 * sizeof(int)));
+    }
+    return !failed();
+  }
+  bool ensure_tmp(size_t bytes) {
+    if (bytes <= tmp_cap && tmp) return true;
+    if (tmp) HIPCHK(hipFree(tmp));
+    HIPCHK(hipMalloc(&tmp, bytes > 0 ? bytes : 16));
+    tmp_cap = bytes;
+    return !failed();
+  }
+};
+static Ic0Scratch &ic0_scratch() {
+  static Ic0Scratch s;
+  return s;
+}
+
+// The counts and the factor's row starts (b64) in the scratch; returns
+// bi[n], or -2 as documented, or -1 on error.
+static i64 ic0_counts(i64 n, const int *rp, const int *ci, Ic0Scratch &S) {
+  if (!S.ensure(n)) return -1;
+  HIPCHK(hipMemsetAsync(S.ccnt, 0, (n + 1) * sizeof(int), stream()));
+  HIPCHK(hipMemsetAsync(S.flags, 0, 4 * sizeof(unsigned), stream()));
+  HIPCHK(hipMemsetAsync(S.rlen + n, 0, sizeof(unsigned long long), stream()));
+  MPK_LAUNCH(k_ic0_count, nblocks(n, 256), 256, n, rp, ci, S.rlen, S.ccnt, S.flags);
+  size_t tb = 0;
+  HIPCHK(hipcub::DeviceScan::ExclusiveSum(nullptr, tb, S.rlen, S.b64, (int)(n + 1), stream()));
+  if (!S.ensure_tmp(tb)) return -1;
+  HIPCHK(hipcub::DeviceScan::ExclusiveSum(S.tmp, tb, S.rlen, S.b64, (int)(n + 1), stream()));
+  unsigned flags[2] = {0, 0};
+  i64 total = 0;
+  copy_dtoh(flags, S.flags, sizeof flags);
+  copy_dtoh(&total, S.b64 + n, sizeof total);
+  if (failed()) return -1;
+  if ((flags[0] & 1u) || flags[1] > (unsigned)kIc0MaxCol || total >= ((i64)1 << 31)) return -2;
+  return total;
+}
+
+// Level lists of a forward sweep whose rows read only earlier rows (off [D]
+// int64[n + 1], col [D] int32): lrow [D] int32[n] the rows level by level,
+// ascending within a level; lptr [H] int64[nlev + 1].  Returns nlev, or -1.
+static i64 level_lists(i64 n, const i64 *off, const int *col, int *lrow, int64_t *lptr, Ic0Scratch &S) {
+  Ctx *c = ctx();
+  watch_install();
+  HIPCHK(hipMemsetAsync(S.lev, 0xff, n * sizeof(int), stream()));
+  HIPCHK(hipMemsetAsync(S.maxlev, 0xff, sizeof(int), stream()));
+  HIPCHK(hipMemsetAsync(c->tickets, 0, sizeof(unsigned), stream()));
+  MPK_LAUNCH(k_plan_glev, nblocks(n, 64), 64, n, 1, off, col, S.lev, c->tickets, S.maxlev);
+  int ml = 0;
+  copy_dtoh(&ml, S.maxlev, sizeof ml);
+  if (failed()) return -1;
+  const i64 nlev = (i64)ml + 1;
+  int bits = 1;
+  while (((i64)1 << bits) <= nlev) bits++;
+  MPK_LAUNCH(k_ic0_iota, nblocks(n, 256), 256, n, S.iota);
+  size_t tb = 0;
+  HIPCHK(hipcub::DeviceRadixSort::SortPairs(nullptr, tb, S.lev, S.slev, S.iota, lrow, (int)n, 0, bits, stream()));
+  if (!S.ensure_tmp(tb)) return -1;
+  HIPCHK(hipcub::DeviceRadixSort::SortPairs(S.tmp, tb, S.lev, S.slev, S.iota, lrow, (int)n, 0, bits, stream()));
+  MPK_LAUNCH(k_ic0_lptr, nblocks(n, 256), 256, n, nlev, (const int *)S.slev, S.lp);
+  std::vector<int> lp(nlev + 1);
+  copy_dtoh(lp.data(), S.lp, (nlev + 1) * sizeof(int));
+  if (failed()) return -1;
+  for (i64 l = 0; l <= nlev; l++) lptr[l] = lp[l];
+  return nlev;
+}
+
+// ------------------------------------------------------ ILU(0) analysis
+// The ILU(0) factor with the natural ordering (mp_ilu_symbolic_ilu0's
+// layout, factor.mojo:40-67) and its level lists (mpk_ilu_levels), built on
+// the device.  Row i's L part (A's columns below i) sits at bi[i]..bi[i+1]-1,
+// bi the prefix sums of the L counts; the U rows run from the end of the L
+// parts backwards, row n-1 first: row i's strict columns above i ascending,
+// then its diagonal slot bdiag[i] = bi[n] - 1 + the sum of (U count + 1)
+// over rows i..n-1.  A row's level is one past its deepest L column's.  Rows
+// of A must hold strictly ascending columns within 0..n-1 and their
+// diagonal (the layouts of rows without one only the host passes handle).
+
+// flags[0] bit 0: a row out of order, out of range or without its diagonal.
+__global__ void k_ilu0_count(i64 n, const int *rp, const int *ci, unsigned long long *lcnt,
+                             unsigned long long *ucnt, unsigned *flags) {
+  const i64 i = (i64)blockIdx.x * 256 + threadIdx.x;
+  if (i >= n) return;
+  int prev = -1;
+  unsigned long long nl = 0, nu = 0;
+  bool bad = false, diag = false;
+  for (int j = rp[i]; j < rp[i + 1]; j++) {
+    const int c = ci[j];
+    bad |= c <= prev || c >= n;
+    prev = c;
+    nl += c < i;
+    nu += c > i;
+    diag |= c == i;
+  }
+  lcnt[i] = nl;
+  ucnt[i] = nu + 1;
+  if (bad || !diag) atomicOr(flags, 1u);
+}
+
+__global__ void k_ilu0_fill(i64 n, const int *rp, const int *ci, const i64 *lp64, const i64 *up64, int *bi, int *bj,
+                            int *bdiag) {
+  const i64 i = (i64)blockIdx.x * 256 + threadIdx.x;
+  if (i >= n) return;
+  const i64 ltot = lp64[n], utot = up64[n];
+  const int d = (int)(ltot - 1 + utot - up64[i]), d1 = (int)(ltot - 1 + utot - up64[i + 1]);
+  int l = (int)lp64[i], u = d1 + 1;
+  bi[i] = l;
+  bdiag[i] = d;
+  for (int j = rp[i]; j < rp[i + 1]; j++) {
+    const int c = ci[j];
+    if (c < i) bj[l++] = c;
+    else if (c > i) bj[u++] = c;
+  }
+  bj[d] = (int)i;
+  if (i == n - 1) {
+    bi[n] = (int)ltot;
+    bdiag[n] = (int)(ltot - 1);
+  }
+}
+
+// The L and U counts' prefix sums in S.b64 / S.t64; returns the factor's
+// entry count, or -2 as documented, or -1 on error.
+static i64 ilu0_counts(i64 n, const int *rp, const int *ci, Ic0Scratch &S) {
+  if (!S.ensure(n)) return -1;
+  HIPCHK(hipMemsetAsync(S.flags, 0, 4 * sizeof(unsigned), stream()));
+  HIPCHK(hipMemsetAsync(S.rlen + n, 0, sizeof(unsigned long long), stream()));
+  HIPCHK(hipMemsetAsync(S.ucnt + n, 0, sizeof(unsigned long long), stream()));
+  MPK_LAUNCH(k_ilu0_count, nblocks(n, 256), 256, n, rp, ci, S.rlen, S.ucnt, S.flags);
+  size_t tb = 0;
+  HIPCHK(hipcub::DeviceScan::ExclusiveSum(nullptr, tb, S.rlen, S.b64, (int)(n + 1), stream()));
+  if (!S.ensure_tmp(tb)) return -1;
+  HIPCHK(hipcub::DeviceScan::ExclusiveSum(S.tmp, tb, S.rlen, S.b64, (int)(n + 1), stream()));
+  HIPCHK(hipcub::DeviceScan::ExclusiveSum(S.tmp, tb, S.ucnt, S.t64, (int)(n + 1), stream()));
+  unsigned flags = 0;
+  i64 tot[2] = {0, 0};
+  copy_dtoh(&flags, S.flags, sizeof flags);
+  copy_dtoh(tot, S.b64 + n, sizeof(i64));
+  copy_dtoh(tot + 1, S.t64 + n, sizeof(i64));
+  if (failed()) return -1;
+  if ((flags & 1u) || tot[0] + tot[1] >= ((i64)1 << 31)) return -2;
+  return tot[0] + tot[1];
+}
+
 }  // namespace mpk
+
+using namespace mpk;
+
+extern "C" {
+
+int64_t mpk_ilu0_size(int64_t irow, int64_t icol, int64_t nrow) {
+  if (nrow <= 0) return 0;
+  if (!ctx()) return -1;
+  return ilu0_counts(nrow, (const int *)irow, (const int *)icol, ic0_scratch());
+}
+
+int64_t mpk_ilu0_analyse(int64_t irow, int64_t icol, int64_t nrow, int64_t bi, int64_t bj, int64_t bdiag,
+                         int64_t lrow, int64_t lptr) {
+  const i64 n = nrow;
+  if (n <= 0) return 0;
+  if (!ctx()) return -1;
+  Ic0Scratch &S = ic0_scratch();
+  const int *rp = (const int *)irow, *ci = (const int *)icol;
+  const i64 total = ilu0_counts(n, rp, ci, S);
+  if (total < 0) {
+    if (total == -2) set_error(2, "mpk_ilu0_analyse: the matrix is not one mpk_ilu0_size accepts");
+    return -1;
+  }
+  MPK_LAUNCH(k_ilu0_fill, nblocks(n, 256), 256, n, rp, ci, (const i64 *)S.b64, (const i64 *)S.t64, (int *)bi,
+             (int *)bj, (int *)bdiag);
+  // levels over the L parts (bi's prefix sums are S.b64)
+  return level_lists(n, S.b64, (const int *)bj, (int *)lrow, (int64_t *)lptr, S);
+}
+
+int64_t mpk_ic0_size(int64_t irow, int64_t icol, int64_t nrow) {
+  if (nrow <= 0) return 0;
+  if (!ctx()) return -1;
+  return ic0_counts(nrow, (const int *)irow, (const int *)icol, ic0_scratch());
+}
+
+int64_t mpk_ic0_analyse(int64_t irow, int64_t icol, int64_t nrow, int64_t bi, int64_t bj, int64_t bdiag, int64_t trp,
+                        int64_t trow, int64_t tpos, int64_t tmid, int64_t vrow, int64_t vpos, int64_t lrow,
+                        int64_t lptr) {
+  const i64 n = nrow;
+  if (n <= 0) return 0;
+  if (!ctx()) return -1;
+  Ic0Scratch &S = ic0_scratch();
+  const int *rp = (const int *)irow, *ci = (const int *)icol;
+  const i64 total = ic0_counts(n, rp, ci, S);
+  if (total < 0) {
+    if (total == -2) set_error(2, "mpk_ic0_analyse: the matrix is not one mpk_ic0_size accepts");
+    return -1;
+  }
+  int *TRP = (int *)trp;
+  // column starts of the strict part
+  size_t tb = 0;
+  HIPCHK(hipcub::DeviceScan::ExclusiveSum(nullptr, tb, S.ccnt, TRP, (int)(n + 1), stream()));
+  if (!S.ensure_tmp(tb)) return -1;
+  HIPCHK(hipcub::DeviceScan::ExclusiveSum(S.tmp, tb, S.ccnt, TRP, (int)(n + 1), stream()));
+  HIPCHK(hipMemcpyAsync(S.cursor, TRP, (n + 1) * sizeof(int), hipMemcpyDeviceToDevice, stream()));
+  MPK_LAUNCH(k_ic0_fill, nblocks(n, 256), 256, n, rp, ci, (const i64 *)S.b64, (int *)bi, (int *)bj, (int *)bdiag,
+             S.cursor, (int *)trow, (int *)tpos);
+  MPK_LAUNCH(k_ic0_order, nblocks(n, 256), 256, n, (const int *)TRP, (int *)trow, (int *)tpos, (const int *)bi,
+             (const int *)bj, (int *)vrow, (int *)vpos, (int *)tmid);
+  // levels over the transpose (a row's visitors come before it)
+  MPK_LAUNCH(k_ic0_widen, nblocks(n + 1, 256), 256, n, (const int *)TRP, S.t64);
+  return level_lists(n, S.t64, (const int *)trow, (int *)lrow, (int64_t *)lptr, S);
+}
+
+}  // extern "C"

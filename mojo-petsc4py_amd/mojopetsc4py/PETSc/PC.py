@@ -12,6 +12,7 @@ then upload the factor (int32 indices) for the device sweeps.
 
 from __future__ import annotations
 
+import os
 from dataclasses import dataclass
 
 import numpy as np
@@ -349,6 +350,10 @@ class PC:
     def _build_factor(self, m: Mat, n: int):
         if n == 0:
             return
+        if self._type == PCType.ICC and self._ic0_on_device(m, n):
+            return
+        if self._type == PCType.ILU and self._ilu0_on_device(m, n):
+            return
         r, ic = self._permutations(m, n)
         adiag = _device_diag_markers(m, n)
         scratch = np.zeros(7 * n + 8, dtype=np.int64)
@@ -417,6 +422,85 @@ class PC:
         L.mp_icc_contiguous_cholesky(_a(uj), _a(ui), _a(bdiag), _a(bi), _a(bdiag), _a(bj), n)
         # the values come from the device pass (_icc_numeric_device)
         self._factor = {"bi": bi, "bj": bj, "ba": None, "bdiag": bdiag, "r": r, "rinv": rinv}
+
+    def _ic0_on_device(self, m: Mat, n: int) -> bool:
+        """IC(0) with the natural ordering, analysed on the device
+        (mpk_ic0_analyse): the factor layout, its transpose, the row chains'
+        visits and the level lists the host passes below give, without them;
+        the host keeps only the level bounds, and downloads the factor on
+        request (_host_factor).  False, for the host passes, when the library
+        declines the matrix (a row not strictly ascending, a column of the
+        strict part over 256 entries) or MPK_IC0_DEVICE=0."""
+        if (self._opt.levels != 0 or self._opt.ordering == MatOrderingType.RCM
+                or os.environ.get("MPK_IC0_DEVICE") == "0"):
+            return False
+        L = lib()
+        d = m._dev()
+        nnz = int(L.mpk_ic0_size(d.rp.ptr, d.ci.ptr, n))
+        check()
+        if nnz < 0:
+            return False
+        ms = max(nnz - n, 1)
+        bi, bj, bd = self._buf("bi", 4 * (n + 1)), self._buf("bj", 4 * nnz), self._buf("bdiag", 4 * (n + 1))
+        trp, trow, tpos = self._buf("trp", 4 * (n + 1)), self._buf("trow", 4 * ms), self._buf("tpos", 4 * ms)
+        tmid = self._buf("tmid", 4 * n)
+        vrow, vpos, lrow = DeviceBuffer(4 * ms), DeviceBuffer(4 * ms), DeviceBuffer(4 * n)
+        lptr = np.zeros(n + 1, dtype=np.int64)
+        nlev = L.mpk_ic0_analyse(d.rp.ptr, d.ci.ptr, n, bi.ptr, bj.ptr, bd.ptr, trp.ptr, trow.ptr, tpos.ptr,
+                                 tmid.ptr, vrow.ptr, vpos.ptr, lrow.ptr, _a(lptr))
+        check()
+        if nlev < 0:
+            raise RuntimeError("the device IC(0) analysis failed")
+        r = np.arange(n, dtype=np.int64)
+        rd = self._dbufs["r"] = DeviceBuffer.from_array(r.astype(np.int32))
+        u0 = DeviceBuffer(8 * nnz)
+        ua = self._buf("ba", 8 * nnz)
+        # the visits of row k start where its column does in the transpose
+        rc = L.mpk_icc_numeric(n, d.rp.ptr, d.ci.ptr, d.v.ptr, rd.ptr, rd.ptr, bi.ptr, bj.ptr, trp.ptr, vrow.ptr,
+                               vpos.ptr, lrow.ptr, _a(lptr), nlev, u0.ptr, ua.ptr)
+        check()
+        if rc != 0:
+            raise RuntimeError("the device IC factorisation failed")
+        L.mpk_synchronize()
+        self._nlate = 0
+        self._factor = {"bi": None, "bj": None, "ba": None, "bdiag": None, "r": r, "rinv": r, "nnz": nnz}
+        return True
+
+    def _ilu0_on_device(self, m: Mat, n: int) -> bool:
+        """ILU(0) with the natural ordering, analysed on the device
+        (mpk_ilu0_analyse): the layout of mp_ilu_symbolic_ilu0 and the level
+        lists of mpk_ilu_levels, without the host passes; the host keeps only
+        the level bounds and downloads the factor on request (_host_factor).
+        False, for the host passes, when the library declines the matrix (a
+        row not strictly ascending or without its diagonal) or
+        MPK_ILU0_DEVICE=0."""
+        if (self._opt.levels != 0 or self._opt.ordering == MatOrderingType.RCM
+                or os.environ.get("MPK_ILU0_DEVICE") == "0"):
+            return False
+        L = lib()
+        d = m._dev()
+        nnz = int(L.mpk_ilu0_size(d.rp.ptr, d.ci.ptr, n))
+        check()
+        if nnz < 0:
+            return False
+        bi, bj, bd = self._buf("bi", 4 * (n + 1)), self._buf("bj", 4 * nnz), self._buf("bdiag", 4 * (n + 1))
+        lrow = DeviceBuffer(4 * n)
+        lptr = np.zeros(n + 1, dtype=np.int64)
+        nlev = L.mpk_ilu0_analyse(d.rp.ptr, d.ci.ptr, n, bi.ptr, bj.ptr, bd.ptr, lrow.ptr, _a(lptr))
+        check()
+        if nlev < 0:
+            raise RuntimeError("the device ILU(0) analysis failed")
+        r = np.arange(n, dtype=np.int64)
+        rd = self._dbufs["r"] = self._dbufs["c"] = DeviceBuffer.from_array(r.astype(np.int32))
+        ba = self._buf("ba", 8 * nnz)
+        rc = L.mpk_ilu_numeric(n, d.rp.ptr, d.ci.ptr, d.v.ptr, rd.ptr, rd.ptr, bi.ptr, bj.ptr, bd.ptr, lrow.ptr,
+                               _a(lptr), nlev, ba.ptr)
+        check()
+        if rc != 0:
+            raise RuntimeError("the device ILU factorisation failed")
+        L.mpk_synchronize()
+        self._factor = {"bi": None, "bj": None, "ba": None, "bdiag": None, "r": r, "c": r, "ic": r, "nnz": nnz}
+        return True
 
     def _upload_factor(self, m: Mat, n: int):
         f = self._factor
@@ -525,6 +609,10 @@ class PC:
         """The factor with its values on the host (the values are computed on
         the device and downloaded on first request)."""
         f = self._factor
+        if f is not None and f["bi"] is None:  # analysed on the device
+            n, b = len(f["r"]), self._dbufs
+            for key, count in (("bi", n + 1), ("bdiag", n + 1), ("bj", int(f["nnz"]))):
+                f[key] = b[key].to_numpy(np.int32, count).astype(np.int64)
         if f is not None and f["ba"] is None:
             f["ba"] = self._dbufs["ba"].to_numpy(np.float64, max(int(f["nnz"]), 1))
         return f

@@ -128,6 +128,10 @@ SIGNATURES = {
     "mpk_ilu_numeric": ([I] * 13, I),
     "mpk_icc_visits": ([I] * 12, I),
     "mpk_icc_numeric": ([I] * 16, I),
+    "mpk_ic0_size": ([I] * 3, I),
+    "mpk_ic0_analyse": ([I] * 14, I),
+    "mpk_ilu0_size": ([I] * 3, I),
+    "mpk_ilu0_analyse": ([I] * 8, I),
 }
 
 # exported names that only a device call can exercise
