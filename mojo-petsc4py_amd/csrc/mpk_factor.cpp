@@ -692,6 +692,68 @@ static int plan_mode() {
   return 0;
 }
 
+// Pencil blocks.  A sweep whose every dependency lies at one of three
+// offsets 1 < a < b (a | b, b | n) reads as the natural ordering of an
+// a x (b / a) x (n / b) grid with x fastest (7-point and lighter stencils).
+// Blocks of consecutive positions are then slabs of the (x, y) planes: each
+// block waits on the one below for every row, and the sweep's end moves one
+// publish-to-poll hop per slab.  Blocks of t x t columns of the (x, y) plane
+// spanning every z, taken in row-major tile order (reversed for a backward
+// sweep), put about (a + b / a) / t hops on that path instead; every row's
+// z chain stays inside its block.  t is the largest with t^2 (n / b) <= bmax.
+// The order is kept only when it is topological for the sweep (every
+// dependency at an earlier position), checked here; MPK_SWEEP_PENCILS=0
+// turns it off.  ord[s] is the row at sweep position s, bstart the blocks'
+// first positions (nblk + 1 of them).
+static bool pencil_order(int64_t n, bool forward, const int64_t *off, const int *col, int64_t bmax,
+                         std::vector<int> &ord, std::vector<int64_t> &bstart) {
+  if (env_off("MPK_SWEEP_PENCILS") || n <= bmax) return false;
+  int64_t d[3] = {0, 0, 0};
+  int nd = 0;
+  for (int64_t i = 0; i < n; i++)
+    for (int64_t e = off[i]; e < off[i + 1]; e++) {
+      const int64_t k = col[e], dist = forward ? i - k : k - i;
+      if (dist <= 0) continue;
+      if (dist != d[0] && dist != d[1] && dist != d[2]) {
+        if (nd == 3) return false;
+        d[nd++] = dist;
+      }
+    }
+  if (nd != 3) return false;
+  std::sort(d, d + 3);
+  if (d[0] != 1 || d[2] % d[1] || n % d[2]) return false;
+  const int64_t nx = d[1], ny = d[2] / d[1], nz = n / d[2];
+  int64_t t = 1;
+  while ((t + 1) * (t + 1) * nz <= bmax) t++;
+  const int64_t tx = std::min(t, nx), ty = std::min(t, ny);
+  const int64_t mx = (nx + tx - 1) / tx, my = (ny + ty - 1) / ty;
+  if (tx * ty < 4 || mx * my < 2) return false;
+  ord.clear();
+  ord.reserve((size_t)n);
+  bstart.assign(1, 0);
+  for (int64_t by = 0; by < my; by++)
+    for (int64_t bx = 0; bx < mx; bx++) {
+      for (int64_t z = 0; z < nz; z++)
+        for (int64_t y = by * ty; y < std::min(ny, (by + 1) * ty); y++)
+          for (int64_t x = bx * tx; x < std::min(nx, (bx + 1) * tx); x++) ord.push_back((int)(z * d[2] + y * nx + x));
+      bstart.push_back((int64_t)ord.size());
+    }
+  if (!forward) {
+    std::reverse(ord.begin(), ord.end());
+    std::vector<int64_t> rb(bstart.size());
+    for (size_t j = 0; j < bstart.size(); j++) rb[j] = n - bstart[bstart.size() - 1 - j];
+    bstart.swap(rb);
+  }
+  std::vector<int> rank((size_t)n);
+  for (int64_t s = 0; s < n; s++) rank[(size_t)ord[(size_t)s]] = (int)s;
+  for (int64_t i = 0; i < n; i++)
+    for (int64_t e = off[i]; e < off[i + 1]; e++) {
+      const int64_t k = col[e];
+      if ((forward ? k < i : k > i) && rank[(size_t)k] >= rank[(size_t)i]) return false;
+    }
+  return true;
+}
+
 // Writes the plan into out [H] (cap bytes).  Returns its size in bytes, or
 // minus the size needed when cap is too small.
 int64_t mpk_sweep_plan(int64_t n, int64_t forward, int64_t off_, int64_t col_, int64_t coef_, int64_t bmax,
@@ -792,16 +854,30 @@ int64_t mpk_sweep_plan(int64_t n, int64_t forward, int64_t off_, int64_t col_, i
     }
   }
 blocks:
-  const int64_t B = plan_rows(n, bmax), nblk = n > 0 ? (n + B - 1) / B : 0;
+  // the blocks: pencils, or runs of B consecutive sweep positions
+  std::vector<int> ord;
+  std::vector<int64_t> bstart;
+  if (mode != 2 && pencil_order(n, forward, off, col, bmax, ord, bstart)) {
+  } else {
+    const int64_t B = plan_rows(n, bmax), nb = n > 0 ? (n + B - 1) / B : 0;
+    ord.resize((size_t)n);
+    for (int64_t s = 0; s < n; s++) ord[(size_t)s] = (int)spos(s);
+    bstart.resize((size_t)nb + 1);
+    for (int64_t b = 0; b <= nb; b++) bstart[(size_t)b] = std::min(n, b * B);
+  }
+  const int64_t nblk = (int64_t)bstart.size() - 1;
+  std::vector<int> rblk(n > 0 ? n : 1);  // each row's block
+  for (int64_t b = 0; b < nblk; b++)
+    for (int64_t s = bstart[(size_t)b]; s < bstart[(size_t)b + 1]; s++) rblk[(size_t)ord[(size_t)s]] = (int)b;
   // local levels
   std::vector<int> lev(n > 0 ? n : 1, 0);
   std::vector<int> nlev(nblk > 0 ? nblk : 1, 0);
   for (int64_t s = 0; s < n; s++) {
-    const int64_t i = forward ? s : n - 1 - s, b = s / B;
+    const int64_t i = ord[(size_t)s], b = rblk[(size_t)i];
     int l = 0;
     for (int64_t e = off[i]; e < off[i + 1]; e++) {
       const int64_t k = col[e];
-      if (earlier(k, i) && spos(k) / B == b) l = std::max(l, lev[k] + 1);
+      if (earlier(k, i) && rblk[(size_t)k] == b) l = std::max(l, lev[k] + 1);
     }
     lev[i] = l;
     nlev[b] = std::max(nlev[b], l + 1);
@@ -831,9 +907,9 @@ blocks:
   std::vector<int> pos(n > 0 ? n : 1);
   int64_t li = 0;
   for (int64_t b = 0; b < nblk; b++) {
-    const int64_t s0 = b * B, s1 = std::min(n, s0 + B);
+    const int64_t s0 = bstart[(size_t)b], s1 = bstart[(size_t)b + 1];
     std::vector<int64_t> start((size_t)nlev[b] + 1, 0);
-    for (int64_t s = s0; s < s1; s++) start[(size_t)lev[forward ? s : n - 1 - s] + 1]++;
+    for (int64_t s = s0; s < s1; s++) start[(size_t)lev[ord[(size_t)s]] + 1]++;
     for (size_t l = 0; l + 1 < start.size(); l++) start[l + 1] += start[l];
     blk_pos[b] = (int)s0;
     blk_lev[b] = (int)li;
@@ -841,7 +917,7 @@ blocks:
     lvl[li + nlev[b]] = (int)s1;
     li += nlev[b] + 1;
     for (int64_t s = s0; s < s1; s++) {
-      const int64_t i = forward ? s : n - 1 - s;
+      const int64_t i = ord[(size_t)s];
       const int64_t p = s0 + start[(size_t)lev[i]]++;
       prow[p] = (int)i;
       pos[i] = (int)p;
@@ -853,7 +929,7 @@ blocks:
   // so that the rows of a level read each slot from consecutive addresses)
   int64_t e_out = 0;
   for (int64_t p = 0; p < n; p++) {
-    const int64_t i = prow[p], b = spos(i) / B;
+    const int64_t i = prow[p], b = rblk[(size_t)i];
     pent[p] = (int)e_out;
     pcnt[p] = (int)(off[i + 1] - off[i]);
     for (int64_t e = off[i]; e < off[i + 1]; e++, e_out++) {
@@ -861,7 +937,7 @@ blocks:
       ecoef[q] = coef[e];
       if (!earlier(k, i))
         edep[q] = (int)(-k - n - 1);
-      else if (spos(k) / B == b)
+      else if (rblk[(size_t)k] == b)
         edep[q] = pos[k] - blk_pos[b];
       else
         edep[q] = (int)(-k - 1);

@@ -108,22 +108,29 @@ __global__ void __launch_bounds__(64) k_plan_glev(i64 n, int forward, const i64 
   }
 }
 
-// Local levels of block b (sweep positions b B .. min(n, (b + 1) B) - 1): a
-// row's level is 1 + the largest level of a dependency earlier in the sweep
-// and in the same block, 0 without one.  One wave per block takes its rows
-// 64 at a time (a row's dependency in an earlier group is done; in its own
-// group the wave repeats until it is); lev gets every row's level, nlev[b]
-// the block's level count.
-__global__ void __launch_bounds__(64) k_plan_local(i64 n, int forward, i64 B, const i64 *off, const int *col,
+// The block order.  ord[s]: the row at sweep position s; rank: its inverse;
+// rblk: each row's block; block b holds positions bstart[b] .. bstart[b+1]-1.
+// Every dependency (an earlier row in the sweep's direction, plan_earlier)
+// sits at an earlier position: the identity order (or its reverse) and the
+// pencil order (checked by k_pen_topo) both are.
+__device__ __forceinline__ bool plan_earlier(i64 k, i64 i, int forward) { return forward ? k < i : k > i; }
+
+// Local levels of block b: a row's level is 1 + the largest level of a
+// dependency in the same block, 0 without one.  One wave per block takes its
+// rows 64 positions at a time (a row's dependency in an earlier group is
+// done; in its own group the wave repeats until it is); lev gets every row's
+// level, nlev[b] the block's level count.
+__global__ void __launch_bounds__(64) k_plan_local(i64 n, int forward, const i64 *bstart, const int *ord,
+                                                   const int *rank, const int *rblk, const i64 *off, const int *col,
                                                    int *lev, int *nlev) {
   __shared__ int llev[MPK_BLK_ROWS];
   const int lane = (int)threadIdx.x;
-  const i64 b = blockIdx.x, s0 = b * B, s1 = s0 + B < n ? s0 + B : n;
+  const i64 b = blockIdx.x, s0 = bstart[b], s1 = bstart[b + 1];
   int mx = -1;
   for (i64 c0 = s0; c0 < s1; c0 += 64) {
     const i64 s = c0 + lane;
     const bool valid = s < s1;
-    const i64 i = valid ? plan_spos(s, n, forward) : 0;
+    const i64 i = valid ? ord[s] : 0;
     bool done = !valid;
     i64 j = valid ? off[i] : 0;
     const i64 je = valid ? off[i + 1] : 0;
@@ -132,8 +139,9 @@ __global__ void __launch_bounds__(64) k_plan_local(i64 n, int forward, i64 B, co
     for (;;) {
       if (!done) {
         while (j < je) {
-          const i64 sk = plan_spos(col[j], n, forward);
-          if (sk >= s0 && sk < s) {  // earlier in the sweep and in this block
+          const i64 k = col[j];
+          if (plan_earlier(k, i, forward) && rblk[k] == b) {  // a dependency in this block
+            const i64 sk = rank[k];
             if (sk >= c0 && !((mask >> (sk - c0)) & 1ull)) break;
             const int v = llev[sk - s0];
             l = v + 1 > l ? v + 1 : l;
@@ -168,16 +176,16 @@ __global__ void __launch_bounds__(64) k_plan_local(i64 n, int forward, i64 B, co
 // one wave).  Writes blk_pos[b], blk_lev[b], the block's lvl entries, prow
 // and pos (row -> position), and wlev[b] = its widest level.  li[b] is the
 // block's first entry in lvl; block 0 also ends blk_pos and blk_lev.
-__global__ void __launch_bounds__(64) k_plan_pos(i64 n, int forward, i64 B, i64 nblk, i64 nlvl, const int *lev,
-                                                 const i64 *li, const int *nlev, int *blk_pos, int *blk_lev, int *lvl,
-                                                 int *prow, int *pos, int *wlev) {
+__global__ void __launch_bounds__(64) k_plan_pos(i64 n, const i64 *bstart, const int *ord, i64 nblk, i64 nlvl,
+                                                 const int *lev, const i64 *li, const int *nlev, int *blk_pos,
+                                                 int *blk_lev, int *lvl, int *prow, int *pos, int *wlev) {
   __shared__ int cnt[MPK_BLK_ROWS + 1];
   const int lane = (int)threadIdx.x;
-  const i64 b = blockIdx.x, s0 = b * B, s1 = s0 + B < n ? s0 + B : n;
+  const i64 b = blockIdx.x, s0 = bstart[b], s1 = bstart[b + 1];
   const int L = nlev[b];
   for (int l = lane; l <= L; l += 64) cnt[l] = 0;
   __syncthreads();
-  for (i64 s = s0 + lane; s < s1; s += 64) atomicAdd(&cnt[lev[plan_spos(s, n, forward)] + 1], 1);
+  for (i64 s = s0 + lane; s < s1; s += 64) atomicAdd(&cnt[lev[ord[s]] + 1], 1);
   __syncthreads();
   // cnt[l] = the rows at levels below l (a running prefix over 64 at a time)
   int carry = 0, widest = 0;
@@ -217,7 +225,7 @@ __global__ void __launch_bounds__(64) k_plan_pos(i64 n, int forward, i64 B, i64 
   for (i64 c0 = s0; c0 < s1; c0 += 64) {
     const i64 s = c0 + lane;
     const bool valid = s < s1;
-    const i64 i = valid ? plan_spos(s, n, forward) : 0;
+    const i64 i = valid ? ord[s] : 0;
     const int l = valid ? lev[i] : -1;
     int rank = 0, tot = 0;
     for (int q = 0; q < 64; q++) {
@@ -244,25 +252,25 @@ __global__ void __launch_bounds__(64) k_plan_pos(i64 n, int forward, i64 B, i64 
 // -n <= d < 0: row -d - 1 of an earlier block; d < -n: row -d - n - 1, not
 // earlier in the sweep).  flags: 1 when an entry reads an earlier block's
 // row, 2 when one reads a row not earlier in the sweep.
-__global__ void k_plan_entries(i64 n, int forward, i64 B, i64 w, const i64 *off, const int *col, const double *coef,
-                               const int *prow, const int *pos, const int *blk_pos, const i64 *eout, int *pcnt,
-                               int *pent, double *ecoef, int *edep, unsigned *flags) {
+__global__ void k_plan_entries(i64 n, int forward, const int *rblk, i64 w, const i64 *off, const int *col,
+                               const double *coef, const int *prow, const int *pos, const int *blk_pos,
+                               const i64 *eout, int *pcnt, int *pent, double *ecoef, int *edep, unsigned *flags) {
   const i64 p = (i64)blockIdx.x * 256 + threadIdx.x;
   unsigned f = 0;
   if (p < n) {
-    const i64 i = prow[p], si = plan_spos(i, n, forward), b = si / B, o = off[i], cnt = off[i + 1] - o;
+    const i64 i = prow[p], b = rblk[i], o = off[i], cnt = off[i + 1] - o;
     pcnt[p] = (int)cnt;
     pent[p] = w ? 0 : (int)eout[p];
     const i64 q0 = w ? p : eout[p], qs = w ? n : 1;
     for (i64 e = 0; e < cnt; e++) {
-      const i64 k = col[o + e], sk = plan_spos(k, n, forward);
+      const i64 k = col[o + e];
       const i64 q = q0 + e * qs;
       ecoef[q] = coef[o + e];
       int d;
-      if (sk >= si) {
+      if (!plan_earlier(k, i, forward)) {
         d = (int)(-k - n - 1);
         f |= 2u;
-      } else if (sk / B == b) {
+      } else if (rblk[k] == b) {
         d = pos[k] - blk_pos[b];
       } else {
         d = (int)(-k - 1);
@@ -400,6 +408,106 @@ __global__ void k_plan_ring_entries(i64 n, int forward, i64 W, i64 w, const i64 
   if ((threadIdx.x & 63) == 0 && (any1 | any2)) atomicOr(flags, (any1 ? 1u : 0u) | (any2 ? 2u : 0u));
 }
 
+// Block orders (the host planner's, mpk_sweep_plan / pencil_order): runs of B
+// consecutive sweep positions ...
+__global__ void k_plan_ident(i64 n, int forward, i64 B, int *rank, int *ord, int *rblk) {
+  const i64 i = (i64)blockIdx.x * 256 + threadIdx.x;
+  if (i >= n) return;
+  const i64 s = plan_spos(i, n, forward);
+  rank[i] = (int)s;
+  ord[s] = (int)i;
+  rblk[i] = (int)(s / B);
+}
+// ... or pencils: tiles of tx x ty columns of an nx x ny x nz grid (x
+// fastest) spanning every z, in row-major tile order, rows in index order
+// within a tile (z, then y, then x); reversed for a backward sweep.  A tile
+// (bx, by) starts after the full tile rows below it and the tiles to its
+// left in its own row.
+__global__ void k_pen_rank(i64 n, int forward, i64 nx, i64 ny, i64 nz, i64 tx, i64 ty, i64 mx, i64 nblk, int *rank,
+                           int *ord, int *rblk) {
+  const i64 i = (i64)blockIdx.x * 256 + threadIdx.x;
+  if (i >= n) return;
+  const i64 plane = nx * ny, z = i / plane, rem = i - z * plane, y = rem / nx, x = rem - y * nx;
+  const i64 by = y / ty, bx = x / tx;
+  const i64 wy = ty < ny - by * ty ? ty : ny - by * ty, wx = tx < nx - bx * tx ? tx : nx - bx * tx;
+  i64 r = nz * (by * ty * nx + wy * bx * tx) + z * wy * wx + (y - by * ty) * wx + (x - bx * tx);
+  i64 t = by * mx + bx;
+  if (!forward) {
+    r = n - 1 - r;
+    t = nblk - 1 - t;
+  }
+  rank[i] = (int)r;
+  ord[r] = (int)i;
+  rblk[i] = (int)t;
+}
+// Pencil detection: the sweep's dependency offsets (forward: i - k, backward:
+// k - i) must be exactly three, 1 < a < b.  pm[0]: the smallest, pm[1]: the
+// largest (k_pen_span); pm[2]: the smallest strictly between 1 and the
+// largest (k_pen_mid); flags bit 0: an offset outside {1, a, b}
+// (k_pen_check), bit 1: a dependency at a later position of the order
+// (k_pen_topo).  Each row's values are folded over its wave first.
+__device__ __forceinline__ i64 pen_dist(i64 i, i64 k, int forward) { return forward ? i - k : k - i; }
+__global__ void k_pen_span(i64 n, int forward, const i64 *off, const int *col, int *pm) {
+  const i64 i = (i64)blockIdx.x * 256 + threadIdx.x;
+  int lo = 0x7fffffff, hi = 0;
+  if (i < n)
+    for (i64 j = off[i]; j < off[i + 1]; j++) {
+      const i64 dd = pen_dist(i, col[j], forward);
+      if (dd > 0) {
+        lo = dd < lo ? (int)dd : lo;
+        hi = dd > hi ? (int)dd : hi;
+      }
+    }
+#pragma unroll
+  for (int w = 32; w >= 1; w >>= 1) {
+    const int a = __shfl_down(lo, w, 64), b = __shfl_down(hi, w, 64);
+    lo = a < lo ? a : lo;
+    hi = b > hi ? b : hi;
+  }
+  if ((threadIdx.x & 63) == 0) {
+    atomicMin(pm, lo);
+    atomicMax(pm + 1, hi);
+  }
+}
+__global__ void k_pen_mid(i64 n, int forward, const i64 *off, const int *col, int *pm) {
+  const i64 i = (i64)blockIdx.x * 256 + threadIdx.x;
+  const i64 top = pm[1];
+  int lo = 0x7fffffff;
+  if (i < n)
+    for (i64 j = off[i]; j < off[i + 1]; j++) {
+      const i64 dd = pen_dist(i, col[j], forward);
+      if (dd > 1 && dd < top) lo = dd < lo ? (int)dd : lo;
+    }
+#pragma unroll
+  for (int w = 32; w >= 1; w >>= 1) {
+    const int a = __shfl_down(lo, w, 64);
+    lo = a < lo ? a : lo;
+  }
+  if ((threadIdx.x & 63) == 0) atomicMin(pm + 2, lo);
+}
+__global__ void k_pen_check(i64 n, int forward, const i64 *off, const int *col, const int *pm, unsigned *flags) {
+  const i64 i = (i64)blockIdx.x * 256 + threadIdx.x;
+  bool bad = false;
+  if (i < n) {
+    const i64 a = pm[2], b = pm[1];
+    for (i64 j = off[i]; j < off[i + 1]; j++) {
+      const i64 dd = pen_dist(i, col[j], forward);
+      bad |= dd > 0 && dd != 1 && dd != a && dd != b;
+    }
+  }
+  if (__ballot(bad) && (threadIdx.x & 63) == 0) atomicOr(flags, 1u);
+}
+__global__ void k_pen_topo(i64 n, int forward, const i64 *off, const int *col, const int *rank, unsigned *flags) {
+  const i64 i = (i64)blockIdx.x * 256 + threadIdx.x;
+  bool bad = false;
+  if (i < n)
+    for (i64 j = off[i]; j < off[i + 1]; j++) {
+      const i64 k = col[j];
+      bad |= plan_earlier(k, i, forward) && rank[k] >= rank[i];
+    }
+  if (__ballot(bad) && (threadIdx.x & 63) == 0) atomicOr(flags, 2u);
+}
+
 // Device scratch of the planner, grown on demand and kept.
 struct PlanScratch {
   int *lev = nullptr, *pos = nullptr, *nlev = nullptr, *wlev = nullptr, *maxlev = nullptr, *cnt = nullptr;
@@ -407,16 +515,19 @@ struct PlanScratch {
   i64 *li = nullptr, *eout = nullptr;
   unsigned *flags = nullptr;
   unsigned long long *old = nullptr, *lw = nullptr;
-  i64 *toff = nullptr;
-  int *tcol = nullptr;
+  i64 *toff = nullptr, *bstart = nullptr;
+  int *tcol = nullptr, *ord = nullptr, *rank = nullptr, *rblk = nullptr, *pen = nullptr;
   void *sort_tmp = nullptr;
   size_t sort_cap = 0;
   i64 cap_n = 0, cap_b = 0, cap_t = 0;
   bool ensure(i64 n, i64 nblk) {
     if (n > cap_n) {
       for (void *q : {(void *)lev, (void *)pos, (void *)eout, (void *)cnt, (void *)late, (void *)key, (void *)skey,
-                      (void *)row, (void *)lw, (void *)toff})
+                      (void *)row, (void *)lw, (void *)toff, (void *)ord, (void *)rank, (void *)rblk})
         if (q) HIPCHK(hipFree(q));
+      HIPCHK(hipMalloc(&ord, n * sizeof(int)));
+      HIPCHK(hipMalloc(&rank, n * sizeof(int)));
+      HIPCHK(hipMalloc(&rblk, n * sizeof(int)));
       HIPCHK(hipMalloc(&lw, (n + 1) * sizeof(unsigned long long)));
       HIPCHK(hipMalloc(&toff, (n + 1) * sizeof(i64)));
       HIPCHK(hipMalloc(&lev, n * sizeof(int)));
@@ -430,14 +541,16 @@ struct PlanScratch {
       cap_n = n;
     }
     if (nblk + 1 > cap_b) {
-      for (void *q : {(void *)nlev, (void *)wlev, (void *)li})
+      for (void *q : {(void *)nlev, (void *)wlev, (void *)li, (void *)bstart})
         if (q) HIPCHK(hipFree(q));
+      HIPCHK(hipMalloc(&bstart, (nblk + 1) * sizeof(i64)));
       HIPCHK(hipMalloc(&nlev, (nblk + 1) * sizeof(int)));
       HIPCHK(hipMalloc(&wlev, (nblk + 1) * sizeof(int)));
       HIPCHK(hipMalloc(&li, (nblk + 1) * sizeof(i64)));
       cap_b = nblk + 1;
     }
     if (!maxlev) {
+      HIPCHK(hipMalloc(&pen, 8 * sizeof(int)));
       HIPCHK(hipMalloc(&maxlev, 4 * sizeof(int)));
       HIPCHK(hipMalloc(&flags, 4 * sizeof(unsigned)));
       HIPCHK(hipMalloc(&old, sizeof(unsigned long long)));
@@ -586,6 +699,64 @@ static PlanScratch &plan_scratch() {
   return s;
 }
 
+// The blocks of a block plan, as the host planner picks them: pencils when
+// the sweep is a structured grid's and their order is topological for it,
+// else runs of plan_rows consecutive positions.  Fills S.ord, S.rank, S.rblk
+// and S.bstart; returns the block count, or -1.
+static i64 plan_order(const DevEntries &E, bool forward, i64 bmax, int mode, PlanScratch &S) {
+  const i64 n = E.n;
+  const int fw = forward ? 1 : 0;
+  std::vector<i64> bs;
+  bool pen = false;
+  if (mode != 2 && n > bmax && !plan_env_off("MPK_SWEEP_PENCILS")) {
+    const int init[4] = {0x7fffffff, 0, 0x7fffffff, 0};
+    copy_htod(S.pen, init, sizeof init);
+    MPK_LAUNCH(k_pen_span, nblocks(n, 256), 256, n, fw, E.off, E.col, S.pen);
+    MPK_LAUNCH(k_pen_mid, nblocks(n, 256), 256, n, fw, E.off, E.col, S.pen);
+    MPK_LAUNCH(k_pen_check, nblocks(n, 256), 256, n, fw, E.off, E.col, (const int *)S.pen,
+               reinterpret_cast<unsigned *>(S.pen + 3));
+    int pm[4];
+    copy_dtoh(pm, S.pen, sizeof pm);
+    if (failed()) return -1;
+    const i64 a = pm[2], b = pm[1];
+    if (pm[0] == 1 && pm[3] == 0 && a < b && b % a == 0 && n % b == 0) {
+      const i64 nx = a, ny = b / a, nz = n / b;
+      i64 t = 1;
+      while ((t + 1) * (t + 1) * nz <= bmax) t++;
+      const i64 tx = std::min(t, nx), ty = std::min(t, ny), mx = (nx + tx - 1) / tx, my = (ny + ty - 1) / ty;
+      if (tx * ty >= 4 && mx * my >= 2) {
+        const i64 nb = mx * my;
+        bs.assign(1, 0);
+        for (i64 by = 0; by < my; by++)
+          for (i64 bx = 0; bx < mx; bx++)
+            bs.push_back(bs.back() + std::min(tx, nx - bx * tx) * std::min(ty, ny - by * ty) * nz);
+        if (!forward) {
+          std::vector<i64> rb(bs.size());
+          for (size_t j = 0; j < bs.size(); j++) rb[j] = n - bs[bs.size() - 1 - j];
+          bs.swap(rb);
+        }
+        MPK_LAUNCH(k_pen_rank, nblocks(n, 256), 256, n, fw, nx, ny, nz, tx, ty, mx, nb, S.rank, S.ord, S.rblk);
+        MPK_LAUNCH(k_pen_topo, nblocks(n, 256), 256, n, fw, E.off, E.col, (const int *)S.rank,
+                   reinterpret_cast<unsigned *>(S.pen + 3));
+        unsigned fl = 0;
+        copy_dtoh(&fl, S.pen + 3, sizeof fl);
+        if (failed()) return -1;
+        pen = fl == 0;
+      }
+    }
+  }
+  if (!pen) {
+    const i64 B = plan_rows_rule(n, bmax), nb = (n + B - 1) / B;
+    bs.resize(nb + 1);
+    for (i64 b = 0; b <= nb; b++) bs[b] = std::min(n, b * B);
+    MPK_LAUNCH(k_plan_ident, nblocks(n, 256), 256, n, fw, B, S.rank, S.ord, S.rblk);
+  }
+  const i64 nblk = (i64)bs.size() - 1;
+  if (!S.ensure(n, nblk)) return -1;
+  copy_htod(S.bstart, bs.data(), bs.size() * sizeof(i64));
+  return failed() ? -1 : nblk;
+}
+
 i64 device_sweep_plan(const DevEntries &E, bool forward, i64 bmax, char *dst, i64 cap, i64 *depth_out,
                       int *longest, int *widest_level, int *mark) {
   Ctx *c = ctx();
@@ -594,9 +765,8 @@ i64 device_sweep_plan(const DevEntries &E, bool forward, i64 bmax, char *dst, i6
   const int fw = forward ? 1 : 0;
   const i64 w = plan_w_rule(E.widest, n, m), slots = w ? w * n : m;
   const int mode = plan_mode_rule();
-  const i64 B = plan_rows_rule(n, bmax), nblk = (n + B - 1) / B;
   PlanScratch &S = plan_scratch();
-  if (!S.ensure(n, nblk)) return -1;
+  if (!S.ensure(n, 1)) return -1;
   watch_install();
   if (n > bmax && mode != 1) {
     // the ring test needs the sweep's depth: a ring is possible when the
@@ -617,8 +787,11 @@ i64 device_sweep_plan(const DevEntries &E, bool forward, i64 bmax, char *dst, i6
       }
     }
   }
-  // blocks: local levels, then the sizes the layout needs
-  MPK_LAUNCH(k_plan_local, nblk, 64, n, fw, B, E.off, E.col, S.lev, S.nlev);
+  // blocks: their order, local levels, then the sizes the layout needs
+  const i64 nblk = plan_order(E, forward, bmax, mode, S);
+  if (nblk < 0) return -1;
+  MPK_LAUNCH(k_plan_local, nblk, 64, n, fw, (const i64 *)S.bstart, (const int *)S.ord, (const int *)S.rank,
+             (const int *)S.rblk, E.off, E.col, S.lev, S.nlev);
   std::vector<int> nlev(nblk);
   copy_dtoh(nlev.data(), S.nlev, nblk * sizeof(int));
   if (failed()) return -1;
@@ -644,11 +817,13 @@ i64 device_sweep_plan(const DevEntries &E, bool forward, i64 bmax, char *dst, i6
   const i64 ints = 2 * (nblk + 1) + nlvl + 3 * n + 1;
   double *ecoef = reinterpret_cast<double *>(dst + 64 + ((4 * ints + 7) & ~(i64)7));
   int *edep = reinterpret_cast<int *>(ecoef + slots);
-  MPK_LAUNCH(k_plan_pos, nblk, 64, n, fw, B, nblk, nlvl, (const int *)S.lev, (const i64 *)S.li,
+  MPK_LAUNCH(k_plan_pos, nblk, 64, n, (const i64 *)S.bstart, (const int *)S.ord, nblk, nlvl, (const int *)S.lev,
+             (const i64 *)S.li,
              (const int *)S.nlev, blk_pos, blk_lev, lvl, prow, S.pos, S.wlev);
   if (!w && !plan_eout(n, E.off, prow, pent, S)) return -1;  // CSR form's offsets
   HIPCHK(hipMemsetAsync(S.flags, 0, sizeof(unsigned), stream()));
-  MPK_LAUNCH(k_plan_entries, nblocks(n, 256), 256, n, fw, B, w, E.off, E.col, E.coef, (const int *)prow,
+  MPK_LAUNCH(k_plan_entries, nblocks(n, 256), 256, n, fw, (const int *)S.rblk, w, E.off, E.col, E.coef,
+             (const int *)prow,
              (const int *)S.pos, (const int *)blk_pos, (const i64 *)S.eout, pcnt, pent, ecoef, edep, S.flags);
   std::vector<int> wl(nblk);
   copy_dtoh(wl.data(), S.wlev, nblk * sizeof(int));

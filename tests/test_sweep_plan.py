@@ -130,30 +130,82 @@ def _restated_ring(n, forward, off, col, coef, bmax, mode):
                 edep=np.array(edep))
 
 
+def _pencils(n, forward, off, col, bmax):
+    """The pencil blocks (rows of each block in sweep order) when the sweep is
+    a structured grid's: dependency offsets exactly {1, a, b}, a | b, b | n,
+    read as an a x (b/a) x (n/b) grid with x fastest; blocks of t x t columns
+    of the (x, y) plane spanning every z (t the largest with t^2 (n/b) <=
+    bmax, at least 4 columns and 2 blocks), in row-major tile order, reversed
+    for a backward sweep, kept when every dependency comes earlier.  None
+    otherwise."""
+    if n <= bmax:
+        return None
+    dists = set()
+    for i in range(n):
+        for e in range(off[i], off[i + 1]):
+            d = i - col[e] if forward else col[e] - i
+            if d > 0:
+                dists.add(int(d))
+    if len(dists) != 3:
+        return None
+    one, a, b = sorted(dists)
+    if one != 1 or b % a or n % b:
+        return None
+    nx, ny, nz = a, b // a, n // b
+    t = 1
+    while (t + 1) ** 2 * nz <= bmax:
+        t += 1
+    tx, ty = min(t, nx), min(t, ny)
+    mx, my = -(-nx // tx), -(-ny // ty)
+    if tx * ty < 4 or mx * my < 2:
+        return None
+    blocks = []
+    for by in range(my):
+        for bx in range(mx):
+            blocks.append([z * b + y * nx + x for z in range(nz) for y in range(by * ty, min(ny, (by + 1) * ty))
+                           for x in range(bx * tx, min(nx, (bx + 1) * tx))])
+    if not forward:
+        blocks = [list(reversed(r)) for r in reversed(blocks)]
+    rank = np.empty(n, np.int64)
+    rank[[i for r in blocks for i in r]] = np.arange(n)
+    earlier = (lambda k, i: k < i) if forward else (lambda k, i: k > i)
+    for i in range(n):
+        for e in range(off[i], off[i + 1]):
+            if earlier(col[e], i) and rank[col[e]] >= rank[i]:
+                return None
+    return blocks
+
+
 def _restated(n, forward, off, col, coef, bmax, mode="auto"):
     ring = _restated_ring(n, forward, off, col, coef, bmax, mode)
     if ring is not None:
         return ring
-    B = n if n <= bmax else min(bmax, max(1024, -(-n // 256)))
-    nblk = -(-n // B)
     spos = (lambda i: i) if forward else (lambda i: n - 1 - i)
     earlier = (lambda k, i: k < i) if forward else (lambda k, i: k > i)
+    blocks = _pencils(n, forward, off, col, bmax) if mode != "ring" else None
+    if blocks is None:
+        B = n if n <= bmax else min(bmax, max(1024, -(-n // 256)))
+        blocks = [[spos(s) for s in range(b * B, min(n, (b + 1) * B))] for b in range(-(-n // B))]
+    nblk = len(blocks)
+    blk = np.empty(n, np.int64)
+    for b, rows in enumerate(blocks):
+        blk[rows] = b
     lev = np.zeros(n, np.int64)
-    for s in range(n):
-        i = spos(s)
-        for e in range(off[i], off[i + 1]):
-            k = col[e]
-            if earlier(k, i) and spos(k) // B == s // B:
-                lev[i] = max(lev[i], lev[k] + 1)
+    for rows in blocks:
+        for i in rows:
+            for e in range(off[i], off[i + 1]):
+                k = col[e]
+                if earlier(k, i) and blk[k] == blk[i]:
+                    lev[i] = max(lev[i], lev[k] + 1)
     prow, pos, blk_pos, blk_lev, lvl = [], np.zeros(n, np.int64), [], [], []
-    for b in range(nblk):
-        rows = [spos(s) for s in range(b * B, min(n, (b + 1) * B))]
-        rows.sort(key=lambda i: lev[i])  # stable: sweep order inside a level
-        blk_pos.append(b * B)
+    for rows in blocks:
+        rows = sorted(rows, key=lambda i: lev[i])  # stable: sweep order inside a level
+        p0 = len(prow)
+        blk_pos.append(p0)
         blk_lev.append(len(lvl))
         nl = int(max(lev[r] for r in rows)) + 1
-        starts = [b * B + sum(1 for r in rows if lev[r] < lv) for lv in range(nl)]
-        lvl += starts + [b * B + len(rows)]
+        starts = [p0 + sum(1 for r in rows if lev[r] < lv) for lv in range(nl)]
+        lvl += starts + [p0 + len(rows)]
         for r in rows:
             pos[r] = len(prow)
             prow.append(r)
@@ -161,13 +213,13 @@ def _restated(n, forward, off, col, coef, bmax, mode="auto"):
     blk_lev.append(len(lvl))
     pent, ecoef, edep = [0], [], []
     for p, i in enumerate(prow):
-        b = spos(i) // B
+        b = blk[i]
         for e in range(off[i], off[i + 1]):
             k = col[e]
             ecoef.append(coef[e])
             if not earlier(k, i):
                 edep.append(-k - n - 1)
-            elif spos(k) // B == b:
+            elif blk[k] == b:
                 edep.append(pos[k] - blk_pos[b])
             else:
                 edep.append(-k - 1)
@@ -181,21 +233,21 @@ def _check(plan, n, forward, off, col, coef):
     the row's level runs: local reads from lower levels of the same block
     (in a ring plan, from the slot of a position still inside the window of
     W positions that ends with the row's level), external reads from earlier
-    blocks."""
-    spos = (lambda i: i) if forward else (lambda i: n - 1 - i)
+    blocks.  The blocks partition the rows in any order."""
     prow, pent, edep, ecoef = plan["prow"], plan["pent"], plan["edep"], plan["ecoef"]
     W = plan["ring"]
     assert sorted(prow.tolist()) == list(range(n))
     pos = np.empty(n, np.int64)
     pos[prow] = np.arange(n)
+    blk = np.empty(n, np.int64)
+    for b in range(plan["nblk"]):
+        blk[prow[plan["blk_pos"][b]:plan["blk_pos"][b + 1]]] = b
     if W:
         assert plan["nblk"] == 1 and W & (W - 1) == 0
     for b in range(plan["nblk"]):
         p0, p1 = plan["blk_pos"][b], plan["blk_pos"][b + 1]
         if W:
             assert sorted(prow[p0:p1].tolist()) == list(range(n))
-        else:
-            assert sorted(spos(i) for i in prow[p0:p1]) == list(range(p0, p1))
         starts = plan["lvl"][plan["blk_lev"][b]:plan["blk_lev"][b + 1]]
         assert starts[0] == p0 and starts[-1] == p1 and np.all(np.diff(starts) > 0)
         for li in range(len(starts) - 1):
@@ -219,7 +271,7 @@ def _check(plan, n, forward, off, col, coef):
                         if not (-d - 1 == k and pos[k] < starts[li]):
                             return False
                     elif d >= -n:
-                        if not (-d - 1 == k and spos(k) < p0):
+                        if not (-d - 1 == k and blk[k] < b):
                             return False
                     else:
                         assert -d - n - 1 == k
@@ -316,13 +368,60 @@ def test_narrow_grid_takes_a_ring(L, forward):
 @pytest.mark.parametrize("forward", [True, False])
 def test_grid_operator_blocks(L, monkeypatch, forward):
     """A 3-D operator in several blocks: 20^3 rows in 8 blocks of 1024 (the
-    last shorter), every read resolvable."""
+    last shorter) with the pencils off, every read resolvable."""
     monkeypatch.setenv("MPK_SWEEP_PLAN", "blocks")
+    monkeypatch.setenv("MPK_SWEEP_PENCILS", "0")
     off, col, coef = _lower(*W.diffusion_3d(20)[0], forward)
     n = len(off) - 1
     plan = _plan(L, n, forward, off, col, coef, 1024)
     assert plan["nblk"] == 8
     assert _check(plan, n, forward, off, col, coef)
+
+
+@pytest.mark.parametrize("name,grid", [("diff20", (20, 20, 20)), ("cd 12x9x10", (12, 9, 10))])
+@pytest.mark.parametrize("forward", [True, False])
+def test_grid_operator_pencils(L, monkeypatch, name, grid, forward):
+    """A 3-D grid operator's sweep in pencils (block plans forced: these
+    grids are shallow enough for a ring): blocks of t x t columns spanning
+    every z (t = 7 for 20^3 with 1024-row blocks: 9 blocks, the last row and
+    column of tiles narrower), matching the restatement, every read
+    resolvable, and each row's z chain inside its block."""
+    monkeypatch.setenv("MPK_SWEEP_PLAN", "blocks")
+    nx, ny, nz = grid
+    csr = W.diffusion_3d(20)[0] if name == "diff20" else _box(nx, ny, nz)
+    off, col, coef = _lower(*csr, forward)
+    n = len(off) - 1
+    plan = _plan(L, n, forward, off, col, coef, 1024)
+    want = _restated(n, forward, off, col, coef, 1024, "blocks")
+    for key in ("nblk", "blk_pos", "blk_lev", "lvl", "prow", "pent", "ecoef", "edep"):
+        assert np.array_equal(np.asarray(plan[key]), np.asarray(want[key])), key
+    assert _check(plan, n, forward, off, col, coef)
+    t = 1
+    while (t + 1) ** 2 * nz <= 1024:
+        t += 1
+    assert plan["nblk"] == -(-nx // min(t, nx)) * -(-ny // min(t, ny)) > 1
+    blk = np.empty(n, np.int64)
+    for b in range(plan["nblk"]):
+        blk[plan["prow"][plan["blk_pos"][b]:plan["blk_pos"][b + 1]]] = b
+    plane = nx * ny
+    assert np.all(blk[plane:] == blk[:-plane])  # z chains stay in their block
+
+
+def _box(nx, ny, nz):
+    """A 7-point operator on an nx x ny x nz box, x fastest, with unequal
+    couplings in each direction."""
+    n = nx * ny * nz
+    rows, cols, vals = [], [], []
+    for i in range(n):
+        z, r = divmod(i, nx * ny)
+        y, x = divmod(r, nx)
+        for dx, dy, dz, v in ((0, 0, -1, -0.3), (0, -1, 0, -0.7), (-1, 0, 0, -1.1), (0, 0, 0, 6.5), (1, 0, 0, -0.9),
+                              (0, 1, 0, -0.5), (0, 0, 1, -0.2)):
+            if 0 <= x + dx < nx and 0 <= y + dy < ny and 0 <= z + dz < nz:
+                rows.append(i)
+                cols.append(i + dx + dy * nx + dz * nx * ny)
+                vals.append(v)
+    return W._csr_from_coo(n, np.array(rows), np.array(cols), np.array(vals))
 
 
 def test_check_catches_a_bad_plan(L):
