@@ -379,10 +379,13 @@ struct LpSweep<double> {
 // divided by the row scale where there is one).
 template <class CT, class VT>
 struct LpCoef {
+  using Raw = CT;
   const CT *c;
   static constexpr int kBytes = sizeof(CT);
   __device__ static LpCoef make(const PlanView &, const void *lc) { return LpCoef{(const CT *)lc}; }
   __device__ VT operator()(i64 q) const { return lp_val<VT>(c[q]); }
+  __device__ Raw raw(i64 q) const { return c[q]; }
+  __device__ static VT conv(Raw r) { return lp_val<VT>(r); }
 };
 
 // The row programs of the inner solve's sweeps: acc = sum of coef * value in

@@ -276,11 +276,18 @@ __host__ __device__ inline PlanView plan_view(const char *base, i64 n) {
 
 // Coefficient of plan entry q: the plan's own FP64 values (other sources, such
 // as the mixed solver's low-precision copies, provide the same two members).
+// A coefficient source: raw(q) loads slot q as stored, conv(raw) gives the
+// value the sum takes.  Rows prefetched a level ahead hold the raw values,
+// converted only at their use, so that no conversion waits on a load still
+// in flight.
 struct PlanCoef {
+  using Raw = double;
   const double *c;
   static constexpr int kBytes = 8;
   __device__ static PlanCoef make(const PlanView &P, const void *) { return PlanCoef{P.ecoef}; }
   __device__ double operator()(i64 q) const { return c[q]; }
+  __device__ Raw raw(i64 q) const { return c[q]; }
+  __device__ static double conv(Raw r) { return r; }
 };
 
 // Writes each position's staged values (Prog::stage of its row) into
@@ -298,19 +305,19 @@ __global__ void k_sweep_stage(Prog prog, i64 n, const char *plan, int second, S 
 // A position's inputs, loaded one level ahead of their use: with fixed slots
 // every load is independent of the others (one round); in CSR form only the
 // entry offset is fetched ahead.
-template <int PF, class Tr, class S>
+template <int PF, class Tr, class S, class R>
 struct PlanRow {
   int p, cnt, base;
   i64 i;
   S s;
-  typename Tr::V c[PF];
+  R c[PF];  // raw coefficients (Coef::raw)
   int d[PF];
 };
 
 template <int PF, class Tr, class Coef, class S>
-__device__ __forceinline__ PlanRow<PF, Tr, S> plan_row(const PlanView &P, const Coef &coef, const S *st, int p,
-                                                       bool valid) {
-  PlanRow<PF, Tr, S> r;
+__device__ __forceinline__ PlanRow<PF, Tr, S, typename Coef::Raw> plan_row(const PlanView &P, const Coef &coef,
+                                                                           const S *st, int p, bool valid) {
+  PlanRow<PF, Tr, S, typename Coef::Raw> r;
   r.p = valid ? p : -1;
   if (!valid) return r;
   r.cnt = P.pcnt[p];
@@ -321,7 +328,7 @@ __device__ __forceinline__ PlanRow<PF, Tr, S> plan_row(const PlanView &P, const 
 #pragma unroll
     for (int e = 0; e < PF; e++)
       if (e < P.w) {
-        r.c[e] = coef((i64)e * P.n + p);
+        r.c[e] = coef.raw((i64)e * P.n + p);
         r.d[e] = P.edep[(i64)e * P.n + p];
       }
   } else {
@@ -387,7 +394,8 @@ __device__ __forceinline__ void gather_resolve(const Prog &prog, i64 n, const ty
 // in g); entries beyond the registers (or CSR form) are gathered here.
 template <int PF, class Tr, class Prog, class Coef, class S>
 __device__ __forceinline__ void plan_finish(const Prog &prog, const Coef &coef, i64 n, const PlanView &P,
-                                            const PlanRow<PF, Tr, S> &r, const Gather<PF, Tr> &g,
+                                            const PlanRow<PF, Tr, S, typename Coef::Raw> &r,
+                                            const Gather<PF, Tr> &g,
                                             typename Tr::V *val, int p0, typename Tr::V *pub, bool &gave) {
   using V = typename Tr::V;
   V acc = row_start(r.s);
@@ -397,7 +405,7 @@ __device__ __forceinline__ void plan_finish(const Prog &prog, const Coef &coef, 
     gather_resolve<PF, Tr>(prog, n, pub, r.d, r.cnt, g, v, gave);
 #pragma unroll
     for (int e = 0; e < PF; e++)
-      if (e < r.cnt) acc = Tr::madd(r.c[e], v[e], acc);
+      if (e < r.cnt) acc = Tr::madd(Coef::conv(r.c[e]), v[e], acc);
     j_rest = PF;
   }
   const i64 stride = P.w ? P.n : 1;  // entry j of the row at base + j stride
@@ -425,7 +433,8 @@ __device__ __forceinline__ void plan_finish(const Prog &prog, const Coef &coef, 
 
 template <int PF, class Tr, class Prog, class Coef, class S>
 __device__ __forceinline__ void plan_compute(const Prog &prog, const Coef &coef, i64 n, const PlanView &P,
-                                             const PlanRow<PF, Tr, S> &r, typename Tr::V *val, int p0,
+                                             const PlanRow<PF, Tr, S, typename Coef::Raw> &r, typename Tr::V *val,
+                                             int p0,
                                              typename Tr::V *pub, bool &gave) {
   Gather<PF, Tr> g;
   if (P.w) gather_issue<PF, Tr>(n, val, pub, r.d, r.cnt, g);
@@ -537,7 +546,7 @@ __device__ __forceinline__ void plan_levels(const Prog &prog, const Coef &coef, 
   // thread holds lvl[l], lvl[l+1], lvl[l+2] and fetches lvl[l+3]
   int pa = P.lvl[l0], pb = P.lvl[l0 + 1];
   int pc = l0 + 1 < l1 ? P.lvl[l0 + 2] : pb;
-  PlanRow<PF, Tr, S> cur = plan_row<PF, Tr>(P, coef, st, pa + t, pa + t < pb);
+  PlanRow<PF, Tr, S, typename Coef::Raw> cur = plan_row<PF, Tr>(P, coef, st, pa + t, pa + t < pb);
   for (int l = l0; l < l1; l++) {
     if (progress && t == 0 && ((l - l0) & (MPK_RING_STEP - 1)) == 0) ring_publish(progress, (unsigned)(l - l0));
     const bool more = l + 1 < l1;
@@ -545,7 +554,7 @@ __device__ __forceinline__ void plan_levels(const Prog &prog, const Coef &coef, 
     // this level's value loads, then the next level's prefetch, then the sums
     Gather<PF, Tr> g;
     if (cur.p >= 0 && P.w) gather_issue<PF, Tr>(n, val, pub, cur.d, cur.cnt, g);
-    const PlanRow<PF, Tr, S> nxt = plan_row<PF, Tr>(P, coef, st, pb + t, more && pb + t < pc);
+    const PlanRow<PF, Tr, S, typename Coef::Raw> nxt = plan_row<PF, Tr>(P, coef, st, pb + t, more && pb + t < pc);
     if (cur.p >= 0) plan_finish<PF, Tr>(prog, coef, n, P, cur, g, val, p0, pub, gave);
     for (int p = pa + t + 256; p < pb; p += 256)
       plan_compute<PF, Tr>(prog, coef, n, P, plan_row<PF, Tr>(P, coef, st, p, true), val, p0, pub, gave);
