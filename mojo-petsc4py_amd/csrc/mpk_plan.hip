@@ -464,7 +464,17 @@ __global__ void k_pen_span(i64 n, int forward, const i64 *off, const int *col, i
     lo = a < lo ? a : lo;
     hi = b > hi ? b : hi;
   }
+  __shared__ int sl[4], sh[4];
   if ((threadIdx.x & 63) == 0) {
+    sl[threadIdx.x >> 6] = lo;
+    sh[threadIdx.x >> 6] = hi;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {  // one pair of atomics per workgroup
+    for (int k = 1; k < 4; k++) {
+      lo = sl[k] < lo ? sl[k] : lo;
+      hi = sh[k] > hi ? sh[k] : hi;
+    }
     atomicMin(pm, lo);
     atomicMax(pm + 1, hi);
   }
@@ -483,7 +493,13 @@ __global__ void k_pen_mid(i64 n, int forward, const i64 *off, const int *col, in
     const int a = __shfl_down(lo, w, 64);
     lo = a < lo ? a : lo;
   }
-  if ((threadIdx.x & 63) == 0) atomicMin(pm + 2, lo);
+  __shared__ int sl[4];
+  if ((threadIdx.x & 63) == 0) sl[threadIdx.x >> 6] = lo;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int k = 1; k < 4; k++) lo = sl[k] < lo ? sl[k] : lo;
+    atomicMin(pm + 2, lo);
+  }
 }
 __global__ void k_pen_check(i64 n, int forward, const i64 *off, const int *col, const int *pm, unsigned *flags) {
   const i64 i = (i64)blockIdx.x * 256 + threadIdx.x;
