@@ -51,10 +51,17 @@ def _progress(msg: str) -> None:
 MEASURED_TRAFFIC = {("k_spmv_sell", 2097152, 14581760): 190142898, ("k_cg_wdot", 2097152, 14581760): 198584162}
 
 
+def algo_bytes(n: int, nnz: int) -> int:
+    """Algorithmic bytes of one FP64 product over int32 CSR: (8 + 4) nnz +
+    4 (n + 1) + 16 n (values and columns, row pointers, x read once, y
+    written once); 216,924,164 B for diffusion_3d 128^3."""
+    return 12 * nnz + 4 * (n + 1) + 16 * n
+
+
 def spmv_bytes(n: int, nnz: int, so=None) -> int:
-    """Algorithmic bytes of one sliced product: values, columns (2 bytes an
-    entry when the sliced copy `so` holds 16-bit offsets, marked by a negative
-    last slice offset; else 4), row pointers, x and y."""
+    """The bytes the sliced product actually moves: values, columns (2 bytes
+    an entry when the sliced copy `so` holds 16-bit offsets, marked by a
+    negative last slice offset; else 4), row pointers, x and y."""
     cb = 4
     if so is not None and int(so.to_numpy(np.int32, (n + 63) // 64 + 1)[-1]) < 0:
         cb = 2
@@ -196,16 +203,18 @@ def spmv_past_cache(lib, N: int = 256, reps: int = 20):
     ys = bv.duplicate()
     lib.mpk_time_spmv(xs._d(), ys._dw(), d.rp.ptr, d.ci.ptr, d.v.ptr, n, so.ptr, sv.ptr, sc.ptr, 3)
     ms = lib.mpk_time_spmv(xs._d(), ys._dw(), d.rp.ptr, d.ci.ptr, d.v.ptr, n, so.ptr, sv.ptr, sc.ptr, reps)
-    b = spmv_bytes(n, nnz, so)
+    b, moved = algo_bytes(n, nnz), spmv_bytes(n, nnz, so)
     gbs = b / (ms * 1e-3) / 1e9
     # the CG iteration's dominant kernel: the same product with its p.w
     # partials (the same algorithmic bytes: the partials are 8 bytes a tile)
     lib.mpk_time_cg_wdot(xs._d(), ys._dw(), d.rp.ptr, n, so.ptr, sv.ptr, sc.ptr, 3)
     ms_w = lib.mpk_time_cg_wdot(xs._d(), ys._dw(), d.rp.ptr, n, so.ptr, sv.ptr, sc.ptr, reps)
     gbs_w = b / (ms_w * 1e-3) / 1e9
-    return {"workload": f"diffusion_3d {N}^3", "n": n, "nnz": nnz, "bytes_per_launch": b, "avg_launch_ms": ms,
-            "achieved": gbs, "frac": gbs / HBM_PEAK_GBS, "unit": "GB/s",
-            "k_cg_wdot": {"avg_launch_ms": ms_w, "achieved": gbs_w, "frac": gbs_w / HBM_PEAK_GBS}}
+    return {"workload": f"diffusion_3d {N}^3", "n": n, "nnz": nnz, "bytes_per_launch": b, "bytes_moved": moved,
+            "avg_launch_ms": ms, "achieved": gbs, "frac": gbs / HBM_PEAK_GBS, "unit": "GB/s",
+            "moved_gbs": moved / (ms * 1e-3) / 1e9,
+            "k_cg_wdot": {"avg_launch_ms": ms_w, "achieved": gbs_w, "frac": gbs_w / HBM_PEAK_GBS,
+                          "moved_gbs": moved / (ms_w * 1e-3) / 1e9}}
 
 
 def _mat(csr):
@@ -606,7 +615,7 @@ def main() -> int:
     lib.mpk_time_cg_wdot(xa, ya, d.rp.ptr, n, so.ptr, sv.ptr, sc.ptr, 5)
     wdot_ms = lib.mpk_time_cg_wdot(xa, ya, d.rp.ptr, n, so.ptr, sv.ptr, sc.ptr, args.spmv_reps)
     _native.check()
-    sbytes = spmv_bytes(n, nnz, so)
+    sbytes, moved = algo_bytes(n, nnz), spmv_bytes(n, nnz, so)
     spmv_gbs = sbytes / (spmv_ms * 1e-3) / 1e9
     wdot_gbs = sbytes / (wdot_ms * 1e-3) / 1e9
     cols32 = spmv_cols32(lib, A, bv, n, nnz, args.spmv_reps)
@@ -667,12 +676,14 @@ def main() -> int:
         "roofline": {"bound": "hbm", "kernel": "k_cg_wdot", "achieved": wdot_gbs, "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": wdot_gbs / HBM_PEAK_GBS,
                      "traffic": MEASURED_TRAFFIC.get(("k_cg_wdot", n, nnz)),
-                     "bytes_per_launch": sbytes, "avg_launch_ms": wdot_ms,
+                     "bytes_per_launch": sbytes, "bytes_moved": moved, "avg_launch_ms": wdot_ms,
+                     "moved_gbs": moved / (wdot_ms * 1e-3) / 1e9,
                      "note": f"the CG product (w = A p with the tile partials of p.w), the dominant kernel of the"
-                             f" solve, timed over back-to-back launches; its {sbytes / 1e6:.0f} MB working set stays"
-                             " in the 256 MB last-level cache across them (past_cache: the same kernels on a"
-                             " 1.74 GB matrix); the entries are 10 bytes when the sliced copy carries 16-bit column"
-                             " offsets (cols32: the product over int32 columns)",
+                             f" solve, timed over back-to-back launches; achieved = the FP64 int32-CSR product's"
+                             f" algorithmic {sbytes / 1e6:.1f} MB per launch over its time; the sliced copy's 16-bit"
+                             f" column offsets move {moved / 1e6:.1f} MB (moved_gbs); the working set stays in the"
+                             " 256 MB last-level cache across launches (past_cache: the same kernels on a 1.74 GB"
+                             " matrix; cols32: the product over int32 columns)",
                      "spmv": {"kernel": "k_spmv_sell", "avg_launch_ms": spmv_ms, "achieved": spmv_gbs,
                               "frac": spmv_gbs / HBM_PEAK_GBS,
                               "traffic": MEASURED_TRAFFIC.get(("k_spmv_sell", n, nnz)),

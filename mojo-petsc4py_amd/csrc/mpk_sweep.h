@@ -891,7 +891,10 @@ __device__ __forceinline__ void wave_sweep(const Prog &prog, i64 n, const char *
                                            double *pub, double *val, int *lv, int nw, bool &gave) {
   // loads and stores in flight within the 63 a wave can count
   constexpr int kWaveAhead = Pub ? (SL == 4 ? 6 : 4) : (SL == 4 ? 8 : 5);
-  constexpr int kPubAhead = Pub ? 2 : 0;
+  // published words kPubAhead levels ahead: their slots come from a set
+  // refetched kWaveAhead - 1 - kPubAhead steps earlier, so a deeper lead
+  // shortens the records' own (6 sets ahead, 4 slots: 3 measured best)
+  constexpr int kPubAhead = Pub ? (SL == 4 ? 3 : 2) : 0;
   constexpr int kStageUnroll = 8;
   if (second) plan += reinterpret_cast<const i64 *>(plan)[4];
   const PlanView P = plan_view(plan, n);
