@@ -105,6 +105,23 @@ int64_t mpk_csr_transpose(int64_t irow, int64_t icol, int64_t nrow, int64_t ncol
                           int64_t tpos);
 void mpk_mat_mult_transpose_t(int64_t x, int64_t y, int64_t data, int64_t ncols, int64_t trp, int64_t trow,
                               int64_t tpos);
+/* No counterpart in the reference: Mat assembly on the device.
+ * mpk_mat_fold folds a log of nw entry writes -- rows, cols [H] int64, vals
+ * [H] float64, adds [H] uint8 (1: add, 0: insert), in call order -- into CSR
+ * arrays with the per-entry semantics of the reference's Mat
+ * (python/mojopetsc4py/PETSc/Mat.py:383-439): an insert replaces, an add
+ * accumulates onto the value so far (0.0 if none), every entry ever written
+ * stays, columns ascending.  Out [D]: rp int32[nrows + 1], ci int32[nw],
+ * v float64[nw] (room for nw entries).  Returns the entry count, or -1
+ * (status 4: a write outside the matrix, or nw / nrows / ncols past int32).
+ * mpk_dense_to_csr: the nonzeros (!= 0.0, NaN included) of dense [D] float64
+ * nrows x ncols row-major (fewer than 2^31 elements) as rp [D] int32
+ * [nrows + 1], ci [D] int32, v [D] float64; returns the count, or minus it
+ * (rp written) when ci or v is 0 or cap is smaller, or -1 on error. */
+int64_t mpk_mat_fold(int64_t rows, int64_t cols, int64_t vals, int64_t adds, int64_t nw, int64_t nrows,
+                     int64_t ncols, int64_t rp, int64_t ci, int64_t v);
+int64_t mpk_dense_to_csr(int64_t dense, int64_t nrows, int64_t ncols, int64_t rp, int64_t ci, int64_t v,
+                         int64_t cap);
 /* No counterpart in the reference: the sliced (64-row, column-major) copy of a
  * CSR matrix that the products read with coalesced loads; same row sums bit
  * for bit.  [D] soff[ceil(nrow/64) + 1] int32, sv[padded] float64, sc of

@@ -28,6 +28,7 @@ SOURCES = [
     "mpk_mat.hip",
     "mpk_pc.hip",
     "mpk_plan.hip",
+    "mpk_assemble.hip",
     "mpk_ksp.hip",
     "mpk_mixed.hip",
     "mpk_batch.hip",

@@ -14,8 +14,11 @@ again after any change.
 
 from __future__ import annotations
 
+import os
+
 import numpy as np
 
+from mpk import _native
 from mpk._native import check, lib
 from mpk.device import DeviceBuffer
 
@@ -103,6 +106,17 @@ class _DevCSR:
         self.v = DeviceBuffer.from_array(np.asarray(data[: max(nnz, 1)], dtype=np.float64))
         self.sellc = None
         self.tsellc = None
+
+    @classmethod
+    def adopt(cls, rp, ci, v, nnz, nrows, ncols):
+        """A device copy over buffers already on the device (the device fold
+        and dense compaction write them)."""
+        self = cls.__new__(cls)
+        self.rp, self.ci, self.v = rp, ci, v
+        self.nnz, self.nrows, self.ncols = int(nnz), nrows, ncols
+        self.sellc = None
+        self.tsellc = None
+        return self
 
     def sell(self):
         """(soff, sv, sc): rows in 64-row slices, column-major inside a slice,
@@ -347,12 +361,38 @@ class Mat:
             self._csr.changed()
 
     def _materialize(self):
-        """Fold the pending writes into the CSR arrays."""
+        """Fold the pending writes into the CSR arrays: on the device
+        (mpk_mat_fold) for logs of at least MPK_FOLD_DEVICE_MIN writes (65536
+        by default) when a device is open, else on the host; the same arrays
+        bit for bit either way."""
         if self._log is None:
             return
-        self._csr = self._log.fold(self._m, self._n)
+        folded = self._fold_on_device()
+        if folded is None:
+            self._csr = self._log.fold(self._m, self._n)
+            self._log = None
+            self._touch()
+            return
+        self._csr, devc = folded
         self._log = None
         self._touch()
+        self._devc = devc
+
+    def _fold_on_device(self):
+        log = self._log
+        nw = sum(len(r) for r in log.rows) + (log.base.nnz if log.base is not None else 0)
+        if nw < _env_int("MPK_FOLD_DEVICE_MIN", 65536) or nw >= 2 ** 31 or not _native.device_ready():
+            return None
+        r, c, v, a = log._stream()
+        r = np.ascontiguousarray(r, np.int64)
+        c = np.ascontiguousarray(c, np.int64)
+        v = np.ascontiguousarray(v, np.float64)
+        a = np.ascontiguousarray(a, np.uint8)
+        rp, ci, vd = DeviceBuffer(4 * (self._m + 1)), DeviceBuffer(4 * max(nw, 1)), DeviceBuffer(8 * max(nw, 1))
+        nnz = lib().mpk_mat_fold(r.ctypes.data, c.ctypes.data, v.ctypes.data, a.ctypes.data, nw, self._m, self._n,
+                                 rp.ptr, ci.ptr, vd.ptr)
+        check()
+        return _downloaded(rp, ci, vd, nnz, self._m, self._n)
 
     def _writes(self) -> _WriteLog:
         if self._log is None:
@@ -785,14 +825,45 @@ def _matmul(a: Mat, b: Mat) -> Mat:
     return from_dense(a._dense() @ b._dense())
 
 
+def _env_int(name: str, default: int) -> int:
+    try:
+        return int(os.environ.get(name, default))
+    except ValueError:
+        return default
+
+
+def _downloaded(rp, ci, v, nnz, nrows, ncols):
+    """(host arrays, device copy) of CSR arrays the device wrote."""
+    nnz = int(nnz)
+    host = _HostCSR(rp.to_numpy(np.int32, nrows + 1), ci.to_numpy(np.int32, max(nnz, 1)),
+                    v.to_numpy(np.float64, max(nnz, 1)), nnz)
+    return host, _DevCSR.adopt(rp, ci, v, nnz, nrows, ncols)
+
+
 def from_dense(array, mat=None) -> Mat:
-    """A CSR matrix of the nonzeros of a dense 2-D array."""
+    """A CSR matrix of the nonzeros of a dense 2-D array: compacted on the
+    device (mpk_dense_to_csr) for arrays of at least MPK_DENSE_DEVICE_MIN
+    elements (2^20 by default) when a device is open, else on the host."""
     arr = _f64(array)
     if arr.ndim != 2:
         raise ValueError("expected a 2-D array")
     n, m = arr.shape
     out = mat if mat is not None else Mat()
     out.createSeqAIJ((n, m), nnz=1)
+    if n * m >= max(_env_int("MPK_DENSE_DEVICE_MIN", 1 << 20), 1) and n * m < 2 ** 31 and _native.device_ready():
+        L = lib()
+        dense = DeviceBuffer.from_array(np.ascontiguousarray(arr))
+        rp = DeviceBuffer(4 * (n + 1))
+        nnz = -L.mpk_dense_to_csr(dense.ptr, n, m, rp.ptr, 0, 0, 0)
+        check()
+        ci, vd = DeviceBuffer(4 * max(nnz, 1)), DeviceBuffer(8 * max(nnz, 1))
+        got = L.mpk_dense_to_csr(dense.ptr, n, m, rp.ptr, ci.ptr, vd.ptr, nnz)
+        check()
+        out._csr, devc = _downloaded(rp, ci, vd, got, n, m)
+        out._log = None
+        out._touch()
+        out._devc = devc
+        return out
     rr, cc = np.nonzero(arr)
     indptr = np.zeros(n + 1, np.int64)
     np.cumsum(np.bincount(rr, minlength=n), out=indptr[1:])

@@ -132,6 +132,8 @@ SIGNATURES = {
     "mpk_ic0_analyse": ([I] * 14, I),
     "mpk_ilu0_size": ([I] * 3, I),
     "mpk_ilu0_analyse": ([I] * 8, I),
+    "mpk_mat_fold": ([I] * 10, I),
+    "mpk_dense_to_csr": ([I] * 7, I),
 }
 
 # exported names that only a device call can exercise
@@ -168,6 +170,23 @@ def load(init: bool = True) -> ctypes.CDLL:
 
 def lib() -> ctypes.CDLL:
     return load(True)
+
+
+_ready = None
+
+
+def device_ready() -> bool:
+    """True when the library is built and opens a device (checked once).  For
+    the optional device paths of host-side work (Mat assembly); the solver
+    paths call lib(), which raises instead."""
+    global _ready
+    if _ready is None:
+        try:
+            load(True)
+            _ready = True
+        except (NativeError, OSError):
+            _ready = False
+    return _ready
 
 
 def last_error() -> str:
