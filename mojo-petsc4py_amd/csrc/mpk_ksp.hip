@@ -1292,12 +1292,20 @@ __device__ __forceinline__ bool small_stopped(const KspState *s) {
 
 // The iterations of KSPSolve_CG (ksp.mojo:337-389) after the initial
 // residual, its PC apply and EPI_CG_INIT, as the general path queues them.
+// The solve's state lives in LDS for the kernel's length (thread 0's
+// epilogues are serial chains of its fields), copied back at the end.
+struct SmallCgLds : SmallLds {
+  KspState ks;
+};
+
 template <int PCT>
 __global__ void __launch_bounds__(MPK_WAVE_THREADS) k_cg_small(SmallCG a) {
-  __shared__ SmallLds L;
-  KspState *s = a.s;
+  __shared__ SmallCgLds L;
   const i64 n = a.n;
   const int t = (int)threadIdx.x;
+  if (t == 0) L.ks = *a.s;
+  __syncthreads();
+  KspState *s = &L.ks;
   const bool al_p = aligned16(a.P) && aligned16(a.Z);
   const bool al_x = aligned16(a.X) && aligned16(a.R) && aligned16(a.P) && aligned16(a.W);
   for (i64 it = 0; it <= a.max_it + 1; it++) {
@@ -1317,6 +1325,8 @@ __global__ void __launch_bounds__(MPK_WAVE_THREADS) k_cg_small(SmallCG a) {
     if (t == 0) ksp_epilogue(s, EPI_CG_NORM, 0, L.res);
     __syncthreads();
   }
+  __syncthreads();
+  if (t == 0) *a.s = L.ks;
 }
 
 // ------------------------------------------------------------ small GMRES / FGMRES / BiCGStab
