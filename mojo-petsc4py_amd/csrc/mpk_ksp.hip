@@ -1044,10 +1044,24 @@ struct Solve {
     return !failed();
   }
 
-  // Queue a flag read-back behind the work queued so far.
-  void mark(int slot) {
-    HIPCHK(hipMemcpyAsync(hflag + slot, stop, sizeof(int64_t), hipMemcpyDeviceToHost, stream()));
+  // Queue a flag read-back behind the work queued so far (`stop`, or
+  // another flag of the state).
+  void mark(int slot, const int64_t *flag = nullptr) {
+    HIPCHK(hipMemcpyAsync(hflag + slot, flag ? flag : stop, sizeof(int64_t), hipMemcpyDeviceToHost, stream()));
     HIPCHK(hipEventRecord(ev[slot], stream()));
+  }
+  // Within a GMRES / FGMRES cycle: after every chunk of Arnoldi steps queued
+  // (step s done), read inner_stop back a chunk behind; true when the loop is
+  // over, so the steps left would only return on the flag and need not be
+  // queued.  The first call of a cycle has s = 0.
+  bool inner_over(i64 s, i64 m, int chunk, int &slot, bool &pending) {
+    if (s == 0) pending = false;
+    if ((s + 1) % chunk != 0 || s + 1 >= m) return false;
+    mark(slot, inner);
+    const bool over = pending && done(slot ^ 1);
+    pending = true;
+    slot ^= 1;
+    return over;
   }
   bool done(int slot) {
     HIPCHK(hipEventSynchronize(ev[slot]));
@@ -2163,6 +2177,9 @@ void mp_ksp_solve_gmres(int64_t n, int64_t irow, int64_t icol, int64_t data, int
     S.finish_out((double *)cfg, (int64_t *)cfgi);
     return;
   }
+  const int chunk = chunk_for(n);
+  int islot = 0;
+  bool ipending = false;
   for (i64 cycle = 0;; cycle++) {
     S.initial_residual(guess_zero, X, TEMP, TEMP_MATOP, vv, B, st);
     S.reduce<1>(RedSet<1>{{{vv, vv, RK_SUMSQ}}}, EPI_GM_START, 0, st);
@@ -2173,6 +2190,7 @@ void mp_ksp_solve_gmres(int64_t n, int64_t irow, int64_t icol, int64_t data, int
       orthogonalize(S, vv, s, vnew);
       S.reduce<1>(RedSet<1>{{{vnew, vnew, RK_SUMSQ}}}, EPI_GM_STEP, 0, inner);
       ew2(n, OpDivDev{vnew, &S.d->tt, aligned16(vnew)}, inner);
+      if (S.inner_over(s, m, chunk, islot, ipending)) break;
     }
     // KSPGMRESBuildSoln(RS, HH, vv, TEMP, TEMP_MATOP, x, x, m, it - 1)
     MPK_LAUNCH(k_gmres_backsub, 1, 1, S.d, 0, st);
@@ -2221,6 +2239,9 @@ void mp_ksp_solve_fgmres(int64_t n, int64_t irow, int64_t icol, int64_t data, in
     S.finish_out((double *)cfg, (int64_t *)cfgi);
     return;
   }
+  const int chunk = chunk_for(n);
+  int islot = 0;
+  bool ipending = false;
   for (i64 cycle = 0;; cycle++) {
     S.reduce<1>(RedSet<1>{{{vv, vv, RK_SUMSQ}}}, EPI_FGM_START, 0, st);
     ew2(n, OpScaleRecipDev{vv, &S.d->res, nullptr, aligned16(vv)}, inner);
@@ -2231,6 +2252,7 @@ void mp_ksp_solve_fgmres(int64_t n, int64_t irow, int64_t icol, int64_t data, in
       orthogonalize(S, vv, s, vnew);
       S.reduce<1>(RedSet<1>{{{vnew, vnew, RK_SUMSQ}}}, EPI_FGM_STEP, 0, inner);
       ew2(n, OpScaleRecipDev{vnew, &S.d->tt, &S.d->hapend, aligned16(vnew)}, inner);
+      if (S.inner_over(s, m, chunk, islot, ipending)) break;
     }
     MPK_LAUNCH(k_gmres_backsub, 1, 1, S.d, 1, st);
     ew2(n, OpBuild{TEMP, prevec, n, S.d, aligned16(TEMP) && aligned16(prevec) && n % 2 == 0}, S.skipbuild);

@@ -38,24 +38,34 @@ __global__ void k_sweep_prepare(i64 n, double *p, unsigned *ticket, const int64_
 // and how (hdr[7] and hdr[5] of each: bits 0-15 the first sweep, 16-31 the
 // second; a sweep's word is its slot count, plus for a wave or wave-block
 // plan 16, 32 when none of its rows has over 2 entries, 64 times log2 of its
-// waves, and 256 for wave blocks; 0 for other plans), recorded on the host
-// by PCView::schedule when it writes them, so an apply picks the kernel
-// without reading the device.  A plan this process did not write counts as
-// neither.
+// waves, and 256 for wave blocks; 0 for other plans), with each sweep's
+// block count, recorded on the host by PCView::schedule when it writes them,
+// so an apply picks the kernel and its grid without reading the device.  A
+// plan this process did not write counts as neither (block count 0).
+struct PlanNote {
+  int bits = 0;
+  int nblk[2] = {0, 0};
+};
 static std::mutex g_narrow_mu;
-static std::map<const char *, int> &narrow_plans() {
-  static std::map<const char *, int> m;
+static std::map<const char *, PlanNote> &narrow_plans() {
+  static std::map<const char *, PlanNote> m;
   return m;
 }
-static void note_narrow(const char *plan, int bits) {
+static void note_narrow(const char *plan, const PlanNote &note) {
   std::lock_guard<std::mutex> lk(g_narrow_mu);
-  narrow_plans()[plan] = bits;
+  narrow_plans()[plan] = note;
 }
 // A sweep's word of the record above.
 static int plan_narrow(const char *plan, int second) {
   std::lock_guard<std::mutex> lk(g_narrow_mu);
   auto it = narrow_plans().find(plan);
-  return it == narrow_plans().end() ? 0 : (it->second >> (second ? 16 : 0)) & 0xffff;
+  return it == narrow_plans().end() ? 0 : (it->second.bits >> (second ? 16 : 0)) & 0xffff;
+}
+// A sweep's block count, or 0 when unknown.
+static int plan_blocks(const char *plan, int second) {
+  std::lock_guard<std::mutex> lk(g_narrow_mu);
+  auto it = narrow_plans().find(plan);
+  return it == narrow_plans().end() ? 0 : it->second.nblk[second ? 1 : 0];
 }
 
 // One sweep: over its block plan when the PC has one (`plan`, and `second`
@@ -72,9 +82,11 @@ static void run_sweep(const Prog &prog, i64 n, bool forward, double *pub, const 
   const int narrow = form & 15;  // the plan's slot count, or 0
   if (form & 256) {  // wave blocks: published values, so the sentinel fill
     const int nw = 1 << ((form >> 6) & 3);
-    // blocks hold at least min(n, 1024) rows (mpk_sweep_plan), so this grid
-    // covers every block; the spare workgroups find no block and return
-    const unsigned nblk = nblocks(n, 1024);
+    // one workgroup per block (a launch past convergence then costs little);
+    // without the count, blocks hold at least min(n, 1024) rows, so this grid
+    // covers every block and the spare workgroups find none and return
+    const int known = plan_blocks(plan, second);
+    const unsigned nblk = known > 0 ? (unsigned)known : nblocks(n, 1024);
     MPK_LAUNCH(k_sweep_prepare, nblocks(n, 256), 256, n, pub, c->tickets, stop);
     if (narrow == 4 && (form & 32))
       MPK_LAUNCH((k_sweep_wblk<Prog, 4, 2>), nblk, MPK_WAVE_THREADS, prog, n, plan, second, st, pub, c->tickets,
@@ -420,7 +432,8 @@ i64 PCView::schedule_size() const {
 i64 PCView::schedule(char *plan, i64 cap) const {
   i64 used = 0, depth = 0;
   bool bad = false;
-  int bits = 0, which = 0;
+  int which = 0;
+  PlanNote note;
   const bool any = with_sweeps(*this, [&](auto prog, bool forward) {
     if (bad) return;
     i64 d = 0;
@@ -430,12 +443,16 @@ i64 PCView::schedule(char *plan, i64 cap) const {
       bad = true;
       return;
     }
-    bits |= narrow << (16 * which++);
+    int64_t nblk = 0;  // the plan header's block count
+    copy_dtoh(&nblk, plan + used, sizeof nblk);
+    if (which < 2) note.nblk[which] = nblk > 0 && nblk < (1 << 30) ? (int)nblk : 0;
+    note.bits |= narrow << (16 * which++);
     used += bytes;
     depth = d > depth ? d : depth;
   });
   if (!any) return 0;
-  note_narrow(plan, bad ? 0 : bits);
+  if (bad) note = PlanNote{};
+  note_narrow(plan, note);
   return bad ? -1 : depth;
 }
 
