@@ -1343,6 +1343,119 @@ __global__ void __launch_bounds__(MPK_WAVE_THREADS) k_cg_small(SmallCG a) {
   if (t == 0) *a.s = L.ks;
 }
 
+// Systems of at most MPK_SMALL_LDS_ROWS rows without sweeps (no PC, or
+// Jacobi) keep their vectors in LDS as well: k_cg_small's steps, on LDS
+// copies of X, R, Z (= W) and P, every step's loads and stores then local.
+// The same operations on the same values, so the same bits; the vectors
+// and the state go back to global memory at the end.
+#define MPK_SMALL_LDS_ROWS 2048
+
+struct SmallCgVecLds {
+  double vec[4][MPK_SMALL_LDS_ROWS] __attribute__((aligned(16)));  // X, R, Z (= W), P
+  double wsum[8][MPK_SMALL_TILES][4];
+  double fsum[8][4];
+  double res[8];
+  KspState ks;
+};
+
+// This thread's rows (t, t + 512, ..) of the matrix held in registers for
+// the solve: at most 8 entries a row (else `fits` is false and the product
+// reads the matrix as small_spmv does).
+constexpr int kSmallRowsPerThread = (MPK_SMALL_LDS_ROWS + MPK_WAVE_THREADS - 1) / MPK_WAVE_THREADS;
+struct SmallRows {
+  int cnt[kSmallRowsPerThread];
+  int c[kSmallRowsPerThread][8];
+  double v[kSmallRowsPerThread][8];
+};
+__device__ __forceinline__ bool small_rows_load(const SmallCG &a, SmallRows &m) {
+  const int t = (int)threadIdx.x;
+  bool fits = true;
+#pragma unroll
+  for (int r = 0; r < kSmallRowsPerThread; r++) {
+    const i64 i = t + (i64)r * MPK_WAVE_THREADS;
+    const int j0 = i < a.n ? a.rp[i] : 0, j1 = i < a.n ? a.rp[i + 1] : 0;
+    m.cnt[r] = j1 - j0;
+    fits = fits && j1 - j0 <= 8;
+#pragma unroll
+    for (int e = 0; e < 8; e++) {
+      const bool in = j0 + e < j1;
+      m.c[r][e] = in ? a.ci[j0 + e] : 0;
+      m.v[r][e] = in ? a.v[j0 + e] : 0.0;
+    }
+  }
+  return __syncthreads_and(fits) != 0;
+}
+// y = A x from the rows in registers: each row's serial sum from 0.0 in CSR
+// order, as small_spmv.
+__device__ __forceinline__ void small_spmv_rows(const SmallRows &m, i64 n, const double *x, double *y) {
+  const int t = (int)threadIdx.x;
+#pragma unroll
+  for (int r = 0; r < kSmallRowsPerThread; r++) {
+    const i64 i = t + (i64)r * MPK_WAVE_THREADS;
+    double xv[8];
+#pragma unroll
+    for (int e = 0; e < 8; e++) xv[e] = x[m.c[r][e]];
+    double acc = 0.0;
+#pragma unroll
+    for (int e = 0; e < 8; e++)
+      if (e < m.cnt[r]) acc = fma(m.v[r][e], xv[e], acc);
+    if (i < n) y[i] = acc;
+  }
+}
+
+template <int PCT>
+__global__ void __launch_bounds__(MPK_WAVE_THREADS) k_cg_small_lds(SmallCG g) {
+  __shared__ SmallCgVecLds L;
+  const i64 n = g.n;
+  const int t = (int)threadIdx.x;
+  SmallRows rows;
+  const bool fits = small_rows_load(g, rows);
+  for (i64 i = t; i < n; i += MPK_WAVE_THREADS) {
+    L.vec[0][i] = g.X[i];
+    L.vec[1][i] = g.R[i];
+    L.vec[2][i] = g.Z[i];
+    L.vec[3][i] = g.P[i];
+  }
+  if (t == 0) L.ks = *g.s;
+  __syncthreads();
+  SmallCG a = g;
+  a.X = L.vec[0], a.R = L.vec[1], a.Z = L.vec[2], a.W = L.vec[2], a.P = L.vec[3];
+  KspState *s = &L.ks;
+  const bool al_p = true, al_x = true;  // LDS rows are 16-byte aligned
+  for (i64 it = 0; it <= a.max_it + 1; it++) {
+    if (small_stopped(s)) break;  // the flag is read after a barrier: uniform
+    for (i64 e = 2 * (i64)t; e < n; e += 2 * MPK_WAVE_THREADS) OpCgP{a.P, a.Z, s, al_p}(e, n);
+    __syncthreads();
+    if (fits)
+      small_spmv_rows(rows, n, a.P, a.W);
+    else
+      small_spmv(a, a.P, a.W);
+    __syncthreads();
+    small_reduce<1>(RedSet<1>{{{a.P, a.W, RK_DOT}}}, n, L);
+    if (t == 0) ksp_epilogue(s, EPI_CG_DPI, 0, L.res);
+    __syncthreads();
+    if (small_stopped(s)) break;
+    for (i64 e = 2 * (i64)t; e < n; e += 2 * MPK_WAVE_THREADS) OpCgXR{a.X, a.R, a.P, a.W, s, al_x}(e, n);
+    __syncthreads();
+    if (PCT == PCT_JACOBI)
+      for (i64 i = t; i < n; i += MPK_WAVE_THREADS) a.Z[i] = a.R[i] * a.jac[i];
+    else
+      for (i64 i = t; i < n; i += MPK_WAVE_THREADS) a.Z[i] = a.R[i];
+    __syncthreads();
+    small_reduce<3>(RedSet<3>{{{a.Z, a.Z, RK_SUMSQ}, {a.R, a.R, RK_SUMSQ}, {a.Z, a.R, RK_DOT}}}, n, L);
+    if (t == 0) ksp_epilogue(s, EPI_CG_NORM, 0, L.res);
+    __syncthreads();
+  }
+  __syncthreads();
+  for (i64 i = t; i < n; i += MPK_WAVE_THREADS) {
+    g.X[i] = L.vec[0][i];
+    g.R[i] = L.vec[1][i];
+    g.Z[i] = L.vec[2][i];
+    g.P[i] = L.vec[3][i];
+  }
+  if (t == 0) *g.s = L.ks;
+}
+
 // ------------------------------------------------------------ small GMRES / FGMRES / BiCGStab
 //
 // The same one-workgroup form for KSPSolve_GMRES (ksp.mojo:667-769),
@@ -1920,9 +2033,13 @@ void mp_ksp_solve_cg(int64_t n, int64_t irow, int64_t icol, int64_t data, int64_
       if (!a.S) return;
       a.st = static_cast<Staged *>(pc_staged_rows(a.S, n));
     }
+    const bool lds = W == Z && n <= MPK_SMALL_LDS_ROWS;
     switch (S.pc.type) {
       case PCT_JACOBI:
-        MPK_LAUNCH(k_cg_small<PCT_JACOBI>, 1, MPK_WAVE_THREADS, a);
+        if (lds)
+          MPK_LAUNCH(k_cg_small_lds<PCT_JACOBI>, 1, MPK_WAVE_THREADS, a);
+        else
+          MPK_LAUNCH(k_cg_small<PCT_JACOBI>, 1, MPK_WAVE_THREADS, a);
         break;
       case PCT_ILU:
         MPK_LAUNCH(k_cg_small<PCT_ILU>, 1, MPK_WAVE_THREADS, a);
@@ -1931,7 +2048,10 @@ void mp_ksp_solve_cg(int64_t n, int64_t irow, int64_t icol, int64_t data, int64_
         MPK_LAUNCH(k_cg_small<PCT_ICC>, 1, MPK_WAVE_THREADS, a);
         break;
       default:
-        MPK_LAUNCH(k_cg_small<PCT_NONE>, 1, MPK_WAVE_THREADS, a);
+        if (lds)
+          MPK_LAUNCH(k_cg_small_lds<PCT_NONE>, 1, MPK_WAVE_THREADS, a);
+        else
+          MPK_LAUNCH(k_cg_small<PCT_NONE>, 1, MPK_WAVE_THREADS, a);
     }
     g_small_solves++;
     S.finish_out((double *)cfg, (int64_t *)cfgi);
