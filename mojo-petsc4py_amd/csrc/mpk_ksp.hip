@@ -1875,6 +1875,85 @@ __global__ void __launch_bounds__(MPK_WAVE_THREADS) k_bcgs_small(SmallKsp a) {
   small_state_out(a.c.s, L, false);
 }
 
+// k_bcgs_small with its vectors in LDS (systems of at most
+// MPK_SMALL_LDS_ROWS rows without sweeps) and the matrix rows in registers,
+// as k_cg_small_lds: the same steps on LDS copies of X, R, RP, V, T, S and P.
+struct SmallBcgsLds {
+  double vec[7][MPK_SMALL_LDS_ROWS] __attribute__((aligned(16)));
+  double wsum[8][MPK_SMALL_TILES][4];
+  double fsum[8][4];
+  double res[8];
+  KspState ks;
+};
+
+template <int PCT>
+__global__ void __launch_bounds__(MPK_WAVE_THREADS) k_bcgs_small_lds(SmallKsp g) {
+  __shared__ SmallBcgsLds L;
+  const i64 n = g.c.n;
+  const int t = (int)threadIdx.x;
+  SmallRows rows;
+  const bool fits = small_rows_load(g.c, rows);
+  double *gv[7] = {g.c.X, g.R, g.RP, g.V, g.T, g.Sv, g.P};
+  for (int k = 0; k < 7; k++)
+    for (i64 i = t; i < n; i += MPK_WAVE_THREADS) L.vec[k][i] = gv[k][i];
+  if (t == 0) L.ks = *g.c.s;
+  __syncthreads();
+  KspState *s = &L.ks;
+  double *X = L.vec[0], *R = L.vec[1], *RP = L.vec[2], *V = L.vec[3], *T = L.vec[4], *Sv = L.vec[5], *P = L.vec[6];
+  const double *jac = g.c.jac;
+  auto spmv = [&](const double *x, double *y) {
+    if (fits)
+      small_spmv_rows(rows, n, x, y);
+    else
+      small_spmv(g.c, x, y);
+  };
+  auto pc = [&](const double *x, double *y) {  // small_pc's arithmetic, then its barrier
+    for (i64 i = t; i < n; i += MPK_WAVE_THREADS) y[i] = PCT == PCT_JACOBI ? x[i] * jac[i] : x[i];
+    __syncthreads();
+  };
+  auto ba = [&](const double *x, double *y, double *w) {  // small_ba
+    if (g.side == PC_RIGHT) {
+      pc(x, w);
+      spmv(w, y);
+    } else {
+      spmv(x, w);
+      __syncthreads();
+      pc(w, y);
+    }
+    __syncthreads();
+  };
+  for (i64 it = 0; it <= g.c.max_it + 1; it++) {
+    if (small_flag(&s->stop)) break;
+    small_reduce<1>(RedSet<1>{{{R, RP, RK_DOT}}}, n, L);
+    if (t == 0) ksp_epilogue(s, EPI_BCGS_RHO, 0, L.res);
+    __syncthreads();
+    small_ew(OpBcgsP{P, R, V, s, true}, n);
+    __syncthreads();
+    ba(P, V, T);
+    small_reduce<1>(RedSet<1>{{{V, RP, RK_DOT}}}, n, L);
+    if (t == 0) ksp_epilogue(s, EPI_BCGS_D1, 0, L.res);
+    __syncthreads();
+    if (small_flag(&s->stop)) break;
+    small_ew(OpWaxpyDev{Sv, V, R, &s->alpha, true}, n);
+    __syncthreads();
+    ba(Sv, T, R);
+    small_reduce<3>(RedSet<3>{{{Sv, T, RK_DOT}, {T, T, RK_SUMSQ}, {Sv, Sv, RK_SUMSQ}}}, n, L);
+    if (t == 0) ksp_epilogue(s, EPI_BCGS_D2, 0, L.res);
+    __syncthreads();
+    if (small_flag(&s->stop)) break;
+    small_ew(OpBcgsX{X, P, Sv, s, true}, n);
+    small_ew(OpWaxpyDev{R, T, Sv, &s->omega, true}, n);
+    __syncthreads();
+    small_reduce<1>(RedSet<1>{{{R, R, RK_SUMSQ}}}, n, L);
+    if (t == 0) ksp_epilogue(s, EPI_BCGS_DP, 0, L.res);
+    __syncthreads();
+  }
+  __syncthreads();
+  for (int k = 0; k < 7; k++)
+    for (i64 i = t; i < n; i += MPK_WAVE_THREADS) gv[k][i] = L.vec[k][i];
+  if (t == 0) *g.c.s = L.ks;
+}
+
 // Whether a solve goes to the one-workgroup drivers: the tile-tree order (not
 // MPK_REDUCTION=ref), n <= MPK_SMALL_ROWS (MPK_SMALL_ROWS=k lowers it; 0
 // switches the drivers off), and a PC with two wave plans, or a PC without
@@ -2190,7 +2269,12 @@ void mp_ksp_solve_bcgs(int64_t n, int64_t irow, int64_t icol, int64_t data, int6
     // the iterations in one launch of one workgroup
     a.side = (int)S.h.side;
     a.R = R, a.RP = RP, a.V = V, a.T = T, a.Sv = Sv, a.P = P;
-    MPK_SMALL_LAUNCH(k_bcgs_small, S.pc.type, a);
+    if (n <= MPK_SMALL_LDS_ROWS && S.pc.type == PCT_JACOBI)
+      MPK_LAUNCH(k_bcgs_small_lds<PCT_JACOBI>, 1, MPK_WAVE_THREADS, a);
+    else if (n <= MPK_SMALL_LDS_ROWS && S.pc.type == PCT_NONE)
+      MPK_LAUNCH(k_bcgs_small_lds<PCT_NONE>, 1, MPK_WAVE_THREADS, a);
+    else
+      MPK_SMALL_LAUNCH(k_bcgs_small, S.pc.type, a);
     g_small_solves++;
   } else
   for (i64 queued = 0;; queued += chunk) {
