@@ -1361,17 +1361,19 @@ struct SmallCgVecLds {
 // This thread's rows (t, t + 512, ..) of the matrix held in registers for
 // the solve: at most 8 entries a row (else `fits` is false and the product
 // reads the matrix as small_spmv does).
-constexpr int kSmallRowsPerThread = (MPK_SMALL_LDS_ROWS + MPK_WAVE_THREADS - 1) / MPK_WAVE_THREADS;
+// RR rows a thread: systems of up to RR * 512 rows.
+template <int RR = (MPK_SMALL_LDS_ROWS + MPK_WAVE_THREADS - 1) / MPK_WAVE_THREADS>
 struct SmallRows {
-  int cnt[kSmallRowsPerThread];
-  int c[kSmallRowsPerThread][8];
-  double v[kSmallRowsPerThread][8];
+  int cnt[RR];
+  int c[RR][8];
+  double v[RR][8];
 };
-__device__ __forceinline__ bool small_rows_load(const SmallCG &a, SmallRows &m) {
+template <int RR>
+__device__ __forceinline__ bool small_rows_load(const SmallCG &a, SmallRows<RR> &m) {
   const int t = (int)threadIdx.x;
-  bool fits = true;
+  bool fits = a.n <= (i64)RR * MPK_WAVE_THREADS;
 #pragma unroll
-  for (int r = 0; r < kSmallRowsPerThread; r++) {
+  for (int r = 0; r < RR; r++) {
     const i64 i = t + (i64)r * MPK_WAVE_THREADS;
     const int j0 = i < a.n ? a.rp[i] : 0, j1 = i < a.n ? a.rp[i + 1] : 0;
     m.cnt[r] = j1 - j0;
@@ -1387,10 +1389,11 @@ __device__ __forceinline__ bool small_rows_load(const SmallCG &a, SmallRows &m) 
 }
 // y = A x from the rows in registers: each row's serial sum from 0.0 in CSR
 // order, as small_spmv.
-__device__ __forceinline__ void small_spmv_rows(const SmallRows &m, i64 n, const double *x, double *y) {
+template <int RR>
+__device__ __forceinline__ void small_spmv_rows(const SmallRows<RR> &m, i64 n, const double *x, double *y) {
   const int t = (int)threadIdx.x;
 #pragma unroll
-  for (int r = 0; r < kSmallRowsPerThread; r++) {
+  for (int r = 0; r < RR; r++) {
     const i64 i = t + (i64)r * MPK_WAVE_THREADS;
     double xv[8];
 #pragma unroll
@@ -1408,7 +1411,7 @@ __global__ void __launch_bounds__(MPK_WAVE_THREADS) k_cg_small_lds(SmallCG g) {
   __shared__ SmallCgVecLds L;
   const i64 n = g.n;
   const int t = (int)threadIdx.x;
-  SmallRows rows;
+  SmallRows<> rows;
   const bool fits = small_rows_load(g, rows);
   for (i64 i = t; i < n; i += MPK_WAVE_THREADS) {
     L.vec[0][i] = g.X[i];
@@ -1789,6 +1792,113 @@ __global__ void __launch_bounds__(MPK_WAVE_THREADS) k_gmres_small(SmallKsp a) {
   small_state_out(a.c.s, L, true);
 }
 
+// k_gmres_small for systems of at most MPK_SMALL_GM_ROWS rows without sweeps:
+// the Arnoldi step's vectors -- v_k, the new vector and the work vector --
+// in LDS and the matrix rows in registers, so the product, the PC, the norm
+// and the scaling touch no global memory; each new basis vector is written
+// to vv once scaled (the Gram-Schmidt sums and BuildSoln read the basis
+// there).  The same operations on the same values.
+#define MPK_SMALL_GM_ROWS 1024
+struct SmallGmresLds : SmallKspLds {
+  double loc[3][MPK_SMALL_GM_ROWS] __attribute__((aligned(16)));
+};
+
+template <int PCT>
+__global__ void __launch_bounds__(MPK_WAVE_THREADS) k_gmres_small_lds(SmallKsp a) {
+  __shared__ SmallGmresLds L;
+  SmallRows<MPK_SMALL_GM_ROWS / MPK_WAVE_THREADS> rows;
+  const bool fits = small_rows_load(a.c, rows);
+  KspState *s = small_state_in(a.c.s, L, true);
+  const i64 n = a.c.n, m = a.m;
+  const int t = (int)threadIdx.x;
+  double *X = a.c.X, *vv = a.vv, *TEMP = a.TEMP, *TMP = a.TMP;
+  const double *jac = a.c.jac;
+  auto spmv = [&](const double *x, double *y) {
+    if (fits)
+      small_spmv_rows(rows, n, x, y);
+    else
+      small_spmv(a.c, x, y);
+  };
+  auto pc = [&](const double *x, double *y) {  // small_pc's arithmetic, then its barrier
+    for (i64 i = t; i < n; i += MPK_WAVE_THREADS) y[i] = PCT == PCT_JACOBI ? x[i] * jac[i] : x[i];
+    __syncthreads();
+  };
+  i64 guess_zero = a.guess_zero;
+  for (i64 cycle = 0;; cycle++) {
+    // KSPInitialResidual(guess_zero, X, TEMP, TMP, vv, B)
+    if (!guess_zero) {
+      small_spmv(a.c, X, TEMP);
+      small_copy(a.B, TMP, n);
+      __syncthreads();
+      small_axpy(TMP, -1.0, TEMP, n);
+    } else {
+      small_copy(a.B, TMP, n);
+    }
+    __syncthreads();
+    if (a.side == PC_LEFT) {
+      small_pc<PCT>(a.c, TMP, vv, L);
+    } else {
+      small_copy(TMP, vv, n);
+      __syncthreads();
+    }
+    small_reduce<1>(RedSet<1>{{{vv, vv, RK_SUMSQ}}}, n, L);
+    if (t == 0) ksp_epilogue(s, EPI_GM_START, 0, L.res);
+    __syncthreads();
+    if (!small_flag(&s->stop)) small_ew(OpDivDev{vv, &s->res, aligned16(vv)}, n);
+    __syncthreads();
+    double *vk = L.loc[0], *vnew = L.loc[1], *wk = L.loc[2];
+    for (i64 i = t; i < n; i += MPK_WAVE_THREADS) vk[i] = vv[i];
+    __syncthreads();
+    for (i64 k = 0; k < m; k++) {
+      if (small_flag(&s->inner_stop)) break;
+      if (a.side == PC_RIGHT) {  // small_ba
+        pc(vk, wk);
+        spmv(wk, vnew);
+      } else {
+        spmv(vk, wk);
+        __syncthreads();
+        pc(wk, vnew);
+      }
+      __syncthreads();
+      small_orthogonalize(a, s, k, vnew, L);
+      small_reduce<1>(RedSet<1>{{{vnew, vnew, RK_SUMSQ}}}, n, L);
+      if (t == 0) ksp_epilogue(s, EPI_GM_STEP, 0, L.res);
+      __syncthreads();
+      if (!small_flag(&s->inner_stop)) small_ew(OpDivDev{vnew, &s->tt, true}, n);
+      __syncthreads();
+      double *g = vv + (k + 1) * n;
+      for (i64 i = t; i < n; i += MPK_WAVE_THREADS) g[i] = vnew[i];
+      __syncthreads();
+      double *q = vk;
+      vk = vnew;
+      vnew = q;
+    }
+    // KSPGMRESBuildSoln
+    if (t == 0 && !small_flag(&s->stop)) gmres_backsub(s, 0);
+    __syncthreads();
+    if (!small_flag(&s->skip_build)) {
+      small_ew(OpBuild{TEMP, vv, n, s, aligned16(TEMP) && aligned16(vv) && n % 2 == 0}, n);
+      __syncthreads();
+      // KSPUnwindPreconditioner(TEMP, TMP)
+      small_copy(TEMP, TMP, n);
+      __syncthreads();
+      if (a.side == PC_RIGHT) {
+        small_pc<PCT>(a.c, TEMP, TMP, L);
+        small_copy(TMP, TEMP, n);
+        __syncthreads();
+      }
+      small_axpy(X, 1.0, TEMP, n);
+      __syncthreads();
+    }
+    if (t == 0) gmres_cycle_end(s, 0);
+    __syncthreads();
+    if (small_flag(&s->stop)) break;
+    guess_zero = 0;
+    if (cycle > a.c.max_it + 2) break;
+  }
+  small_state_out(a.c.s, L, true);
+}
+
 // KSPSolve_FGMRES's cycles after the first residual (mp_ksp_solve_fgmres's
 // launch sequence).
 template <int PCT>
@@ -1816,6 +1926,77 @@ __global__ void __launch_bounds__(MPK_WAVE_THREADS) k_fgmres_small(SmallKsp a) {
       __syncthreads();
       if (!small_flag(&s->inner_stop)) small_ew(OpScaleRecipDev{vnew, &s->tt, &s->hapend, aligned16(vnew)}, n);
       __syncthreads();
+    }
+    if (t == 0 && !small_flag(&s->stop)) gmres_backsub(s, 1);
+    __syncthreads();
+    if (!small_flag(&s->skip_build)) {
+      small_ew(OpBuild{TEMP, a.prevec, n, s, aligned16(TEMP) && aligned16(a.prevec) && n % 2 == 0}, n);
+      __syncthreads();
+      small_axpy(X, 1.0, TEMP, n);
+      __syncthreads();
+    }
+    if (t == 0) gmres_cycle_end(s, 1);
+    __syncthreads();
+    if (small_flag(&s->stop)) break;
+    // the next cycle's residual: vv = -1 (A x) + b
+    small_spmv(a.c, X, TEMP);
+    __syncthreads();
+    for (i64 i = t; i < n; i += MPK_WAVE_THREADS) vv[i] = fma(-1.0, TEMP[i], a.B[i]);
+    __syncthreads();
+    if (cycle > a.c.max_it + 2) break;
+  }
+  small_state_out(a.c.s, L, true);
+}
+
+// k_fgmres_small in the form of k_gmres_small_lds: v_k, the preconditioned
+// vector and the new vector in LDS, each preconditioned vector written to
+// prevec and each scaled new vector to vv (BuildSoln and the Gram-Schmidt
+// sums read them there).
+template <int PCT>
+__global__ void __launch_bounds__(MPK_WAVE_THREADS) k_fgmres_small_lds(SmallKsp a) {
+  __shared__ SmallGmresLds L;
+  SmallRows<MPK_SMALL_GM_ROWS / MPK_WAVE_THREADS> rows;
+  const bool fits = small_rows_load(a.c, rows);
+  KspState *s = small_state_in(a.c.s, L, true);
+  const i64 n = a.c.n, m = a.m;
+  const int t = (int)threadIdx.x;
+  double *X = a.c.X, *vv = a.vv, *TEMP = a.TEMP;
+  const double *jac = a.c.jac;
+  for (i64 cycle = 0;; cycle++) {
+    small_reduce<1>(RedSet<1>{{{vv, vv, RK_SUMSQ}}}, n, L);
+    if (t == 0) ksp_epilogue(s, EPI_FGM_START, 0, L.res);
+    __syncthreads();
+    if (!small_flag(&s->inner_stop)) small_ew(OpScaleRecipDev{vv, &s->res, nullptr, aligned16(vv)}, n);
+    __syncthreads();
+    double *vk = L.loc[0], *vnew = L.loc[1], *pk = L.loc[2];
+    for (i64 i = t; i < n; i += MPK_WAVE_THREADS) vk[i] = vv[i];
+    __syncthreads();
+    for (i64 k = 0; k < m; k++) {
+      if (small_flag(&s->inner_stop)) break;
+      double *gp = a.prevec + k * n;
+      for (i64 i = t; i < n; i += MPK_WAVE_THREADS) {  // small_pc
+        const double z = PCT == PCT_JACOBI ? vk[i] * jac[i] : vk[i];
+        pk[i] = z;
+        gp[i] = z;
+      }
+      __syncthreads();
+      if (fits)
+        small_spmv_rows(rows, n, pk, vnew);
+      else
+        small_spmv(a.c, pk, vnew);
+      __syncthreads();
+      small_orthogonalize(a, s, k, vnew, L);
+      small_reduce<1>(RedSet<1>{{{vnew, vnew, RK_SUMSQ}}}, n, L);
+      if (t == 0) ksp_epilogue(s, EPI_FGM_STEP, 0, L.res);
+      __syncthreads();
+      if (!small_flag(&s->inner_stop)) small_ew(OpScaleRecipDev{vnew, &s->tt, &s->hapend, true}, n);
+      __syncthreads();
+      double *g = vv + (k + 1) * n;
+      for (i64 i = t; i < n; i += MPK_WAVE_THREADS) g[i] = vnew[i];
+      __syncthreads();
+      double *q = vk;
+      vk = vnew;
+      vnew = q;
     }
     if (t == 0 && !small_flag(&s->stop)) gmres_backsub(s, 1);
     __syncthreads();
@@ -1891,7 +2072,7 @@ __global__ void __launch_bounds__(MPK_WAVE_THREADS) k_bcgs_small_lds(SmallKsp g)
   __shared__ SmallBcgsLds L;
   const i64 n = g.c.n;
   const int t = (int)threadIdx.x;
-  SmallRows rows;
+  SmallRows<> rows;
   const bool fits = small_rows_load(g.c, rows);
   double *gv[7] = {g.c.X, g.R, g.RP, g.V, g.T, g.Sv, g.P};
   for (int k = 0; k < 7; k++)
@@ -2376,7 +2557,12 @@ void mp_ksp_solve_gmres(int64_t n, int64_t irow, int64_t icol, int64_t data, int
     a.side = (int)S.h.side;
     a.m = m, a.orthog = S.h.orthog, a.cgstype = S.h.cgstype, a.guess_zero = guess_zero;
     a.vv = vv, a.TEMP = TEMP, a.TMP = TEMP_MATOP, a.B = B;
-    MPK_SMALL_LAUNCH(k_gmres_small, S.pc.type, a);
+    if (n <= MPK_SMALL_GM_ROWS && S.pc.type == PCT_JACOBI)
+      MPK_LAUNCH(k_gmres_small_lds<PCT_JACOBI>, 1, MPK_WAVE_THREADS, a);
+    else if (n <= MPK_SMALL_GM_ROWS && S.pc.type == PCT_NONE)
+      MPK_LAUNCH(k_gmres_small_lds<PCT_NONE>, 1, MPK_WAVE_THREADS, a);
+    else
+      MPK_SMALL_LAUNCH(k_gmres_small, S.pc.type, a);
     g_small_solves++;
     S.finish_out((double *)cfg, (int64_t *)cfgi);
     return;
@@ -2438,7 +2624,12 @@ void mp_ksp_solve_fgmres(int64_t n, int64_t irow, int64_t icol, int64_t data, in
     a.side = (int)S.h.side, a.flexible = 1;
     a.m = m, a.orthog = S.h.orthog, a.cgstype = S.h.cgstype;
     a.vv = vv, a.prevec = prevec, a.TEMP = TEMP, a.B = B;
-    MPK_SMALL_LAUNCH(k_fgmres_small, S.pc.type, a);
+    if (n <= MPK_SMALL_GM_ROWS && S.pc.type == PCT_JACOBI)
+      MPK_LAUNCH(k_fgmres_small_lds<PCT_JACOBI>, 1, MPK_WAVE_THREADS, a);
+    else if (n <= MPK_SMALL_GM_ROWS && S.pc.type == PCT_NONE)
+      MPK_LAUNCH(k_fgmres_small_lds<PCT_NONE>, 1, MPK_WAVE_THREADS, a);
+    else
+      MPK_SMALL_LAUNCH(k_fgmres_small, S.pc.type, a);
     g_small_solves++;
     S.finish_out((double *)cfg, (int64_t *)cfgi);
     return;
