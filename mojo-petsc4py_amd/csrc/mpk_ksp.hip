@@ -1160,13 +1160,16 @@ struct Solve {
 #define MPK_SMALL_PLAIN 4096  // the limit for a PC without sweeps, and for GMRES / FGMRES / BiCGStab
 #define MPK_SMALL_TILES (MPK_SMALL_ROWS / MPK_TILE)
 
-template <int LVLS>
-struct SmallLdsT {
-  double val[MPK_BLK_ROWS + 1];  // the sweeps' values
-  int lv[LVLS + 1];              // the sweeps' level bounds
+// The reductions' LDS (small_reduce).
+struct SmallRedLds {
   double wsum[8][MPK_SMALL_TILES][4];  // per tile, its four wave sums
   double fsum[8][4];                   // the fold's wave sums
   double res[8];
+};
+template <int LVLS>
+struct SmallLdsT : SmallRedLds {
+  double val[MPK_BLK_ROWS + 1];  // the sweeps' values
+  int lv[LVLS + 1];              // the sweeps' level bounds
 };
 using SmallLds = SmallLdsT<MPK_WAVE_LVLS>;
 
@@ -1274,15 +1277,15 @@ template <int PCT, class LD>
 __device__ __forceinline__ void small_pc(const SmallCG &a, const double *r, double *z, LD &L) {
   const i64 n = a.n;
   const int t = (int)threadIdx.x;
-  if (PCT == PCT_JACOBI) {
+  if constexpr (PCT == PCT_JACOBI) {
     for (i64 i = t; i < n; i += MPK_WAVE_THREADS) z[i] = r[i] * a.jac[i];
-  } else if (PCT == PCT_ILU) {
+  } else if constexpr (PCT == PCT_ILU) {
     double *F = a.S, *B = a.S + n;
     const IluFactor &f = a.ilu;
     wave_sweep_form(a.form0, IluFwd{f.bi, f.bj, f.r, f.ba, r}, n, f.plan, 0, a.st, F, L.val, L.lv);
     __syncthreads();
     wave_sweep_form(a.form1, IluBwd{f.bj, f.bdiag, f.c, f.ba, F, z}, n, f.plan, 1, a.st, B, L.val, L.lv);
-  } else if (PCT == PCT_ICC) {
+  } else if constexpr (PCT == PCT_ICC) {
     double *Y = a.S, *F = a.S + n, *B = a.S + 2 * n;
     const IccFactor &f = a.icc;
     wave_sweep_form(a.form0, IccFwd{f.trp, f.trow, f.tpos, f.tmid, f.udiag, f.c, f.ua, r, F}, n, f.plan, 0, a.st,
@@ -1470,24 +1473,28 @@ __global__ void __launch_bounds__(MPK_WAVE_THREADS) k_cg_small_lds(SmallCG g) {
 // read after the barrier that follows the epilogue setting it.
 
 #define MPK_SMALL_M 31  // the largest restart of the one-workgroup GMRES
+#define MPK_SMALL_GM_ROWS 1024  // the limit of the LDS-resident GMRES / FGMRES
+#define MPK_SMALL_GM_BAS 12     // the basis vectors they keep in LDS (a multiple of 4)
 
 // The LDS of the GMRES / FGMRES / BiCGStab drivers: the sweeps' and
 // reductions' space (levels up to the rows, n <= MPK_SMALL_PLAIN), the
 // Gram-Schmidt sums of a step, and the solve's state with its Hessenberg,
 // Givens and right-hand-side arrays, moved here for the kernel's length so
 // that the epilogues' serial steps in thread 0 touch LDS, not global memory.
-struct SmallKspLds : SmallLdsT<MPK_SMALL_PLAIN> {
+struct SmallGmFields {
   double mres[MPK_SMALL_M + 1];
   double mw[MPK_SMALL_M + 1][2][4];  // the Gram-Schmidt sums' wave sums per tile (n <= 4096: two tiles)
   KspState ks;
   double HH[(MPK_SMALL_M + 2) * (MPK_SMALL_M + 1)], HES[(MPK_SMALL_M + 1) * (MPK_SMALL_M + 1)];
   double CC[MPK_SMALL_M + 1], SS[MPK_SMALL_M + 1], RS[MPK_SMALL_M + 2], LHH[MPK_SMALL_M + 1];
 };
+struct SmallKspLds : SmallLdsT<MPK_SMALL_PLAIN>, SmallGmFields {};
 
 // The state into LDS (with the GMRES arrays when `arrays`: m <= MPK_SMALL_M);
 // returns the LDS copy, which the kernel then uses throughout.  Every thread
 // calls it.
-__device__ __forceinline__ KspState *small_state_in(KspState *g, SmallKspLds &L, bool arrays) {
+template <class LD>
+__device__ __forceinline__ KspState *small_state_in(KspState *g, LD &L, bool arrays) {
   const int t = (int)threadIdx.x;
   if (arrays) {
     const i64 m = g->m;
@@ -1512,7 +1519,8 @@ __device__ __forceinline__ KspState *small_state_in(KspState *g, SmallKspLds &L,
 }
 // The LDS state back into g (g keeps its own array pointers).  Every thread
 // calls it.
-__device__ __forceinline__ void small_state_out(KspState *g, SmallKspLds &L, bool arrays) {
+template <class LD>
+__device__ __forceinline__ void small_state_out(KspState *g, LD &L, bool arrays) {
   __syncthreads();
   const int t = (int)threadIdx.x;
   if (arrays) {
@@ -1581,9 +1589,9 @@ __device__ __forceinline__ void small_ba(const SmallKsp &a, const double *x, dou
 // `h` is the half of the workgroup (256 threads, a block of the general
 // path) that takes the tiles: both halves when they split the tiles, one
 // when they split the groups.
-template <int G>
+template <int G, class LD>
 __device__ __forceinline__ void small_mdot_tiles(const double *x, const double *v0, i64 ld, int j0, i64 n, int T,
-                                                 int h0, int hs, SmallKspLds &L) {
+                                                 int h0, int hs, LD &L) {
   const int t = (int)threadIdx.x, tl = t & 255, lane = t & 63, wv = (t >> 6) & 3;
   const bool al = aligned16(x) && aligned16(v0) && ld % 2 == 0;
   for (int j = (t >> 8) - h0; j >= 0 && j < T; j += hs) {
@@ -1610,6 +1618,37 @@ __device__ __forceinline__ void small_mdot_tiles(const double *x, const double *
     }
   }
 }
+// small_mdot_tiles for n <= 1024 (one tile; a thread's pairs at base + 1024
+// and base + 1536 lie past n, their products +0.0, added as (0.0 + 0.0)),
+// for the cnt <= 8 vectors vec(j0), ..: every vector's loads issued before
+// any product (past cnt the last vector is reloaded, unused).  Called by
+// the half of the workgroup that takes the group.
+template <class VF, class LD>
+__device__ __forceinline__ void small_mdot_tile1k(const double *x, VF vec, bool al, int j0, int cnt, i64 n,
+                                                  LD &L) {
+  const int t = (int)threadIdx.x, tl = t & 255, lane = t & 63, wv = (t >> 6) & 3;
+  const i64 base = 2 * tl;
+  const D2 x0 = ld2(x, base, n, al), x1 = ld2(x, base + 512, n, al);
+  D2 v0[8], v1[8];
+#pragma unroll
+  for (int g = 0; g < 8; g++) {
+    const double *v = vec(j0 + (g < cnt ? g : cnt - 1));
+    v0[g] = ld2(v, base, n, al);
+    v1[g] = ld2(v, base + 512, n, al);
+  }
+  double u[8];
+#pragma unroll
+  for (int g = 0; g < 8; g++) {
+    const double p0 = fma(x0.a, v0[g].a, x0.b * v0[g].b), p1 = fma(x1.a, v1[g].a, x1.b * v1[g].b);
+    u[g] = (p0 + p1) + (0.0 + 0.0);
+  }
+#pragma unroll
+  for (int g = 0; g < 8; g++) {
+    const double w = wave_tree(u[g]);
+    if (lane == 0 && g < cnt) L.mw[j0 + g][0][wv] = w;
+  }
+}
+
 // x . V_j, j < nv, into L.mres, n <= MPK_SMALL_PLAIN (at most two tiles):
 // every sum's tile partials (block_tree's (W0 + W2) + (W1 + W3) of its wave
 // sums), then k_finalize's fold of the T partials -- thread t < 256 holds
@@ -1618,24 +1657,51 @@ __device__ __forceinline__ void small_mdot_tiles(const double *x, const double *
 // zeros (F1 = F2 = F3 = +0.0), so a sum's fold is one wave's tree of
 // P_0, .., P_{T-1} and (F0 + 0.0) + (0.0 + 0.0): the same arithmetic, done
 // by one wave per sum, the waves taking different sums.  Ends with a barrier.
-__device__ __forceinline__ void small_mdot(const double *x, const double *V, i64 ld, int nv, i64 n,
-                                           SmallKspLds &L) {
+// V_j for j < KL (a multiple of 4) is read from Vl (leading dimension
+// MPK_SMALL_GM_ROWS: the basis vectors a kernel keeps in LDS), the others
+// from V (leading dimension ld).
+template <int KL, class LD>
+__device__ __forceinline__ void small_mdot(const double *x, const double *V, i64 ld, int nv, i64 n, LD &L,
+                                           const double *Vl) {
+  constexpr int kl = KL;
   const int T = (int)((n + MPK_TILE - 1) / MPK_TILE);
-  // one tile: the halves take alternate groups of four; two: one tile each
+  // one tile: the halves take alternate groups of four (of eight when n <=
+  // 1024); two: one tile each
   const int half = (int)threadIdx.x >> 8;
-  int j0 = 0;
-  for (int gi = 0; j0 + 4 <= nv; j0 += 4, gi++)
-    if (T == 2 || (gi & 1) == half) small_mdot_tiles<4>(x, V + (i64)j0 * ld, ld, j0, n, T, T == 2 ? 0 : half, T == 2 ? 2 : 1, L);
-  const double *v0 = V + (i64)j0 * ld;
-  const int gl = nv / 4;  // the tail's group index
-  if (T == 2 || (gl & 1) == half) {
-    const int h0 = T == 2 ? 0 : half, hs = T == 2 ? 2 : 1;
-    if (nv - j0 == 3)
-      small_mdot_tiles<3>(x, v0, ld, j0, n, T, h0, hs, L);
-    else if (nv - j0 == 2)
-      small_mdot_tiles<2>(x, v0, ld, j0, n, T, h0, hs, L);
-    else if (nv - j0 == 1)
-      small_mdot_tiles<1>(x, v0, ld, j0, n, T, h0, hs, L);
+  auto vec = [&](int j, i64 &ldj) -> const double * {
+    ldj = j < kl ? (i64)MPK_SMALL_GM_ROWS : ld;
+    return j < kl ? Vl + (i64)j * MPK_SMALL_GM_ROWS : V + (i64)j * ld;
+  };
+  bool one_k = false;  // the LDS kernels' one-tile form (n <= 1024)
+  if constexpr (KL > 0) {
+    if (n <= 1024) {
+      auto vj = [&](int j) { return j < kl ? Vl + (i64)j * MPK_SMALL_GM_ROWS : V + (i64)j * ld; };
+      const bool al = aligned16(x) && aligned16(V) && ld % 2 == 0 && aligned16(Vl);
+      for (int j0 = 0, gi = 0; j0 < nv; j0 += 8, gi++)
+        if ((gi & 1) == half) small_mdot_tile1k(x, vj, al, j0, nv - j0 < 8 ? nv - j0 : 8, n, L);
+      one_k = true;
+    }
+  }
+  if (!one_k) {
+    int j0 = 0;
+    for (int gi = 0; j0 + 4 <= nv; j0 += 4, gi++)
+      if (T == 2 || (gi & 1) == half) {
+        i64 ldj;
+        const double *vj = vec(j0, ldj);
+        small_mdot_tiles<4>(x, vj, ldj, j0, n, T, T == 2 ? 0 : half, T == 2 ? 2 : 1, L);
+      }
+    i64 ld0;
+    const double *v0 = vec(j0, ld0);
+    const int gl = nv / 4;  // the tail's group index
+    if (T == 2 || (gl & 1) == half) {
+      const int h0 = T == 2 ? 0 : half, hs = T == 2 ? 2 : 1;
+      if (nv - j0 == 3)
+        small_mdot_tiles<3>(x, v0, ld0, j0, n, T, h0, hs, L);
+      else if (nv - j0 == 2)
+        small_mdot_tiles<2>(x, v0, ld0, j0, n, T, h0, hs, L);
+      else if (nv - j0 == 1)
+        small_mdot_tiles<1>(x, v0, ld0, j0, n, T, h0, hs, L);
+    }
   }
   __syncthreads();
   const int lane = (int)threadIdx.x & 63;
@@ -1648,9 +1714,12 @@ __device__ __forceinline__ void small_mdot(const double *x, const double *V, i64
 }
 
 // One Arnoldi orthogonalisation of vnew against v_0..v_it (orthogonalize's
-// steps); ends with a barrier.
-__device__ __forceinline__ void small_orthogonalize(const SmallKsp &a, KspState *s, i64 it, double *vnew,
-                                                    SmallKspLds &L) {
+// steps); ends with a barrier.  The CGS sums and MAXPY read v_j, j < KL, from
+// Vl (small_mdot's LDS copy of the basis).
+template <int KL = 0, class LD>
+__device__ __forceinline__ void small_orthogonalize(const SmallKsp &a, KspState *s, i64 it, double *vnew, LD &L,
+                                                    const double *Vl = nullptr) {
+  constexpr int kl = KL;
   const i64 n = a.c.n;
   const int t = (int)threadIdx.x;
   if (a.orthog == 1) {  // MGS
@@ -1667,7 +1736,7 @@ __device__ __forceinline__ void small_orthogonalize(const SmallKsp &a, KspState 
   const int nv = (int)(it + 1);
   for (int pass = 0; pass < 2; pass++) {
     if (pass == 1 && small_flag(&s->skip_ref)) break;
-    small_mdot(vnew, a.vv, n, nv, n, L);
+    small_mdot<KL>(vnew, a.vv, n, nv, n, L, Vl);
     // EPI_GM_CGS_A / EPI_GM_CGS_B with each entry j <= it on thread j: the
     // same operations per entry (HH(j) = 0.0 - LHH(j) in the first pass, as
     // zeroing then subtracting gives), then the serial norm in thread 0
@@ -1700,19 +1769,32 @@ __device__ __forceinline__ void small_orthogonalize(const SmallKsp &a, KspState 
     }
     // vmaxpy_dev(vnew, vv, n, LHH, nv, 1.0): each element's chain over the nv
     // vectors, eight vectors' loads issued before their multiply-adds
+    // (rows i0 and i0 + 512 together; a group's loads past the last vector
+    // reload the last one, unused, so that every load is unconditional)
     const double *coef = s->LHH;
-    for (i64 i = t; i < n; i += MPK_WAVE_THREADS) {
-      double acc = vnew[i];
-      int k = 0;
-      for (; k + 8 <= nv; k += 8) {
-        double v[8];
+    auto vec = [&](int j) { return j < kl ? Vl + (i64)j * MPK_SMALL_GM_ROWS : a.vv + (i64)j * n; };
+    for (i64 i0 = t; i0 < n; i0 += 2 * MPK_WAVE_THREADS) {
+      const i64 i1 = i0 + MPK_WAVE_THREADS < n ? i0 + MPK_WAVE_THREADS : i0;
+      double acc0 = vnew[i0], acc1 = vnew[i1];
+      for (int k = 0; k < nv; k += 8) {
+        double v0[8], v1[8];
 #pragma unroll
-        for (int u = 0; u < 8; u++) v[u] = a.vv[(i64)(k + u) * n + i];
+        for (int u = 0; u < 8; u++) {
+          const double *vp = vec(k + u < nv ? k + u : nv - 1);
+          v0[u] = vp[i0];
+          v1[u] = vp[i1];
+        }
 #pragma unroll
-        for (int u = 0; u < 8; u++) acc = fma(1.0 * coef[k + u], v[u], acc);
+        for (int u = 0; u < 8; u++) {  // past the last vector: acc kept
+          const bool in = k + u < nv;
+          const double c = 1.0 * coef[in ? k + u : nv - 1];
+          const double r0 = fma(c, v0[u], acc0), r1 = fma(c, v1[u], acc1);
+          acc0 = in ? r0 : acc0;
+          acc1 = in ? r1 : acc1;
+        }
       }
-      for (; k < nv; k++) acc = fma(1.0 * coef[k], a.vv[(i64)k * n + i], acc);
-      vnew[i] = acc;
+      vnew[i0] = acc0;
+      if (i1 != i0) vnew[i1] = acc1;
     }
     __syncthreads();
     if (pass == 0 && a.cgstype == 1) {
@@ -1796,11 +1878,13 @@ __global__ void __launch_bounds__(MPK_WAVE_THREADS) k_gmres_small(SmallKsp a) {
 // the Arnoldi step's vectors -- v_k, the new vector and the work vector --
 // in LDS and the matrix rows in registers, so the product, the PC, the norm
 // and the scaling touch no global memory; each new basis vector is written
-// to vv once scaled (the Gram-Schmidt sums and BuildSoln read the basis
-// there).  The same operations on the same values.
-#define MPK_SMALL_GM_ROWS 1024
-struct SmallGmresLds : SmallKspLds {
+// to vv once scaled, and the first MPK_SMALL_GM_BAS of them to LDS as well:
+// the Gram-Schmidt sums and MAXPY read those there, the rest (and BuildSoln
+// all of them) from vv.  The same operations on the same values.
+// No sweeps: the reductions' and GMRES's LDS without the sweeps' arrays.
+struct SmallGmresLds : SmallRedLds, SmallGmFields {
   double loc[3][MPK_SMALL_GM_ROWS] __attribute__((aligned(16)));
+  double bas[MPK_SMALL_GM_BAS][MPK_SMALL_GM_ROWS] __attribute__((aligned(16)));
 };
 
 template <int PCT>
@@ -1847,7 +1931,10 @@ __global__ void __launch_bounds__(MPK_WAVE_THREADS) k_gmres_small_lds(SmallKsp a
     if (!small_flag(&s->stop)) small_ew(OpDivDev{vv, &s->res, aligned16(vv)}, n);
     __syncthreads();
     double *vk = L.loc[0], *vnew = L.loc[1], *wk = L.loc[2];
-    for (i64 i = t; i < n; i += MPK_WAVE_THREADS) vk[i] = vv[i];
+    for (i64 i = t; i < n; i += MPK_WAVE_THREADS) {
+      vk[i] = vv[i];
+      L.bas[0][i] = vv[i];
+    }
     __syncthreads();
     for (i64 k = 0; k < m; k++) {
       if (small_flag(&s->inner_stop)) break;
@@ -1860,7 +1947,7 @@ __global__ void __launch_bounds__(MPK_WAVE_THREADS) k_gmres_small_lds(SmallKsp a
         pc(wk, vnew);
       }
       __syncthreads();
-      small_orthogonalize(a, s, k, vnew, L);
+      small_orthogonalize<MPK_SMALL_GM_BAS>(a, s, k, vnew, L, &L.bas[0][0]);
       small_reduce<1>(RedSet<1>{{{vnew, vnew, RK_SUMSQ}}}, n, L);
       if (t == 0) ksp_epilogue(s, EPI_GM_STEP, 0, L.res);
       __syncthreads();
@@ -1868,6 +1955,8 @@ __global__ void __launch_bounds__(MPK_WAVE_THREADS) k_gmres_small_lds(SmallKsp a
       __syncthreads();
       double *g = vv + (k + 1) * n;
       for (i64 i = t; i < n; i += MPK_WAVE_THREADS) g[i] = vnew[i];
+      if (k + 1 < MPK_SMALL_GM_BAS)  // uniform
+        for (i64 i = t; i < n; i += MPK_WAVE_THREADS) L.bas[k + 1][i] = vnew[i];
       __syncthreads();
       double *q = vk;
       vk = vnew;
@@ -1969,7 +2058,10 @@ __global__ void __launch_bounds__(MPK_WAVE_THREADS) k_fgmres_small_lds(SmallKsp 
     if (!small_flag(&s->inner_stop)) small_ew(OpScaleRecipDev{vv, &s->res, nullptr, aligned16(vv)}, n);
     __syncthreads();
     double *vk = L.loc[0], *vnew = L.loc[1], *pk = L.loc[2];
-    for (i64 i = t; i < n; i += MPK_WAVE_THREADS) vk[i] = vv[i];
+    for (i64 i = t; i < n; i += MPK_WAVE_THREADS) {
+      vk[i] = vv[i];
+      L.bas[0][i] = vv[i];
+    }
     __syncthreads();
     for (i64 k = 0; k < m; k++) {
       if (small_flag(&s->inner_stop)) break;
@@ -1985,7 +2077,7 @@ __global__ void __launch_bounds__(MPK_WAVE_THREADS) k_fgmres_small_lds(SmallKsp 
       else
         small_spmv(a.c, pk, vnew);
       __syncthreads();
-      small_orthogonalize(a, s, k, vnew, L);
+      small_orthogonalize<MPK_SMALL_GM_BAS>(a, s, k, vnew, L, &L.bas[0][0]);
       small_reduce<1>(RedSet<1>{{{vnew, vnew, RK_SUMSQ}}}, n, L);
       if (t == 0) ksp_epilogue(s, EPI_FGM_STEP, 0, L.res);
       __syncthreads();
@@ -1993,6 +2085,8 @@ __global__ void __launch_bounds__(MPK_WAVE_THREADS) k_fgmres_small_lds(SmallKsp 
       __syncthreads();
       double *g = vv + (k + 1) * n;
       for (i64 i = t; i < n; i += MPK_WAVE_THREADS) g[i] = vnew[i];
+      if (k + 1 < MPK_SMALL_GM_BAS)  // uniform
+        for (i64 i = t; i < n; i += MPK_WAVE_THREADS) L.bas[k + 1][i] = vnew[i];
       __syncthreads();
       double *q = vk;
       vk = vnew;
