@@ -176,11 +176,22 @@ enum {
 __device__ i64 update_hessenberg(KspState *s, i64 it, i64 hapend, double *res) {
   const i64 m = s->m;
   double *hh = s->HH, *cc = s->CC, *ss = s->SS, *rs = s->RS;
-  for (i64 j = 1; j <= it; j++) {
-    i64 idx = it * (m + 2) + (j - 1);
-    double tt = hh[idx];
-    hh[idx] = fma(cc[j - 1], tt, ss[j - 1] * hh[idx + 1]);
-    hh[idx + 1] = fma(cc[j - 1], hh[idx + 1], -(ss[j - 1] * tt));
+  if (it > 0) {
+    // rotation j rewrites entries j-1 and j of the column; entry j, its
+    // second result, is carried to rotation j+1 in a register, so that no
+    // load of the chain waits on the store before it (the arrays are
+    // distinct, which lets the next rotations' loads issue early)
+    double *__restrict__ col = hh + it * (m + 2);
+    const double *__restrict__ c = cc;
+    const double *__restrict__ sn = ss;
+    double carry = col[0];
+#pragma unroll 4
+    for (i64 j = 1; j <= it; j++) {
+      const double tt = carry, nx = col[j];
+      col[j - 1] = fma(c[j - 1], tt, sn[j - 1] * nx);
+      carry = fma(c[j - 1], nx, -(sn[j - 1] * tt));
+    }
+    col[it] = carry;
   }
   if (!hapend) {
     i64 base = it * (m + 2) + it;
@@ -883,7 +894,11 @@ __device__ void gmres_backsub(KspState *s, int flexible) {
   for (i64 ii = 1; ii <= it; ii++) {
     i64 k = it - ii;
     double tt = rs[k];
-    for (i64 j = k + 1; j <= it; j++) tt = fma(-hh[j * (m + 2) + k], rs[j], tt);
+    // (no stores in the sum: unrolled, its loads run ahead of the chain)
+    const double *__restrict__ h = hh;
+    const double *__restrict__ r = rs;
+#pragma unroll 4
+    for (i64 j = k + 1; j <= it; j++) tt = fma(-h[j * (m + 2) + k], r[j], tt);
     if (!flexible && hh[k * (m + 2) + k] == 0.0) {
       s->breason = KSP_DIVERGED_BREAKDOWN;
       return;
@@ -933,6 +948,7 @@ struct OpBuild {
     if (e >= n) return;
     i64 nb = s->nbuild;
     D2 acc{0.0, 0.0};  // VecMAXPBY with beta = 0: the zeroed destination
+#pragma unroll 4
     for (i64 k = 0; k < nb; k++) {
       double c = s->RS[k];
       D2 v = ld2(base + k * ld, e, n, al);
