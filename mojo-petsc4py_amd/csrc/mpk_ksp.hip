@@ -1720,11 +1720,22 @@ __device__ __forceinline__ void small_mdot(const double *x, const double *V, i64
     }
   }
   __syncthreads();
-  const int lane = (int)threadIdx.x & 63;
-  for (int g = (int)threadIdx.x >> 6; g < nv; g += MPK_WAVE_THREADS / 64) {
-    const double u = lane < T ? (L.mw[g][lane][0] + L.mw[g][lane][2]) + (L.mw[g][lane][1] + L.mw[g][lane][3]) : 0.0;
-    const double f0 = wave_tree(u);
-    if (lane == 0) L.mres[g] = (f0 + 0.0) + (0.0 + 0.0);
+  if (T == 1) {
+    // one partial: the wave tree adds +0.0 (the other lanes' sums) to it at
+    // each of its six steps, so one thread a sum does the same additions
+    for (int g = (int)threadIdx.x; g < nv; g += MPK_WAVE_THREADS) {
+      const double u = (L.mw[g][0][0] + L.mw[g][0][2]) + (L.mw[g][0][1] + L.mw[g][0][3]);
+      const double f0 = (((((u + 0.0) + 0.0) + 0.0) + 0.0) + 0.0) + 0.0;
+      L.mres[g] = (f0 + 0.0) + (0.0 + 0.0);
+    }
+  } else {
+    const int lane = (int)threadIdx.x & 63;
+    for (int g = (int)threadIdx.x >> 6; g < nv; g += MPK_WAVE_THREADS / 64) {
+      const double u =
+          lane < T ? (L.mw[g][lane][0] + L.mw[g][lane][2]) + (L.mw[g][lane][1] + L.mw[g][lane][3]) : 0.0;
+      const double f0 = wave_tree(u);
+      if (lane == 0) L.mres[g] = (f0 + 0.0) + (0.0 + 0.0);
+    }
   }
   __syncthreads();
 }
@@ -1755,7 +1766,10 @@ __device__ __forceinline__ void small_orthogonalize(const SmallKsp &a, KspState 
     small_mdot<KL>(vnew, a.vv, n, nv, n, L, Vl);
     // EPI_GM_CGS_A / EPI_GM_CGS_B with each entry j <= it on thread j: the
     // same operations per entry (HH(j) = 0.0 - LHH(j) in the first pass, as
-    // zeroing then subtracting gives), then the serial norm in thread 0
+    // zeroing then subtracting gives), then, for refine-if-needed, the
+    // serial norm in thread 0.  Nothing below reads an entry's LHH, HH or
+    // HES before the next barrier but its own thread and that norm (MAXPY
+    // takes LHH(j) = -mres(j) from mres), so only the norm waits for them.
     {
       const i64 itc = s->it, m = s->m;
       if (t <= itc) {
@@ -1770,24 +1784,24 @@ __device__ __forceinline__ void small_orthogonalize(const SmallKsp &a, KspState 
           *he = *he - l;
         }
       }
-      __syncthreads();
-      if (t == 0 && pass == 0) {
-        if (s->cgstype == 1) {
+      if (s->cgstype == 1) {  // uniform
+        __syncthreads();
+        if (t == 0 && pass == 0) {
           double hnrm = 0.0;
           for (i64 j = 0; j <= itc; j++) hnrm = fma(s->LHH[j], s->LHH[j], hnrm);
           s->hnrm = sqrt(hnrm);
           s->skip_ref = 1;  // decided after the norm of the projected vector
-        } else {
-          s->skip_ref = s->cgstype == 2 ? 0 : 1;
         }
+        __syncthreads();
+      } else if (t == 0 && pass == 0) {
+        s->skip_ref = s->cgstype == 2 ? 0 : 1;  // read after MAXPY's barrier
       }
-      __syncthreads();
     }
     // vmaxpy_dev(vnew, vv, n, LHH, nv, 1.0): each element's chain over the nv
     // vectors, eight vectors' loads issued before their multiply-adds
     // (rows i0 and i0 + 512 together; a group's loads past the last vector
     // reload the last one, unused, so that every load is unconditional)
-    const double *coef = s->LHH;
+    const double *mres = L.mres;  // LHH(j) = -mres(j)
     auto vec = [&](int j) { return j < kl ? Vl + (i64)j * MPK_SMALL_GM_ROWS : a.vv + (i64)j * n; };
     for (i64 i0 = t; i0 < n; i0 += 2 * MPK_WAVE_THREADS) {
       const i64 i1 = i0 + MPK_WAVE_THREADS < n ? i0 + MPK_WAVE_THREADS : i0;
@@ -1803,7 +1817,7 @@ __device__ __forceinline__ void small_orthogonalize(const SmallKsp &a, KspState 
 #pragma unroll
         for (int u = 0; u < 8; u++) {  // past the last vector: acc kept
           const bool in = k + u < nv;
-          const double c = 1.0 * coef[in ? k + u : nv - 1];
+          const double c = 1.0 * -mres[in ? k + u : nv - 1];
           const double r0 = fma(c, v0[u], acc0), r1 = fma(c, v1[u], acc1);
           acc0 = in ? r0 : acc0;
           acc1 = in ? r1 : acc1;
