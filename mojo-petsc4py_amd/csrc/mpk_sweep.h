@@ -231,6 +231,11 @@ __global__ void __launch_bounds__(64) k_sweep(Prog prog, i64 n, int forward, uns
 // loads are all contiguous and none depends on another.
 #define MPK_BLK_ROWS 8192
 #define MPK_PLAN_PF 16  // slots of a row held in registers (loaded a level ahead)
+// A ring plan (one sweeping workgroup) of rows wider than MPK_PLAN_PF slots
+// holds this many: a row past the registers loads the rest when it runs, one
+// more round trip on its level's path (with the 1e6-row random SPD matrix's
+// lower triangle, 10% of rows pass 16 entries, 0.07% pass 24).
+#define MPK_RING_WIDE 24
 
 struct PlanView {
   i64 n, nblk, w;
@@ -590,6 +595,8 @@ __global__ void __launch_bounds__(256) k_sweep_blk(Prog prog, i64 n, const char 
       if (!(stop && *stop)) {
         if (P.w == 4)
           plan_levels<4, Tr>(prog, coef, n, P, st, 0, val, pub, ticket + 1, gave);
+        else if (P.w > MPK_PLAN_PF)
+          plan_levels<MPK_RING_WIDE, Tr>(prog, coef, n, P, st, 0, val, pub, ticket + 1, gave);
         else
           plan_levels<MPK_PLAN_PF, Tr>(prog, coef, n, P, st, 0, val, pub, ticket + 1, gave);
       }
